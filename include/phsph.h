@@ -1,0 +1,231 @@
+/*
+ * phsph.h -- C-ABI of the MI355X-native SPH hot path (kd-tree build, density
+ * iteration, primitive-variable update and force evaluation).
+ *
+ * The entry points replace, one for one, the three calls that Phantom's
+ * derivative driver makes on its hot path (src/main/deriv.f90:114,135,189):
+ *
+ *   phsph_maketree       <- maketree        (src/main/kdtree.F90:117-128,
+ *                                            reached via build_tree,
+ *                                            src/main/neigh_kdtree.f90:161)
+ *   phsph_densityiterate <- densityiterate  (src/main/dens.F90:117-152)
+ *   phsph_cons2prim      <- cons2prim_everything (src/main/cons2prim.f90:274)
+ *   phsph_force          <- force           (src/main/force.F90:193-263)
+ *   phsph_get_neighbour_stats <- get_neighbour_stats (src/main/dens.F90:1137)
+ *
+ * Conventions
+ *  - Every particle array is a Fortran column-major array a(ncomp, maxp)
+ *    passed as a pointer to a(1,1) plus its leading dimension `ld` (>= ncomp).
+ *    Particle i (0-based here, i+1 in Fortran) component c is a[i*ld + c].
+ *  - Pointers may be host memory or HIP device memory; the library detects
+ *    which (hipPointerGetAttributes) and stages host arrays through its own
+ *    device buffers.  Device pointers must be on the context's device.
+ *  - The storage kind of every array is the one the reference uses in its
+ *    default FP64 build (src/main/part.F90:47-542): positions, velocities,
+ *    B/rho, forces are double; divcurlv, divcurlB, alphaind, gradh, dvdx,
+ *    divBsymm are float (real(kind=4)); iphase is int8.
+ *  - Every call returns PHSPH_OK (0) or a negative error code; it never
+ *    aborts the process.  phsph_last_error() returns the message of the most
+ *    recent failure on a context.  The Fortran side maps a non-zero code to
+ *    Phantom's fatal(), which keeps the reference's behaviour
+ *    (src/main/io.F90:525-551).
+ *  - Calls on one context are not re-entrant (the reference's module state is
+ *    not either); each context owns one HIP stream and returns synchronously.
+ */
+#ifndef PHSPH_H
+#define PHSPH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------- */
+#define PHSPH_OK              0
+#define PHSPH_ERR_INPUT      -1  /* invalid argument, h <= 0, eps outside [0,1] */
+#define PHSPH_ERR_NAN        -2  /* NaN in particle positions (kdtree.F90:391) */
+#define PHSPH_ERR_NOCONVERGE -3  /* h-rho iteration hit maxdensits (dens.F90:1443) */
+#define PHSPH_ERR_CAPACITY   -4  /* tree depth / list capacity exceeded */
+#define PHSPH_ERR_HIP        -5  /* a HIP runtime call failed */
+#define PHSPH_ERR_STATE      -6  /* call order violated (e.g. force before tree) */
+#define PHSPH_ERR_NODATA     -7  /* no live particles (kdtree.F90:162) */
+
+/* ---- compile-time choices of the reference, made runtime here -------- */
+#define PHSPH_KERNEL_CUBIC    0  /* kernel_cubic.f90   (M4, radkern 2) */
+#define PHSPH_KERNEL_QUINTIC  1  /* kernel_quintic.f90 (M6, radkern 3) */
+
+#define PHSPH_PREC_FP64       0  /* all pair arithmetic in FP64 */
+#define PHSPH_PREC_MIXED      1  /* FP32 pair arithmetic, FP64 accumulation */
+
+/*
+ * Runtime parameters.  Each field names the reference variable it carries;
+ * defaults are set by phsph_default_params() to the reference defaults
+ * (options.f90:69-128, shock_capturing.f90:47-65, timestep.f90:52-62,
+ * eos.f90:1888-1905, dust.f90:26-33, kdtree.F90:46).
+ */
+typedef struct phsph_params {
+  int32_t kernel;              /* KERNEL= (build/Makefile:213-218)            */
+  int32_t periodic;            /* dim:periodic                                */
+  int32_t maxvxyzu;            /* 3 with ISOTHERMAL=yes, else 4               */
+  int32_t mhd;                 /* dim:mhd (ideal MHD + div B cleaning)        */
+  int32_t use_dustfrac;        /* one-fluid dust (options:use_dustfrac)       */
+  int32_t ndustsmall;          /* part:ndustsmall (0 or 1 supported)          */
+  int32_t gravity;             /* dim:gravity                                 */
+  int32_t ind_timesteps;       /* dim:ind_timesteps                           */
+  int32_t ieos;                /* eos:ieos, 1 (isothermal) or 2 (ideal gas)   */
+  int32_t maxp;                /* dim:maxp; sets ncellsmax = 2*maxp           */
+  int32_t precision;           /* PHSPH_PREC_*                                */
+  int32_t idrag;               /* dust:idrag, 2 (const K) or 3 (const t_s)    */
+  int32_t ishock_heating;      /* eos:ishock_heating                          */
+  int32_t ipdv_heating;        /* eos:ipdv_heating                            */
+  int32_t iresistive_heating;  /* eos:iresistive_heating                      */
+  int32_t ilimitdustflux;      /* dust:ilimitdustflux                         */
+  double xmin, xmax, ymin, ymax, zmin, zmax;   /* boundary (boundary.f90:63) */
+  double hfact;                /* part:hfact                                  */
+  double tolh;                 /* part:tolh                                   */
+  double massoftype_gas;       /* part:massoftype(igas)                       */
+  double gamma;                /* eos:gamma                                   */
+  double polyk;                /* eos:polyk (isothermal c_s^2)                */
+  double alpha;                /* options:alpha (minimum, Cullen-Dehnen)      */
+  double alphamax;             /* options:alphamax                            */
+  double alphau;               /* options:alphau (conductivity)               */
+  double alphaB;               /* options:alphaB (resistivity)                */
+  double beta;                 /* options:beta                                */
+  double C_cour, C_force;      /* timestep:C_cour, C_force                    */
+  double dtmax;                /* timestep:dtmax                              */
+  double psidecayfac;          /* timestep:psidecayfac                        */
+  double overcleanfac;         /* timestep:overcleanfac                       */
+  double K_code;               /* dust:K_code(1)                              */
+  double tree_accuracy;        /* kdtree:tree_accuracy (opening angle)        */
+} phsph_params;
+
+/* Reference tree node, layout of type(kdnode) in a non-GRAVITY build
+ * (src/main/dtype_kdtree.F90:53-68, sequence type, 64 bytes). */
+typedef struct phsph_kdnode {
+  double  xcen[3];
+  double  size;
+  double  hmax;
+  double  mass;
+  int32_t leftchild;
+  int32_t rightchild;
+  int32_t parent;
+  int32_t idum;
+} phsph_kdnode;
+
+typedef struct phsph_ctx phsph_ctx;
+
+/* Fill *p with the reference defaults for a given kernel (hfact_default
+ * from the kernel file, tolh 1e-4, alpha 0 / alphamax 1 / beta 2,
+ * alphau 1, alphaB 1, C_cour 0.3, C_force 0.25, psidecayfac 1, ...). */
+void phsph_default_params(phsph_params *p, int32_t kernel);
+
+/* Create a context on HIP device `device`. */
+int phsph_create(const phsph_params *p, int32_t device, phsph_ctx **ctx);
+int phsph_destroy(phsph_ctx *ctx);
+int phsph_set_params(phsph_ctx *ctx, const phsph_params *p);
+const char *phsph_last_error(const phsph_ctx *ctx);
+
+/*
+ * maketree (kdtree.F90:117): build the kd-tree over the live particles
+ * (h > 0) of xyzh(4,np).  xyzh is inout: periodic particles outside the box
+ * are wrapped (construct_root_node -> cross_boundary, boundary.f90:123).
+ * iphase(np) (may be NULL when every particle is an active gas particle)
+ * marks active (>0) / inactive (<0) particles (part.F90:1013-1075).
+ * Outputs (each may be NULL):
+ *   leaf_is_active(ncellsmax+1): 1 active leaf, -1 inactive leaf, 0 other
+ *   ncells: number of node slots (2^(maxlevel+1)-1 when indexed)
+ * The node array, inoderange and inodeparts are retrieved with
+ * phsph_get_tree.
+ */
+int phsph_maketree(phsph_ctx *ctx, double *xyzh, int32_t ldxyzh, int32_t np,
+                   const int8_t *iphase, int32_t *leaf_is_active,
+                   int64_t *ncells);
+
+/*
+ * Mirror the tree of the last phsph_maketree / phsph_densityiterate into
+ * host arrays in the reference's layout (kdtree.F90:31-32,
+ * neigh_kdtree.f90:30-36): node(ncells), inoderange(2,ncells) and
+ * inodeparts(np) (1-based particle indices, negative = inactive).
+ * Any pointer may be NULL.  `cap_nodes` is the length of node/inoderange.
+ */
+int phsph_get_tree(phsph_ctx *ctx, phsph_kdnode *node, int32_t *inoderange,
+                   int32_t *inodeparts, int64_t cap_nodes, int32_t *maxlevel);
+
+/*
+ * densityiterate (dens.F90:117-152).  icall 0 = one summation pass without
+ * h iteration, 1 = converge h-rho (Newton-Raphson, tolh, clamp [0.8h,1.2h]).
+ * xyzh(4,np) inout: h is replaced by hfact*(m/rho)^(1/3) of active
+ * particles.  vxyzu(maxvxyzu,np), fxyzu(>=3,np), fext(3,np) (NULL = zero),
+ * Bevol(4,np) (MHD only, else NULL).
+ * Outputs (float): divcurlv(lddivcurlv,np) component 1 = div v;
+ * divcurlB(4,np) (MHD); alphaind(3,np) component 3 = d(div v)/dt;
+ * gradh(ldgradh,np) component 1 = 1/Omega; dvdx(9,np).
+ * The tree of the preceding phsph_maketree must still be current.
+ */
+int phsph_densityiterate(phsph_ctx *ctx, int32_t icall, int32_t np,
+                         double *xyzh, int32_t ldxyzh,
+                         const double *vxyzu, int32_t ldvxyzu,
+                         const double *fxyzu, int32_t ldfxyzu,
+                         const double *fext, const double *Bevol,
+                         const int8_t *iphase,
+                         float *divcurlv, int32_t lddivcurlv,
+                         float *divcurlB, float *alphaind,
+                         float *gradh, int32_t ldgradh, float *dvdx);
+
+/*
+ * cons2prim_everything (cons2prim.f90:274-456) for ieos 1 and 2: fills
+ * eos_vars(ldeos,np) components 1..3 (P, c_s, T placeholder 0), the dust
+ * fraction dustfrac(1,np) = s^2/(1+s^2) from dustevol(1,np) (one-fluid
+ * dust only, else both NULL) and alphaind(2,:) = alpha_loc from
+ * alphaind(3,:) and dvdx (Cullen-Dehnen, shock_capturing.f90:131-178).
+ */
+int phsph_cons2prim(phsph_ctx *ctx, int32_t np, const double *xyzh,
+                    int32_t ldxyzh, const double *vxyzu, int32_t ldvxyzu,
+                    const float *dvdx, float *alphaind,
+                    const double *dustevol, double *dustfrac,
+                    double *eos_vars, int32_t ldeos, const int8_t *iphase);
+
+/*
+ * force (force.F90:193-263) for hydro + Cullen-Dehnen viscosity +
+ * conductivity, ideal MHD with div B cleaning and one-fluid dust.
+ * Inputs: xyzh, vxyzu, gradh (1/Omega, float), alphaind (alpha in comp. 1),
+ * eos_vars (P, c_s), divcurlB (MHD), Bevol (MHD), dustfrac (dust).
+ * Outputs: fxyzu(ldfxyzu,np) (du/dt in component 4 when maxvxyzu = 4),
+ * divcurlv(1,:) = div v from the force loop (float), dBevol(4,np) (MHD),
+ * divBsymm(np) (MHD, float), ddustevol(1,np), deltav(3,np), tstop(np) (dust),
+ * and the global timestep constraints dtcourant, dtforce (global-timestep
+ * build, force.F90:3312-3327).  Arrays for disabled physics may be NULL.
+ */
+int phsph_force(phsph_ctx *ctx, int32_t icall, int32_t np,
+                const double *xyzh, int32_t ldxyzh,
+                const double *vxyzu, int32_t ldvxyzu,
+                double *fxyzu, int32_t ldfxyzu,
+                float *divcurlv, int32_t lddivcurlv,
+                const float *divcurlB, const double *Bevol, double *dBevol,
+                float *divBsymm,
+                const double *dustfrac, double *ddustevol, double *deltav,
+                double *tstop,
+                const float *gradh, int32_t ldgradh, const float *alphaind,
+                const double *eos_vars, int32_t ldeos, const int8_t *iphase,
+                double *dtcourant, double *dtforce);
+
+/* get_neighbour_stats (dens.F90:1137-1155) of the last densityiterate. */
+int phsph_get_neighbour_stats(phsph_ctx *ctx, double *trialmean,
+                              double *actualmean, int32_t *maxtrial,
+                              int32_t *maxactual, int64_t *nrhocalc,
+                              int64_t *nactualtot);
+
+/*
+ * Kernel timing of the last call on the context's stream, measured with
+ * HIP events around each launch: up to `cap` (name, milliseconds) entries
+ * are written; returns the number of entries available.
+ */
+int phsph_get_timings(phsph_ctx *ctx, const char **names, double *ms,
+                      int32_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHSPH_H */

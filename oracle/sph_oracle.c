@@ -1,0 +1,1433 @@
+/*
+ * oracle/sph_oracle.c -- CPU restatement (plain C, FP64) of Phantom's SPH
+ * hot path: kd-tree build, tree walk, density iteration, cons2prim and the
+ * force pair loop (hydro, Cullen-Dehnen viscosity, conductivity, ideal MHD
+ * with div B cleaning, one-fluid dust).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity checker of the HIP
+ * implementation: only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline may load it.  The product library never links or calls it.
+ *
+ * Semantics follow the reference executed with one OpenMP thread (the only
+ * execution whose tree is independent of the thread count), cell by cell
+ * in node order.  One deliberate difference: in the div B sum the density
+ * rho_j = rhoh(h_j) of a neighbour uses the smoothing length it had when
+ * densityiterate was entered.  The reference reads h_j while other cells
+ * are overwriting it (an acknowledged race, config.F90:215-217,
+ * dens.F90:810-811), so its value there depends on the cell schedule.
+ *
+ * Each routine cites the reference lines it restates.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "../include/phsph.h"
+
+#define EPS64 2.220446049250313e-16      /* epsilon(1d0)                  */
+#define TINY64 2.2250738585072014e-308   /* tiny(1d0)                     */
+#define HUGE64 1.7976931348623157e308    /* huge(1d0)                     */
+#define BIGNUMBER 1.e29                  /* timestep.f90:40               */
+#define PI_PH 3.1415926536               /* physcon.f90:23 (sic)          */
+#define MAXDENSITS 100                   /* dens.F90:101                  */
+#define ISIZECELLCACHE 1000              /* dens.F90:99, force.F90:56     */
+#define MINPART 10                       /* config.F90:113                */
+#define ISTACKSIZE 512                   /* kdtree.F90:131                */
+
+/* rhosum indices (dens.F90:51-93), 0-based */
+enum { R_RHO = 0, R_GRADH, R_GRADSOFT, R_DIVV,
+       R_DVXDX, R_DVXDY, R_DVXDZ, R_DVYDX, R_DVYDY, R_DVYDZ, R_DVZDX, R_DVZDY, R_DVZDZ,
+       R_DAXDX, R_DAXDY, R_DAXDZ, R_DAYDX, R_DAYDY, R_DAYDZ, R_DAZDX, R_DAZDY, R_DAZDZ,
+       R_RXX, R_RXY, R_RXZ, R_RYY, R_RYZ, R_RZZ,
+       R_DIVB, R_DBXDX, R_DBXDY, R_DBXDZ, R_DBYDX, R_DBYDY, R_DBYDZ, R_DBZDX, R_DBZDY, R_DBZDZ,
+       NRHOSUM };
+
+typedef struct {
+  phsph_params p;
+  /* kernel constants (kernel_cubic.f90:23-30, kernel_quintic.f90:23-30) */
+  double radkern, radkern2, cnormk, wab0, gradh0, dphidh0;
+  double dxbound, dybound, dzbound, hdlx, hdly, hdlz;
+  int32_t np;
+  int64_t ncellsmax;
+  /* tree (kdtree.F90:31-36, neigh_kdtree.f90:30-36), 1-based node index */
+  phsph_kdnode *node;
+  int32_t *leaf_is_active;   /* [ncellsmax+2] */
+  int32_t *ir_lo, *ir_hi;    /* inoderange(1:2, n), 1-based positions   */
+  int32_t *inodeparts;       /* [np+1], 1-based, signed                 */
+  double *treecache;         /* 5 per position, position p at 5*(p-1)   */
+  int64_t ncells;
+  int32_t maxlevel, minlevel, maxlevel_indexed;
+  int tree_built;
+  /* neighbour statistics (dens.F90:106-108) */
+  int64_t nneightry, nneighact, maxneightry, ncalc, nptot;
+  int32_t maxneighact;
+  char err[256];
+} orc_ctx;
+
+/* ------------------------------------------------------------------ */
+/* kernels                                                             */
+/* ------------------------------------------------------------------ */
+
+/* get_kernel (kernel_cubic.f90:34-51, kernel_quintic.f90:34-56) */
+static inline void get_kernel(int kind, double q2, double q, double *w, double *g) {
+  if (kind == PHSPH_KERNEL_CUBIC) {
+    if (q < 1.) {
+      *w = 0.75 * q2 * q - 1.5 * q2 + 1.;
+      *g = q * (2.25 * q - 3.);
+    } else if (q < 2.) {
+      double t = q - 2.;
+      *w = -0.25 * (t * t * t);
+      *g = -0.75 * (t * t);
+    } else { *w = 0.; *g = 0.; }
+  } else {
+    if (q < 1.) {
+      double q4 = q2 * q2;
+      *w = -10. * q4 * q + 30. * q4 - 60. * q2 + 66.;
+      *g = q * (-50. * q2 * q + 120. * q2 - 120.);
+    } else if (q < 2.) {
+      double a = q - 3., b = q - 2.;
+      double a2 = a * a, b2 = b * b;
+      *w = -(a2 * a2 * a) + 6. * (b2 * b2 * b);
+      *g = -5. * (a2 * a2) + 30. * (b2 * b2);
+    } else if (q < 3.) {
+      double a = q - 3.;
+      double a2 = a * a;
+      *w = -(a2 * a2 * a);
+      *g = -5. * (a2 * a2);
+    } else { *w = 0.; *g = 0.; }
+  }
+}
+
+/* grkern (kernel_cubic.f90:67-78, kernel_quintic.f90:75-88) */
+static inline double grkern(int kind, double q2, double q) {
+  double w, g;
+  get_kernel(kind, q2, q, &w, &g);
+  return g;
+}
+
+/* get_kernel_grav1 dphidh term (kernel_cubic.f90:80-101,
+ * kernel_quintic.f90:90-122) */
+static inline double dphidh_kern(int kind, double q2, double q) {
+  double q4 = q2 * q2;
+  if (kind == PHSPH_KERNEL_CUBIC) {
+    if (q < 1.) return -0.6 * q4 * q + 1.5 * q4 - 2. * q2 + 1.4;
+    if (q < 2.) return 0.2 * q4 * q - 1.5 * q4 + 4. * q2 * q - 4. * q2 + 1.6;
+    return 0.;
+  } else {
+    double q6 = q4 * q2;
+    if (q < 1.) return q6 * q / 21. - q6 / 6. + q4 / 2. - 11. * q2 / 10. + 239. / 210.;
+    if (q < 2.) return -q6 * q / 42. + q6 / 4. - q4 * q + 7. * q4 / 4. - 5. * q2 * q / 6. - 17. * q2 / 20. + 473. / 420.;
+    if (q < 3.) return q6 * q / 210. - q6 / 12. + 3. * q4 * q / 5. - 9. * q4 / 4. + 9. * q2 * q / 2. - 81. * q2 / 20. + 243. / 140.;
+    return 0.;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* particle helpers (part.F90:779-858)                                 */
+/* ------------------------------------------------------------------ */
+static inline double rhoh(const orc_ctx *o, double h, double m) {
+  double t = o->p.hfact / fabs(h);
+  return m * (t * t * t);
+}
+static inline double dhdrho(const orc_ctx *o, double h, double m) {
+  return -h / (3. * rhoh(o, h, m));
+}
+static inline double hrho8(const orc_ctx *o, double rho, double m) {
+  return o->p.hfact * pow(m / fabs(rho), 1.0 / 3.0);
+}
+
+/* ------------------------------------------------------------------ */
+/* context                                                             */
+/* ------------------------------------------------------------------ */
+static void set_derived(orc_ctx *o) {
+  if (o->p.kernel == PHSPH_KERNEL_CUBIC) {
+    o->radkern = 2.; o->radkern2 = 4.; o->cnormk = 1. / PI_PH;
+    o->wab0 = 1.; o->gradh0 = -3.; o->dphidh0 = 1.4;
+  } else {
+    o->radkern = 3.; o->radkern2 = 9.; o->cnormk = 1. / (120. * PI_PH);
+    o->wab0 = 66.; o->gradh0 = -3. * 66.; o->dphidh0 = 239. / 210.;
+  }
+  /* set_boundary (boundary.f90:103-108) */
+  o->dxbound = o->p.xmax - o->p.xmin;
+  o->dybound = o->p.ymax - o->p.ymin;
+  o->dzbound = o->p.zmax - o->p.zmin;
+  o->hdlx = 0.5 * o->dxbound; o->hdly = 0.5 * o->dybound; o->hdlz = 0.5 * o->dzbound;
+}
+
+void orc_default_params(phsph_params *p, int32_t kernel) {
+  memset(p, 0, sizeof(*p));
+  p->kernel = kernel;
+  p->periodic = 1;
+  p->maxvxyzu = 4;
+  p->ieos = 2;
+  p->idrag = 2;
+  p->ishock_heating = 1; p->ipdv_heating = 1; p->iresistive_heating = 1;
+  p->xmin = -0.5; p->xmax = 0.5; p->ymin = -0.5; p->ymax = 0.5; p->zmin = -0.5; p->zmax = 0.5;
+  p->hfact = (kernel == PHSPH_KERNEL_CUBIC) ? 1.2 : 1.0;
+  p->tolh = 1.e-4;
+  p->massoftype_gas = 1.;
+  p->gamma = 5. / 3.; p->polyk = 1.;
+  p->alpha = 0.; p->alphamax = 1.; p->alphau = 1.; p->alphaB = 1.; p->beta = 2.;
+  p->C_cour = 0.3; p->C_force = 0.25; p->dtmax = 1.e29;
+  p->psidecayfac = 1.; p->overcleanfac = 1.;
+  p->K_code = 1.;
+  p->tree_accuracy = 0.5;
+}
+
+orc_ctx *orc_create(const phsph_params *p) {
+  orc_ctx *o = (orc_ctx *)calloc(1, sizeof(orc_ctx));
+  o->p = *p;
+  set_derived(o);
+  o->nptot = -1;
+  return o;
+}
+
+static void free_tree(orc_ctx *o) {
+  free(o->node); free(o->leaf_is_active); free(o->ir_lo); free(o->ir_hi);
+  free(o->inodeparts); free(o->treecache);
+  o->node = NULL; o->leaf_is_active = NULL; o->ir_lo = o->ir_hi = NULL;
+  o->inodeparts = NULL; o->treecache = NULL;
+  o->tree_built = 0;
+}
+
+void orc_destroy(orc_ctx *o) {
+  if (!o) return;
+  free_tree(o);
+  free(o);
+}
+
+const char *orc_last_error(const orc_ctx *o) { return o->err; }
+
+/* ------------------------------------------------------------------ */
+/* tree build                                                          */
+/* ------------------------------------------------------------------ */
+typedef struct { int32_t node, parent, level, npnode; double xmin[3], xmax[3]; } bstack_t;
+
+#define TC(o, pos, c) ((o)->treecache[5 * (size_t)((pos) - 1) + (c)])
+
+/* sort_particles_in_cell (kdtree.F90:937-1001): two-pointer partition on
+ * treecache(iaxis,:) <= xpivot, swapping inodeparts and treecache(1:5). */
+static void sort_particles_in_cell(orc_ctx *o, int iaxis, int32_t imin, int32_t imax,
+                                   int32_t *min_l, int32_t *max_l, int32_t *min_r, int32_t *max_r,
+                                   int32_t *nl, int32_t *nr, double xpivot) {
+  int32_t i = imin, j = imax;
+  double xi_c = TC(o, i, iaxis), xj_c = TC(o, j, iaxis);
+  int i_lt = xi_c <= xpivot, j_lt = xj_c <= xpivot;
+  while (i < j) {
+    if (i_lt) {
+      i = i + 1;
+      xi_c = TC(o, i, iaxis);
+      i_lt = xi_c <= xpivot;
+    } else {
+      if (!j_lt) {
+        j = j - 1;
+        xj_c = TC(o, j, iaxis);
+        j_lt = xj_c <= xpivot;
+      } else {
+        int32_t ps = o->inodeparts[i];
+        double sw[5];
+        memcpy(sw, &TC(o, i, 0), sizeof(sw));
+        o->inodeparts[i] = o->inodeparts[j];
+        memcpy(&TC(o, i, 0), &TC(o, j, 0), sizeof(sw));
+        o->inodeparts[j] = ps;
+        memcpy(&TC(o, j, 0), sw, sizeof(sw));
+        i = i + 1;
+        j = j - 1;
+        xi_c = TC(o, i, iaxis);
+        xj_c = TC(o, j, iaxis);
+        i_lt = xi_c <= xpivot;
+        j_lt = xj_c <= xpivot;
+      }
+    }
+  }
+  if (!i_lt) i = i - 1;
+  if (j_lt) j = j + 1;
+  *min_l = imin; *max_l = i; *min_r = j; *max_r = imax;
+  *nl = *max_l - *min_l + 1;
+  *nr = *max_r - *min_r + 1;
+}
+
+/* construct_node (kdtree.F90:531-929), non-MPI, non-APR branch. */
+static int construct_node(orc_ctx *o, int32_t nnode, int32_t mymum, int32_t level,
+                          const double xmini[3], const double xmaxi[3], int32_t npnode,
+                          int32_t *il, int32_t *ir, int32_t *nl, int32_t *nr,
+                          double xminl[3], double xmaxl[3], double xminr[3], double xmaxr[3],
+                          int *wassplit) {
+  phsph_kdnode *nd = &o->node[nnode];
+  int nodeisactive = 0;
+  int32_t npcounter = 0;
+  if (o->ir_lo[nnode] > 0) {
+    for (int32_t i = o->ir_lo[nnode]; i <= o->ir_hi[nnode]; i++)
+      if (o->inodeparts[i] > 0) { nodeisactive = 1; break; }
+    npcounter = o->ir_hi[nnode] - o->ir_lo[nnode] + 1;
+  }
+  if (npcounter != npnode) {
+    snprintf(o->err, sizeof(o->err), "maketree: node %d has %d particles, expected %d", nnode, npcounter, npnode);
+    return PHSPH_ERR_STATE;
+  }
+  *il = *ir = *nl = *nr = 0;
+  *wassplit = 0;
+  if (npnode < 1) return PHSPH_OK;
+
+  double hmax = 0., xcofm = 0., ycofm = 0., zcofm = 0., totmass = 0.;
+  double pmassi = o->p.massoftype_gas, dfac, fac;
+  dfac = (pmassi > 0.) ? 1. / pmassi : 1.;
+  int32_t i1 = o->ir_lo[nnode];
+  for (int32_t i = i1; i <= i1 + npnode - 1; i++) {
+    double xi = TC(o, i, 0), yi = TC(o, i, 1), zi = TC(o, i, 2), hi = TC(o, i, 3);
+    pmassi = TC(o, i, 4);
+    fac = pmassi * dfac;
+    hmax = (hi > hmax) ? hi : hmax;
+    totmass = totmass + pmassi;
+    xcofm = xcofm + fac * xi;
+    ycofm = ycofm + fac * yi;
+    zcofm = zcofm + fac * zi;
+  }
+  double cofm[3] = {xcofm, ycofm, zcofm};
+  if (totmass > 0.) {
+    double d = totmass * dfac;
+    cofm[0] = cofm[0] / d; cofm[1] = cofm[1] / d; cofm[2] = cofm[2] / d;
+  }
+  if (totmass <= 0.) {
+    snprintf(o->err, sizeof(o->err), "maketree: totmass_node <= 0");
+    return PHSPH_ERR_INPUT;
+  }
+  double r2max = 0.;
+  for (int32_t i = i1; i <= i1 + npnode - 1; i++) {
+    double dx = TC(o, i, 0) - cofm[0];
+    double dy = TC(o, i, 1) - cofm[1];
+    double dz = TC(o, i, 2) - cofm[2];
+    double dr2 = dx * dx + dy * dy + dz * dz;
+    r2max = (dr2 > r2max) ? dr2 : r2max;
+  }
+  nd->xcen[0] = cofm[0]; nd->xcen[1] = cofm[1]; nd->xcen[2] = cofm[2];
+  nd->size = sqrt(r2max) + EPS64;
+  nd->hmax = hmax;
+  nd->mass = totmass;
+  nd->parent = mymum;
+  nd->idum = 0;
+
+  *wassplit = (npnode > MINPART);
+  if (!*wassplit) {
+    nd->leftchild = 0; nd->rightchild = 0;
+    if (level > o->maxlevel) o->maxlevel = level;
+    if (level < o->minlevel) o->minlevel = level;
+    if (o->p.ind_timesteps) o->leaf_is_active[nnode] = nodeisactive ? 1 : -1;
+    else o->leaf_is_active[nnode] = 1;
+    return PHSPH_OK;
+  }
+  /* split along the longest axis at the centre of mass (:815-816) */
+  int iaxis = 0;
+  double ext = xmaxi[0] - xmini[0];
+  for (int a = 1; a < 3; a++) {
+    double e = xmaxi[a] - xmini[a];
+    if (e > ext) { ext = e; iaxis = a; }
+  }
+  double xpivot = cofm[iaxis];
+  if (level < o->maxlevel_indexed) {
+    *il = 2 * nnode; *ir = *il + 1;
+  } else {
+    o->ncells += 2;
+    *ir = (int32_t)o->ncells; *il = (int32_t)(o->ncells - 1);
+    if (*ir > o->ncellsmax) {
+      snprintf(o->err, sizeof(o->err), "maketree: number of nodes exceeds ncellsmax");
+      return PHSPH_ERR_CAPACITY;
+    }
+  }
+  nd->leftchild = *il; nd->rightchild = *ir;
+  o->leaf_is_active[nnode] = 0;
+
+  sort_particles_in_cell(o, iaxis, o->ir_lo[nnode], o->ir_hi[nnode], &o->ir_lo[*il], &o->ir_hi[*il],
+                         &o->ir_lo[*ir], &o->ir_hi[*ir], nl, nr, xpivot);
+  if (*nl == npnode || *nr == npnode) {
+    *nl = npnode / 2;
+    o->ir_lo[*il] = o->ir_lo[nnode];
+    o->ir_hi[*il] = o->ir_lo[nnode] + *nl - 1;
+    o->ir_lo[*ir] = o->ir_lo[nnode] + *nl;
+    o->ir_hi[*ir] = o->ir_hi[nnode];
+    *nr = npnode - *nl;
+  }
+  for (int c = 0; c < 3; c++) { xminl[c] = xmaxl[c] = TC(o, o->ir_lo[*il], c); }
+  for (int32_t ip = o->ir_lo[*il] + 1; ip <= o->ir_hi[*il]; ip++)
+    for (int c = 0; c < 3; c++) {
+      double v = TC(o, ip, c);
+      if (v < xminl[c]) xminl[c] = v;
+      if (v > xmaxl[c]) xmaxl[c] = v;
+    }
+  for (int c = 0; c < 3; c++) { xminr[c] = xmaxr[c] = TC(o, o->ir_lo[*ir], c); }
+  for (int32_t ip = o->ir_lo[*ir] + 1; ip <= o->ir_hi[*ir]; ip++)
+    for (int c = 0; c < 3; c++) {
+      double v = TC(o, ip, c);
+      if (v < xminr[c]) xminr[c] = v;
+      if (v > xmaxr[c]) xmaxr[c] = v;
+    }
+  return PHSPH_OK;
+}
+
+/* maketree (kdtree.F90:117-314) with construct_root_node (:347-488),
+ * executed as with numthreads = 1: no queue phase, one depth-first stack. */
+int orc_maketree(orc_ctx *o, double *xyzh, int32_t np, const int8_t *iphase) {
+  free_tree(o);
+  o->np = np;
+  int64_t maxp = (o->p.maxp > np) ? o->p.maxp : np;
+  o->ncellsmax = 2 * maxp;
+  size_t nslots = (size_t)o->ncellsmax + 2;
+  o->node = (phsph_kdnode *)calloc(nslots, sizeof(phsph_kdnode));
+  o->leaf_is_active = (int32_t *)calloc(nslots, sizeof(int32_t));
+  o->ir_lo = (int32_t *)calloc(nslots, sizeof(int32_t));
+  o->ir_hi = (int32_t *)calloc(nslots, sizeof(int32_t));
+  o->inodeparts = (int32_t *)calloc((size_t)np + 2, sizeof(int32_t));
+  o->treecache = (double *)calloc(5 * ((size_t)np + 1), sizeof(double));
+  if (!o->node || !o->leaf_is_active || !o->ir_lo || !o->ir_hi || !o->inodeparts || !o->treecache) {
+    snprintf(o->err, sizeof(o->err), "maketree: out of host memory");
+    return PHSPH_ERR_CAPACITY;
+  }
+  /* construct_root_node: bounds start from particle 1 before any wrap */
+  double xminp = xyzh[0], yminp = xyzh[1], zminp = xyzh[2];
+  double xmaxp = xminp, ymaxp = yminp, zmaxp = zminp;
+  for (int32_t i = 0; i < np; i++) {
+    double *r = &xyzh[4 * (size_t)i];
+    if (r[3] < TINY64) continue;
+    if (o->p.periodic) {  /* cross_boundary (boundary.f90:123-160) */
+      if (r[0] < o->p.xmin) r[0] = r[0] + o->dxbound;
+      else if (r[0] > o->p.xmax) r[0] = r[0] - o->dxbound;
+      if (r[1] < o->p.ymin) r[1] = r[1] + o->dybound;
+      else if (r[1] > o->p.ymax) r[1] = r[1] - o->dybound;
+      if (r[2] < o->p.zmin) r[2] = r[2] + o->dzbound;
+      else if (r[2] > o->p.zmax) r[2] = r[2] - o->dzbound;
+    }
+    if (isnan(r[0]) || isnan(r[1]) || isnan(r[2])) {
+      snprintf(o->err, sizeof(o->err), "maketree: NaN in particle position %d", i + 1);
+      return PHSPH_ERR_NAN;
+    }
+    if (r[0] < xminp) xminp = r[0];
+    if (r[1] < yminp) yminp = r[1];
+    if (r[2] < zminp) zminp = r[2];
+    if (r[0] > xmaxp) xmaxp = r[0];
+    if (r[1] > ymaxp) ymaxp = r[1];
+    if (r[2] > zmaxp) zmaxp = r[2];
+  }
+  int32_t nproot = 0;
+  for (int32_t i = 0; i < np; i++) {
+    const double *r = &xyzh[4 * (size_t)i];
+    if (r[3] < TINY64) continue;
+    nproot++;
+    int active = (iphase == NULL) ? 1 : (iphase[i] > 0);
+    o->inodeparts[nproot] = (o->p.ind_timesteps && !active) ? -(i + 1) : (i + 1);
+    TC(o, nproot, 0) = r[0]; TC(o, nproot, 1) = r[1]; TC(o, nproot, 2) = r[2]; TC(o, nproot, 3) = r[3];
+    TC(o, nproot, 4) = o->p.massoftype_gas;
+  }
+  if (nproot == 0) {
+    snprintf(o->err, sizeof(o->err), "maketree: no particles or all particles dead/accreted");
+    return PHSPH_ERR_NODATA;
+  }
+  o->ir_lo[1] = 1; o->ir_hi[1] = nproot;
+
+  o->maxlevel = 0;
+  o->minlevel = 32 - 1;
+  o->maxlevel_indexed = (int32_t)(log((double)(o->ncellsmax + 1)) / log(2.)) - 1;
+  o->ncells = ((int64_t)1 << (o->maxlevel_indexed + 1)) - 1;
+
+  bstack_t *stack = (bstack_t *)malloc(ISTACKSIZE * sizeof(bstack_t));
+  int istack = 1;
+  stack[0].node = 1; stack[0].parent = 0; stack[0].level = 0; stack[0].npnode = nproot;
+  stack[0].xmin[0] = xminp; stack[0].xmin[1] = yminp; stack[0].xmin[2] = zminp;
+  stack[0].xmax[0] = xmaxp; stack[0].xmax[1] = ymaxp; stack[0].xmax[2] = zmaxp;
+  int rc = PHSPH_OK;
+  while (istack > 0) {
+    bstack_t s = stack[istack - 1];
+    istack--;
+    int32_t il, ir, nl, nr;
+    double xminl[3], xmaxl[3], xminr[3], xmaxr[3];
+    int wassplit;
+    rc = construct_node(o, s.node, s.parent, s.level, s.xmin, s.xmax, s.npnode, &il, &ir, &nl, &nr,
+                        xminl, xmaxl, xminr, xmaxr, &wassplit);
+    if (rc) break;
+    if (wassplit) {
+      if (istack + 2 > ISTACKSIZE) {
+        snprintf(o->err, sizeof(o->err), "maketree: stack size exceeded");
+        rc = PHSPH_ERR_CAPACITY; break;
+      }
+      bstack_t *a = &stack[istack++];
+      a->node = il; a->parent = s.node; a->level = s.level + 1; a->npnode = nl;
+      memcpy(a->xmin, xminl, sizeof(xminl)); memcpy(a->xmax, xmaxl, sizeof(xmaxl));
+      bstack_t *b = &stack[istack++];
+      b->node = ir; b->parent = s.node; b->level = s.level + 1; b->npnode = nr;
+      memcpy(b->xmin, xminr, sizeof(xminr)); memcpy(b->xmax, xmaxr, sizeof(xmaxr));
+    }
+  }
+  free(stack);
+  if (rc) return rc;
+  if (o->maxlevel < o->maxlevel_indexed) o->ncells = ((int64_t)1 << (o->maxlevel + 1)) - 1;
+  o->tree_built = 1;
+  return PHSPH_OK;
+}
+
+int orc_get_tree(const orc_ctx *o, phsph_kdnode *node, int32_t *inoderange, int32_t *inodeparts,
+                 int32_t *leaf_is_active, int64_t cap_nodes, int64_t *ncells, int32_t *maxlevel) {
+  if (!o->tree_built) return PHSPH_ERR_STATE;
+  int64_t n = o->ncells < cap_nodes ? o->ncells : cap_nodes;
+  for (int64_t k = 0; k < n; k++) {
+    if (node) node[k] = o->node[k + 1];
+    if (inoderange) { inoderange[2 * k] = o->ir_lo[k + 1]; inoderange[2 * k + 1] = o->ir_hi[k + 1]; }
+    if (leaf_is_active) leaf_is_active[k] = o->leaf_is_active[k + 1];
+  }
+  if (inodeparts) for (int32_t p = 0; p < o->np; p++) inodeparts[p] = o->inodeparts[p + 1];
+  if (ncells) *ncells = o->ncells;
+  if (maxlevel) *maxlevel = o->maxlevel;
+  return PHSPH_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* tree walk                                                           */
+/* ------------------------------------------------------------------ */
+
+/* get_sep (kdtree.F90:1468-1501) */
+static inline void get_sep(const orc_ctx *o, const double x1[3], const double x2[3], double *dx,
+                           double *dy, double *dz, double *xoff, double *yoff, double *zoff, double *r2) {
+  *xoff = *yoff = *zoff = 0.;
+  *dx = x1[0] - x2[0]; *dy = x1[1] - x2[1]; *dz = x1[2] - x2[2];
+  if (o->p.periodic) {
+    if (fabs(*dx) > o->hdlx) { *xoff = copysign(o->dxbound, *dx); *dx = *dx - *xoff; }
+    if (fabs(*dy) > o->hdly) { *yoff = copysign(o->dybound, *dy); *dy = *dy - *yoff; }
+    if (fabs(*dz) > o->hdlz) { *zoff = copysign(o->dzbound, *dz); *dz = *dz - *zoff; }
+  }
+  *r2 = *dx * *dx + *dy * *dy + *dz * *dz;
+}
+
+typedef struct {
+  int32_t *list;     /* 1-based particle indices                  */
+  double *cache;     /* 4 per entry: x, y, z (offset), 1/h        */
+  int64_t cap;
+  int32_t n;
+} nlist_t;
+
+static int nlist_reserve(nlist_t *l, int64_t need) {
+  if (need <= l->cap) return 0;
+  int64_t nc = l->cap ? l->cap : 4096;
+  while (nc < need) nc *= 2;
+  int32_t *nl2 = (int32_t *)realloc(l->list, nc * sizeof(int32_t));
+  double *nc2 = (double *)realloc(l->cache, nc * 4 * sizeof(double));
+  if (!nl2 || !nc2) return -1;
+  l->list = nl2; l->cache = nc2; l->cap = nc;
+  return 0;
+}
+
+/* getneigh (kdtree.F90:1221-1347) + cache_neighbours (:1175-1213) for
+ * the non-gravity walk.  get_hj selects rcut = max(rcuti, radkern*hmax_j). */
+static int getneigh(const orc_ctx *o, const double xpos[3], double xsizei, double rcuti, int get_hj,
+                    nlist_t *l) {
+  int32_t nstack[64];
+  int istack = 1;
+  nstack[0] = 1;
+  l->n = 0;
+  while (istack != 0) {
+    int32_t n = nstack[--istack];
+    const phsph_kdnode *nd = &o->node[n];
+    double dx, dy, dz, xo, yo, zo, r2;
+    get_sep(o, xpos, nd->xcen, &dx, &dy, &dz, &xo, &yo, &zo, &r2);
+    double rcut = rcuti;
+    if (get_hj) {
+      double rcutj = o->radkern * nd->hmax;
+      rcut = (rcuti > rcutj) ? rcuti : rcutj;
+    }
+    double s = xsizei + nd->size + rcut;
+    if (r2 < s * s) {
+      if (o->leaf_is_active[n] != 0) {
+        int32_t npnode = o->ir_hi[n] - o->ir_lo[n] + 1;
+        if (nlist_reserve(l, (int64_t)l->n + npnode)) return PHSPH_ERR_CAPACITY;
+        for (int32_t k = 0; k < npnode; k++) {
+          int32_t pos = o->ir_lo[n] + k;
+          int32_t idx = l->n + k;
+          l->list[idx] = abs(o->inodeparts[pos]);
+          l->cache[4 * idx + 0] = TC(o, pos, 0) + xo;
+          l->cache[4 * idx + 1] = TC(o, pos, 1) + yo;
+          l->cache[4 * idx + 2] = TC(o, pos, 2) + zo;
+          l->cache[4 * idx + 3] = 1. / TC(o, pos, 3);
+        }
+        l->n += npnode;
+      } else {
+        if (istack + 2 > 64) return PHSPH_ERR_CAPACITY;
+        if (nd->leftchild != 0) nstack[istack++] = nd->leftchild;
+        if (nd->rightchild != 0) nstack[istack++] = nd->rightchild;
+      }
+    }
+  }
+  return PHSPH_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* density                                                             */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int32_t icell, npcell, nits, nneightry;
+  int32_t arr_index[MINPART + 1];
+  int32_t nneigh[MINPART + 1];
+  double h[MINPART + 1], h_old[MINPART + 1];
+  double xpv[MINPART + 1][14];  /* x y z vx vy vz u Bx By Bz psi fx fy fz (dens.F90:33-47) */
+  double rhosums[MINPART + 1][NRHOSUM];
+  double xpos[3], xsizei, rcuti, hmax;
+} celldens_t;
+
+typedef struct {
+  const double *xyzh, *vxyzu, *fxyzu, *fext, *Bevol;
+  const double *h_in;  /* snapshot of h at entry (div B rho_j) */
+  int32_t ldv, ldf;
+} densin_t;
+
+/* get_density_sums (dens.F90:578-857) for one particle. */
+static void get_density_sums(const orc_ctx *o, int32_t i, const double *xpv, double hi, double hi1,
+                             double hi21, const nlist_t *l, int32_t *nneighi, double *rs,
+                             const densin_t *in) {
+  const int kind = o->p.kernel;
+  const double pm = o->p.massoftype_gas;
+  const int getdB = o->p.mhd;
+  memset(rs, 0, NRHOSUM * sizeof(double));
+  *nneighi = 1;
+  double fxi = xpv[11], fyi = xpv[12], fzi = xpv[13];
+  for (int32_t n = 0; n < l->n; n++) {
+    int32_t j = l->list[n];
+    if (j == i) continue;
+    double dx, dy, dz;
+    if (n < ISIZECELLCACHE) {
+      dx = xpv[0] - l->cache[4 * n + 0];
+      dy = xpv[1] - l->cache[4 * n + 1];
+      dz = xpv[2] - l->cache[4 * n + 2];
+    } else {
+      const double *rj = &in->xyzh[4 * (size_t)(j - 1)];
+      dx = xpv[0] - rj[0]; dy = xpv[1] - rj[1]; dz = xpv[2] - rj[2];
+    }
+    if (o->p.periodic) {
+      if (fabs(dx) > 0.5 * o->dxbound) dx = dx - copysign(o->dxbound, dx);
+      if (fabs(dy) > 0.5 * o->dybound) dy = dy - copysign(o->dybound, dy);
+      if (fabs(dz) > 0.5 * o->dzbound) dz = dz - copysign(o->dzbound, dz);
+    }
+    double rij2 = dx * dx + dy * dy + dz * dz;
+    double q2i = rij2 * hi21;
+    if (!(q2i < o->radkern2)) continue;
+    double rij = sqrt(rij2);
+    double qi = rij * hi1;
+    double wabi, grkerni, dphidhi = 0.;
+    get_kernel(kind, q2i, qi, &wabi, &grkerni);
+    if (o->p.gravity) dphidhi = dphidh_kern(kind, q2i, qi);
+    double dwdhi = (-qi * grkerni - 3. * wabi);
+    rs[R_RHO] = rs[R_RHO] + wabi * pm;
+    rs[R_GRADH] = rs[R_GRADH] + dwdhi * pm;
+    rs[R_GRADSOFT] = rs[R_GRADSOFT] + dphidhi * pm;
+    *nneighi += 1;
+    double rij1 = 1. / (rij + EPS64);
+    double rij1grkern = rij1 * grkerni;
+    double runix = dx * rij1grkern * pm;
+    double runiy = dy * rij1grkern * pm;
+    double runiz = dz * rij1grkern * pm;
+    const double *vj = &in->vxyzu[(size_t)in->ldv * (j - 1)];
+    double dvx = xpv[3] - vj[0], dvy = xpv[4] - vj[1], dvz = xpv[5] - vj[2];
+    double projv = dvx * runix + dvy * runiy + dvz * runiz;
+    rs[R_DIVV] += projv;
+    rs[R_DVXDX] += dvx * runix; rs[R_DVXDY] += dvx * runiy; rs[R_DVXDZ] += dvx * runiz;
+    rs[R_DVYDX] += dvy * runix; rs[R_DVYDY] += dvy * runiy; rs[R_DVYDZ] += dvy * runiz;
+    rs[R_DVZDX] += dvz * runix; rs[R_DVZDY] += dvz * runiy; rs[R_DVZDZ] += dvz * runiz;
+    double fxj = 0., fyj = 0., fzj = 0.;
+    if (in->fxyzu) {
+      const double *fj = &in->fxyzu[(size_t)in->ldf * (j - 1)];
+      fxj = fj[0]; fyj = fj[1]; fzj = fj[2];
+    }
+    if (in->fext) {
+      const double *ej = &in->fext[3 * (size_t)(j - 1)];
+      fxj = fxj + ej[0]; fyj = fyj + ej[1]; fzj = fzj + ej[2];
+    }
+    double dax = fxi - fxj, day = fyi - fyj, daz = fzi - fzj;
+    rs[R_DAXDX] += dax * runix; rs[R_DAXDY] += dax * runiy; rs[R_DAXDZ] += dax * runiz;
+    rs[R_DAYDX] += day * runix; rs[R_DAYDY] += day * runiy; rs[R_DAYDZ] += day * runiz;
+    rs[R_DAZDX] += daz * runix; rs[R_DAZDY] += daz * runiy; rs[R_DAZDZ] += daz * runiz;
+    rs[R_RXX] -= dx * runix; rs[R_RXY] -= dx * runiy; rs[R_RXZ] -= dx * runiz;
+    rs[R_RYY] -= dy * runiy; rs[R_RYZ] -= dy * runiz; rs[R_RZZ] -= dz * runiz;
+    if (getdB) {
+      double rhoi = rhoh(o, hi, pm);
+      double rhoj = rhoh(o, in->h_in[j - 1], pm);
+      const double *Bj = &in->Bevol[4 * (size_t)(j - 1)];
+      double dBx = xpv[7] * rhoi - Bj[0] * rhoj;
+      double dBy = xpv[8] * rhoi - Bj[1] * rhoj;
+      double dBz = xpv[9] * rhoi - Bj[2] * rhoj;
+      double projdB = dBx * runix + dBy * runiy + dBz * runiz;
+      rs[R_DIVB] += projdB;
+      rs[R_DBXDX] += dBx * runix; rs[R_DBXDY] += dBx * runiy; rs[R_DBXDZ] += dBx * runiz;
+      rs[R_DBYDX] += dBy * runix; rs[R_DBYDY] += dBy * runiy; rs[R_DBYDZ] += dBy * runiz;
+      rs[R_DBZDX] += dBz * runix; rs[R_DBZDY] += dBz * runiy; rs[R_DBZDZ] += dBz * runiz;
+    }
+  }
+}
+
+/* finish_rhosum (dens.F90:1470-1507) */
+static void finish_rhosum(const orc_ctx *o, const double *rs, double pm, double hi, int iterating,
+                          double *rhoi, double *gradhi, double *rhohi, double *gradsofti,
+                          double *dhdrhoi_out, double *omegai_out) {
+  double hi1 = 1. / hi, hi21 = hi1 * hi1, hi31 = hi1 * hi21, hi41 = hi21 * hi21;
+  *rhoi = o->cnormk * (rs[R_RHO] + o->wab0 * pm) * hi31;
+  double gh = o->cnormk * (rs[R_GRADH] + o->gradh0 * pm) * hi41;
+  double dhdrhoi = dhdrho(o, hi, pm);
+  double omegai = 1. - dhdrhoi * gh;
+  *gradhi = 1. / omegai;
+  if (iterating) {
+    *rhohi = rhoh(o, hi, pm);
+    *dhdrhoi_out = dhdrhoi;
+    *omegai_out = omegai;
+  } else {
+    double gs = (rs[R_GRADSOFT] + o->dphidh0 * pm) * hi21;
+    *gradsofti = gs * dhdrhoi;
+  }
+}
+
+/* exactlinear (dens.F90:1086-1101) */
+static inline void exactlinear(double *gx, double *gy, double *gz, double dAx, double dAy, double dAz,
+                               const double rm[6], double ddenom) {
+  *gx = (dAx * rm[0] + dAy * rm[1] + dAz * rm[2]) * ddenom;
+  *gy = (dAx * rm[1] + dAy * rm[3] + dAz * rm[4]) * ddenom;
+  *gz = (dAx * rm[2] + dAy * rm[4] + dAz * rm[5]) * ddenom;
+}
+
+/* calculate_rmatrix_from_sums (dens.F90:866-891) */
+static void rmatrix_from_sums(const double *rs, double *denom, double rm[6]) {
+  double rxx = rs[R_RXX], rxy = rs[R_RXY], rxz = rs[R_RXZ], ryy = rs[R_RYY], ryz = rs[R_RYZ], rzz = rs[R_RZZ];
+  *denom = rxx * ryy * rzz + 2. * rxy * rxz * ryz - rxx * ryz * ryz - ryy * rxz * rxz - rzz * rxy * rxy;
+  rm[0] = ryy * rzz - ryz * ryz;
+  rm[1] = rxz * ryz - rzz * rxy;
+  rm[2] = rxy * ryz - rxz * ryy;
+  rm[3] = rzz * rxx - rxz * rxz;
+  rm[4] = rxy * rxz - rxx * ryz;
+  rm[5] = rxx * ryy - rxy * rxy;
+}
+
+/* velocity gradient (exact-linear when det != 0), as used by both
+ * calculate_divcurlv_from_sums (dens.F90:899-967) and
+ * calculate_strain_from_sums (:997-1052) */
+static void grad_from_sums(const double *rs, int i0, double termnorm, double denom, const double rm[6],
+                           double g[9]) {
+  if (fabs(denom) > TINY64) {
+    double dd = 1. / denom, gx, gy, gz;
+    for (int c = 0; c < 3; c++) {
+      exactlinear(&gx, &gy, &gz, rs[i0 + 3 * c], rs[i0 + 3 * c + 1], rs[i0 + 3 * c + 2], rm, dd);
+      g[3 * c] = -gx; g[3 * c + 1] = -gy; g[3 * c + 2] = -gz;
+    }
+  } else {
+    for (int k = 0; k < 9; k++) g[k] = -rs[i0 + k] * termnorm;
+  }
+}
+
+typedef struct {
+  float *divcurlv; int32_t lddcv;
+  float *divcurlB, *alphaind, *gradh; int32_t ldgradh;
+  float *dvdx;
+  double *xyzh;
+} densout_t;
+
+/* store_results (dens.F90:1511-1681) */
+static void store_results(orc_ctx *o, const celldens_t *c, densout_t *out, int64_t *nneightry,
+                          int64_t *nneighact, int64_t *maxneightry, int32_t *maxneighact, int64_t *npc,
+                          int64_t *ncalc) {
+  const double pm = o->p.massoftype_gas;
+  for (int ip = 0; ip < c->npcell; ip++) {
+    int32_t lli = o->inodeparts[c->arr_index[ip]];
+    double hi = c->h[ip];
+    const double *rs = c->rhosums[ip];
+    double hi1 = 1. / hi, hi21 = hi1 * hi1, hi41 = hi21 * hi21;
+    double rhoi, gradhi, gradsofti, dum1, dum2, dum3;
+    finish_rhosum(o, rs, pm, hi, 0, &rhoi, &gradhi, &dum1, &gradsofti, &dum2, &dum3);
+    double hnew = hrho8(o, rhoi, pm);
+    out->xyzh[4 * (size_t)(lli - 1) + 3] = hnew;
+    TC(o, c->arr_index[ip], 3) = hnew;
+    float gh32 = (float)gradhi;
+    out->gradh[(size_t)out->ldgradh * (lli - 1)] = gh32;
+    if (o->p.gravity && out->ldgradh > 1) out->gradh[(size_t)out->ldgradh * (lli - 1) + 1] = (float)gradsofti;
+    /* calculate_divcurlB branch (fast_divcurlB = .true.) */
+    double gradh_r4 = (double)gh32;
+    double rho1i = 1. / rhoi;
+    double term = o->cnormk * gradh_r4 * rho1i * hi41;
+    double denom, rm[6], g[9], ga[9];
+    rmatrix_from_sums(rs, &denom, rm);
+    out->divcurlv[(size_t)out->lddcv * (lli - 1)] = (float)(-rs[R_DIVV] * term);
+    /* d(div v)/dt (dens.F90:927-964) */
+    grad_from_sums(rs, R_DVXDX, term, denom, rm, g);
+    double div_a;
+    if (fabs(denom) > TINY64) {
+      grad_from_sums(rs, R_DAXDX, term, denom, rm, ga);
+      div_a = -(-ga[0] + -ga[4] + -ga[8]);
+    } else {
+      div_a = -term * (rs[R_DAXDX] + rs[R_DAYDY] + rs[R_DAZDZ]);
+    }
+    double d5 = div_a - (g[0] * g[0] + g[4] * g[4] + g[8] * g[8] +
+                         2. * (g[1] * g[3] + g[2] * g[6] + g[5] * g[7]));
+    if (out->alphaind) out->alphaind[3 * (size_t)(lli - 1) + 2] = (float)d5;
+    if (o->p.mhd && out->divcurlB) {
+      float *dB = &out->divcurlB[4 * (size_t)(lli - 1)];
+      dB[0] = (float)(-rs[R_DIVB] * term);
+      dB[1] = (float)(-(rs[R_DBZDY] - rs[R_DBYDZ]) * term);
+      dB[2] = (float)(-(rs[R_DBXDZ] - rs[R_DBZDX]) * term);
+      dB[3] = (float)(-(rs[R_DBYDX] - rs[R_DBXDY]) * term);
+    }
+    if (out->dvdx)
+      for (int k = 0; k < 9; k++) out->dvdx[9 * (size_t)(lli - 1) + k] = (float)g[k];
+    *nneightry += c->nneightry;
+    *nneighact += c->nneigh[ip];
+    if (c->nneightry > *maxneightry) *maxneightry = c->nneightry;
+    if (c->nneigh[ip] > *maxneighact) *maxneighact = c->nneigh[ip];
+  }
+  *npc += c->npcell;
+  *ncalc += (int64_t)c->npcell * c->nits;
+}
+
+/* densityiterate (dens.F90:117-568), non-MPI path.  nits_part (may be
+ * NULL) receives per-particle the number of the iteration at which its own
+ * h converged (1-based), for comparison with per-particle iteration. */
+int orc_densityiterate(orc_ctx *o, int32_t icall, int32_t np, double *xyzh, const double *vxyzu,
+                       int32_t ldv, const double *fxyzu, int32_t ldf, const double *fext,
+                       const double *Bevol, const int8_t *iphase, float *divcurlv, int32_t lddcv,
+                       float *divcurlB, float *alphaind, float *gradh, int32_t ldgradh, float *dvdx,
+                       int32_t *nits_part) {
+  if (!o->tree_built || np != o->np) {
+    snprintf(o->err, sizeof(o->err), "densityiterate: tree not built for this np");
+    return PHSPH_ERR_STATE;
+  }
+  double *h_in = (double *)malloc((size_t)np * sizeof(double));
+  for (int32_t i = 0; i < np; i++) h_in[i] = xyzh[4 * (size_t)i + 3];
+  densin_t in = {xyzh, vxyzu, fxyzu, fext, Bevol, h_in, ldv, ldf};
+  densout_t out = {divcurlv, lddcv, divcurlB, alphaind, gradh, ldgradh, dvdx, xyzh};
+  int64_t nneightry = 0, nneighact = 0, maxneightry = 0, ncalc = 0, npc = 0;
+  int32_t maxneighact = 0;
+  int rc_all = PHSPH_OK;
+  const double pm = o->p.massoftype_gas;
+  if (nits_part) for (int32_t i = 0; i < np; i++) nits_part[i] = 0;
+
+#pragma omp parallel reduction(+ : nneightry, nneighact, ncalc, npc) reduction(max : maxneightry, maxneighact)
+  {
+    nlist_t l = {0};
+    celldens_t c;
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t icell = 1; icell <= o->ncells; icell++) {
+      if (o->leaf_is_active[icell] <= 0) continue;
+      if (rc_all) continue;
+      const phsph_kdnode *nd = &o->node[icell];
+      /* get_cell_location (neigh_kdtree.f90:354-365) */
+      c.icell = (int32_t)icell;
+      c.xpos[0] = nd->xcen[0]; c.xpos[1] = nd->xcen[1]; c.xpos[2] = nd->xcen[2];
+      c.xsizei = nd->size;
+      c.rcuti = o->radkern * nd->hmax;
+      c.hmax = nd->hmax;
+      c.nits = 0;
+      int rc = getneigh(o, c.xpos, c.xsizei, c.rcuti, 0, &l);
+      if (rc) { rc_all = rc; continue; }
+      /* start_cell (dens.F90:1293-1378) */
+      c.npcell = 0;
+      for (int32_t ip = o->ir_lo[icell]; ip <= o->ir_hi[icell]; ip++) {
+        int32_t i = o->inodeparts[ip];
+        if (i < 0) continue;
+        if (iphase && iphase[i - 1] <= 0) continue;
+        int k = c.npcell++;
+        c.arr_index[k] = ip;
+        const double *r = &xyzh[4 * (size_t)(i - 1)];
+        const double *v = &vxyzu[(size_t)ldv * (i - 1)];
+        c.xpv[k][0] = r[0]; c.xpv[k][1] = r[1]; c.xpv[k][2] = r[2];
+        c.h[k] = h_in[i - 1]; c.h_old[k] = h_in[i - 1];
+        c.xpv[k][3] = v[0]; c.xpv[k][4] = v[1]; c.xpv[k][5] = v[2];
+        c.xpv[k][6] = (o->p.maxvxyzu >= 4) ? v[3] : 0.;
+        double fx = 0., fy = 0., fz = 0.;
+        if (fxyzu) { const double *f = &fxyzu[(size_t)ldf * (i - 1)]; fx = f[0]; fy = f[1]; fz = f[2]; }
+        if (fext) { const double *e = &fext[3 * (size_t)(i - 1)]; fx = fx + e[0]; fy = fy + e[1]; fz = fz + e[2]; }
+        c.xpv[k][11] = fx; c.xpv[k][12] = fy; c.xpv[k][13] = fz;
+        if (o->p.mhd) {
+          const double *B = &Bevol[4 * (size_t)(i - 1)];
+          c.xpv[k][7] = B[0]; c.xpv[k][8] = B[1]; c.xpv[k][9] = B[2]; c.xpv[k][10] = B[3];
+        } else {
+          c.xpv[k][7] = c.xpv[k][8] = c.xpv[k][9] = c.xpv[k][10] = 0.;
+        }
+      }
+      int32_t conv_at[MINPART + 1];
+      for (int k = 0; k < c.npcell; k++) conv_at[k] = 0;
+      /* compute_cell (dens.F90:1203-1271) */
+#define COMPUTE_CELL()                                                                          \
+  for (int k = 0; k < c.npcell; k++) {                                                          \
+    int32_t lli = o->inodeparts[c.arr_index[k]];                                                \
+    double hi = c.h[k], hi1 = 1. / hi, hi21 = hi1 * hi1;                                        \
+    get_density_sums(o, lli, c.xpv[k], hi, hi1, hi21, &l, &c.nneigh[k], c.rhosums[k], &in);      \
+  }                                                                                             \
+  c.nneightry = l.n;
+      COMPUTE_CELL();
+      int converged = 0;
+      while (!converged) {
+        /* finish_cell (dens.F90:1382-1466) */
+        c.nits++;
+        int cell_conv = 1;
+        for (int k = 0; k < c.npcell; k++) {
+          double hi = c.h[k], hi_old = c.h_old[k];
+          double rhoi, gradhi, rhohi, gs, dhdrhoi, omegai;
+          finish_rhosum(o, c.rhosums[k], pm, hi, 1, &rhoi, &gradhi, &rhohi, &gs, &dhdrhoi, &omegai);
+          double func = rhohi - rhoi;
+          double dfdh1 = (omegai > TINY64) ? dhdrhoi / omegai : dhdrhoi / fabs(omegai + EPS64);
+          double hnew = hi - func * dfdh1;
+          if (hnew > 1.2 * hi) hnew = 1.2 * hi;
+          else if (hnew < 0.8 * hi) hnew = 0.8 * hi;
+          int conv = ((fabs(hnew - hi) / hi_old) < o->p.tolh && omegai > 0. && hi > 0.);
+          if (cell_conv) cell_conv = conv;
+          if (!conv && c.nits >= MAXDENSITS) {
+            snprintf(o->err, sizeof(o->err), "densityiterate: could not converge in density");
+            rc_all = PHSPH_ERR_NOCONVERGE;
+          }
+          if (conv) { c.h[k] = hi; if (!conv_at[k]) conv_at[k] = c.nits; }
+          else { c.h[k] = hnew; conv_at[k] = 0; }
+        }
+        converged = cell_conv;
+        if (rc_all) break;
+        /* compute_hmax (dens.F90:1275-1289) */
+        int redo = 0;
+        if (c.npcell > 0) {
+          double hmx = c.h[0];
+          for (int k = 1; k < c.npcell; k++) if (c.h[k] > hmx) hmx = c.h[k];
+          double hmax = 1.01 * hmx;
+          if (hmax > c.hmax) redo = 1;
+          c.hmax = hmax;
+          c.rcuti = o->radkern * hmax;
+        }
+        if (icall == 0) converged = 1;
+        if (!converged) {
+          if (redo) {
+            /* set_hmaxcell (neigh_kdtree.f90:115-131) */
+#pragma omp critical(orc_hmax)
+            {
+              int32_t nn = c.icell;
+              o->node[nn].hmax = c.hmax;
+              while (o->node[nn].parent != 0) {
+                nn = o->node[nn].parent;
+                if (c.hmax > o->node[nn].hmax) o->node[nn].hmax = c.hmax;
+              }
+            }
+            rc = getneigh(o, c.xpos, c.xsizei, c.rcuti, 0, &l);
+            if (rc) { rc_all = rc; break; }
+          }
+          COMPUTE_CELL();
+        }
+      }
+#undef COMPUTE_CELL
+      if (rc_all) continue;
+      if (nits_part)
+        for (int k = 0; k < c.npcell; k++)
+          nits_part[o->inodeparts[c.arr_index[k]] - 1] = conv_at[k] ? conv_at[k] : c.nits;
+      store_results(o, &c, &out, &nneightry, &nneighact, &maxneightry, &maxneighact, &npc, &ncalc);
+    }
+    free(l.list); free(l.cache);
+  }
+  free(h_in);
+  o->nneightry = nneightry; o->nneighact = nneighact; o->maxneightry = maxneightry;
+  o->maxneighact = maxneighact; o->ncalc = ncalc; o->nptot = npc;
+  return rc_all;
+}
+
+int orc_get_neighbour_stats(const orc_ctx *o, double *trialmean, double *actualmean, int32_t *maxtrial,
+                            int32_t *maxactual, int64_t *nrhocalc, int64_t *nactualtot) {
+  if (o->nptot > 0) {
+    *trialmean = (double)o->nneightry / (double)o->nptot;
+    *actualmean = (double)o->nneighact / (double)o->nptot;
+    *maxtrial = (int32_t)o->maxneightry;
+    *maxactual = o->maxneighact;
+    *nrhocalc = o->ncalc;
+    *nactualtot = o->nneighact;
+  } else {
+    *trialmean = *actualmean = -1; *maxtrial = *maxactual = -1; *nrhocalc = *nactualtot = -1;
+  }
+  return PHSPH_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* cons2prim                                                           */
+/* ------------------------------------------------------------------ */
+
+/* xi_limiter (shock_capturing.f90:150-178) */
+static double xi_limiter(const float *dv) {
+  double divv = (double)dv[0] + (double)dv[4] + (double)dv[8];
+  double cx = (double)dv[7] - (double)dv[5];
+  double cy = (double)dv[2] - (double)dv[6];
+  double cz = (double)dv[3] - (double)dv[1];
+  double m = (-divv > 0.) ? -divv : 0.;
+  double fac = m * m;
+  double traceS = cx * cx + cy * cy + cz * cz;
+  if (fac + traceS > EPS64) return fac / (fac + traceS);
+  return 1.;
+}
+
+/* get_alphaloc (shock_capturing.f90:131-144) */
+static double get_alphaloc(double divvdti, double cs, double hi, double xil, double amin, double amax) {
+  double m = (-divvdti > 0.) ? -divvdti : 0.;
+  double source = 10. * (hi * hi) * xil * m;
+  double temp = cs * cs;
+  if (temp > EPS64) {
+    double a = source / temp;
+    if (a > amax) a = amax;
+    if (a < amin) a = amin;
+    return a;
+  }
+  return amin;
+}
+
+/* equationofstate ieos 1 and 2 (eos.f90:184-240): returns P/rho_gas, c_s */
+static void eos12(const orc_ctx *o, double rhogas, double u, double *ponrho, double *cs) {
+  if (o->p.ieos == 1) {
+    *ponrho = o->p.polyk;
+    *cs = sqrt(*ponrho);
+  } else {
+    double g = o->p.gamma;
+    if (o->p.maxvxyzu >= 4) {
+      *ponrho = (g > 1.0001) ? (g - 1.) * u : 2. / 3. * u;
+    } else {
+      *ponrho = o->p.polyk * pow(rhogas, g - 1.);
+    }
+    *cs = sqrt(g * *ponrho);
+  }
+}
+
+/* cons2prim_everything (cons2prim.f90:274-456) */
+int orc_cons2prim(orc_ctx *o, int32_t np, const double *xyzh, const double *vxyzu, int32_t ldv,
+                  const float *dvdx, float *alphaind, const double *dustevol, double *dustfrac,
+                  double *eos_vars, int32_t ldeos, const int8_t *iphase) {
+  (void)iphase;
+  const double pm = o->p.massoftype_gas;
+#pragma omp parallel for schedule(static)
+  for (int32_t i = 0; i < np; i++) {
+    double hi = xyzh[4 * (size_t)i + 3];
+    if (hi < TINY64) continue;
+    double rhoi = rhoh(o, hi, pm);
+    double rhogas = rhoi;
+    if (o->p.use_dustfrac && o->p.ndustsmall > 0) {
+      double s = dustevol[i];
+      double eps = s * s / (1. + s * s);
+      dustfrac[i] = eps;
+      rhogas = rhoi * (1. - eps);
+    }
+    double u = (o->p.maxvxyzu >= 4) ? vxyzu[(size_t)ldv * i + 3] : 0.;
+    double ponrho, cs;
+    eos12(o, rhogas, u, &ponrho, &cs);
+    eos_vars[(size_t)ldeos * i + 0] = ponrho * rhogas;
+    eos_vars[(size_t)ldeos * i + 1] = cs;
+    if (ldeos > 2) eos_vars[(size_t)ldeos * i + 2] = 0.;
+    if (alphaind) {
+      double xil = xi_limiter(&dvdx[9 * (size_t)i]);
+      alphaind[3 * (size_t)i + 1] =
+          (float)get_alphaloc((double)alphaind[3 * (size_t)i + 2], cs, hi, xil, o->p.alpha, o->p.alphamax);
+    }
+  }
+  return PHSPH_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* force                                                               */
+/* ------------------------------------------------------------------ */
+
+/* get_ts (dust.f90:161-276) for idrag = 2 (constant K) and 3 (constant
+ * stopping time) */
+static double get_ts(const orc_ctx *o, double rhogas, double rhodust) {
+  if (o->p.idrag == 2) {
+    double rhosum = rhogas + rhodust;
+    if (o->p.K_code > 0.) return rhogas * rhodust / (o->p.K_code * rhosum);
+    return HUGE64;
+  }
+  if (o->p.idrag == 3) return o->p.K_code;
+  return 0.;
+}
+
+typedef struct {
+  /* per active particle of the cell (force.F90:2172-2514) */
+  double x, y, z, h, vx, vy, vz, u, rho, rhogas, pr, cs, pro2, vwave, alpha, gradh;
+  double Bx_e, By_e, Bz_e, psi, divB;   /* Bevol and divcurlB(1) */
+  double sxx, sxy, sxz, syy, syz, szz;  /* MHD stress / rho^2 terms */
+  double eps, tstop;
+} fpart_t;
+
+/* get_stress (force.F90:2068-2168) without physical viscosity */
+static void get_stress(const orc_ctx *o, double pr, double cs, double rho, double rho1, double pm,
+                       double Bx, double By, double Bz, double *pro2, double *vwave, double s[6]) {
+  s[0] = s[1] = s[2] = s[3] = s[4] = s[5] = 0.;
+  if (o->p.mhd) {
+    double Brx = Bx * rho1, Bry = By * rho1, Brz = Bz * rho1;
+    double Bro2 = Brx * Brx + Bry * Bry + Brz * Brz;
+    double va2 = Bro2 * rho;
+    *vwave = sqrt(cs * cs + va2);
+    s[0] = s[0] - pm * Brx * Brx;
+    s[1] = s[1] - pm * Brx * Bry;
+    s[2] = s[2] - pm * Brx * Brz;
+    s[3] = s[3] - pm * Bry * Bry;
+    s[4] = s[4] - pm * Bry * Brz;
+    s[5] = s[5] - pm * Brz * Brz;
+    *pro2 = pr * rho1 * rho1 + 0. + 0.5 * Bro2;
+  } else {
+    *pro2 = pr * rho1 * rho1 + 0.;
+    *vwave = cs;
+  }
+}
+
+typedef struct {
+  const double *xyzh, *vxyzu, *Bevol, *dustfrac, *eos_vars;
+  const float *gradh, *alphaind, *divcurlB;
+  int32_t ldv, ldgradh, ldeos;
+} forcein_t;
+
+enum { F_FX = 0, F_FY, F_FZ, F_POT, F_DRHODT, F_DUDTDISS, F_DENDTDISS, F_DIVBSYM, F_DBX, F_DBY,
+       F_DBZ, F_DIVBDIFF, F_DDUSTEVOL, F_DUDTDUST, F_DVX, F_DVY, F_DVZ, NFSUM };
+
+/* compute_forces (force.F90:914-2060) for one active gas particle. */
+static void compute_forces(const orc_ctx *o, int32_t i, const fpart_t *pi, const nlist_t *l,
+                           const forcein_t *in, double *fsum, double *vsigmax) {
+  const int kind = o->p.kernel;
+  const double pm = o->p.massoftype_gas;
+  const double beta = o->p.beta, alphau = o->p.alphau, alphaB = o->p.alphaB;
+  const int mhd = o->p.mhd, dust = o->p.use_dustfrac && o->p.ndustsmall > 0;
+  const int thermal = o->p.maxvxyzu >= 4;
+  double hi = pi->h, hi1 = 1. / hi, hi21 = hi1 * hi1, hi41 = hi21 * hi21;
+  double gradhi = pi->gradh;
+  double rhoi = pi->rho, rho1i = 1. / rhoi;
+  double hfacgrkern = hi41 * o->cnormk * gradhi;
+  double Bxi = 0., Byi = 0., Bzi = 0., psii = 0.;
+  if (mhd) { Bxi = pi->Bx_e * rhoi; Byi = pi->By_e * rhoi; Bzi = pi->Bz_e * rhoi; psii = pi->psi; }
+  double epsi = 0., epssumi = 0., tsi = 0., epstsi = 0., sqrtrhoepsi = 0.;
+  if (dust) {
+    epsi = pi->eps; epssumi = epsi;
+    tsi = pi->tstop;
+    if (o->p.ilimitdustflux) { double lim = hi / pi->cs; if (lim < tsi) tsi = lim; }
+    epstsi = epsi * tsi;
+    sqrtrhoepsi = sqrt(epsi / (1. - epsi));
+  }
+  double rho21i = rho1i * rho1i;
+  double mrhoi5 = 0.5 * pm * rho1i;
+  double auterm = mrhoi5 * alphau;
+  double avBterm = mrhoi5 * alphaB * rho1i;
+  const int useresistiveheat = o->p.iresistive_heating > 0;
+  memset(fsum, 0, NFSUM * sizeof(double));
+  *vsigmax = 0.;
+  for (int32_t n = 0; n < l->n; n++) {
+    int32_t j = l->list[n];
+    if (j == i) continue;
+    double xj, yj, zj, hj1;
+    if (n < ISIZECELLCACHE) {
+      xj = l->cache[4 * n]; yj = l->cache[4 * n + 1]; zj = l->cache[4 * n + 2]; hj1 = l->cache[4 * n + 3];
+    } else {
+      const double *r = &in->xyzh[4 * (size_t)(j - 1)];
+      xj = r[0]; yj = r[1]; zj = r[2]; hj1 = 1. / r[3];
+    }
+    double dx = pi->x - xj, dy = pi->y - yj, dz = pi->z - zj;
+    if (o->p.periodic) {
+      if (fabs(dx) > 0.5 * o->dxbound) dx = dx - copysign(o->dxbound, dx);
+      if (fabs(dy) > 0.5 * o->dybound) dy = dy - copysign(o->dybound, dy);
+      if (fabs(dz) > 0.5 * o->dzbound) dz = dz - copysign(o->dzbound, dz);
+    }
+    double rij2 = dx * dx + dy * dy + dz * dz;
+    double q2i = rij2 * hi21;
+    double hj21 = hj1 * hj1;
+    double q2j = rij2 * hj21;
+    if (!(q2i < o->radkern2 || q2j < o->radkern2)) continue;
+    double rij1, qi;
+    if (rij2 > TINY64) { rij1 = 1. / sqrt(rij2); qi = (rij2 * rij1) * hi1; }
+    else { rij1 = 0.; qi = 0.; }
+    double grkerni = 0., grkernj = 0.;
+    if (q2i < o->radkern2) grkerni = grkern(kind, q2i, qi) * hfacgrkern;
+    int usej;
+    const double gradhj = (double)in->gradh[(size_t)in->ldgradh * (j - 1)];
+    if (q2j < o->radkern2) {
+      double qj = (rij2 * rij1) * hj1;
+      grkernj = grkern(kind, q2j, qj) * hj21 * hj21 * o->cnormk * gradhj;
+      usej = 1;
+    } else {
+      grkernj = 0.; usej = 0;
+    }
+    if (mhd) usej = 1;
+    if (dust) usej = 1;
+    if (thermal && !o->p.gravity) usej = 1;
+    double runix = dx * rij1, runiy = dy * rij1, runiz = dz * rij1;
+    const double *vj = &in->vxyzu[(size_t)in->ldv * (j - 1)];
+    double dvx = pi->vx - vj[0], dvy = pi->vy - vj[1], dvz = pi->vz - vj[2];
+    double projv = dvx * runix + dvy * runiy + dvz * runiz;
+    double denij = 0.;
+    if (thermal) denij = pi->u - vj[3];
+    double vsigi = pi->vwave - beta * projv; if (vsigi < 0.) vsigi = 0.;
+    double vsigavi = pi->alpha * pi->vwave - beta * projv; if (vsigavi < 0.) vsigavi = 0.;
+    if (vsigi > *vsigmax) *vsigmax = vsigi;
+    double Bxj = 0., Byj = 0., Bzj = 0., psij = 0., dBx = 0., dBy = 0., dBz = 0., projBi = 0., projBj = 0.,
+           projdB = 0., dB2 = 0., divBdiffterm = 0.;
+    if (mhd) {
+      double hjm = in->xyzh[4 * (size_t)(j - 1) + 3];
+      double rhojm = rhoh(o, hjm, pm);
+      const double *Bj = &in->Bevol[4 * (size_t)(j - 1)];
+      Bxj = Bj[0] * rhojm; Byj = Bj[1] * rhojm; Bzj = Bj[2] * rhojm; psij = Bj[3];
+      dBx = Bxi - Bxj; dBy = Byi - Byj; dBz = Bzi - Bzj;
+      projBi = Bxi * runix + Byi * runiy + Bzi * runiz;
+      if (usej) projBj = Bxj * runix + Byj * runiy + Bzj * runiz;
+      projdB = dBx * runix + dBy * runiy + dBz * runiz;
+      dB2 = dBx * dBx + dBy * dBy + dBz * dBz;
+      divBdiffterm = -pm * projdB * grkerni;
+    }
+    double rhoj = 0., rho1j = 0., rho21j = 0., mrhoj5 = 0., autermj = 0., avBtermj = 0., prj = 0., pro2j = 0.,
+           vwavej = 0., vsigavj = 0., epsj = 0., sqrtrhoepsj = 0., rhogasj = 0., csj = 0.;
+    double sj[6] = {0, 0, 0, 0, 0, 0};
+    if (usej) {
+      double hj = 1. / hj1;
+      rhoj = rhoh(o, hj, pm);
+      rho1j = 1. / rhoj;
+      rho21j = rho1j * rho1j;
+      if (dust) {
+        epsj = in->dustfrac[j - 1];
+        rhogasj = rhoj * (1. - epsj);
+        sqrtrhoepsj = sqrt(epsj / (1. - epsj));
+      } else {
+        rhogasj = rhoj;
+      }
+      double alphaj = (double)in->alphaind[3 * (size_t)(j - 1)];
+      prj = in->eos_vars[(size_t)in->ldeos * (j - 1)];
+      csj = in->eos_vars[(size_t)in->ldeos * (j - 1) + 1];
+      get_stress(o, prj, csj, rhoj, rho1j, pm, Bxj, Byj, Bzj, &pro2j, &vwavej, sj);
+      mrhoj5 = 0.5 * pm * rho1j;
+      autermj = mrhoj5 * alphau;
+      avBtermj = mrhoj5 * alphaB * rho1j;
+      double vsigj = vwavej - beta * projv; if (vsigj < 0.) vsigj = 0.;
+      vsigavj = alphaj * vwavej - beta * projv; if (vsigavj < 0.) vsigavj = 0.;
+      if (vsigj > *vsigmax) *vsigmax = vsigj;
+    }
+    /* gas-gas pair */
+    double qrho2i = 0., qrho2j = 0.;
+    if (projv < 0.) {
+      qrho2i = -0.5 * rho1i * vsigavi * projv;
+      if (usej) qrho2j = -0.5 * rho1j * vsigavj * projv;
+    }
+    double dudtdissi = pm * qrho2i * projv * grkerni;
+    double gradpi = pm * (pi->pro2 + qrho2i) * grkerni;
+    double gradpj = 0.;
+    if (usej) gradpj = pm * (pro2j + qrho2j) * grkernj;
+    double dendissterm = 0.;
+    if (thermal) {
+      double vsigu;
+      if (o->p.gravity) vsigu = fabs(projv);
+      else {
+        double rhoav1 = 2. / (rhoi + rhoj);
+        vsigu = sqrt(fabs(pi->pr - prj) * rhoav1);
+      }
+      dendissterm = vsigu * denij * (auterm * grkerni + autermj * grkernj);
+    }
+    double dBevolx = 0., dBevoly = 0., dBevolz = 0., divBsymmterm = 0., dudtresist = 0.;
+    if (mhd) {
+      double ax = dvx - projv * runix, ay = dvy - projv * runiy, az = dvz - projv * runiz;
+      double vsigB = sqrt(ax * ax + ay * ay + az * az);
+      double dBdissterm = (avBterm * grkerni + avBtermj * grkernj) * vsigB;
+      if (useresistiveheat) dudtresist = -0.5 * dB2 * dBdissterm;
+      double pmjrho21grkerni = pm * rho21i * grkerni;
+      double pmjrho21grkernj = pm * rho21j * grkernj;
+      double termi = pmjrho21grkerni * projBi;
+      divBsymmterm = termi + pmjrho21grkernj * projBj;
+      double dBrhoterm = -termi;
+      double dpsiterm = o->p.overcleanfac * (pmjrho21grkerni * psii * pi->vwave + pmjrho21grkernj * psij * vwavej);
+      dBevolx = dBrhoterm * dvx + dBdissterm * dBx - dpsiterm * runix;
+      dBevoly = dBrhoterm * dvy + dBdissterm * dBy - dpsiterm * runiy;
+      dBevolz = dBrhoterm * dvz + dBdissterm * dBz - dpsiterm * runiz;
+    }
+    double projsxi = (pi->sxx * runix + pi->sxy * runiy + pi->sxz * runiz) * grkerni;
+    double projsyi = (pi->sxy * runix + pi->syy * runiy + pi->syz * runiz) * grkerni;
+    double projszi = (pi->sxz * runix + pi->syz * runiy + pi->szz * runiz) * grkerni;
+    double projsxj = 0., projsyj = 0., projszj = 0.;
+    if (usej) {
+      projsxj = (sj[0] * runix + sj[1] * runiy + sj[2] * runiz) * grkernj;
+      projsyj = (sj[1] * runix + sj[3] * runiy + sj[4] * runiz) * grkernj;
+      projszj = (sj[2] * runix + sj[4] * runiy + sj[5] * runiz) * grkernj;
+    }
+    double gradp = gradpi + gradpj;
+    double fgrav = 0.;
+    fsum[F_FX] = fsum[F_FX] - runix * (gradp + fgrav) - (projsxi + projsxj);
+    fsum[F_FY] = fsum[F_FY] - runiy * (gradp + fgrav) - (projsyi + projsyj);
+    fsum[F_FZ] = fsum[F_FZ] - runiz * (gradp + fgrav) - (projszi + projszj);
+    fsum[F_DRHODT] = fsum[F_DRHODT] + projv * grkerni;
+    if (thermal) {
+      fsum[F_DUDTDISS] = fsum[F_DUDTDISS] + dudtdissi + dudtresist;
+      fsum[F_DENDTDISS] = fsum[F_DENDTDISS] + dendissterm;
+    }
+    if (mhd) {
+      fsum[F_DIVBSYM] += divBsymmterm;
+      fsum[F_DBX] += dBevolx; fsum[F_DBY] += dBevoly; fsum[F_DBZ] += dBevolz;
+      fsum[F_DIVBDIFF] += divBdiffterm;
+    }
+    if (dust) {
+      double tsj = get_ts(o, rhogasj, rhoj * epsj);
+      if (o->p.ilimitdustflux) { double lim = (1. / hj1) / csj; if (lim < tsj) tsj = lim; }
+      double epstsj = epsj * tsj;
+      if (epsi > 0. || epsj > 0.) {
+        double grkernav = 0.5 * (grkerni + grkernj);
+        double dft = pm * sqrtrhoepsj * rho1j *
+                     ((tsi - epstsi) * (1. - epsi) / (1. - epssumi) + (tsj - epstsj) * (1. - epsj) / (1. - epsj)) *
+                     (pi->pr - prj) * grkernav * rij1;
+        fsum[F_DDUSTEVOL] = fsum[F_DDUSTEVOL] - dft;
+        if (thermal) fsum[F_DUDTDUST] = fsum[F_DUDTDUST] - sqrtrhoepsi * dft * denij;
+      }
+      if (epsi < 1.) {
+        double term = tsi / (1. - epssumi) * pm * (pi->pro2 * grkerni + pro2j * grkernj);
+        fsum[F_DVX] += term * runix; fsum[F_DVY] += term * runiy; fsum[F_DVZ] += term * runiz;
+      }
+    }
+  }
+}
+
+/* force (force.F90:193-905) + start_cell (:2172-2514) +
+ * finish_cell_and_store_results (:2649-3330), global timestep build. */
+int orc_force(orc_ctx *o, int32_t icall, int32_t np, const double *xyzh, const double *vxyzu, int32_t ldv,
+              double *fxyzu, int32_t ldf, float *divcurlv, int32_t lddcv, const float *divcurlB,
+              const double *Bevol, double *dBevol, float *divBsymm, const double *dustfrac, double *ddustevol,
+              double *deltav, double *tstop, const float *gradh, int32_t ldgradh, const float *alphaind,
+              const double *eos_vars, int32_t ldeos, const int8_t *iphase, double *dtcourant_out,
+              double *dtforce_out) {
+  (void)icall;
+  if (!o->tree_built || np != o->np) {
+    snprintf(o->err, sizeof(o->err), "force: tree not built for this np");
+    return PHSPH_ERR_STATE;
+  }
+  const double pm = o->p.massoftype_gas;
+  const int mhd = o->p.mhd, dust = o->p.use_dustfrac && o->p.ndustsmall > 0;
+  const int thermal = o->p.maxvxyzu >= 4;
+  forcein_t in = {xyzh, vxyzu, Bevol, dustfrac, eos_vars, gradh, alphaind, divcurlB, ldv, ldgradh, ldeos};
+  double dtcourant = BIGNUMBER, dtforce = BIGNUMBER;
+  int rc_all = PHSPH_OK;
+#pragma omp parallel reduction(min : dtcourant, dtforce)
+  {
+    nlist_t l = {0};
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t icell = 1; icell <= o->ncells; icell++) {
+      if (o->leaf_is_active[icell] <= 0) continue;
+      if (rc_all) continue;
+      fpart_t parts[MINPART + 1];
+      int32_t gidx[MINPART + 1];
+      int npcell = 0;
+      for (int32_t ip = o->ir_lo[icell]; ip <= o->ir_hi[icell]; ip++) {
+        int32_t i = o->inodeparts[ip];
+        if (i < 0) continue;
+        if (iphase && iphase[i - 1] <= 0) continue;
+        fpart_t *q = &parts[npcell];
+        gidx[npcell] = i;
+        npcell++;
+        const double *r = &xyzh[4 * (size_t)(i - 1)];
+        const double *v = &vxyzu[(size_t)ldv * (i - 1)];
+        double hi = r[3];
+        /* rhoanddhdrho (part.F90:805-816) */
+        double hi1 = 1. / fabs(hi);
+        double t = o->p.hfact * hi1;
+        double rhoi = pm * (t * t * t);
+        double rho1i = 1. / rhoi;
+        q->x = r[0]; q->y = r[1]; q->z = r[2]; q->h = hi;
+        q->vx = v[0]; q->vy = v[1]; q->vz = v[2]; q->u = thermal ? v[3] : 0.;
+        q->rho = rhoi;
+        q->eps = dust ? dustfrac[i - 1] : 0.;
+        q->rhogas = rhoi * (1. - q->eps);
+        q->pr = eos_vars[(size_t)ldeos * (i - 1)];
+        q->cs = eos_vars[(size_t)ldeos * (i - 1) + 1];
+        if (mhd) {
+          const double *B = &Bevol[4 * (size_t)(i - 1)];
+          q->Bx_e = B[0]; q->By_e = B[1]; q->Bz_e = B[2]; q->psi = B[3];
+          q->divB = (double)divcurlB[4 * (size_t)(i - 1)];
+        } else {
+          q->Bx_e = q->By_e = q->Bz_e = q->psi = q->divB = 0.;
+        }
+        double s[6];
+        get_stress(o, q->pr, q->cs, rhoi, rho1i, pm, q->Bx_e * rhoi, q->By_e * rhoi, q->Bz_e * rhoi, &q->pro2,
+                   &q->vwave, s);
+        q->sxx = s[0]; q->sxy = s[1]; q->sxz = s[2]; q->syy = s[3]; q->syz = s[4]; q->szz = s[5];
+        q->tstop = dust ? get_ts(o, q->rhogas, rhoi * q->eps) : 0.;
+        q->alpha = (double)alphaind[3 * (size_t)(i - 1)];
+        double g = (double)gradh[(size_t)ldgradh * (i - 1)];
+        q->gradh = (g > 0.) ? g : 1.;
+      }
+      if (npcell == 0) continue;
+      const phsph_kdnode *nd = &o->node[icell];
+      int rc = getneigh(o, nd->xcen, nd->size, o->radkern * nd->hmax, 1, &l);
+      if (rc) { rc_all = rc; continue; }
+      for (int k = 0; k < npcell; k++) {
+        const fpart_t *q = &parts[k];
+        int32_t i = gidx[k];
+        double fsum[NFSUM], vsigmax;
+        compute_forces(o, i, q, &l, &in, fsum, &vsigmax);
+        /* finish_cell_and_store_results */
+        double hi = q->h, hi1 = 1. / hi, rhoi = q->rho, rho1i = 1. / rhoi;
+        double dtc = o->p.dtmax, dtf = BIGNUMBER, dtclean = BIGNUMBER, dtdusti = BIGNUMBER;
+        if (mhd) {
+          double Bx = q->Bx_e * rhoi, By = q->By_e * rhoi, Bz = q->Bz_e * rhoi;
+          double B2i = Bx * Bx + By * By + Bz * Bz;
+          double divBsymmi = fsum[F_DIVBSYM];
+          double frac_divB = 0.;
+          if (B2i > 0.) {
+            double betai = 2. * q->pr / B2i;
+            if (betai < 2.) frac_divB = 1.;
+            else if (betai < 10.) frac_divB = (10. - betai) * 0.125;
+          }
+          fsum[F_FX] = fsum[F_FX] - Bx * divBsymmi * frac_divB;
+          fsum[F_FY] = fsum[F_FY] - By * divBsymmi * frac_divB;
+          fsum[F_FZ] = fsum[F_FZ] - Bz * divBsymmi * frac_divB;
+          if (divBsymm) divBsymm[i - 1] = (float)(rhoi * divBsymmi);
+        }
+        double f2i = fsum[F_FX] * fsum[F_FX] + fsum[F_FY] * fsum[F_FY] + fsum[F_FZ] * fsum[F_FZ];
+        double *fo = &fxyzu[(size_t)ldf * (i - 1)];
+        fo[0] = fsum[F_FX]; fo[1] = fsum[F_FY]; fo[2] = fsum[F_FZ];
+        double drhodti = pm * fsum[F_DRHODT];
+        double divvi = -drhodti * rho1i;
+        divcurlv[(size_t)lddcv * (i - 1)] = (float)divvi;
+        if (thermal) {
+          double fac = (q->rhogas > TINY64) ? rhoi / q->rhogas : 0.;
+          double pdv_work = fac * q->pr * rho1i * rho1i * drhodti;
+          double fxyz4 = 0.;
+          if (o->p.ipdv_heating > 0) fxyz4 = fxyz4 + pdv_work;
+          if (o->p.ishock_heating > 0) fxyz4 = fxyz4 + fac * fsum[F_DUDTDISS];
+          fxyz4 = fxyz4 + fac * fsum[F_DENDTDISS];
+          if (dust) fxyz4 = fxyz4 + 0.5 * fac * rho1i * fsum[F_DUDTDUST];
+          fo[3] = fxyz4;
+        }
+        if (mhd) {
+          double *dB = &dBevol[4 * (size_t)(i - 1)];
+          dB[0] = fsum[F_DBX]; dB[1] = fsum[F_DBY]; dB[2] = fsum[F_DBZ];
+          if (o->p.psidecayfac > 0.) {
+            double vcleani = o->p.overcleanfac * q->vwave;
+            double dtau = o->p.psidecayfac * vcleani * hi1;
+            dB[3] = -vcleani * fsum[F_DIVBDIFF] * rho1i - q->psi * dtau - 0.5 * q->psi * divvi;
+            dtclean = o->p.C_cour * hi / (vcleani + TINY64);
+          }
+        }
+        if (dust) {
+          double om = 1. - q->eps;
+          ddustevol[i - 1] = 0.5 * (fsum[F_DDUSTEVOL] * rho1i / (om * om));
+          double *dv = &deltav[3 * (size_t)(i - 1)];
+          dv[0] = fsum[F_DVX]; dv[1] = fsum[F_DVY]; dv[2] = fsum[F_DVZ];
+        }
+        double vsigdtc = (vsigmax > q->vwave) ? vsigmax : q->vwave;
+        if (vsigdtc > TINY64) {
+          double am = (o->p.alpha > 1.) ? o->p.alpha : 1.;
+          dtc = o->p.C_cour * hi / (vsigdtc * am);
+        }
+        if (thermal) {
+          double eni = q->u;
+          if (eni + dtc * fo[3] < EPS64 && eni > EPS64) fo[3] = fo[3] / (1. - dtc * fo[3] / eni);
+        }
+        if (fabs(f2i) > EPS64) dtf = o->p.C_force * sqrt(hi / sqrt(f2i));
+        if (dust) {
+          if (q->eps > 0. && q->cs > 0. && q->eps > EPS64) {
+            double tseff = (1. - q->eps) / q->eps * (q->eps * q->tstop);
+            double dtdustdenom = q->eps * tseff * q->cs * q->cs;
+            if (dtdustdenom > TINY64) dtdusti = o->p.C_force * hi * hi / dtdustdenom;
+          }
+          if (tstop) tstop[i - 1] = q->tstop;
+        }
+        if (dtc < dtcourant) dtcourant = dtc;
+        double m = dtf;
+        if (dtdusti < m) m = dtdusti;
+        if (dtclean < m) m = dtclean;
+        if (m < dtforce) dtforce = m;
+      }
+    }
+    free(l.list); free(l.cache);
+  }
+  if (dtcourant_out) *dtcourant_out = dtcourant;
+  if (dtforce_out) *dtforce_out = dtforce;
+  return rc_all;
+}
+
+/* Kernel value helpers exported for the kernel identity tests. */
+void orc_kernel(int32_t kind, double q, double *w, double *g, double *dphidh) {
+  get_kernel(kind, q * q, q, w, g);
+  *dphidh = dphidh_kern(kind, q * q, q);
+}

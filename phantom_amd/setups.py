@@ -1,0 +1,219 @@
+"""Synthetic particle sets for the hot-path workloads.
+
+Every generator returns a dict of numpy arrays in the reference's storage
+layout: an array a(ncomp, n) of Fortran is a C-contiguous numpy array of
+shape (n, ncomp).  Keys: ``xyzh`` (n,4) f8, ``vxyzu`` (n,maxvxyzu) f8,
+``alphaind`` (n,3) f4, optional ``Bevol`` (n,4) f8, ``dustevol`` (n,) f8,
+plus ``params`` (a phantom_amd.abi.Params) and ``info`` (a dict).
+
+Lattices follow set_unifdis('cubic') (src/setup/set_unifdis.f90:182-222):
+particle (i,j,k) sits at xmin + (i-1/2) dx.  Smoothing lengths start at
+hfact * dx, as set_unifdis sets them.
+"""
+import math
+
+import numpy as np
+
+from . import abi
+
+PI_PH = 3.1415926536  # physcon.f90:23
+
+
+def _cubic_lattice(nx, ny, nz, x0, x1, y0, y1, z0, z1):
+    dx = (x1 - x0) / nx
+    dy = (y1 - y0) / ny
+    dz = (z1 - z0) / nz
+    xs = x0 + (np.arange(nx) + 0.5) * dx
+    ys = y0 + (np.arange(ny) + 0.5) * dy
+    zs = z0 + (np.arange(nz) + 0.5) * dz
+    z, y, x = np.meshgrid(zs, ys, xs, indexing="ij")  # x fastest, as set_unifdis
+    return np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1), dx
+
+
+def _wrap(p, lo, hi):
+    L = hi - lo
+    return np.where(p < lo, p + L, np.where(p > hi, p - L, p))
+
+
+def uniform_box(nx=64, ny=None, nz=None, jitter=0.05, seed=1234, kernel=abi.KERNEL_CUBIC,
+                isothermal=True, box=(-0.5, 0.5, -0.5, 0.5, -0.5, 0.5), rho0=1.0, vamp=0.1,
+                u0=1.5, alpha_seed=None):
+    """Jittered cubic lattice in a periodic box with v_x = vamp sin(2 pi y).
+
+    With the defaults this is the 64^3 isothermal box: rho = 1, c_s = 1,
+    every coordinate jittered by U(-0.05 dx, 0.05 dx).
+    """
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    x0, x1, y0, y1, z0, z1 = box
+    pos, dx = _cubic_lattice(nx, ny, nz, x0, x1, y0, y1, z0, z1)
+    rng = np.random.default_rng(seed)
+    n = pos.shape[0]
+    if jitter:
+        pos = pos + rng.uniform(-jitter * dx, jitter * dx, size=pos.shape)
+        pos[:, 0] = _wrap(pos[:, 0], x0, x1)
+        pos[:, 1] = _wrap(pos[:, 1], y0, y1)
+        pos[:, 2] = _wrap(pos[:, 2], z0, z1)
+    vol = (x1 - x0) * (y1 - y0) * (z1 - z0)
+    p = abi.default_params(kernel)
+    p.xmin, p.xmax, p.ymin, p.ymax, p.zmin, p.zmax = box
+    p.massoftype_gas = rho0 * vol / n
+    p.maxp = n
+    if isothermal:
+        p.maxvxyzu = 3
+        p.ieos = 1
+        p.polyk = 1.0
+    xyzh = np.empty((n, 4))
+    xyzh[:, :3] = pos
+    xyzh[:, 3] = p.hfact * dx
+    vxyzu = np.zeros((n, p.maxvxyzu))
+    vxyzu[:, 0] = vamp * np.sin(2.0 * PI_PH * (pos[:, 1] - y0) / (y1 - y0))
+    if not isothermal:
+        vxyzu[:, 3] = u0
+    alphaind = np.zeros((n, 3), dtype=np.float32)
+    arng = np.random.default_rng(seed + 1 if alpha_seed is None else alpha_seed)
+    alphaind[:, 0] = arng.uniform(0.0, 1.0, size=n).astype(np.float32)
+    return dict(xyzh=xyzh, vxyzu=vxyzu, alphaind=alphaind, params=p,
+                info=dict(workload="uniform_box", n=n, nx=nx, ny=ny, nz=nz, dx=dx, jitter=jitter,
+                          seed=seed, kernel=kernel, isothermal=isothermal))
+
+
+def shock_tube(ny=58, seed=4321, kernel=abi.KERNEL_QUINTIC):
+    """3D shock tube in [-0.5,0.5]x[-0.05,0.05]^2, periodic, equal masses.
+
+    Left half on a lattice of spacing dx (rho_L = 1, P_L = 1), right half on
+    spacing 2 dx (rho_R = 1/8, P_R = 0.1), gamma = 5/3, v = 0.  With ny = 58
+    lattice rows across the tube the particle count is 290*58*58 +
+    145*29*29 = 1,097,505, the lattice count nearest 2^20 that a 2:1 spacing
+    ratio allows.
+    """
+    gamma = 5.0 / 3.0
+    dx = 0.1 / ny
+    nxl = int(round(0.5 / dx))
+    left, _ = _cubic_lattice(nxl, ny, ny, -0.5, 0.0, -0.05, 0.05, -0.05, 0.05)
+    right, _ = _cubic_lattice(nxl // 2, ny // 2, ny // 2, 0.0, 0.5, -0.05, 0.05, -0.05, 0.05)
+    pos = np.concatenate([left, right])
+    n = pos.shape[0]
+    mtot = 1.0 * 0.5 * 0.01 + 0.125 * 0.5 * 0.01
+    p = abi.default_params(kernel)
+    p.xmin, p.xmax, p.ymin, p.ymax, p.zmin, p.zmax = -0.5, 0.5, -0.05, 0.05, -0.05, 0.05
+    p.massoftype_gas = mtot / n
+    p.maxp = n
+    p.maxvxyzu = 4
+    p.ieos = 2
+    p.gamma = gamma
+    xyzh = np.empty((n, 4))
+    xyzh[:, :3] = pos
+    nl = left.shape[0]
+    xyzh[:nl, 3] = p.hfact * dx
+    xyzh[nl:, 3] = p.hfact * 2.0 * dx
+    vxyzu = np.zeros((n, 4))
+    # u = P / ((gamma-1) rho)
+    vxyzu[:nl, 3] = 1.0 / ((gamma - 1.0) * 1.0)
+    vxyzu[nl:, 3] = 0.1 / ((gamma - 1.0) * 0.125)
+    rng = np.random.default_rng(seed)
+    alphaind = np.zeros((n, 3), dtype=np.float32)
+    alphaind[:, 0] = rng.uniform(0.0, 1.0, size=n).astype(np.float32)
+    return dict(xyzh=xyzh, vxyzu=vxyzu, alphaind=alphaind, params=p,
+                info=dict(workload="shock_tube", n=n, ny=ny, dx=dx, seed=seed, kernel=kernel))
+
+
+def mhd_dust_box(nx=128, seed=777, kernel=abi.KERNEL_QUINTIC, jitter=0.0, K_code=None):
+    """Orszag-Tang-type MHD box with one-fluid dust in a periodic unit cube.
+
+    rho = 25/(36 pi), P = 5/(12 pi), gamma = 5/3,
+    v = (-sin 2 pi y, sin 2 pi x, 0.01), B = (-sin 2 pi y, sin 4 pi x, 0)/sqrt(4 pi)
+    stored as Bevol = B/rho with psi = 0.  Dust fraction eps is log-uniform in
+    [0.01, 0.5] (seeded) and is passed as dustevol = sqrt(eps/(1-eps))
+    (cons2prim.f90:350 inverts it).  Stopping times use the constant drag
+    coefficient law t_s = rho_g rho_d / (K rho) (dust.f90:249-256).
+    """
+    pos, dx = _cubic_lattice(nx, nx, nx, -0.5, 0.5, -0.5, 0.5, -0.5, 0.5)
+    rng = np.random.default_rng(seed)
+    n = pos.shape[0]
+    if jitter:
+        pos = pos + rng.uniform(-jitter * dx, jitter * dx, size=pos.shape)
+        for c in range(3):
+            pos[:, c] = _wrap(pos[:, c], -0.5, 0.5)
+    gamma = 5.0 / 3.0
+    rho0 = 25.0 / (36.0 * PI_PH)
+    P0 = 5.0 / (12.0 * PI_PH)
+    p = abi.default_params(kernel)
+    p.massoftype_gas = rho0 / n
+    p.maxp = n
+    p.mhd = 1
+    p.use_dustfrac = 1
+    p.ndustsmall = 1
+    p.idrag = 2
+    # K chosen so that t_s = eps (1-eps) rho / K spans ~0.01 .. 0.5
+    p.K_code = rho0 * 0.0099 / 0.01 if K_code is None else K_code
+    p.gamma = gamma
+    x, y = pos[:, 0] + 0.5, pos[:, 1] + 0.5
+    xyzh = np.empty((n, 4))
+    xyzh[:, :3] = pos
+    xyzh[:, 3] = p.hfact * dx
+    vxyzu = np.zeros((n, 4))
+    vxyzu[:, 0] = -np.sin(2 * PI_PH * y)
+    vxyzu[:, 1] = np.sin(2 * PI_PH * x)
+    vxyzu[:, 2] = 0.01
+    eps = np.exp(rng.uniform(math.log(0.01), math.log(0.5), size=n))
+    rhogas = rho0 * (1.0 - eps)
+    vxyzu[:, 3] = P0 / ((gamma - 1.0) * rhogas)
+    B = np.zeros((n, 3))
+    B[:, 0] = -np.sin(2 * PI_PH * y) / math.sqrt(4 * PI_PH)
+    B[:, 1] = np.sin(4 * PI_PH * x) / math.sqrt(4 * PI_PH)
+    Bevol = np.zeros((n, 4))
+    Bevol[:, :3] = B / rho0
+    dustevol = np.sqrt(eps / (1.0 - eps))
+    alphaind = np.zeros((n, 3), dtype=np.float32)
+    alphaind[:, 0] = rng.uniform(0.0, 1.0, size=n).astype(np.float32)
+    return dict(xyzh=xyzh, vxyzu=vxyzu, alphaind=alphaind, Bevol=Bevol, dustevol=dustevol,
+                params=p, info=dict(workload="mhd_dust_box", n=n, nx=nx, dx=dx, seed=seed,
+                                    kernel=kernel, jitter=jitter))
+
+
+def derivs_test_box(nx=100, kernel=abi.KERNEL_CUBIC, rhozero=5.0):
+    """The lattice of Phantom's derivative unit test (test_derivs.f90:128-160):
+    psep = dxbound/100 in the default box, rho = 5, with the analytic
+    velocity and thermal energy fields vx, vy, vz, utherm
+    (test_derivs.f90:1216-1242, 1604-1613)."""
+    pos, dx = _cubic_lattice(nx, nx, nx, -0.5, 0.5, -0.5, 0.5, -0.5, 0.5)
+    n = pos.shape[0]
+    p = abi.default_params(kernel)
+    p.massoftype_gas = rhozero * 1.0 / n
+    p.maxp = n
+    p.tolh = 1.e-5
+    x, y, z = pos[:, 0] + 0.5, pos[:, 1] + 0.5, pos[:, 2] + 0.5
+    xyzh = np.empty((n, 4))
+    xyzh[:, :3] = pos
+    xyzh[:, 3] = p.hfact * dx
+    vxyzu = np.zeros((n, 4))
+    vxyzu[:, 0] = 0.5 / PI_PH * np.sin(2 * PI_PH * x)
+    vxyzu[:, 1] = 0.5 / PI_PH * np.sin(2 * PI_PH * x) - 0.5 / PI_PH * np.sin(2 * PI_PH * z)
+    vxyzu[:, 2] = 0.05 / PI_PH * np.cos(4 * PI_PH * y)
+    vxyzu[:, 3] = 0.5 / PI_PH * (3.0 + np.sin(2 * PI_PH * x) + np.cos(2 * PI_PH * y) + np.sin(2 * PI_PH * z))
+    alphaind = np.zeros((n, 3), dtype=np.float32)
+    hzero = p.hfact * (p.massoftype_gas / rhozero) ** (1.0 / 3.0)
+    return dict(xyzh=xyzh, vxyzu=vxyzu, alphaind=alphaind, params=p,
+                info=dict(workload="derivs_test_box", n=n, nx=nx, dx=dx, hzero=hzero, rhozero=rhozero))
+
+
+def random_box(n=20000, seed=99, kernel=abi.KERNEL_CUBIC, hmin_frac=0.5, hmax_frac=2.0):
+    """Uniform random positions with random h (for neighbour-count checks in
+    the spirit of test_neigh.f90:122-396)."""
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-0.5, 0.5, size=(n, 3))
+    p = abi.default_params(kernel)
+    p.massoftype_gas = 1.0 / n
+    p.maxp = n
+    dx = n ** (-1.0 / 3.0)
+    xyzh = np.empty((n, 4))
+    xyzh[:, :3] = pos
+    xyzh[:, 3] = p.hfact * dx * rng.uniform(hmin_frac, hmax_frac, size=n)
+    vxyzu = np.zeros((n, 4))
+    vxyzu[:, :3] = rng.normal(0.0, 0.1, size=(n, 3))
+    vxyzu[:, 3] = 1.0 + 0.1 * rng.uniform(size=n)
+    alphaind = np.zeros((n, 3), dtype=np.float32)
+    alphaind[:, 0] = rng.uniform(0.0, 1.0, size=n).astype(np.float32)
+    return dict(xyzh=xyzh, vxyzu=vxyzu, alphaind=alphaind, params=p,
+                info=dict(workload="random_box", n=n, seed=seed, kernel=kernel))
