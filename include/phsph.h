@@ -217,6 +217,16 @@ int phsph_get_neighbour_stats(phsph_ctx *ctx, double *trialmean,
                               int64_t *nactualtot);
 
 /*
+ * Per-particle statistics of the last densityiterate, in the caller's
+ * particle order: the actual neighbour count including the particle itself
+ * (cell%nneigh, dens.F90:1267) and the number of the h iteration at which
+ * the particle's own h converged (dens.F90:1425-1440).  Inactive and dead
+ * particles get 0.  Either pointer may be NULL.
+ */
+int phsph_get_particle_stats(phsph_ctx *ctx, int32_t np, int32_t *nneigh,
+                             int32_t *nits);
+
+/*
  * Kernel timing of the last call on the context's stream, measured with
  * HIP events around each launch: up to `cap` (name, milliseconds) entries
  * are written; returns the number of entries available.

@@ -43,7 +43,7 @@ def lib():
         L.orc_maketree.argtypes = [vp, vp, C.c_int32, vp]
         L.orc_get_tree.argtypes = [vp, vp, vp, vp, vp, C.c_int64, P(C.c_int64), P(C.c_int32)]
         L.orc_densityiterate.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, C.c_int32, vp, C.c_int32, vp, vp,
-                                         vp, vp, C.c_int32, vp, vp, vp, C.c_int32, vp, vp]
+                                         vp, vp, C.c_int32, vp, vp, vp, C.c_int32, vp, vp, vp]
         L.orc_get_neighbour_stats.argtypes = [vp, P(C.c_double), P(C.c_double), P(C.c_int32), P(C.c_int32),
                                               P(C.c_int64), P(C.c_int64)]
         L.orc_cons2prim.argtypes = [vp, C.c_int32, vp, vp, C.c_int32, vp, vp, vp, vp, vp, C.c_int32, vp]
@@ -121,13 +121,15 @@ class Oracle:
         gradh = np.zeros((n, ngradh), dtype=np.float32)
         dvdx = np.zeros((n, 9), dtype=np.float32)
         nits = np.zeros(n, dtype=np.int32)
+        nneigh = np.zeros(n, dtype=np.int32)
         ldv = vxyzu.shape[1]
         ldf = fxyzu.shape[1] if fxyzu is not None else 0
         self._check(lib().orc_densityiterate(self._ctx, icall, n, _ptr(xyzh), _ptr(vxyzu), ldv, _ptr(fxyzu), ldf,
                                              _ptr(fext), _ptr(Bevol), _ptr(iphase), _ptr(divcurlv), 1,
                                              _ptr(divcurlB), _ptr(alphaind), _ptr(gradh), ngradh, _ptr(dvdx),
-                                             _ptr(nits)))
-        return dict(divcurlv=divcurlv, divcurlB=divcurlB, alphaind=alphaind, gradh=gradh, dvdx=dvdx, nits=nits)
+                                             _ptr(nits), _ptr(nneigh)))
+        return dict(divcurlv=divcurlv, divcurlB=divcurlB, alphaind=alphaind, gradh=gradh, dvdx=dvdx, nits=nits,
+                    nneigh=nneigh)
 
     def neighbour_stats(self):
         tm, am = C.c_double(), C.c_double()
@@ -177,8 +179,10 @@ def run_all(setup, icall=1):
     vxyzu = setup["vxyzu"]
     alphaind = setup["alphaind"].copy()
     o.maketree(xyzh)
+    tree = o.get_tree()
     d = o.densityiterate(xyzh, vxyzu, Bevol=setup.get("Bevol"), icall=icall, alphaind=alphaind)
     c = o.cons2prim(xyzh, vxyzu, d["dvdx"], alphaind, dustevol=setup.get("dustevol"))
     f = o.force(xyzh, vxyzu, d["gradh"], alphaind, c["eos_vars"], divcurlB=d["divcurlB"],
                 Bevol=setup.get("Bevol"), dustfrac=c["dustfrac"])
-    return dict(oracle=o, xyzh=xyzh, dens=d, c2p=c, force=f, stats=o.neighbour_stats(), alphaind=alphaind)
+    return dict(oracle=o, xyzh=xyzh, tree=tree, dens=d, c2p=c, force=f, stats=o.neighbour_stats(),
+                alphaind=alphaind)

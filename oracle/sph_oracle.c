@@ -784,7 +784,7 @@ int orc_densityiterate(orc_ctx *o, int32_t icall, int32_t np, double *xyzh, cons
                        int32_t ldv, const double *fxyzu, int32_t ldf, const double *fext,
                        const double *Bevol, const int8_t *iphase, float *divcurlv, int32_t lddcv,
                        float *divcurlB, float *alphaind, float *gradh, int32_t ldgradh, float *dvdx,
-                       int32_t *nits_part) {
+                       int32_t *nits_part, int32_t *nneigh_part) {
   if (!o->tree_built || np != o->np) {
     snprintf(o->err, sizeof(o->err), "densityiterate: tree not built for this np");
     return PHSPH_ERR_STATE;
@@ -798,6 +798,7 @@ int orc_densityiterate(orc_ctx *o, int32_t icall, int32_t np, double *xyzh, cons
   int rc_all = PHSPH_OK;
   const double pm = o->p.massoftype_gas;
   if (nits_part) for (int32_t i = 0; i < np; i++) nits_part[i] = 0;
+  if (nneigh_part) for (int32_t i = 0; i < np; i++) nneigh_part[i] = 0;
 
 #pragma omp parallel reduction(+ : nneightry, nneighact, ncalc, npc) reduction(max : maxneightry, maxneighact)
   {
@@ -912,6 +913,8 @@ int orc_densityiterate(orc_ctx *o, int32_t icall, int32_t np, double *xyzh, cons
       if (nits_part)
         for (int k = 0; k < c.npcell; k++)
           nits_part[o->inodeparts[c.arr_index[k]] - 1] = conv_at[k] ? conv_at[k] : c.nits;
+      if (nneigh_part)
+        for (int k = 0; k < c.npcell; k++) nneigh_part[o->inodeparts[c.arr_index[k]] - 1] = c.nneigh[k];
       store_results(o, &c, &out, &nneightry, &nneighact, &maxneightry, &maxneighact, &npc, &ncalc);
     }
     free(l.list); free(l.cache);

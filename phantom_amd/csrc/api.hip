@@ -1,0 +1,438 @@
+// api.hip -- the extern "C" entry points declared in include/phsph.h.
+//
+// Each entry point accepts host or device pointers.  Host arrays are staged
+// through context-owned device buffers: inputs and in/out arrays are copied
+// in, outputs are copied back after the kernels complete.  Device arrays
+// are used in place, so a caller that keeps its particle arrays in HBM pays
+// no transfer.
+#include <algorithm>
+#include <cmath>
+
+#include "ctx.h"
+
+DevConst make_devconst(const phsph_params &p) {
+  DevConst d;
+  memset(&d, 0, sizeof(d));
+  d.kernel = p.kernel;
+  d.periodic = p.periodic;
+  d.maxvxyzu = p.maxvxyzu;
+  d.mhd = p.mhd;
+  d.dust = (p.use_dustfrac && p.ndustsmall > 0) ? 1 : 0;
+  d.gravity = p.gravity;
+  d.ind_timesteps = p.ind_timesteps;
+  d.ieos = p.ieos;
+  d.ishock_heating = p.ishock_heating;
+  d.ipdv_heating = p.ipdv_heating;
+  d.iresistive_heating = p.iresistive_heating;
+  d.ilimitdustflux = p.ilimitdustflux;
+  d.idrag = p.idrag;
+  if (p.kernel == PHSPH_KERNEL_CUBIC) {  // kernel_cubic.f90:23-30
+    d.radkern = 2.; d.radkern2 = 4.; d.cnormk = 1. / PH_PI;
+    d.wab0 = 1.; d.gradh0 = -3.; d.dphidh0 = 1.4;
+  } else {  // kernel_quintic.f90:23-30
+    d.radkern = 3.; d.radkern2 = 9.; d.cnormk = 1. / (120. * PH_PI);
+    d.wab0 = 66.; d.gradh0 = -3. * 66.; d.dphidh0 = 239. / 210.;
+  }
+  d.xmin = p.xmin; d.xmax = p.xmax; d.ymin = p.ymin; d.ymax = p.ymax; d.zmin = p.zmin; d.zmax = p.zmax;
+  d.dxbound = p.xmax - p.xmin; d.dybound = p.ymax - p.ymin; d.dzbound = p.zmax - p.zmin;
+  d.hdlx = 0.5 * d.dxbound; d.hdly = 0.5 * d.dybound; d.hdlz = 0.5 * d.dzbound;
+  d.hfact = p.hfact; d.tolh = p.tolh; d.pmass = p.massoftype_gas; d.gamma = p.gamma; d.polyk = p.polyk;
+  d.alpha = p.alpha; d.alphamax = p.alphamax; d.alphau = p.alphau; d.alphaB = p.alphaB; d.beta = p.beta;
+  d.C_cour = p.C_cour; d.C_force = p.C_force; d.dtmax = p.dtmax; d.psidecayfac = p.psidecayfac;
+  d.overcleanfac = p.overcleanfac; d.K_code = p.K_code;
+  return d;
+}
+
+void ctx_clear_timings(phsph_ctx *c) {
+  for (auto &t : c->timings) {
+    hipEventDestroy(t.t0);
+    hipEventDestroy(t.t1);
+  }
+  c->timings.clear();
+}
+
+static const char *validate(const phsph_params *p) {
+  if (p->kernel != PHSPH_KERNEL_CUBIC && p->kernel != PHSPH_KERNEL_QUINTIC) return "kernel must be cubic or quintic";
+  if (p->ieos != 1 && p->ieos != 2) return "only ieos 1 (isothermal) and 2 (ideal gas) are supported";
+  if (p->maxvxyzu != 3 && p->maxvxyzu != 4) return "maxvxyzu must be 3 or 4";
+  if (!(p->massoftype_gas > 0.)) return "massoftype_gas must be > 0";
+  if (!(p->hfact > 0.)) return "hfact must be > 0";
+  if (!(p->tolh > 0.)) return "tolh must be > 0";
+  if (!(p->xmax > p->xmin && p->ymax > p->ymin && p->zmax > p->zmin)) return "empty box";
+  if (p->use_dustfrac && (p->ndustsmall < 0 || p->ndustsmall > 1)) return "ndustsmall must be 0 or 1";
+  if (p->use_dustfrac && p->ndustsmall == 1 && p->idrag != 2 && p->idrag != 3)
+    return "one-fluid dust supports idrag 2 (constant K) and 3 (constant stopping time)";
+  if (p->gravity) return "self-gravity is not available in this build";
+  if (p->precision != PHSPH_PREC_FP64 && p->precision != PHSPH_PREC_MIXED) return "unknown precision";
+  return nullptr;
+}
+
+static bool is_device_ptr(const void *ptr) {
+  if (!ptr) return false;
+  hipPointerAttribute_t a;
+  hipError_t e = hipPointerGetAttributes(&a, ptr);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+// Staging of one caller array.  copy_in: the device copy starts with the
+// caller's contents; copy_out: copied back by Stager::finish.
+struct Stager {
+  phsph_ctx *c;
+  int slot = 0;
+  struct Back {
+    void *host;
+    void *dev;
+    size_t bytes;
+  };
+  std::vector<Back> backs;
+  explicit Stager(phsph_ctx *ctx) : c(ctx) {}
+  template <class T>
+  int stage(T *ptr, size_t count, bool copy_in, bool copy_out, T **dev) {
+    *dev = nullptr;
+    if (!ptr) return PHSPH_OK;
+    if (is_device_ptr(ptr)) {
+      *dev = ptr;
+      return PHSPH_OK;
+    }
+    if (slot >= 16) PH_RET(c, PHSPH_ERR_STATE, "internal: too many staged arrays");
+    DBuf<char> &b = c->stage[slot++];
+    const size_t bytes = count * sizeof(T);
+    PH_HIPCHECK(c, b.ensure(bytes));
+    if (copy_in) PH_HIPCHECK(c, hipMemcpyAsync(b.p, (const void *)ptr, bytes, hipMemcpyHostToDevice, c->stream));
+    if (copy_out) backs.push_back({(void *)ptr, b.p, bytes});
+    *dev = reinterpret_cast<T *>(b.p);
+    return PHSPH_OK;
+  }
+  int finish() {
+    for (auto &bk : backs)
+      PH_HIPCHECK(c, hipMemcpyAsync(bk.host, bk.dev, bk.bytes, hipMemcpyDeviceToHost, c->stream));
+    PH_HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return PHSPH_OK;
+  }
+};
+
+#define STAGE(st, ptr, count, in, out, dev)                           \
+  do {                                                                \
+    int _rc = (st).stage((ptr), (size_t)(count), (in), (out), (dev)); \
+    if (_rc) return _rc;                                              \
+  } while (0)
+
+__global__ void k_zero_i32_api(int32_t *a, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+extern "C" {
+
+void phsph_default_params(phsph_params *p, int32_t kernel) {
+  memset(p, 0, sizeof(*p));
+  p->kernel = kernel;
+  p->periodic = 1;
+  p->maxvxyzu = 4;
+  p->ieos = 2;
+  p->idrag = 2;
+  p->ishock_heating = 1;
+  p->ipdv_heating = 1;
+  p->iresistive_heating = 1;
+  p->xmin = -0.5; p->xmax = 0.5; p->ymin = -0.5; p->ymax = 0.5; p->zmin = -0.5; p->zmax = 0.5;
+  p->hfact = (kernel == PHSPH_KERNEL_CUBIC) ? 1.2 : 1.0;
+  p->tolh = 1.e-4;
+  p->massoftype_gas = 1.;
+  p->gamma = 5. / 3.;
+  p->polyk = 1.;
+  p->alpha = 0.; p->alphamax = 1.; p->alphau = 1.; p->alphaB = 1.; p->beta = 2.;
+  p->C_cour = 0.3; p->C_force = 0.25; p->dtmax = 1.e29;
+  p->psidecayfac = 1.; p->overcleanfac = 1.;
+  p->K_code = 1.;
+  p->tree_accuracy = 0.5;
+}
+
+int phsph_create(const phsph_params *p, int32_t device, phsph_ctx **out) {
+  if (!out || !p) return PHSPH_ERR_INPUT;
+  *out = nullptr;
+  phsph_ctx *c = new phsph_ctx();
+  c->device = device;
+  const char *msg = validate(p);
+  if (msg) {
+    c->err = msg;
+    *out = c;  // so that phsph_last_error can report; caller destroys
+    return PHSPH_ERR_INPUT;
+  }
+  c->p = *p;
+  c->dc = make_devconst(*p);
+  if (p->kernel == PHSPH_KERNEL_QUINTIC) c->caps.nlist = 384;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    c->err = std::string("HIP error ") + hipGetErrorString(e);
+    *out = c;
+    return PHSPH_ERR_HIP;
+  }
+  *out = c;
+  return PHSPH_OK;
+}
+
+int phsph_destroy(phsph_ctx *c) {
+  if (!c) return PHSPH_OK;
+  if (c->stream) {
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    ctx_clear_timings(c);
+    hipStreamDestroy(c->stream);
+  }
+  delete c;  // device buffers are released by their destructors
+  return PHSPH_OK;
+}
+
+int phsph_set_params(phsph_ctx *c, const phsph_params *p) {
+  if (!c || !p) return PHSPH_ERR_INPUT;
+  const char *msg = validate(p);
+  if (msg) PH_RET(c, PHSPH_ERR_INPUT, msg);
+  c->p = *p;
+  c->dc = make_devconst(*p);
+  return PHSPH_OK;
+}
+
+const char *phsph_last_error(const phsph_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int phsph_maketree(phsph_ctx *c, double *xyzh, int32_t ld, int32_t np, const int8_t *iphase,
+                   int32_t *leaf_is_active, int64_t *ncells) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (!xyzh || np <= 0 || ld < 4) PH_RET(c, PHSPH_ERR_INPUT, "maketree: bad xyzh/np/ld");
+  if ((int64_t)np >= (1 << 27)) PH_RET(c, PHSPH_ERR_CAPACITY, "maketree: at most 2^27-1 particles");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  ctx_clear_timings(c);
+  Stager st(c);
+  double *dx;
+  int8_t *dph;
+  STAGE(st, xyzh, (size_t)ld * np, true, true, &dx);
+  STAGE(st, const_cast<int8_t *>(iphase), (size_t)np, true, false, &dph);
+  int rc = tree_build(c, dx, ld, np, dph);
+  int rc2 = st.finish();
+  if (rc) return rc;
+  if (rc2) return rc2;
+  if (ncells) *ncells = c->ncells;
+  if (leaf_is_active) {
+    if (is_device_ptr(leaf_is_active)) {
+      PH_HIPCHECK(c, hipMemcpyAsync(leaf_is_active, c->leaf_flag.p + 1, c->ncells * sizeof(int32_t),
+                                    hipMemcpyDeviceToDevice, c->stream));
+    } else {
+      PH_HIPCHECK(c, hipMemcpyAsync(leaf_is_active, c->leaf_flag.p + 1, c->ncells * sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, c->stream));
+    }
+    PH_HIPCHECK(c, hipStreamSynchronize(c->stream));
+  }
+  return PHSPH_OK;
+}
+
+int phsph_get_tree(phsph_ctx *c, phsph_kdnode *node, int32_t *inoderange, int32_t *inodeparts, int64_t cap_nodes,
+                   int32_t *maxlevel) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (!c->tree_valid) PH_RET(c, PHSPH_ERR_STATE, "get_tree: no tree");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  const int64_t n = std::min<int64_t>(cap_nodes, c->ncells);
+  if (node && n > 0)
+    PH_HIPCHECK(c, hipMemcpy(node, c->nd.p + 1, n * sizeof(phsph_kdnode), hipMemcpyDeviceToHost));
+  if (inoderange && n > 0) {
+    std::vector<int32_t> lo(n), hi(n), fl(n);
+    PH_HIPCHECK(c, hipMemcpy(lo.data(), c->nd_lo.p + 1, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    PH_HIPCHECK(c, hipMemcpy(hi.data(), c->nd_hi.p + 1, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    PH_HIPCHECK(c, hipMemcpy(fl.data(), c->leaf_flag.p + 1, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<phsph_kdnode> nd(n);
+    PH_HIPCHECK(c, hipMemcpy(nd.data(), c->nd.p + 1, n * sizeof(phsph_kdnode), hipMemcpyDeviceToHost));
+    // slots below the deepest leaves are never constructed: report them
+    // empty, as the reference leaves them untouched (kdtree.F90:161)
+    std::vector<char> used(n + 1, 0);
+    used[0] = 1;  // root (index 0 = node 1)
+    for (int64_t k = 0; k < n; k++) {
+      if (!used[k]) continue;
+      if (fl[k] == 0) {
+        int64_t l = (int64_t)nd[k].leftchild - 1, r = (int64_t)nd[k].rightchild - 1;
+        if (l >= 0 && l < n) used[l] = 1;
+        if (r >= 0 && r < n) used[r] = 1;
+      }
+    }
+    for (int64_t k = 0; k < n; k++) {
+      inoderange[2 * k] = used[k] ? lo[k] + 1 : 0;
+      inoderange[2 * k + 1] = used[k] ? hi[k] + 1 : 0;
+    }
+  }
+  if (inodeparts)
+    PH_HIPCHECK(c, hipMemcpy(inodeparts, c->iparts.p, (size_t)c->nlive * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (maxlevel) *maxlevel = c->maxlevel;
+  return PHSPH_OK;
+}
+
+int phsph_densityiterate(phsph_ctx *c, int32_t icall, int32_t np, double *xyzh, int32_t ldx, const double *vxyzu,
+                         int32_t ldv, const double *fxyzu, int32_t ldf, const double *fext, const double *Bevol,
+                         const int8_t *iphase, float *divcurlv, int32_t lddcv, float *divcurlB, float *alphaind,
+                         float *gradh, int32_t ldgradh, float *dvdx) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (!c->tree_valid || np != c->np) PH_RET(c, PHSPH_ERR_STATE, "densityiterate: call maketree first");
+  if (icall != 0 && icall != 1) PH_RET(c, PHSPH_ERR_INPUT, "densityiterate: icall must be 0 or 1");
+  if (!xyzh || !vxyzu || !divcurlv || !gradh || ldx < 4 || ldv < 3 || lddcv < 1 || ldgradh < 1)
+    PH_RET(c, PHSPH_ERR_INPUT, "densityiterate: missing array");
+  if (c->p.mhd && !Bevol) PH_RET(c, PHSPH_ERR_INPUT, "densityiterate: MHD needs Bevol");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  ctx_clear_timings(c);
+  Stager st(c);
+  DensIOArgs a;
+  memset(&a, 0, sizeof(a));
+  STAGE(st, xyzh, (size_t)ldx * np, true, true, &a.xyzh);
+  a.ldx = ldx;
+  double *t;
+  STAGE(st, const_cast<double *>(vxyzu), (size_t)ldv * np, true, false, &t);
+  a.vxyzu = t; a.ldv = ldv;
+  STAGE(st, const_cast<double *>(fxyzu), (size_t)ldf * np, true, false, &t);
+  a.fxyzu = t; a.ldf = ldf;
+  STAGE(st, const_cast<double *>(fext), (size_t)3 * np, true, false, &t);
+  a.fext = t;
+  STAGE(st, const_cast<double *>(c->p.mhd ? Bevol : nullptr), (size_t)4 * np, true, false, &t);
+  a.Bevol = t;
+  int8_t *ph;
+  STAGE(st, const_cast<int8_t *>(iphase), (size_t)np, true, false, &ph);
+  a.iphase = ph;
+  STAGE(st, divcurlv, (size_t)lddcv * np, true, true, &a.divcurlv);
+  a.lddcv = lddcv;
+  STAGE(st, c->p.mhd ? divcurlB : nullptr, (size_t)4 * np, true, true, &a.divcurlB);
+  STAGE(st, alphaind, (size_t)3 * np, true, true, &a.alphaind);
+  STAGE(st, gradh, (size_t)ldgradh * np, true, true, &a.gradh);
+  a.ldgradh = ldgradh;
+  STAGE(st, dvdx, (size_t)9 * np, true, true, &a.dvdx);
+  int rc = dens_run(c, icall, a);
+  int rc2 = st.finish();
+  return rc ? rc : rc2;
+}
+
+int phsph_cons2prim(phsph_ctx *c, int32_t np, const double *xyzh, int32_t ldx, const double *vxyzu, int32_t ldv,
+                    const float *dvdx, float *alphaind, const double *dustevol, double *dustfrac, double *eos_vars,
+                    int32_t ldeos, const int8_t *iphase) {
+  (void)iphase;
+  if (!c) return PHSPH_ERR_INPUT;
+  if (!xyzh || !vxyzu || !eos_vars || ldeos < 2 || ldx < 4) PH_RET(c, PHSPH_ERR_INPUT, "cons2prim: missing array");
+  const bool dust = c->p.use_dustfrac && c->p.ndustsmall > 0;
+  if (dust && (!dustevol || !dustfrac)) PH_RET(c, PHSPH_ERR_INPUT, "cons2prim: dust needs dustevol and dustfrac");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  ctx_clear_timings(c);
+  Stager st(c);
+  double *dx, *dv, *de, *dd, *deos;
+  float *ddv, *da;
+  STAGE(st, const_cast<double *>(xyzh), (size_t)ldx * np, true, false, &dx);
+  STAGE(st, const_cast<double *>(vxyzu), (size_t)ldv * np, true, false, &dv);
+  STAGE(st, const_cast<float *>(dvdx), (size_t)9 * np, true, false, &ddv);
+  STAGE(st, alphaind, (size_t)3 * np, true, true, &da);
+  STAGE(st, const_cast<double *>(dust ? dustevol : nullptr), (size_t)np, true, false, &de);
+  STAGE(st, dust ? dustfrac : nullptr, (size_t)np, true, true, &dd);
+  STAGE(st, eos_vars, (size_t)ldeos * np, true, true, &deos);
+  int rc = cons2prim_run(c, np, dx, ldx, dv, ldv, ddv, da, de, dd, deos, ldeos);
+  int rc2 = st.finish();
+  return rc ? rc : rc2;
+}
+
+int phsph_force(phsph_ctx *c, int32_t icall, int32_t np, const double *xyzh, int32_t ldx, const double *vxyzu,
+                int32_t ldv, double *fxyzu, int32_t ldf, float *divcurlv, int32_t lddcv, const float *divcurlB,
+                const double *Bevol, double *dBevol, float *divBsymm, const double *dustfrac, double *ddustevol,
+                double *deltav, double *tstop, const float *gradh, int32_t ldgradh, const float *alphaind,
+                const double *eos_vars, int32_t ldeos, const int8_t *iphase, double *dtcourant, double *dtforce) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (!c->tree_valid || np != c->np) PH_RET(c, PHSPH_ERR_STATE, "force: call maketree first");
+  if (!xyzh || !vxyzu || !fxyzu || !divcurlv || !gradh || !alphaind || !eos_vars || ldx < 4 || ldv < 3 ||
+      ldf < 3 || lddcv < 1 || ldgradh < 1 || ldeos < 2)
+    PH_RET(c, PHSPH_ERR_INPUT, "force: missing array");
+  if (c->p.maxvxyzu >= 4 && (ldv < 4 || ldf < 4)) PH_RET(c, PHSPH_ERR_INPUT, "force: vxyzu/fxyzu need 4 components");
+  const bool mhd = c->p.mhd, dust = c->p.use_dustfrac && c->p.ndustsmall > 0;
+  if (mhd && (!Bevol || !dBevol || !divcurlB)) PH_RET(c, PHSPH_ERR_INPUT, "force: MHD needs Bevol, dBevol, divcurlB");
+  if (dust && (!dustfrac || !ddustevol || !deltav)) PH_RET(c, PHSPH_ERR_INPUT, "force: dust needs dust arrays");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  ctx_clear_timings(c);
+  Stager st(c);
+  ForceIOArgs a;
+  memset(&a, 0, sizeof(a));
+  double *t;
+  float *tf;
+  int8_t *ph;
+  STAGE(st, const_cast<double *>(xyzh), (size_t)ldx * np, true, false, &t);
+  a.xyzh = t; a.ldx = ldx;
+  STAGE(st, const_cast<double *>(vxyzu), (size_t)ldv * np, true, false, &t);
+  a.vxyzu = t; a.ldv = ldv;
+  STAGE(st, fxyzu, (size_t)ldf * np, true, true, &a.fxyzu);
+  a.ldf = ldf;
+  STAGE(st, divcurlv, (size_t)lddcv * np, true, true, &a.divcurlv);
+  a.lddcv = lddcv;
+  STAGE(st, const_cast<float *>(mhd ? divcurlB : nullptr), (size_t)4 * np, true, false, &tf);
+  a.divcurlB = tf;
+  STAGE(st, const_cast<double *>(mhd ? Bevol : nullptr), (size_t)4 * np, true, false, &t);
+  a.Bevol = t;
+  STAGE(st, mhd ? dBevol : nullptr, (size_t)4 * np, true, true, &a.dBevol);
+  STAGE(st, mhd ? divBsymm : nullptr, (size_t)np, true, true, &a.divBsymm);
+  STAGE(st, const_cast<double *>(dust ? dustfrac : nullptr), (size_t)np, true, false, &t);
+  a.dustfrac = t;
+  STAGE(st, dust ? ddustevol : nullptr, (size_t)np, true, true, &a.ddustevol);
+  STAGE(st, dust ? deltav : nullptr, (size_t)3 * np, true, true, &a.deltav);
+  STAGE(st, dust ? tstop : nullptr, (size_t)np, true, true, &a.tstop);
+  STAGE(st, const_cast<float *>(gradh), (size_t)ldgradh * np, true, false, &tf);
+  a.gradh = tf; a.ldgradh = ldgradh;
+  STAGE(st, const_cast<float *>(alphaind), (size_t)3 * np, true, false, &tf);
+  a.alphaind = tf;
+  STAGE(st, const_cast<double *>(eos_vars), (size_t)ldeos * np, true, false, &t);
+  a.eos_vars = t; a.ldeos = ldeos;
+  STAGE(st, const_cast<int8_t *>(iphase), (size_t)np, true, false, &ph);
+  a.iphase = ph;
+  int rc = force_run(c, icall, a, dtcourant, dtforce);
+  int rc2 = st.finish();
+  return rc ? rc : rc2;
+}
+
+int phsph_get_neighbour_stats(phsph_ctx *c, double *trialmean, double *actualmean, int32_t *maxtrial,
+                              int32_t *maxactual, int64_t *nrhocalc, int64_t *nactualtot) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (trialmean) *trialmean = c->trialmean;
+  if (actualmean) *actualmean = c->actualmean;
+  if (maxtrial) *maxtrial = c->maxtrial;
+  if (maxactual) *maxactual = c->maxactual;
+  if (nrhocalc) *nrhocalc = c->nrhocalc;
+  if (nactualtot) *nactualtot = c->nactualtot;
+  return PHSPH_OK;
+}
+
+int phsph_get_particle_stats(phsph_ctx *c, int32_t np, int32_t *nneigh, int32_t *nits) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (!c->tree_valid || np != c->np || c->nrhocalc < 0)
+    PH_RET(c, PHSPH_ERR_STATE, "get_particle_stats: no densityiterate on this tree");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  Stager st(c);
+  int32_t *dn, *di;
+  if (nneigh) {
+    STAGE(st, nneigh, (size_t)np, false, true, &dn);
+    hipLaunchKernelGGL(k_zero_i32_api, dim3(nblocks(np, 256)), dim3(256), 0, c->stream, dn, np);
+  } else dn = nullptr;
+  if (nits) {
+    STAGE(st, nits, (size_t)np, false, true, &di);
+    hipLaunchKernelGGL(k_zero_i32_api, dim3(nblocks(np, 256)), dim3(256), 0, c->stream, di, np);
+  } else di = nullptr;
+  int rc = dens_particle_stats(c, dn, di);
+  int rc2 = st.finish();
+  return rc ? rc : rc2;
+}
+
+int phsph_get_timings(phsph_ctx *c, const char **names, double *ms, int32_t cap) {
+  if (!c) return PHSPH_ERR_INPUT;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  const int n = (int)c->timings.size();
+  c->timing_names.resize(n);
+  for (int k = 0; k < n && k < cap; k++) {
+    float t = 0.f;
+    hipEventElapsedTime(&t, c->timings[k].t0, c->timings[k].t1);
+    c->timing_names[k] = c->timings[k].name;
+    if (names) names[k] = c->timing_names[k].c_str();
+    if (ms) ms[k] = (double)t;
+  }
+  return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,214 @@
+// ctx.h -- host-side context of the library: device buffers that persist
+// between calls (the tree and the tree-ordered particle mirror), staging of
+// caller arrays, error reporting and per-launch timing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+// Growable device buffer.  Contents are not preserved on growth.
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t cap = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  hipError_t ensure(size_t n) {
+    if (n <= cap) return hipSuccess;
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = n < 64 ? 64 : n;
+    hipError_t e = hipMalloc((void **)&p, want * sizeof(T));
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct TimedLaunch {
+  std::string name;
+  hipEvent_t t0, t1;
+};
+
+// Capacities of the per-leaf candidate lists and per-particle neighbour
+// lists.  They grow (and the pass is re-run) when a launch reports overflow.
+struct Caps {
+  int cand = 192;    // candidate leaves per leaf
+  int nlist = 256;   // neighbours per particle
+};
+
+// per-particle quantities of the force loop (tree order)
+struct ForceBufs {
+  DBuf<double> hinv, u, rho, rho1, pr, pro2, vwave, Bx, By, Bz, psi, s[6], eps, sqeps, ts, cs, h;
+  DBuf<double> Bex, Bey, Bez, divB;
+};
+
+struct phsph_ctx {
+  phsph_params p;
+  ForceBufs fb;
+  DBuf<double> hmaxf;                 // node hmax of the final h (force walk)
+  DBuf<unsigned long long> dtred;     // global dt minima
+  DevConst dc;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  Caps caps;
+
+  // ---- tree (valid after maketree) ----
+  int np = 0;            // particles passed to maketree
+  int nlive = 0;         // live particles (h > 0) = tree positions
+  int64_t ncells = 0;    // reference ncells
+  int64_t nslots = 0;    // allocated node slots (1-based ids < nslots)
+  int maxlevel = 0, minlevel = 0, maxlevel_indexed = 0;
+  int nleaves = 0;       // leaves (leaf_is_active != 0), in node-id order
+  int tree_fallbacks = 0; // top-level nodes re-summed sequentially
+  bool tree_valid = false;
+
+  DBuf<phsph_kdnode> nd;     // node(1:ncells), index = node id
+  DBuf<int32_t> nd_lo, nd_hi;  // inoderange (0-based positions, inclusive)
+  DBuf<int32_t> leaf_flag;   // leaf_is_active
+  DBuf<int32_t> leaves;      // leaf node ids
+  DBuf<int32_t> pleaf;       // tree position -> index into leaves
+  DBuf<int32_t> iparts;      // inodeparts: signed 1-based original index
+  DBuf<double> tx, ty, tz, th;  // treecache x, y, z, h (tree order)
+  DBuf<double> th_in;        // h at densityiterate entry (tree order)
+  DBuf<double> tmpd;         // scratch (tree build)
+  DBuf<int32_t> tmpi, tmpi2;  // scratch (tree build)
+  DBuf<int32_t> frontier_a, frontier_b;  // node ids of a tree level
+  DBuf<double> fbox_a, fbox_b;           // their bounding boxes (6 each)
+  DBuf<int32_t> counters;    // small device counters
+  DBuf<double> dred;         // small device reductions
+
+  // ---- neighbour search ----
+  DBuf<uint32_t> cand;       // candidate leaves per leaf [nleaves*caps.cand]
+  DBuf<int32_t> ncand;       // count per leaf
+  DBuf<int32_t> ntrial;      // trial particles per leaf
+  DBuf<uint32_t> nlist;      // neighbour lists [caps.nlist][nact]
+  DBuf<int32_t> nlist_n;
+
+  // ---- tree-ordered particle mirror ----
+  DBuf<double> tvx, tvy, tvz, tu;        // velocities, thermal energy
+  DBuf<double> tfx, tfy, tfz;            // fxyzu + fext (density's div a)
+  DBuf<double> tBx, tBy, tBz, tpsi;      // Bevol
+  DBuf<float> tgradh, talpha;            // gradh(1), alphaind(1)
+  DBuf<double> tpr, tcs, teps;           // eos P, c_s, dust fraction
+  DBuf<int8_t> tphase;                   // iphase
+  DBuf<int32_t> act;                     // active tree positions
+  int nact = 0;
+
+  // per-particle / per-leaf outputs of density kept for statistics
+  DBuf<int32_t> p_nneigh, p_nits, leaf_nits;
+  DBuf<double> stats_d;
+  DBuf<long long> stats_i;
+
+  // neighbour statistics of the last densityiterate (dens.F90:1137)
+  double trialmean = -1, actualmean = -1;
+  int maxtrial = -1, maxactual = -1;
+  int64_t nrhocalc = -1, nactualtot = -1;
+
+  // staging buffers for host arrays
+  DBuf<char> stage[16];
+
+  std::vector<TimedLaunch> timings;
+  std::vector<std::string> timing_names;
+  bool timing_enabled = true;
+};
+
+// ---- error helpers -----------------------------------------------------
+#define PH_HIPCHECK(ctx, call)                                                     \
+  do {                                                                             \
+    hipError_t _e = (call);                                                        \
+    if (_e != hipSuccess) {                                                        \
+      (ctx)->err = std::string("HIP error ") + hipGetErrorString(_e) + " at " +    \
+                   __FILE__ + ":" + std::to_string(__LINE__) + " (" #call ")";    \
+      return PHSPH_ERR_HIP;                                                        \
+    }                                                                              \
+  } while (0)
+
+#define PH_RET(ctx, code, msg)   \
+  do {                           \
+    (ctx)->err = (msg);          \
+    return (code);               \
+  } while (0)
+
+// Launch wrapper: records a HIP event pair around the launch on the
+// context's stream so that kernel time can be reported per call.
+struct LaunchTimer {
+  phsph_ctx *c;
+  TimedLaunch tl;
+  bool on;
+  LaunchTimer(phsph_ctx *ctx, const char *name) : c(ctx), on(ctx->timing_enabled) {
+    if (on) {
+      tl.name = name;
+      hipEventCreate(&tl.t0);
+      hipEventCreate(&tl.t1);
+      hipEventRecord(tl.t0, c->stream);
+    }
+  }
+  ~LaunchTimer() {
+    if (on) {
+      hipEventRecord(tl.t1, c->stream);
+      c->timings.push_back(tl);
+    }
+  }
+};
+
+void ctx_clear_timings(phsph_ctx *c);
+DevConst make_devconst(const phsph_params &p);
+
+static inline unsigned nblocks(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// tree.hip
+int tree_build(phsph_ctx *c, double *d_xyzh, int ldxyzh, int np, const int8_t *d_iphase);
+// density.hip
+struct DensIOArgs {  // device pointers in the caller's layout
+  double *xyzh; int ldx;
+  const double *vxyzu; int ldv;
+  const double *fxyzu; int ldf;
+  const double *fext;
+  const double *Bevol;
+  const int8_t *iphase;
+  float *divcurlv; int lddcv;
+  float *divcurlB;
+  float *alphaind;
+  float *gradh; int ldgradh;
+  float *dvdx;
+};
+int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a);
+int walk_leaves(phsph_ctx *c, int mode, double rfac, const double *hmaxn);
+int compact_active(phsph_ctx *c, const int32_t *flag, int n);
+int dens_particle_stats(phsph_ctx *c, int32_t *d_nneigh, int32_t *d_nits);
+// force.hip
+struct ForceIOArgs {  // device pointers in the caller's layout
+  const double *xyzh; int ldx;
+  const double *vxyzu; int ldv;
+  double *fxyzu; int ldf;
+  float *divcurlv; int lddcv;
+  const float *divcurlB;
+  const double *Bevol;
+  double *dBevol;
+  float *divBsymm;
+  const double *dustfrac;
+  double *ddustevol, *deltav, *tstop;
+  const float *gradh; int ldgradh;
+  const float *alphaind;
+  const double *eos_vars; int ldeos;
+  const int8_t *iphase;
+};
+int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc, double *dtf);
+int cons2prim_run(phsph_ctx *c, int np, const double *xyzh, int ldx, const double *vxyzu, int ldv,
+                  const float *dvdx, float *alphaind, const double *dustevol, double *dustfrac,
+                  double *eos, int ldeos);

@@ -1,0 +1,608 @@
+// force.hip -- primitive variables and the force pair loop
+// (cons2prim_everything, cons2prim.f90:274-456; force, force.F90:193-905;
+// compute_forces :914-2060; get_stress :2068-2168; start_cell
+// :2172-2514; finish_cell_and_store_results :2649-3330).
+//
+// Physics covered: hydro with the grad-h terms, Monaghan viscosity with the
+// Cullen-Dehnen alpha, artificial conductivity, ideal MHD (anisotropic
+// stress, Borve div B source subtraction, artificial resistivity, psi
+// cleaning) and one-fluid dust (sqrt(eps/(1-eps)) evolution, Delta v).
+//
+// One thread per active particle.  Pass 1 collects the symmetric neighbour
+// set {j : q_i < R or q_j < R} (force.F90:1287) with the same separately
+// rounded FP64 tests as the reference; pass 2 evaluates the pair terms.
+// Quantities of particle j that the reference recomputes per pair
+// (rho_j, P/rho^2, wave speed, MHD stress, stopping time) are evaluated once
+// per particle by k_force_prep with the same arithmetic.
+#include "ctx.h"
+
+enum { FFL_LIST = 2, FFL_DTC = 32 };
+
+struct JTerms {  // per particle, tree order
+  const double *x, *y, *z, *hinv, *vx, *vy, *vz, *u;
+  const float *gradh, *alpha;
+  const double *rho, *rho1, *pr, *pro2, *vwave;
+  const double *Bx, *By, *Bz, *psi;                 // B (not B/rho), psi
+  const double *sxx, *sxy, *sxz, *syy, *syz, *szz;  // MHD stress
+  const double *eps, *sqeps, *ts;                   // dust
+};
+
+
+// get_ts (dust.f90:161-276), idrag 2 (constant K) and 3 (constant t_s)
+__device__ __forceinline__ double get_ts_d(const DevConst &dc, double rhogas, double rhodust) {
+  if (dc.idrag == 2) {
+    if (dc.K_code > 0.) return rhogas * rhodust / (dc.K_code * (rhogas + rhodust));
+    return PH_HUGE64;
+  }
+  if (dc.idrag == 3) return dc.K_code;
+  return 0.;
+}
+
+// per-particle j terms (force.F90:1455-1509 with get_stress)
+__global__ void k_force_prep(DevConst dc, int nlive, const int32_t *__restrict__ iparts,
+                             const double *__restrict__ xyzh, int ldx, const double *__restrict__ vxyzu, int ldv,
+                             const float *__restrict__ gradh, int ldgradh, const float *__restrict__ alphaind,
+                             const double *__restrict__ eos, int ldeos, const double *__restrict__ Bevol,
+                             const double *__restrict__ dustfrac, const float *__restrict__ divcurlB, double *h,
+                             double *hinv, double *vx, double *vy, double *vz, double *u, float *tgradh,
+                             float *talpha, double *rho, double *rho1, double *pr, double *cs, double *pro2,
+                             double *vwave, double *Bx, double *By, double *Bz, double *psi, double *Bex,
+                             double *Bey, double *Bez, double *sxx, double *sxy, double *sxz, double *syy,
+                             double *syz, double *szz, double *eps, double *sqeps, double *ts, double *divB) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nlive) return;
+  const int o = abs(iparts[p]) - 1;
+  const double pm = dc.pmass;
+  const double hh = xyzh[(size_t)o * ldx + 3];
+  h[p] = hh;
+  const double hj1 = 1. / hh;  // xyzcache(4) = 1/treecache(4) (kdtree.F90:1202)
+  hinv[p] = hj1;
+  vx[p] = vxyzu[(size_t)o * ldv + 0];
+  vy[p] = vxyzu[(size_t)o * ldv + 1];
+  vz[p] = vxyzu[(size_t)o * ldv + 2];
+  u[p] = (dc.maxvxyzu >= 4) ? vxyzu[(size_t)o * ldv + 3] : 0.;
+  tgradh[p] = gradh[(size_t)o * ldgradh];
+  talpha[p] = alphaind[(size_t)o * 3];
+  const double prj = eos[(size_t)o * ldeos], csj = eos[(size_t)o * ldeos + 1];
+  pr[p] = prj;
+  cs[p] = csj;
+  const double hj = 1. / hj1;
+  const double rhoj = rhoh_d(hj, dc.hfact, pm);
+  const double rho1j = 1. / rhoj;
+  rho[p] = rhoj;
+  rho1[p] = rho1j;
+  double Bxj = 0., Byj = 0., Bzj = 0.;
+  if (dc.mhd) {
+    const double rhom = rhoh_d(hh, dc.hfact, pm);  // force.F90:1439-1443
+    const double *Be = Bevol + (size_t)o * 4;
+    Bex[p] = Be[0]; Bey[p] = Be[1]; Bez[p] = Be[2];
+    Bxj = Be[0] * rhom; Byj = Be[1] * rhom; Bzj = Be[2] * rhom;
+    Bx[p] = Bxj; By[p] = Byj; Bz[p] = Bzj;
+    psi[p] = Be[3];
+    divB[p] = (double)divcurlB[(size_t)o * 4];
+    // get_stress MHD branch (force.F90:2136-2158)
+    const double Brx = Bxj * rho1j, Bry = Byj * rho1j, Brz = Bzj * rho1j;
+    const double Bro2 = Brx * Brx + Bry * Bry + Brz * Brz;
+    const double va2 = Bro2 * rhoj;
+    vwave[p] = sqrt(csj * csj + va2);
+    sxx[p] = -(pm * Brx * Brx); sxy[p] = -(pm * Brx * Bry); sxz[p] = -(pm * Brx * Brz);
+    syy[p] = -(pm * Bry * Bry); syz[p] = -(pm * Bry * Brz); szz[p] = -(pm * Brz * Brz);
+    pro2[p] = prj * rho1j * rho1j + 0.5 * Bro2;
+  } else {
+    vwave[p] = csj;
+    pro2[p] = prj * rho1j * rho1j;
+  }
+  if (dc.dust) {
+    const double e = dustfrac[o];
+    eps[p] = e;
+    sqeps[p] = sqrt(e / (1. - e));
+    double tsj = get_ts_d(dc, rhoj * (1. - e), rhoj * e);
+    if (dc.ilimitdustflux) tsj = fmin(tsj, hj / csj);
+    ts[p] = tsj;
+  }
+}
+
+// One active particle; returns false if its neighbour list overflowed.
+template <int K, bool MHD, bool DUST, bool THERMAL>
+__device__ __forceinline__ bool force_one(
+    const DevConst &dc, int t, const int32_t *__restrict__ act, int nact, const int32_t *__restrict__ pleaf,
+    const uint32_t *__restrict__ cand, const int32_t *__restrict__ ncand, int capc,
+    const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi, const int32_t *__restrict__ iparts,
+    const JTerms &J, const double *__restrict__ h, const double *__restrict__ cs, const double *__restrict__ Bex,
+    const double *__restrict__ Bey, const double *__restrict__ Bez, uint32_t *__restrict__ nlist, int capl,
+    const ForceIOArgs &out, int32_t *__restrict__ flags, double &dtc_out, double &dtf_out) {
+  const int p = act[t];
+  const int l = pleaf[p];
+  const double pm = dc.pmass;
+  const double R2 = dc.radkern2;
+  const double xi = J.x[p], yi = J.y[p], zi = J.z[p];
+  const double hi = h[p];
+  // rhoanddhdrho (part.F90:805-816)
+  const double hi1 = 1. / fabs(hi);
+  const double tt = dc.hfact * hi1;
+  const double rhoi = pm * (tt * tt * tt);
+  const double rho1i = 1. / rhoi;
+  const double hi21 = xmul(hi1, hi1), hi41 = hi21 * hi21;
+  double gradhi = (double)J.gradh[p];
+  if (!(gradhi > 0.)) gradhi = 1.;  // force.F90:2616-2621
+  const double hfacgrkern = hi41 * dc.cnormk * gradhi;
+  const double vxi = J.vx[p], vyi = J.vy[p], vzi = J.vz[p], eni = J.u[p];
+  const double pri = J.pr[p], csi = cs[p], alphai = (double)J.alpha[p];
+  // i terms: get_stress with rho from rhoanddhdrho
+  double Bxi = 0., Byi = 0., Bzi = 0., psii = 0., pro2i, vwavei;
+  double sxi = 0., sxyi = 0., sxzi = 0., syi = 0., syzi = 0., szi = 0.;
+  if (MHD) {
+    Bxi = Bex[p] * rhoi; Byi = Bey[p] * rhoi; Bzi = Bez[p] * rhoi;
+    psii = J.psi[p];
+    const double Brx = Bxi * rho1i, Bry = Byi * rho1i, Brz = Bzi * rho1i;
+    const double Bro2 = Brx * Brx + Bry * Bry + Brz * Brz;
+    vwavei = sqrt(csi * csi + Bro2 * rhoi);
+    sxi = -(pm * Brx * Brx); sxyi = -(pm * Brx * Bry); sxzi = -(pm * Brx * Brz);
+    syi = -(pm * Bry * Bry); syzi = -(pm * Bry * Brz); szi = -(pm * Brz * Brz);
+    pro2i = pri * rho1i * rho1i + 0.5 * Bro2;
+  } else {
+    vwavei = csi;
+    pro2i = pri * rho1i * rho1i;
+  }
+  double epsi = 0., tsi = 0., epstsi = 0., sqepsi = 0.;
+  if (DUST) {
+    epsi = J.eps[p];
+    tsi = get_ts_d(dc, rhoi * (1. - epsi), rhoi * epsi);
+    if (dc.ilimitdustflux) tsi = fmin(tsi, hi / csi);
+    epstsi = epsi * tsi;
+    sqepsi = sqrt(epsi / (1. - epsi));
+  }
+  const double rho21i = rho1i * rho1i;
+  const double mrhoi5 = 0.5 * pm * rho1i;
+  const double auterm = mrhoi5 * dc.alphau;
+  const double avBterm = mrhoi5 * dc.alphaB * rho1i;
+  // --- pass 1: symmetric neighbour set ---
+  int nl = 0;
+  const int nc = ncand[l];
+  for (int k = 0; k < nc; k++) {
+    const uint32_t e = cand[(size_t)l * capc + k];
+    const int cn = cand_node(e), code = cand_code(e);
+    int sx, sy, sz;
+    off_decode(code, sx, sy, sz);
+    const double ox = sx ? copysign(dc.dxbound, (double)sx) : 0.;
+    const double oy = sy ? copysign(dc.dybound, (double)sy) : 0.;
+    const double oz = sz ? copysign(dc.dzbound, (double)sz) : 0.;
+    const int j1 = nd_hi[cn];
+    for (int j = nd_lo[cn]; j <= j1; j++) {
+      if (j == p) continue;
+      double dx = xsub(xi, xadd(J.x[j], ox)), dy = xsub(yi, xadd(J.y[j], oy)), dz = xsub(zi, xadd(J.z[j], oz));
+      if (dc.periodic) {
+        dx = min_image(dx, dc.dxbound, 0.5 * dc.dxbound);
+        dy = min_image(dy, dc.dybound, 0.5 * dc.dybound);
+        dz = min_image(dz, dc.dzbound, 0.5 * dc.dzbound);
+      }
+      const double r2 = r2_exact(dx, dy, dz);
+      const double hj1 = J.hinv[j];
+      if (xmul(r2, hi21) < R2 || xmul(r2, xmul(hj1, hj1)) < R2) {
+        if (nl < capl) nlist[(size_t)nl * nact + t] = (uint32_t)j | ((uint32_t)code << 27);
+        nl++;
+      }
+    }
+  }
+  if (nl > capl) {
+    atomicOr(flags, FFL_LIST);
+    return false;
+  }
+  // --- pass 2: pair terms (force.F90:1227-2056) ---
+  double fx = 0., fy = 0., fz = 0., drhodt = 0., dudtdiss = 0., dendtdiss = 0.;
+  double divBsym = 0., dBx_s = 0., dBy_s = 0., dBz_s = 0., divBdiff = 0.;
+  double ddust = 0., dudtdust = 0., dvx_s = 0., dvy_s = 0., dvz_s = 0.;
+  double vsigmax = 0.;
+  const bool usej_always = MHD || DUST || (THERMAL && !dc.gravity);
+  for (int k = 0; k < nl; k++) {
+    const uint32_t e = nlist[(size_t)k * nact + t];
+    const int j = (int)(e & 0x07ffffffu);
+    int sx, sy, sz;
+    off_decode((int)(e >> 27), sx, sy, sz);
+    const double ox = sx ? copysign(dc.dxbound, (double)sx) : 0.;
+    const double oy = sy ? copysign(dc.dybound, (double)sy) : 0.;
+    const double oz = sz ? copysign(dc.dzbound, (double)sz) : 0.;
+    double dx = xsub(xi, xadd(J.x[j], ox)), dy = xsub(yi, xadd(J.y[j], oy)), dz = xsub(zi, xadd(J.z[j], oz));
+    if (dc.periodic) {
+      dx = min_image(dx, dc.dxbound, 0.5 * dc.dxbound);
+      dy = min_image(dy, dc.dybound, 0.5 * dc.dybound);
+      dz = min_image(dz, dc.dzbound, 0.5 * dc.dzbound);
+    }
+    const double rij2 = r2_exact(dx, dy, dz);
+    const double q2i = xmul(rij2, hi21);
+    const double hj1 = J.hinv[j];
+    const double hj21 = xmul(hj1, hj1);
+    const double q2j = xmul(rij2, hj21);
+    double rij1, qi;
+    if (rij2 > PH_TINY64) {
+      rij1 = 1. / sqrt(rij2);
+      qi = (rij2 * rij1) * hi1;
+    } else {
+      rij1 = 0.;
+      qi = 0.;
+    }
+    const double grkerni = (q2i < R2) ? kern_g<K, double>(q2i, qi) * hfacgrkern : 0.;
+    double grkernj = 0.;
+    bool usej = false;
+    if (q2j < R2) {
+      const double qj = (rij2 * rij1) * hj1;
+      grkernj = kern_g<K, double>(q2j, qj) * hj21 * hj21 * dc.cnormk * (double)J.gradh[j];
+      usej = true;
+    }
+    if (usej_always) usej = true;
+    const double runix = dx * rij1, runiy = dy * rij1, runiz = dz * rij1;
+    const double dvx = vxi - J.vx[j], dvy = vyi - J.vy[j], dvz = vzi - J.vz[j];
+    const double projv = dvx * runix + dvy * runiy + dvz * runiz;
+    const double denij = THERMAL ? (eni - J.u[j]) : 0.;
+    const double vsigi = fmax(vwavei - dc.beta * projv, 0.);
+    const double vsigavi = fmax(alphai * vwavei - dc.beta * projv, 0.);
+    vsigmax = fmax(vsigmax, vsigi);
+    double rho1j = 0., rho21j = 0., pro2j = 0., vwavej = 0., vsigavj = 0., autermj = 0., avBtermj = 0.;
+    double rhoj = 0., prj = 0.;
+    if (usej) {
+      rhoj = J.rho[j];
+      rho1j = J.rho1[j];
+      rho21j = rho1j * rho1j;
+      prj = J.pr[j];
+      pro2j = J.pro2[j];
+      vwavej = J.vwave[j];
+      const double mrhoj5 = 0.5 * pm * rho1j;
+      autermj = mrhoj5 * dc.alphau;
+      avBtermj = mrhoj5 * dc.alphaB * rho1j;
+      vsigmax = fmax(vsigmax, fmax(vwavej - dc.beta * projv, 0.));
+      vsigavj = fmax((double)J.alpha[j] * vwavej - dc.beta * projv, 0.);
+    }
+    double qrho2i = 0., qrho2j = 0.;
+    if (projv < 0.) {
+      qrho2i = -0.5 * rho1i * vsigavi * projv;
+      if (usej) qrho2j = -0.5 * rho1j * vsigavj * projv;
+    }
+    const double dudtdissi = pm * qrho2i * projv * grkerni;
+    const double gradpi = pm * (pro2i + qrho2i) * grkerni;
+    const double gradpj = usej ? pm * (pro2j + qrho2j) * grkernj : 0.;
+    double dendissterm = 0.;
+    if (THERMAL) {
+      double vsigu;
+      if (dc.gravity) vsigu = fabs(projv);
+      else vsigu = sqrt(fabs(pri - prj) * (2. / (rhoi + rhoj)));
+      dendissterm = vsigu * denij * (auterm * grkerni + autermj * grkernj);
+    }
+    double projsx = 0., projsy = 0., projsz = 0., dudtresist = 0.;
+    if (MHD) {
+      const double Bxj = J.Bx[j], Byj = J.By[j], Bzj = J.Bz[j], psij = J.psi[j];
+      const double dBx = Bxi - Bxj, dBy = Byi - Byj, dBz = Bzi - Bzj;
+      const double projBi = Bxi * runix + Byi * runiy + Bzi * runiz;
+      const double projBj = Bxj * runix + Byj * runiy + Bzj * runiz;
+      const double projdB = dBx * runix + dBy * runiy + dBz * runiz;
+      const double dB2 = dBx * dBx + dBy * dBy + dBz * dBz;
+      divBdiff += -pm * projdB * grkerni;
+      const double ax = dvx - projv * runix, ay = dvy - projv * runiy, az = dvz - projv * runiz;
+      const double vsigB = sqrt(ax * ax + ay * ay + az * az);
+      const double dBdissterm = (avBterm * grkerni + avBtermj * grkernj) * vsigB;
+      if (dc.iresistive_heating > 0) dudtresist = -0.5 * dB2 * dBdissterm;
+      const double pmi = pm * rho21i * grkerni;
+      const double pmj = pm * rho21j * grkernj;
+      const double termi = pmi * projBi;
+      divBsym += termi + pmj * projBj;
+      const double dpsiterm = dc.overcleanfac * (pmi * psii * vwavei + pmj * psij * vwavej);
+      dBx_s += -termi * dvx + dBdissterm * dBx - dpsiterm * runix;
+      dBy_s += -termi * dvy + dBdissterm * dBy - dpsiterm * runiy;
+      dBz_s += -termi * dvz + dBdissterm * dBz - dpsiterm * runiz;
+      projsx = (sxi * runix + sxyi * runiy + sxzi * runiz) * grkerni +
+               (J.sxx[j] * runix + J.sxy[j] * runiy + J.sxz[j] * runiz) * grkernj;
+      projsy = (sxyi * runix + syi * runiy + syzi * runiz) * grkerni +
+               (J.sxy[j] * runix + J.syy[j] * runiy + J.syz[j] * runiz) * grkernj;
+      projsz = (sxzi * runix + syzi * runiy + szi * runiz) * grkerni +
+               (J.sxz[j] * runix + J.syz[j] * runiy + J.szz[j] * runiz) * grkernj;
+    }
+    const double gradp = gradpi + gradpj;
+    fx = fx - runix * gradp - projsx;
+    fy = fy - runiy * gradp - projsy;
+    fz = fz - runiz * gradp - projsz;
+    drhodt += projv * grkerni;
+    if (THERMAL) {
+      dudtdiss += dudtdissi + dudtresist;
+      dendtdiss += dendissterm;
+    }
+    if (DUST) {
+      const double epsj = J.eps[j];
+      const double tsj = J.ts[j];
+      const double epstsj = epsj * tsj;
+      if (epsi > 0. || epsj > 0.) {
+        const double grkernav = 0.5 * (grkerni + grkernj);
+        const double dft = pm * J.sqeps[j] * rho1j *
+                           ((tsi - epstsi) * (1. - epsi) / (1. - epsi) + (tsj - epstsj) * (1. - epsj) / (1. - epsj)) *
+                           (pri - prj) * grkernav * rij1;
+        ddust -= dft;
+        if (THERMAL) dudtdust -= sqepsi * dft * denij;
+      }
+      if (epsi < 1.) {
+        const double term = tsi / (1. - epsi) * pm * (pro2i * grkerni + pro2j * grkernj);
+        dvx_s += term * runix; dvy_s += term * runiy; dvz_s += term * runiz;
+      }
+    }
+  }
+  // --- finish_cell_and_store_results ---
+  const int o = abs(iparts[p]) - 1;
+  if (MHD) {
+    const double B2i = Bxi * Bxi + Byi * Byi + Bzi * Bzi;
+    double frac = 0.;
+    if (B2i > 0.) {
+      const double betai = 2. * pri / B2i;
+      frac = (betai < 2.) ? 1. : (betai < 10.) ? (10. - betai) * 0.125 : 0.;
+    }
+    fx = fx - Bxi * divBsym * frac;
+    fy = fy - Byi * divBsym * frac;
+    fz = fz - Bzi * divBsym * frac;
+    if (out.divBsymm) out.divBsymm[o] = (float)(rhoi * divBsym);
+  }
+  const double f2i = fx * fx + fy * fy + fz * fz;
+  double *fo = out.fxyzu + (size_t)o * out.ldf;
+  fo[0] = fx; fo[1] = fy; fo[2] = fz;
+  const double drhodti = pm * drhodt;
+  const double divvi = -drhodti * rho1i;
+  out.divcurlv[(size_t)o * out.lddcv] = (float)divvi;
+  double dtc = dc.dtmax, dtf = PH_BIGNUMBER, dtclean = PH_BIGNUMBER, dtdusti = PH_BIGNUMBER;
+  double fxyz4 = 0.;
+  if (THERMAL) {
+    const double rhogasi = DUST ? rhoi * (1. - epsi) : rhoi;
+    const double fac = (rhogasi > PH_TINY64) ? rhoi / rhogasi : 0.;
+    const double pdv_work = fac * pri * rho1i * rho1i * drhodti;
+    if (dc.ipdv_heating > 0) fxyz4 += pdv_work;
+    if (dc.ishock_heating > 0) fxyz4 += fac * dudtdiss;
+    fxyz4 += fac * dendtdiss;
+    if (DUST) fxyz4 += 0.5 * fac * rho1i * dudtdust;
+  }
+  if (MHD) {
+    double *dB = out.dBevol + (size_t)o * 4;
+    dB[0] = dBx_s; dB[1] = dBy_s; dB[2] = dBz_s;
+    if (dc.psidecayfac > 0.) {
+      const double vclean = dc.overcleanfac * vwavei;
+      const double dtau = dc.psidecayfac * vclean * hi1;
+      dB[3] = -vclean * divBdiff * rho1i - psii * dtau - 0.5 * psii * divvi;
+      dtclean = dc.C_cour * hi / (vclean + PH_TINY64);
+    }
+  }
+  if (DUST) {
+    const double om = 1. - epsi;
+    out.ddustevol[o] = 0.5 * (ddust * rho1i / (om * om));
+    out.deltav[(size_t)o * 3 + 0] = dvx_s;
+    out.deltav[(size_t)o * 3 + 1] = dvy_s;
+    out.deltav[(size_t)o * 3 + 2] = dvz_s;
+    if (out.tstop) out.tstop[o] = tsi;
+    if (epsi > 0. && csi > 0. && epsi > PH_EPS64) {
+      const double tseff = (1. - epsi) / epsi * (epsi * tsi);
+      const double den = epsi * tseff * csi * csi;
+      if (den > PH_TINY64) dtdusti = dc.C_force * hi * hi / den;
+    }
+  }
+  const double vsigdtc = fmax(vsigmax, vwavei);
+  if (vsigdtc > PH_TINY64) dtc = dc.C_cour * hi / (vsigdtc * fmax(dc.alpha, 1.0));
+  if (THERMAL) {
+    if (eni + dtc * fxyz4 < PH_EPS64 && eni > PH_EPS64) fxyz4 = fxyz4 / (1. - dtc * fxyz4 / eni);
+    fo[3] = fxyz4;
+  }
+  if (fabs(f2i) > PH_EPS64) dtf = dc.C_force * sqrt(hi / sqrt(f2i));
+  if (!(dtc > PH_TINY64) || !(dtc < PH_HUGE64)) atomicOr(flags, FFL_DTC);
+  dtc_out = dtc;
+  dtf_out = fmin(fmin(dtf, dtdusti), dtclean);
+  return true;
+}
+
+template <int K, bool MHD, bool DUST, bool THERMAL>
+__global__ void __launch_bounds__(256)
+k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *__restrict__ pleaf,
+        const uint32_t *__restrict__ cand, const int32_t *__restrict__ ncand, int capc,
+        const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi, const int32_t *__restrict__ iparts,
+        JTerms J, const double *__restrict__ h, const double *__restrict__ cs, const double *__restrict__ Bex,
+        const double *__restrict__ Bey, const double *__restrict__ Bez, uint32_t *__restrict__ nlist, int capl,
+        ForceIOArgs out, unsigned long long *__restrict__ dtred, int32_t *__restrict__ flags) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double dtc = PH_HUGE64, dtfm = PH_HUGE64;
+  if (t < nact) {
+    if (!force_one<K, MHD, DUST, THERMAL>(dc, t, act, nact, pleaf, cand, ncand, capc, nd_lo, nd_hi, iparts, J, h,
+                                          cs, Bex, Bey, Bez, nlist, capl, out, flags, dtc, dtfm)) {
+      dtc = PH_HUGE64;
+      dtfm = PH_HUGE64;
+    }
+  }
+  // global minima (force.F90:3312-3327): positive doubles order as integers;
+  // every lane of the wave takes part in the shuffles
+  unsigned long long a = __double_as_longlong(dtc), b = __double_as_longlong(dtfm);
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long a2 = __shfl_xor(a, off, 64), b2 = __shfl_xor(b, off, 64);
+    a = a < a2 ? a : a2;
+    b = b < b2 ? b : b2;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&dtred[0], a);
+    atomicMin(&dtred[1], b);
+  }
+}
+
+__global__ void k_force_flags(DevConst dc, int nlive, const int32_t *__restrict__ iparts,
+                              const int8_t *__restrict__ iphase, int32_t *actflag) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nlive) return;
+  int ip = iparts[p];
+  int o = abs(ip) - 1;
+  actflag[p] = (ip > 0) && (iphase == nullptr || iphase[o] > 0);
+}
+
+// node hmax from the current smoothing lengths: leaves first, then every
+// ancestor takes the maximum (positive doubles order as their bit patterns)
+__global__ void k_leaf_hmax(int nleaves, const int32_t *leaves, const int32_t *nd_lo, const int32_t *nd_hi,
+                            const phsph_kdnode *nd, const double *h, unsigned long long *hmaxn) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nleaves) return;
+  int c = leaves[l];
+  double hm = 0.;
+  for (int p = nd_lo[c]; p <= nd_hi[c]; p++) hm = fmax(hm, h[p]);
+  unsigned long long v = __double_as_longlong(hm);
+  int n = c;
+  while (n != 0) {
+    atomicMax(&hmaxn[n], v);
+    n = nd[n].parent;
+  }
+}
+__global__ void k_zero_u64(unsigned long long *a, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0ull;
+}
+
+// ------------------------------------------------------------------------
+// cons2prim_everything for ieos 1 and 2 (cons2prim.f90:274-456)
+// ------------------------------------------------------------------------
+__global__ void k_cons2prim(DevConst dc, int np, const double *__restrict__ xyzh, int ldx,
+                            const double *__restrict__ vxyzu, int ldv, const float *__restrict__ dvdx,
+                            float *__restrict__ alphaind, const double *__restrict__ dustevol,
+                            double *__restrict__ dustfrac, double *__restrict__ eos, int ldeos) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const double hi = xyzh[(size_t)i * ldx + 3];
+  if (hi < PH_TINY64) return;
+  const double rhoi = rhoh_d(hi, dc.hfact, dc.pmass);
+  double rhogas = rhoi;
+  if (dc.dust && dustevol) {
+    const double sv = dustevol[i];
+    const double e = sv * sv / (1. + sv * sv);
+    dustfrac[i] = e;
+    rhogas = rhoi * (1. - e);
+  }
+  double ponrho, cs;
+  if (dc.ieos == 1) {  // eos.f90:184-194
+    ponrho = dc.polyk;
+    cs = sqrt(ponrho);
+  } else {  // eos.f90:196-240
+    if (dc.maxvxyzu >= 4) {
+      const double u = vxyzu[(size_t)i * ldv + 3];
+      ponrho = (dc.gamma > 1.0001) ? (dc.gamma - 1.) * u : 2. / 3. * u;
+    } else {
+      ponrho = dc.polyk * pow(rhogas, dc.gamma - 1.);
+    }
+    cs = sqrt(dc.gamma * ponrho);
+  }
+  eos[(size_t)i * ldeos + 0] = ponrho * rhogas;
+  eos[(size_t)i * ldeos + 1] = cs;
+  if (ldeos > 2) eos[(size_t)i * ldeos + 2] = 0.;
+  if (alphaind && dvdx) {
+    // xi_limiter and get_alphaloc (shock_capturing.f90:131-178)
+    const float *d = dvdx + (size_t)i * 9;
+    const double divv = (double)d[0] + (double)d[4] + (double)d[8];
+    const double cx = (double)d[7] - (double)d[5], cy = (double)d[2] - (double)d[6], cz = (double)d[3] - (double)d[1];
+    const double m = fmax(-divv, 0.);
+    const double fac = m * m;
+    const double tr = cx * cx + cy * cy + cz * cz;
+    const double xil = (fac + tr > PH_EPS64) ? fac / (fac + tr) : 1.;
+    const double divvdt = (double)alphaind[(size_t)i * 3 + 2];
+    const double source = 10. * (hi * hi) * xil * fmax(-divvdt, 0.);
+    const double temp = cs * cs;
+    double aloc = dc.alpha;
+    if (temp > PH_EPS64) aloc = fmax(fmin(source / temp, dc.alphamax), dc.alpha);
+    alphaind[(size_t)i * 3 + 1] = (float)aloc;
+  }
+}
+
+int cons2prim_run(phsph_ctx *c, int np, const double *xyzh, int ldx, const double *vxyzu, int ldv,
+                  const float *dvdx, float *alphaind, const double *dustevol, double *dustfrac, double *eos,
+                  int ldeos) {
+  LaunchTimer lt(c, "cons2prim");
+  hipLaunchKernelGGL(k_cons2prim, dim3(nblocks(np, 256)), dim3(256), 0, c->stream, c->dc, np, xyzh, ldx, vxyzu,
+                     ldv, dvdx, alphaind, dustevol, dustfrac, eos, ldeos);
+  PH_HIPCHECK(c, hipGetLastError());
+  return PHSPH_OK;
+}
+
+// ------------------------------------------------------------------------
+int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, double *dtf_out) {
+  (void)icall;
+  hipStream_t s = c->stream;
+  const int n = c->nlive;
+  ForceBufs &B = c->fb;
+  DBuf<double> *dl[] = {&B.hinv, &B.u, &B.rho, &B.rho1, &B.pr, &B.pro2, &B.vwave, &B.cs, &B.h};
+  for (auto *d : dl) PH_HIPCHECK(c, d->ensure(n));
+  PH_HIPCHECK(c, c->tvx.ensure(n)); PH_HIPCHECK(c, c->tvy.ensure(n)); PH_HIPCHECK(c, c->tvz.ensure(n));
+  PH_HIPCHECK(c, c->tgradh.ensure(n)); PH_HIPCHECK(c, c->talpha.ensure(n));
+  const bool mhd = c->p.mhd, dust = c->p.use_dustfrac && c->p.ndustsmall > 0;
+  if (mhd) {
+    DBuf<double> *ml[] = {&B.Bx, &B.By, &B.Bz, &B.psi, &B.Bex, &B.Bey, &B.Bez, &B.divB,
+                          &B.s[0], &B.s[1], &B.s[2], &B.s[3], &B.s[4], &B.s[5]};
+    for (auto *d : ml) PH_HIPCHECK(c, d->ensure(n));
+  }
+  if (dust) {
+    PH_HIPCHECK(c, B.eps.ensure(n)); PH_HIPCHECK(c, B.sqeps.ensure(n)); PH_HIPCHECK(c, B.ts.ensure(n));
+  }
+  {
+    LaunchTimer lt(c, "force_prep");
+    hipLaunchKernelGGL(k_force_prep, dim3(nblocks(n, 256)), dim3(256), 0, s, c->dc, n, c->iparts.p, a.xyzh, a.ldx,
+                       a.vxyzu, a.ldv, a.gradh, a.ldgradh, a.alphaind, a.eos_vars, a.ldeos, a.Bevol, a.dustfrac,
+                       a.divcurlB, B.h.p, B.hinv.p, c->tvx.p, c->tvy.p, c->tvz.p, B.u.p, c->tgradh.p, c->talpha.p,
+                       B.rho.p, B.rho1.p, B.pr.p, B.cs.p, B.pro2.p, B.vwave.p, B.Bx.p, B.By.p, B.Bz.p, B.psi.p,
+                       B.Bex.p, B.Bey.p, B.Bez.p, B.s[0].p, B.s[1].p, B.s[2].p, B.s[3].p, B.s[4].p, B.s[5].p,
+                       B.eps.p, B.sqeps.p, B.ts.p, B.divB.p);
+  }
+  PH_HIPCHECK(c, hipGetLastError());
+  PH_HIPCHECK(c, c->tmpi.ensure(n));
+  hipLaunchKernelGGL(k_force_flags, dim3(nblocks(n, 256)), dim3(256), 0, s, c->dc, n, c->iparts.p, a.iphase,
+                     c->tmpi.p);
+  int rc = compact_active(c, c->tmpi.p, n);
+  if (rc) return rc;
+  const int nact = c->nact;
+  // node hmax of the final smoothing lengths for the h_j search radius
+  PH_HIPCHECK(c, c->hmaxf.ensure(c->ncells + 2));
+  hipLaunchKernelGGL(k_zero_u64, dim3(nblocks(c->ncells + 2, 256)), dim3(256), 0, s,
+                     (unsigned long long *)c->hmaxf.p, (int64_t)(c->ncells + 2));
+  hipLaunchKernelGGL(k_leaf_hmax, dim3(nblocks(c->nleaves, 256)), dim3(256), 0, s, c->nleaves, c->leaves.p,
+                     c->nd_lo.p, c->nd_hi.p, c->nd.p, B.h.p, (unsigned long long *)c->hmaxf.p);
+  rc = walk_leaves(c, 1, 1.0, c->hmaxf.p);
+  if (rc) return rc;
+  JTerms J{c->tx.p, c->ty.p, c->tz.p, B.hinv.p, c->tvx.p, c->tvy.p, c->tvz.p, B.u.p, c->tgradh.p, c->talpha.p,
+           B.rho.p, B.rho1.p, B.pr.p, B.pro2.p, B.vwave.p, B.Bx.p, B.By.p, B.Bz.p, B.psi.p,
+           B.s[0].p, B.s[1].p, B.s[2].p, B.s[3].p, B.s[4].p, B.s[5].p, B.eps.p, B.sqeps.p, B.ts.p};
+  PH_HIPCHECK(c, c->dtred.ensure(2));
+  for (int attempt = 0; attempt < 6; attempt++) {
+    PH_HIPCHECK(c, c->nlist.ensure((size_t)c->caps.nlist * (nact + 1)));
+    unsigned long long init[2];
+    double big = PH_HUGE64;
+    memcpy(&init[0], &big, 8);
+    init[1] = init[0];
+    PH_HIPCHECK(c, hipMemcpyAsync(c->dtred.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+    int32_t zero = 0;
+    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    if (nact > 0) {
+      LaunchTimer lt(c, "force");
+      const dim3 g(nblocks(nact, 256)), b(256);
+      const bool th = c->p.maxvxyzu >= 4;
+#define FL(KK, M, D, T)                                                                                          \
+  hipLaunchKernelGGL((k_force<KK, M, D, T>), g, b, 0, s, c->dc, c->act.p, nact, c->pleaf.p, c->cand.p, c->ncand.p,  \
+                     c->caps.cand, c->nd_lo.p, c->nd_hi.p, c->iparts.p, J, B.h.p, B.cs.p, B.Bex.p, B.Bey.p,         \
+                     B.Bez.p, c->nlist.p, c->caps.nlist, a, c->dtred.p, c->counters.p)
+#define FLK(KK)                                          \
+  if (mhd && dust) { if (th) FL(KK, true, true, true); else FL(KK, true, true, false); }     \
+  else if (mhd) { if (th) FL(KK, true, false, true); else FL(KK, true, false, false); }      \
+  else if (dust) { if (th) FL(KK, false, true, true); else FL(KK, false, true, false); }     \
+  else { if (th) FL(KK, false, false, true); else FL(KK, false, false, false); }
+      if (c->p.kernel == PHSPH_KERNEL_CUBIC) { FLK(PHSPH_KERNEL_CUBIC) } else { FLK(PHSPH_KERNEL_QUINTIC) }
+#undef FLK
+#undef FL
+    }
+    PH_HIPCHECK(c, hipGetLastError());
+    int32_t fl = 0;
+    unsigned long long red[2];
+    PH_HIPCHECK(c, hipMemcpyAsync(&fl, c->counters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    PH_HIPCHECK(c, hipMemcpyAsync(red, c->dtred.p, sizeof(red), hipMemcpyDeviceToHost, s));
+    PH_HIPCHECK(c, hipStreamSynchronize(s));
+    if (fl & FFL_LIST) {
+      c->caps.nlist = c->caps.nlist * 3 / 2;
+      continue;
+    }
+    if (fl & FFL_DTC) PH_RET(c, PHSPH_ERR_INPUT, "force: invalid dtc");
+    double dtc, dtf;
+    memcpy(&dtc, &red[0], 8);
+    memcpy(&dtf, &red[1], 8);
+    if (dtc_out) *dtc_out = (nact > 0) ? dtc : PH_BIGNUMBER;
+    if (dtf_out) *dtf_out = (nact > 0) ? fmin(dtf, PH_BIGNUMBER) : PH_BIGNUMBER;
+    return PHSPH_OK;
+  }
+  PH_RET(c, PHSPH_ERR_CAPACITY, "force: neighbour list capacity could not be satisfied");
+}

@@ -1,0 +1,293 @@
+"""Host-side interface to libphsph.so, mirroring the reference's calls.
+
+``Context`` keeps one library context (one HIP stream, the device-resident
+tree and particle mirror) and exposes the reference's entry points with the
+reference's names and argument meaning:
+
+    maketree(xyzh, iphase)                     kdtree.F90:117
+    densityiterate(icall, xyzh, vxyzu, ...)    dens.F90:117
+    cons2prim(xyzh, vxyzu, dvdx, alphaind,...) cons2prim.f90:274
+    force(icall, xyzh, vxyzu, ...)             force.F90:193
+    get_neighbour_stats()                      dens.F90:1137
+
+Arrays are numpy arrays (host memory) or torch tensors on the GPU, in the
+reference layout: Fortran a(ncomp, n) is a C-contiguous array of shape
+(n, ncomp).  Errors raise PhsphError carrying the C status code, as the
+reference would call fatal().  The library must be built (python -m
+phantom_amd.build); there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libphsph.so")
+
+EXPORTED = [
+    "phsph_default_params", "phsph_create", "phsph_destroy", "phsph_set_params", "phsph_last_error",
+    "phsph_maketree", "phsph_get_tree", "phsph_densityiterate", "phsph_cons2prim", "phsph_force",
+    "phsph_get_neighbour_stats", "phsph_get_particle_stats", "phsph_get_timings",
+]
+
+_lib = None
+
+
+class PhsphError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{abi.ERROR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def load_library(path=LIB_PATH):
+    """Load libphsph.so and declare the C signatures.  Raises if missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run `python -m phantom_amd.build`")
+    L = C.CDLL(path)
+    vp, i32, i64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    P = C.POINTER
+    L.phsph_default_params.argtypes = [P(abi.Params), i32]
+    L.phsph_default_params.restype = None
+    L.phsph_create.argtypes = [P(abi.Params), i32, P(vp)]
+    L.phsph_destroy.argtypes = [vp]
+    L.phsph_set_params.argtypes = [vp, P(abi.Params)]
+    L.phsph_last_error.argtypes = [vp]
+    L.phsph_last_error.restype = C.c_char_p
+    L.phsph_maketree.argtypes = [vp, vp, i32, i32, vp, vp, P(i64)]
+    L.phsph_get_tree.argtypes = [vp, vp, vp, vp, i64, P(i32)]
+    L.phsph_densityiterate.argtypes = [vp, i32, i32, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, i32, vp, vp, vp,
+                                       i32, vp]
+    L.phsph_cons2prim.argtypes = [vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp, i32, vp]
+    L.phsph_force.argtypes = [vp, i32, i32, vp, i32, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                              i32, vp, vp, i32, vp, P(dbl), P(dbl)]
+    L.phsph_get_neighbour_stats.argtypes = [vp, P(dbl), P(dbl), P(i32), P(i32), P(i64), P(i64)]
+    L.phsph_get_particle_stats.argtypes = [vp, i32, vp, vp]
+    L.phsph_get_timings.argtypes = [vp, P(C.c_char_p), P(dbl), i32]
+    for name in EXPORTED:
+        if name != "phsph_default_params" and name != "phsph_last_error":
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    """(pointer, leading dimension) of a numpy array or torch tensor."""
+    if a is None:
+        return None, 0
+    if isinstance(a, np.ndarray):
+        if not a.flags.c_contiguous:
+            raise ValueError("arrays must be C-contiguous (Fortran column-major a(ncomp,n))")
+        ld = a.shape[1] if a.ndim == 2 else 1
+        return C.c_void_p(a.ctypes.data), ld
+    # torch tensor
+    if not a.is_contiguous():
+        raise ValueError("tensors must be contiguous")
+    ld = a.shape[1] if a.dim() == 2 else 1
+    return C.c_void_p(a.data_ptr()), ld
+
+
+def _zeros_like_layout(ref, shape, dtype):
+    """Allocate an output array on the same side (host/device) as ``ref``."""
+    if isinstance(ref, np.ndarray):
+        return np.zeros(shape, dtype=dtype)
+    import torch
+    tdt = {np.float32: torch.float32, np.float64: torch.float64, np.int32: torch.int32}[dtype]
+    return torch.zeros(shape, dtype=tdt, device=ref.device)
+
+
+class Context:
+    """One MI355X context for the SPH hot path."""
+
+    def __init__(self, params, device=0):
+        self.params = params
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.phsph_create(C.byref(params), device, C.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = self.lib.phsph_last_error(h).decode() if h.value else "create failed"
+            if h.value:
+                self.lib.phsph_destroy(h)
+                self._h = None
+            raise PhsphError(rc, msg)
+        self.np = 0
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self.lib.phsph_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise PhsphError(rc, self.lib.phsph_last_error(self._h).decode())
+
+    def set_params(self, params):
+        self._check(self.lib.phsph_set_params(self._h, C.byref(params)))
+        self.params = params
+
+    # -- maketree -----------------------------------------------------------
+    def maketree(self, xyzh, iphase=None):
+        """Build the tree; xyzh is wrapped in place (periodic).  Returns
+        (leaf_is_active (ncells,), ncells)."""
+        n = xyzh.shape[0]
+        px, ldx = _ptr(xyzh)
+        pph, _ = _ptr(iphase)
+        maxp = max(self.params.maxp, n)
+        leaf = np.zeros(2 * maxp + 1, dtype=np.int32)
+        ncells = C.c_int64()
+        self._check(self.lib.phsph_maketree(self._h, px, ldx, n, pph, leaf.ctypes.data_as(C.c_void_p),
+                                            C.byref(ncells)))
+        self.np = n
+        self._ncells = ncells.value
+        h = xyzh[:, 3]
+        self._nlive = int((h >= 2.2250738585072014e-308).sum())
+        return leaf[:ncells.value], ncells.value
+
+    def get_tree(self):
+        """Mirror of node(1:ncells), inoderange(2,ncells) (1-based positions),
+        inodeparts(nlive) (signed 1-based particle indices) and maxlevel."""
+        ml = C.c_int32()
+        node_dt = np.dtype([("xcen", "f8", 3), ("size", "f8"), ("hmax", "f8"), ("mass", "f8"),
+                            ("leftchild", "i4"), ("rightchild", "i4"), ("parent", "i4"), ("idum", "i4")])
+        nc = self._ncells
+        node = np.zeros(nc, dtype=node_dt)
+        inoderange = np.zeros((nc, 2), dtype=np.int32)
+        inodeparts = np.zeros(self._nlive, dtype=np.int32)
+        self._check(self.lib.phsph_get_tree(self._h, node.ctypes.data_as(C.c_void_p),
+                                            inoderange.ctypes.data_as(C.c_void_p),
+                                            inodeparts.ctypes.data_as(C.c_void_p), nc, C.byref(ml)))
+        return dict(node=node, inoderange=inoderange, inodeparts=inodeparts, maxlevel=ml.value, ncells=nc)
+
+    # -- densityiterate ------------------------------------------------------
+    def densityiterate(self, icall, xyzh, vxyzu, fxyzu=None, fext=None, Bevol=None, iphase=None,
+                       divcurlv=None, divcurlB=None, alphaind=None, gradh=None, dvdx=None):
+        """densityiterate (dens.F90:117).  Outputs not passed in are
+        allocated on the side of xyzh.  Returns a dict of the outputs."""
+        n = xyzh.shape[0]
+        p = self.params
+        if divcurlv is None:
+            divcurlv = _zeros_like_layout(xyzh, (n, 1), np.float32)
+        if p.mhd and divcurlB is None:
+            divcurlB = _zeros_like_layout(xyzh, (n, 4), np.float32)
+        if alphaind is None:
+            alphaind = _zeros_like_layout(xyzh, (n, 3), np.float32)
+        if gradh is None:
+            gradh = _zeros_like_layout(xyzh, (n, 2 if p.gravity else 1), np.float32)
+        if dvdx is None:
+            dvdx = _zeros_like_layout(xyzh, (n, 9), np.float32)
+        px, ldx = _ptr(xyzh)
+        pv, ldv = _ptr(vxyzu)
+        pf, ldf = _ptr(fxyzu)
+        pe, _ = _ptr(fext)
+        pB, _ = _ptr(Bevol)
+        pph, _ = _ptr(iphase)
+        pdc, lddc = _ptr(divcurlv)
+        pdB, _ = _ptr(divcurlB)
+        pa, _ = _ptr(alphaind)
+        pg, ldg = _ptr(gradh)
+        pdv, _ = _ptr(dvdx)
+        self._check(self.lib.phsph_densityiterate(self._h, icall, n, px, ldx, pv, ldv, pf, ldf, pe, pB, pph, pdc,
+                                                  lddc, pdB, pa, pg, ldg, pdv))
+        return dict(divcurlv=divcurlv, divcurlB=divcurlB, alphaind=alphaind, gradh=gradh, dvdx=dvdx)
+
+    def get_neighbour_stats(self):
+        tm, am = C.c_double(), C.c_double()
+        mt, ma = C.c_int32(), C.c_int32()
+        nr, na = C.c_int64(), C.c_int64()
+        self._check(self.lib.phsph_get_neighbour_stats(self._h, C.byref(tm), C.byref(am), C.byref(mt),
+                                                       C.byref(ma), C.byref(nr), C.byref(na)))
+        return dict(trialmean=tm.value, actualmean=am.value, maxtrial=mt.value, maxactual=ma.value,
+                    nrhocalc=nr.value, nactualtot=na.value)
+
+    def get_particle_stats(self, n):
+        nn = np.zeros(n, dtype=np.int32)
+        it = np.zeros(n, dtype=np.int32)
+        self._check(self.lib.phsph_get_particle_stats(self._h, n, nn.ctypes.data_as(C.c_void_p),
+                                                      it.ctypes.data_as(C.c_void_p)))
+        return nn, it
+
+    # -- cons2prim -----------------------------------------------------------
+    def cons2prim(self, xyzh, vxyzu, dvdx, alphaind, dustevol=None, dustfrac=None, eos_vars=None):
+        n = xyzh.shape[0]
+        p = self.params
+        if eos_vars is None:
+            eos_vars = _zeros_like_layout(xyzh, (n, 3), np.float64)
+        dust = p.use_dustfrac and p.ndustsmall > 0
+        if dust and dustfrac is None:
+            dustfrac = _zeros_like_layout(xyzh, (n,), np.float64)
+        px, ldx = _ptr(xyzh)
+        pv, ldv = _ptr(vxyzu)
+        pdv, _ = _ptr(dvdx)
+        pa, _ = _ptr(alphaind)
+        pde, _ = _ptr(dustevol if dust else None)
+        pdf, _ = _ptr(dustfrac if dust else None)
+        pe, lde = _ptr(eos_vars)
+        self._check(self.lib.phsph_cons2prim(self._h, n, px, ldx, pv, ldv, pdv, pa, pde, pdf, pe, lde, None))
+        return dict(eos_vars=eos_vars, dustfrac=dustfrac)
+
+    # -- force ---------------------------------------------------------------
+    def force(self, icall, xyzh, vxyzu, gradh, alphaind, eos_vars, divcurlB=None, Bevol=None, dustfrac=None,
+              iphase=None, fxyzu=None, divcurlv=None, dBevol=None, divBsymm=None, ddustevol=None, deltav=None,
+              tstop=None):
+        """force (force.F90:193).  Returns a dict of the outputs and the
+        global dtcourant / dtforce."""
+        n = xyzh.shape[0]
+        p = self.params
+        dust = p.use_dustfrac and p.ndustsmall > 0
+        if fxyzu is None:
+            fxyzu = _zeros_like_layout(xyzh, (n, 4), np.float64)
+        if divcurlv is None:
+            divcurlv = _zeros_like_layout(xyzh, (n, 1), np.float32)
+        if p.mhd:
+            if dBevol is None:
+                dBevol = _zeros_like_layout(xyzh, (n, 4), np.float64)
+            if divBsymm is None:
+                divBsymm = _zeros_like_layout(xyzh, (n,), np.float32)
+        if dust:
+            if ddustevol is None:
+                ddustevol = _zeros_like_layout(xyzh, (n,), np.float64)
+            if deltav is None:
+                deltav = _zeros_like_layout(xyzh, (n, 3), np.float64)
+            if tstop is None:
+                tstop = _zeros_like_layout(xyzh, (n,), np.float64)
+        dtc, dtf = C.c_double(), C.c_double()
+        px, ldx = _ptr(xyzh)
+        pv, ldv = _ptr(vxyzu)
+        pf, ldf = _ptr(fxyzu)
+        pdc, lddc = _ptr(divcurlv)
+        pdB, _ = _ptr(divcurlB if p.mhd else None)
+        pB, _ = _ptr(Bevol if p.mhd else None)
+        pdBe, _ = _ptr(dBevol)
+        pdBs, _ = _ptr(divBsymm)
+        pdf, _ = _ptr(dustfrac if dust else None)
+        pdd, _ = _ptr(ddustevol)
+        pdl, _ = _ptr(deltav)
+        pts, _ = _ptr(tstop)
+        pg, ldg = _ptr(gradh)
+        pa, _ = _ptr(alphaind)
+        pe, lde = _ptr(eos_vars)
+        pph, _ = _ptr(iphase)
+        self._check(self.lib.phsph_force(self._h, icall, n, px, ldx, pv, ldv, pf, ldf, pdc, lddc, pdB, pB, pdBe,
+                                         pdBs, pdf, pdd, pdl, pts, pg, ldg, pa, pe, lde, pph, C.byref(dtc),
+                                         C.byref(dtf)))
+        return dict(fxyzu=fxyzu, divcurlv=divcurlv, dBevol=dBevol, divBsymm=divBsymm, ddustevol=ddustevol,
+                    deltav=deltav, tstop=tstop, dtcourant=dtc.value, dtforce=dtf.value)
+
+    def timings(self):
+        """Kernel times (ms) of the last call, measured with HIP events."""
+        cap = 4096
+        names = (C.c_char_p * cap)()
+        ms = (C.c_double * cap)()
+        n = self.lib.phsph_get_timings(self._h, names, ms, cap)
+        return [(names[k].decode(), ms[k]) for k in range(min(n, cap))]

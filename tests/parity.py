@@ -1,0 +1,57 @@
+"""Helpers shared by the GPU parity tests: run the same setup through the
+HIP library and the oracle and compare."""
+import numpy as np
+
+from oracle import oracle
+from phantom_amd import sph
+
+
+def run_gpu(setup, icall=1):
+    p = setup["params"]
+    ctx = sph.Context(p)
+    xyzh = setup["xyzh"].copy()
+    vxyzu = setup["vxyzu"]
+    alphaind = setup["alphaind"].copy()
+    leaf, ncells = ctx.maketree(xyzh)
+    tree = ctx.get_tree()
+    d = ctx.densityiterate(icall, xyzh, vxyzu, Bevol=setup.get("Bevol"), alphaind=alphaind)
+    stats = ctx.get_neighbour_stats()
+    nneigh, nits = ctx.get_particle_stats(xyzh.shape[0])
+    c = ctx.cons2prim(xyzh, vxyzu, d["dvdx"], alphaind, dustevol=setup.get("dustevol"))
+    f = ctx.force(1, xyzh, vxyzu, d["gradh"], alphaind, c["eos_vars"], divcurlB=d["divcurlB"],
+                  Bevol=setup.get("Bevol"), dustfrac=c["dustfrac"])
+    out = dict(ctx=ctx, xyzh=xyzh, leaf=leaf, ncells=ncells, tree=tree, dens=d, stats=stats, nneigh=nneigh,
+               nits=nits, c2p=c, force=f, alphaind=alphaind)
+    return out
+
+
+def run_oracle(setup, icall=1):
+    return oracle.run_all(setup, icall=icall)
+
+
+def rel_rms(a, b):
+    """max |a - b| / rms(b) (a normalised error that stays meaningful where
+    b crosses zero)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = np.sqrt(np.mean(b * b))
+    if scale == 0.0:
+        return float(np.max(np.abs(a - b)))
+    return float(np.max(np.abs(a - b)) / scale)
+
+
+def reachable_nodes(tree):
+    """Indices (0-based) of the nodes reachable from the root."""
+    node = tree["node"]
+    leaf = tree.get("leaf_is_active")
+    n = len(node)
+    used = np.zeros(n, dtype=bool)
+    used[0] = True
+    for k in range(n):
+        if not used[k]:
+            continue
+        l, r = node["leftchild"][k], node["rightchild"][k]
+        if l > 0 and (leaf is None or leaf[k] == 0):
+            used[l - 1] = True
+            used[r - 1] = True
+    return np.nonzero(used)[0]
