@@ -1,0 +1,209 @@
+"""Benchmark of the SPH hot path on one MI355X.
+
+One step = densityiterate(icall=1) + cons2prim + force over every particle
+of the workload, on arrays resident in HBM.  Before each step the smoothing
+lengths are reset to their initial values (one device-to-device copy of
+xyzh), so every step does the same work; the kd-tree is built once before
+the timed region and its build time is reported separately.
+
+Prints one JSON line (see the repository's measurement notes for fields).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from phantom_amd import abi, setups, work_model  # noqa: E402
+
+WORKLOADS = {
+    # the configuration the headline metric is quoted on: 3D shock tube,
+    # quintic kernel, Cullen-Dehnen viscosity + conductivity
+    "shock_tube_1M": lambda: setups.shock_tube(ny=58),
+    "box64_isothermal": lambda: setups.uniform_box(nx=64),
+    "mhd_dust_128": lambda: setups.mhd_dust_box(nx=128, jitter=0.05),
+    "hydro_box_8M": lambda: setups.uniform_box(nx=256, ny=256, nz=128, box=(-0.5, 0.5, -0.5, 0.5, -0.25, 0.25),
+                                                isothermal=False, seed=8),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="shock_tube_1M", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design note, "
+                                   "not an executable path", "n_gpus": args.gpus}))
+        return 2
+    from phantom_amd import device, sph
+
+    setup = WORKLOADS[args.workload]()
+    p = setup["params"]
+    p.precision = abi.PREC_MIXED if args.precision == "mixed" else abi.PREC_FP64
+    n = setup["xyzh"].shape[0]
+    log(f"workload {args.workload} n={n} precision={args.precision}")
+    D = device.DeviceArray
+    xyzh0 = D.from_numpy(setup["xyzh"])
+    xyzh = D.from_numpy(setup["xyzh"])
+    vxyzu = D.from_numpy(setup["vxyzu"])
+    alphaind = D.from_numpy(setup["alphaind"])
+    Bevol = D.from_numpy(setup["Bevol"]) if "Bevol" in setup else None
+    dustevol = D.from_numpy(setup["dustevol"]) if "dustevol" in setup else None
+    divcurlv = D((n, 1), np.float32)
+    divcurlB = D((n, 4), np.float32) if p.mhd else None
+    gradh = D((n, 1), np.float32)
+    dvdx = D((n, 9), np.float32)
+    eos = D((n, 3), np.float64)
+    dustfrac = D((n,), np.float64) if p.use_dustfrac else None
+    fxyzu = D((n, 4), np.float64)
+    dBevol = D((n, 4), np.float64) if p.mhd else None
+    divBsymm = D((n,), np.float32) if p.mhd else None
+    ddustevol = D((n,), np.float64) if p.use_dustfrac else None
+    deltav = D((n, 3), np.float64) if p.use_dustfrac else None
+    tstop = D((n,), np.float64) if p.use_dustfrac else None
+
+    ctx = sph.Context(p)
+    t0 = time.perf_counter()
+    ctx.maketree(xyzh)
+    tree_ms = 1e3 * (time.perf_counter() - t0)
+    tree_kernels = ctx.timings()
+    xyzh0.copy_from(xyzh)  # wrapped positions, initial h
+    log(f"tree built in {tree_ms:.2f} ms (first call)")
+
+    kt = {"density": [], "force": []}
+    phase = {"density": 0.0, "cons2prim": 0.0, "force": 0.0}
+
+    def step(record):
+        xyzh.copy_from(xyzh0)
+        a = time.perf_counter()
+        ctx.densityiterate(1, xyzh, vxyzu, Bevol=Bevol, divcurlv=divcurlv, divcurlB=divcurlB, alphaind=alphaind,
+                           gradh=gradh, dvdx=dvdx)
+        if record:
+            kt["density"] += [ms for nm, ms in ctx.timings() if nm == "density"]
+        b = time.perf_counter()
+        ctx.cons2prim(xyzh, vxyzu, dvdx, alphaind, dustevol=dustevol, dustfrac=dustfrac, eos_vars=eos)
+        c = time.perf_counter()
+        ctx.force(1, xyzh, vxyzu, gradh, alphaind, eos, divcurlB=divcurlB, Bevol=Bevol, dustfrac=dustfrac,
+                  fxyzu=fxyzu, divcurlv=divcurlv, dBevol=dBevol, divBsymm=divBsymm, ddustevol=ddustevol,
+                  deltav=deltav, tstop=tstop)
+        if record:
+            kt["force"] += [ms for nm, ms in ctx.timings() if nm == "force"]
+        d = time.perf_counter()
+        if record:
+            phase["density"] += b - a
+            phase["cons2prim"] += c - b
+            phase["force"] += d - c
+
+    for k in range(args.warmup):
+        step(False)
+        log(f"warmup step {k + 1}/{args.warmup} done")
+    device.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(True)
+    device.synchronize()
+    elapsed = time.perf_counter() - t0
+    log(f"timed {args.steps} steps in {elapsed:.3f} s")
+
+    stats = ctx.get_neighbour_stats()
+    nneigh, nits = ctx.get_particle_stats(n)
+    npairs = ctx.get_force_stats()["npairs"]
+    ms_step = 1e3 * elapsed / args.steps
+    updates = n * args.steps
+    value = updates / elapsed
+
+    fl_dens = work_model.density_flops(nneigh, nits, mhd=bool(p.mhd))
+    fl_force = work_model.force_flops(npairs, mhd=bool(p.mhd), dust=bool(p.use_dustfrac))
+    t_dens = float(np.mean(kt["density"])) * 1e-3
+    t_force = float(np.mean(kt["force"])) * 1e-3
+    peak = work_model.PEAK_FP64_TFLOPS if args.precision == "fp64" else work_model.PEAK_FP32_TFLOPS
+    dens_tf = fl_dens / t_dens / 1e12
+    force_tf = fl_force / t_force / 1e12
+    dominant = "force" if t_force >= t_dens else "density"
+    ach = force_tf if dominant == "force" else dens_tf
+    roofline = {"bound": "mfma", "kernel": dominant, "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(ach / peak, 5), "traffic": None,
+                "note": "VALU-bound pair loop; peak is the MI355X FP64 (or FP32) vector/matrix rate"}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(setup, args)
+
+    out = {
+        "metric": "density+force particle updates/s",
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-pair/f64-accum",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "n_particles": n, "kernel": "cubic" if p.kernel == 0 else "quintic",
+                   "precision": args.precision, "mhd": bool(p.mhd), "dust": bool(p.use_dustfrac),
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "breakdown_ms": {"density_call": 1e3 * phase["density"] / args.steps,
+                         "cons2prim_call": 1e3 * phase["cons2prim"] / args.steps,
+                         "force_call": 1e3 * phase["force"] / args.steps,
+                         "density_kernel": 1e3 * t_dens, "force_kernel": 1e3 * t_force,
+                         "tree_build_first_call": tree_ms},
+        "kernel_tflops": {"density": round(dens_tf, 3), "force": round(force_tf, 3)},
+        "neighbours": {"dens_mean": stats["actualmean"], "dens_max": stats["maxactual"],
+                       "force_pairs_per_particle": npairs / n, "nrhocalc": stats["nrhocalc"]},
+        "tree_kernels_ms": {nm: round(ms, 4) for nm, ms in tree_kernels[:3]},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_baseline(setup, args):
+    """The oracle (a C restatement of the reference algorithm) on the host
+    cores of the same machine, same particles, density + cons2prim + force."""
+    from oracle import oracle
+
+    threads = args.cpu_threads or min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    p = setup["params"]
+    o = oracle.Oracle(p)
+    xyzh = setup["xyzh"].copy()
+    o.maketree(xyzh)
+    alphaind = setup["alphaind"].copy()
+    t0 = time.perf_counter()
+    d = o.densityiterate(xyzh, setup["vxyzu"], Bevol=setup.get("Bevol"), alphaind=alphaind)
+    c = o.cons2prim(xyzh, setup["vxyzu"], d["dvdx"], alphaind, dustevol=setup.get("dustevol"))
+    o.force(xyzh, setup["vxyzu"], d["gradh"], alphaind, c["eos_vars"], divcurlB=d["divcurlB"],
+            Bevol=setup.get("Bevol"), dustfrac=c["dustfrac"])
+    dt = time.perf_counter() - t0
+    n = xyzh.shape[0]
+    log(f"cpu baseline: {dt:.2f} s on {threads} threads")
+    return {"value": n / dt, "unit": "particle-updates/s", "cores": threads, "kind": "port",
+            "sample": f"one density+cons2prim+force call over all {n} particles of the same workload "
+                      f"(OpenMP, {threads} threads)"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -227,6 +227,12 @@ int phsph_get_particle_stats(phsph_ctx *ctx, int32_t np, int32_t *nneigh,
                              int32_t *nits);
 
 /*
+ * Number of interacting pairs (j with q_i < R or q_j < R, force.F90:1287)
+ * summed over the active particles of the last force call.
+ */
+int phsph_get_force_stats(phsph_ctx *ctx, int64_t *npairs);
+
+/*
  * Kernel timing of the last call on the context's stream, measured with
  * HIP events around each launch: up to `cap` (name, milliseconds) entries
  * are written; returns the number of entries available.

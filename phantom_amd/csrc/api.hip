@@ -419,6 +419,12 @@ int phsph_get_particle_stats(phsph_ctx *c, int32_t np, int32_t *nneigh, int32_t 
   return rc ? rc : rc2;
 }
 
+int phsph_get_force_stats(phsph_ctx *c, int64_t *npairs) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (npairs) *npairs = c->force_npairs;
+  return PHSPH_OK;
+}
+
 int phsph_get_timings(phsph_ctx *c, const char **names, double *ms, int32_t cap) {
   if (!c) return PHSPH_ERR_INPUT;
   hipSetDevice(c->device);

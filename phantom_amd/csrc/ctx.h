@@ -118,6 +118,7 @@ struct phsph_ctx {
   double trialmean = -1, actualmean = -1;
   int maxtrial = -1, maxactual = -1;
   int64_t nrhocalc = -1, nactualtot = -1;
+  int64_t force_npairs = -1;  // interacting pairs of the last force call
 
   // staging buffers for host arrays
   DBuf<char> stage[16];

@@ -110,7 +110,7 @@ __device__ __forceinline__ bool force_one(
     const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi, const int32_t *__restrict__ iparts,
     const JTerms &J, const double *__restrict__ h, const double *__restrict__ cs, const double *__restrict__ Bex,
     const double *__restrict__ Bey, const double *__restrict__ Bez, uint32_t *__restrict__ nlist, int capl,
-    const ForceIOArgs &out, int32_t *__restrict__ flags, double &dtc_out, double &dtf_out) {
+    const ForceIOArgs &out, int32_t *__restrict__ flags, double &dtc_out, double &dtf_out, int &nl_out) {
   const int p = act[t];
   const int l = pleaf[p];
   const double pm = dc.pmass;
@@ -184,6 +184,7 @@ __device__ __forceinline__ bool force_one(
       }
     }
   }
+  nl_out = nl;
   if (nl > capl) {
     atomicOr(flags, FFL_LIST);
     return false;
@@ -396,16 +397,21 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *_
         const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi, const int32_t *__restrict__ iparts,
         JTerms J, const double *__restrict__ h, const double *__restrict__ cs, const double *__restrict__ Bex,
         const double *__restrict__ Bey, const double *__restrict__ Bez, uint32_t *__restrict__ nlist, int capl,
-        ForceIOArgs out, unsigned long long *__restrict__ dtred, int32_t *__restrict__ flags) {
+        ForceIOArgs out, unsigned long long *__restrict__ dtred, int32_t *__restrict__ flags,
+        unsigned long long *__restrict__ npairs) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   double dtc = PH_HUGE64, dtfm = PH_HUGE64;
+  int nl = 0;
   if (t < nact) {
     if (!force_one<K, MHD, DUST, THERMAL>(dc, t, act, nact, pleaf, cand, ncand, capc, nd_lo, nd_hi, iparts, J, h,
-                                          cs, Bex, Bey, Bez, nlist, capl, out, flags, dtc, dtfm)) {
+                                          cs, Bex, Bey, Bez, nlist, capl, out, flags, dtc, dtfm, nl)) {
       dtc = PH_HUGE64;
       dtfm = PH_HUGE64;
     }
   }
+  unsigned long long np64 = (unsigned long long)nl;
+  for (int off = 32; off > 0; off >>= 1) np64 += __shfl_xor(np64, off, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(npairs, np64);
   // global minima (force.F90:3312-3327): positive doubles order as integers;
   // every lane of the wave takes part in the shuffles
   unsigned long long a = __double_as_longlong(dtc), b = __double_as_longlong(dtfm);
@@ -559,13 +565,14 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
   JTerms J{c->tx.p, c->ty.p, c->tz.p, B.hinv.p, c->tvx.p, c->tvy.p, c->tvz.p, B.u.p, c->tgradh.p, c->talpha.p,
            B.rho.p, B.rho1.p, B.pr.p, B.pro2.p, B.vwave.p, B.Bx.p, B.By.p, B.Bz.p, B.psi.p,
            B.s[0].p, B.s[1].p, B.s[2].p, B.s[3].p, B.s[4].p, B.s[5].p, B.eps.p, B.sqeps.p, B.ts.p};
-  PH_HIPCHECK(c, c->dtred.ensure(2));
+  PH_HIPCHECK(c, c->dtred.ensure(3));
   for (int attempt = 0; attempt < 6; attempt++) {
     PH_HIPCHECK(c, c->nlist.ensure((size_t)c->caps.nlist * (nact + 1)));
-    unsigned long long init[2];
+    unsigned long long init[3];
     double big = PH_HUGE64;
     memcpy(&init[0], &big, 8);
     init[1] = init[0];
+    init[2] = 0ull;
     PH_HIPCHECK(c, hipMemcpyAsync(c->dtred.p, init, sizeof(init), hipMemcpyHostToDevice, s));
     int32_t zero = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
@@ -576,7 +583,8 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
 #define FL(KK, M, D, T)                                                                                          \
   hipLaunchKernelGGL((k_force<KK, M, D, T>), g, b, 0, s, c->dc, c->act.p, nact, c->pleaf.p, c->cand.p, c->ncand.p,  \
                      c->caps.cand, c->nd_lo.p, c->nd_hi.p, c->iparts.p, J, B.h.p, B.cs.p, B.Bex.p, B.Bey.p,         \
-                     B.Bez.p, c->nlist.p, c->caps.nlist, a, c->dtred.p, c->counters.p)
+                     B.Bez.p, c->nlist.p, c->caps.nlist, a, c->dtred.p, c->counters.p,                    \
+                     (unsigned long long *)c->dtred.p + 2)
 #define FLK(KK)                                          \
   if (mhd && dust) { if (th) FL(KK, true, true, true); else FL(KK, true, true, false); }     \
   else if (mhd) { if (th) FL(KK, true, false, true); else FL(KK, true, false, false); }      \
@@ -588,7 +596,7 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
     }
     PH_HIPCHECK(c, hipGetLastError());
     int32_t fl = 0;
-    unsigned long long red[2];
+    unsigned long long red[3];
     PH_HIPCHECK(c, hipMemcpyAsync(&fl, c->counters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     PH_HIPCHECK(c, hipMemcpyAsync(red, c->dtred.p, sizeof(red), hipMemcpyDeviceToHost, s));
     PH_HIPCHECK(c, hipStreamSynchronize(s));
@@ -600,6 +608,7 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
     double dtc, dtf;
     memcpy(&dtc, &red[0], 8);
     memcpy(&dtf, &red[1], 8);
+    c->force_npairs = (int64_t)red[2];
     if (dtc_out) *dtc_out = (nact > 0) ? dtc : PH_BIGNUMBER;
     if (dtf_out) *dtf_out = (nact > 0) ? fmin(dtf, PH_BIGNUMBER) : PH_BIGNUMBER;
     return PHSPH_OK;

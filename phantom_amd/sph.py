@@ -10,8 +10,8 @@ reference's names and argument meaning:
     force(icall, xyzh, vxyzu, ...)             force.F90:193
     get_neighbour_stats()                      dens.F90:1137
 
-Arrays are numpy arrays (host memory) or torch tensors on the GPU, in the
-reference layout: Fortran a(ncomp, n) is a C-contiguous array of shape
+Arrays are numpy arrays (host memory) or phantom_amd.device.DeviceArray
+(HBM), in the reference layout: Fortran a(ncomp, n) is a C-contiguous array of shape
 (n, ncomp).  Errors raise PhsphError carrying the C status code, as the
 reference would call fatal().  The library must be built (python -m
 phantom_amd.build); there is no CPU fallback.
@@ -29,7 +29,7 @@ LIB_PATH = os.path.join(_HERE, "libphsph.so")
 EXPORTED = [
     "phsph_default_params", "phsph_create", "phsph_destroy", "phsph_set_params", "phsph_last_error",
     "phsph_maketree", "phsph_get_tree", "phsph_densityiterate", "phsph_cons2prim", "phsph_force",
-    "phsph_get_neighbour_stats", "phsph_get_particle_stats", "phsph_get_timings",
+    "phsph_get_neighbour_stats", "phsph_get_particle_stats", "phsph_get_force_stats", "phsph_get_timings",
 ]
 
 _lib = None
@@ -67,6 +67,7 @@ def load_library(path=LIB_PATH):
                               i32, vp, vp, i32, vp, P(dbl), P(dbl)]
     L.phsph_get_neighbour_stats.argtypes = [vp, P(dbl), P(dbl), P(i32), P(i32), P(i64), P(i64)]
     L.phsph_get_particle_stats.argtypes = [vp, i32, vp, vp]
+    L.phsph_get_force_stats.argtypes = [vp, P(i64)]
     L.phsph_get_timings.argtypes = [vp, P(C.c_char_p), P(dbl), i32]
     for name in EXPORTED:
         if name != "phsph_default_params" and name != "phsph_last_error":
@@ -76,7 +77,8 @@ def load_library(path=LIB_PATH):
 
 
 def _ptr(a):
-    """(pointer, leading dimension) of a numpy array or torch tensor."""
+    """(pointer, leading dimension) of a numpy array (host memory) or a
+    phantom_amd.device.DeviceArray (HBM)."""
     if a is None:
         return None, 0
     if isinstance(a, np.ndarray):
@@ -84,20 +86,19 @@ def _ptr(a):
             raise ValueError("arrays must be C-contiguous (Fortran column-major a(ncomp,n))")
         ld = a.shape[1] if a.ndim == 2 else 1
         return C.c_void_p(a.ctypes.data), ld
-    # torch tensor
-    if not a.is_contiguous():
-        raise ValueError("tensors must be contiguous")
-    ld = a.shape[1] if a.dim() == 2 else 1
-    return C.c_void_p(a.data_ptr()), ld
+    ptr = getattr(a, "ptr", None)
+    if ptr is None:
+        raise TypeError(f"unsupported array type {type(a)}")
+    ld = a.shape[1] if len(a.shape) == 2 else 1
+    return C.c_void_p(ptr), ld
 
 
 def _zeros_like_layout(ref, shape, dtype):
     """Allocate an output array on the same side (host/device) as ``ref``."""
     if isinstance(ref, np.ndarray):
         return np.zeros(shape, dtype=dtype)
-    import torch
-    tdt = {np.float32: torch.float32, np.float64: torch.float64, np.int32: torch.int32}[dtype]
-    return torch.zeros(shape, dtype=tdt, device=ref.device)
+    from .device import DeviceArray
+    return DeviceArray(shape, dtype)
 
 
 class Context:
@@ -150,7 +151,7 @@ class Context:
                                             C.byref(ncells)))
         self.np = n
         self._ncells = ncells.value
-        h = xyzh[:, 3]
+        h = xyzh[:, 3] if isinstance(xyzh, np.ndarray) else xyzh.numpy()[:, 3]
         self._nlive = int((h >= 2.2250738585072014e-308).sum())
         return leaf[:ncells.value], ncells.value
 
@@ -283,6 +284,11 @@ class Context:
                                          C.byref(dtf)))
         return dict(fxyzu=fxyzu, divcurlv=divcurlv, dBevol=dBevol, divBsymm=divBsymm, ddustevol=ddustevol,
                     deltav=deltav, tstop=tstop, dtcourant=dtc.value, dtforce=dtf.value)
+
+    def get_force_stats(self):
+        n = C.c_int64()
+        self._check(self.lib.phsph_get_force_stats(self._h, C.byref(n)))
+        return dict(npairs=n.value)
 
     def timings(self):
         """Kernel times (ms) of the last call, measured with HIP events."""

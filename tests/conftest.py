@@ -13,9 +13,10 @@ def pytest_configure(config):
 
 
 def gpu_available():
+    """A HIP device is visible to the runtime the library links."""
     try:
-        import torch
-        return torch.cuda.is_available()
+        from phantom_amd import device
+        return device.device_count() > 0
     except Exception:
         return False
 
