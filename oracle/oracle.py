@@ -51,6 +51,7 @@ def lib():
                                 vp, vp, vp, vp, vp, vp, vp, vp, C.c_int32, vp, vp, C.c_int32, vp,
                                 P(C.c_double), P(C.c_double)]
         L.orc_kernel.argtypes = [C.c_int32, C.c_double, P(C.c_double), P(C.c_double), P(C.c_double)]
+        L.orc_kernel_softening.argtypes = [C.c_int32, C.c_double, P(C.c_double), P(C.c_double)]
         _lib = L
     return _lib
 
@@ -69,6 +70,12 @@ def kernel(kind, q):
     w, g, d = C.c_double(), C.c_double(), C.c_double()
     lib().orc_kernel(kind, q, C.byref(w), C.byref(g), C.byref(d))
     return w.value, g.value, d.value
+
+
+def kernel_softening(kind, q):
+    pot, fs = C.c_double(), C.c_double()
+    lib().orc_kernel_softening(kind, q, C.byref(pot), C.byref(fs))
+    return pot.value, fs.value
 
 
 class Oracle:

@@ -1429,6 +1429,47 @@ int orc_force(orc_ctx *o, int32_t icall, int32_t np, const double *xyzh, const d
   return rc_all;
 }
 
+/* kernel_softening (kernel_cubic.f90:103-121, kernel_quintic.f90:124-152):
+ * softened potential and force of a unit mass at q = r/h */
+static void kernel_softening(int kind, double q2, double q, double *pot, double *fs) {
+  if (kind == PHSPH_KERNEL_CUBIC) {
+    if (q < 1.) {
+      double q4 = q2 * q2;
+      *pot = q4 * q / 10. - 3. * q4 / 10. + 2. * q2 / 3. - 7. / 5.;
+      *fs = q * (15. * q2 * q - 36. * q2 + 40.) / 30.;
+    } else if (q < 2.) {
+      double q4 = q2 * q2, q6 = q4 * q2;
+      *pot = (q * (-q4 * q + 9. * q4 - 30. * q2 * q + 40. * q2 - 48.) + 2.) / (30. * q);
+      *fs = (-5. * q6 + 36. * q4 * q - 90. * q4 + 80. * q2 * q - 2.) / (30. * q2);
+    } else {
+      *pot = -1. / q;
+      *fs = 1. / q2;
+    }
+  } else {
+    if (q < 1.) {
+      double q4 = q2 * q2, q6 = q4 * q2;
+      *pot = -q6 * q / 168. + q6 / 42. - q4 / 10. + 11. * q2 / 30. - 239. / 210.;
+      *fs = q * (-35. * q4 * q + 120. * q4 - 336. * q2 + 616.) / 840.;
+    } else if (q < 2.) {
+      double q4 = q2 * q2, q6 = q4 * q2, q8 = q6 * q2;
+      *pot = (q * (5. * q6 * q - 60. * q6 + 280. * q4 * q - 588. * q4 + 350. * q2 * q + 476. * q2 - 1892.) - 5.) /
+             (1680. * q);
+      *fs = (35. * q8 - 360. * q6 * q + 1400. * q6 - 2352. * q4 * q + 1050. * q4 + 952. * q2 * q + 5.) / (1680. * q2);
+    } else if (q < 3.) {
+      double q4 = q2 * q2, q6 = q4 * q2, q8 = q6 * q2;
+      *pot = (q * (-q6 * q + 20. * q6 - 168. * q4 * q + 756. * q4 - 1890. * q2 * q + 2268. * q2 - 2916.) + 507.) /
+             (1680. * q);
+      *fs = (-7. * q8 + 120. * q6 * q - 840. * q6 + 3024. * q4 * q - 5670. * q4 + 4536. * q2 * q - 507.) /
+            (1680. * q2);
+    } else {
+      *pot = -1. / q;
+      *fs = 1. / q2;
+    }
+  }
+}
+
+void orc_kernel_softening(int32_t kind, double q, double *pot, double *fs) { kernel_softening(kind, q * q, q, pot, fs); }
+
 /* Kernel value helpers exported for the kernel identity tests. */
 void orc_kernel(int32_t kind, double q, double *w, double *g, double *dphidh) {
   get_kernel(kind, q * q, q, w, g);

@@ -1,0 +1,67 @@
+"""The C-ABI library loads, exports every symbol include/phsph.h declares,
+and its host-side logic (defaults, argument validation) works without a GPU.
+No compute entry point is called here."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from phantom_amd import abi, sph
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(ROOT, "include", "phsph.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(phsph_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_declared_symbols():
+    L = sph.load_library()
+    names = declared_functions()
+    assert len(names) >= 12
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) == set(sph.EXPORTED)
+
+
+def test_params_struct_matches_header():
+    # every field of phsph_params, in order
+    txt = open(os.path.join(ROOT, "include", "phsph.h")).read()
+    body = txt[txt.index("typedef struct phsph_params"):txt.index("} phsph_params;")]
+    fields = []
+    for line in body.splitlines()[1:]:
+        line = re.sub(r"/\*.*?\*/", "", line).strip()
+        m = re.match(r"(int32_t|double)\s+(.*);", line)
+        if m:
+            fields += [f.strip() for f in m.group(2).split(",")]
+    assert fields == [n for n, _ in abi.Params._fields_]
+    assert C.sizeof(abi.Params) == 16 * 4 + 23 * 8
+
+
+@pytest.mark.parametrize("kernel", [abi.KERNEL_CUBIC, abi.KERNEL_QUINTIC])
+def test_default_params_agree(kernel):
+    L = sph.load_library()
+    c = abi.Params()
+    L.phsph_default_params(C.byref(c), kernel)
+    py = abi.default_params(kernel)
+    assert abi.params_to_dict(c) == abi.params_to_dict(py)
+
+
+@pytest.mark.parametrize("field,value", [("kernel", 7), ("ieos", 3), ("massoftype_gas", 0.0), ("gravity", 1),
+                                         ("maxvxyzu", 5), ("tolh", -1.0)])
+def test_invalid_params_rejected(field, value):
+    p = abi.default_params()
+    setattr(p, field, value)
+    with pytest.raises(sph.PhsphError) as e:
+        sph.Context(p)
+    assert e.value.code == abi.ERR_INPUT
+
+
+def test_oracle_library_loads():
+    from oracle import oracle
+    L = oracle.lib()
+    for n in ("orc_create", "orc_maketree", "orc_densityiterate", "orc_cons2prim", "orc_force"):
+        assert hasattr(L, n)
