@@ -21,6 +21,10 @@ FLAGS = [
 ]
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FC = os.environ.get("FC_PHSPH", "/opt/rocm/bin/amdflang")
+
+
 def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
@@ -29,7 +33,31 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
+
+
+def build_fortran(verbose=False):
+    """The iso_c_binding module and the Fortran driver that calls the
+    library the way Phantom's deriv.f90 does (skipped without amdflang)."""
+    if not os.path.exists(FC):
+        if verbose:
+            print("amdflang not found: Fortran interface not built", flush=True)
+        return None
+    mod = os.path.join(FORTRAN, "phsph.F90")
+    drv = os.path.join(FORTRAN, "shim_driver.f90")
+    obj = os.path.join(FORTRAN, "phsph.o")
+    exe = os.path.join(FORTRAN, "shim_driver")
+    cmds = [
+        [FC, "-O2", "-c", mod, "-J", FORTRAN, "-o", obj],
+        [FC, "-O2", "-I", FORTRAN, "-J", FORTRAN, drv, obj, "-L", HERE, "-lphsph",
+         "-Wl,-rpath,$ORIGIN/..", "-o", exe],
+    ]
+    for c in cmds:
+        if verbose:
+            print(" ".join(c), flush=True)
+        subprocess.run(c, check=True)
+    return exe
 
 
 if __name__ == "__main__":

@@ -1,0 +1,226 @@
+!--------------------------------------------------------------------------!
+! phsph: Fortran 2008 iso_c_binding interface to libphsph.so, the MI355X  !
+! implementation of the SPH hot path.  The wrappers keep the argument     !
+! lists of the reference routines they stand in for:                      !
+!   phsph_maketree       -> maketree       (src/main/kdtree.F90:117)      !
+!   phsph_densityiterate -> densityiterate (src/main/dens.F90:117)        !
+!   phsph_cons2prim      -> cons2prim_everything (cons2prim.f90:274)      !
+!   phsph_force          -> force          (src/main/force.F90:193)       !
+! Implicit module inputs of the reference (massoftype, hfact, tolh, the   !
+! shock-capturing and timestep options, the box) travel in phsph_params, !
+! set once with phsph_init.  A non-zero status from the library is passed !
+! to the caller's error handler (in Phantom: fatal()).                    !
+!--------------------------------------------------------------------------!
+module phsph
+ use, intrinsic :: iso_c_binding
+ implicit none
+ private
+
+ integer, parameter, public :: PHSPH_KERNEL_CUBIC = 0, PHSPH_KERNEL_QUINTIC = 1
+ integer, parameter, public :: PHSPH_PREC_FP64 = 0, PHSPH_PREC_MIXED = 1
+
+ ! layout of phsph_params (include/phsph.h)
+ type, bind(c), public :: phsph_params
+    integer(c_int32_t) :: kernel, periodic, maxvxyzu, mhd, use_dustfrac, ndustsmall
+    integer(c_int32_t) :: gravity, ind_timesteps, ieos, maxp, precision, idrag
+    integer(c_int32_t) :: ishock_heating, ipdv_heating, iresistive_heating, ilimitdustflux
+    real(c_double)     :: xmin, xmax, ymin, ymax, zmin, zmax
+    real(c_double)     :: hfact, tolh, massoftype_gas, gamma, polyk
+    real(c_double)     :: alpha, alphamax, alphau, alphaB, beta
+    real(c_double)     :: C_cour, C_force, dtmax, psidecayfac, overcleanfac, K_code
+    real(c_double)     :: tree_accuracy
+ end type phsph_params
+
+ ! type(kdnode) of a non-GRAVITY build (dtype_kdtree.F90:53-68)
+ type, bind(c), public :: phsph_kdnode
+    real(c_double)     :: xcen(3), size, hmax, mass
+    integer(c_int32_t) :: leftchild, rightchild, parent, idum
+ end type phsph_kdnode
+
+ type(c_ptr), save :: ctx = c_null_ptr
+
+ public :: phsph_default, phsph_init, phsph_finalise, phsph_errmsg
+ public :: phsph_maketree, phsph_densityiterate, phsph_cons2prim, phsph_force
+ public :: phsph_neighbour_stats
+
+ interface
+    subroutine c_default_params(p, kernel) bind(c, name='phsph_default_params')
+       import :: phsph_params, c_int32_t
+       type(phsph_params), intent(out) :: p
+       integer(c_int32_t), value :: kernel
+    end subroutine
+    integer(c_int) function c_create(p, device, ctx) bind(c, name='phsph_create')
+       import :: phsph_params, c_int, c_int32_t, c_ptr
+       type(phsph_params), intent(in) :: p
+       integer(c_int32_t), value :: device
+       type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function c_destroy(ctx) bind(c, name='phsph_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+    end function
+    type(c_ptr) function c_last_error(ctx) bind(c, name='phsph_last_error')
+       import :: c_ptr
+       type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function c_maketree(ctx, xyzh, ld, np, iphase, leaf, ncells) bind(c, name='phsph_maketree')
+       import :: c_int, c_int32_t, c_int64_t, c_ptr
+       type(c_ptr), value :: ctx, xyzh, iphase, leaf
+       integer(c_int32_t), value :: ld, np
+       integer(c_int64_t), intent(out) :: ncells
+    end function
+    integer(c_int) function c_densityiterate(ctx, icall, np, xyzh, ldx, vxyzu, ldv, fxyzu, ldf, fext, Bevol, &
+                                             iphase, divcurlv, lddcv, divcurlB, alphaind, gradh, ldgradh, dvdx) &
+                                             bind(c, name='phsph_densityiterate')
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx, xyzh, vxyzu, fxyzu, fext, Bevol, iphase, divcurlv, divcurlB, alphaind, gradh, dvdx
+       integer(c_int32_t), value :: icall, np, ldx, ldv, ldf, lddcv, ldgradh
+    end function
+    integer(c_int) function c_cons2prim(ctx, np, xyzh, ldx, vxyzu, ldv, dvdx, alphaind, dustevol, dustfrac, &
+                                        eos_vars, ldeos, iphase) bind(c, name='phsph_cons2prim')
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx, xyzh, vxyzu, dvdx, alphaind, dustevol, dustfrac, eos_vars, iphase
+       integer(c_int32_t), value :: np, ldx, ldv, ldeos
+    end function
+    integer(c_int) function c_force(ctx, icall, np, xyzh, ldx, vxyzu, ldv, fxyzu, ldf, divcurlv, lddcv, divcurlB, &
+                                    Bevol, dBevol, divBsymm, dustfrac, ddustevol, deltav, tstop, gradh, ldgradh, &
+                                    alphaind, eos_vars, ldeos, iphase, dtcourant, dtforce) bind(c, name='phsph_force')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx, xyzh, vxyzu, fxyzu, divcurlv, divcurlB, Bevol, dBevol, divBsymm, dustfrac, &
+                             ddustevol, deltav, tstop, gradh, alphaind, eos_vars, iphase
+       integer(c_int32_t), value :: icall, np, ldx, ldv, ldf, lddcv, ldgradh, ldeos
+       real(c_double), intent(out) :: dtcourant, dtforce
+    end function
+    integer(c_int) function c_neighbour_stats(ctx, trialmean, actualmean, maxtrial, maxactual, nrhocalc, &
+                                              nactualtot) bind(c, name='phsph_get_neighbour_stats')
+       import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: trialmean, actualmean
+       integer(c_int32_t), intent(out) :: maxtrial, maxactual
+       integer(c_int64_t), intent(out) :: nrhocalc, nactualtot
+    end function
+ end interface
+
+contains
+
+!-- reference defaults for a kernel (options.f90, shock_capturing.f90, timestep.f90)
+function phsph_default(kernel) result(p)
+ integer, intent(in) :: kernel
+ type(phsph_params)  :: p
+ call c_default_params(p, int(kernel, c_int32_t))
+end function phsph_default
+
+integer function phsph_init(p, device) result(ierr)
+ type(phsph_params), intent(in) :: p
+ integer, intent(in), optional  :: device
+ integer(c_int32_t) :: dev
+ dev = 0
+ if (present(device)) dev = int(device, c_int32_t)
+ ierr = c_create(p, dev, ctx)
+end function phsph_init
+
+subroutine phsph_finalise()
+ integer(c_int) :: ierr
+ if (c_associated(ctx)) ierr = c_destroy(ctx)
+ ctx = c_null_ptr
+end subroutine phsph_finalise
+
+function phsph_errmsg() result(msg)
+ character(len=256) :: msg
+ character(kind=c_char), pointer :: s(:)
+ type(c_ptr) :: p
+ integer :: i
+ msg = ' '
+ p = c_last_error(ctx)
+ if (.not.c_associated(p)) return
+ call c_f_pointer(p, s, [256])
+ do i=1,256
+    if (s(i) == c_null_char) exit
+    msg(i:i) = s(i)
+ enddo
+end function phsph_errmsg
+
+!-- maketree(node,xyzh,np,leaf_is_active,ncells) without the node array,
+!   which stays on the device (phsph_get_tree mirrors it when needed)
+integer function phsph_maketree(xyzh, np, iphase, leaf_is_active, ncells) result(ierr)
+ real(c_double),     intent(inout), target :: xyzh(:,:)
+ integer,            intent(in)            :: np
+ integer(c_int8_t),  intent(in),    target :: iphase(:)
+ integer(c_int32_t), intent(out),   target :: leaf_is_active(:)
+ integer(kind=8),    intent(out)           :: ncells
+ integer(c_int64_t) :: nc
+ ierr = c_maketree(ctx, c_loc(xyzh), int(size(xyzh,1), c_int32_t), int(np, c_int32_t), c_loc(iphase), &
+                   c_loc(leaf_is_active), nc)
+ ncells = nc
+end function phsph_maketree
+
+!-- densityiterate(icall,npart,nactive,xyzh,vxyzu,divcurlv,divcurlB,Bevol,
+!   stressmax,fxyzu,fext,alphaind,gradh,rad,radprop,dvdx) for hydro/MHD
+integer function phsph_densityiterate(icall, npart, xyzh, vxyzu, divcurlv, divcurlB, Bevol, fxyzu, fext, &
+                                      alphaind, gradh, dvdx, iphase) result(ierr)
+ integer,           intent(in)            :: icall, npart
+ real(c_double),    intent(inout), target :: xyzh(:,:)
+ real(c_double),    intent(in),    target :: vxyzu(:,:), fxyzu(:,:), fext(:,:), Bevol(:,:)
+ real(c_float),     intent(inout), target :: divcurlv(:,:), divcurlB(:,:), alphaind(:,:), gradh(:,:), dvdx(:,:)
+ integer(c_int8_t), intent(in),    target :: iphase(:)
+ type(c_ptr) :: pB, pdB
+ pB = c_null_ptr
+ pdB = c_null_ptr
+ if (size(Bevol) > 0) pB = c_loc(Bevol)
+ if (size(divcurlB) > 0) pdB = c_loc(divcurlB)
+ ierr = c_densityiterate(ctx, int(icall, c_int32_t), int(npart, c_int32_t), c_loc(xyzh), &
+                         int(size(xyzh,1), c_int32_t), c_loc(vxyzu), int(size(vxyzu,1), c_int32_t), &
+                         c_loc(fxyzu), int(size(fxyzu,1), c_int32_t), c_loc(fext), pB, c_loc(iphase), &
+                         c_loc(divcurlv), int(size(divcurlv,1), c_int32_t), pdB, c_loc(alphaind), &
+                         c_loc(gradh), int(size(gradh,1), c_int32_t), c_loc(dvdx))
+end function phsph_densityiterate
+
+!-- cons2prim_everything for ieos 1 and 2 (eos_vars(1:2,:) = P, c_s)
+integer function phsph_cons2prim(npart, xyzh, vxyzu, dvdx, alphaind, eos_vars) result(ierr)
+ integer,        intent(in)            :: npart
+ real(c_double), intent(in),    target :: xyzh(:,:), vxyzu(:,:)
+ real(c_float),  intent(in),    target :: dvdx(:,:)
+ real(c_float),  intent(inout), target :: alphaind(:,:)
+ real(c_double), intent(inout), target :: eos_vars(:,:)
+ ierr = c_cons2prim(ctx, int(npart, c_int32_t), c_loc(xyzh), int(size(xyzh,1), c_int32_t), c_loc(vxyzu), &
+                    int(size(vxyzu,1), c_int32_t), c_loc(dvdx), c_loc(alphaind), c_null_ptr, c_null_ptr, &
+                    c_loc(eos_vars), int(size(eos_vars,1), c_int32_t), c_null_ptr)
+end function phsph_cons2prim
+
+!-- force(icall,npart,xyzh,vxyzu,fxyzu,divcurlv,divcurlB,Bevol,dBevol,...)
+!   for hydro (+ MHD when the arrays are non-empty); global timestep
+integer function phsph_force(icall, npart, xyzh, vxyzu, fxyzu, divcurlv, divcurlB, Bevol, dBevol, divBsymm, &
+                             gradh, alphaind, eos_vars, iphase, dtcourant, dtforce) result(ierr)
+ integer,           intent(in)            :: icall, npart
+ real(c_double),    intent(in),    target :: xyzh(:,:), vxyzu(:,:), Bevol(:,:), eos_vars(:,:)
+ real(c_double),    intent(inout), target :: fxyzu(:,:), dBevol(:,:)
+ real(c_float),     intent(inout), target :: divcurlv(:,:), divBsymm(:)
+ real(c_float),     intent(in),    target :: divcurlB(:,:), gradh(:,:), alphaind(:,:)
+ integer(c_int8_t), intent(in),    target :: iphase(:)
+ real(c_double),    intent(out)           :: dtcourant, dtforce
+ type(c_ptr) :: pB, pdB, pdBe, pdBs
+ pB = c_null_ptr; pdB = c_null_ptr; pdBe = c_null_ptr; pdBs = c_null_ptr
+ if (size(Bevol) > 0) then
+    pB = c_loc(Bevol); pdB = c_loc(divcurlB); pdBe = c_loc(dBevol); pdBs = c_loc(divBsymm)
+ endif
+ ierr = c_force(ctx, int(icall, c_int32_t), int(npart, c_int32_t), c_loc(xyzh), int(size(xyzh,1), c_int32_t), &
+                c_loc(vxyzu), int(size(vxyzu,1), c_int32_t), c_loc(fxyzu), int(size(fxyzu,1), c_int32_t), &
+                c_loc(divcurlv), int(size(divcurlv,1), c_int32_t), pdB, pB, pdBe, pdBs, c_null_ptr, c_null_ptr, &
+                c_null_ptr, c_null_ptr, c_loc(gradh), int(size(gradh,1), c_int32_t), c_loc(alphaind), &
+                c_loc(eos_vars), int(size(eos_vars,1), c_int32_t), c_loc(iphase), dtcourant, dtforce)
+end function phsph_force
+
+!-- get_neighbour_stats (dens.F90:1137)
+subroutine phsph_neighbour_stats(trialmean, actualmean, maxtrial, maxactual, nrhocalc, nactualtot)
+ real,            intent(out) :: trialmean, actualmean
+ integer,         intent(out) :: maxtrial, maxactual
+ integer(kind=8), intent(out) :: nrhocalc, nactualtot
+ real(c_double)     :: tm, am
+ integer(c_int32_t) :: mt, ma
+ integer(c_int64_t) :: nr, na
+ integer(c_int)     :: ierr
+ ierr = c_neighbour_stats(ctx, tm, am, mt, ma, nr, na)
+ trialmean = tm; actualmean = am; maxtrial = mt; maxactual = ma; nrhocalc = nr; nactualtot = na
+end subroutine phsph_neighbour_stats
+
+end module phsph
