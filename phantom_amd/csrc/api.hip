@@ -390,6 +390,11 @@ int phsph_force(phsph_ctx *c, int32_t icall, int32_t np, const double *xyzh, int
 int phsph_get_neighbour_stats(phsph_ctx *c, double *trialmean, double *actualmean, int32_t *maxtrial,
                               int32_t *maxactual, int64_t *nrhocalc, int64_t *nactualtot) {
   if (!c) return PHSPH_ERR_INPUT;
+  if ((trialmean || maxtrial) && c->tree_valid && c->nrhocalc >= 0) {
+    PH_HIPCHECK(c, hipSetDevice(c->device));
+    int rc = dens_trial_stats(c);
+    if (rc) return rc;
+  }
   if (trialmean) *trialmean = c->trialmean;
   if (actualmean) *actualmean = c->actualmean;
   if (maxtrial) *maxtrial = c->maxtrial;

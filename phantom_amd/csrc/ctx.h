@@ -43,22 +43,15 @@ struct TimedLaunch {
   hipEvent_t t0, t1;
 };
 
-// Capacities of the per-leaf candidate lists and per-particle neighbour
+// Capacities of the per-group candidate lists and per-particle neighbour
 // lists.  They grow (and the pass is re-run) when a launch reports overflow.
 struct Caps {
-  int cand = 192;    // candidate leaves per leaf
-  int nlist = 256;   // neighbours per particle
-};
-
-// per-particle quantities of the force loop (tree order)
-struct ForceBufs {
-  DBuf<double> hinv, u, rho, rho1, pr, pro2, vwave, Bx, By, Bz, psi, s[6], eps, sqeps, ts, cs, h;
-  DBuf<double> Bex, Bey, Bez, divB;
+  int gcand = 512;   // candidate leaves per group of 64 particles
+  int nlist = 256;   // neighbours per particle (a multiple of 8)
 };
 
 struct phsph_ctx {
   phsph_params p;
-  ForceBufs fb;
   DBuf<double> hmaxf;                 // node hmax of the final h (force walk)
   DBuf<unsigned long long> dtred;     // global dt minima
   DevConst dc;
@@ -93,21 +86,23 @@ struct phsph_ctx {
   DBuf<double> dred;         // small device reductions
 
   // ---- neighbour search ----
-  DBuf<uint32_t> cand;       // candidate leaves per leaf [nleaves*caps.cand]
-  DBuf<int32_t> ncand;       // count per leaf
-  DBuf<int32_t> ntrial;      // trial particles per leaf
-  DBuf<uint32_t> nlist;      // neighbour lists [caps.nlist][nact]
-  DBuf<int32_t> nlist_n;
+  DBuf<uint32_t> gcand;      // candidate leaves per group [ngroups*caps.gcand]
+  DBuf<int32_t> gncand;      // count per group
+  int ngroups = 0;
+  DBuf<int32_t> ntrial;      // trial particles per leaf (statistics)
+  DBuf<uint32_t> nlist;      // neighbour lists, blocks of 8 per particle
 
   // ---- tree-ordered particle mirror ----
-  DBuf<double> tvx, tvy, tvz, tu;        // velocities, thermal energy
-  DBuf<double> tfx, tfy, tfz;            // fxyzu + fext (density's div a)
-  DBuf<double> tBx, tBy, tBz, tpsi;      // Bevol
-  DBuf<float> tgradh, talpha;            // gradh(1), alphaind(1)
-  DBuf<double> tpr, tcs, teps;           // eos P, c_s, dust fraction
-  DBuf<int8_t> tphase;                   // iphase
+  DBuf<double4> tp4;                     // density: x, y, z, h at entry
+  DBuf<double4> tv4, tf4, tB4;           // density: v, f + fext, Bevol and rho(h)
+  DBuf<double4> fq4;                     // force: x, y, z, 1/h
+  DBuf<char> frec, frecB, frecD;         // force: packed j records
+  DBuf<double> fh;                       // force: h (tree order)
   DBuf<int32_t> act;                     // active tree positions
   int nact = 0;
+  DBuf<int32_t> act_dens;                // active positions of the last densityiterate
+  int nact_dens = 0;
+  bool trial_stale = true;               // trial statistics not yet computed
 
   // per-particle / per-leaf outputs of density kept for statistics
   DBuf<int32_t> p_nneigh, p_nits, leaf_nits;
@@ -189,7 +184,7 @@ struct DensIOArgs {  // device pointers in the caller's layout
   float *dvdx;
 };
 int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a);
-int walk_leaves(phsph_ctx *c, int mode, double rfac, const double *hmaxn);
+int dens_trial_stats(phsph_ctx *c);
 int compact_active(phsph_ctx *c, const int32_t *flag, int n);
 int dens_particle_stats(phsph_ctx *c, int32_t *d_nneigh, int32_t *d_nits);
 // force.hip

@@ -1,36 +1,39 @@
-// density.hip -- leaf neighbour search and the h-rho density iteration
+// density.hip -- neighbour search and the h-rho density iteration
 // (densityiterate, dens.F90:117-568; get_density_sums :578-857;
 // finish_cell :1382-1466; finish_rhosum :1470-1507; store_results
 // :1511-1681).
 //
 // Work decomposition:
-//  * k_walk: one thread per leaf walks the tree exactly as getneigh does
-//    (kdtree.F90:1221-1347): open a node when r^2 < (size_i+size_j+rcut)^2,
-//    with the minimum-image offset of get_sep (:1468-1501).  The output is
-//    a per-leaf list of candidate leaves with their periodic offsets, in
-//    walk order.  Leaves are contiguous in the tree-ordered particle arrays,
-//    so a candidate leaf is a contiguous range.
-//  * k_density: one thread per active particle.  It first compacts the
-//    candidates within the walk radius into a per-particle neighbour list,
-//    then runs its own Newton-Raphson iterations on rho and Omega only,
-//    and finally makes one pass with every sum (div v, the velocity,
-//    acceleration and exact-linear matrices, div B) at the converged h.
-//    Per-particle iteration gives the reference's result: in a cell the
-//    reference keeps re-evaluating converged particles at their unchanged
-//    h, so each particle's h sequence depends only on its own sums.
+//  * k_group_walk: one wavefront per group of 64 active particles walks the
+//    tree breadth-parallel (64 nodes per step) and lists the candidate leaves
+//    of the whole group (stream.h).
+//  * k_density: one wavefront per group, one particle per lane.  The lanes
+//    pull the group's candidate stream through LDS and keep, each for its own
+//    particle, the candidates within rfac*radkern*h (the list covers one
+//    Newton step of +20%, the reference's clamp).  Each lane then runs its
+//    own Newton-Raphson iterations on rho and Omega only, and makes one final
+//    pass with every sum (div v, the velocity, acceleration and exact-linear
+//    matrices, div B) at the converged h.  Per-particle iteration gives the
+//    reference's result: in a cell the reference keeps re-evaluating
+//    converged particles at their unchanged h, so each particle's h sequence
+//    depends only on its own sums.
+//  * k_walk: the reference's per-leaf walk (getneigh, kdtree.F90:1221-1347),
+//    used only for the trial-neighbour statistics.
 //
 // Every neighbour membership test computes r^2 = ((dx*dx + dy*dy) + dz*dz)
 // and q^2 = r^2 * (1/h)^2 with separately rounded FP64 operations, exactly
 // as the reference, so neighbour counts agree bit for bit.
-#include "ctx.h"
+#include "stream.h"
+
+#include <cstdlib>
 
 enum { DFL_CAND = 1, DFL_LIST = 2, DFL_REWALK = 4, DFL_NOCONV = 8, DFL_NEGH = 16 };
 
 struct DensIn {
-  const double *tx, *ty, *tz, *th_in;
-  const double *vx, *vy, *vz;
-  const double *fx, *fy, *fz;    // fxyzu + fext (null: zero)
-  const double *Bx, *By, *Bz;    // Bevol (MHD)
+  const double4 *tp;  // x, y, z (treecache) and h at entry
+  const double4 *tv;  // vx, vy, vz
+  const double4 *tf;  // fxyzu + fext (null: zero)
+  const double4 *tB;  // Bevol x, y, z and rho(h at entry) (MHD)
 };
 struct DensOut {
   double *xyzh;  int ldx;
@@ -41,75 +44,41 @@ struct DensOut {
   float *dvdx;
 };
 
-__device__ __forceinline__ void offsets_of(const DevConst &dc, int code, double &ox, double &oy, double &oz) {
-  int sx, sy, sz;
-  off_decode(code, sx, sy, sz);
-  ox = (sx == 0) ? 0. : copysign(dc.dxbound, (double)sx);
-  oy = (sy == 0) ? 0. : copysign(dc.dybound, (double)sy);
-  oz = (sz == 0) ? 0. : copysign(dc.dzbound, (double)sz);
-}
-
-// separation i - j with the candidate offset and minimum image
-// (dens.F90:653-673)
-__device__ __forceinline__ double sep2(const DevConst &dc, double xi, double yi, double zi, double xj, double yj,
-                                       double zj, double ox, double oy, double oz, double &dx, double &dy,
-                                       double &dz) {
-  dx = xsub(xi, xadd(xj, ox));
-  dy = xsub(yi, xadd(yj, oy));
-  dz = xsub(zi, xadd(zj, oz));
-  if (dc.periodic) {
-    dx = min_image(dx, dc.dxbound, 0.5 * dc.dxbound);
-    dy = min_image(dy, dc.dybound, 0.5 * dc.dybound);
-    dz = min_image(dz, dc.dzbound, 0.5 * dc.dzbound);
-  }
-  return r2_exact(dx, dy, dz);
-}
-
 // ------------------------------------------------------------------------
-// tree walk per leaf (getneigh)
-// mode 0: rcut = rfac*radkern*hmax_i; mode 1: rcut = max(that, radkern*hmax_j)
+// per-leaf walk of the reference (trial-neighbour statistics only)
 // ------------------------------------------------------------------------
-__global__ void k_walk(DevConst dc, int mode, double rfac, const phsph_kdnode *__restrict__ nd,
-                       const double *__restrict__ hmaxn, const int32_t *__restrict__ leaf_flag,
-                       const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi,
-                       const int32_t *__restrict__ leaves, int nleaves, uint32_t *__restrict__ cand,
-                       int32_t *__restrict__ ncand, int32_t *__restrict__ ntrial, int capc,
-                       int32_t *__restrict__ flags) {
+__global__ void k_walk(DevConst dc, double rfac, const phsph_kdnode *__restrict__ nd,
+                       const int32_t *__restrict__ leaf_flag, const int32_t *__restrict__ nd_lo,
+                       const int32_t *__restrict__ nd_hi, const int32_t *__restrict__ leaves, int nleaves,
+                       int32_t *__restrict__ ntrial, int32_t *__restrict__ flags) {
   int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= nleaves) return;
   const int c = leaves[l];
   if (leaf_flag[c] <= 0) {
-    ncand[l] = 0;
     ntrial[l] = 0;
     return;
   }
   const double xp0 = nd[c].xcen[0], xp1 = nd[c].xcen[1], xp2 = nd[c].xcen[2];
   const double xsizei = nd[c].size;
-  // rcut = radkern*hmax (get_cell_location, neigh_kdtree.f90:363), rounded
-  // on its own before it enters (size_i + size_j + rcut)^2
-  const double rcuti = xmul(xmul(rfac, dc.radkern), hmaxn ? hmaxn[c] : nd[c].hmax);
+  // rcut = radkern*hmax (get_cell_location, neigh_kdtree.f90:363)
+  const double rcuti = xmul(xmul(rfac, dc.radkern), nd[c].hmax);
   int stack[64];
   int ns = 1;
   stack[0] = 1;
-  int k = 0, ntr = 0;
+  int ntr = 0;
   while (ns > 0) {
     const int n = stack[--ns];
     const phsph_kdnode e = nd[n];
     double dx = xsub(xp0, e.xcen[0]), dy = xsub(xp1, e.xcen[1]), dz = xsub(xp2, e.xcen[2]);
-    int sx = 0, sy = 0, sz = 0;
     if (dc.periodic) {
-      if (fabs(dx) > dc.hdlx) { sx = (dx > 0.) ? 1 : -1; dx = xsub(dx, copysign(dc.dxbound, dx)); }
-      if (fabs(dy) > dc.hdly) { sy = (dy > 0.) ? 1 : -1; dy = xsub(dy, copysign(dc.dybound, dy)); }
-      if (fabs(dz) > dc.hdlz) { sz = (dz > 0.) ? 1 : -1; dz = xsub(dz, copysign(dc.dzbound, dz)); }
+      if (fabs(dx) > dc.hdlx) dx = xsub(dx, copysign(dc.dxbound, dx));
+      if (fabs(dy) > dc.hdly) dy = xsub(dy, copysign(dc.dybound, dy));
+      if (fabs(dz) > dc.hdlz) dz = xsub(dz, copysign(dc.dzbound, dz));
     }
     const double r2 = r2_exact(dx, dy, dz);
-    double rcut = rcuti;
-    if (mode == 1) rcut = fmax(rcuti, xmul(dc.radkern, hmaxn ? hmaxn[n] : e.hmax));
-    const double s = xadd(xadd(xsizei, e.size), rcut);
+    const double s = xadd(xadd(xsizei, e.size), rcuti);
     if (r2 < xmul(s, s)) {
       if (leaf_flag[n] != 0) {
-        if (k < capc) cand[(size_t)l * capc + k] = cand_pack(n, off_code(sx, sy, sz));
-        k++;
         ntr += nd_hi[n] - nd_lo[n] + 1;
       } else {
         if (ns + 2 > 64) {
@@ -121,39 +90,132 @@ __global__ void k_walk(DevConst dc, int mode, double rfac, const phsph_kdnode *_
       }
     }
   }
-  ncand[l] = k;
   ntrial[l] = ntr;
-  if (k > capc) atomicOr(flags, DFL_CAND);
+}
+
+// ------------------------------------------------------------------------
+// group walk: candidate leaves of 64 consecutive active particles
+// mode 0: rcut = rfac*radkern*max(h of the group)
+// mode 1: rcut = max(that, radkern*hmax of the node)      (force, get_hj)
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(64)
+k_group_walk(DevConst dc, int mode, double rfac, const phsph_kdnode *__restrict__ nd,
+             const double *__restrict__ hmaxn, const int32_t *__restrict__ leaf_flag,
+             const int32_t *__restrict__ act, int nact, const double *__restrict__ tx,
+             const double *__restrict__ ty, const double *__restrict__ tz, const double *__restrict__ hsrc,
+             int ngroups, uint32_t *__restrict__ gcand, int32_t *__restrict__ gncand, int capg,
+             int32_t *__restrict__ flags) {
+  __shared__ int stk[WALK_STACK];
+  const int g = xcd_block(blockIdx.x, gridDim.x);
+  if (g >= ngroups) return;
+  const int lane = threadIdx.x;
+  const int t = g * GROUP + lane;
+  double xl = PH_HUGE64, yl = PH_HUGE64, zl = PH_HUGE64, xh = -PH_HUGE64, yh = -PH_HUGE64, zh = -PH_HUGE64;
+  double hm = 0.;
+  if (t < nact) {
+    const int p = act[t];
+    xl = xh = tx[p];
+    yl = yh = ty[p];
+    zl = zh = tz[p];
+    hm = hsrc[p];
+  }
+  xl = wave_min(xl); yl = wave_min(yl); zl = wave_min(zl);
+  xh = wave_max(xh); yh = wave_max(yh); zh = wave_max(zh);
+  hm = wave_max(hm);
+  const double bx = 0.5 * (xl + xh), by = 0.5 * (yl + yh), bz = 0.5 * (zl + zh);
+  // half extents, rounded up so that every particle lies inside
+  const double ex = 0.5 * (xh - xl) * (1. + 1e-12) + 1e-300, ey = 0.5 * (yh - yl) * (1. + 1e-12) + 1e-300,
+               ez = 0.5 * (zh - zl) * (1. + 1e-12) + 1e-300;
+  const double rcg = rfac * dc.radkern * hm;
+  if (lane == 0) stk[0] = 1;
+  int top = 1, nc = 0;
+  bool ovf = false;
+  __syncthreads();
+  while (top > 0) {
+    const int take = min(top, 64);
+    int node = 0;
+    if (lane < take) node = stk[top - take + lane];
+    __syncthreads();
+    top -= take;
+    bool open = false, isleaf = false;
+    int lc = 0, rc = 0;
+    if (node > 0) {
+      const phsph_kdnode e = nd[node];
+      double dx = e.xcen[0] - bx, dy = e.xcen[1] - by, dz = e.xcen[2] - bz;
+      if (dc.periodic) {
+        if (fabs(dx) > dc.hdlx) dx -= copysign(dc.dxbound, dx);
+        if (fabs(dy) > dc.hdly) dy -= copysign(dc.dybound, dy);
+        if (fabs(dz) > dc.hdlz) dz -= copysign(dc.dzbound, dz);
+      }
+      const double qx = fmax(fabs(dx) - ex, 0.), qy = fmax(fabs(dy) - ey, 0.), qz = fmax(fabs(dz) - ez, 0.);
+      const double d2 = qx * qx + qy * qy + qz * qz;
+      double rc_ = rcg;
+      if (mode == 1) rc_ = fmax(rc_, dc.radkern * hmaxn[node]);
+      const double s = (e.size + rc_) * (1. + 1e-10);
+      open = d2 < s * s;
+      isleaf = leaf_flag[node] != 0;
+      lc = e.leftchild;
+      rc = e.rightchild;
+    }
+    const unsigned long long ml = __ballot(open && isleaf);
+    if (open && isleaf) {
+      const int k = nc + __popcll(ml & lanemask_lt());
+      if (k < capg) gcand[(size_t)g * capg + k] = (uint32_t)node;
+    }
+    nc += __popcll(ml);
+    const bool push = open && !isleaf;
+    const unsigned long long mi = __ballot(push);
+    const int npush = 2 * __popcll(mi);
+    if (top + npush > WALK_STACK) {
+      ovf = true;
+      break;
+    }
+    if (push) {
+      const int k = top + 2 * __popcll(mi & lanemask_lt());
+      stk[k] = lc;
+      stk[k + 1] = rc;
+    }
+    top += npush;
+    __syncthreads();
+  }
+  if (lane == 0) {
+    gncand[g] = nc;
+    if (nc > capg || ovf) atomicOr(flags, DFL_CAND);
+  }
 }
 
 // ------------------------------------------------------------------------
 // gathers: caller layout (original order) -> tree order
 // ------------------------------------------------------------------------
-__global__ void k_gather_dens(int nlive, const int32_t *__restrict__ iparts, const double *__restrict__ xyzh,
-                              int ldx, const double *__restrict__ vxyzu, int ldv, const double *__restrict__ fxyzu,
-                              int ldf, const double *__restrict__ fext, const double *__restrict__ Bevol,
-                              const int8_t *__restrict__ iphase, double *th_in, double *vx, double *vy, double *vz,
-                              double *fx, double *fy, double *fz, double *Bx, double *By, double *Bz,
-                              int32_t *actflag) {
+__global__ void k_gather_dens(DevConst dc, int nlive, const int32_t *__restrict__ iparts,
+                              const double *__restrict__ tx, const double *__restrict__ ty,
+                              const double *__restrict__ tz, const double *__restrict__ xyzh, int ldx,
+                              const double *__restrict__ vxyzu, int ldv, const double *__restrict__ fxyzu, int ldf,
+                              const double *__restrict__ fext, const double *__restrict__ Bevol,
+                              const int8_t *__restrict__ iphase, double *th_in, double4 *tp, double4 *tv,
+                              double4 *tf, double4 *tB, int32_t *actflag) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nlive) return;
   int ip = iparts[p];
   int o = abs(ip) - 1;
-  th_in[p] = xyzh[(size_t)o * ldx + 3];
-  vx[p] = vxyzu[(size_t)o * ldv + 0];
-  vy[p] = vxyzu[(size_t)o * ldv + 1];
-  vz[p] = vxyzu[(size_t)o * ldv + 2];
-  double a0 = 0., a1 = 0., a2 = 0.;
-  if (fxyzu) {
-    a0 = fxyzu[(size_t)o * ldf + 0]; a1 = fxyzu[(size_t)o * ldf + 1]; a2 = fxyzu[(size_t)o * ldf + 2];
+  const double h = xyzh[(size_t)o * ldx + 3];
+  th_in[p] = h;
+  tp[p] = make_double4(tx[p], ty[p], tz[p], h);
+  tv[p] = make_double4(vxyzu[(size_t)o * ldv + 0], vxyzu[(size_t)o * ldv + 1], vxyzu[(size_t)o * ldv + 2], 0.);
+  if (tf) {
+    double a0 = 0., a1 = 0., a2 = 0.;
+    if (fxyzu) {
+      a0 = fxyzu[(size_t)o * ldf + 0]; a1 = fxyzu[(size_t)o * ldf + 1]; a2 = fxyzu[(size_t)o * ldf + 2];
+    }
+    if (fext) {
+      a0 = xadd(a0, fext[(size_t)o * 3 + 0]); a1 = xadd(a1, fext[(size_t)o * 3 + 1]);
+      a2 = xadd(a2, fext[(size_t)o * 3 + 2]);
+    }
+    tf[p] = make_double4(a0, a1, a2, 0.);
   }
-  if (fext) {
-    a0 = xadd(a0, fext[(size_t)o * 3 + 0]); a1 = xadd(a1, fext[(size_t)o * 3 + 1]); a2 = xadd(a2, fext[(size_t)o * 3 + 2]);
-  }
-  fx[p] = a0; fy[p] = a1; fz[p] = a2;
-  if (Bevol) {
-    Bx[p] = Bevol[(size_t)o * 4 + 0]; By[p] = Bevol[(size_t)o * 4 + 1]; Bz[p] = Bevol[(size_t)o * 4 + 2];
-  }
+  if (Bevol)
+    tB[p] = make_double4(Bevol[(size_t)o * 4 + 0], Bevol[(size_t)o * 4 + 1], Bevol[(size_t)o * 4 + 2],
+                         rhoh_d(h, dc.hfact, dc.pmass));
   // start_cell (dens.F90:1310-1331): skip inactive particles
   actflag[p] = (ip > 0) && (iphase == nullptr || iphase[o] > 0);
 }
@@ -207,107 +269,148 @@ __global__ void k_scan_i32(int32_t *a, int n, int32_t *total) {
 }
 
 // ------------------------------------------------------------------------
-// density: one thread per active particle
+// density: one wavefront per group, one active particle per lane
 // ------------------------------------------------------------------------
 template <int K, bool MHD>
-__global__ void __launch_bounds__(256)
-k_density(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, int nact,
-          const int32_t *__restrict__ pleaf, const int32_t *__restrict__ leaves, const uint32_t *__restrict__ cand,
-          const int32_t *__restrict__ ncand, int capc, const int32_t *__restrict__ nd_lo,
+__global__ void __launch_bounds__(64)
+k_density(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, int nact, int ngroups,
+          const int32_t *__restrict__ pleaf, const int32_t *__restrict__ leaves, const uint32_t *__restrict__ gcand,
+          const int32_t *__restrict__ gncand, int capg, const int32_t *__restrict__ nd_lo,
           const int32_t *__restrict__ nd_hi, const phsph_kdnode *__restrict__ nd,
           const int32_t *__restrict__ iparts, DensIn in, uint32_t *__restrict__ nlist, int capl,
           double *__restrict__ th_out, DensOut out, int32_t *__restrict__ p_nneigh, int32_t *__restrict__ p_nits,
-          int32_t *__restrict__ leaf_nits, int32_t *__restrict__ flags) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nact) return;
-  const int p = act[t];
-  const int l = pleaf[p];
-  const int c = leaves[l];
-  const double xi = in.tx[p], yi = in.ty[p], zi = in.tz[p];
-  const double h0 = in.th_in[p];
+          int32_t *__restrict__ leaf_nits, int32_t *__restrict__ retry, int32_t *__restrict__ flags) {
+  __shared__ StageLDS S;
+  __shared__ uint32_t lbuf[8][GROUP];
+  const int g = xcd_block(blockIdx.x, gridDim.x);
+  if (g >= ngroups) return;
+  const int lane = threadIdx.x;
+  const int t = g * GROUP + lane;
+  const bool live = t < nact;
+  const size_t nrows = (size_t)ngroups * GROUP;
+  int p = 0, l = 0;
+  double xi = 0., yi = 0., zi = 0., h0 = 1., lcx = 0., lcy = 0., lcz = 0., lsz = 0.;
+  if (live) {
+    p = act[t];
+    l = pleaf[p];
+    const int c = leaves[l];
+    const double4 r = in.tp[p];
+    xi = r.x; yi = r.y; zi = r.z; h0 = r.w;
+    lcx = nd[c].xcen[0]; lcy = nd[c].xcen[1]; lcz = nd[c].xcen[2]; lsz = nd[c].size;
+  }
   const double pm = dc.pmass;
-  const double rcut = rfac * dc.radkern * nd[c].hmax;
-  const double rc2 = xmul(rcut, rcut);
-  // --- neighbour list: candidates within the walk radius ---
-  int nl = 0;
-  const int nc = ncand[l];
-  if (nc > capc) {
-    atomicOr(flags, DFL_CAND);
-    return;
-  }
-  for (int k = 0; k < nc; k++) {
-    const uint32_t e = cand[(size_t)l * capc + k];
-    const int cn = cand_node(e), code = cand_code(e);
-    double ox, oy, oz;
-    offsets_of(dc, code, ox, oy, oz);
-    const int j1 = nd_hi[cn];
-    for (int j = nd_lo[cn]; j <= j1; j++) {
-      if (j == p) continue;
-      double dx, dy, dz;
-      const double r2 = sep2(dc, xi, yi, zi, in.tx[j], in.ty[j], in.tz[j], ox, oy, oz, dx, dy, dz);
-      if (r2 < rc2) {
-        if (nl < capl) nlist[(size_t)nl * nact + t] = (uint32_t)j | ((uint32_t)code << 27);
-        nl++;
-      }
-    }
-  }
-  if (nl > capl) {
-    atomicOr(flags, DFL_LIST);
-    return;
-  }
-  // --- Newton-Raphson on h with rho and Omega only ---
+  // The group's candidate stream covers rcg = rfac*radkern*max(h0) (the
+  // same expression as in k_group_walk).  A lane lists the candidates within
+  // rfac*radkern*h (one Newton step of +20%, the clamp of finish_cell,
+  // dens.F90:1419); when its h outgrows the list it lists again from the
+  // stream, and only when h outgrows the stream itself is the particle
+  // handed back to the host for a wider walk.
+  const double rcg = rfac * dc.radkern * wave_max(live ? h0 : 0.);
+  double rcut = rfac * dc.radkern * h0;
+  enum { ST_LIST = 0, ST_ITER = 1, ST_DONE = 2, ST_FAIL = 3 };
+  int state = live ? ST_LIST : ST_DONE;
+  const int nc = gncand[g];
+  const uint32_t *gc = gcand + (size_t)g * capg;
   double hi = h0, hfin = h0;
-  int it = 0;
-  for (;;) {
-    it++;
-    const double hi1 = 1. / hi, hi21 = xmul(hi1, hi1);
-    double srho = 0., sgh = 0.;
-    for (int k = 0; k < nl; k++) {
-      const uint32_t e = nlist[(size_t)k * nact + t];
-      const int j = (int)(e & 0x07ffffffu);
-      double ox, oy, oz, dx, dy, dz;
-      offsets_of(dc, (int)(e >> 27), ox, oy, oz);
-      const double r2 = sep2(dc, xi, yi, zi, in.tx[j], in.ty[j], in.tz[j], ox, oy, oz, dx, dy, dz);
-      const double q2 = xmul(r2, hi21);
-      if (q2 < dc.radkern2) {
-        const double q = sqrt(r2) * hi1;
-        double w, g;
-        kern_wg<K, double>(q2, q, w, g);
-        srho += w * pm;
-        sgh += (-q * g - 3. * w) * pm;
+  int it = 0, nl = 0;
+  while (__ballot(state == ST_LIST) != 0ull) {
+    // --- neighbour list from the group's candidate stream ---
+    const bool listing = (state == ST_LIST);
+    const double rc2 = xmul(rcut, rcut);
+    ListWriter L{nlist, nrows, t, capl, 0, lbuf, lane};
+    for (int b0 = 0; b0 < nc; b0 += SB_LEAVES) {
+      const int nb = stage_batch(S, gc, b0, nc, nd, nd_lo, nd_hi, in.tp, lane);
+      for (int cl = 0; cl < nb; cl++) {
+        const int code = leaf_sep_code(dc, S, cl, lcx, lcy, lcz, lsz);
+        const int s1 = S.lstart[cl + 1];
+        for (int s = S.lstart[cl]; s < s1; s++) {
+          const double4 r = S.pos[s];
+          const int j = S.jpos[s];
+          double dx, dy, dz;
+          sep_of(dc, code, xi, yi, zi, r.x, r.y, r.z, dx, dy, dz);
+          const double r2 = r2_exact(dx, dy, dz);
+          if (listing && j != p && r2 < rc2) L.push(list_pack(j, code));
+        }
+      }
+      __syncthreads();
+    }
+    L.finish();
+    if (listing) {
+      nl = L.n;
+      if (nl > capl) {
+        atomicOr(flags, DFL_LIST);
+        state = ST_FAIL;
+      } else {
+        state = ST_ITER;
       }
     }
-    // finish_rhosum + finish_cell
-    const double hi31 = hi1 * hi21, hi41 = hi21 * hi21;
-    const double rhoi = dc.cnormk * (srho + dc.wab0 * pm) * hi31;
-    const double gradhi = dc.cnormk * (sgh + dc.gradh0 * pm) * hi41;
-    const double rhohi = rhoh_d(hi, dc.hfact, pm);
-    const double dhdrhoi = -hi / (3. * rhohi);
-    const double omegai = 1. - dhdrhoi * gradhi;
-    const double func = rhohi - rhoi;
-    const double dfdh1 = (omegai > PH_TINY64) ? dhdrhoi / omegai : dhdrhoi / fabs(omegai + PH_EPS64);
-    double hnew = hi - func * dfdh1;
-    if (hnew > 1.2 * hi) hnew = 1.2 * hi;
-    else if (hnew < 0.8 * hi) hnew = 0.8 * hi;
-    const bool conv = (fabs(hnew - hi) / h0 < dc.tolh) && omegai > 0. && hi > 0.;
-    if (icall == 0) {  // one pass: store at hnew with the sums taken at h0 (dens.F90:341)
-      hfin = conv ? hi : hnew;
-      break;
+    if (state != ST_ITER) continue;
+    // --- Newton-Raphson on h with rho and Omega only ---
+    for (;;) {
+      it++;
+      const double hi1 = 1. / hi, hi21 = xmul(hi1, hi1);
+      double srho = 0., sgh = 0.;
+      for (int k = 0; k < nl; k++) {
+        const uint32_t e = list_get(nlist, nrows, t, k);
+        const int j = (int)(e & 0x07ffffffu);
+        const double4 r = in.tp[j];
+        double dx, dy, dz;
+        sep_of(dc, (int)(e >> 27), xi, yi, zi, r.x, r.y, r.z, dx, dy, dz);
+        const double r2 = r2_exact(dx, dy, dz);
+        const double q2 = xmul(r2, hi21);
+        if (q2 < dc.radkern2) {
+          const double q = sqrt(r2) * hi1;
+          double w, gk;
+          kern_wg<K, double>(q2, q, w, gk);
+          srho += w * pm;
+          sgh += (-q * gk - 3. * w) * pm;
+        }
+      }
+      // finish_rhosum + finish_cell
+      const double hi31 = hi1 * hi21, hi41 = hi21 * hi21;
+      const double rhoi = dc.cnormk * (srho + dc.wab0 * pm) * hi31;
+      const double gradhi = dc.cnormk * (sgh + dc.gradh0 * pm) * hi41;
+      const double rhohi = rhoh_d(hi, dc.hfact, pm);
+      const double dhdrhoi = -hi / (3. * rhohi);
+      const double omegai = 1. - dhdrhoi * gradhi;
+      const double func = rhohi - rhoi;
+      const double dfdh1 = (omegai > PH_TINY64) ? dhdrhoi / omegai : dhdrhoi / fabs(omegai + PH_EPS64);
+      double hnew = hi - func * dfdh1;
+      if (hnew > 1.2 * hi) hnew = 1.2 * hi;
+      else if (hnew < 0.8 * hi) hnew = 0.8 * hi;
+      const bool conv = (fabs(hnew - hi) / h0 < dc.tolh) && omegai > 0. && hi > 0.;
+      if (icall == 0) {  // one pass: store at hnew with the sums taken at h0 (dens.F90:341)
+        hfin = conv ? hi : hnew;
+        state = ST_DONE;
+        break;
+      }
+      if (conv) {
+        hfin = hi;
+        state = ST_DONE;
+        break;
+      }
+      if (it >= PH_MAXDENSITS) {
+        atomicOr(flags, DFL_NOCONV);
+        state = ST_FAIL;
+        break;
+      }
+      hi = hnew;
+      if (dc.radkern * hi * (1. + 1e-12) > rcut) {  // the list no longer covers radkern*h
+        if (dc.radkern * hi * (1. + 1e-12) > rcg) {
+          atomicOr(flags, DFL_REWALK);
+          state = ST_FAIL;
+        } else {
+          rcut = fmin(rfac * dc.radkern * hi, rcg);
+          state = ST_LIST;
+        }
+        break;
+      }
     }
-    if (conv) {
-      hfin = hi;
-      break;
-    }
-    if (it >= PH_MAXDENSITS) {
-      atomicOr(flags, DFL_NOCONV);
-      hfin = hi;
-      break;
-    }
-    hi = hnew;
-    if (dc.radkern * hi > rcut) {  // list no longer covers 2h: walk again wider
-      atomicOr(flags, DFL_REWALK);
-      return;
-    }
+  }
+  if (!live) return;
+  if (state != ST_DONE) {
+    if (state == ST_FAIL) retry[p] = 1;
+    return;
   }
   // --- all sums at h = hi (the h of the last summation) ---
   const double hs1 = 1. / hi, hs21 = xmul(hs1, hs1);
@@ -315,40 +418,45 @@ k_density(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, 
   double dv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, da[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   double rr[6] = {0, 0, 0, 0, 0, 0};
   double rs_divB = 0., dB[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  const double vxi = in.vx[p], vyi = in.vy[p], vzi = in.vz[p];
-  const double fxi = in.fx ? in.fx[p] : 0., fyi = in.fx ? in.fy[p] : 0., fzi = in.fx ? in.fz[p] : 0.;
-  double Bxi = 0., Byi = 0., Bzi = 0., rhoiB = 0.;
+  const double4 vi = in.tv[p];
+  double4 fi = make_double4(0., 0., 0., 0.);
+  if (in.tf) fi = in.tf[p];
+  double Bxi = 0., Byi = 0., Bzi = 0.;
   if (MHD) {
-    rhoiB = rhoh_d(hi, dc.hfact, pm);
-    Bxi = in.Bx[p] * rhoiB; Byi = in.By[p] * rhoiB; Bzi = in.Bz[p] * rhoiB;
+    const double rhoiB = rhoh_d(hi, dc.hfact, pm);
+    const double4 Bi = in.tB[p];
+    Bxi = Bi.x * rhoiB; Byi = Bi.y * rhoiB; Bzi = Bi.z * rhoiB;
   }
   int nneigh = 1;
   for (int k = 0; k < nl; k++) {
-    const uint32_t e = nlist[(size_t)k * nact + t];
+    const uint32_t e = list_get(nlist, nrows, t, k);
     const int j = (int)(e & 0x07ffffffu);
-    double ox, oy, oz, dx, dy, dz;
-    offsets_of(dc, (int)(e >> 27), ox, oy, oz);
-    const double r2 = sep2(dc, xi, yi, zi, in.tx[j], in.ty[j], in.tz[j], ox, oy, oz, dx, dy, dz);
+    const double4 r = in.tp[j];
+    double dx, dy, dz;
+    sep_of(dc, (int)(e >> 27), xi, yi, zi, r.x, r.y, r.z, dx, dy, dz);
+    const double r2 = r2_exact(dx, dy, dz);
     const double q2 = xmul(r2, hs21);
     if (!(q2 < dc.radkern2)) continue;
     const double rij = sqrt(r2);
     const double q = rij * hs1;
-    double w, g;
-    kern_wg<K, double>(q2, q, w, g);
+    double w, gk;
+    kern_wg<K, double>(q2, q, w, gk);
     rs_rho += w * pm;
-    rs_gh += (-q * g - 3. * w) * pm;
+    rs_gh += (-q * gk - 3. * w) * pm;
     if (dc.gravity) rs_gs += kern_dphidh<K, double>(q2, q) * pm;
     nneigh++;
     const double rij1 = 1. / (rij + PH_EPS64);
-    const double rg = rij1 * g;
+    const double rg = rij1 * gk;
     const double runix = dx * rg * pm, runiy = dy * rg * pm, runiz = dz * rg * pm;
-    const double dvx = vxi - in.vx[j], dvy = vyi - in.vy[j], dvz = vzi - in.vz[j];
+    const double4 vj = in.tv[j];
+    const double dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
     rs_divv += dvx * runix + dvy * runiy + dvz * runiz;
     dv[0] += dvx * runix; dv[1] += dvx * runiy; dv[2] += dvx * runiz;
     dv[3] += dvy * runix; dv[4] += dvy * runiy; dv[5] += dvy * runiz;
     dv[6] += dvz * runix; dv[7] += dvz * runiy; dv[8] += dvz * runiz;
-    if (in.fx) {
-      const double dax = fxi - in.fx[j], day = fyi - in.fy[j], daz = fzi - in.fz[j];
+    if (in.tf) {
+      const double4 fj = in.tf[j];
+      const double dax = fi.x - fj.x, day = fi.y - fj.y, daz = fi.z - fj.z;
       da[0] += dax * runix; da[1] += dax * runiy; da[2] += dax * runiz;
       da[3] += day * runix; da[4] += day * runiy; da[5] += day * runiz;
       da[6] += daz * runix; da[7] += daz * runiy; da[8] += daz * runiz;
@@ -356,8 +464,8 @@ k_density(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, 
     rr[0] -= dx * runix; rr[1] -= dx * runiy; rr[2] -= dx * runiz;
     rr[3] -= dy * runiy; rr[4] -= dy * runiz; rr[5] -= dz * runiz;
     if (MHD) {
-      const double rhoj = rhoh_d(in.th_in[j], dc.hfact, pm);
-      const double dBx = Bxi - in.Bx[j] * rhoj, dBy = Byi - in.By[j] * rhoj, dBz = Bzi - in.Bz[j] * rhoj;
+      const double4 Bj = in.tB[j];  // rho_j from the h it had on entry
+      const double dBx = Bxi - Bj.x * Bj.w, dBy = Byi - Bj.y * Bj.w, dBz = Bzi - Bj.z * Bj.w;
       rs_divB += dBx * runix + dBy * runiy + dBz * runiz;
       dB[0] += dBx * runix; dB[1] += dBx * runiy; dB[2] += dBx * runiz;
       dB[3] += dBy * runix; dB[4] += dBy * runiy; dB[5] += dBy * runiz;
@@ -416,23 +524,28 @@ k_density(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, 
     out.divcurlB[(size_t)o * 4 + 2] = (float)(-(dB[2] - dB[6]) * term);
     out.divcurlB[(size_t)o * 4 + 3] = (float)(-(dB[3] - dB[1]) * term);
   }
-  p_nneigh[t] = nneigh;
-  p_nits[t] = (icall == 0) ? 1 : it;
+  p_nneigh[p] = nneigh;
+  p_nits[p] = (icall == 0) ? 1 : it;
   atomicMax(&leaf_nits[l], (icall == 0) ? 1 : it);
 }
 
-// per-leaf statistics: active particle count x iterations, trial counts
+// actual-neighbour statistics and density evaluations (dens.F90:1670-1679)
 __global__ void k_dens_stats(int nact, const int32_t *act, const int32_t *pleaf, const int32_t *p_nneigh,
                              const int32_t *leaf_nits, const int32_t *ntrial, long long *acc) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   long long nn = 0, nit = 0, ntr = 0, mx = 0, mxt = 0;
   if (t < nact) {
-    int l = pleaf[act[t]];
-    nn = p_nneigh[t];
-    nit = leaf_nits[l];
-    ntr = ntrial[l];
-    mx = nn;
-    mxt = ntr;
+    const int p = act[t];
+    int l = pleaf[p];
+    if (p_nneigh) {
+      nn = p_nneigh[p];
+      nit = leaf_nits[l];
+      mx = nn;
+    }
+    if (ntrial) {
+      ntr = ntrial[l];
+      mxt = ntr;
+    }
   }
   for (int o = 32; o > 0; o >>= 1) {
     nn += __shfl_xor(nn, o, 64);
@@ -460,40 +573,75 @@ __global__ void k_scatter_pstats(int nact, const int32_t *act, const int32_t *ip
                                  const int32_t *p_nits, int32_t *nneigh, int32_t *nits) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nact) return;
-  int o = abs(iparts[act[t]]) - 1;
-  if (nneigh) nneigh[o] = p_nneigh[t];
-  if (nits) nits[o] = p_nits[t];
+  const int p = act[t];
+  int o = abs(iparts[p]) - 1;
+  if (nneigh) nneigh[o] = p_nneigh[p];
+  if (nits) nits[o] = p_nits[p];
 }
 
 int dens_particle_stats(phsph_ctx *c, int32_t *d_nneigh, int32_t *d_nits) {
-  if (c->nact > 0)
-    hipLaunchKernelGGL(k_scatter_pstats, dim3(nblocks(c->nact, 256)), dim3(256), 0, c->stream, c->nact, c->act.p,
-                       c->iparts.p, c->p_nneigh.p, c->p_nits.p, d_nneigh, d_nits);
+  if (c->nact_dens > 0)
+    hipLaunchKernelGGL(k_scatter_pstats, dim3(nblocks(c->nact_dens, 256)), dim3(256), 0, c->stream, c->nact_dens,
+                       c->act_dens.p, c->iparts.p, c->p_nneigh.p, c->p_nits.p, d_nneigh, d_nits);
   PH_HIPCHECK(c, hipGetLastError());
   return PHSPH_OK;
 }
 
-// ------------------------------------------------------------------------
-int walk_leaves(phsph_ctx *c, int mode, double rfac, const double *hmaxn) {
+// trial neighbours of the reference's per-leaf walk at radkern*hmax, over
+// the active particles of the last densityiterate (computed on request)
+int dens_trial_stats(phsph_ctx *c) {
+  if (!c->trial_stale) return PHSPH_OK;
   hipStream_t s = c->stream;
-  for (int attempt = 0; attempt < 4; attempt++) {
-    PH_HIPCHECK(c, c->cand.ensure((size_t)c->nleaves * c->caps.cand));
-    PH_HIPCHECK(c, c->ncand.ensure(c->nleaves));
-    PH_HIPCHECK(c, c->ntrial.ensure(c->nleaves));
+  PH_HIPCHECK(c, c->ntrial.ensure(c->nleaves + 1));
+  int32_t zero = 0;
+  PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+  if (c->nleaves > 0)
+    hipLaunchKernelGGL(k_walk, dim3(nblocks(c->nleaves, 128)), dim3(128), 0, s, c->dc, 1.0, c->nd.p,
+                       c->leaf_flag.p, c->nd_lo.p, c->nd_hi.p, c->leaves.p, c->nleaves, c->ntrial.p,
+                       c->counters.p);
+  PH_HIPCHECK(c, hipGetLastError());
+  PH_HIPCHECK(c, c->stats_i.ensure(8));
+  long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  PH_HIPCHECK(c, hipMemcpyAsync(c->stats_i.p, z, sizeof(z), hipMemcpyHostToDevice, s));
+  if (c->nact_dens > 0)
+    hipLaunchKernelGGL(k_dens_stats, dim3(nblocks(c->nact_dens, 256)), dim3(256), 0, s, c->nact_dens,
+                       c->act_dens.p, c->pleaf.p, nullptr, nullptr, c->ntrial.p, c->stats_i.p);
+  long long acc[8];
+  int32_t fl = 0;
+  PH_HIPCHECK(c, hipMemcpyAsync(acc, c->stats_i.p, sizeof(acc), hipMemcpyDeviceToHost, s));
+  PH_HIPCHECK(c, hipMemcpyAsync(&fl, c->counters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  PH_HIPCHECK(c, hipStreamSynchronize(s));
+  if (fl & DFL_CAND) PH_RET(c, PHSPH_ERR_CAPACITY, "neighbour statistics: walk stack overflow");
+  if (c->nact_dens > 0) {
+    c->trialmean = (double)acc[2] / (double)c->nact_dens;
+    c->maxtrial = (int)acc[4];
+  }
+  c->trial_stale = false;
+  return PHSPH_OK;
+}
+
+// ------------------------------------------------------------------------
+int walk_groups(phsph_ctx *c, int mode, double rfac, const double *d_hsrc, const double *d_hmaxn) {
+  hipStream_t s = c->stream;
+  const int ng = (c->nact + GROUP - 1) / GROUP;
+  c->ngroups = ng;
+  for (int attempt = 0; attempt < 6; attempt++) {
+    PH_HIPCHECK(c, c->gcand.ensure((size_t)(ng + 1) * c->caps.gcand));
+    PH_HIPCHECK(c, c->gncand.ensure(ng + 1));
     int32_t zero = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
-    {
+    if (ng > 0) {
       LaunchTimer lt(c, mode == 0 ? "walk_density" : "walk_force");
-      hipLaunchKernelGGL(k_walk, dim3(nblocks(c->nleaves, 128)), dim3(128), 0, s, c->dc, mode, rfac, c->nd.p,
-                         hmaxn, c->leaf_flag.p, c->nd_lo.p, c->nd_hi.p, c->leaves.p, c->nleaves, c->cand.p,
-                         c->ncand.p, c->ntrial.p, c->caps.cand, c->counters.p);
+      hipLaunchKernelGGL(k_group_walk, dim3(xcd_grid(ng)), dim3(64), 0, s, c->dc, mode, rfac, c->nd.p, d_hmaxn,
+                         c->leaf_flag.p, c->act.p, c->nact, c->tx.p, c->ty.p, c->tz.p, d_hsrc, ng, c->gcand.p,
+                         c->gncand.p, c->caps.gcand, c->counters.p);
     }
     PH_HIPCHECK(c, hipGetLastError());
     int32_t fl = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(&fl, c->counters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     PH_HIPCHECK(c, hipStreamSynchronize(s));
     if (!(fl & DFL_CAND)) return PHSPH_OK;
-    c->caps.cand *= 2;
+    c->caps.gcand *= 2;
   }
   PH_RET(c, PHSPH_ERR_CAPACITY, "neighbour walk: candidate list capacity exceeded");
 }
@@ -516,52 +664,65 @@ int compact_active(phsph_ctx *c, const int32_t *flag, int n) {
 int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   hipStream_t s = c->stream;
   const int n = c->nlive;
-  PH_HIPCHECK(c, c->th_in.ensure(n));
-  PH_HIPCHECK(c, c->tvx.ensure(n)); PH_HIPCHECK(c, c->tvy.ensure(n)); PH_HIPCHECK(c, c->tvz.ensure(n));
-  PH_HIPCHECK(c, c->tfx.ensure(n)); PH_HIPCHECK(c, c->tfy.ensure(n)); PH_HIPCHECK(c, c->tfz.ensure(n));
-  if (c->p.mhd) {
-    PH_HIPCHECK(c, c->tBx.ensure(n)); PH_HIPCHECK(c, c->tBy.ensure(n)); PH_HIPCHECK(c, c->tBz.ensure(n));
-  }
-  PH_HIPCHECK(c, c->tmpi.ensure(n));
+  const bool mhd = c->p.mhd != 0;
   const bool have_f = (a.fxyzu != nullptr) || (a.fext != nullptr);
+  PH_HIPCHECK(c, c->th_in.ensure(n));
+  PH_HIPCHECK(c, c->tp4.ensure(n));
+  PH_HIPCHECK(c, c->tv4.ensure(n));
+  if (have_f) PH_HIPCHECK(c, c->tf4.ensure(n));
+  if (mhd) PH_HIPCHECK(c, c->tB4.ensure(n));
+  PH_HIPCHECK(c, c->tmpi.ensure(n));
   {
     LaunchTimer lt(c, "dens_gather");
-    hipLaunchKernelGGL(k_gather_dens, dim3(nblocks(n, 256)), dim3(256), 0, s, n, c->iparts.p, a.xyzh, a.ldx,
-                       a.vxyzu, a.ldv, a.fxyzu, a.ldf, a.fext, c->p.mhd ? a.Bevol : nullptr, a.iphase, c->th_in.p,
-                       c->tvx.p, c->tvy.p, c->tvz.p, c->tfx.p, c->tfy.p, c->tfz.p, c->tBx.p, c->tBy.p, c->tBz.p,
-                       c->tmpi.p);
+    hipLaunchKernelGGL(k_gather_dens, dim3(nblocks(n, 256)), dim3(256), 0, s, c->dc, n, c->iparts.p, c->tx.p,
+                       c->ty.p, c->tz.p, a.xyzh, a.ldx, a.vxyzu, a.ldv, a.fxyzu, a.ldf, a.fext,
+                       mhd ? a.Bevol : nullptr, a.iphase, c->th_in.p, c->tp4.p, c->tv4.p,
+                       have_f ? c->tf4.p : nullptr, c->tB4.p, c->tmpi.p);
   }
   PH_HIPCHECK(c, hipGetLastError());
   int rc = compact_active(c, c->tmpi.p, n);
   if (rc) return rc;
   const int nact = c->nact;
-  PH_HIPCHECK(c, c->p_nneigh.ensure(nact + 1));
-  PH_HIPCHECK(c, c->p_nits.ensure(nact + 1));
+  PH_HIPCHECK(c, c->act_dens.ensure(nact + 1));
+  if (nact > 0)
+    PH_HIPCHECK(c, hipMemcpyAsync(c->act_dens.p, c->act.p, sizeof(int32_t) * nact, hipMemcpyDeviceToDevice, s));
+  c->nact_dens = nact;
+  c->trial_stale = true;
+  PH_HIPCHECK(c, c->p_nneigh.ensure(n + 1));
+  PH_HIPCHECK(c, c->p_nits.ensure(n + 1));
   PH_HIPCHECK(c, c->leaf_nits.ensure(c->nleaves + 1));
-  DensIn in{c->tx.p, c->ty.p, c->tz.p, c->th_in.p, c->tvx.p, c->tvy.p, c->tvz.p,
-            have_f ? c->tfx.p : nullptr, c->tfy.p, c->tfz.p, c->tBx.p, c->tBy.p, c->tBz.p};
+  PH_HIPCHECK(c, c->tmpi.ensure(n + 1));
+  DensIn in{c->tp4.p, c->tv4.p, have_f ? c->tf4.p : nullptr, c->tB4.p};
   DensOut out{a.xyzh, a.ldx, a.divcurlv, a.lddcv, a.divcurlB, a.alphaind, a.gradh, a.ldgradh, a.dvdx};
+  hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(c->nleaves, 256)), dim3(256), 0, s, c->leaf_nits.p, c->nleaves, 0);
+  hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(n, 256)), dim3(256), 0, s, c->p_nits.p, n, -1);
+  hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(n, 256)), dim3(256), 0, s, c->p_nneigh.p, n, -1);
+  const bool dbg = getenv("PHSPH_DEBUG") != nullptr;
+  // Particles whose h outgrows their group's candidate stream, or whose list
+  // overflows, are handed back and run again (alone) with a wider search.
   double rfac = 1.2;
-  for (int attempt = 0; attempt < 8; attempt++) {
-    rc = walk_leaves(c, 0, rfac, nullptr);
+  for (int attempt = 0; attempt < 12; attempt++) {
+    const int na = c->nact;
+    if (na == 0) break;
+    rc = walk_groups(c, 0, rfac, c->th_in.p, nullptr);
     if (rc) return rc;
-    PH_HIPCHECK(c, c->nlist.ensure((size_t)c->caps.nlist * (nact + 1)));
-    hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(c->nleaves, 256)), dim3(256), 0, s, c->leaf_nits.p,
-                       c->nleaves, 0);
+    const int ng = c->ngroups;
+    PH_HIPCHECK(c, c->nlist.ensure((size_t)c->caps.nlist * ((size_t)ng * GROUP + 1)));
+    hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(n, 256)), dim3(256), 0, s, c->tmpi.p, n, 0);
     int32_t zero = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
-    if (nact > 0) {
+    {
       LaunchTimer lt(c, "density");
-      const dim3 g(nblocks(nact, 256)), b(256);
-#define DLAUNCH(KK, MM)                                                                                      \
-  hipLaunchKernelGGL((k_density<KK, MM>), g, b, 0, s, c->dc, icall, rfac, c->act.p, nact, c->pleaf.p,          \
-                     c->leaves.p, c->cand.p, c->ncand.p, c->caps.cand, c->nd_lo.p, c->nd_hi.p, c->nd.p,        \
-                     c->iparts.p, in, c->nlist.p, c->caps.nlist, c->th.p, out, c->p_nneigh.p, c->p_nits.p,     \
-                     c->leaf_nits.p, c->counters.p)
+      const dim3 gr(xcd_grid(ng)), b(64);
+#define DLAUNCH(KK, MM)                                                                                        \
+  hipLaunchKernelGGL((k_density<KK, MM>), gr, b, 0, s, c->dc, icall, rfac, c->act.p, na, ng, c->pleaf.p,         \
+                     c->leaves.p, c->gcand.p, c->gncand.p, c->caps.gcand, c->nd_lo.p, c->nd_hi.p, c->nd.p,       \
+                     c->iparts.p, in, c->nlist.p, c->caps.nlist, c->th.p, out, c->p_nneigh.p, c->p_nits.p,       \
+                     c->leaf_nits.p, c->tmpi.p, c->counters.p)
       if (c->p.kernel == PHSPH_KERNEL_CUBIC) {
-        if (c->p.mhd) DLAUNCH(PHSPH_KERNEL_CUBIC, true); else DLAUNCH(PHSPH_KERNEL_CUBIC, false);
+        if (mhd) DLAUNCH(PHSPH_KERNEL_CUBIC, true); else DLAUNCH(PHSPH_KERNEL_CUBIC, false);
       } else {
-        if (c->p.mhd) DLAUNCH(PHSPH_KERNEL_QUINTIC, true); else DLAUNCH(PHSPH_KERNEL_QUINTIC, false);
+        if (mhd) DLAUNCH(PHSPH_KERNEL_QUINTIC, true); else DLAUNCH(PHSPH_KERNEL_QUINTIC, false);
       }
 #undef DLAUNCH
     }
@@ -569,33 +730,44 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
     int32_t fl = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(&fl, c->counters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     PH_HIPCHECK(c, hipStreamSynchronize(s));
+    if (dbg) fprintf(stderr, "phsph densityiterate: attempt %d particles %d groups %d rfac %g flags %d\n", attempt, na, ng, rfac, fl);
     if (fl & DFL_NOCONV) PH_RET(c, PHSPH_ERR_NOCONVERGE, "densityiterate: could not converge in density");
     if (fl & DFL_NEGH) PH_RET(c, PHSPH_ERR_INPUT, "densityiterate: setting negative h from hrho");
-    if (fl & DFL_CAND) { c->caps.cand *= 2; continue; }
-    if (fl & DFL_LIST) { c->caps.nlist = c->caps.nlist * 3 / 2; continue; }
-    if (fl & DFL_REWALK) { rfac *= 1.5; continue; }
-    // statistics (dens.F90:1670-1679): actual neighbours, density calcs and
-    // trial neighbours of a walk at the reference's search radius
-    rc = walk_leaves(c, 0, 1.0, nullptr);
-    if (rc) return rc;
-    PH_HIPCHECK(c, c->stats_i.ensure(8));
-    long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    PH_HIPCHECK(c, hipMemcpyAsync(c->stats_i.p, z, sizeof(z), hipMemcpyHostToDevice, s));
-    if (nact > 0)
-      hipLaunchKernelGGL(k_dens_stats, dim3(nblocks(nact, 256)), dim3(256), 0, s, nact, c->act.p, c->pleaf.p,
-                         c->p_nneigh.p, c->leaf_nits.p, c->ntrial.p, c->stats_i.p);
-    long long acc[8];
-    PH_HIPCHECK(c, hipMemcpyAsync(acc, c->stats_i.p, sizeof(acc), hipMemcpyDeviceToHost, s));
-    PH_HIPCHECK(c, hipStreamSynchronize(s));
-    if (nact > 0) {
-      c->actualmean = (double)acc[0] / (double)nact;
-      c->nactualtot = acc[0];
-      c->nrhocalc = acc[1];
-      c->trialmean = (double)acc[2] / (double)nact;
-      c->maxactual = (int)acc[3];
-      c->maxtrial = (int)acc[4];
+    if (!(fl & (DFL_LIST | DFL_REWALK))) {
+      c->nact = 0;
+      break;
     }
-    return PHSPH_OK;
+    if (fl & DFL_LIST) c->caps.nlist = ((c->caps.nlist * 3 / 2) + 7) & ~7;
+    if (fl & DFL_REWALK) rfac *= 1.5;
+    rc = compact_active(c, c->tmpi.p, n);
+    if (rc) return rc;
   }
-  PH_RET(c, PHSPH_ERR_CAPACITY, "densityiterate: neighbour list capacity could not be satisfied");
+  if (c->nact != 0) PH_RET(c, PHSPH_ERR_CAPACITY, "densityiterate: neighbour search could not be widened enough");
+  // statistics (dens.F90:1670-1679); trial counts are computed on request
+  PH_HIPCHECK(c, c->stats_i.ensure(8));
+  long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  PH_HIPCHECK(c, hipMemcpyAsync(c->stats_i.p, z, sizeof(z), hipMemcpyHostToDevice, s));
+  if (nact > 0)
+    hipLaunchKernelGGL(k_dens_stats, dim3(nblocks(nact, 256)), dim3(256), 0, s, nact, c->act_dens.p, c->pleaf.p,
+                       c->p_nneigh.p, c->leaf_nits.p, nullptr, c->stats_i.p);
+  PH_HIPCHECK(c, hipGetLastError());
+  long long acc[8];
+  PH_HIPCHECK(c, hipMemcpyAsync(acc, c->stats_i.p, sizeof(acc), hipMemcpyDeviceToHost, s));
+  // the active list again holds every active particle of this call
+  if (nact > 0)
+    PH_HIPCHECK(c, hipMemcpyAsync(c->act.p, c->act_dens.p, sizeof(int32_t) * nact, hipMemcpyDeviceToDevice, s));
+  PH_HIPCHECK(c, hipStreamSynchronize(s));
+  c->nact = nact;
+  if (nact > 0) {
+    c->actualmean = (double)acc[0] / (double)nact;
+    c->nactualtot = acc[0];
+    c->nrhocalc = acc[1];
+    c->maxactual = (int)acc[3];
+  } else {
+    c->actualmean = 0.;
+    c->nactualtot = 0;
+    c->nrhocalc = 0;
+    c->maxactual = 0;
+  }
+  return PHSPH_OK;
 }

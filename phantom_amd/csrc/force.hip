@@ -8,25 +8,41 @@
 // stress, Borve div B source subtraction, artificial resistivity, psi
 // cleaning) and one-fluid dust (sqrt(eps/(1-eps)) evolution, Delta v).
 //
-// One thread per active particle.  Pass 1 collects the symmetric neighbour
-// set {j : q_i < R or q_j < R} (force.F90:1287) with the same separately
-// rounded FP64 tests as the reference; pass 2 evaluates the pair terms.
-// Quantities of particle j that the reference recomputes per pair
-// (rho_j, P/rho^2, wave speed, MHD stress, stopping time) are evaluated once
-// per particle by k_force_prep with the same arithmetic.
-#include "ctx.h"
+// One wavefront per group of 64 active particles, one particle per lane
+// (stream.h).  Pass 1 pulls the group's candidate stream through LDS and
+// collects each lane's symmetric neighbour set {j : q_i < R or q_j < R}
+// (force.F90:1287) with the same separately rounded FP64 tests as the
+// reference; pass 2 evaluates the pair terms.  Quantities of particle j that
+// the reference recomputes per pair (rho_j, P/rho^2, wave speed, MHD stress,
+// stopping time) are evaluated once per particle by k_force_prep with the
+// same arithmetic and packed into one 128-byte record per particle, so a
+// pair reads one cache line of j data.
+#include "stream.h"
 
 enum { FFL_LIST = 2, FFL_DTC = 32 };
 
-struct JTerms {  // per particle, tree order
-  const double *x, *y, *z, *hinv, *vx, *vy, *vz, *u;
-  const float *gradh, *alpha;
-  const double *rho, *rho1, *pr, *pro2, *vwave;
-  const double *Bx, *By, *Bz, *psi;                 // B (not B/rho), psi
-  const double *sxx, *sxy, *sxz, *syy, *syz, *szz;  // MHD stress
-  const double *eps, *sqeps, *ts;                   // dust
+// j quantities of the hydro pair terms (tree order); cs and h are read for i
+struct __align__(16) FRec {
+  double x, y, z, hinv;
+  double vx, vy, vz, u;
+  double rho, rho1, pr, pro2;
+  double vwave;
+  float gradh, alpha;
+  double cs, h;
 };
-
+static_assert(sizeof(FRec) == 128, "FRec is one 128-byte line");
+// MHD: B = Bevol*rho, psi, the stress -m B(x)B/rho^2, and Bevol itself (i)
+struct __align__(16) FRecB {
+  double Bx, By, Bz, psi;
+  double sxx, sxy, sxz, syy;
+  double syz, szz, Bex, Bey;
+  double Bez, pad0, pad1, pad2;
+};
+static_assert(sizeof(FRecB) == 128, "FRecB is one 128-byte line");
+// one-fluid dust
+struct __align__(16) FRecD {
+  double eps, sqeps, ts, pad;
+};
 
 // get_ts (dust.f90:161-276), idrag 2 (constant K) and 3 (constant t_s)
 __device__ __forceinline__ double get_ts_d(const DevConst &dc, double rhogas, double rhodust) {
@@ -40,100 +56,107 @@ __device__ __forceinline__ double get_ts_d(const DevConst &dc, double rhogas, do
 
 // per-particle j terms (force.F90:1455-1509 with get_stress)
 __global__ void k_force_prep(DevConst dc, int nlive, const int32_t *__restrict__ iparts,
-                             const double *__restrict__ xyzh, int ldx, const double *__restrict__ vxyzu, int ldv,
-                             const float *__restrict__ gradh, int ldgradh, const float *__restrict__ alphaind,
-                             const double *__restrict__ eos, int ldeos, const double *__restrict__ Bevol,
-                             const double *__restrict__ dustfrac, const float *__restrict__ divcurlB, double *h,
-                             double *hinv, double *vx, double *vy, double *vz, double *u, float *tgradh,
-                             float *talpha, double *rho, double *rho1, double *pr, double *cs, double *pro2,
-                             double *vwave, double *Bx, double *By, double *Bz, double *psi, double *Bex,
-                             double *Bey, double *Bez, double *sxx, double *sxy, double *sxz, double *syy,
-                             double *syz, double *szz, double *eps, double *sqeps, double *ts, double *divB) {
+                             const double *__restrict__ tx, const double *__restrict__ ty,
+                             const double *__restrict__ tz, const double *__restrict__ xyzh, int ldx,
+                             const double *__restrict__ vxyzu, int ldv, const float *__restrict__ gradh,
+                             int ldgradh, const float *__restrict__ alphaind, const double *__restrict__ eos,
+                             int ldeos, const double *__restrict__ Bevol, const double *__restrict__ dustfrac,
+                             double *__restrict__ th, double4 *__restrict__ fq, FRec *__restrict__ rec,
+                             FRecB *__restrict__ recB, FRecD *__restrict__ recD) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nlive) return;
   const int o = abs(iparts[p]) - 1;
   const double pm = dc.pmass;
   const double hh = xyzh[(size_t)o * ldx + 3];
-  h[p] = hh;
+  th[p] = hh;
   const double hj1 = 1. / hh;  // xyzcache(4) = 1/treecache(4) (kdtree.F90:1202)
-  hinv[p] = hj1;
-  vx[p] = vxyzu[(size_t)o * ldv + 0];
-  vy[p] = vxyzu[(size_t)o * ldv + 1];
-  vz[p] = vxyzu[(size_t)o * ldv + 2];
-  u[p] = (dc.maxvxyzu >= 4) ? vxyzu[(size_t)o * ldv + 3] : 0.;
-  tgradh[p] = gradh[(size_t)o * ldgradh];
-  talpha[p] = alphaind[(size_t)o * 3];
+  FRec R;
+  R.x = tx[p]; R.y = ty[p]; R.z = tz[p];
+  R.hinv = hj1;
+  fq[p] = make_double4(R.x, R.y, R.z, hj1);
+  R.vx = vxyzu[(size_t)o * ldv + 0];
+  R.vy = vxyzu[(size_t)o * ldv + 1];
+  R.vz = vxyzu[(size_t)o * ldv + 2];
+  R.u = (dc.maxvxyzu >= 4) ? vxyzu[(size_t)o * ldv + 3] : 0.;
+  R.gradh = gradh[(size_t)o * ldgradh];
+  R.alpha = alphaind[(size_t)o * 3];
   const double prj = eos[(size_t)o * ldeos], csj = eos[(size_t)o * ldeos + 1];
-  pr[p] = prj;
-  cs[p] = csj;
+  R.pr = prj;
+  R.cs = csj;
+  R.h = hh;
   const double hj = 1. / hj1;
   const double rhoj = rhoh_d(hj, dc.hfact, pm);
   const double rho1j = 1. / rhoj;
-  rho[p] = rhoj;
-  rho1[p] = rho1j;
-  double Bxj = 0., Byj = 0., Bzj = 0.;
+  R.rho = rhoj;
+  R.rho1 = rho1j;
   if (dc.mhd) {
+    FRecB M;
     const double rhom = rhoh_d(hh, dc.hfact, pm);  // force.F90:1439-1443
     const double *Be = Bevol + (size_t)o * 4;
-    Bex[p] = Be[0]; Bey[p] = Be[1]; Bez[p] = Be[2];
-    Bxj = Be[0] * rhom; Byj = Be[1] * rhom; Bzj = Be[2] * rhom;
-    Bx[p] = Bxj; By[p] = Byj; Bz[p] = Bzj;
-    psi[p] = Be[3];
-    divB[p] = (double)divcurlB[(size_t)o * 4];
+    M.Bex = Be[0]; M.Bey = Be[1]; M.Bez = Be[2];
+    const double Bxj = Be[0] * rhom, Byj = Be[1] * rhom, Bzj = Be[2] * rhom;
+    M.Bx = Bxj; M.By = Byj; M.Bz = Bzj;
+    M.psi = Be[3];
     // get_stress MHD branch (force.F90:2136-2158)
     const double Brx = Bxj * rho1j, Bry = Byj * rho1j, Brz = Bzj * rho1j;
     const double Bro2 = Brx * Brx + Bry * Bry + Brz * Brz;
     const double va2 = Bro2 * rhoj;
-    vwave[p] = sqrt(csj * csj + va2);
-    sxx[p] = -(pm * Brx * Brx); sxy[p] = -(pm * Brx * Bry); sxz[p] = -(pm * Brx * Brz);
-    syy[p] = -(pm * Bry * Bry); syz[p] = -(pm * Bry * Brz); szz[p] = -(pm * Brz * Brz);
-    pro2[p] = prj * rho1j * rho1j + 0.5 * Bro2;
+    R.vwave = sqrt(csj * csj + va2);
+    M.sxx = -(pm * Brx * Brx); M.sxy = -(pm * Brx * Bry); M.sxz = -(pm * Brx * Brz);
+    M.syy = -(pm * Bry * Bry); M.syz = -(pm * Bry * Brz); M.szz = -(pm * Brz * Brz);
+    M.pad0 = M.pad1 = M.pad2 = 0.;
+    R.pro2 = prj * rho1j * rho1j + 0.5 * Bro2;
+    recB[p] = M;
   } else {
-    vwave[p] = csj;
-    pro2[p] = prj * rho1j * rho1j;
+    R.vwave = csj;
+    R.pro2 = prj * rho1j * rho1j;
   }
+  rec[p] = R;
   if (dc.dust) {
+    FRecD D;
     const double e = dustfrac[o];
-    eps[p] = e;
-    sqeps[p] = sqrt(e / (1. - e));
+    D.eps = e;
+    D.sqeps = sqrt(e / (1. - e));
     double tsj = get_ts_d(dc, rhoj * (1. - e), rhoj * e);
     if (dc.ilimitdustflux) tsj = fmin(tsj, hj / csj);
-    ts[p] = tsj;
+    D.ts = tsj;
+    D.pad = 0.;
+    recD[p] = D;
   }
 }
 
-// One active particle; returns false if its neighbour list overflowed.
+// One active particle given its neighbour list: pair terms and
+// finish_cell_and_store_results.  Returns dtcourant and the force/dust/
+// cleaning timestep minimum.
 template <int K, bool MHD, bool DUST, bool THERMAL>
-__device__ __forceinline__ bool force_one(
-    const DevConst &dc, int t, const int32_t *__restrict__ act, int nact, const int32_t *__restrict__ pleaf,
-    const uint32_t *__restrict__ cand, const int32_t *__restrict__ ncand, int capc,
-    const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi, const int32_t *__restrict__ iparts,
-    const JTerms &J, const double *__restrict__ h, const double *__restrict__ cs, const double *__restrict__ Bex,
-    const double *__restrict__ Bey, const double *__restrict__ Bez, uint32_t *__restrict__ nlist, int capl,
-    const ForceIOArgs &out, int32_t *__restrict__ flags, double &dtc_out, double &dtf_out, int &nl_out) {
-  const int p = act[t];
-  const int l = pleaf[p];
+__device__ __forceinline__ void force_pairs(const DevConst &dc, int p, int t, int nl, size_t nrows,
+                                            const uint32_t *__restrict__ nlist, const int32_t *__restrict__ iparts,
+                                            const FRec *__restrict__ rec, const FRecB *__restrict__ recB,
+                                            const FRecD *__restrict__ recD, const ForceIOArgs &out,
+                                            int32_t *__restrict__ flags, double &dtc_out, double &dtf_out) {
   const double pm = dc.pmass;
   const double R2 = dc.radkern2;
-  const double xi = J.x[p], yi = J.y[p], zi = J.z[p];
-  const double hi = h[p];
+  const FRec Ri = rec[p];
+  const double xi = Ri.x, yi = Ri.y, zi = Ri.z;
+  const double hi = Ri.h;
   // rhoanddhdrho (part.F90:805-816)
   const double hi1 = 1. / fabs(hi);
   const double tt = dc.hfact * hi1;
   const double rhoi = pm * (tt * tt * tt);
   const double rho1i = 1. / rhoi;
   const double hi21 = xmul(hi1, hi1), hi41 = hi21 * hi21;
-  double gradhi = (double)J.gradh[p];
+  double gradhi = (double)Ri.gradh;
   if (!(gradhi > 0.)) gradhi = 1.;  // force.F90:2616-2621
   const double hfacgrkern = hi41 * dc.cnormk * gradhi;
-  const double vxi = J.vx[p], vyi = J.vy[p], vzi = J.vz[p], eni = J.u[p];
-  const double pri = J.pr[p], csi = cs[p], alphai = (double)J.alpha[p];
+  const double vxi = Ri.vx, vyi = Ri.vy, vzi = Ri.vz, eni = Ri.u;
+  const double pri = Ri.pr, csi = Ri.cs, alphai = (double)Ri.alpha;
   // i terms: get_stress with rho from rhoanddhdrho
   double Bxi = 0., Byi = 0., Bzi = 0., psii = 0., pro2i, vwavei;
   double sxi = 0., sxyi = 0., sxzi = 0., syi = 0., syzi = 0., szi = 0.;
   if (MHD) {
-    Bxi = Bex[p] * rhoi; Byi = Bey[p] * rhoi; Bzi = Bez[p] * rhoi;
-    psii = J.psi[p];
+    const FRecB Mi = recB[p];
+    Bxi = Mi.Bex * rhoi; Byi = Mi.Bey * rhoi; Bzi = Mi.Bez * rhoi;
+    psii = Mi.psi;
     const double Brx = Bxi * rho1i, Bry = Byi * rho1i, Brz = Bzi * rho1i;
     const double Bro2 = Brx * Brx + Bry * Bry + Brz * Brz;
     vwavei = sqrt(csi * csi + Bro2 * rhoi);
@@ -146,7 +169,7 @@ __device__ __forceinline__ bool force_one(
   }
   double epsi = 0., tsi = 0., epstsi = 0., sqepsi = 0.;
   if (DUST) {
-    epsi = J.eps[p];
+    epsi = recD[p].eps;
     tsi = get_ts_d(dc, rhoi * (1. - epsi), rhoi * epsi);
     if (dc.ilimitdustflux) tsi = fmin(tsi, hi / csi);
     epstsi = epsi * tsi;
@@ -156,62 +179,21 @@ __device__ __forceinline__ bool force_one(
   const double mrhoi5 = 0.5 * pm * rho1i;
   const double auterm = mrhoi5 * dc.alphau;
   const double avBterm = mrhoi5 * dc.alphaB * rho1i;
-  // --- pass 1: symmetric neighbour set ---
-  int nl = 0;
-  const int nc = ncand[l];
-  for (int k = 0; k < nc; k++) {
-    const uint32_t e = cand[(size_t)l * capc + k];
-    const int cn = cand_node(e), code = cand_code(e);
-    int sx, sy, sz;
-    off_decode(code, sx, sy, sz);
-    const double ox = sx ? copysign(dc.dxbound, (double)sx) : 0.;
-    const double oy = sy ? copysign(dc.dybound, (double)sy) : 0.;
-    const double oz = sz ? copysign(dc.dzbound, (double)sz) : 0.;
-    const int j1 = nd_hi[cn];
-    for (int j = nd_lo[cn]; j <= j1; j++) {
-      if (j == p) continue;
-      double dx = xsub(xi, xadd(J.x[j], ox)), dy = xsub(yi, xadd(J.y[j], oy)), dz = xsub(zi, xadd(J.z[j], oz));
-      if (dc.periodic) {
-        dx = min_image(dx, dc.dxbound, 0.5 * dc.dxbound);
-        dy = min_image(dy, dc.dybound, 0.5 * dc.dybound);
-        dz = min_image(dz, dc.dzbound, 0.5 * dc.dzbound);
-      }
-      const double r2 = r2_exact(dx, dy, dz);
-      const double hj1 = J.hinv[j];
-      if (xmul(r2, hi21) < R2 || xmul(r2, xmul(hj1, hj1)) < R2) {
-        if (nl < capl) nlist[(size_t)nl * nact + t] = (uint32_t)j | ((uint32_t)code << 27);
-        nl++;
-      }
-    }
-  }
-  nl_out = nl;
-  if (nl > capl) {
-    atomicOr(flags, FFL_LIST);
-    return false;
-  }
-  // --- pass 2: pair terms (force.F90:1227-2056) ---
+  // --- pair terms (force.F90:1227-2056) ---
   double fx = 0., fy = 0., fz = 0., drhodt = 0., dudtdiss = 0., dendtdiss = 0.;
   double divBsym = 0., dBx_s = 0., dBy_s = 0., dBz_s = 0., divBdiff = 0.;
   double ddust = 0., dudtdust = 0., dvx_s = 0., dvy_s = 0., dvz_s = 0.;
   double vsigmax = 0.;
   const bool usej_always = MHD || DUST || (THERMAL && !dc.gravity);
   for (int k = 0; k < nl; k++) {
-    const uint32_t e = nlist[(size_t)k * nact + t];
+    const uint32_t e = list_get(nlist, nrows, t, k);
     const int j = (int)(e & 0x07ffffffu);
-    int sx, sy, sz;
-    off_decode((int)(e >> 27), sx, sy, sz);
-    const double ox = sx ? copysign(dc.dxbound, (double)sx) : 0.;
-    const double oy = sy ? copysign(dc.dybound, (double)sy) : 0.;
-    const double oz = sz ? copysign(dc.dzbound, (double)sz) : 0.;
-    double dx = xsub(xi, xadd(J.x[j], ox)), dy = xsub(yi, xadd(J.y[j], oy)), dz = xsub(zi, xadd(J.z[j], oz));
-    if (dc.periodic) {
-      dx = min_image(dx, dc.dxbound, 0.5 * dc.dxbound);
-      dy = min_image(dy, dc.dybound, 0.5 * dc.dybound);
-      dz = min_image(dz, dc.dzbound, 0.5 * dc.dzbound);
-    }
+    const FRec &Rj = rec[j];
+    double dx, dy, dz;
+    sep_of(dc, (int)(e >> 27), xi, yi, zi, Rj.x, Rj.y, Rj.z, dx, dy, dz);
     const double rij2 = r2_exact(dx, dy, dz);
     const double q2i = xmul(rij2, hi21);
-    const double hj1 = J.hinv[j];
+    const double hj1 = Rj.hinv;
     const double hj21 = xmul(hj1, hj1);
     const double q2j = xmul(rij2, hj21);
     double rij1, qi;
@@ -227,31 +209,31 @@ __device__ __forceinline__ bool force_one(
     bool usej = false;
     if (q2j < R2) {
       const double qj = (rij2 * rij1) * hj1;
-      grkernj = kern_g<K, double>(q2j, qj) * hj21 * hj21 * dc.cnormk * (double)J.gradh[j];
+      grkernj = kern_g<K, double>(q2j, qj) * hj21 * hj21 * dc.cnormk * (double)Rj.gradh;
       usej = true;
     }
     if (usej_always) usej = true;
     const double runix = dx * rij1, runiy = dy * rij1, runiz = dz * rij1;
-    const double dvx = vxi - J.vx[j], dvy = vyi - J.vy[j], dvz = vzi - J.vz[j];
+    const double dvx = vxi - Rj.vx, dvy = vyi - Rj.vy, dvz = vzi - Rj.vz;
     const double projv = dvx * runix + dvy * runiy + dvz * runiz;
-    const double denij = THERMAL ? (eni - J.u[j]) : 0.;
+    const double denij = THERMAL ? (eni - Rj.u) : 0.;
     const double vsigi = fmax(vwavei - dc.beta * projv, 0.);
     const double vsigavi = fmax(alphai * vwavei - dc.beta * projv, 0.);
     vsigmax = fmax(vsigmax, vsigi);
     double rho1j = 0., rho21j = 0., pro2j = 0., vwavej = 0., vsigavj = 0., autermj = 0., avBtermj = 0.;
     double rhoj = 0., prj = 0.;
     if (usej) {
-      rhoj = J.rho[j];
-      rho1j = J.rho1[j];
+      rhoj = Rj.rho;
+      rho1j = Rj.rho1;
       rho21j = rho1j * rho1j;
-      prj = J.pr[j];
-      pro2j = J.pro2[j];
-      vwavej = J.vwave[j];
+      prj = Rj.pr;
+      pro2j = Rj.pro2;
+      vwavej = Rj.vwave;
       const double mrhoj5 = 0.5 * pm * rho1j;
       autermj = mrhoj5 * dc.alphau;
       avBtermj = mrhoj5 * dc.alphaB * rho1j;
       vsigmax = fmax(vsigmax, fmax(vwavej - dc.beta * projv, 0.));
-      vsigavj = fmax((double)J.alpha[j] * vwavej - dc.beta * projv, 0.);
+      vsigavj = fmax((double)Rj.alpha * vwavej - dc.beta * projv, 0.);
     }
     double qrho2i = 0., qrho2j = 0.;
     if (projv < 0.) {
@@ -270,7 +252,8 @@ __device__ __forceinline__ bool force_one(
     }
     double projsx = 0., projsy = 0., projsz = 0., dudtresist = 0.;
     if (MHD) {
-      const double Bxj = J.Bx[j], Byj = J.By[j], Bzj = J.Bz[j], psij = J.psi[j];
+      const FRecB &Mj = recB[j];
+      const double Bxj = Mj.Bx, Byj = Mj.By, Bzj = Mj.Bz, psij = Mj.psi;
       const double dBx = Bxi - Bxj, dBy = Byi - Byj, dBz = Bzi - Bzj;
       const double projBi = Bxi * runix + Byi * runiy + Bzi * runiz;
       const double projBj = Bxj * runix + Byj * runiy + Bzj * runiz;
@@ -290,11 +273,11 @@ __device__ __forceinline__ bool force_one(
       dBy_s += -termi * dvy + dBdissterm * dBy - dpsiterm * runiy;
       dBz_s += -termi * dvz + dBdissterm * dBz - dpsiterm * runiz;
       projsx = (sxi * runix + sxyi * runiy + sxzi * runiz) * grkerni +
-               (J.sxx[j] * runix + J.sxy[j] * runiy + J.sxz[j] * runiz) * grkernj;
+               (Mj.sxx * runix + Mj.sxy * runiy + Mj.sxz * runiz) * grkernj;
       projsy = (sxyi * runix + syi * runiy + syzi * runiz) * grkerni +
-               (J.sxy[j] * runix + J.syy[j] * runiy + J.syz[j] * runiz) * grkernj;
+               (Mj.sxy * runix + Mj.syy * runiy + Mj.syz * runiz) * grkernj;
       projsz = (sxzi * runix + syzi * runiy + szi * runiz) * grkerni +
-               (J.sxz[j] * runix + J.syz[j] * runiy + J.szz[j] * runiz) * grkernj;
+               (Mj.sxz * runix + Mj.syz * runiy + Mj.szz * runiz) * grkernj;
     }
     const double gradp = gradpi + gradpj;
     fx = fx - runix * gradp - projsx;
@@ -306,12 +289,13 @@ __device__ __forceinline__ bool force_one(
       dendtdiss += dendissterm;
     }
     if (DUST) {
-      const double epsj = J.eps[j];
-      const double tsj = J.ts[j];
+      const FRecD &Dj = recD[j];
+      const double epsj = Dj.eps;
+      const double tsj = Dj.ts;
       const double epstsj = epsj * tsj;
       if (epsi > 0. || epsj > 0.) {
         const double grkernav = 0.5 * (grkerni + grkernj);
-        const double dft = pm * J.sqeps[j] * rho1j *
+        const double dft = pm * Dj.sqeps * rho1j *
                            ((tsi - epstsi) * (1. - epsi) / (1. - epsi) + (tsj - epstsj) * (1. - epsj) / (1. - epsj)) *
                            (pri - prj) * grkernav * rij1;
         ddust -= dft;
@@ -387,42 +371,82 @@ __device__ __forceinline__ bool force_one(
   if (!(dtc > PH_TINY64) || !(dtc < PH_HUGE64)) atomicOr(flags, FFL_DTC);
   dtc_out = dtc;
   dtf_out = fmin(fmin(dtf, dtdusti), dtclean);
-  return true;
 }
 
 template <int K, bool MHD, bool DUST, bool THERMAL>
-__global__ void __launch_bounds__(256)
-k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *__restrict__ pleaf,
-        const uint32_t *__restrict__ cand, const int32_t *__restrict__ ncand, int capc,
-        const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi, const int32_t *__restrict__ iparts,
-        JTerms J, const double *__restrict__ h, const double *__restrict__ cs, const double *__restrict__ Bex,
-        const double *__restrict__ Bey, const double *__restrict__ Bez, uint32_t *__restrict__ nlist, int capl,
-        ForceIOArgs out, unsigned long long *__restrict__ dtred, int32_t *__restrict__ flags,
-        unsigned long long *__restrict__ npairs) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(64)
+k_force(DevConst dc, const int32_t *__restrict__ act, int nact, int ngroups, const int32_t *__restrict__ pleaf,
+        const int32_t *__restrict__ leaves, const uint32_t *__restrict__ gcand, const int32_t *__restrict__ gncand,
+        int capg, const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi,
+        const phsph_kdnode *__restrict__ nd, const int32_t *__restrict__ iparts, const double4 *__restrict__ fq,
+        const FRec *__restrict__ rec, const FRecB *__restrict__ recB, const FRecD *__restrict__ recD,
+        uint32_t *__restrict__ nlist, int capl, ForceIOArgs out, unsigned long long *__restrict__ dtred,
+        int32_t *__restrict__ flags) {
+  __shared__ StageLDS S;
+  __shared__ uint32_t lbuf[8][GROUP];
+  const int g = xcd_block(blockIdx.x, gridDim.x);
+  if (g >= ngroups) return;
+  const int lane = threadIdx.x;
+  const int t = g * GROUP + lane;
+  const bool live = t < nact;
+  const size_t nrows = (size_t)ngroups * GROUP;
+  const double R2 = dc.radkern2;
+  int p = 0;
+  double xi = 0., yi = 0., zi = 0., hi21 = 0., lcx = 0., lcy = 0., lcz = 0., lsz = 0.;
+  if (live) {
+    p = act[t];
+    const int c = leaves[pleaf[p]];
+    const double4 r = fq[p];
+    xi = r.x; yi = r.y; zi = r.z;
+    const double hi1 = 1. / fabs(rec[p].h);
+    hi21 = xmul(hi1, hi1);
+    lcx = nd[c].xcen[0]; lcy = nd[c].xcen[1]; lcz = nd[c].xcen[2]; lsz = nd[c].size;
+  }
+  // --- pass 1: symmetric neighbour set (force.F90:1250-1290) ---
+  ListWriter L{nlist, nrows, t, capl, 0, lbuf, lane};
+  const int nc = gncand[g];
+  const uint32_t *gc = gcand + (size_t)g * capg;
+  for (int b0 = 0; b0 < nc; b0 += SB_LEAVES) {
+    const int nb = stage_batch(S, gc, b0, nc, nd, nd_lo, nd_hi, fq, lane);
+    for (int cl = 0; cl < nb; cl++) {
+      const int code = leaf_sep_code(dc, S, cl, lcx, lcy, lcz, lsz);
+      const int s1 = S.lstart[cl + 1];
+      for (int s = S.lstart[cl]; s < s1; s++) {
+        const double4 r = S.pos[s];
+        const int j = S.jpos[s];
+        double dx, dy, dz;
+        sep_of(dc, code, xi, yi, zi, r.x, r.y, r.z, dx, dy, dz);
+        const double r2 = r2_exact(dx, dy, dz);
+        const bool nb_ = xmul(r2, hi21) < R2 || xmul(r2, xmul(r.w, r.w)) < R2;
+        if (live && j != p && nb_) L.push(list_pack(j, code));
+      }
+    }
+    __syncthreads();
+  }
+  L.finish();
+  const int nl = live ? L.n : 0;
   double dtc = PH_HUGE64, dtfm = PH_HUGE64;
-  int nl = 0;
-  if (t < nact) {
-    if (!force_one<K, MHD, DUST, THERMAL>(dc, t, act, nact, pleaf, cand, ncand, capc, nd_lo, nd_hi, iparts, J, h,
-                                          cs, Bex, Bey, Bez, nlist, capl, out, flags, dtc, dtfm, nl)) {
-      dtc = PH_HUGE64;
-      dtfm = PH_HUGE64;
+  if (live) {
+    if (nl > capl) {
+      atomicOr(flags, FFL_LIST);
+    } else {
+      force_pairs<K, MHD, DUST, THERMAL>(dc, p, t, nl, nrows, nlist, iparts, rec, recB, recD, out, flags, dtc,
+                                         dtfm);
     }
   }
   unsigned long long np64 = (unsigned long long)nl;
   for (int off = 32; off > 0; off >>= 1) np64 += __shfl_xor(np64, off, 64);
-  if ((threadIdx.x & 63) == 0) atomicAdd(npairs, np64);
-  // global minima (force.F90:3312-3327): positive doubles order as integers;
-  // every lane of the wave takes part in the shuffles
+  // global minima (force.F90:3312-3327): positive doubles order as integers
   unsigned long long a = __double_as_longlong(dtc), b = __double_as_longlong(dtfm);
   for (int off = 32; off > 0; off >>= 1) {
     unsigned long long a2 = __shfl_xor(a, off, 64), b2 = __shfl_xor(b, off, 64);
     a = a < a2 ? a : a2;
     b = b < b2 ? b : b2;
   }
-  if ((threadIdx.x & 63) == 0) {
+  if (lane == 0) {
     atomicMin(&dtred[0], a);
     atomicMin(&dtred[1], b);
+    atomicAdd(&dtred[2], np64);
   }
 }
 
@@ -524,28 +548,20 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
   (void)icall;
   hipStream_t s = c->stream;
   const int n = c->nlive;
-  ForceBufs &B = c->fb;
-  DBuf<double> *dl[] = {&B.hinv, &B.u, &B.rho, &B.rho1, &B.pr, &B.pro2, &B.vwave, &B.cs, &B.h};
-  for (auto *d : dl) PH_HIPCHECK(c, d->ensure(n));
-  PH_HIPCHECK(c, c->tvx.ensure(n)); PH_HIPCHECK(c, c->tvy.ensure(n)); PH_HIPCHECK(c, c->tvz.ensure(n));
-  PH_HIPCHECK(c, c->tgradh.ensure(n)); PH_HIPCHECK(c, c->talpha.ensure(n));
   const bool mhd = c->p.mhd, dust = c->p.use_dustfrac && c->p.ndustsmall > 0;
-  if (mhd) {
-    DBuf<double> *ml[] = {&B.Bx, &B.By, &B.Bz, &B.psi, &B.Bex, &B.Bey, &B.Bez, &B.divB,
-                          &B.s[0], &B.s[1], &B.s[2], &B.s[3], &B.s[4], &B.s[5]};
-    for (auto *d : ml) PH_HIPCHECK(c, d->ensure(n));
-  }
-  if (dust) {
-    PH_HIPCHECK(c, B.eps.ensure(n)); PH_HIPCHECK(c, B.sqeps.ensure(n)); PH_HIPCHECK(c, B.ts.ensure(n));
-  }
+  PH_HIPCHECK(c, c->fq4.ensure(n));
+  PH_HIPCHECK(c, c->frec.ensure((size_t)n * sizeof(FRec)));
+  if (mhd) PH_HIPCHECK(c, c->frecB.ensure((size_t)n * sizeof(FRecB)));
+  if (dust) PH_HIPCHECK(c, c->frecD.ensure((size_t)n * sizeof(FRecD)));
+  PH_HIPCHECK(c, c->fh.ensure(n));
+  FRec *rec = (FRec *)c->frec.p;
+  FRecB *recB = mhd ? (FRecB *)c->frecB.p : nullptr;
+  FRecD *recD = dust ? (FRecD *)c->frecD.p : nullptr;
   {
     LaunchTimer lt(c, "force_prep");
-    hipLaunchKernelGGL(k_force_prep, dim3(nblocks(n, 256)), dim3(256), 0, s, c->dc, n, c->iparts.p, a.xyzh, a.ldx,
-                       a.vxyzu, a.ldv, a.gradh, a.ldgradh, a.alphaind, a.eos_vars, a.ldeos, a.Bevol, a.dustfrac,
-                       a.divcurlB, B.h.p, B.hinv.p, c->tvx.p, c->tvy.p, c->tvz.p, B.u.p, c->tgradh.p, c->talpha.p,
-                       B.rho.p, B.rho1.p, B.pr.p, B.cs.p, B.pro2.p, B.vwave.p, B.Bx.p, B.By.p, B.Bz.p, B.psi.p,
-                       B.Bex.p, B.Bey.p, B.Bez.p, B.s[0].p, B.s[1].p, B.s[2].p, B.s[3].p, B.s[4].p, B.s[5].p,
-                       B.eps.p, B.sqeps.p, B.ts.p, B.divB.p);
+    hipLaunchKernelGGL(k_force_prep, dim3(nblocks(n, 256)), dim3(256), 0, s, c->dc, n, c->iparts.p, c->tx.p,
+                       c->ty.p, c->tz.p, a.xyzh, a.ldx, a.vxyzu, a.ldv, a.gradh, a.ldgradh, a.alphaind, a.eos_vars,
+                       a.ldeos, a.Bevol, a.dustfrac, c->fh.p, c->fq4.p, rec, recB, recD);
   }
   PH_HIPCHECK(c, hipGetLastError());
   PH_HIPCHECK(c, c->tmpi.ensure(n));
@@ -559,15 +575,13 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
   hipLaunchKernelGGL(k_zero_u64, dim3(nblocks(c->ncells + 2, 256)), dim3(256), 0, s,
                      (unsigned long long *)c->hmaxf.p, (int64_t)(c->ncells + 2));
   hipLaunchKernelGGL(k_leaf_hmax, dim3(nblocks(c->nleaves, 256)), dim3(256), 0, s, c->nleaves, c->leaves.p,
-                     c->nd_lo.p, c->nd_hi.p, c->nd.p, B.h.p, (unsigned long long *)c->hmaxf.p);
-  rc = walk_leaves(c, 1, 1.0, c->hmaxf.p);
+                     c->nd_lo.p, c->nd_hi.p, c->nd.p, c->fh.p, (unsigned long long *)c->hmaxf.p);
+  rc = walk_groups(c, 1, 1.0, c->fh.p, c->hmaxf.p);
   if (rc) return rc;
-  JTerms J{c->tx.p, c->ty.p, c->tz.p, B.hinv.p, c->tvx.p, c->tvy.p, c->tvz.p, B.u.p, c->tgradh.p, c->talpha.p,
-           B.rho.p, B.rho1.p, B.pr.p, B.pro2.p, B.vwave.p, B.Bx.p, B.By.p, B.Bz.p, B.psi.p,
-           B.s[0].p, B.s[1].p, B.s[2].p, B.s[3].p, B.s[4].p, B.s[5].p, B.eps.p, B.sqeps.p, B.ts.p};
+  const int ng = c->ngroups;
   PH_HIPCHECK(c, c->dtred.ensure(3));
   for (int attempt = 0; attempt < 6; attempt++) {
-    PH_HIPCHECK(c, c->nlist.ensure((size_t)c->caps.nlist * (nact + 1)));
+    PH_HIPCHECK(c, c->nlist.ensure((size_t)c->caps.nlist * ((size_t)ng * GROUP + 1)));
     unsigned long long init[3];
     double big = PH_HUGE64;
     memcpy(&init[0], &big, 8);
@@ -578,13 +592,12 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
     PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
     if (nact > 0) {
       LaunchTimer lt(c, "force");
-      const dim3 g(nblocks(nact, 256)), b(256);
+      const dim3 gr(xcd_grid(ng)), b(64);
       const bool th = c->p.maxvxyzu >= 4;
-#define FL(KK, M, D, T)                                                                                          \
-  hipLaunchKernelGGL((k_force<KK, M, D, T>), g, b, 0, s, c->dc, c->act.p, nact, c->pleaf.p, c->cand.p, c->ncand.p,  \
-                     c->caps.cand, c->nd_lo.p, c->nd_hi.p, c->iparts.p, J, B.h.p, B.cs.p, B.Bex.p, B.Bey.p,         \
-                     B.Bez.p, c->nlist.p, c->caps.nlist, a, c->dtred.p, c->counters.p,                    \
-                     (unsigned long long *)c->dtred.p + 2)
+#define FL(KK, M, D, T)                                                                                        \
+  hipLaunchKernelGGL((k_force<KK, M, D, T>), gr, b, 0, s, c->dc, c->act.p, nact, ng, c->pleaf.p, c->leaves.p,    \
+                     c->gcand.p, c->gncand.p, c->caps.gcand, c->nd_lo.p, c->nd_hi.p, c->nd.p, c->iparts.p,     \
+                     c->fq4.p, rec, recB, recD, c->nlist.p, c->caps.nlist, a, c->dtred.p, c->counters.p)
 #define FLK(KK)                                          \
   if (mhd && dust) { if (th) FL(KK, true, true, true); else FL(KK, true, true, false); }     \
   else if (mhd) { if (th) FL(KK, true, false, true); else FL(KK, true, false, false); }      \
@@ -601,7 +614,7 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
     PH_HIPCHECK(c, hipMemcpyAsync(red, c->dtred.p, sizeof(red), hipMemcpyDeviceToHost, s));
     PH_HIPCHECK(c, hipStreamSynchronize(s));
     if (fl & FFL_LIST) {
-      c->caps.nlist = c->caps.nlist * 3 / 2;
+      c->caps.nlist = ((c->caps.nlist * 3 / 2) + 7) & ~7;
       continue;
     }
     if (fl & FFL_DTC) PH_RET(c, PHSPH_ERR_INPUT, "force: invalid dtc");

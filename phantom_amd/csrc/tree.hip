@@ -699,5 +699,10 @@ int tree_build(phsph_ctx *c, double *d_xyzh, int ld, int np, const int8_t *d_iph
                      c->nd_lo.p, c->nd_hi.p, c->pleaf.p);
   PH_HIPCHECK(c, hipGetLastError());
   c->tree_valid = true;
+  // statistics of a density call belong to the tree it ran on
+  c->nact_dens = 0;
+  c->trial_stale = true;
+  c->nrhocalc = -1;
+  c->nactualtot = -1;
   return PHSPH_OK;
 }
