@@ -210,6 +210,40 @@ int phsph_force(phsph_ctx *ctx, int32_t icall, int32_t np,
                 const double *eos_vars, int32_t ldeos, const int8_t *iphase,
                 double *dtcourant, double *dtforce);
 
+/*
+ * Block (individual) timesteps.  The reference keeps the bins as module
+ * state (part:ibin, ibin_old, ibin_wake; timestep_ind:nbinmax, istepfrac,
+ * ibinnow); here they are registered on the context and used by every
+ * phsph_force call while params.ind_timesteps is set.  The arrays must stay
+ * valid until the next phsph_set_bins or phsph_destroy.  phsph_force then,
+ * per active particle (force.F90:3272-3310): takes dti = min of the force,
+ * cleaning and dust timesteps if below dtc and dtmax, places it with
+ * get_newbin (utils_indtimesteps.f90:230-287, decreases allowed when
+ * icall < 2), applies the Saitoh-Makino limiter ibin >= max(ibin_old of the
+ * neighbours) - 1, and flags every neighbour j with
+ * ibin_wake(j) = max(ibinnow-1, ibin_wake(j)) (force.F90:1346-1352).
+ * dtcourant and dtforce are returned as 1e29, as in the reference.
+ */
+int phsph_set_bins(phsph_ctx *ctx, int8_t *ibin, const int8_t *ibin_old,
+                   int8_t *ibin_wake, int32_t nbinmax, int32_t istepfrac,
+                   int32_t ibinnow);
+
+/* nbinmax after the last phsph_force with block timesteps: the largest
+ * bin of its active particles (force.F90:792). */
+int phsph_get_nbinmax(phsph_ctx *ctx, int32_t *nbinmax);
+
+/*
+ * set_active_particles (utils_indtimesteps.f90:114-185): a live particle
+ * (h > 0) is active when mod(istepfrac, 2**(nbinmax - ibin)) == 0; iphase is
+ * set to +1 (active) or -1 (inactive) for gas.  Returns the counts and
+ * ibinnow, the largest active bin.  A bin above nbinmax is an error.
+ */
+int phsph_set_active_particles(phsph_ctx *ctx, int32_t np, const double *xyzh,
+                               int32_t ldxyzh, int8_t *iphase,
+                               const int8_t *ibin, int32_t nbinmax,
+                               int32_t istepfrac, int32_t *nactive,
+                               int32_t *nalive, int32_t *ibinnow);
+
 /* get_neighbour_stats (dens.F90:1137-1155) of the last densityiterate. */
 int phsph_get_neighbour_stats(phsph_ctx *ctx, double *trialmean,
                               double *actualmean, int32_t *maxtrial,

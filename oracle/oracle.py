@@ -51,6 +51,15 @@ def lib():
                                 vp, vp, vp, vp, vp, vp, vp, vp, C.c_int32, vp, vp, C.c_int32, vp,
                                 P(C.c_double), P(C.c_double)]
         L.orc_kernel.argtypes = [C.c_int32, C.c_double, P(C.c_double), P(C.c_double), P(C.c_double)]
+        L.orc_get_newbin.argtypes = [C.c_double, C.c_double, P(C.c_int8), C.c_int, C.c_int32, C.c_int32]
+        L.orc_get_newbin.restype = None
+        L.orc_set_bins.argtypes = [vp, vp, vp, vp, C.c_int32, C.c_int32, C.c_int32]
+        L.orc_set_bins.restype = None
+        L.orc_get_nbinmax.argtypes = [vp]
+        L.orc_get_nbinmax.restype = C.c_int32
+        L.orc_set_active_particles.argtypes = [C.c_int32, vp, vp, vp, C.c_int32, C.c_int32, P(C.c_int32),
+                                               P(C.c_int32)]
+        L.orc_set_active_particles.restype = C.c_int32
         L.orc_kernel_softening.argtypes = [C.c_int32, C.c_double, P(C.c_double), P(C.c_double)]
         _lib = L
     return _lib
@@ -64,6 +73,27 @@ class OracleError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{abi.ERROR_NAMES.get(code, code)}: {msg}")
         self.code = code
+
+
+def get_newbin(dti, dtmax, ibin, allow_decrease=True, nbinmax=30, istepfrac=0):
+    """get_newbin (utils_indtimesteps.f90:230-287): the bin after the step."""
+    b = C.c_int8(ibin)
+    lib().orc_get_newbin(float(dti), float(dtmax), C.byref(b), int(bool(allow_decrease)), nbinmax, istepfrac)
+    return b.value
+
+
+def set_active_particles(xyzh, ibin, nbinmax, istepfrac):
+    """set_active_particles (utils_indtimesteps.f90:114-185): returns
+    (iphase, nactive, nalive, ibinnow)."""
+    n = xyzh.shape[0]
+    iphase = np.ones(n, dtype=np.int8)
+    na, nl = C.c_int32(), C.c_int32()
+    ibin = np.ascontiguousarray(ibin, dtype=np.int8)
+    ibinnow = lib().orc_set_active_particles(n, _ptr(np.ascontiguousarray(xyzh)), _ptr(iphase), _ptr(ibin),
+                                             nbinmax, istepfrac, C.byref(na), C.byref(nl))
+    if ibinnow < 0:
+        raise OracleError(-1, "set_active_particles: timestep bin exceeds max bins")
+    return iphase, na.value, nl.value, ibinnow
 
 
 def kernel(kind, q):
@@ -147,6 +177,15 @@ class Oracle:
         return dict(trialmean=tm.value, actualmean=am.value, maxtrial=mt.value, maxactual=ma.value,
                     nrhocalc=nr.value, nactualtot=na.value)
 
+    def set_bins(self, ibin, ibin_old, ibin_wake, nbinmax, istepfrac, ibinnow):
+        """Register the block-timestep arrays used by force (part:ibin,
+        ibin_old, ibin_wake).  ibin and ibin_wake are updated in place."""
+        self._bins = (ibin, ibin_old, ibin_wake)
+        lib().orc_set_bins(self._ctx, _ptr(ibin), _ptr(ibin_old), _ptr(ibin_wake), nbinmax, istepfrac, ibinnow)
+
+    def nbinmax(self):
+        return lib().orc_get_nbinmax(self._ctx)
+
     def cons2prim(self, xyzh, vxyzu, dvdx, alphaind, dustevol=None, iphase=None):
         n = xyzh.shape[0]
         eos_vars = np.zeros((n, 3))
@@ -185,11 +224,19 @@ def run_all(setup, icall=1):
     xyzh = setup["xyzh"].copy()
     vxyzu = setup["vxyzu"]
     alphaind = setup["alphaind"].copy()
-    o.maketree(xyzh)
+    o.maketree(xyzh, iphase=setup.get("iphase"))
     tree = o.get_tree()
-    d = o.densityiterate(xyzh, vxyzu, Bevol=setup.get("Bevol"), icall=icall, alphaind=alphaind)
+    d = o.densityiterate(xyzh, vxyzu, Bevol=setup.get("Bevol"), icall=icall, alphaind=alphaind,
+                         iphase=setup.get("iphase"))
     c = o.cons2prim(xyzh, vxyzu, d["dvdx"], alphaind, dustevol=setup.get("dustevol"))
+    bins = setup.get("bins")
+    if bins is not None:
+        ibin = bins["ibin"].copy()
+        ibin_wake = bins["ibin_wake"].copy()
+        o.set_bins(ibin, bins["ibin_old"], ibin_wake, bins["nbinmax"], bins["istepfrac"], bins["ibinnow"])
     f = o.force(xyzh, vxyzu, d["gradh"], alphaind, c["eos_vars"], divcurlB=d["divcurlB"],
-                Bevol=setup.get("Bevol"), dustfrac=c["dustfrac"])
+                Bevol=setup.get("Bevol"), dustfrac=c["dustfrac"], iphase=setup.get("iphase"))
+    if bins is not None:
+        f.update(ibin=ibin, ibin_wake=ibin_wake, nbinmax=o.nbinmax())
     return dict(oracle=o, xyzh=xyzh, tree=tree, dens=d, c2p=c, force=f, stats=o.neighbour_stats(),
                 alphaind=alphaind)

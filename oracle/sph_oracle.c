@@ -34,6 +34,7 @@
 #define ISIZECELLCACHE 1000              /* dens.F90:99, force.F90:56     */
 #define MINPART 10                       /* config.F90:113                */
 #define ISTACKSIZE 512                   /* kdtree.F90:131                */
+#define MAXBINS 30                       /* utils_indtimesteps.f90:29     */
 
 /* rhosum indices (dens.F90:51-93), 0-based */
 enum { R_RHO = 0, R_GRADH, R_GRADSOFT, R_DIVV,
@@ -62,6 +63,10 @@ typedef struct {
   /* neighbour statistics (dens.F90:106-108) */
   int64_t nneightry, nneighact, maxneightry, ncalc, nptot;
   int32_t maxneighact;
+  /* block timesteps (part:ibin, ibin_old, ibin_wake; timestep_ind module) */
+  int8_t *ibin, *ibin_wake;
+  const int8_t *ibin_old;
+  int32_t nbinmax, istepfrac, ibinnow, nbinmaxnew;
   char err[256];
 } orc_ctx;
 
@@ -1077,7 +1082,7 @@ enum { F_FX = 0, F_FY, F_FZ, F_POT, F_DRHODT, F_DUDTDISS, F_DENDTDISS, F_DIVBSYM
 
 /* compute_forces (force.F90:914-2060) for one active gas particle. */
 static void compute_forces(const orc_ctx *o, int32_t i, const fpart_t *pi, const nlist_t *l,
-                           const forcein_t *in, double *fsum, double *vsigmax) {
+                           const forcein_t *in, double *fsum, double *vsigmax, int *ibin_neighi) {
   const int kind = o->p.kernel;
   const double pm = o->p.massoftype_gas;
   const double beta = o->p.beta, alphau = o->p.alphau, alphaB = o->p.alphaB;
@@ -1104,6 +1109,9 @@ static void compute_forces(const orc_ctx *o, int32_t i, const fpart_t *pi, const
   const int useresistiveheat = o->p.iresistive_heating > 0;
   memset(fsum, 0, NFSUM * sizeof(double));
   *vsigmax = 0.;
+  *ibin_neighi = 0;
+  const int ind = o->p.ind_timesteps && o->ibin_old;
+  const int8_t ibinnow_m1 = (int8_t)(o->ibinnow - 1);
   for (int32_t n = 0; n < l->n; n++) {
     int32_t j = l->list[n];
     if (j == i) continue;
@@ -1142,6 +1150,15 @@ static void compute_forces(const orc_ctx *o, int32_t i, const fpart_t *pi, const
     if (mhd) usej = 1;
     if (dust) usej = 1;
     if (thermal && !o->p.gravity) usej = 1;
+    if (ind) {
+      /* force.F90:1346-1356: j is a neighbour of an active particle; flag
+         it to be woken, and note the largest bin among the neighbours */
+      if (o->ibin_wake[j - 1] < ibinnow_m1) {
+#pragma omp atomic write
+        o->ibin_wake[j - 1] = ibinnow_m1;
+      }
+      if (o->ibin_old[j - 1] > *ibin_neighi) *ibin_neighi = o->ibin_old[j - 1];
+    }
     double runix = dx * rij1, runiy = dy * rij1, runiz = dz * rij1;
     const double *vj = &in->vxyzu[(size_t)in->ldv * (j - 1)];
     double dvx = pi->vx - vj[0], dvy = pi->vy - vj[1], dvz = pi->vz - vj[2];
@@ -1271,6 +1288,9 @@ static void compute_forces(const orc_ctx *o, int32_t i, const fpart_t *pi, const
   }
 }
 
+void orc_get_newbin(double dti, double dtmax, int8_t *ibini, int allow_decrease, int32_t nbinmax,
+                    int32_t istepfrac);
+
 /* force (force.F90:193-905) + start_cell (:2172-2514) +
  * finish_cell_and_store_results (:2649-3330), global timestep build. */
 int orc_force(orc_ctx *o, int32_t icall, int32_t np, const double *xyzh, const double *vxyzu, int32_t ldv,
@@ -1279,7 +1299,6 @@ int orc_force(orc_ctx *o, int32_t icall, int32_t np, const double *xyzh, const d
               double *deltav, double *tstop, const float *gradh, int32_t ldgradh, const float *alphaind,
               const double *eos_vars, int32_t ldeos, const int8_t *iphase, double *dtcourant_out,
               double *dtforce_out) {
-  (void)icall;
   if (!o->tree_built || np != o->np) {
     snprintf(o->err, sizeof(o->err), "force: tree not built for this np");
     return PHSPH_ERR_STATE;
@@ -1289,8 +1308,14 @@ int orc_force(orc_ctx *o, int32_t icall, int32_t np, const double *xyzh, const d
   const int thermal = o->p.maxvxyzu >= 4;
   forcein_t in = {xyzh, vxyzu, Bevol, dustfrac, eos_vars, gradh, alphaind, divcurlB, ldv, ldgradh, ldeos};
   double dtcourant = BIGNUMBER, dtforce = BIGNUMBER;
+  int nbinmaxnew = 0;
+  const int ind = o->p.ind_timesteps != 0;
+  if (ind && (!o->ibin || !o->ibin_old || !o->ibin_wake)) {
+    snprintf(o->err, sizeof(o->err), "force: individual timesteps need orc_set_bins");
+    return PHSPH_ERR_STATE;
+  }
   int rc_all = PHSPH_OK;
-#pragma omp parallel reduction(min : dtcourant, dtforce)
+#pragma omp parallel reduction(min : dtcourant, dtforce) reduction(max : nbinmaxnew)
   {
     nlist_t l = {0};
 #pragma omp for schedule(dynamic, 64)
@@ -1346,7 +1371,8 @@ int orc_force(orc_ctx *o, int32_t icall, int32_t np, const double *xyzh, const d
         const fpart_t *q = &parts[k];
         int32_t i = gidx[k];
         double fsum[NFSUM], vsigmax;
-        compute_forces(o, i, q, &l, &in, fsum, &vsigmax);
+        int ibin_neighi;
+        compute_forces(o, i, q, &l, &in, fsum, &vsigmax, &ibin_neighi);
         /* finish_cell_and_store_results */
         double hi = q->h, hi1 = 1. / hi, rhoi = q->rho, rho1i = 1. / rhoi;
         double dtc = o->p.dtmax, dtf = BIGNUMBER, dtclean = BIGNUMBER, dtdusti = BIGNUMBER;
@@ -1415,18 +1441,88 @@ int orc_force(orc_ctx *o, int32_t icall, int32_t np, const double *xyzh, const d
           }
           if (tstop) tstop[i - 1] = q->tstop;
         }
-        if (dtc < dtcourant) dtcourant = dtc;
-        double m = dtf;
-        if (dtdusti < m) m = dtdusti;
-        if (dtclean < m) m = dtclean;
-        if (m < dtforce) dtforce = m;
+        if (ind) {
+          /* force.F90:3272-3310: per-particle timestep, new bin, limiter */
+          double dti = dtc;
+          double dtitmp = dtf;
+          if (dtclean < dtitmp) dtitmp = dtclean;
+          if (dtdusti < dtitmp) dtitmp = dtdusti;
+          if (dtitmp < dti + TINY64 && dtitmp < o->p.dtmax) dti = dtitmp;
+          int8_t b = o->ibin[i - 1];
+          orc_get_newbin(dti, o->p.dtmax, &b, icall < 2, o->nbinmax, o->istepfrac);
+          if (b < ibin_neighi - 1) b = (int8_t)(ibin_neighi - 1);
+          o->ibin[i - 1] = b;
+          if (b > nbinmaxnew) nbinmaxnew = b;
+        } else {
+          if (dtc < dtcourant) dtcourant = dtc;
+          double m = dtf;
+          if (dtdusti < m) m = dtdusti;
+          if (dtclean < m) m = dtclean;
+          if (m < dtforce) dtforce = m;
+        }
       }
     }
     free(l.list); free(l.cache);
   }
   if (dtcourant_out) *dtcourant_out = dtcourant;
   if (dtforce_out) *dtforce_out = dtforce;
+  o->nbinmaxnew = nbinmaxnew;
   return rc_all;
+}
+
+/* get_newbin (utils_indtimesteps.f90:230-287) */
+void orc_get_newbin(double dti, double dtmax, int8_t *ibini, int allow_decrease, int32_t nbinmax,
+                    int32_t istepfrac) {
+  const double dlog2 = 1.4426950408889634; /* 1/log(2) */
+  int8_t ibin_oldi = *ibini;
+  int ibin_newi;
+  if (dti > dtmax) ibin_newi = 0;
+  else if (dti < TINY64) ibin_newi = MAXBINS;
+  else {
+    ibin_newi = (int)(log(2. * dtmax / dti) * dlog2 - EPS64);
+    if (ibin_newi < 0) ibin_newi = 0;
+  }
+  if (ibin_newi > ibin_oldi) {
+    *ibini = (int8_t)ibin_newi;
+  } else if (ibin_newi < ibin_oldi && ibin_oldi <= nbinmax && allow_decrease) {
+    /* timestep can only go up if bins are synchronised, one bin at a time */
+    if (istepfrac % (1 << (nbinmax - (ibin_oldi - 1))) == 0) *ibini = (int8_t)(ibin_oldi - 1);
+  }
+}
+
+void orc_set_bins(orc_ctx *o, int8_t *ibin, const int8_t *ibin_old, int8_t *ibin_wake, int32_t nbinmax,
+                  int32_t istepfrac, int32_t ibinnow) {
+  o->ibin = ibin;
+  o->ibin_old = ibin_old;
+  o->ibin_wake = ibin_wake;
+  o->nbinmax = nbinmax;
+  o->istepfrac = istepfrac;
+  o->ibinnow = ibinnow;
+}
+
+int32_t orc_get_nbinmax(const orc_ctx *o) { return o->nbinmaxnew; }
+
+/* set_active_particles (utils_indtimesteps.f90:114-185): a live particle is
+ * active when mod(istepfrac, 2**(nbinmax-ibin)) == 0; iphase = +-1 (gas).
+ * Returns ibinnow, or -1 if a bin exceeds nbinmax. */
+int32_t orc_set_active_particles(int32_t np, const double *xyzh, int8_t *iphase, const int8_t *ibin,
+                                 int32_t nbinmax, int32_t istepfrac, int32_t *nactive, int32_t *nalive) {
+  int32_t na = 0, nl = 0;
+  for (int32_t i = 0; i < np; i++) {
+    if (!(xyzh[4 * (size_t)i + 3] > 0.)) continue;
+    nl++;
+    int b = ibin[i];
+    if (b > nbinmax) return -1;
+    int act = (istepfrac % (1 << (nbinmax - b))) == 0;
+    if (act) na++;
+    iphase[i] = act ? 1 : -1;
+  }
+  int32_t ibinnow = nbinmax;
+  for (int32_t k = 0; ibinnow == nbinmax && k < nbinmax; k++)
+    if (istepfrac % (1 << (nbinmax - k)) == 0) ibinnow = k;
+  if (nactive) *nactive = na;
+  if (nalive) *nalive = nl;
+  return ibinnow;
 }
 
 /* kernel_softening (kernel_cubic.f90:103-121, kernel_quintic.f90:124-152):

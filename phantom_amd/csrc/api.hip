@@ -98,7 +98,7 @@ struct Stager {
       *dev = ptr;
       return PHSPH_OK;
     }
-    if (slot >= 16) PH_RET(c, PHSPH_ERR_STATE, "internal: too many staged arrays");
+    if (slot >= 24) PH_RET(c, PHSPH_ERR_STATE, "internal: too many staged arrays");
     DBuf<char> &b = c->stage[slot++];
     const size_t bytes = count * sizeof(T);
     PH_HIPCHECK(c, b.ensure(bytes));
@@ -164,7 +164,6 @@ int phsph_create(const phsph_params *p, int32_t device, phsph_ctx **out) {
   }
   c->p = *p;
   c->dc = make_devconst(*p);
-  if (p->kernel == PHSPH_KERNEL_QUINTIC) c->caps.nlist = 384;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -382,9 +381,70 @@ int phsph_force(phsph_ctx *c, int32_t icall, int32_t np, const double *xyzh, int
   a.eos_vars = t; a.ldeos = ldeos;
   STAGE(st, const_cast<int8_t *>(iphase), (size_t)np, true, false, &ph);
   a.iphase = ph;
+  if (c->p.ind_timesteps) {
+    if (!c->bins_set) PH_RET(c, PHSPH_ERR_STATE, "force: individual timesteps need phsph_set_bins");
+    STAGE(st, c->bins_ibin, (size_t)np, true, true, &a.ibin);
+    STAGE(st, const_cast<int8_t *>(c->bins_ibin_old), (size_t)np, true, false, &ph);
+    a.ibin_old = ph;
+    STAGE(st, c->bins_ibin_wake, (size_t)np, true, true, &a.ibin_wake);
+    a.nbinmax = c->bins_nbinmax;
+    a.istepfrac = c->bins_istepfrac;
+    a.ibinnow = c->bins_ibinnow;
+  }
   int rc = force_run(c, icall, a, dtcourant, dtforce);
   int rc2 = st.finish();
   return rc ? rc : rc2;
+}
+
+int phsph_set_bins(phsph_ctx *c, int8_t *ibin, const int8_t *ibin_old, int8_t *ibin_wake, int32_t nbinmax,
+                   int32_t istepfrac, int32_t ibinnow) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (!ibin || !ibin_old || !ibin_wake) PH_RET(c, PHSPH_ERR_INPUT, "set_bins: missing array");
+  if (nbinmax < 0 || nbinmax > 30) PH_RET(c, PHSPH_ERR_INPUT, "set_bins: nbinmax outside 0..30");
+  c->bins_ibin = ibin;
+  c->bins_ibin_old = ibin_old;
+  c->bins_ibin_wake = ibin_wake;
+  c->bins_nbinmax = nbinmax;
+  c->bins_istepfrac = istepfrac;
+  c->bins_ibinnow = ibinnow;
+  c->bins_set = true;
+  return PHSPH_OK;
+}
+
+int phsph_get_nbinmax(phsph_ctx *c, int32_t *nbinmax) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (c->nbinmax_new < 0) PH_RET(c, PHSPH_ERR_STATE, "get_nbinmax: no force call with block timesteps");
+  if (nbinmax) *nbinmax = c->nbinmax_new;
+  return PHSPH_OK;
+}
+
+int phsph_set_active_particles(phsph_ctx *c, int32_t np, const double *xyzh, int32_t ldx, int8_t *iphase,
+                               const int8_t *ibin, int32_t nbinmax, int32_t istepfrac, int32_t *nactive,
+                               int32_t *nalive, int32_t *ibinnow) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (np < 0 || !xyzh || !iphase || !ibin || ldx < 4) PH_RET(c, PHSPH_ERR_INPUT, "set_active_particles: missing array");
+  if (nbinmax < 0 || nbinmax > 30) PH_RET(c, PHSPH_ERR_INPUT, "set_active_particles: nbinmax outside 0..30");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  ctx_clear_timings(c);
+  Stager st(c);
+  double *dx;
+  int8_t *dph, *db;
+  STAGE(st, const_cast<double *>(xyzh), (size_t)ldx * np, true, false, &dx);
+  STAGE(st, iphase, (size_t)np, true, true, &dph);
+  STAGE(st, const_cast<int8_t *>(ibin), (size_t)np, true, false, &db);
+  int32_t na = 0, nl = 0;
+  int rc = set_active_run(c, np, dx, ldx, dph, db, nbinmax, istepfrac, &na, &nl);
+  int rc2 = st.finish();
+  if (rc) return rc;
+  if (rc2) return rc2;
+  // the largest active bin (utils_indtimesteps.f90:170-176)
+  int32_t bnow = nbinmax;
+  for (int32_t k = 0; bnow == nbinmax && k < nbinmax; k++)
+    if (istepfrac % (1 << (nbinmax - k)) == 0) bnow = k;
+  if (nactive) *nactive = na;
+  if (nalive) *nalive = nl;
+  if (ibinnow) *ibinnow = bnow;
+  return PHSPH_OK;
 }
 
 int phsph_get_neighbour_stats(phsph_ctx *c, double *trialmean, double *actualmean, int32_t *maxtrial,

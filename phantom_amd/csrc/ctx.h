@@ -43,11 +43,10 @@ struct TimedLaunch {
   hipEvent_t t0, t1;
 };
 
-// Capacities of the per-group candidate lists and per-particle neighbour
-// lists.  They grow (and the pass is re-run) when a launch reports overflow.
+// Capacity of the per-group candidate lists.  It grows (and the walk is
+// re-run) when a launch reports overflow.
 struct Caps {
   int gcand = 512;   // candidate leaves per group of 64 particles
-  int nlist = 256;   // neighbours per particle (a multiple of 8)
 };
 
 struct phsph_ctx {
@@ -90,12 +89,10 @@ struct phsph_ctx {
   DBuf<int32_t> gncand;      // count per group
   int ngroups = 0;
   DBuf<int32_t> ntrial;      // trial particles per leaf (statistics)
-  DBuf<uint32_t> nlist;      // neighbour lists, blocks of 8 per particle
 
   // ---- tree-ordered particle mirror ----
   DBuf<double4> tp4;                     // density: x, y, z, h at entry
   DBuf<double4> tv4, tf4, tB4;           // density: v, f + fext, Bevol and rho(h)
-  DBuf<double4> fq4;                     // force: x, y, z, 1/h
   DBuf<char> frec, frecB, frecD;         // force: packed j records
   DBuf<double> fh;                       // force: h (tree order)
   DBuf<int32_t> act;                     // active tree positions
@@ -115,8 +112,16 @@ struct phsph_ctx {
   int64_t nrhocalc = -1, nactualtot = -1;
   int64_t force_npairs = -1;  // interacting pairs of the last force call
 
+  // block timesteps registered by phsph_set_bins (host or device pointers)
+  int8_t *bins_ibin = nullptr, *bins_ibin_wake = nullptr;
+  const int8_t *bins_ibin_old = nullptr;
+  int32_t bins_nbinmax = 0, bins_istepfrac = 0, bins_ibinnow = 0;
+  bool bins_set = false;
+  int32_t nbinmax_new = -1;   // result of the last force call with bins
+  DBuf<int8_t> tbin_old;      // ibin_old in tree order
+
   // staging buffers for host arrays
-  DBuf<char> stage[16];
+  DBuf<char> stage[24];
 
   std::vector<TimedLaunch> timings;
   std::vector<std::string> timing_names;
@@ -203,8 +208,13 @@ struct ForceIOArgs {  // device pointers in the caller's layout
   const float *alphaind;
   const double *eos_vars; int ldeos;
   const int8_t *iphase;
+  // block timesteps (null ibin: global timestep)
+  int8_t *ibin; const int8_t *ibin_old; int8_t *ibin_wake;
+  int nbinmax, istepfrac, ibinnow;
 };
 int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc, double *dtf);
+int set_active_run(phsph_ctx *c, int np, const double *xyzh, int ldx, int8_t *iphase, const int8_t *ibin,
+                   int nbinmax, int istepfrac, int32_t *nactive, int32_t *nalive);
 int cons2prim_run(phsph_ctx *c, int np, const double *xyzh, int ldx, const double *vxyzu, int ldv,
                   const float *dvdx, float *alphaind, const double *dustevol, double *dustfrac,
                   double *eos, int ldeos);

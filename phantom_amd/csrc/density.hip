@@ -7,16 +7,16 @@
 //  * k_group_walk: one wavefront per group of 64 active particles walks the
 //    tree breadth-parallel (64 nodes per step) and lists the candidate leaves
 //    of the whole group (stream.h).
-//  * k_density: one wavefront per group, one particle per lane.  The lanes
-//    pull the group's candidate stream through LDS and keep, each for its own
-//    particle, the candidates within rfac*radkern*h (the list covers one
-//    Newton step of +20%, the reference's clamp).  Each lane then runs its
-//    own Newton-Raphson iterations on rho and Omega only, and makes one final
-//    pass with every sum (div v, the velocity, acceleration and exact-linear
-//    matrices, div B) at the converged h.  Per-particle iteration gives the
-//    reference's result: in a cell the reference keeps re-evaluating
-//    converged particles at their unchanged h, so each particle's h sequence
-//    depends only on its own sums.
+//  * k_density: one wavefront per group, one particle per lane.  Each lane
+//    runs its own Newton-Raphson iteration on h; every iteration streams the
+//    group's candidates through LDS, tests them against the lane's current
+//    h and evaluates all the sums of get_density_sums over the hits.  The
+//    sums of the last iteration are stored, as in the reference.
+//    Per-particle iteration gives the reference's result: in a cell the
+//    reference keeps re-evaluating converged particles at their unchanged h,
+//    so each particle's h sequence depends only on its own sums.  The
+//    candidates cover rfac*radkern*max(h) of the group; a particle whose h
+//    grows past that is run again with a wider walk.
 //  * k_walk: the reference's per-leaf walk (getneigh, kdtree.F90:1221-1347),
 //    used only for the trial-neighbour statistics.
 //
@@ -27,7 +27,7 @@
 
 #include <cstdlib>
 
-enum { DFL_CAND = 1, DFL_LIST = 2, DFL_REWALK = 4, DFL_NOCONV = 8, DFL_NEGH = 16 };
+enum { DFL_CAND = 1, DFL_REWALK = 4, DFL_NOCONV = 8, DFL_NEGH = 16 };
 
 struct DensIn {
   const double4 *tp;  // x, y, z (treecache) and h at entry
@@ -277,19 +277,26 @@ k_density(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, 
           const int32_t *__restrict__ pleaf, const int32_t *__restrict__ leaves, const uint32_t *__restrict__ gcand,
           const int32_t *__restrict__ gncand, int capg, const int32_t *__restrict__ nd_lo,
           const int32_t *__restrict__ nd_hi, const phsph_kdnode *__restrict__ nd,
-          const int32_t *__restrict__ iparts, DensIn in, uint32_t *__restrict__ nlist, int capl,
-          double *__restrict__ th_out, DensOut out, int32_t *__restrict__ p_nneigh, int32_t *__restrict__ p_nits,
-          int32_t *__restrict__ leaf_nits, int32_t *__restrict__ retry, int32_t *__restrict__ flags) {
-  __shared__ StageLDS S;
-  __shared__ uint32_t lbuf[8][GROUP];
+          const int32_t *__restrict__ iparts, DensIn in, double *__restrict__ th_out, DensOut out,
+          int32_t *__restrict__ p_nneigh, int32_t *__restrict__ p_nits, int32_t *__restrict__ leaf_nits,
+          int32_t *__restrict__ retry, int32_t *__restrict__ flags) {
+  constexpr int NL = 16, NP = NL * PH_MINPART, NPB = MHD ? NP : 1;
+  __shared__ BatchMeta<NL> M;
+  __shared__ double sx[NP], sy[NP], sz[NP];
+  __shared__ double svx[NP], svy[NP], svz[NP];
+  __shared__ double sfx[NP], sfy[NP], sfz[NP];
+  __shared__ double sBx[NPB], sBy[NPB], sBz[NPB];
+  __shared__ int sj[NP];
+  __shared__ unsigned char sl[NP];
   const int g = xcd_block(blockIdx.x, gridDim.x);
   if (g >= ngroups) return;
   const int lane = threadIdx.x;
   const int t = g * GROUP + lane;
   const bool live = t < nact;
-  const size_t nrows = (size_t)ngroups * GROUP;
+  const bool have_f = in.tf != nullptr;
   int p = 0, l = 0;
   double xi = 0., yi = 0., zi = 0., h0 = 1., lcx = 0., lcy = 0., lcz = 0., lsz = 0.;
+  double4 vi = make_double4(0., 0., 0., 0.), fi = make_double4(0., 0., 0., 0.), Bev = make_double4(0., 0., 0., 0.);
   if (live) {
     p = act[t];
     l = pleaf[p];
@@ -297,113 +304,154 @@ k_density(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, 
     const double4 r = in.tp[p];
     xi = r.x; yi = r.y; zi = r.z; h0 = r.w;
     lcx = nd[c].xcen[0]; lcy = nd[c].xcen[1]; lcz = nd[c].xcen[2]; lsz = nd[c].size;
+    vi = in.tv[p];
+    if (have_f) fi = in.tf[p];
+    if (MHD) Bev = in.tB[p];
   }
   const double pm = dc.pmass;
-  // The group's candidate stream covers rcg = rfac*radkern*max(h0) (the
-  // same expression as in k_group_walk).  A lane lists the candidates within
-  // rfac*radkern*h (one Newton step of +20%, the clamp of finish_cell,
-  // dens.F90:1419); when its h outgrows the list it lists again from the
-  // stream, and only when h outgrows the stream itself is the particle
-  // handed back to the host for a wider walk.
+  // The group's candidate stream covers rcg = rfac*radkern*max(h0), the same
+  // expression as in k_group_walk.  A particle whose h grows past it is
+  // handed back to the host and run again with a wider walk.
   const double rcg = rfac * dc.radkern * wave_max(live ? h0 : 0.);
-  double rcut = rfac * dc.radkern * h0;
-  enum { ST_LIST = 0, ST_ITER = 1, ST_DONE = 2, ST_FAIL = 3 };
-  int state = live ? ST_LIST : ST_DONE;
+  enum { ST_ITER = 1, ST_DONE = 2, ST_FAIL = 3 };
+  int state = live ? ST_ITER : ST_DONE;
   const int nc = gncand[g];
   const uint32_t *gc = gcand + (size_t)g * capg;
   double hi = h0, hfin = h0;
-  int it = 0, nl = 0;
-  while (__ballot(state == ST_LIST) != 0ull) {
-    // --- neighbour list from the group's candidate stream ---
-    const bool listing = (state == ST_LIST);
-    const double rc2 = xmul(rcut, rcut);
-    ListWriter L{nlist, nrows, t, capl, 0, lbuf, lane};
-    for (int b0 = 0; b0 < nc; b0 += SB_LEAVES) {
-      const int nb = stage_batch(S, gc, b0, nc, nd, nd_lo, nd_hi, in.tp, lane);
-      for (int cl = 0; cl < nb; cl++) {
-        const int code = leaf_sep_code(dc, S, cl, lcx, lcy, lcz, lsz);
-        const int s1 = S.lstart[cl + 1];
-        for (int s = S.lstart[cl]; s < s1; s++) {
-          const double4 r = S.pos[s];
-          const int j = S.jpos[s];
+  int it = 0, nneigh = 1;
+  double rs_rho = 0., rs_gh = 0., rs_gs = 0., rs_divv = 0., rs_divB = 0.;
+  double dv[9], da[9], rr[6], dB[9];
+  // Every iteration evaluates all the sums of get_density_sums
+  // (dens.F90:578-857) at the current h, as the reference does; the sums of
+  // the last iteration are the ones stored.
+  while (__ballot(state == ST_ITER) != 0ull) {
+    const bool iter = (state == ST_ITER);
+    if (iter) {
+      it++;
+      nneigh = 1;
+      rs_rho = rs_gh = rs_gs = rs_divv = rs_divB = 0.;
+      for (int k = 0; k < 9; k++) { dv[k] = 0.; da[k] = 0.; dB[k] = 0.; }
+      for (int k = 0; k < 6; k++) rr[k] = 0.;
+    }
+    const double hi1 = 1. / hi, hi21 = xmul(hi1, hi1);
+    double Bxi = 0., Byi = 0., Bzi = 0.;
+    if (MHD) {
+      const double rhoiB = rhoh_d(hi, dc.hfact, pm);
+      Bxi = Bev.x * rhoiB; Byi = Bev.y * rhoiB; Bzi = Bev.z * rhoiB;
+    }
+    for (int b0 = 0; b0 < nc; b0 += NL) {
+      int nb;
+      const int total = batch_meta<NL>(M, gc, b0, nc, nd, nd_lo, nd_hi, lane, nb);
+      for (int s = lane; s < total; s += GROUP) {
+        const int L = batch_leaf_of<NL>(M, nb, s);
+        const int j = M.lo[L] + (s - M.start[L]);
+        const double4 r = in.tp[j];
+        sx[s] = r.x; sy[s] = r.y; sz[s] = r.z;
+        const double4 v = in.tv[j];
+        svx[s] = v.x; svy[s] = v.y; svz[s] = v.z;
+        if (have_f) {
+          const double4 f = in.tf[j];
+          sfx[s] = f.x; sfy[s] = f.y; sfz[s] = f.z;
+        }
+        if (MHD) {  // B_j = Bevol_j * rho_j with rho_j from the h it had on entry
+          const double4 B = in.tB[j];
+          sBx[s] = B.x * B.w; sBy[s] = B.y * B.w; sBz[s] = B.z * B.w;
+        }
+        sj[s] = j;
+        sl[s] = (unsigned char)L;
+      }
+      __syncthreads();
+      // --- each lane tests every staged particle: hits as a bit mask ---
+      unsigned long long m0 = 0ull, m1 = 0ull, m2 = 0ull;
+      SepCodes codes;
+      for (int L = 0; L < nb; L++) {
+        const int code = leaf_sep_code(dc, M.cx[L], M.cy[L], M.cz[L], M.sz[L], lcx, lcy, lcz, lsz);
+        codes.set(L, code);
+        const int s1 = M.start[L + 1];
+        for (int s = M.start[L]; s < s1; s++) {
           double dx, dy, dz;
-          sep_of(dc, code, xi, yi, zi, r.x, r.y, r.z, dx, dy, dz);
+          sep_of(dc, code, xi, yi, zi, sx[s], sy[s], sz[s], dx, dy, dz);
           const double r2 = r2_exact(dx, dy, dz);
-          if (listing && j != p && r2 < rc2) L.push(list_pack(j, code));
+          const bool hit = iter && sj[s] != p && xmul(r2, hi21) < dc.radkern2;
+          const unsigned long long bit = (unsigned long long)hit << (s & 63);
+          if (s < 64) m0 |= bit;
+          else if (s < 128) m1 |= bit;
+          else m2 |= bit;
+        }
+      }
+      // --- pair sums over the hits ---
+      for (int w = 0; w < 3; w++) {
+        unsigned long long m = (w == 0) ? m0 : (w == 1) ? m1 : m2;
+        while (m != 0ull) {
+          const int s = w * 64 + (int)__builtin_ctzll(m);
+          m &= m - 1ull;
+          double dx, dy, dz;
+          sep_of(dc, codes.get(sl[s]), xi, yi, zi, sx[s], sy[s], sz[s], dx, dy, dz);
+          const double r2 = r2_exact(dx, dy, dz);
+          const double q2 = xmul(r2, hi21);
+          const double rij = sqrt(r2);
+          const double q = rij * hi1;
+          double wk, gk;
+          kern_wg<K, double>(q2, q, wk, gk);
+          rs_rho += wk * pm;
+          rs_gh += (-q * gk - 3. * wk) * pm;
+          if (dc.gravity) rs_gs += kern_dphidh<K, double>(q2, q) * pm;
+          nneigh++;
+          const double rij1 = 1. / (rij + PH_EPS64);
+          const double rg = rij1 * gk;
+          const double runix = dx * rg * pm, runiy = dy * rg * pm, runiz = dz * rg * pm;
+          const double dvx = vi.x - svx[s], dvy = vi.y - svy[s], dvz = vi.z - svz[s];
+          rs_divv += dvx * runix + dvy * runiy + dvz * runiz;
+          dv[0] += dvx * runix; dv[1] += dvx * runiy; dv[2] += dvx * runiz;
+          dv[3] += dvy * runix; dv[4] += dvy * runiy; dv[5] += dvy * runiz;
+          dv[6] += dvz * runix; dv[7] += dvz * runiy; dv[8] += dvz * runiz;
+          if (have_f) {
+            const double dax = fi.x - sfx[s], day = fi.y - sfy[s], daz = fi.z - sfz[s];
+            da[0] += dax * runix; da[1] += dax * runiy; da[2] += dax * runiz;
+            da[3] += day * runix; da[4] += day * runiy; da[5] += day * runiz;
+            da[6] += daz * runix; da[7] += daz * runiy; da[8] += daz * runiz;
+          }
+          rr[0] -= dx * runix; rr[1] -= dx * runiy; rr[2] -= dx * runiz;
+          rr[3] -= dy * runiy; rr[4] -= dy * runiz; rr[5] -= dz * runiz;
+          if (MHD) {
+            const double dBx = Bxi - sBx[s], dBy = Byi - sBy[s], dBz = Bzi - sBz[s];
+            rs_divB += dBx * runix + dBy * runiy + dBz * runiz;
+            dB[0] += dBx * runix; dB[1] += dBx * runiy; dB[2] += dBx * runiz;
+            dB[3] += dBy * runix; dB[4] += dBy * runiy; dB[5] += dBy * runiz;
+            dB[6] += dBz * runix; dB[7] += dBz * runiy; dB[8] += dBz * runiz;
+          }
         }
       }
       __syncthreads();
     }
-    L.finish();
-    if (listing) {
-      nl = L.n;
-      if (nl > capl) {
-        atomicOr(flags, DFL_LIST);
-        state = ST_FAIL;
-      } else {
-        state = ST_ITER;
-      }
-    }
-    if (state != ST_ITER) continue;
-    // --- Newton-Raphson on h with rho and Omega only ---
-    for (;;) {
-      it++;
-      const double hi1 = 1. / hi, hi21 = xmul(hi1, hi1);
-      double srho = 0., sgh = 0.;
-      for (int k = 0; k < nl; k++) {
-        const uint32_t e = list_get(nlist, nrows, t, k);
-        const int j = (int)(e & 0x07ffffffu);
-        const double4 r = in.tp[j];
-        double dx, dy, dz;
-        sep_of(dc, (int)(e >> 27), xi, yi, zi, r.x, r.y, r.z, dx, dy, dz);
-        const double r2 = r2_exact(dx, dy, dz);
-        const double q2 = xmul(r2, hi21);
-        if (q2 < dc.radkern2) {
-          const double q = sqrt(r2) * hi1;
-          double w, gk;
-          kern_wg<K, double>(q2, q, w, gk);
-          srho += w * pm;
-          sgh += (-q * gk - 3. * w) * pm;
-        }
-      }
-      // finish_rhosum + finish_cell
-      const double hi31 = hi1 * hi21, hi41 = hi21 * hi21;
-      const double rhoi = dc.cnormk * (srho + dc.wab0 * pm) * hi31;
-      const double gradhi = dc.cnormk * (sgh + dc.gradh0 * pm) * hi41;
-      const double rhohi = rhoh_d(hi, dc.hfact, pm);
-      const double dhdrhoi = -hi / (3. * rhohi);
-      const double omegai = 1. - dhdrhoi * gradhi;
-      const double func = rhohi - rhoi;
-      const double dfdh1 = (omegai > PH_TINY64) ? dhdrhoi / omegai : dhdrhoi / fabs(omegai + PH_EPS64);
-      double hnew = hi - func * dfdh1;
-      if (hnew > 1.2 * hi) hnew = 1.2 * hi;
-      else if (hnew < 0.8 * hi) hnew = 0.8 * hi;
-      const bool conv = (fabs(hnew - hi) / h0 < dc.tolh) && omegai > 0. && hi > 0.;
-      if (icall == 0) {  // one pass: store at hnew with the sums taken at h0 (dens.F90:341)
-        hfin = conv ? hi : hnew;
-        state = ST_DONE;
-        break;
-      }
-      if (conv) {
-        hfin = hi;
-        state = ST_DONE;
-        break;
-      }
-      if (it >= PH_MAXDENSITS) {
-        atomicOr(flags, DFL_NOCONV);
-        state = ST_FAIL;
-        break;
-      }
+    if (!iter) continue;
+    // --- finish_rhosum + finish_cell (dens.F90:1382-1507) ---
+    const double hi31 = hi1 * hi21, hi41 = hi21 * hi21;
+    const double rhoi = dc.cnormk * (rs_rho + dc.wab0 * pm) * hi31;
+    const double gradhi = dc.cnormk * (rs_gh + dc.gradh0 * pm) * hi41;
+    const double rhohi = rhoh_d(hi, dc.hfact, pm);
+    const double dhdrhoi = -hi / (3. * rhohi);
+    const double omegai = 1. - dhdrhoi * gradhi;
+    const double func = rhohi - rhoi;
+    const double dfdh1 = (omegai > PH_TINY64) ? dhdrhoi / omegai : dhdrhoi / fabs(omegai + PH_EPS64);
+    double hnew = hi - func * dfdh1;
+    if (hnew > 1.2 * hi) hnew = 1.2 * hi;
+    else if (hnew < 0.8 * hi) hnew = 0.8 * hi;
+    const bool conv = (fabs(hnew - hi) / h0 < dc.tolh) && omegai > 0. && hi > 0.;
+    if (icall == 0) {  // one pass: store at hnew with the sums taken at h0 (dens.F90:341)
+      hfin = conv ? hi : hnew;
+      state = ST_DONE;
+    } else if (conv) {
+      hfin = hi;
+      state = ST_DONE;
+    } else if (it >= PH_MAXDENSITS) {
+      atomicOr(flags, DFL_NOCONV);
+      state = ST_FAIL;
+    } else {
       hi = hnew;
-      if (dc.radkern * hi * (1. + 1e-12) > rcut) {  // the list no longer covers radkern*h
-        if (dc.radkern * hi * (1. + 1e-12) > rcg) {
-          atomicOr(flags, DFL_REWALK);
-          state = ST_FAIL;
-        } else {
-          rcut = fmin(rfac * dc.radkern * hi, rcg);
-          state = ST_LIST;
-        }
-        break;
+      if (dc.radkern * hi * (1. + 1e-12) > rcg) {  // outgrew the group's candidate stream
+        atomicOr(flags, DFL_REWALK);
+        state = ST_FAIL;
       }
     }
   }
@@ -411,66 +459,6 @@ k_density(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, 
   if (state != ST_DONE) {
     if (state == ST_FAIL) retry[p] = 1;
     return;
-  }
-  // --- all sums at h = hi (the h of the last summation) ---
-  const double hs1 = 1. / hi, hs21 = xmul(hs1, hs1);
-  double rs_rho = 0., rs_gh = 0., rs_gs = 0., rs_divv = 0.;
-  double dv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, da[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  double rr[6] = {0, 0, 0, 0, 0, 0};
-  double rs_divB = 0., dB[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  const double4 vi = in.tv[p];
-  double4 fi = make_double4(0., 0., 0., 0.);
-  if (in.tf) fi = in.tf[p];
-  double Bxi = 0., Byi = 0., Bzi = 0.;
-  if (MHD) {
-    const double rhoiB = rhoh_d(hi, dc.hfact, pm);
-    const double4 Bi = in.tB[p];
-    Bxi = Bi.x * rhoiB; Byi = Bi.y * rhoiB; Bzi = Bi.z * rhoiB;
-  }
-  int nneigh = 1;
-  for (int k = 0; k < nl; k++) {
-    const uint32_t e = list_get(nlist, nrows, t, k);
-    const int j = (int)(e & 0x07ffffffu);
-    const double4 r = in.tp[j];
-    double dx, dy, dz;
-    sep_of(dc, (int)(e >> 27), xi, yi, zi, r.x, r.y, r.z, dx, dy, dz);
-    const double r2 = r2_exact(dx, dy, dz);
-    const double q2 = xmul(r2, hs21);
-    if (!(q2 < dc.radkern2)) continue;
-    const double rij = sqrt(r2);
-    const double q = rij * hs1;
-    double w, gk;
-    kern_wg<K, double>(q2, q, w, gk);
-    rs_rho += w * pm;
-    rs_gh += (-q * gk - 3. * w) * pm;
-    if (dc.gravity) rs_gs += kern_dphidh<K, double>(q2, q) * pm;
-    nneigh++;
-    const double rij1 = 1. / (rij + PH_EPS64);
-    const double rg = rij1 * gk;
-    const double runix = dx * rg * pm, runiy = dy * rg * pm, runiz = dz * rg * pm;
-    const double4 vj = in.tv[j];
-    const double dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
-    rs_divv += dvx * runix + dvy * runiy + dvz * runiz;
-    dv[0] += dvx * runix; dv[1] += dvx * runiy; dv[2] += dvx * runiz;
-    dv[3] += dvy * runix; dv[4] += dvy * runiy; dv[5] += dvy * runiz;
-    dv[6] += dvz * runix; dv[7] += dvz * runiy; dv[8] += dvz * runiz;
-    if (in.tf) {
-      const double4 fj = in.tf[j];
-      const double dax = fi.x - fj.x, day = fi.y - fj.y, daz = fi.z - fj.z;
-      da[0] += dax * runix; da[1] += dax * runiy; da[2] += dax * runiz;
-      da[3] += day * runix; da[4] += day * runiy; da[5] += day * runiz;
-      da[6] += daz * runix; da[7] += daz * runiy; da[8] += daz * runiz;
-    }
-    rr[0] -= dx * runix; rr[1] -= dx * runiy; rr[2] -= dx * runiz;
-    rr[3] -= dy * runiy; rr[4] -= dy * runiz; rr[5] -= dz * runiz;
-    if (MHD) {
-      const double4 Bj = in.tB[j];  // rho_j from the h it had on entry
-      const double dBx = Bxi - Bj.x * Bj.w, dBy = Byi - Bj.y * Bj.w, dBz = Bzi - Bj.z * Bj.w;
-      rs_divB += dBx * runix + dBy * runiy + dBz * runiz;
-      dB[0] += dBx * runix; dB[1] += dBx * runiy; dB[2] += dBx * runiz;
-      dB[3] += dBy * runix; dB[4] += dBy * runiy; dB[5] += dBy * runiz;
-      dB[6] += dBz * runix; dB[7] += dBz * runiy; dB[8] += dBz * runiz;
-    }
   }
   // --- store_results at h = hfin ---
   const double hf1 = 1. / hfin, hf21 = hf1 * hf1, hf31 = hf1 * hf21, hf41 = hf21 * hf21;
@@ -698,16 +686,15 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(n, 256)), dim3(256), 0, s, c->p_nits.p, n, -1);
   hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(n, 256)), dim3(256), 0, s, c->p_nneigh.p, n, -1);
   const bool dbg = getenv("PHSPH_DEBUG") != nullptr;
-  // Particles whose h outgrows their group's candidate stream, or whose list
-  // overflows, are handed back and run again (alone) with a wider search.
-  double rfac = 1.2;
+  // Particles whose h outgrows their group's candidate stream are handed
+  // back and run again (alone) with a wider search.
+  double rfac = 1.1;
   for (int attempt = 0; attempt < 12; attempt++) {
     const int na = c->nact;
     if (na == 0) break;
     rc = walk_groups(c, 0, rfac, c->th_in.p, nullptr);
     if (rc) return rc;
     const int ng = c->ngroups;
-    PH_HIPCHECK(c, c->nlist.ensure((size_t)c->caps.nlist * ((size_t)ng * GROUP + 1)));
     hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(n, 256)), dim3(256), 0, s, c->tmpi.p, n, 0);
     int32_t zero = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
@@ -717,7 +704,7 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
 #define DLAUNCH(KK, MM)                                                                                        \
   hipLaunchKernelGGL((k_density<KK, MM>), gr, b, 0, s, c->dc, icall, rfac, c->act.p, na, ng, c->pleaf.p,         \
                      c->leaves.p, c->gcand.p, c->gncand.p, c->caps.gcand, c->nd_lo.p, c->nd_hi.p, c->nd.p,       \
-                     c->iparts.p, in, c->nlist.p, c->caps.nlist, c->th.p, out, c->p_nneigh.p, c->p_nits.p,       \
+                     c->iparts.p, in, c->th.p, out, c->p_nneigh.p, c->p_nits.p,                                 \
                      c->leaf_nits.p, c->tmpi.p, c->counters.p)
       if (c->p.kernel == PHSPH_KERNEL_CUBIC) {
         if (mhd) DLAUNCH(PHSPH_KERNEL_CUBIC, true); else DLAUNCH(PHSPH_KERNEL_CUBIC, false);
@@ -733,12 +720,11 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
     if (dbg) fprintf(stderr, "phsph densityiterate: attempt %d particles %d groups %d rfac %g flags %d\n", attempt, na, ng, rfac, fl);
     if (fl & DFL_NOCONV) PH_RET(c, PHSPH_ERR_NOCONVERGE, "densityiterate: could not converge in density");
     if (fl & DFL_NEGH) PH_RET(c, PHSPH_ERR_INPUT, "densityiterate: setting negative h from hrho");
-    if (!(fl & (DFL_LIST | DFL_REWALK))) {
+    if (!(fl & DFL_REWALK)) {
       c->nact = 0;
       break;
     }
-    if (fl & DFL_LIST) c->caps.nlist = ((c->caps.nlist * 3 / 2) + 7) & ~7;
-    if (fl & DFL_REWALK) rfac *= 1.5;
+    rfac *= 1.5;
     rc = compact_active(c, c->tmpi.p, n);
     if (rc) return rc;
   }

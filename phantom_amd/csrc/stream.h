@@ -13,33 +13,29 @@
 // neighbour sets themselves are decided per pair by the reference's q^2
 // tests, so they do not change.
 //
-// The pair kernels pull the stream through LDS, SB_LEAVES leaves at a time:
-// each leaf's particles are loaded once per group with vector loads and read
-// by all lanes as LDS broadcasts.  Per candidate leaf every lane computes the
-// periodic offset that get_sep (kdtree.F90:1468-1501) gives for its own leaf
-// centre against the candidate's centre, so a separation rounds exactly as
-// the reference's xpartveci - xyzcache (dens.F90:653-670).
-//
-// Neighbour lists are kept per lane in blocks of 8 entries: entry k of the
-// lane of active index t is nlist[((k/8)*nrows + t)*8 + k%8].  A lane stages
-// 8 entries in LDS and writes them as two 16-byte stores.
+// The pair kernels pull the stream through LDS, NL leaves (at most
+// NL*minpart particles) at a time: every staged quantity of a leaf's
+// particles is loaded once per group with coalesced loads.  Each lane tests
+// every staged particle against its own (positions read as LDS broadcasts),
+// keeps the hits as a bit mask, and then evaluates its hits reading the j
+// data from LDS, so no pair ever gathers from global memory and no neighbour
+// list is stored.  Per candidate leaf every lane computes the periodic
+// offset that get_sep (kdtree.F90:1468-1501) gives for its own leaf centre
+// against the candidate's centre, so a separation rounds exactly as the
+// reference's xpartveci - xyzcache (dens.F90:653-670).
 #pragma once
 
 #include "ctx.h"
 
 constexpr int GROUP = 64;                         // particles per group = wavefront size
-constexpr int SB_LEAVES = 32;                     // candidate leaves staged per batch
-constexpr int SB_PARTS = SB_LEAVES * PH_MINPART;  // leaves hold at most minpart particles
 constexpr int WALK_STACK = 2048;                  // node stack of the group walk (LDS)
 
-// list entry: tree position j (27 bits) and a 5-bit separation code:
-// 0..26 periodic offset (sx+1)+3(sy+1)+9(sz+1) with minimum image,
-// 13 = no offset and no minimum image possible, 27 = no offset but the
-// minimum image may apply
+// separation code of a candidate leaf for a lane: 0..26 periodic offset
+// (sx+1)+3(sy+1)+9(sz+1) with the minimum image; SEP_PLAIN = no offset and
+// no minimum image possible; SEP_WRAP0 = no offset but the minimum image may
+// apply
 constexpr int SEP_PLAIN = 13;
 constexpr int SEP_WRAP0 = 27;
-
-__device__ __forceinline__ uint32_t list_pack(int j, int code) { return (uint32_t)j | ((uint32_t)code << 27); }
 
 // Blocks are dispatched to the 8 XCDs round robin; give each XCD a contiguous
 // range of groups so that neighbouring groups share its L2.  The grid must
@@ -55,54 +51,77 @@ __device__ __forceinline__ unsigned long long lanemask_lt() {
   return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 }
 
-struct StageLDS {
-  double4 pos[SB_PARTS];   // x, y, z, w of the staged candidate particles
-  int jpos[SB_PARTS];      // their tree positions
-  int lstart[SB_LEAVES + 1];
-  double cx[SB_LEAVES], cy[SB_LEAVES], cz[SB_LEAVES], sz[SB_LEAVES];
+// Metadata of one batch of NL candidate leaves staged in LDS: first tree
+// position, start of each leaf's particles in the batch, centre and size.
+template <int NL>
+struct BatchMeta {
+  int lo[NL];
+  int start[NL + 1];
+  double cx[NL], cy[NL], cz[NL], sz[NL];
 };
 
-// Stage one batch of candidate leaves (gc[b0 .. b0+nb)) and their particles
-// from a tree-ordered double4 array.  Returns the number of staged leaves.
-// Every lane of the wave must call it.
-__device__ __forceinline__ int stage_batch(StageLDS &S, const uint32_t *__restrict__ gc, int b0, int nc,
-                                           const phsph_kdnode *__restrict__ nd, const int32_t *__restrict__ nd_lo,
-                                           const int32_t *__restrict__ nd_hi, const double4 *__restrict__ src,
-                                           int lane) {
-  const int nb = min(SB_LEAVES, nc - b0);
-  int lo = 0, cnt = 0;
+// Load the metadata of candidate leaves gc[b0 .. b0+nb) and return the
+// number of staged particles (uniform).  Every lane of the wave must call it.
+template <int NL>
+__device__ __forceinline__ int batch_meta(BatchMeta<NL> &M, const uint32_t *__restrict__ gc, int b0, int nc,
+                                          const phsph_kdnode *__restrict__ nd, const int32_t *__restrict__ nd_lo,
+                                          const int32_t *__restrict__ nd_hi, int lane, int &nb_out) {
+  const int nb = min(NL, nc - b0);
+  int cnt = 0;
   if (lane < nb) {
     const int n = (int)gc[b0 + lane];
-    lo = nd_lo[n];
+    const int lo = nd_lo[n];
     cnt = min(nd_hi[n] - lo + 1, PH_MINPART);
-    S.cx[lane] = nd[n].xcen[0];
-    S.cy[lane] = nd[n].xcen[1];
-    S.cz[lane] = nd[n].xcen[2];
-    S.sz[lane] = nd[n].size;
+    M.lo[lane] = lo;
+    M.cx[lane] = nd[n].xcen[0];
+    M.cy[lane] = nd[n].xcen[1];
+    M.cz[lane] = nd[n].xcen[2];
+    M.sz[lane] = nd[n].size;
   }
   int incl = cnt;
-  for (int o = 1; o < 64; o <<= 1) {
+  for (int o = 1; o < NL; o <<= 1) {
     const int v = __shfl_up(incl, o, 64);
     if (lane >= o) incl += v;
   }
-  const int excl = incl - cnt;
-  if (lane < nb) S.lstart[lane] = excl;
-  if (lane == nb - 1) S.lstart[nb] = incl;
-  for (int k = 0; k < cnt; k++) {
-    S.pos[excl + k] = src[lo + k];
-    S.jpos[excl + k] = lo + k;
-  }
+  if (lane < nb) M.start[lane] = incl - cnt;
+  if (lane == nb - 1) M.start[nb] = incl;
+  nb_out = nb;
+  const int total = __shfl(incl, nb - 1, 64);
   __syncthreads();
-  return nb;
+  return total;
 }
+
+// leaf of staged particle s (M.start is increasing, nb <= NL leaves)
+template <int NL>
+__device__ __forceinline__ int batch_leaf_of(const BatchMeta<NL> &M, int nb, int s) {
+  int lo = 0, hi = nb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (M.start[mid] <= s) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// per-lane separation codes of up to 16 staged leaves, 5 bits each
+struct SepCodes {
+  unsigned long long lo = 0ull, hi = 0ull;
+  __device__ __forceinline__ void set(int L, int code) {
+    if (L < 12) lo |= (unsigned long long)code << (5 * L);
+    else hi |= (unsigned long long)code << (5 * (L - 12));
+  }
+  __device__ __forceinline__ int get(int L) const {
+    return (int)(((L < 12) ? (lo >> (5 * L)) : (hi >> (5 * (L - 12)))) & 31ull);
+  }
+};
 
 // Separation code of candidate leaf L for a lane whose own leaf has centre
 // (lcx,lcy,lcz) and size lsz: the get_sep offset, and whether any pair of
 // the two leaves can need the minimum-image correction.
-__device__ __forceinline__ int leaf_sep_code(const DevConst &dc, const StageLDS &S, int L, double lcx, double lcy,
-                                             double lcz, double lsz) {
+__device__ __forceinline__ int leaf_sep_code(const DevConst &dc, double ccx, double ccy, double ccz, double csz,
+                                             double lcx, double lcy, double lcz, double lsz) {
   if (!dc.periodic) return SEP_PLAIN;
-  double dx = xsub(lcx, S.cx[L]), dy = xsub(lcy, S.cy[L]), dz = xsub(lcz, S.cz[L]);
+  double dx = xsub(lcx, ccx), dy = xsub(lcy, ccy), dz = xsub(lcz, ccz);
   int sx = 0, sy = 0, sz = 0;
   if (fabs(dx) > dc.hdlx) { sx = (dx > 0.) ? 1 : -1; dx = xsub(dx, copysign(dc.dxbound, dx)); }
   if (fabs(dy) > dc.hdly) { sy = (dy > 0.) ? 1 : -1; dy = xsub(dy, copysign(dc.dybound, dy)); }
@@ -110,7 +129,7 @@ __device__ __forceinline__ int leaf_sep_code(const DevConst &dc, const StageLDS 
   const int code = off_code(sx, sy, sz);
   // |x_i - x_j - offset| <= |dx| + lsz + size_L: below half the box no
   // separation of the two leaves is ever corrected
-  const double reach = (lsz + S.sz[L]) * (1. + 1e-9);
+  const double reach = (lsz + csz) * (1. + 1e-9);
   const double m = 1. - 1e-9;
   const bool wrap = (fabs(dx) + reach >= m * dc.hdlx) || (fabs(dy) + reach >= m * dc.hdly) ||
                     (fabs(dz) + reach >= m * dc.hdlz);
@@ -142,34 +161,6 @@ __device__ __forceinline__ void sep_of(const DevConst &dc, int code, double xi, 
   dx = min_image(dx, dc.dxbound, 0.5 * dc.dxbound);
   dy = min_image(dy, dc.dybound, 0.5 * dc.dybound);
   dz = min_image(dz, dc.dzbound, 0.5 * dc.dzbound);
-}
-
-// per-lane neighbour list writer (blocked layout, see above)
-struct ListWriter {
-  uint32_t *nlist;
-  size_t nrows;
-  int t, capl, n;
-  uint32_t (*buf)[GROUP];
-  int lane;
-  __device__ __forceinline__ void flush_block(int blk) {
-    const uint4 a = make_uint4(buf[0][lane], buf[1][lane], buf[2][lane], buf[3][lane]);
-    const uint4 b = make_uint4(buf[4][lane], buf[5][lane], buf[6][lane], buf[7][lane]);
-    uint4 *dst = (uint4 *)(nlist + (((size_t)blk * nrows + (size_t)t) << 3));
-    dst[0] = a;
-    dst[1] = b;
-  }
-  __device__ __forceinline__ void push(uint32_t e) {
-    buf[n & 7][lane] = e;
-    n++;
-    if ((n & 7) == 0 && n <= capl) flush_block((n >> 3) - 1);
-  }
-  __device__ __forceinline__ void finish() {
-    if ((n & 7) != 0 && n <= capl) flush_block(n >> 3);
-  }
-};
-
-__device__ __forceinline__ uint32_t list_get(const uint32_t *__restrict__ nlist, size_t nrows, int t, int k) {
-  return nlist[((((size_t)(k >> 3)) * nrows + (size_t)t) << 3) + (k & 7)];
 }
 
 // host side (density.hip)

@@ -217,3 +217,33 @@ def random_box(n=20000, seed=99, kernel=abi.KERNEL_CUBIC, hmin_frac=0.5, hmax_fr
     alphaind[:, 0] = rng.uniform(0.0, 1.0, size=n).astype(np.float32)
     return dict(xyzh=xyzh, vxyzu=vxyzu, alphaind=alphaind, params=p,
                 info=dict(workload="random_box", n=n, seed=seed, kernel=kernel))
+
+
+def with_bins(setup, nbinmax=3, istepfrac=2, seed=5, dtmax=0.016):
+    """Block-timestep version of a setup: ibin_old (= ibin on entry) set in
+    slabs along x (bin k in the k-th of nbinmax+1 slabs) with 10% of the
+    particles redrawn uniformly, ibin_wake = 0, and iphase set as
+    set_active_particles does (utils_indtimesteps.f90:114-185): a particle is
+    active when mod(istepfrac, 2**(nbinmax - ibin)) == 0.  dtmax places the
+    particles' timesteps a few bins below it."""
+    s = dict(setup)
+    p = s["params"]
+    p.ind_timesteps = 1
+    p.dtmax = dtmax
+    n = s["xyzh"].shape[0]
+    rng = np.random.default_rng(seed)
+    fx = (s["xyzh"][:, 0] - p.xmin) / (p.xmax - p.xmin)
+    ibin_old = np.minimum((fx * (nbinmax + 1)).astype(np.int64), nbinmax)
+    redraw = rng.random(n) < 0.1
+    ibin_old[redraw] = rng.integers(0, nbinmax + 1, size=int(redraw.sum()))
+    ibin_old = ibin_old.astype(np.int8)
+    active = (istepfrac % (2 ** (nbinmax - ibin_old.astype(np.int64)))) == 0
+    s["iphase"] = np.where(active, 1, -1).astype(np.int8)
+    ibinnow = nbinmax
+    for k in range(nbinmax):
+        if istepfrac % 2 ** (nbinmax - k) == 0:
+            ibinnow = k
+            break
+    s["bins"] = dict(ibin=ibin_old.copy(), ibin_old=ibin_old, ibin_wake=np.zeros(n, np.int8), nbinmax=nbinmax,
+                     istepfrac=istepfrac, ibinnow=ibinnow)
+    return s

@@ -30,6 +30,7 @@ EXPORTED = [
     "phsph_default_params", "phsph_create", "phsph_destroy", "phsph_set_params", "phsph_last_error",
     "phsph_maketree", "phsph_get_tree", "phsph_densityiterate", "phsph_cons2prim", "phsph_force",
     "phsph_get_neighbour_stats", "phsph_get_particle_stats", "phsph_get_force_stats", "phsph_get_timings",
+    "phsph_set_bins", "phsph_get_nbinmax", "phsph_set_active_particles",
 ]
 
 _lib = None
@@ -69,6 +70,9 @@ def load_library(path=LIB_PATH):
     L.phsph_get_particle_stats.argtypes = [vp, i32, vp, vp]
     L.phsph_get_force_stats.argtypes = [vp, P(i64)]
     L.phsph_get_timings.argtypes = [vp, P(C.c_char_p), P(dbl), i32]
+    L.phsph_set_bins.argtypes = [vp, vp, vp, vp, i32, i32, i32]
+    L.phsph_get_nbinmax.argtypes = [vp, P(i32)]
+    L.phsph_set_active_particles.argtypes = [vp, i32, vp, i32, vp, vp, i32, i32, P(i32), P(i32), P(i32)]
     for name in EXPORTED:
         if name != "phsph_default_params" and name != "phsph_last_error":
             getattr(L, name).restype = C.c_int
@@ -284,6 +288,33 @@ class Context:
                                          C.byref(dtf)))
         return dict(fxyzu=fxyzu, divcurlv=divcurlv, dBevol=dBevol, divBsymm=divBsymm, ddustevol=ddustevol,
                     deltav=deltav, tstop=tstop, dtcourant=dtc.value, dtforce=dtf.value)
+
+    # -- block timesteps ------------------------------------------------------
+    def set_bins(self, ibin, ibin_old, ibin_wake, nbinmax, istepfrac, ibinnow):
+        """Register part:ibin / ibin_old / ibin_wake (int8, updated in place by
+        force while params.ind_timesteps is set) and the timestep_ind state."""
+        self._bins = (ibin, ibin_old, ibin_wake)  # keep the arrays alive
+        pb, _ = _ptr(ibin)
+        po, _ = _ptr(ibin_old)
+        pw, _ = _ptr(ibin_wake)
+        self._check(self.lib.phsph_set_bins(self._h, pb, po, pw, nbinmax, istepfrac, ibinnow))
+
+    def get_nbinmax(self):
+        v = C.c_int32()
+        self._check(self.lib.phsph_get_nbinmax(self._h, C.byref(v)))
+        return v.value
+
+    def set_active_particles(self, xyzh, iphase, ibin, nbinmax, istepfrac):
+        """set_active_particles (utils_indtimesteps.f90:114-185); iphase is
+        set in place.  Returns (nactive, nalive, ibinnow)."""
+        n = xyzh.shape[0]
+        px, ldx = _ptr(xyzh)
+        pph, _ = _ptr(iphase)
+        pb, _ = _ptr(ibin)
+        na, nl, bn = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.lib.phsph_set_active_particles(self._h, n, px, ldx, pph, pb, nbinmax, istepfrac,
+                                                        C.byref(na), C.byref(nl), C.byref(bn)))
+        return na.value, nl.value, bn.value
 
     def get_force_stats(self):
         n = C.c_int64()

@@ -12,14 +12,22 @@ def run_gpu(setup, icall=1):
     xyzh = setup["xyzh"].copy()
     vxyzu = setup["vxyzu"]
     alphaind = setup["alphaind"].copy()
-    leaf, ncells = ctx.maketree(xyzh)
+    iphase = setup.get("iphase")
+    leaf, ncells = ctx.maketree(xyzh, iphase=iphase)
     tree = ctx.get_tree()
-    d = ctx.densityiterate(icall, xyzh, vxyzu, Bevol=setup.get("Bevol"), alphaind=alphaind)
+    d = ctx.densityiterate(icall, xyzh, vxyzu, Bevol=setup.get("Bevol"), alphaind=alphaind, iphase=iphase)
     stats = ctx.get_neighbour_stats()
     nneigh, nits = ctx.get_particle_stats(xyzh.shape[0])
     c = ctx.cons2prim(xyzh, vxyzu, d["dvdx"], alphaind, dustevol=setup.get("dustevol"))
+    bins = setup.get("bins")
+    if bins is not None:
+        ibin = bins["ibin"].copy()
+        ibin_wake = bins["ibin_wake"].copy()
+        ctx.set_bins(ibin, bins["ibin_old"], ibin_wake, bins["nbinmax"], bins["istepfrac"], bins["ibinnow"])
     f = ctx.force(1, xyzh, vxyzu, d["gradh"], alphaind, c["eos_vars"], divcurlB=d["divcurlB"],
-                  Bevol=setup.get("Bevol"), dustfrac=c["dustfrac"])
+                  Bevol=setup.get("Bevol"), dustfrac=c["dustfrac"], iphase=iphase)
+    if bins is not None:
+        f.update(ibin=ibin, ibin_wake=ibin_wake, nbinmax=ctx.get_nbinmax())
     out = dict(ctx=ctx, xyzh=xyzh, leaf=leaf, ncells=ncells, tree=tree, dens=d, stats=stats, nneigh=nneigh,
                nits=nits, c2p=c, force=f, alphaind=alphaind)
     return out

@@ -261,3 +261,42 @@ def test_tree_invariants():
             assert np.all(xyzh[left, ax] <= piv) and np.all(xyzh[right, ax] > piv)
             stack += [l, r]
     assert np.all(seen == 1)  # the leaves partition the particles
+
+
+def test_get_newbin_known_answers():
+    """test_indtstep.f90:50-72: dt = 3*dtmax/2**i lands in bin i-1 (starting
+    from bin 1), dt = huge in bin 0 and dt = 0 in maxbins = 30."""
+    dt = 3.0
+    for i in range(1, 31):
+        dt = dt / 2.0
+        assert oracle.get_newbin(dt, 1.0, 1, nbinmax=30, istepfrac=0) == i - 1, i
+    assert oracle.get_newbin(1.7976931348623157e308, 1.0, 0) == 0
+    assert oracle.get_newbin(0.0, 1.0, 0) == 30
+
+
+def test_get_newbin_decrease_rules():
+    """utils_indtimesteps.f90:275-284: a bin can only go up (dt grow) by one
+    bin, when that bin is synchronised and decreases are allowed."""
+    # dt wants bin 0 from bin 3; nbinmax 3: synchronised when istepfrac % 2**(3-2) == 0
+    assert oracle.get_newbin(2.0, 1.0, 3, nbinmax=3, istepfrac=2) == 2
+    assert oracle.get_newbin(2.0, 1.0, 3, nbinmax=3, istepfrac=1) == 3
+    assert oracle.get_newbin(2.0, 1.0, 3, allow_decrease=False, nbinmax=3, istepfrac=2) == 3
+    # a bin above nbinmax never decreases
+    assert oracle.get_newbin(2.0, 1.0, 3, nbinmax=2, istepfrac=0) == 3
+
+
+def test_set_active_particles_known_answers():
+    """utils_indtimesteps.f90:134-176 by hand: nbinmax 3, istepfrac 4 makes
+    bins 1..3 active (4 is a multiple of 4, 2 and 1) and bin 0 inactive
+    (8 does not divide 4); the largest active bin is 1.  A dead particle
+    (h < 0) is neither alive nor active."""
+    xyzh = np.zeros((5, 4))
+    xyzh[:, 3] = [1., 1., 1., 1., -1.]
+    ibin = np.array([0, 1, 2, 3, 3], np.int8)
+    iphase, na, nl, ibinnow = oracle.set_active_particles(xyzh, ibin, 3, 4)
+    assert list(iphase[:4]) == [-1, 1, 1, 1]
+    assert (na, nl, ibinnow) == (3, 4, 1)
+    iphase, na, nl, ibinnow = oracle.set_active_particles(xyzh, ibin, 3, 8)
+    assert (na, nl, ibinnow) == (4, 4, 0)
+    iphase, na, nl, ibinnow = oracle.set_active_particles(xyzh, ibin, 3, 1)
+    assert list(iphase[:4]) == [-1, -1, -1, 1] and ibinnow == 3
