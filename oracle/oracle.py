@@ -60,6 +60,9 @@ def lib():
         L.orc_set_active_particles.argtypes = [C.c_int32, vp, vp, vp, C.c_int32, C.c_int32, P(C.c_int32),
                                                P(C.c_int32)]
         L.orc_set_active_particles.restype = C.c_int32
+        L.orc_set_poten.argtypes = [vp, vp]
+        L.orc_set_poten.restype = None
+        L.orc_get_quads.argtypes = [vp, vp, C.c_int64]
         L.orc_kernel_softening.argtypes = [C.c_int32, C.c_double, P(C.c_double), P(C.c_double)]
         _lib = L
     return _lib
@@ -208,13 +211,23 @@ class Oracle:
         deltav = np.zeros((n, 3)) if dust else None
         tstop = np.zeros(n) if dust else None
         dtc, dtf = C.c_double(), C.c_double()
+        poten = np.zeros(n, dtype=np.float32) if p.gravity else None
+        lib().orc_set_poten(self._ctx, _ptr(poten))
         self._check(lib().orc_force(self._ctx, icall, n, _ptr(xyzh), _ptr(vxyzu), vxyzu.shape[1], _ptr(fxyzu), 4,
                                     _ptr(divcurlv), 1, _ptr(divcurlB), _ptr(Bevol), _ptr(dBevol), _ptr(divBsymm),
                                     _ptr(dustfrac), _ptr(ddustevol), _ptr(deltav), _ptr(tstop), _ptr(gradh),
                                     gradh.shape[1], _ptr(alphaind), _ptr(eos_vars), eos_vars.shape[1],
                                     _ptr(iphase), C.byref(dtc), C.byref(dtf)))
+        lib().orc_set_poten(self._ctx, None)
         return dict(fxyzu=fxyzu, divcurlv=divcurlv, dBevol=dBevol, divBsymm=divBsymm, ddustevol=ddustevol,
-                    deltav=deltav, tstop=tstop, dtcourant=dtc.value, dtforce=dtf.value)
+                    deltav=deltav, tstop=tstop, dtcourant=dtc.value, dtforce=dtf.value, poten=poten)
+
+    def get_quads(self):
+        """Second moments quads(6) of every node (node ids 1..ncells)."""
+        nc = self.get_tree()["ncells"]
+        q = np.zeros((nc, 6))
+        self._check(lib().orc_get_quads(self._ctx, _ptr(q), nc))
+        return q
 
 
 def run_all(setup, icall=1):

@@ -66,6 +66,10 @@ typedef struct {
   /* block timesteps (part:ibin, ibin_old, ibin_wake; timestep_ind module) */
   int8_t *ibin, *ibin_wake;
   const int8_t *ibin_old;
+  /* self-gravity: node second moments quads(6) about the centre of mass
+   * (kdtree.F90:724-750), 6 per node slot; poten output (part:poten) */
+  double *quads;
+  float *poten;
   int32_t nbinmax, istepfrac, ibinnow, nbinmaxnew;
   char err[256];
 } orc_ctx;
@@ -189,6 +193,7 @@ orc_ctx *orc_create(const phsph_params *p) {
 }
 
 static void free_tree(orc_ctx *o) {
+  free(o->quads); o->quads = NULL;
   free(o->node); free(o->leaf_is_active); free(o->ir_lo); free(o->ir_hi);
   free(o->inodeparts); free(o->treecache);
   o->node = NULL; o->leaf_is_active = NULL; o->ir_lo = o->ir_hi = NULL;
@@ -299,13 +304,25 @@ static int construct_node(orc_ctx *o, int32_t nnode, int32_t mymum, int32_t leve
     return PHSPH_ERR_INPUT;
   }
   double r2max = 0.;
+  double q[6] = {0., 0., 0., 0., 0., 0.};
   for (int32_t i = i1; i <= i1 + npnode - 1; i++) {
     double dx = TC(o, i, 0) - cofm[0];
     double dy = TC(o, i, 1) - cofm[1];
     double dz = TC(o, i, 2) - cofm[2];
     double dr2 = dx * dx + dy * dy + dz * dz;
     r2max = (dr2 > r2max) ? dr2 : r2max;
+    if (o->quads) {
+      double pm = TC(o, i, 4);
+      q[0] = q[0] + pm * (dx * dx);
+      q[1] = q[1] + pm * (dx * dy);
+      q[2] = q[2] + pm * (dx * dz);
+      q[3] = q[3] + pm * (dy * dy);
+      q[4] = q[4] + pm * (dy * dz);
+      q[5] = q[5] + pm * (dz * dz);
+    }
   }
+  if (o->quads)
+    for (int k = 0; k < 6; k++) o->quads[6 * (size_t)nnode + k] = q[k];
   nd->xcen[0] = cofm[0]; nd->xcen[1] = cofm[1]; nd->xcen[2] = cofm[2];
   nd->size = sqrt(r2max) + EPS64;
   nd->hmax = hmax;
@@ -379,6 +396,7 @@ int orc_maketree(orc_ctx *o, double *xyzh, int32_t np, const int8_t *iphase) {
   o->ncellsmax = 2 * maxp;
   size_t nslots = (size_t)o->ncellsmax + 2;
   o->node = (phsph_kdnode *)calloc(nslots, sizeof(phsph_kdnode));
+  if (o->p.gravity) o->quads = (double *)calloc((size_t)nslots * 6, sizeof(double));
   o->leaf_is_active = (int32_t *)calloc(nslots, sizeof(int32_t));
   o->ir_lo = (int32_t *)calloc(nslots, sizeof(int32_t));
   o->ir_hi = (int32_t *)calloc(nslots, sizeof(int32_t));
@@ -559,6 +577,201 @@ static int getneigh(const orc_ctx *o, const double xpos[3], double xsizei, doubl
       }
     }
   }
+  return PHSPH_OK;
+}
+
+static void kernel_softening(int kind, double q2, double q, double *pot, double *fs);
+
+/* ------------------------------------------------------------------ */
+/* self-gravity: dual tree walk and expansions                         */
+/* ------------------------------------------------------------------ */
+#define LENFGRAV 20
+#define MAXDEPTH 32   /* kdtree.F90:42 */
+
+/* compute_M2L (kdtree.F90:1702-1781): quadrupole source at separation
+ * (dx,dy,dz), dr1 = 1/r, into the 20 coefficients of fnode */
+static void compute_M2L(double dx, double dy, double dz, double dr1, double q0, const double *quads,
+                        double *fnode) {
+  double dr12 = dr1 * dr1;
+  double dx2 = dx * dx, dx3 = dx * dx2, dy2 = dy * dy, dy3 = dy * dy2, dz2 = dz * dz, dz3 = dz * dz2;
+  double g0 = -dr1;
+  double g1 = 1. * dr12 * g0;
+  double g2 = -3. * dr12 * g1;
+  double g3 = -5. * dr12 * g2;
+  double g2dx = g2 * dx, g2dy = g2 * dy, g2dz = g2 * dz;
+  double D3[10], D2[6], D1[3];
+  D3[0] = 3. * g2dx + g3 * dx3;
+  D3[1] = g2dy + g3 * dx2 * dy;
+  D3[2] = g2dz + g3 * dx2 * dz;
+  D3[3] = g2dx + g3 * dy2 * dx;
+  D3[4] = g3 * dx * dy * dz;
+  D3[5] = g2dx + g3 * dz2 * dx;
+  D3[6] = 3. * g2dy + g3 * dy3;
+  D3[7] = g2dz + g3 * dy2 * dz;
+  D3[8] = g2dy + g3 * dz2 * dy;
+  D3[9] = 3. * g2dz + g3 * dz3;
+  D2[0] = g1 + g2 * dx2;
+  D2[1] = g2dx * dy;
+  D2[2] = g2dx * dz;
+  D2[3] = g1 + g2 * dy2;
+  D2[4] = g2dy * dz;
+  D2[5] = g1 + g2 * dz2;
+  D1[0] = g1 * dx; D1[1] = g1 * dy; D1[2] = g1 * dz;
+  double qxx = quads[0], qxy = quads[1], qxz = quads[2], qyy = quads[3], qyz = quads[4], qzz = quads[5];
+  fnode[0] = fnode[0] + (D1[0] * q0 + 0.5 * (D3[0] * qxx + 2. * (D3[1] * qxy + D3[2] * qxz + D3[4] * qyz) +
+                                             D3[3] * qyy + D3[5] * qzz));
+  fnode[1] = fnode[1] + (D1[1] * q0 + 0.5 * (D3[1] * qxx + 2. * (D3[3] * qxy + D3[4] * qxz + D3[7] * qyz) +
+                                             D3[6] * qyy + D3[8] * qzz));
+  fnode[2] = fnode[2] + (D1[2] * q0 + 0.5 * (D3[2] * qxx + 2. * (D3[4] * qxy + D3[5] * qxz + D3[8] * qyz) +
+                                             D3[7] * qyy + D3[9] * qzz));
+  for (int k = 0; k < 6; k++) fnode[3 + k] = fnode[3 + k] + D2[k] * q0;
+  for (int k = 0; k < 10; k++) fnode[9 + k] = fnode[9 + k] + D3[k] * q0;
+  fnode[19] = fnode[19] + g0 * q0 -
+              0.5 * (D2[0] * qxx + D2[3] * qyy + D2[5] * qzz + 2 * (D2[1] * qxy + D2[2] * qxz + D2[4] * qyz));
+}
+
+/* propagate_fnode_to_node (kdtree.F90:1527-1560): shift the expansion
+ * fs about the parent centre by (dx,dy,dz) into f */
+static void propagate_fnode_to_node(double *f, const double *fs, double dx, double dy, double dz) {
+  f[0] = fs[0] + dx * (fs[3] + 0.5 * (dx * fs[9] + dy * fs[10] + dz * fs[11])) +
+         dy * (fs[4] + 0.5 * (dx * fs[10] + dy * fs[12] + dz * fs[13])) +
+         dz * (fs[5] + 0.5 * (dx * fs[11] + dy * fs[13] + dz * fs[14]));
+  f[1] = fs[1] + dx * (fs[4] + 0.5 * (dx * fs[10] + dy * fs[12] + dz * fs[13])) +
+         dy * (fs[6] + 0.5 * (dx * fs[12] + dy * fs[15] + dz * fs[16])) +
+         dz * (fs[7] + 0.5 * (dx * fs[13] + dy * fs[16] + dz * fs[17]));
+  f[2] = fs[2] + dx * (fs[5] + 0.5 * (dx * fs[11] + dy * fs[13] + dz * fs[14])) +
+         dy * (fs[7] + 0.5 * (dx * fs[13] + dy * fs[16] + dz * fs[17])) +
+         dz * (fs[8] + 0.5 * (dx * fs[14] + dy * fs[17] + dz * fs[18]));
+  f[3] = fs[3] + dx * fs[9] + dy * fs[10] + dz * fs[11];
+  f[4] = fs[4] + dx * fs[10] + dy * fs[12] + dz * fs[13];
+  f[5] = fs[5] + dx * fs[11] + dy * fs[13] + dz * fs[14];
+  f[6] = fs[6] + dx * fs[12] + dy * fs[15] + dz * fs[16];
+  f[7] = fs[7] + dx * fs[13] + dy * fs[16] + dz * fs[17];
+  f[8] = fs[8] + dx * fs[14] + dy * fs[17] + dz * fs[18];
+  for (int k = 9; k < 19; k++) f[k] = fs[k];
+  f[19] = fs[19] + dx * (fs[0] + 0.5 * (dx * fs[3] + dy * fs[4] + dz * fs[5])) +
+          dy * (fs[1] + 0.5 * (dx * fs[4] + dy * fs[6] + dz * fs[7])) +
+          dz * (fs[2] + 0.5 * (dx * fs[5] + dy * fs[7] + dz * fs[8]));
+}
+
+/* expand_fgrav_in_taylor_series (kdtree.F90:1799-1840), including the
+ * reference's use of dfxy in the z term of the first potential bracket */
+static void expand_fgrav(const double *fn, double dx, double dy, double dz, double *fx, double *fy, double *fz,
+                         double *pot) {
+  double fxi = fn[0], fyi = fn[1], fzi = fn[2];
+  double dfxx = fn[3], dfxy = fn[4], dfxz = fn[5], dfyy = fn[6], dfyz = fn[7], dfzz = fn[8];
+  double d2fxxx = fn[9], d2fxxy = fn[10], d2fxxz = fn[11], d2fxyy = fn[12], d2fxyz = fn[13], d2fxzz = fn[14];
+  double d2fyyy = fn[15], d2fyyz = fn[16], d2fyzz = fn[17], d2fzzz = fn[18];
+  double poti = fn[19];
+  fxi = fxi + dx * (dfxx + 0.5 * (dx * d2fxxx + dy * d2fxxy + dz * d2fxxz)) +
+        dy * (dfxy + 0.5 * (dx * d2fxxy + dy * d2fxyy + dz * d2fxyz)) +
+        dz * (dfxz + 0.5 * (dx * d2fxxz + dy * d2fxyz + dz * d2fxzz));
+  fyi = fyi + dx * (dfxy + 0.5 * (dx * d2fxxy + dy * d2fxyy + dz * d2fxyz)) +
+        dy * (dfyy + 0.5 * (dx * d2fxyy + dy * d2fyyy + dz * d2fyyz)) +
+        dz * (dfyz + 0.5 * (dx * d2fxyz + dy * d2fyyz + dz * d2fyzz));
+  fzi = fzi + dx * (dfxz + 0.5 * (dx * d2fxxz + dy * d2fxyz + dz * d2fxzz)) +
+        dy * (dfyz + 0.5 * (dx * d2fxyz + dy * d2fyyz + dz * d2fyzz)) +
+        dz * (dfzz + 0.5 * (dx * d2fxzz + dy * d2fyzz + dz * d2fzzz));
+  poti = poti - (dx * (fxi - 0.5 * (dx * dfxx + dy * dfxy + dz * dfxy)) +
+                 dy * (fyi - 0.5 * (dx * dfxy + dy * dfyy + dz * dfyz)) +
+                 dz * (fzi - 0.5 * (dx * dfxz + dy * dfyz + dz * dfzz)));
+  *fx = fxi; *fy = fyi; *fz = fzi; *pot = poti;
+}
+
+/* cache_neighbours (kdtree.F90:1175-1213) of leaf n with offsets */
+static int cache_leaf(const orc_ctx *o, int32_t n, double xo, double yo, double zo, nlist_t *l) {
+  int32_t npnode = o->ir_hi[n] - o->ir_lo[n] + 1;
+  if (nlist_reserve(l, (int64_t)l->n + npnode)) return PHSPH_ERR_CAPACITY;
+  for (int32_t k = 0; k < npnode; k++) {
+    int32_t pos = o->ir_lo[n] + k;
+    int32_t idx = l->n + k;
+    l->list[idx] = abs(o->inodeparts[pos]);
+    l->cache[4 * idx + 0] = TC(o, pos, 0) + xo;
+    l->cache[4 * idx + 1] = TC(o, pos, 1) + yo;
+    l->cache[4 * idx + 2] = TC(o, pos, 2) + zo;
+    l->cache[4 * idx + 3] = 1. / TC(o, pos, 3);
+  }
+  l->n += npnode;
+  return PHSPH_OK;
+}
+
+/* getneigh_dual (kdtree.F90:1357-1461) with open_nodes (:1591-1645) and
+ * node_interaction (:1653-1694).  The reference caches a parent's
+ * per-level expansion (fnodecache) for the next leaf below it; the cached
+ * values are the very ones recomputed here, so the result is the same. */
+static int getneigh_dual(const orc_ctx *o, int32_t icell, nlist_t *l, double fnode[LENFGRAV]) {
+  int32_t branch[MAXDEPTH + 1];
+  int nparents = 1;
+  branch[1] = icell;
+  int32_t j = icell;
+  while (o->node[j].parent != 0) {
+    j = o->node[j].parent;
+    if (nparents + 1 > MAXDEPTH) return PHSPH_ERR_CAPACITY;
+    branch[++nparents] = j;
+  }
+  double fb[MAXDEPTH + 1][LENFGRAV];
+  memset(fb, 0, sizeof(fb));
+  const double tree_acc2 = o->p.tree_accuracy * o->p.tree_accuracy;
+  int32_t st_dst[4 * MAXDEPTH], st_src[4 * MAXDEPTH], st_br[4 * MAXDEPTH];
+  int istack = 1;
+  st_dst[0] = 1; st_src[0] = 1; st_br[0] = nparents;
+  l->n = 0;
+  while (istack > 0) {
+    istack--;
+    int32_t idst = st_dst[istack], isrc = st_src[istack], idstbranch = st_br[istack];
+    int stackit;
+    double xo = 0., yo = 0., zo = 0.;
+    if (idst == isrc) {
+      stackit = 1;
+    } else {
+      const phsph_kdnode *nd = &o->node[idst], *ns = &o->node[isrc];
+      double dx, dy, dz, r2;
+      get_sep(o, nd->xcen, ns->xcen, &dx, &dy, &dz, &xo, &yo, &zo, &r2);
+      double rcut_dst = nd->hmax * o->radkern, rcut_src = ns->hmax * o->radkern;
+      double rcut = (rcut_dst > rcut_src) ? rcut_dst : rcut_src;
+      double ss = nd->size + ns->size;
+      double rcut2 = (ss + rcut) * (ss + rcut);
+      int wellsep = (tree_acc2 * r2 > ss * ss) && (r2 > rcut2);
+      if (wellsep) {
+        double dr1 = 1. / sqrt(r2);
+        compute_M2L(dx, dy, dz, dr1, ns->mass, &o->quads[6 * (size_t)isrc], fb[idstbranch]);
+        stackit = 0;
+      } else {
+        stackit = 1;
+      }
+    }
+    if (!stackit) continue;
+    /* open_nodes */
+    const phsph_kdnode *ns = &o->node[isrc];
+    int ibranchnext, isdstleaf;
+    if (idstbranch - 1 > 0) { ibranchnext = idstbranch - 1; isdstleaf = 0; }
+    else { ibranchnext = idstbranch; isdstleaf = 1; }
+    int32_t idstnext = branch[ibranchnext];
+    if (o->leaf_is_active[isrc] != 0) {
+      if (isdstleaf) {
+        int rc = cache_leaf(o, isrc, xo, yo, zo, l);
+        if (rc) return rc;
+      } else {
+        if (istack + 1 > 4 * MAXDEPTH) return PHSPH_ERR_CAPACITY;
+        st_dst[istack] = idstnext; st_src[istack] = isrc; st_br[istack] = ibranchnext; istack++;
+      }
+    } else {
+      if (istack + 2 > 4 * MAXDEPTH) return PHSPH_ERR_CAPACITY;
+      if (ns->leftchild != 0) { st_dst[istack] = idstnext; st_src[istack] = ns->leftchild; st_br[istack] = ibranchnext; istack++; }
+      if (ns->rightchild != 0) { st_dst[istack] = idstnext; st_src[istack] = ns->rightchild; st_br[istack] = ibranchnext; istack++; }
+    }
+  }
+  /* downward pass: shift each level's expansion to the next centre */
+  double facc[LENFGRAV], ftmp[LENFGRAV];
+  memset(facc, 0, sizeof(facc));
+  for (int i = nparents; i >= 2; i--) {
+    int32_t iparent = branch[i];
+    double dx, dy, dz, xo, yo, zo, r2;
+    get_sep(o, o->node[branch[i - 1]].xcen, o->node[iparent].xcen, &dx, &dy, &dz, &xo, &yo, &zo, &r2);
+    for (int k = 0; k < LENFGRAV; k++) ftmp[k] = facc[k] + fb[i][k];
+    propagate_fnode_to_node(facc, ftmp, dx, dy, dz);
+  }
+  for (int k = 0; k < LENFGRAV; k++) fnode[k] = facc[k] + fb[1][k];
   return PHSPH_OK;
 }
 
@@ -1047,6 +1260,7 @@ typedef struct {
   double Bx_e, By_e, Bz_e, psi, divB;   /* Bevol and divcurlB(1) */
   double sxx, sxy, sxz, syy, syz, szz;  /* MHD stress / rho^2 terms */
   double eps, tstop;
+  double gradsoft;                      /* gradh(2) (self-gravity) */
 } fpart_t;
 
 /* get_stress (force.F90:2068-2168) without physical viscosity */
@@ -1088,6 +1302,7 @@ static void compute_forces(const orc_ctx *o, int32_t i, const fpart_t *pi, const
   const double beta = o->p.beta, alphau = o->p.alphau, alphaB = o->p.alphaB;
   const int mhd = o->p.mhd, dust = o->p.use_dustfrac && o->p.ndustsmall > 0;
   const int thermal = o->p.maxvxyzu >= 4;
+  const int grav = o->p.gravity != 0;
   double hi = pi->h, hi1 = 1. / hi, hi21 = hi1 * hi1, hi41 = hi21 * hi21;
   double gradhi = pi->gradh;
   double rhoi = pi->rho, rho1i = 1. / rhoi;
@@ -1132,21 +1347,57 @@ static void compute_forces(const orc_ctx *o, int32_t i, const fpart_t *pi, const
     double q2i = rij2 * hi21;
     double hj21 = hj1 * hj1;
     double q2j = rij2 * hj21;
-    if (!(q2i < o->radkern2 || q2j < o->radkern2)) continue;
+    if (!(q2i < o->radkern2 || q2j < o->radkern2)) {
+      if (grav) {
+        /* trial neighbour outside both kernels: 1/r^2 (force.F90:1991-2053) */
+        double rij1 = 1. / sqrt(rij2);
+        double fgrav = rij1 * rij1 * rij1;
+        double phii = -rij1;
+        double fgravj = fgrav * pm;
+        fsum[F_FX] = fsum[F_FX] - dx * fgravj;
+        fsum[F_FY] = fsum[F_FY] - dy * fgravj;
+        fsum[F_FZ] = fsum[F_FZ] - dz * fgravj;
+        fsum[F_POT] = fsum[F_POT] + pm * phii;
+      }
+      continue;
+    }
     double rij1, qi;
     if (rij2 > TINY64) { rij1 = 1. / sqrt(rij2); qi = (rij2 * rij1) * hi1; }
     else { rij1 = 0.; qi = 0.; }
     double grkerni = 0., grkernj = 0.;
-    if (q2i < o->radkern2) grkerni = grkern(kind, q2i, qi) * hfacgrkern;
+    double phii = 0., fgravi = 0., fgravj_s = 0.;
+    if (q2i < o->radkern2) {
+      grkerni = grkern(kind, q2i, qi) * hfacgrkern;
+      if (grav) {
+        double fmi;
+        kernel_softening(kind, q2i, qi, &phii, &fmi);
+        phii = phii * hi1;
+        fmi = fmi * hi21;
+        double dsofti = pi->gradsoft * grkerni;
+        fgravi = fmi + dsofti;
+      }
+    } else if (grav) {
+      phii = -rij1;
+      fgravi = rij1 * rij1;
+    }
     int usej;
     const double gradhj = (double)in->gradh[(size_t)in->ldgradh * (j - 1)];
     if (q2j < o->radkern2) {
       double qj = (rij2 * rij1) * hj1;
       grkernj = grkern(kind, q2j, qj) * hj21 * hj21 * o->cnormk * gradhj;
       usej = 1;
+      if (grav) {
+        double phij, fmj;
+        kernel_softening(kind, q2j, qj, &phij, &fmj);
+        fmj = fmj * hj21;
+        double dsoftj = (double)in->gradh[(size_t)in->ldgradh * (j - 1) + 1] * grkernj;
+        fgravj_s = fmj + dsoftj;
+      }
     } else {
       grkernj = 0.; usej = 0;
+      if (grav) fgravj_s = rij1 * rij1;
     }
+    double fgrav = grav ? 0.5 * (pm * fgravi + pm * fgravj_s) : 0.;
     if (mhd) usej = 1;
     if (dust) usej = 1;
     if (thermal && !o->p.gravity) usej = 1;
@@ -1254,10 +1505,10 @@ static void compute_forces(const orc_ctx *o, int32_t i, const fpart_t *pi, const
       projszj = (sj[2] * runix + sj[4] * runiy + sj[5] * runiz) * grkernj;
     }
     double gradp = gradpi + gradpj;
-    double fgrav = 0.;
     fsum[F_FX] = fsum[F_FX] - runix * (gradp + fgrav) - (projsxi + projsxj);
     fsum[F_FY] = fsum[F_FY] - runiy * (gradp + fgrav) - (projsyi + projsyj);
     fsum[F_FZ] = fsum[F_FZ] - runiz * (gradp + fgrav) - (projszi + projszj);
+    if (grav) fsum[F_POT] = fsum[F_POT] + pm * phii;
     fsum[F_DRHODT] = fsum[F_DRHODT] + projv * grkerni;
     if (thermal) {
       fsum[F_DUDTDISS] = fsum[F_DUDTDISS] + dudtdissi + dudtresist;
@@ -1362,10 +1613,13 @@ int orc_force(orc_ctx *o, int32_t icall, int32_t np, const double *xyzh, const d
         q->alpha = (double)alphaind[3 * (size_t)(i - 1)];
         double g = (double)gradh[(size_t)ldgradh * (i - 1)];
         q->gradh = (g > 0.) ? g : 1.;
+        q->gradsoft = (o->p.gravity && ldgradh > 1) ? (double)gradh[(size_t)ldgradh * (i - 1) + 1] : 0.;
       }
       if (npcell == 0) continue;
       const phsph_kdnode *nd = &o->node[icell];
-      int rc = getneigh(o, nd->xcen, nd->size, o->radkern * nd->hmax, 1, &l);
+      double fnode[LENFGRAV];
+      int rc = o->p.gravity ? getneigh_dual(o, (int32_t)icell, &l, fnode)
+                            : getneigh(o, nd->xcen, nd->size, o->radkern * nd->hmax, 1, &l);
       if (rc) { rc_all = rc; continue; }
       for (int k = 0; k < npcell; k++) {
         const fpart_t *q = &parts[k];
@@ -1376,6 +1630,19 @@ int orc_force(orc_ctx *o, int32_t icall, int32_t np, const double *xyzh, const d
         /* finish_cell_and_store_results */
         double hi = q->h, hi1 = 1. / hi, rhoi = q->rho, rho1i = 1. / rhoi;
         double dtc = o->p.dtmax, dtf = BIGNUMBER, dtclean = BIGNUMBER, dtdusti = BIGNUMBER;
+        if (o->p.gravity) {
+          /* self-contribution and the far field (force.F90:2909-2927) */
+          double potensoft0, dum;
+          kernel_softening(o->p.kernel, 0., 0., &potensoft0, &dum);
+          double epoti = 0.5 * pm * (fsum[F_POT] + pm * potensoft0 * hi1);
+          double fxi, fyi, fzi, poti;
+          expand_fgrav(fnode, q->x - nd->xcen[0], q->y - nd->xcen[1], q->z - nd->xcen[2], &fxi, &fyi, &fzi, &poti);
+          fsum[F_FX] = fsum[F_FX] + fxi;
+          fsum[F_FY] = fsum[F_FY] + fyi;
+          fsum[F_FZ] = fsum[F_FZ] + fzi;
+          epoti = epoti + 0.5 * pm * poti;
+          if (o->poten) o->poten[i - 1] = (float)epoti;
+        }
         if (mhd) {
           double Bx = q->Bx_e * rhoi, By = q->By_e * rhoi, Bz = q->Bz_e * rhoi;
           double B2i = Bx * Bx + By * By + Bz * Bz;
@@ -1501,6 +1768,17 @@ void orc_set_bins(orc_ctx *o, int8_t *ibin, const int8_t *ibin_old, int8_t *ibin
 }
 
 int32_t orc_get_nbinmax(const orc_ctx *o) { return o->nbinmaxnew; }
+
+/* register the potential energy output of force (part:poten, real*4) */
+void orc_set_poten(orc_ctx *o, float *poten) { o->poten = poten; }
+
+/* node second moments (quads) of the tree, 6 per node, node ids 1..ncells */
+int orc_get_quads(const orc_ctx *o, double *quads, int64_t cap) {
+  if (!o->quads) return PHSPH_ERR_STATE;
+  for (int64_t n = 1; n <= o->ncells && n - 1 < cap; n++)
+    for (int k = 0; k < 6; k++) quads[6 * (n - 1) + k] = o->quads[6 * n + k];
+  return PHSPH_OK;
+}
 
 /* set_active_particles (utils_indtimesteps.f90:114-185): a live particle is
  * active when mod(istepfrac, 2**(nbinmax-ibin)) == 0; iphase = +-1 (gas).

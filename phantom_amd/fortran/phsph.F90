@@ -42,6 +42,7 @@ module phsph
  public :: phsph_default, phsph_init, phsph_finalise, phsph_errmsg
  public :: phsph_maketree, phsph_densityiterate, phsph_cons2prim, phsph_force
  public :: phsph_neighbour_stats
+ public :: phsph_set_bins, phsph_get_nbinmax, phsph_set_active_particles
 
  interface
     subroutine c_default_params(p, kernel) bind(c, name='phsph_default_params')
@@ -98,6 +99,24 @@ module phsph
        real(c_double), intent(out) :: trialmean, actualmean
        integer(c_int32_t), intent(out) :: maxtrial, maxactual
        integer(c_int64_t), intent(out) :: nrhocalc, nactualtot
+    end function
+    integer(c_int) function c_set_bins(ctx, ibin, ibin_old, ibin_wake, nbinmax, istepfrac, ibinnow) &
+                                       bind(c, name='phsph_set_bins')
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx, ibin, ibin_old, ibin_wake
+       integer(c_int32_t), value :: nbinmax, istepfrac, ibinnow
+    end function
+    integer(c_int) function c_get_nbinmax(ctx, nbinmax) bind(c, name='phsph_get_nbinmax')
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: nbinmax
+    end function
+    integer(c_int) function c_set_active(ctx, np, xyzh, ldx, iphase, ibin, nbinmax, istepfrac, nactive, nalive, &
+                                         ibinnow) bind(c, name='phsph_set_active_particles')
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx, xyzh, iphase, ibin
+       integer(c_int32_t), value :: np, ldx, nbinmax, istepfrac
+       integer(c_int32_t), intent(out) :: nactive, nalive, ibinnow
     end function
  end interface
 
@@ -222,5 +241,38 @@ subroutine phsph_neighbour_stats(trialmean, actualmean, maxtrial, maxactual, nrh
  ierr = c_neighbour_stats(ctx, tm, am, mt, ma, nr, na)
  trialmean = tm; actualmean = am; maxtrial = mt; maxactual = ma; nrhocalc = nr; nactualtot = na
 end subroutine phsph_neighbour_stats
+
+!-- block timesteps: register part:ibin, ibin_old, ibin_wake and the
+!   timestep_ind state (nbinmax, istepfrac, ibinnow) for the next force calls
+integer function phsph_set_bins(ibin, ibin_old, ibin_wake, nbinmax, istepfrac, ibinnow) result(ierr)
+ integer(c_int8_t), intent(inout), target :: ibin(:), ibin_wake(:)
+ integer(c_int8_t), intent(in),    target :: ibin_old(:)
+ integer,           intent(in)            :: nbinmax, istepfrac, ibinnow
+ ierr = c_set_bins(ctx, c_loc(ibin), c_loc(ibin_old), c_loc(ibin_wake), int(nbinmax, c_int32_t), &
+                   int(istepfrac, c_int32_t), int(ibinnow, c_int32_t))
+end function phsph_set_bins
+
+!-- nbinmax after the last force call (force.F90:792)
+integer function phsph_get_nbinmax(nbinmax) result(ierr)
+ integer(kind=1), intent(out) :: nbinmax
+ integer(c_int32_t) :: nb
+ ierr = c_get_nbinmax(ctx, nb)
+ nbinmax = int(nb, kind=1)
+end function phsph_get_nbinmax
+
+!-- set_active_particles (utils_indtimesteps.f90:114)
+integer function phsph_set_active_particles(npart, xyzh, iphase, ibin, nbinmax, istepfrac, nactive, nalive, &
+                                            ibinnow) result(ierr)
+ integer,           intent(in)            :: npart, nbinmax, istepfrac
+ real(c_double),    intent(in),    target :: xyzh(:,:)
+ integer(c_int8_t), intent(inout), target :: iphase(:)
+ integer(c_int8_t), intent(in),    target :: ibin(:)
+ integer,           intent(out)           :: nactive, nalive
+ integer(kind=1),   intent(out)           :: ibinnow
+ integer(c_int32_t) :: na, nl, bn
+ ierr = c_set_active(ctx, int(npart, c_int32_t), c_loc(xyzh), int(size(xyzh,1), c_int32_t), c_loc(iphase), &
+                     c_loc(ibin), int(nbinmax, c_int32_t), int(istepfrac, c_int32_t), na, nl, bn)
+ nactive = na; nalive = nl; ibinnow = int(bn, kind=1)
+end function phsph_set_active_particles
 
 end module phsph

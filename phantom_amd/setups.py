@@ -247,3 +247,88 @@ def with_bins(setup, nbinmax=3, istepfrac=2, seed=5, dtmax=0.016):
     s["bins"] = dict(ibin=ibin_old.copy(), ibin_old=ibin_old, ibin_wake=np.zeros(n, np.int8), nbinmax=nbinmax,
                      istepfrac=istepfrac, ibinnow=ibinnow)
     return s
+
+
+def lane_emden(nindex, dxi=1e-4):
+    """Lane-Emden solution theta(xi) for polytropic index nindex, integrated
+    with RK4 from the centre to the first zero xi1.  Returns (xi, theta,
+    dtheta/dxi) up to and including xi1."""
+    def rhs(xi, y):
+        th, dth = y
+        return np.array([dth, -max(th, 0.0) ** nindex - 2.0 * dth / xi])
+    xi = dxi
+    y = np.array([1.0 - xi * xi / 6.0, -xi / 3.0])
+    xs, ths, dths = [0.0, xi], [1.0, y[0]], [0.0, y[1]]
+    while y[0] > 0.0:
+        k1 = rhs(xi, y)
+        k2 = rhs(xi + 0.5 * dxi, y + 0.5 * dxi * k1)
+        k3 = rhs(xi + 0.5 * dxi, y + 0.5 * dxi * k2)
+        k4 = rhs(xi + dxi, y + dxi * k3)
+        ynew = y + dxi / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
+        if ynew[0] <= 0.0:  # linear interpolation to the surface
+            f = y[0] / (y[0] - ynew[0])
+            xi = xi + f * dxi
+            y = y + f * (ynew - y)
+            y[0] = 0.0
+        else:
+            xi, y = xi + dxi, ynew
+        xs.append(xi); ths.append(y[0]); dths.append(y[1])
+    return np.array(xs), np.array(ths), np.array(dths)
+
+
+def polytrope(n=8000, nindex=1.5, seed=42, kernel=abi.KERNEL_CUBIC, jitter=0.1, gamma=5.0 / 3.0,
+              pressureless=False, theta=0.5):
+    """Self-gravitating polytrope, M = R = G = 1: a jittered cubic lattice
+    cut to the unit sphere and stretched radially so that the enclosed mass
+    follows the Lane-Emden profile of index nindex; u = K rho^(1/n) n (the
+    polytropic relation P = K rho^(1+1/n) with gamma = 1 + 1/n for the
+    thermal energy).  pressureless=True sets an isothermal gas with c_s = 0
+    (pure gravity, for comparisons with direct summation)."""
+    rng = np.random.default_rng(seed)
+    # lattice with about n points inside the unit sphere
+    nside = int(round((n * 6.0 / math.pi) ** (1.0 / 3.0)))
+    dx = 2.0 / nside
+    g = (np.arange(nside) + 0.5) * dx - 1.0
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    pos += rng.uniform(-jitter * dx, jitter * dx, size=pos.shape)
+    r0 = np.sqrt((pos * pos).sum(axis=1))
+    pos = pos[r0 < 1.0]
+    r0 = r0[r0 < 1.0]
+    npart = pos.shape[0]
+    # stretch map: mass fraction r0^3 -> radius with the same enclosed mass
+    xi, th, dth = lane_emden(nindex)
+    xi1 = xi[-1]
+    menc = -(xi * xi) * dth            # proportional to the enclosed mass
+    menc = menc / menc[-1]
+    r_of_m = np.interp(r0 ** 3, menc, xi / xi1)
+    pos = pos * (r_of_m / np.maximum(r0, 1e-300))[:, None]
+    rho_c = 1.0 / (4.0 * math.pi * (1.0 / xi1) ** 3 * xi1 * xi1 * (-dth[-1]))
+    alpha = 1.0 / xi1
+    K = 4.0 * math.pi * alpha * alpha * rho_c ** (1.0 - 1.0 / nindex) / (nindex + 1.0)
+    rho = rho_c * np.interp(r_of_m * xi1, xi, th) ** nindex
+    p = abi.default_params(kernel)
+    p.periodic = 0
+    p.gravity = 1
+    p.tree_accuracy = theta
+    p.massoftype_gas = 1.0 / npart
+    p.maxp = npart
+    p.xmin, p.xmax, p.ymin, p.ymax, p.zmin, p.zmax = -2.0, 2.0, -2.0, 2.0, -2.0, 2.0
+    xyzh = np.empty((npart, 4))
+    xyzh[:, :3] = pos
+    rho_est = np.maximum(rho, 1e-3 * rho_c)
+    xyzh[:, 3] = p.hfact * (p.massoftype_gas / rho_est) ** (1.0 / 3.0)
+    if pressureless:
+        p.ieos = 1
+        p.polyk = 0.0
+        p.maxvxyzu = 3
+        vxyzu = np.zeros((npart, 3))
+    else:
+        p.ieos = 2
+        p.gamma = gamma
+        p.maxvxyzu = 4
+        vxyzu = np.zeros((npart, 4))
+        vxyzu[:, 3] = K * rho_est ** (1.0 / nindex) / (gamma - 1.0)
+    alphaind = np.zeros((npart, 3), dtype=np.float32)
+    return dict(xyzh=xyzh, vxyzu=vxyzu, alphaind=alphaind, params=p,
+                info=dict(workload="polytrope", n=npart, nindex=nindex, seed=seed, K=K, rho_c=rho_c))

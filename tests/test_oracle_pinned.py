@@ -300,3 +300,65 @@ def test_set_active_particles_known_answers():
     assert (na, nl, ibinnow) == (4, 4, 0)
     iphase, na, nl, ibinnow = oracle.set_active_particles(xyzh, ibin, 3, 1)
     assert list(iphase[:4]) == [-1, -1, -1, 1] and ibinnow == 3
+
+
+def _direct_gravity(xyzh, gradh, pm, kind):
+    """Pairwise softened gravity by direct summation, the sum directsum.f90
+    (src/tests/directsum.f90:48-197) evaluates: inside a particle's kernel
+    the softened kernel plus the grad-h softening term, outside 1/r^2."""
+    radkern2 = 4.0 if kind == abi.KERNEL_CUBIC else 9.0
+    cnormk = 1.0 / math.pi if kind == abi.KERNEL_CUBIC else 1.0 / (120.0 * math.pi)
+    soft = np.vectorize(lambda q: oracle.kernel_softening(kind, q))
+    grk = np.vectorize(lambda q: oracle.kernel(kind, q)[1])
+    n = xyzh.shape[0]
+    x, h = xyzh[:, :3], xyzh[:, 3]
+    g1, g2 = gradh[:, 0].astype(np.float64), gradh[:, 1].astype(np.float64)
+    acc = np.zeros((n, 3))
+    for i in range(n):
+        dx = x[i] - x
+        r2 = (dx * dx).sum(1)
+        r2[i] = 1.0
+        r1 = 1.0 / np.sqrt(r2)
+        f = np.empty((2, n))
+        for side, (hh, gg1, gg2) in enumerate(((h[i], g1[i], g2[i]), (h, g1, g2))):
+            hh1 = np.broadcast_to(1.0 / hh, (n,))
+            q = np.sqrt(r2) * hh1
+            inside = q * q < radkern2
+            inside[i] = False
+            fs = r1 * r1
+            if inside.any():
+                _, fsoft = soft(q[inside])
+                gk = grk(q[inside]) * hh1[inside] ** 4 * cnormk * np.broadcast_to(gg1, (n,))[inside]
+                fs = fs.copy()
+                fs[inside] = fsoft * hh1[inside] ** 2 + np.broadcast_to(gg2, (n,))[inside] * gk
+            f[side] = fs
+        fg = 0.5 * pm * (f[0] + f[1])
+        fg[i] = 0.0
+        acc[i] = -(dx * (fg * r1)[:, None]).sum(0)
+    return acc
+
+
+@pytest.mark.parametrize("theta,tol_max,tol_rms", [(0.0, 1e-12, 1e-13), (0.5, 2e-2, 3e-3)])
+def test_tree_gravity_vs_direct_sum(theta, tol_max, tol_rms):
+    """Tree gravity (dual walk, M2L, L2L, L2P, softening) against direct
+    summation on a pressureless n=1.5 polytrope (test_gravity.f90 with
+    directsum.f90): exact up to round-off when no node is well separated
+    (theta = 0), and within the quadrupole error at theta = 0.5.  Total
+    momentum is conserved to round-off."""
+    s = setups.polytrope(n=1500, pressureless=True, theta=theta)
+    r = oracle.run_all(s)
+    p = s["params"]
+    f = r["force"]["fxyzu"][:, :3]
+    a = _direct_gravity(r["xyzh"], r["dens"]["gradh"], p.massoftype_gas, p.kernel)
+    err = np.sqrt(((f - a) ** 2).sum(1)) / np.sqrt((a ** 2).sum(1))
+    assert err.max() <= tol_max and np.sqrt((err ** 2).mean()) <= tol_rms, (err.max(), np.sqrt((err ** 2).mean()))
+    assert np.abs(f.sum(0)).max() <= 1e-12 * np.abs(f).sum(0).max()
+
+
+def test_polytrope_potential_energy():
+    """The gravitational energy of the n=1.5 polytrope, -3/(5-n) G M^2/R
+    = -6/7, is reproduced to the resolution of 1500 particles."""
+    s = setups.polytrope(n=1500, pressureless=True, theta=0.5)
+    r = oracle.run_all(s)
+    epot = float(r["force"]["poten"].astype(np.float64).sum())
+    assert abs(epot - (-6.0 / 7.0)) < 0.02 * 6.0 / 7.0, epot
