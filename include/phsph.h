@@ -152,6 +152,13 @@ int phsph_maketree(phsph_ctx *ctx, double *xyzh, int32_t ldxyzh, int32_t np,
 int phsph_get_tree(phsph_ctx *ctx, phsph_kdnode *node, int32_t *inoderange,
                    int32_t *inodeparts, int64_t cap_nodes, int32_t *maxlevel);
 
+/* Second moments quads(6) = sum m dx dx (xx, xy, xz, yy, yz, zz) about each
+ * node's centre of mass, for node ids 1..min(cap_nodes, ncells), row-major
+ * (6 per node).  The reference stores them in node(i)%quads under GRAVITY
+ * (kdtree.F90:724-750, dtype_kdtree.F90:66).  Unreachable node slots hold
+ * unspecified values.  Only with params.gravity. */
+int phsph_get_quads(phsph_ctx *ctx, double *quads, int64_t cap_nodes);
+
 /*
  * densityiterate (dens.F90:117-152).  icall 0 = one summation pass without
  * h iteration, 1 = converge h-rho (Newton-Raphson, tolh, clamp [0.8h,1.2h]).
@@ -209,6 +216,17 @@ int phsph_force(phsph_ctx *ctx, int32_t icall, int32_t np,
                 const float *gradh, int32_t ldgradh, const float *alphaind,
                 const double *eos_vars, int32_t ldeos, const int8_t *iphase,
                 double *dtcourant, double *dtforce);
+
+/*
+ * Self-gravity potential energy output.  The reference accumulates
+ * poten(i) = epoti in force, with epoti = 0.5*m*(phi_pairs + m*potensoft0/h)
+ * + 0.5*m*phi_tree (force.F90:2912-2927); it is module state of part, so the array
+ * (np floats, host or device) is registered once and written by every
+ * phsph_force call while params.gravity is set.  Passing NULL stops the
+ * output.  The array must stay valid until the next phsph_set_poten or
+ * phsph_destroy.
+ */
+int phsph_set_poten(phsph_ctx *ctx, float *poten);
 
 /*
  * Block (individual) timesteps.  The reference keeps the bins as module

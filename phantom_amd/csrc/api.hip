@@ -40,6 +40,7 @@ DevConst make_devconst(const phsph_params &p) {
   d.alpha = p.alpha; d.alphamax = p.alphamax; d.alphau = p.alphau; d.alphaB = p.alphaB; d.beta = p.beta;
   d.C_cour = p.C_cour; d.C_force = p.C_force; d.dtmax = p.dtmax; d.psidecayfac = p.psidecayfac;
   d.overcleanfac = p.overcleanfac; d.K_code = p.K_code;
+  d.tree_acc2 = p.tree_accuracy * p.tree_accuracy;
   return d;
 }
 
@@ -62,7 +63,10 @@ static const char *validate(const phsph_params *p) {
   if (p->use_dustfrac && (p->ndustsmall < 0 || p->ndustsmall > 1)) return "ndustsmall must be 0 or 1";
   if (p->use_dustfrac && p->ndustsmall == 1 && p->idrag != 2 && p->idrag != 3)
     return "one-fluid dust supports idrag 2 (constant K) and 3 (constant stopping time)";
-  if (p->gravity) return "self-gravity is not available in this build";
+  if (p->gravity && (p->mhd || (p->use_dustfrac && p->ndustsmall > 0)))
+    return "self-gravity is available for hydrodynamics (not with MHD or dust)";
+  // read only with gravity, in 0..1 (neigh_kdtree.f90:345)
+  if (p->gravity && !(p->tree_accuracy >= 0. && p->tree_accuracy <= 1.)) return "tree_accuracy must be in 0..1";
   if (p->precision != PHSPH_PREC_FP64 && p->precision != PHSPH_PREC_MIXED) return "unknown precision";
   return nullptr;
 }
@@ -234,8 +238,16 @@ int phsph_get_tree(phsph_ctx *c, phsph_kdnode *node, int32_t *inoderange, int32_
   if (!c->tree_valid) PH_RET(c, PHSPH_ERR_STATE, "get_tree: no tree");
   PH_HIPCHECK(c, hipSetDevice(c->device));
   const int64_t n = std::min<int64_t>(cap_nodes, c->ncells);
-  if (node && n > 0)
+  if (node && n > 0) {
     PH_HIPCHECK(c, hipMemcpy(node, c->nd.p + 1, n * sizeof(phsph_kdnode), hipMemcpyDeviceToHost));
+    if (c->p.gravity && c->ghmax.p) {
+      // with self-gravity the node hmax follows the density relists
+      // (set_hmaxcell), as the reference's node array does
+      std::vector<unsigned long long> hm(n);
+      PH_HIPCHECK(c, hipMemcpy(hm.data(), c->ghmax.p + 1, n * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+      for (int64_t k = 0; k < n; k++) memcpy(&node[k].hmax, &hm[k], sizeof(double));
+    }
+  }
   if (inoderange && n > 0) {
     std::vector<int32_t> lo(n), hi(n), fl(n);
     PH_HIPCHECK(c, hipMemcpy(lo.data(), c->nd_lo.p + 1, n * sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -266,6 +278,18 @@ int phsph_get_tree(phsph_ctx *c, phsph_kdnode *node, int32_t *inoderange, int32_
   return PHSPH_OK;
 }
 
+int phsph_get_quads(phsph_ctx *c, double *quads, int64_t cap_nodes) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (!c->tree_valid) PH_RET(c, PHSPH_ERR_STATE, "get_quads: no tree");
+  if (!c->p.gravity || !c->quads.p) PH_RET(c, PHSPH_ERR_STATE, "get_quads: self-gravity is off");
+  if (!quads) PH_RET(c, PHSPH_ERR_INPUT, "get_quads: missing array");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  const int64_t n = std::min<int64_t>(cap_nodes, c->ncells);
+  if (n > 0)
+    PH_HIPCHECK(c, hipMemcpy(quads, c->quads.p + 6, (size_t)n * 6 * sizeof(double), hipMemcpyDeviceToHost));
+  return PHSPH_OK;
+}
+
 int phsph_densityiterate(phsph_ctx *c, int32_t icall, int32_t np, double *xyzh, int32_t ldx, const double *vxyzu,
                          int32_t ldv, const double *fxyzu, int32_t ldf, const double *fext, const double *Bevol,
                          const int8_t *iphase, float *divcurlv, int32_t lddcv, float *divcurlB, float *alphaind,
@@ -275,6 +299,7 @@ int phsph_densityiterate(phsph_ctx *c, int32_t icall, int32_t np, double *xyzh, 
   if (icall != 0 && icall != 1) PH_RET(c, PHSPH_ERR_INPUT, "densityiterate: icall must be 0 or 1");
   if (!xyzh || !vxyzu || !divcurlv || !gradh || ldx < 4 || ldv < 3 || lddcv < 1 || ldgradh < 1)
     PH_RET(c, PHSPH_ERR_INPUT, "densityiterate: missing array");
+  if (c->p.gravity && ldgradh < 2) PH_RET(c, PHSPH_ERR_INPUT, "densityiterate: self-gravity needs gradh(2,:)");
   if (c->p.mhd && !Bevol) PH_RET(c, PHSPH_ERR_INPUT, "densityiterate: MHD needs Bevol");
   PH_HIPCHECK(c, hipSetDevice(c->device));
   ctx_clear_timings(c);
@@ -343,6 +368,7 @@ int phsph_force(phsph_ctx *c, int32_t icall, int32_t np, const double *xyzh, int
       ldf < 3 || lddcv < 1 || ldgradh < 1 || ldeos < 2)
     PH_RET(c, PHSPH_ERR_INPUT, "force: missing array");
   if (c->p.maxvxyzu >= 4 && (ldv < 4 || ldf < 4)) PH_RET(c, PHSPH_ERR_INPUT, "force: vxyzu/fxyzu need 4 components");
+  if (c->p.gravity && ldgradh < 2) PH_RET(c, PHSPH_ERR_INPUT, "force: self-gravity needs gradh(2,:)");
   const bool mhd = c->p.mhd, dust = c->p.use_dustfrac && c->p.ndustsmall > 0;
   if (mhd && (!Bevol || !dBevol || !divcurlB)) PH_RET(c, PHSPH_ERR_INPUT, "force: MHD needs Bevol, dBevol, divcurlB");
   if (dust && (!dustfrac || !ddustevol || !deltav)) PH_RET(c, PHSPH_ERR_INPUT, "force: dust needs dust arrays");
@@ -391,9 +417,17 @@ int phsph_force(phsph_ctx *c, int32_t icall, int32_t np, const double *xyzh, int
     a.istepfrac = c->bins_istepfrac;
     a.ibinnow = c->bins_ibinnow;
   }
+  c->poten_out = nullptr;
+  if (c->p.gravity && c->poten_user) STAGE(st, c->poten_user, (size_t)np, false, true, &c->poten_out);
   int rc = force_run(c, icall, a, dtcourant, dtforce);
   int rc2 = st.finish();
   return rc ? rc : rc2;
+}
+
+int phsph_set_poten(phsph_ctx *c, float *poten) {
+  if (!c) return PHSPH_ERR_INPUT;
+  c->poten_user = poten;
+  return PHSPH_OK;
 }
 
 int phsph_set_bins(phsph_ctx *c, int8_t *ibin, const int8_t *ibin_old, int8_t *ibin_wake, int32_t nbinmax,

@@ -30,6 +30,7 @@ struct DevConst {
   double hfact, tolh, pmass, gamma, polyk;
   double alpha, alphamax, alphau, alphaB, beta;
   double C_cour, C_force, dtmax, psidecayfac, overcleanfac, K_code;
+  double tree_acc2;  // tree_accuracy^2 (opening angle of the gravity walk)
 };
 
 // ---- exact (non-contracted) FP64 arithmetic ----------------------------
@@ -157,4 +158,50 @@ __device__ __forceinline__ double wave_min(double v) {
 __device__ __forceinline__ int wave_sum_i(int v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
+}
+
+// softened potential and force of a unit mass at q = r/h
+// (kernel_softening, kernel_cubic.f90:103-121, kernel_quintic.f90:124-152)
+template <int K>
+__device__ __forceinline__ void kern_soft(double q2, double q, double &pot, double &fs) {
+  if (K == PHSPH_KERNEL_CUBIC) {
+    if (q < 1.) {
+      const double q4 = q2 * q2;
+      pot = q4 * q / 10. - 3. * q4 / 10. + 2. * q2 / 3. - 7. / 5.;
+      fs = q * (15. * q2 * q - 36. * q2 + 40.) / 30.;
+    } else if (q < 2.) {
+      const double q4 = q2 * q2, q6 = q4 * q2;
+      pot = (q * (-q4 * q + 9. * q4 - 30. * q2 * q + 40. * q2 - 48.) + 2.) / (30. * q);
+      fs = (-5. * q6 + 36. * q4 * q - 90. * q4 + 80. * q2 * q - 2.) / (30. * q2);
+    } else {
+      pot = -1. / q;
+      fs = 1. / q2;
+    }
+  } else {
+    if (q < 1.) {
+      const double q4 = q2 * q2, q6 = q4 * q2;
+      pot = -q6 * q / 168. + q6 / 42. - q4 / 10. + 11. * q2 / 30. - 239. / 210.;
+      fs = q * (-35. * q4 * q + 120. * q4 - 336. * q2 + 616.) / 840.;
+    } else if (q < 2.) {
+      const double q4 = q2 * q2, q6 = q4 * q2, q8 = q6 * q2;
+      pot = (q * (5. * q6 * q - 60. * q6 + 280. * q4 * q - 588. * q4 + 350. * q2 * q + 476. * q2 - 1892.) - 5.) /
+            (1680. * q);
+      fs = (35. * q8 - 360. * q6 * q + 1400. * q6 - 2352. * q4 * q + 1050. * q4 + 952. * q2 * q + 5.) /
+           (1680. * q2);
+    } else if (q < 3.) {
+      const double q4 = q2 * q2, q6 = q4 * q2, q8 = q6 * q2;
+      pot = (q * (-q6 * q + 20. * q6 - 168. * q4 * q + 756. * q4 - 1890. * q2 * q + 2268. * q2 - 2916.) + 507.) /
+            (1680. * q);
+      fs = (-7. * q8 + 120. * q6 * q - 840. * q6 + 3024. * q4 * q - 5670. * q4 + 4536. * q2 * q - 507.) /
+           (1680. * q2);
+    } else {
+      pot = -1. / q;
+      fs = 1. / q2;
+    }
+  }
+}
+
+// positive doubles order as their bit patterns: max by integer atomics
+__device__ __forceinline__ void atomic_max_pos(unsigned long long *a, double v) {
+  atomicMax(a, (unsigned long long)__double_as_longlong(v));
 }

@@ -47,6 +47,7 @@ struct TimedLaunch {
 // re-run) when a launch reports overflow.
 struct Caps {
   int gcand = 512;   // candidate leaves per group of 64 particles
+  int p2p = 256;     // near-field leaves per leaf (self-gravity)
 };
 
 struct phsph_ctx {
@@ -119,6 +120,17 @@ struct phsph_ctx {
   bool bins_set = false;
   int32_t nbinmax_new = -1;   // result of the last force call with bins
   DBuf<int8_t> tbin_old;      // ibin_old in tree order
+
+  // self-gravity
+  DBuf<double> quads;                 // node second moments, 6 per slot
+  DBuf<unsigned long long> ghmax;     // node hmax as the force walk sees it
+  DBuf<double> htraj, hconv;          // h per density iteration, converged h
+  DBuf<double> fnode;                 // far-field expansion per leaf (20)
+  DBuf<uint32_t> p2p;                 // near-field leaves per leaf
+  DBuf<int32_t> np2p;
+  DBuf<float> gsoft;                  // gradh(2) in tree order
+  float *poten_user = nullptr;        // registered potential energy output (host or device)
+  float *poten_out = nullptr;         // device view of poten_user during a force call
 
   // staging buffers for host arrays
   DBuf<char> stage[24];
@@ -213,6 +225,11 @@ struct ForceIOArgs {  // device pointers in the caller's layout
   int nbinmax, istepfrac, ibinnow;
 };
 int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc, double *dtf);
+struct FRec;
+int grav_tree(phsph_ctx *c);
+int grav_hmax_replay(phsph_ctx *c, int icall);
+int grav_force(phsph_ctx *c, int icall, const ForceIOArgs &a, const FRec *rec, const float *gsoft, double *dtc_out,
+               double *dtf_out);
 int set_active_run(phsph_ctx *c, int np, const double *xyzh, int ldx, int8_t *iphase, const int8_t *ibin,
                    int nbinmax, int istepfrac, int32_t *nactive, int32_t *nalive);
 int cons2prim_run(phsph_ctx *c, int np, const double *xyzh, int ldx, const double *vxyzu, int ldv,

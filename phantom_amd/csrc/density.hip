@@ -279,7 +279,8 @@ k_density(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, 
           const int32_t *__restrict__ nd_hi, const phsph_kdnode *__restrict__ nd,
           const int32_t *__restrict__ iparts, DensIn in, double *__restrict__ th_out, DensOut out,
           int32_t *__restrict__ p_nneigh, int32_t *__restrict__ p_nits, int32_t *__restrict__ leaf_nits,
-          int32_t *__restrict__ retry, int32_t *__restrict__ flags) {
+          int32_t *__restrict__ retry, double *__restrict__ htraj, double *__restrict__ hconv,
+          int32_t *__restrict__ flags) {
   constexpr int NL = 16, NP = NL * PH_MINPART, NPB = MHD ? NP : 1;
   __shared__ BatchMeta<NL> M;
   __shared__ double sx[NP], sy[NP], sz[NP];
@@ -449,6 +450,7 @@ k_density(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, 
       state = ST_FAIL;
     } else {
       hi = hnew;
+      if (htraj && it <= HTRAJ) htraj[(size_t)p * HTRAJ + it - 1] = hi;
       if (dc.radkern * hi * (1. + 1e-12) > rcg) {  // outgrew the group's candidate stream
         atomicOr(flags, DFL_REWALK);
         state = ST_FAIL;
@@ -460,6 +462,7 @@ k_density(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, 
     if (state == ST_FAIL) retry[p] = 1;
     return;
   }
+  if (hconv) hconv[p] = hi;  // the h of the last summation (cell%h once converged)
   // --- store_results at h = hfin ---
   const double hf1 = 1. / hfin, hf21 = hf1 * hf1, hf31 = hf1 * hf21, hf41 = hf21 * hf21;
   const double rhoi = dc.cnormk * (rs_rho + dc.wab0 * pm) * hf31;
@@ -682,6 +685,11 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   PH_HIPCHECK(c, c->tmpi.ensure(n + 1));
   DensIn in{c->tp4.p, c->tv4.p, have_f ? c->tf4.p : nullptr, c->tB4.p};
   DensOut out{a.xyzh, a.ldx, a.divcurlv, a.lddcv, a.divcurlB, a.alphaind, a.gradh, a.ldgradh, a.dvdx};
+  const bool grav = c->p.gravity != 0;
+  if (grav) {
+    PH_HIPCHECK(c, c->htraj.ensure((size_t)n * HTRAJ + 1));
+    PH_HIPCHECK(c, c->hconv.ensure(n + 1));
+  }
   hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(c->nleaves, 256)), dim3(256), 0, s, c->leaf_nits.p, c->nleaves, 0);
   hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(n, 256)), dim3(256), 0, s, c->p_nits.p, n, -1);
   hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(n, 256)), dim3(256), 0, s, c->p_nneigh.p, n, -1);
@@ -705,7 +713,8 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   hipLaunchKernelGGL((k_density<KK, MM>), gr, b, 0, s, c->dc, icall, rfac, c->act.p, na, ng, c->pleaf.p,         \
                      c->leaves.p, c->gcand.p, c->gncand.p, c->caps.gcand, c->nd_lo.p, c->nd_hi.p, c->nd.p,       \
                      c->iparts.p, in, c->th.p, out, c->p_nneigh.p, c->p_nits.p,                                 \
-                     c->leaf_nits.p, c->tmpi.p, c->counters.p)
+                     c->leaf_nits.p, c->tmpi.p, grav ? c->htraj.p : nullptr, grav ? c->hconv.p : nullptr,         \
+                     c->counters.p)
       if (c->p.kernel == PHSPH_KERNEL_CUBIC) {
         if (mhd) DLAUNCH(PHSPH_KERNEL_CUBIC, true); else DLAUNCH(PHSPH_KERNEL_CUBIC, false);
       } else {
@@ -729,6 +738,10 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
     if (rc) return rc;
   }
   if (c->nact != 0) PH_RET(c, PHSPH_ERR_CAPACITY, "densityiterate: neighbour search could not be widened enough");
+  if (grav) {
+    rc = grav_hmax_replay(c, icall);
+    if (rc) return rc;
+  }
   // statistics (dens.F90:1670-1679); trial counts are computed on request
   PH_HIPCHECK(c, c->stats_i.ensure(8));
   long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};

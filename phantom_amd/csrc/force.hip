@@ -17,42 +17,9 @@
 // stopping time) are evaluated once per particle by k_force_prep with the
 // same arithmetic and packed into one 128-byte record per particle, so a
 // pair reads one cache line of j data.
-#include "stream.h"
+#include "records.h"
 
 enum { FFL_DTC = 32 };
-
-// j quantities of the hydro pair terms (tree order); cs and h are read for i
-struct __align__(16) FRec {
-  double x, y, z, hinv;
-  double vx, vy, vz, u;
-  double rho, rho1, pr, pro2;
-  double vwave;
-  float gradh, alpha;
-  double cs, h;
-};
-static_assert(sizeof(FRec) == 128, "FRec is one 128-byte line");
-// MHD: B = Bevol*rho, psi, the stress -m B(x)B/rho^2, and Bevol itself (i)
-struct __align__(16) FRecB {
-  double Bx, By, Bz, psi;
-  double sxx, sxy, sxz, syy;
-  double syz, szz, Bex, Bey;
-  double Bez, pad0, pad1, pad2;
-};
-static_assert(sizeof(FRecB) == 128, "FRecB is one 128-byte line");
-// one-fluid dust
-struct __align__(16) FRecD {
-  double eps, sqeps, ts, pad;
-};
-
-// get_ts (dust.f90:161-276), idrag 2 (constant K) and 3 (constant t_s)
-__device__ __forceinline__ double get_ts_d(const DevConst &dc, double rhogas, double rhodust) {
-  if (dc.idrag == 2) {
-    if (dc.K_code > 0.) return rhogas * rhodust / (dc.K_code * (rhogas + rhodust));
-    return PH_HUGE64;
-  }
-  if (dc.idrag == 3) return dc.K_code;
-  return 0.;
-}
 
 // per-particle j terms (force.F90:1455-1509 with get_stress)
 __global__ void k_force_prep(DevConst dc, int nlive, const int32_t *__restrict__ iparts,
@@ -63,7 +30,8 @@ __global__ void k_force_prep(DevConst dc, int nlive, const int32_t *__restrict__
                              int ldeos, const double *__restrict__ Bevol, const double *__restrict__ dustfrac,
                              double *__restrict__ th, FRec *__restrict__ rec,
                              FRecB *__restrict__ recB, FRecD *__restrict__ recD,
-                             const int8_t *__restrict__ ibin_old, int8_t *__restrict__ tbin_old) {
+                             const int8_t *__restrict__ ibin_old, int8_t *__restrict__ tbin_old,
+                             float *__restrict__ gsoft) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nlive) return;
   const int o = abs(iparts[p]) - 1;
@@ -71,6 +39,7 @@ __global__ void k_force_prep(DevConst dc, int nlive, const int32_t *__restrict__
   const double hh = xyzh[(size_t)o * ldx + 3];
   th[p] = hh;
   if (ibin_old) tbin_old[p] = ibin_old[o];
+  if (gsoft) gsoft[p] = gradh[(size_t)o * ldgradh + 1];
   const double hj1 = 1. / hh;  // xyzcache(4) = 1/treecache(4) (kdtree.F90:1202)
   FRec R;
   R.x = tx[p]; R.y = ty[p]; R.z = tz[p];
@@ -141,22 +110,6 @@ struct ForceStage {
   unsigned char leaf[NP];
   int8_t bold[NP];  // ibin_old (block timesteps)
 };
-
-// get_newbin (utils_indtimesteps.f90:230-287)
-__device__ __forceinline__ int get_newbin_d(double dti, double dtmax, int ibin_oldi, bool allow_decrease,
-                                            int nbinmax, int istepfrac) {
-  const double dlog2 = 1.4426950408889634;  // 1/log(2)
-  int ibin_newi;
-  if (dti > dtmax) ibin_newi = 0;
-  else if (dti < PH_TINY64) ibin_newi = 30;  // maxbins (utils_indtimesteps.f90:29)
-  else ibin_newi = max((int)(log(2. * dtmax / dti) * dlog2 - PH_EPS64), 0);
-  if (ibin_newi > ibin_oldi) return ibin_newi;
-  if (ibin_newi < ibin_oldi && ibin_oldi <= nbinmax && allow_decrease) {
-    // up only when the bins are synchronised, one bin at a time
-    if (istepfrac % (1 << (nbinmax - (ibin_oldi - 1))) == 0) return ibin_oldi - 1;
-  }
-  return ibin_oldi;
-}
 
 // Force: one wavefront per group of 64 active particles, one per lane.
 template <int K, bool MHD, bool DUST, bool THERMAL>
@@ -630,12 +583,13 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
   FRecB *recB = mhd ? (FRecB *)c->frecB.p : nullptr;
   FRecD *recD = dust ? (FRecD *)c->frecD.p : nullptr;
   if (a.ibin) PH_HIPCHECK(c, c->tbin_old.ensure(n));
+  if (c->p.gravity) PH_HIPCHECK(c, c->gsoft.ensure(n));
   {
     LaunchTimer lt(c, "force_prep");
     hipLaunchKernelGGL(k_force_prep, dim3(nblocks(n, 256)), dim3(256), 0, s, c->dc, n, c->iparts.p, c->tx.p,
                        c->ty.p, c->tz.p, a.xyzh, a.ldx, a.vxyzu, a.ldv, a.gradh, a.ldgradh, a.alphaind, a.eos_vars,
                        a.ldeos, a.Bevol, a.dustfrac, c->fh.p, rec, recB, recD, a.ibin ? a.ibin_old : nullptr,
-                       c->tbin_old.p);
+                       c->tbin_old.p, c->p.gravity ? c->gsoft.p : nullptr);
   }
   PH_HIPCHECK(c, hipGetLastError());
   PH_HIPCHECK(c, c->tmpi.ensure(n));
@@ -644,6 +598,7 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
   int rc = compact_active(c, c->tmpi.p, n);
   if (rc) return rc;
   const int nact = c->nact;
+  if (c->p.gravity) return grav_force(c, icall, a, rec, c->gsoft.p, dtc_out, dtf_out);
   // node hmax of the final smoothing lengths for the h_j search radius
   PH_HIPCHECK(c, c->hmaxf.ensure(c->ncells + 2));
   hipLaunchKernelGGL(k_zero_u64, dim3(nblocks(c->ncells + 2, 256)), dim3(256), 0, s,

@@ -1,0 +1,54 @@
+// records.h -- packed per-particle records of the force pair loops (tree
+// order), shared by the hydro and the self-gravity force kernels.
+#pragma once
+
+#include "stream.h"
+
+// j quantities of the hydro pair terms (tree order); cs and h are read for i
+struct __align__(16) FRec {
+  double x, y, z, hinv;
+  double vx, vy, vz, u;
+  double rho, rho1, pr, pro2;
+  double vwave;
+  float gradh, alpha;
+  double cs, h;
+};
+static_assert(sizeof(FRec) == 128, "FRec is one 128-byte line");
+// MHD: B = Bevol*rho, psi, the stress -m B(x)B/rho^2, and Bevol itself (i)
+struct __align__(16) FRecB {
+  double Bx, By, Bz, psi;
+  double sxx, sxy, sxz, syy;
+  double syz, szz, Bex, Bey;
+  double Bez, pad0, pad1, pad2;
+};
+static_assert(sizeof(FRecB) == 128, "FRecB is one 128-byte line");
+// one-fluid dust
+struct __align__(16) FRecD {
+  double eps, sqeps, ts, pad;
+};
+
+// get_ts (dust.f90:161-276), idrag 2 (constant K) and 3 (constant t_s)
+__device__ __forceinline__ double get_ts_d(const DevConst &dc, double rhogas, double rhodust) {
+  if (dc.idrag == 2) {
+    if (dc.K_code > 0.) return rhogas * rhodust / (dc.K_code * (rhogas + rhodust));
+    return PH_HUGE64;
+  }
+  if (dc.idrag == 3) return dc.K_code;
+  return 0.;
+}
+
+// get_newbin (utils_indtimesteps.f90:230-287)
+__device__ __forceinline__ int get_newbin_d(double dti, double dtmax, int ibin_oldi, bool allow_decrease,
+                                            int nbinmax, int istepfrac) {
+  const double dlog2 = 1.4426950408889634;  // 1/log(2)
+  int ibin_newi;
+  if (dti > dtmax) ibin_newi = 0;
+  else if (dti < PH_TINY64) ibin_newi = 30;  // maxbins (utils_indtimesteps.f90:29)
+  else ibin_newi = max((int)(log(2. * dtmax / dti) * dlog2 - PH_EPS64), 0);
+  if (ibin_newi > ibin_oldi) return ibin_newi;
+  if (ibin_newi < ibin_oldi && ibin_oldi <= nbinmax && allow_decrease) {
+    // up only when the bins are synchronised, one bin at a time
+    if (istepfrac % (1 << (nbinmax - (ibin_oldi - 1))) == 0) return ibin_oldi - 1;
+  }
+  return ibin_oldi;
+}

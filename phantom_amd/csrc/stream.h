@@ -29,6 +29,7 @@
 
 constexpr int GROUP = 64;                         // particles per group = wavefront size
 constexpr int WALK_STACK = 2048;                  // node stack of the group walk (LDS)
+constexpr int HTRAJ = 16;                         // h after each density iteration kept per particle (self-gravity)
 
 // separation code of a candidate leaf for a lane: 0..26 periodic offset
 // (sx+1)+3(sy+1)+9(sz+1) with the minimum image; SEP_PLAIN = no offset and

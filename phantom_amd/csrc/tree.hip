@@ -704,5 +704,6 @@ int tree_build(phsph_ctx *c, double *d_xyzh, int ld, int np, const int8_t *d_iph
   c->trial_stale = true;
   c->nrhocalc = -1;
   c->nactualtot = -1;
+  if (c->p.gravity) return grav_tree(c);
   return PHSPH_OK;
 }

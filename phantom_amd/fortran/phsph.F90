@@ -42,7 +42,7 @@ module phsph
  public :: phsph_default, phsph_init, phsph_finalise, phsph_errmsg
  public :: phsph_maketree, phsph_densityiterate, phsph_cons2prim, phsph_force
  public :: phsph_neighbour_stats
- public :: phsph_set_bins, phsph_get_nbinmax, phsph_set_active_particles
+ public :: phsph_set_bins, phsph_get_nbinmax, phsph_set_active_particles, phsph_set_poten
 
  interface
     subroutine c_default_params(p, kernel) bind(c, name='phsph_default_params')
@@ -105,6 +105,10 @@ module phsph
        import :: c_int, c_int32_t, c_ptr
        type(c_ptr), value :: ctx, ibin, ibin_old, ibin_wake
        integer(c_int32_t), value :: nbinmax, istepfrac, ibinnow
+    end function
+    integer(c_int) function c_set_poten(ctx, poten) bind(c, name='phsph_set_poten')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, poten
     end function
     integer(c_int) function c_get_nbinmax(ctx, nbinmax) bind(c, name='phsph_get_nbinmax')
        import :: c_int, c_int32_t, c_ptr
@@ -251,6 +255,12 @@ integer function phsph_set_bins(ibin, ibin_old, ibin_wake, nbinmax, istepfrac, i
  ierr = c_set_bins(ctx, c_loc(ibin), c_loc(ibin_old), c_loc(ibin_wake), int(nbinmax, c_int32_t), &
                    int(istepfrac, c_int32_t), int(ibinnow, c_int32_t))
 end function phsph_set_bins
+
+!-- part:poten, written by force with self-gravity (force.F90:2927)
+integer function phsph_set_poten(poten) result(ierr)
+ real(c_float), intent(inout), target :: poten(:)
+ ierr = c_set_poten(ctx, c_loc(poten))
+end function phsph_set_poten
 
 !-- nbinmax after the last force call (force.F90:792)
 integer function phsph_get_nbinmax(nbinmax) result(ierr)

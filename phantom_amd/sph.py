@@ -30,7 +30,8 @@ EXPORTED = [
     "phsph_default_params", "phsph_create", "phsph_destroy", "phsph_set_params", "phsph_last_error",
     "phsph_maketree", "phsph_get_tree", "phsph_densityiterate", "phsph_cons2prim", "phsph_force",
     "phsph_get_neighbour_stats", "phsph_get_particle_stats", "phsph_get_force_stats", "phsph_get_timings",
-    "phsph_set_bins", "phsph_get_nbinmax", "phsph_set_active_particles",
+    "phsph_set_bins", "phsph_get_nbinmax", "phsph_set_active_particles", "phsph_set_poten",
+    "phsph_get_quads",
 ]
 
 _lib = None
@@ -72,6 +73,8 @@ def load_library(path=LIB_PATH):
     L.phsph_get_timings.argtypes = [vp, P(C.c_char_p), P(dbl), i32]
     L.phsph_set_bins.argtypes = [vp, vp, vp, vp, i32, i32, i32]
     L.phsph_get_nbinmax.argtypes = [vp, P(i32)]
+    L.phsph_set_poten.argtypes = [vp, vp]
+    L.phsph_get_quads.argtypes = [vp, vp, i64]
     L.phsph_set_active_particles.argtypes = [vp, i32, vp, i32, vp, vp, i32, i32, P(i32), P(i32), P(i32)]
     for name in EXPORTED:
         if name != "phsph_default_params" and name != "phsph_last_error":
@@ -121,6 +124,7 @@ class Context:
                 self._h = None
             raise PhsphError(rc, msg)
         self.np = 0
+        self._poten = None
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -173,6 +177,13 @@ class Context:
                                             inoderange.ctypes.data_as(C.c_void_p),
                                             inodeparts.ctypes.data_as(C.c_void_p), nc, C.byref(ml)))
         return dict(node=node, inoderange=inoderange, inodeparts=inodeparts, maxlevel=ml.value, ncells=nc)
+
+    def get_quads(self):
+        """node(1:ncells)%quads: second moments about each node's centre of
+        mass (self-gravity only)."""
+        q = np.zeros((self._ncells, 6))
+        self._check(self.lib.phsph_get_quads(self._h, q.ctypes.data_as(C.c_void_p), self._ncells))
+        return q
 
     # -- densityiterate ------------------------------------------------------
     def densityiterate(self, icall, xyzh, vxyzu, fxyzu=None, fext=None, Bevol=None, iphase=None,
@@ -266,6 +277,8 @@ class Context:
                 deltav = _zeros_like_layout(xyzh, (n, 3), np.float64)
             if tstop is None:
                 tstop = _zeros_like_layout(xyzh, (n,), np.float64)
+        if p.gravity and (self._poten is None or self._poten.shape[0] != n):
+            self.set_poten(_zeros_like_layout(xyzh, (n,), np.float32))
         dtc, dtf = C.c_double(), C.c_double()
         px, ldx = _ptr(xyzh)
         pv, ldv = _ptr(vxyzu)
@@ -287,7 +300,15 @@ class Context:
                                          pdBs, pdf, pdd, pdl, pts, pg, ldg, pa, pe, lde, pph, C.byref(dtc),
                                          C.byref(dtf)))
         return dict(fxyzu=fxyzu, divcurlv=divcurlv, dBevol=dBevol, divBsymm=divBsymm, ddustevol=ddustevol,
-                    deltav=deltav, tstop=tstop, dtcourant=dtc.value, dtforce=dtf.value)
+                    deltav=deltav, tstop=tstop, dtcourant=dtc.value, dtforce=dtf.value,
+                    poten=self._poten if p.gravity else None)
+
+    def set_poten(self, poten):
+        """Register part:poten (float32, np): the potential energy that force
+        writes per active particle while params.gravity is set."""
+        self._poten = poten
+        pp, _ = _ptr(poten)
+        self._check(self.lib.phsph_set_poten(self._h, pp))
 
     # -- block timesteps ------------------------------------------------------
     def set_bins(self, ibin, ibin_old, ibin_wake, nbinmax, istepfrac, ibinnow):

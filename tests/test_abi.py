@@ -50,11 +50,38 @@ def test_default_params_agree(kernel):
     assert abi.params_to_dict(c) == abi.params_to_dict(py)
 
 
-@pytest.mark.parametrize("field,value", [("kernel", 7), ("ieos", 3), ("massoftype_gas", 0.0), ("gravity", 1),
-                                         ("maxvxyzu", 5), ("tolh", -1.0)])
+@pytest.mark.parametrize("field,value", [("kernel", 7), ("ieos", 3), ("massoftype_gas", 0.0), ("maxvxyzu", 5),
+                                         ("tolh", -1.0)])
 def test_invalid_params_rejected(field, value):
     p = abi.default_params()
     setattr(p, field, value)
+    with pytest.raises(sph.PhsphError) as e:
+        sph.Context(p)
+    assert e.value.code == abi.ERR_INPUT
+
+
+@pytest.mark.parametrize("acc", [-0.5, 1.5])
+def test_tree_accuracy_range(acc):
+    """tree_accuracy is read with gravity on, in 0..1 (neigh_kdtree.f90:345)."""
+    p = abi.default_params()
+    p.gravity = 1
+    p.tree_accuracy = acc
+    with pytest.raises(sph.PhsphError) as e:
+        sph.Context(p)
+    assert e.value.code == abi.ERR_INPUT
+
+
+@pytest.mark.parametrize("extra", ["mhd", "dust"])
+def test_gravity_only_with_hydro(extra):
+    """Self-gravity is built for the hydro force; with MHD or one-fluid dust
+    the context refuses it rather than computing without gravity."""
+    p = abi.default_params()
+    p.gravity = 1
+    if extra == "mhd":
+        p.mhd = 1
+    else:
+        p.use_dustfrac = 1
+        p.ndustsmall = 1
     with pytest.raises(sph.PhsphError) as e:
         sph.Context(p)
     assert e.value.code == abi.ERR_INPUT

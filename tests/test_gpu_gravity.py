@@ -1,0 +1,161 @@
+"""Parity of the self-gravity path (kd-tree multipoles, dual-tree walk,
+softened pair gravity) with the oracle on identical particles.
+
+Cases: a pressureless polytrope with the tree opened everywhere
+(tree_accuracy 0, so every pair is summed directly and momentum is conserved
+to rounding), the same at tree_accuracy 0.5 (quadrupole far field), an
+adiabatic polytrope with random velocities (viscosity and conductivity
+alongside gravity, quintic kernel) and the adiabatic case on block
+timesteps.  Tolerances as the hydro parity tests: neighbour counts and
+iteration counts bit-exact, FP64 results 1e-12 of the rms, float32 outputs
+to float rounding."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from phantom_amd import abi, setups, sph
+from tests import parity
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-12
+
+
+def _moving(s, seed=7, sigma=0.1):
+    rng = np.random.default_rng(seed)
+    s["vxyzu"][:, :3] = rng.normal(0.0, sigma, size=(s["vxyzu"].shape[0], 3))
+    return s
+
+
+def _cases():
+    return {
+        "polytrope_p0_open": lambda: setups.polytrope(n=1500, pressureless=True, theta=0.0),
+        "polytrope_p0_theta05": lambda: setups.polytrope(n=4000, pressureless=True, theta=0.5),
+        "polytrope_adiabatic_quintic": lambda: _moving(
+            setups.polytrope(n=3000, theta=0.5, kernel=abi.KERNEL_QUINTIC)),
+        "polytrope_adiabatic_bins": lambda: setups.with_bins(_moving(setups.polytrope(n=3000, theta=0.5)),
+                                                             nbinmax=3, istepfrac=2, dtmax=0.05),
+    }
+
+
+def _run_oracle(setup):
+    p = setup["params"]
+    o = oracle.Oracle(p)
+    xyzh = setup["xyzh"].copy()
+    vxyzu = setup["vxyzu"]
+    alphaind = setup["alphaind"].copy()
+    iphase = setup.get("iphase")
+    o.maketree(xyzh, iphase=iphase)
+    tree0 = o.get_tree()
+    quads = o.get_quads()
+    d = o.densityiterate(xyzh, vxyzu, icall=1, alphaind=alphaind, iphase=iphase)
+    tree1 = o.get_tree()
+    c = o.cons2prim(xyzh, vxyzu, d["dvdx"], alphaind)
+    bins = setup.get("bins")
+    if bins is not None:
+        ibin = bins["ibin"].copy()
+        ibin_wake = bins["ibin_wake"].copy()
+        o.set_bins(ibin, bins["ibin_old"], ibin_wake, bins["nbinmax"], bins["istepfrac"], bins["ibinnow"])
+    f = o.force(xyzh, vxyzu, d["gradh"], alphaind, c["eos_vars"], iphase=iphase)
+    if bins is not None:
+        f.update(ibin=ibin, ibin_wake=ibin_wake, nbinmax=o.nbinmax())
+    return dict(xyzh=xyzh, tree0=tree0, tree1=tree1, quads=quads, dens=d, force=f)
+
+
+def _run_gpu(setup):
+    p = setup["params"]
+    ctx = sph.Context(p)
+    xyzh = setup["xyzh"].copy()
+    vxyzu = setup["vxyzu"]
+    alphaind = setup["alphaind"].copy()
+    iphase = setup.get("iphase")
+    ctx.maketree(xyzh, iphase=iphase)
+    tree0 = ctx.get_tree()
+    quads = ctx.get_quads()
+    d = ctx.densityiterate(1, xyzh, vxyzu, alphaind=alphaind, iphase=iphase)
+    tree1 = ctx.get_tree()
+    nneigh, nits = ctx.get_particle_stats(xyzh.shape[0])
+    c = ctx.cons2prim(xyzh, vxyzu, d["dvdx"], alphaind)
+    bins = setup.get("bins")
+    if bins is not None:
+        ibin = bins["ibin"].copy()
+        ibin_wake = bins["ibin_wake"].copy()
+        ctx.set_bins(ibin, bins["ibin_old"], ibin_wake, bins["nbinmax"], bins["istepfrac"], bins["ibinnow"])
+    f = ctx.force(1, xyzh, vxyzu, d["gradh"], alphaind, c["eos_vars"], iphase=iphase)
+    if bins is not None:
+        f.update(ibin=ibin, ibin_wake=ibin_wake, nbinmax=ctx.get_nbinmax())
+    out = dict(xyzh=xyzh, tree0=tree0, tree1=tree1, quads=quads, dens=d, nneigh=nneigh, nits=nits, force=f)
+    ctx.close()
+    return out
+
+
+@pytest.fixture(scope="module", params=list(_cases().keys()))
+def pair(request, gpu):
+    setup = _cases()[request.param]()
+    print(f"\n[case {request.param}] n={setup['xyzh'].shape[0]}", flush=True)
+    g = _run_gpu(setup)
+    o = _run_oracle(setup)
+    return request.param, setup, g, o
+
+
+def test_quads(pair):
+    name, setup, g, o = pair
+    used = parity.reachable_nodes(o["tree0"])
+    scale = np.abs(o["quads"][used]).max()
+    assert np.abs(g["quads"][used] - o["quads"][used]).max() <= 1e-13 * scale
+
+
+def test_density_and_node_hmax(pair):
+    """h, neighbour and iteration counts as without gravity; the node hmax
+    after densityiterate follows the relists of every cell (set_hmaxcell)."""
+    name, setup, g, o = pair
+    np.testing.assert_array_equal(g["nneigh"], o["dens"]["nneigh"])
+    np.testing.assert_array_equal(g["nits"], o["dens"]["nits"])
+    assert parity.rel_rms(g["xyzh"][:, 3], o["xyzh"][:, 3]) <= TOL_FP64
+    # gradh(2) (the softening correction) is a float32 output
+    gs, os_ = np.asarray(g["dens"]["gradh"][:, 1], np.float64), np.asarray(o["dens"]["gradh"][:, 1], np.float64)
+    assert parity.rel_rms(gs, os_) <= 2e-7
+    used = parity.reachable_nodes(o["tree1"])
+    hg, ho = g["tree1"]["node"]["hmax"][used], o["tree1"]["node"]["hmax"][used]
+    assert np.max(np.abs(hg - ho) / ho) <= 1e-12
+    # the relists raised at least one node above its maketree value
+    assert np.any(ho > o["tree0"]["node"]["hmax"][used])
+
+
+def test_force_fp64(pair):
+    name, setup, g, o = pair
+    gf, of = g["force"], o["force"]
+    ncomp = 4 if setup["params"].maxvxyzu >= 4 else 3
+    for c in range(ncomp):
+        assert parity.rel_rms(gf["fxyzu"][:, c], of["fxyzu"][:, c]) <= TOL_FP64, (name, c)
+    assert parity.rel_rms(gf["divcurlv"], of["divcurlv"]) <= 2e-7
+    if setup.get("bins") is None:
+        assert abs(gf["dtcourant"] - of["dtcourant"]) <= 1e-12 * of["dtcourant"]
+        assert abs(gf["dtforce"] - of["dtforce"]) <= 1e-12 * of["dtforce"]
+
+
+def test_potential(pair):
+    name, setup, g, o = pair
+    pg = np.asarray(g["force"]["poten"], np.float64)
+    po = np.asarray(o["force"]["poten"], np.float64)
+    assert parity.rel_rms(pg, po) <= 2e-7
+    assert np.all(po[np.asarray(setup.get("iphase", np.ones(len(po))), np.int64) > 0] < 0.0)
+
+
+def test_bins(pair):
+    name, setup, g, o = pair
+    if setup.get("bins") is None:
+        pytest.skip("global timestep case")
+    np.testing.assert_array_equal(g["force"]["ibin"], o["force"]["ibin"])
+    np.testing.assert_array_equal(g["force"]["ibin_wake"], o["force"]["ibin_wake"])
+    assert g["force"]["nbinmax"] == o["force"]["nbinmax"]
+
+
+def test_momentum_open_tree(pair):
+    """With the tree opened everywhere every pair is summed once each way:
+    the total force vanishes to rounding."""
+    name, setup, g, o = pair
+    if setup["params"].tree_accuracy != 0.0:
+        pytest.skip("multipole far field is not pairwise symmetric")
+    f = g["force"]["fxyzu"][:, :3]
+    assert np.abs(f.sum(axis=0)).max() <= 1e-12 * np.abs(f).sum(axis=0).max()
