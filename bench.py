@@ -89,8 +89,12 @@ def main():
     xyzh0.copy_from(xyzh)  # wrapped positions, initial h
     log(f"tree built in {tree_ms:.2f} ms (first call)")
 
-    kt = {"density": [], "force": []}
+    kt = {}  # per-launch kernel times (HIP events on the library's stream), by name
     phase = {"density": 0.0, "cons2prim": 0.0, "force": 0.0}
+
+    def record_timings():
+        for nm, ms in ctx.timings():
+            kt.setdefault(nm, []).append(ms)
 
     def step(record):
         xyzh.copy_from(xyzh0)
@@ -98,7 +102,7 @@ def main():
         ctx.densityiterate(1, xyzh, vxyzu, Bevol=Bevol, divcurlv=divcurlv, divcurlB=divcurlB, alphaind=alphaind,
                            gradh=gradh, dvdx=dvdx)
         if record:
-            kt["density"] += [ms for nm, ms in ctx.timings() if nm == "density"]
+            record_timings()
         b = time.perf_counter()
         ctx.cons2prim(xyzh, vxyzu, dvdx, alphaind, dustevol=dustevol, dustfrac=dustfrac, eos_vars=eos)
         c = time.perf_counter()
@@ -106,7 +110,7 @@ def main():
                   fxyzu=fxyzu, divcurlv=divcurlv, dBevol=dBevol, divBsymm=divBsymm, ddustevol=ddustevol,
                   deltav=deltav, tstop=tstop)
         if record:
-            kt["force"] += [ms for nm, ms in ctx.timings() if nm == "force"]
+            record_timings()
         d = time.perf_counter()
         if record:
             phase["density"] += b - a
@@ -133,8 +137,12 @@ def main():
 
     fl_dens = work_model.density_flops(nneigh, nits, mhd=bool(p.mhd))
     fl_force = work_model.force_flops(npairs, mhd=bool(p.mhd), dust=bool(p.use_dustfrac))
-    t_dens = float(np.mean(kt["density"])) * 1e-3
-    t_force = float(np.mean(kt["force"])) * 1e-3
+    def kmean(name):
+        return float(np.mean(kt[name])) * 1e-3 if name in kt else 0.0
+
+    # per step: the kernels of one densityiterate and one force call
+    t_dens = sum(float(np.sum(kt.get(nm, []))) for nm in ("density_iter", "density_sums")) * 1e-3 / args.steps
+    t_force = sum(float(np.sum(kt.get(nm, []))) for nm in ("force_list", "force")) * 1e-3 / args.steps
     peak = work_model.PEAK_FP64_TFLOPS if args.precision == "fp64" else work_model.PEAK_FP32_TFLOPS
     dens_tf = fl_dens / t_dens / 1e12
     force_tf = fl_force / t_force / 1e12
@@ -169,7 +177,8 @@ def main():
         "breakdown_ms": {"density_call": 1e3 * phase["density"] / args.steps,
                          "cons2prim_call": 1e3 * phase["cons2prim"] / args.steps,
                          "force_call": 1e3 * phase["force"] / args.steps,
-                         "density_kernel": 1e3 * t_dens, "force_kernel": 1e3 * t_force,
+                         "density_kernels": 1e3 * t_dens, "force_kernels": 1e3 * t_force,
+                         "kernels": {nm: round(1e3 * kmean(nm), 4) for nm in sorted(kt)},
                          "tree_build_first_call": tree_ms},
         "kernel_tflops": {"density": round(dens_tf, 3), "force": round(force_tf, 3)},
         "neighbours": {"dens_mean": stats["actualmean"], "dens_max": stats["maxactual"],

@@ -31,6 +31,9 @@
  *    (src/main/io.F90:525-551).
  *  - Calls on one context are not re-entrant (the reference's module state is
  *    not either); each context owns one HIP stream and returns synchronously.
+ *    That stream synchronises with the null stream, so device arrays the
+ *    caller filled by work on the null stream are ready when the library's
+ *    kernels start; work on other streams must be complete before the call.
  */
 #ifndef PHSPH_H
 #define PHSPH_H

@@ -169,7 +169,9 @@ int phsph_create(const phsph_params *p, int32_t device, phsph_ctx **out) {
   c->p = *p;
   c->dc = make_devconst(*p);
   hipError_t e = hipSetDevice(device);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  // a blocking stream: work the caller queued on the null stream (copies
+  // into device arrays it passes) completes before the library's kernels
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamDefault);
   if (e != hipSuccess) {
     c->err = std::string("HIP error ") + hipGetErrorString(e);
     *out = c;
@@ -418,7 +420,7 @@ int phsph_force(phsph_ctx *c, int32_t icall, int32_t np, const double *xyzh, int
     a.ibinnow = c->bins_ibinnow;
   }
   c->poten_out = nullptr;
-  if (c->p.gravity && c->poten_user) STAGE(st, c->poten_user, (size_t)np, false, true, &c->poten_out);
+  if (c->p.gravity && c->poten_user) STAGE(st, c->poten_user, (size_t)np, true, true, &c->poten_out);
   int rc = force_run(c, icall, a, dtcourant, dtforce);
   int rc2 = st.finish();
   return rc ? rc : rc2;

@@ -200,8 +200,3 @@ __device__ __forceinline__ void kern_soft(double q2, double q, double &pot, doub
     }
   }
 }
-
-// positive doubles order as their bit patterns: max by integer atomics
-__device__ __forceinline__ void atomic_max_pos(unsigned long long *a, double v) {
-  atomicMax(a, (unsigned long long)__double_as_longlong(v));
-}

@@ -43,11 +43,13 @@ struct TimedLaunch {
   hipEvent_t t0, t1;
 };
 
-// Capacity of the per-group candidate lists.  It grows (and the walk is
-// re-run) when a launch reports overflow.
+// Capacities of the candidate and neighbour lists.  They grow (and the
+// affected work is re-run) when a launch reports overflow.
 struct Caps {
   int gcand = 512;   // candidate leaves per group of 64 particles
   int p2p = 256;     // near-field leaves per leaf (self-gravity)
+  int dlist = 256;   // density neighbour list per particle
+  int flist = 256;   // force neighbour list per particle
 };
 
 struct phsph_ctx {
@@ -90,6 +92,9 @@ struct phsph_ctx {
   DBuf<int32_t> gncand;      // count per group
   int ngroups = 0;
   DBuf<int32_t> ntrial;      // trial particles per leaf (statistics)
+  DBuf<uint32_t> nlist;      // neighbour lists, column-major [k*nact + t] (node | code << 27)
+  DBuf<int32_t> t_nl;        // list length per active slot
+  DBuf<double> t_hsum, t_hfin;  // density: h of the last summation, h to store
 
   // ---- tree-ordered particle mirror ----
   DBuf<double4> tp4;                     // density: x, y, z, h at entry

@@ -7,16 +7,18 @@
 //  * k_group_walk: one wavefront per group of 64 active particles walks the
 //    tree breadth-parallel (64 nodes per step) and lists the candidate leaves
 //    of the whole group (stream.h).
-//  * k_density: one wavefront per group, one particle per lane.  Each lane
-//    runs its own Newton-Raphson iteration on h; every iteration streams the
-//    group's candidates through LDS, tests them against the lane's current
-//    h and evaluates all the sums of get_density_sums over the hits.  The
-//    sums of the last iteration are stored, as in the reference.
+//  * k_dens_iter: one wavefront per group, one particle per lane.  Each lane
+//    visits the candidate leaves its particle can reach (build_list),
+//    keeps the particles within rfac*radkern*h of its own particle in a
+//    neighbour list, then runs its own Newton-Raphson
+//    iteration on h over that list with the two sums the step needs.
 //    Per-particle iteration gives the reference's result: in a cell the
 //    reference keeps re-evaluating converged particles at their unchanged h,
-//    so each particle's h sequence depends only on its own sums.  The
-//    candidates cover rfac*radkern*max(h) of the group; a particle whose h
-//    grows past that is run again with a wider walk.
+//    so each particle's h sequence depends only on its own sums.  A particle
+//    whose h grows past its list radius (or whose list overflows) is run
+//    again with a wider search.
+//  * k_dens_final: one thread per particle; every sum of get_density_sums
+//    at the h of the last iteration over the list, then store_results.
 //  * k_walk: the reference's per-leaf walk (getneigh, kdtree.F90:1221-1347),
 //    used only for the trial-neighbour statistics.
 //
@@ -25,9 +27,10 @@
 // as the reference, so neighbour counts agree bit for bit.
 #include "stream.h"
 
+#include <algorithm>
 #include <cstdlib>
 
-enum { DFL_CAND = 1, DFL_REWALK = 4, DFL_NOCONV = 8, DFL_NEGH = 16 };
+enum { DFL_CAND = 1, DFL_LIST = 2, DFL_REWALK = 4, DFL_NOCONV = 8, DFL_NEGH = 16 };
 
 struct DensIn {
   const double4 *tp;  // x, y, z (treecache) and h at entry
@@ -269,167 +272,92 @@ __global__ void k_scan_i32(int32_t *a, int n, int32_t *total) {
 }
 
 // ------------------------------------------------------------------------
-// density: one wavefront per group, one active particle per lane
+// density, part 1: neighbour list and Newton-Raphson iteration on h.
+// One wavefront per group of 64 active particles, one particle per lane.
+// Each lane visits the candidate leaves of its group that its particle can
+// reach and keeps the particles within its own radius rfac*radkern*h0 (a superset of
+// every kernel support the iteration can reach) in a list, column-major over
+// the active particles.  The iteration itself evaluates only the sums the
+// Newton step needs (rho and dOmega); the full set of get_density_sums is
+// taken once, at the h of the last iteration, by part 2.
 // ------------------------------------------------------------------------
-template <int K, bool MHD>
+template <int K>
 __global__ void __launch_bounds__(64)
-k_density(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, int nact, int ngroups,
-          const int32_t *__restrict__ pleaf, const int32_t *__restrict__ leaves, const uint32_t *__restrict__ gcand,
-          const int32_t *__restrict__ gncand, int capg, const int32_t *__restrict__ nd_lo,
-          const int32_t *__restrict__ nd_hi, const phsph_kdnode *__restrict__ nd,
-          const int32_t *__restrict__ iparts, DensIn in, double *__restrict__ th_out, DensOut out,
-          int32_t *__restrict__ p_nneigh, int32_t *__restrict__ p_nits, int32_t *__restrict__ leaf_nits,
-          int32_t *__restrict__ retry, double *__restrict__ htraj, double *__restrict__ hconv,
-          int32_t *__restrict__ flags) {
-  constexpr int NL = 16, NP = NL * PH_MINPART, NPB = MHD ? NP : 1;
-  __shared__ BatchMeta<NL> M;
-  __shared__ double sx[NP], sy[NP], sz[NP];
-  __shared__ double svx[NP], svy[NP], svz[NP];
-  __shared__ double sfx[NP], sfy[NP], sfz[NP];
-  __shared__ double sBx[NPB], sBy[NPB], sBz[NPB];
-  __shared__ int sj[NP];
-  __shared__ unsigned char sl[NP];
+k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, int nact, int ngroups,
+            const int32_t *__restrict__ pleaf, const int32_t *__restrict__ leaves,
+            const uint32_t *__restrict__ gcand, const int32_t *__restrict__ gncand, int capg,
+            const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi,
+            const phsph_kdnode *__restrict__ nd, const double4 *__restrict__ tp, uint32_t *__restrict__ nlist,
+            int capl, int32_t *__restrict__ t_nl, double *__restrict__ t_hsum, double *__restrict__ t_hfin,
+            int32_t *__restrict__ p_nits, int32_t *__restrict__ leaf_nits, int32_t *__restrict__ retry,
+            double *__restrict__ htraj, double *__restrict__ hconv, int32_t *__restrict__ flags) {
+  __shared__ ListStage S;
   const int g = xcd_block(blockIdx.x, gridDim.x);
   if (g >= ngroups) return;
   const int lane = threadIdx.x;
   const int t = g * GROUP + lane;
   const bool live = t < nact;
-  const bool have_f = in.tf != nullptr;
-  int p = 0, l = 0;
+  int p = -1, l = 0;
   double xi = 0., yi = 0., zi = 0., h0 = 1., lcx = 0., lcy = 0., lcz = 0., lsz = 0.;
-  double4 vi = make_double4(0., 0., 0., 0.), fi = make_double4(0., 0., 0., 0.), Bev = make_double4(0., 0., 0., 0.);
   if (live) {
     p = act[t];
     l = pleaf[p];
     const int c = leaves[l];
-    const double4 r = in.tp[p];
+    const double4 r = tp[p];
     xi = r.x; yi = r.y; zi = r.z; h0 = r.w;
     lcx = nd[c].xcen[0]; lcy = nd[c].xcen[1]; lcz = nd[c].xcen[2]; lsz = nd[c].size;
-    vi = in.tv[p];
-    if (have_f) fi = in.tf[p];
-    if (MHD) Bev = in.tB[p];
   }
+  const double rcut = rfac * dc.radkern * h0;
+  const double rc2 = rcut * rcut;
+  // --- neighbour list over the leaves this particle can reach ---
+  int nl = 0;
+  build_list(
+      S, dc, gcand + (size_t)g * capg, gncand[g], nd, nd_lo, nd_hi, (const double *)nullptr, tp, lcx, lcy, lcz, lsz,
+      [&](double ccx, double ccy, double ccz, double csz, double) {
+        return live && leaf_in_reach(dc, xi, yi, zi, ccx, ccy, ccz, csz, rcut);
+      },
+      [&](int j, int code, double xj, double yj, double zj, double) {
+        double dx, dy, dz;
+        sep_of(dc, code, xi, yi, zi, xj, yj, zj, dx, dy, dz);
+        const double r2 = r2_exact(dx, dy, dz);
+        if (j != p && r2 < rc2) {
+          if (nl < capl) nlist[(size_t)nl * nact + t] = cand_pack(j, code);
+          nl++;
+        }
+      });
+  if (!live) return;
+  if (nl > capl) {
+    atomicOr(flags, DFL_LIST);
+    retry[p] = 1;
+    return;
+  }
+  // --- Newton-Raphson on h (finish_cell, dens.F90:1382-1466) ---
   const double pm = dc.pmass;
-  // The group's candidate stream covers rcg = rfac*radkern*max(h0), the same
-  // expression as in k_group_walk.  A particle whose h grows past it is
-  // handed back to the host and run again with a wider walk.
-  const double rcg = rfac * dc.radkern * wave_max(live ? h0 : 0.);
-  enum { ST_ITER = 1, ST_DONE = 2, ST_FAIL = 3 };
-  int state = live ? ST_ITER : ST_DONE;
-  const int nc = gncand[g];
-  const uint32_t *gc = gcand + (size_t)g * capg;
+  const double R2 = dc.radkern2;
   double hi = h0, hfin = h0;
-  int it = 0, nneigh = 1;
-  double rs_rho = 0., rs_gh = 0., rs_gs = 0., rs_divv = 0., rs_divB = 0.;
-  double dv[9], da[9], rr[6], dB[9];
-  // Every iteration evaluates all the sums of get_density_sums
-  // (dens.F90:578-857) at the current h, as the reference does; the sums of
-  // the last iteration are the ones stored.
-  while (__ballot(state == ST_ITER) != 0ull) {
-    const bool iter = (state == ST_ITER);
-    if (iter) {
-      it++;
-      nneigh = 1;
-      rs_rho = rs_gh = rs_gs = rs_divv = rs_divB = 0.;
-      for (int k = 0; k < 9; k++) { dv[k] = 0.; da[k] = 0.; dB[k] = 0.; }
-      for (int k = 0; k < 6; k++) rr[k] = 0.;
-    }
+  int it = 0;
+  for (;;) {
+    it++;
     const double hi1 = 1. / hi, hi21 = xmul(hi1, hi1);
-    double Bxi = 0., Byi = 0., Bzi = 0.;
-    if (MHD) {
-      const double rhoiB = rhoh_d(hi, dc.hfact, pm);
-      Bxi = Bev.x * rhoiB; Byi = Bev.y * rhoiB; Bzi = Bev.z * rhoiB;
+    double srho = 0., sgh = 0.;
+    for (int k = 0; k < nl; k++) {
+      const uint32_t e = nlist[(size_t)k * nact + t];
+      const double4 r = tp[cand_node(e)];
+      double dx, dy, dz;
+      sep_of(dc, cand_code(e), xi, yi, zi, r.x, r.y, r.z, dx, dy, dz);
+      const double r2 = r2_exact(dx, dy, dz);
+      const double q2 = xmul(r2, hi21);
+      if (q2 < R2) {
+        const double q = sqrt(r2) * hi1;
+        double wk, gk;
+        kern_wg<K, double>(q2, q, wk, gk);
+        srho += wk * pm;
+        sgh += (-q * gk - 3. * wk) * pm;
+      }
     }
-    for (int b0 = 0; b0 < nc; b0 += NL) {
-      int nb;
-      const int total = batch_meta<NL>(M, gc, b0, nc, nd, nd_lo, nd_hi, lane, nb);
-      for (int s = lane; s < total; s += GROUP) {
-        const int L = batch_leaf_of<NL>(M, nb, s);
-        const int j = M.lo[L] + (s - M.start[L]);
-        const double4 r = in.tp[j];
-        sx[s] = r.x; sy[s] = r.y; sz[s] = r.z;
-        const double4 v = in.tv[j];
-        svx[s] = v.x; svy[s] = v.y; svz[s] = v.z;
-        if (have_f) {
-          const double4 f = in.tf[j];
-          sfx[s] = f.x; sfy[s] = f.y; sfz[s] = f.z;
-        }
-        if (MHD) {  // B_j = Bevol_j * rho_j with rho_j from the h it had on entry
-          const double4 B = in.tB[j];
-          sBx[s] = B.x * B.w; sBy[s] = B.y * B.w; sBz[s] = B.z * B.w;
-        }
-        sj[s] = j;
-        sl[s] = (unsigned char)L;
-      }
-      __syncthreads();
-      // --- each lane tests every staged particle: hits as a bit mask ---
-      unsigned long long m0 = 0ull, m1 = 0ull, m2 = 0ull;
-      SepCodes codes;
-      for (int L = 0; L < nb; L++) {
-        const int code = leaf_sep_code(dc, M.cx[L], M.cy[L], M.cz[L], M.sz[L], lcx, lcy, lcz, lsz);
-        codes.set(L, code);
-        const int s1 = M.start[L + 1];
-        for (int s = M.start[L]; s < s1; s++) {
-          double dx, dy, dz;
-          sep_of(dc, code, xi, yi, zi, sx[s], sy[s], sz[s], dx, dy, dz);
-          const double r2 = r2_exact(dx, dy, dz);
-          const bool hit = iter && sj[s] != p && xmul(r2, hi21) < dc.radkern2;
-          const unsigned long long bit = (unsigned long long)hit << (s & 63);
-          if (s < 64) m0 |= bit;
-          else if (s < 128) m1 |= bit;
-          else m2 |= bit;
-        }
-      }
-      // --- pair sums over the hits ---
-      for (int w = 0; w < 3; w++) {
-        unsigned long long m = (w == 0) ? m0 : (w == 1) ? m1 : m2;
-        while (m != 0ull) {
-          const int s = w * 64 + (int)__builtin_ctzll(m);
-          m &= m - 1ull;
-          double dx, dy, dz;
-          sep_of(dc, codes.get(sl[s]), xi, yi, zi, sx[s], sy[s], sz[s], dx, dy, dz);
-          const double r2 = r2_exact(dx, dy, dz);
-          const double q2 = xmul(r2, hi21);
-          const double rij = sqrt(r2);
-          const double q = rij * hi1;
-          double wk, gk;
-          kern_wg<K, double>(q2, q, wk, gk);
-          rs_rho += wk * pm;
-          rs_gh += (-q * gk - 3. * wk) * pm;
-          if (dc.gravity) rs_gs += kern_dphidh<K, double>(q2, q) * pm;
-          nneigh++;
-          const double rij1 = 1. / (rij + PH_EPS64);
-          const double rg = rij1 * gk;
-          const double runix = dx * rg * pm, runiy = dy * rg * pm, runiz = dz * rg * pm;
-          const double dvx = vi.x - svx[s], dvy = vi.y - svy[s], dvz = vi.z - svz[s];
-          rs_divv += dvx * runix + dvy * runiy + dvz * runiz;
-          dv[0] += dvx * runix; dv[1] += dvx * runiy; dv[2] += dvx * runiz;
-          dv[3] += dvy * runix; dv[4] += dvy * runiy; dv[5] += dvy * runiz;
-          dv[6] += dvz * runix; dv[7] += dvz * runiy; dv[8] += dvz * runiz;
-          if (have_f) {
-            const double dax = fi.x - sfx[s], day = fi.y - sfy[s], daz = fi.z - sfz[s];
-            da[0] += dax * runix; da[1] += dax * runiy; da[2] += dax * runiz;
-            da[3] += day * runix; da[4] += day * runiy; da[5] += day * runiz;
-            da[6] += daz * runix; da[7] += daz * runiy; da[8] += daz * runiz;
-          }
-          rr[0] -= dx * runix; rr[1] -= dx * runiy; rr[2] -= dx * runiz;
-          rr[3] -= dy * runiy; rr[4] -= dy * runiz; rr[5] -= dz * runiz;
-          if (MHD) {
-            const double dBx = Bxi - sBx[s], dBy = Byi - sBy[s], dBz = Bzi - sBz[s];
-            rs_divB += dBx * runix + dBy * runiy + dBz * runiz;
-            dB[0] += dBx * runix; dB[1] += dBx * runiy; dB[2] += dBx * runiz;
-            dB[3] += dBy * runix; dB[4] += dBy * runiy; dB[5] += dBy * runiz;
-            dB[6] += dBz * runix; dB[7] += dBz * runiy; dB[8] += dBz * runiz;
-          }
-        }
-      }
-      __syncthreads();
-    }
-    if (!iter) continue;
-    // --- finish_rhosum + finish_cell (dens.F90:1382-1507) ---
     const double hi31 = hi1 * hi21, hi41 = hi21 * hi21;
-    const double rhoi = dc.cnormk * (rs_rho + dc.wab0 * pm) * hi31;
-    const double gradhi = dc.cnormk * (rs_gh + dc.gradh0 * pm) * hi41;
+    const double rhoi = dc.cnormk * (srho + dc.wab0 * pm) * hi31;
+    const double gradhi = dc.cnormk * (sgh + dc.gradh0 * pm) * hi41;
     const double rhohi = rhoh_d(hi, dc.hfact, pm);
     const double dhdrhoi = -hi / (3. * rhohi);
     const double omegai = 1. - dhdrhoi * gradhi;
@@ -441,28 +369,113 @@ k_density(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, 
     const bool conv = (fabs(hnew - hi) / h0 < dc.tolh) && omegai > 0. && hi > 0.;
     if (icall == 0) {  // one pass: store at hnew with the sums taken at h0 (dens.F90:341)
       hfin = conv ? hi : hnew;
-      state = ST_DONE;
-    } else if (conv) {
+      break;
+    }
+    if (conv) {
       hfin = hi;
-      state = ST_DONE;
-    } else if (it >= PH_MAXDENSITS) {
+      break;
+    }
+    if (it >= PH_MAXDENSITS) {
       atomicOr(flags, DFL_NOCONV);
-      state = ST_FAIL;
-    } else {
-      hi = hnew;
-      if (htraj && it <= HTRAJ) htraj[(size_t)p * HTRAJ + it - 1] = hi;
-      if (dc.radkern * hi * (1. + 1e-12) > rcg) {  // outgrew the group's candidate stream
-        atomicOr(flags, DFL_REWALK);
-        state = ST_FAIL;
-      }
+      retry[p] = 1;
+      return;
+    }
+    hi = hnew;
+    if (htraj && it <= HTRAJ) htraj[(size_t)p * HTRAJ + it - 1] = hi;
+    if (dc.radkern * hi * (1. + 1e-12) > rcut) {  // outgrew the list: run again with a wider one
+      atomicOr(flags, DFL_REWALK);
+      retry[p] = 1;
+      return;
     }
   }
-  if (!live) return;
-  if (state != ST_DONE) {
-    if (state == ST_FAIL) retry[p] = 1;
-    return;
-  }
   if (hconv) hconv[p] = hi;  // the h of the last summation (cell%h once converged)
+  t_nl[t] = nl;
+  t_hsum[t] = hi;
+  t_hfin[t] = hfin;
+  p_nits[p] = (icall == 0) ? 1 : it;
+  atomicMax(&leaf_nits[l], (icall == 0) ? 1 : it);
+}
+
+// ------------------------------------------------------------------------
+// density, part 2: every sum of get_density_sums (dens.F90:578-857) at the h
+// of the last iteration, over the particle's list, and store_results
+// (dens.F90:1511-1681) at the final h.  One thread per active particle.
+// ------------------------------------------------------------------------
+template <int K, bool MHD>
+__global__ void __launch_bounds__(256)
+k_dens_final(DevConst dc, const int32_t *__restrict__ act, int nact, const uint32_t *__restrict__ nlist,
+             const int32_t *__restrict__ t_nl, const double *__restrict__ t_hsum, const double *__restrict__ t_hfin,
+             const int32_t *__restrict__ retry, const int32_t *__restrict__ iparts, DensIn in,
+             double *__restrict__ th_out, DensOut out, int32_t *__restrict__ p_nneigh, int32_t *__restrict__ flags) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nact) return;
+  const int p = act[t];
+  if (retry[p]) return;
+  const double pm = dc.pmass;
+  const double4 ri = in.tp[p];
+  const double xi = ri.x, yi = ri.y, zi = ri.z;
+  const double hi = t_hsum[t], hfin = t_hfin[t];
+  const int nl = t_nl[t];
+  const bool have_f = in.tf != nullptr;
+  const double4 vi = in.tv[p];
+  double4 fi = make_double4(0., 0., 0., 0.);
+  if (have_f) fi = in.tf[p];
+  const double hi1 = 1. / hi, hi21 = xmul(hi1, hi1);
+  double Bxi = 0., Byi = 0., Bzi = 0.;
+  if (MHD) {
+    const double4 Bev = in.tB[p];
+    const double rhoiB = rhoh_d(hi, dc.hfact, pm);
+    Bxi = Bev.x * rhoiB; Byi = Bev.y * rhoiB; Bzi = Bev.z * rhoiB;
+  }
+  double rs_rho = 0., rs_gh = 0., rs_gs = 0., rs_divv = 0., rs_divB = 0.;
+  double dv[9], da[9], rr[6], dB[9];
+  for (int k = 0; k < 9; k++) { dv[k] = 0.; da[k] = 0.; dB[k] = 0.; }
+  for (int k = 0; k < 6; k++) rr[k] = 0.;
+  int nneigh = 1;
+  for (int k = 0; k < nl; k++) {
+    const uint32_t e = nlist[(size_t)k * nact + t];
+    const int j = cand_node(e);
+    const double4 r = in.tp[j];
+    double dx, dy, dz;
+    sep_of(dc, cand_code(e), xi, yi, zi, r.x, r.y, r.z, dx, dy, dz);
+    const double r2 = r2_exact(dx, dy, dz);
+    const double q2 = xmul(r2, hi21);
+    if (!(q2 < dc.radkern2)) continue;
+    const double rij = sqrt(r2);
+    const double q = rij * hi1;
+    double wk, gk;
+    kern_wg<K, double>(q2, q, wk, gk);
+    rs_rho += wk * pm;
+    rs_gh += (-q * gk - 3. * wk) * pm;
+    if (dc.gravity) rs_gs += kern_dphidh<K, double>(q2, q) * pm;
+    nneigh++;
+    const double rij1 = 1. / (rij + PH_EPS64);
+    const double rg = rij1 * gk;
+    const double runix = dx * rg * pm, runiy = dy * rg * pm, runiz = dz * rg * pm;
+    const double4 vj = in.tv[j];
+    const double dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
+    rs_divv += dvx * runix + dvy * runiy + dvz * runiz;
+    dv[0] += dvx * runix; dv[1] += dvx * runiy; dv[2] += dvx * runiz;
+    dv[3] += dvy * runix; dv[4] += dvy * runiy; dv[5] += dvy * runiz;
+    dv[6] += dvz * runix; dv[7] += dvz * runiy; dv[8] += dvz * runiz;
+    if (have_f) {
+      const double4 fj = in.tf[j];
+      const double dax = fi.x - fj.x, day = fi.y - fj.y, daz = fi.z - fj.z;
+      da[0] += dax * runix; da[1] += dax * runiy; da[2] += dax * runiz;
+      da[3] += day * runix; da[4] += day * runiy; da[5] += day * runiz;
+      da[6] += daz * runix; da[7] += daz * runiy; da[8] += daz * runiz;
+    }
+    rr[0] -= dx * runix; rr[1] -= dx * runiy; rr[2] -= dx * runiz;
+    rr[3] -= dy * runiy; rr[4] -= dy * runiz; rr[5] -= dz * runiz;
+    if (MHD) {  // B_j = Bevol_j * rho_j with rho_j from the h it had on entry
+      const double4 B = in.tB[j];
+      const double dBx = Bxi - B.x * B.w, dBy = Byi - B.y * B.w, dBz = Bzi - B.z * B.w;
+      rs_divB += dBx * runix + dBy * runiy + dBz * runiz;
+      dB[0] += dBx * runix; dB[1] += dBx * runiy; dB[2] += dBx * runiz;
+      dB[3] += dBy * runix; dB[4] += dBy * runiy; dB[5] += dBy * runiz;
+      dB[6] += dBz * runix; dB[7] += dBz * runiy; dB[8] += dBz * runiz;
+    }
+  }
   // --- store_results at h = hfin ---
   const double hf1 = 1. / hfin, hf21 = hf1 * hf1, hf31 = hf1 * hf21, hf41 = hf21 * hf21;
   const double rhoi = dc.cnormk * (rs_rho + dc.wab0 * pm) * hf31;
@@ -516,26 +529,26 @@ k_density(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, 
     out.divcurlB[(size_t)o * 4 + 3] = (float)(-(dB[3] - dB[1]) * term);
   }
   p_nneigh[p] = nneigh;
-  p_nits[p] = (icall == 0) ? 1 : it;
-  atomicMax(&leaf_nits[l], (icall == 0) ? 1 : it);
 }
 
 // actual-neighbour statistics and density evaluations (dens.F90:1670-1679)
-__global__ void k_dens_stats(int nact, const int32_t *act, const int32_t *pleaf, const int32_t *p_nneigh,
-                             const int32_t *leaf_nits, const int32_t *ntrial, long long *acc) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256)
+k_dens_stats(int nact, const int32_t *act, const int32_t *pleaf, const int32_t *p_nneigh,
+             const int32_t *leaf_nits, const int32_t *ntrial, long long *acc) {
   long long nn = 0, nit = 0, ntr = 0, mx = 0, mxt = 0;
-  if (t < nact) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nact; t += gridDim.x * blockDim.x) {
     const int p = act[t];
-    int l = pleaf[p];
+    const int l = pleaf[p];
     if (p_nneigh) {
-      nn = p_nneigh[p];
-      nit = leaf_nits[l];
-      mx = nn;
+      const long long v = p_nneigh[p];
+      nn += v;
+      nit += leaf_nits[l];
+      mx = mx > v ? mx : v;
     }
     if (ntrial) {
-      ntr = ntrial[l];
-      mxt = ntr;
+      const long long v = ntrial[l];
+      ntr += v;
+      mxt = mxt > v ? mxt : v;
     }
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -546,7 +559,18 @@ __global__ void k_dens_stats(int nact, const int32_t *act, const int32_t *pleaf,
     mx = mx > a ? mx : a;
     mxt = mxt > b ? mxt : b;
   }
+  __shared__ long long sh[4][5];
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
+    sh[w][0] = nn; sh[w][1] = nit; sh[w][2] = ntr; sh[w][3] = mx; sh[w][4] = mxt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; k++) {
+      nn += sh[k][0]; nit += sh[k][1]; ntr += sh[k][2];
+      mx = mx > sh[k][3] ? mx : sh[k][3];
+      mxt = mxt > sh[k][4] ? mxt : sh[k][4];
+    }
     atomicAdd((unsigned long long *)&acc[0], (unsigned long long)nn);
     atomicAdd((unsigned long long *)&acc[1], (unsigned long long)nit);
     atomicAdd((unsigned long long *)&acc[2], (unsigned long long)ntr);
@@ -554,6 +578,8 @@ __global__ void k_dens_stats(int nact, const int32_t *act, const int32_t *pleaf,
     atomicMax(&acc[4], mxt);
   }
 }
+
+static inline unsigned stats_grid(int n) { return std::min(nblocks(n, 256), 512u); }
 
 __global__ void k_zero_i32(int32_t *a, int n, int32_t v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -595,7 +621,7 @@ int dens_trial_stats(phsph_ctx *c) {
   long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   PH_HIPCHECK(c, hipMemcpyAsync(c->stats_i.p, z, sizeof(z), hipMemcpyHostToDevice, s));
   if (c->nact_dens > 0)
-    hipLaunchKernelGGL(k_dens_stats, dim3(nblocks(c->nact_dens, 256)), dim3(256), 0, s, c->nact_dens,
+    hipLaunchKernelGGL(k_dens_stats, dim3(stats_grid(c->nact_dens)), dim3(256), 0, s, c->nact_dens,
                        c->act_dens.p, c->pleaf.p, nullptr, nullptr, c->ntrial.p, c->stats_i.p);
   long long acc[8];
   int32_t fl = 0;
@@ -694,8 +720,12 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(n, 256)), dim3(256), 0, s, c->p_nits.p, n, -1);
   hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(n, 256)), dim3(256), 0, s, c->p_nneigh.p, n, -1);
   const bool dbg = getenv("PHSPH_DEBUG") != nullptr;
-  // Particles whose h outgrows their group's candidate stream are handed
-  // back and run again (alone) with a wider search.
+  // Particles whose h outgrows their neighbour list, or whose list does not
+  // fit, are handed back and run again (alone) with a wider search or a
+  // longer list.
+  PH_HIPCHECK(c, c->t_nl.ensure(n + 1));
+  PH_HIPCHECK(c, c->t_hsum.ensure(n + 1));
+  PH_HIPCHECK(c, c->t_hfin.ensure(n + 1));
   double rfac = 1.1;
   for (int attempt = 0; attempt < 12; attempt++) {
     const int na = c->nact;
@@ -703,37 +733,49 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
     rc = walk_groups(c, 0, rfac, c->th_in.p, nullptr);
     if (rc) return rc;
     const int ng = c->ngroups;
+    PH_HIPCHECK(c, c->nlist.ensure((size_t)c->caps.dlist * na + 1));
     hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(n, 256)), dim3(256), 0, s, c->tmpi.p, n, 0);
     int32_t zero = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
     {
-      LaunchTimer lt(c, "density");
-      const dim3 gr(xcd_grid(ng)), b(64);
-#define DLAUNCH(KK, MM)                                                                                        \
-  hipLaunchKernelGGL((k_density<KK, MM>), gr, b, 0, s, c->dc, icall, rfac, c->act.p, na, ng, c->pleaf.p,         \
-                     c->leaves.p, c->gcand.p, c->gncand.p, c->caps.gcand, c->nd_lo.p, c->nd_hi.p, c->nd.p,       \
-                     c->iparts.p, in, c->th.p, out, c->p_nneigh.p, c->p_nits.p,                                 \
-                     c->leaf_nits.p, c->tmpi.p, grav ? c->htraj.p : nullptr, grav ? c->hconv.p : nullptr,         \
-                     c->counters.p)
+      LaunchTimer lt(c, "density_iter");
+#define DITER(KK)                                                                                              \
+  hipLaunchKernelGGL((k_dens_iter<KK>), dim3(xcd_grid(ng)), dim3(64), 0, s, c->dc, icall, rfac, c->act.p, na, ng, \
+                     c->pleaf.p, c->leaves.p, c->gcand.p, c->gncand.p, c->caps.gcand, c->nd_lo.p, c->nd_hi.p,     \
+                     c->nd.p, c->tp4.p, c->nlist.p, c->caps.dlist, c->t_nl.p, c->t_hsum.p, c->t_hfin.p,           \
+                     c->p_nits.p, c->leaf_nits.p, c->tmpi.p, grav ? c->htraj.p : nullptr,                       \
+                     grav ? c->hconv.p : nullptr, c->counters.p)
+      if (c->p.kernel == PHSPH_KERNEL_CUBIC) DITER(PHSPH_KERNEL_CUBIC); else DITER(PHSPH_KERNEL_QUINTIC);
+#undef DITER
+    }
+    {
+      LaunchTimer lt(c, "density_sums");
+#define DFIN(KK, MM)                                                                                           \
+  hipLaunchKernelGGL((k_dens_final<KK, MM>), dim3(nblocks(na, 256)), dim3(256), 0, s, c->dc, c->act.p, na,      \
+                     c->nlist.p, c->t_nl.p, c->t_hsum.p, c->t_hfin.p, c->tmpi.p, c->iparts.p, in, c->th.p, out,   \
+                     c->p_nneigh.p, c->counters.p)
       if (c->p.kernel == PHSPH_KERNEL_CUBIC) {
-        if (mhd) DLAUNCH(PHSPH_KERNEL_CUBIC, true); else DLAUNCH(PHSPH_KERNEL_CUBIC, false);
+        if (mhd) DFIN(PHSPH_KERNEL_CUBIC, true); else DFIN(PHSPH_KERNEL_CUBIC, false);
       } else {
-        if (mhd) DLAUNCH(PHSPH_KERNEL_QUINTIC, true); else DLAUNCH(PHSPH_KERNEL_QUINTIC, false);
+        if (mhd) DFIN(PHSPH_KERNEL_QUINTIC, true); else DFIN(PHSPH_KERNEL_QUINTIC, false);
       }
-#undef DLAUNCH
+#undef DFIN
     }
     PH_HIPCHECK(c, hipGetLastError());
     int32_t fl = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(&fl, c->counters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     PH_HIPCHECK(c, hipStreamSynchronize(s));
-    if (dbg) fprintf(stderr, "phsph densityiterate: attempt %d particles %d groups %d rfac %g flags %d\n", attempt, na, ng, rfac, fl);
+    if (dbg)
+      fprintf(stderr, "phsph densityiterate: attempt %d particles %d groups %d rfac %g list %d flags %d\n", attempt,
+              na, ng, rfac, c->caps.dlist, fl);
     if (fl & DFL_NOCONV) PH_RET(c, PHSPH_ERR_NOCONVERGE, "densityiterate: could not converge in density");
     if (fl & DFL_NEGH) PH_RET(c, PHSPH_ERR_INPUT, "densityiterate: setting negative h from hrho");
-    if (!(fl & DFL_REWALK)) {
+    if (!(fl & (DFL_REWALK | DFL_LIST))) {
       c->nact = 0;
       break;
     }
-    rfac *= 1.5;
+    if (fl & DFL_REWALK) rfac *= 1.5;
+    if (fl & DFL_LIST) c->caps.dlist *= 2;
     rc = compact_active(c, c->tmpi.p, n);
     if (rc) return rc;
   }
@@ -747,7 +789,7 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   PH_HIPCHECK(c, hipMemcpyAsync(c->stats_i.p, z, sizeof(z), hipMemcpyHostToDevice, s));
   if (nact > 0)
-    hipLaunchKernelGGL(k_dens_stats, dim3(nblocks(nact, 256)), dim3(256), 0, s, nact, c->act_dens.p, c->pleaf.p,
+    hipLaunchKernelGGL(k_dens_stats, dim3(stats_grid(nact)), dim3(256), 0, s, nact, c->act_dens.p, c->pleaf.p,
                        c->p_nneigh.p, c->leaf_nits.p, nullptr, c->stats_i.p);
   PH_HIPCHECK(c, hipGetLastError());
   long long acc[8];

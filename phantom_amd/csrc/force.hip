@@ -8,18 +8,18 @@
 // stress, Borve div B source subtraction, artificial resistivity, psi
 // cleaning) and one-fluid dust (sqrt(eps/(1-eps)) evolution, Delta v).
 //
-// One wavefront per group of 64 active particles, one particle per lane
-// (stream.h).  Pass 1 pulls the group's candidate stream through LDS and
-// collects each lane's symmetric neighbour set {j : q_i < R or q_j < R}
-// (force.F90:1287) with the same separately rounded FP64 tests as the
-// reference; pass 2 evaluates the pair terms.  Quantities of particle j that
-// the reference recomputes per pair (rho_j, P/rho^2, wave speed, MHD stress,
-// stopping time) are evaluated once per particle by k_force_prep with the
-// same arithmetic and packed into one 128-byte record per particle, so a
-// pair reads one cache line of j data.
+// k_flist (one wavefront per group of 64 active particles, one particle per
+// lane) collects each particle's symmetric neighbour set
+// {j : q_i < R or q_j < R} (force.F90:1287) from the group's candidate leaves
+// with the same separately rounded FP64 tests as the reference; k_force (one
+// thread per particle) evaluates the pair terms over that list.  Quantities
+// of particle j that the reference recomputes per pair (rho_j, P/rho^2, wave
+// speed, MHD stress, stopping time) are evaluated once per particle by
+// k_force_prep with the same arithmetic and packed into one 128-byte record
+// per particle, so a pair reads one cache line of j data.
 #include "records.h"
 
-enum { FFL_DTC = 32 };
+enum { FFL_LIST = 2, FFL_DTC = 32 };
 
 // per-particle j terms (force.F90:1455-1509 with get_stress)
 __global__ void k_force_prep(DevConst dc, int nlive, const int32_t *__restrict__ iparts,
@@ -95,45 +95,68 @@ __global__ void k_force_prep(DevConst dc, int nlive, const int32_t *__restrict__
   }
 }
 
-// LDS copy of the j quantities of one batch of candidate leaves, as arrays
-// (the lanes read them at independent positions)
-template <int NP, bool MHD, bool DUST, bool THERMAL>
-struct ForceStage {
-  static constexpr int NB = MHD ? NP : 1, ND = DUST ? NP : 1, NT = THERMAL ? NP : 1;
-  double x[NP], y[NP], z[NP], hinv[NP];
-  double vx[NP], vy[NP], vz[NP], u[NT];
-  double rho[NP], rho1[NP], pr[NP], pro2[NP], vwave[NP];
-  float gradh[NP], alpha[NP];
-  double Bx[NB], By[NB], Bz[NB], psi[NB], sxx[NB], sxy[NB], sxz[NB], syy[NB], syz[NB], szz[NB];
-  double eps[ND], sqeps[ND], ts[ND];
-  int j[NP];
-  unsigned char leaf[NP];
-  int8_t bold[NP];  // ibin_old (block timesteps)
-};
-
-// Force: one wavefront per group of 64 active particles, one per lane.
-template <int K, bool MHD, bool DUST, bool THERMAL>
+// Force, part 1: the symmetric neighbour set {j : q_i < R or q_j < R}
+// (force.F90:1287) of every active particle, with the reference's separately
+// rounded FP64 tests.  One wavefront per group of 64 active particles; each
+// lane visits the candidate leaves it can reach, a leaf's particles lying
+// within its size of its centre and having h at most the leaf's hmax.
 __global__ void __launch_bounds__(64)
-k_force(DevConst dc, const int32_t *__restrict__ act, int nact, int ngroups, const int32_t *__restrict__ pleaf,
+k_flist(DevConst dc, const int32_t *__restrict__ act, int nact, int ngroups, const int32_t *__restrict__ pleaf,
         const int32_t *__restrict__ leaves, const uint32_t *__restrict__ gcand, const int32_t *__restrict__ gncand,
         int capg, const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi,
-        const phsph_kdnode *__restrict__ nd, const int32_t *__restrict__ iparts, const FRec *__restrict__ rec,
-        const FRecB *__restrict__ recB, const FRecD *__restrict__ recD, ForceIOArgs out,
-        const int8_t *__restrict__ tbin_old, int icall, unsigned long long *__restrict__ dtred,
-        int32_t *__restrict__ flags) {
-  constexpr int NL = (MHD || DUST) ? 8 : 16, NP = NL * PH_MINPART, NM = (NP + 63) / 64;
-  __shared__ BatchMeta<NL> M;
-  __shared__ ForceStage<NP, MHD, DUST, THERMAL> S;
+        const phsph_kdnode *__restrict__ nd, const double *__restrict__ hmaxn, const FRec *__restrict__ rec,
+        uint32_t *__restrict__ nlist, int capl, int32_t *__restrict__ t_nl, int32_t *__restrict__ flags) {
+  __shared__ ListStage S;
   const int g = xcd_block(blockIdx.x, gridDim.x);
   if (g >= ngroups) return;
   const int lane = threadIdx.x;
   const int t = g * GROUP + lane;
   const bool live = t < nact;
+  const double R2 = dc.radkern2;
+  int p = -1;
+  double xi = 0., yi = 0., zi = 0., hi = 0., lcx = 0., lcy = 0., lcz = 0., lsz = 0.;
+  if (live) {
+    p = act[t];
+    const int c = leaves[pleaf[p]];
+    xi = rec[p].x; yi = rec[p].y; zi = rec[p].z; hi = rec[p].h;
+    lcx = nd[c].xcen[0]; lcy = nd[c].xcen[1]; lcz = nd[c].xcen[2]; lsz = nd[c].size;
+  }
+  const double hi1 = 1. / fabs(hi);
+  const double hi21 = xmul(hi1, hi1);
+  int nl = 0;
+  build_list(
+      S, dc, gcand + (size_t)g * capg, gncand[g], nd, nd_lo, nd_hi, hmaxn, rec, lcx, lcy, lcz, lsz,
+      [&](double ccx, double ccy, double ccz, double csz, double chm) {
+        return live && leaf_in_reach(dc, xi, yi, zi, ccx, ccy, ccz, csz, dc.radkern * fmax(hi, chm));
+      },
+      [&](int j, int code, double xj, double yj, double zj, double hj1) {
+        double dx, dy, dz;
+        sep_of(dc, code, xi, yi, zi, xj, yj, zj, dx, dy, dz);
+        const double r2 = r2_exact(dx, dy, dz);
+        if (j != p && (xmul(r2, hi21) < R2 || xmul(r2, xmul(hj1, hj1)) < R2)) {
+          if (nl < capl) nlist[(size_t)nl * nact + t] = cand_pack(j, code);
+          nl++;
+        }
+      });
+  if (!live) return;
+  t_nl[t] = nl;
+  if (nl > capl) atomicOr(flags, FFL_LIST);
+}
+
+// Force, part 2: the pair terms (force.F90:1227-2056) over each particle's
+// list and finish_cell_and_store_results.  One thread per active particle.
+template <int K, bool MHD, bool DUST, bool THERMAL>
+__global__ void __launch_bounds__(256)
+k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const uint32_t *__restrict__ nlist,
+        const int32_t *__restrict__ t_nl, const int32_t *__restrict__ iparts, const FRec *__restrict__ rec,
+        const FRecB *__restrict__ recB, const FRecD *__restrict__ recD, ForceIOArgs out,
+        const int8_t *__restrict__ tbin_old, int icall, unsigned long long *__restrict__ dtred,
+        int32_t *__restrict__ flags) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = t < nact;
   const int p = live ? act[t] : act[0];
   const double pm = dc.pmass;
   const double R2 = dc.radkern2;
-  const int c = leaves[pleaf[p]];
-  const double lcx = nd[c].xcen[0], lcy = nd[c].xcen[1], lcz = nd[c].xcen[2], lsz = nd[c].size;
   const FRec Ri = rec[p];
   const double xi = Ri.x, yi = Ri.y, zi = Ri.z;
   const double hi = Ri.h;
@@ -177,212 +200,148 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, int ngroups, con
   const double mrhoi5 = 0.5 * pm * rho1i;
   const double auterm = mrhoi5 * dc.alphau;
   const double avBterm = mrhoi5 * dc.alphaB * rho1i;
-  // --- pair terms over the candidate stream (force.F90:1227-2056) ---
+  // --- pair terms over the neighbour list (force.F90:1227-2056) ---
   double fx = 0., fy = 0., fz = 0., drhodt = 0., dudtdiss = 0., dendtdiss = 0.;
   double divBsym = 0., dBx_s = 0., dBy_s = 0., dBz_s = 0., divBdiff = 0.;
   double ddust = 0., dudtdust = 0., dvx_s = 0., dvy_s = 0., dvz_s = 0.;
   double vsigmax = 0.;
   const bool usej_always = MHD || DUST || (THERMAL && !dc.gravity);
-  int nl = 0;
   const bool ind = out.ibin != nullptr;
   int ibin_neighi = 0;
   const int8_t ibinnow_m1 = (int8_t)(out.ibinnow - 1);
-  const int nc = gncand[g];
-  const uint32_t *gc = gcand + (size_t)g * capg;
-  for (int b0 = 0; b0 < nc; b0 += NL) {
-    int nb;
-    const int total = batch_meta<NL>(M, gc, b0, nc, nd, nd_lo, nd_hi, lane, nb);
-    for (int s = lane; s < total; s += GROUP) {
-      const int L = batch_leaf_of<NL>(M, nb, s);
-      const int j = M.lo[L] + (s - M.start[L]);
-      const FRec R = rec[j];
-      S.x[s] = R.x; S.y[s] = R.y; S.z[s] = R.z; S.hinv[s] = R.hinv;
-      S.vx[s] = R.vx; S.vy[s] = R.vy; S.vz[s] = R.vz;
-      if (THERMAL) S.u[s] = R.u;
-      S.rho[s] = R.rho; S.rho1[s] = R.rho1; S.pr[s] = R.pr; S.pro2[s] = R.pro2; S.vwave[s] = R.vwave;
-      S.gradh[s] = R.gradh; S.alpha[s] = R.alpha;
-      if (MHD) {
-        const FRecB B = recB[j];
-        S.Bx[s] = B.Bx; S.By[s] = B.By; S.Bz[s] = B.Bz; S.psi[s] = B.psi;
-        S.sxx[s] = B.sxx; S.sxy[s] = B.sxy; S.sxz[s] = B.sxz; S.syy[s] = B.syy; S.syz[s] = B.syz; S.szz[s] = B.szz;
-      }
-      if (DUST) {
-        const FRecD D = recD[j];
-        S.eps[s] = D.eps; S.sqeps[s] = D.sqeps; S.ts[s] = D.ts;
-      }
-      S.j[s] = j;
-      S.leaf[s] = (unsigned char)L;
-      if (ind) S.bold[s] = tbin_old[j];
-    }
-    __syncthreads();
-    // symmetric neighbour test {q_i < R or q_j < R} (force.F90:1250-1290)
-    unsigned long long mk[NM];
-    for (int w = 0; w < NM; w++) mk[w] = 0ull;
-    SepCodes codes;
-    for (int L = 0; L < nb; L++) {
-      const int code = leaf_sep_code(dc, M.cx[L], M.cy[L], M.cz[L], M.sz[L], lcx, lcy, lcz, lsz);
-      codes.set(L, code);
-      const int s1 = M.start[L + 1];
-      for (int s = M.start[L]; s < s1; s++) {
-        double dx, dy, dz;
-        sep_of(dc, code, xi, yi, zi, S.x[s], S.y[s], S.z[s], dx, dy, dz);
-        const double r2 = r2_exact(dx, dy, dz);
-        const double hj1 = S.hinv[s];
-        const bool hit = live && S.j[s] != p && (xmul(r2, hi21) < R2 || xmul(r2, xmul(hj1, hj1)) < R2);
-        const unsigned long long bit = (unsigned long long)hit << (s & 63);
-#pragma unroll
-        for (int w = 0; w < NM; w++)
-          if ((s >> 6) == w) mk[w] |= bit;
-      }
-    }
+  const int nl = live ? t_nl[t] : 0;
+  for (int k = 0; k < nl; k++) {
+    const uint32_t e = nlist[(size_t)k * nact + t];
+    const int j = cand_node(e);
+    const FRec &Rj = rec[j];
     if (ind) {
       // force.F90:1346-1352: every neighbour of an active particle is
       // flagged to be woken; the limiter needs the largest neighbour bin
-#pragma unroll
-      for (int w = 0; w < NM; w++) {
-        unsigned long long m = mk[w], u = mk[w];
-        while (m != 0ull) {
-          const int s = w * 64 + (int)__builtin_ctzll(m);
-          m &= m - 1ull;
-          ibin_neighi = max(ibin_neighi, (int)S.bold[s]);
-        }
-        for (int off = 32; off > 0; off >>= 1) u |= __shfl_xor(u, off, 64);
-        const int s = w * 64 + lane;
-        if ((u >> lane) & 1ull) {
-          const int o = abs(iparts[S.j[s]]) - 1;
-          if (out.ibin_wake[o] < ibinnow_m1) out.ibin_wake[o] = ibinnow_m1;
-        }
+      ibin_neighi = max(ibin_neighi, (int)tbin_old[j]);
+      const int oj = abs(iparts[j]) - 1;
+      if (out.ibin_wake[oj] < ibinnow_m1) out.ibin_wake[oj] = ibinnow_m1;
+    }
+    double dx, dy, dz;
+    sep_of(dc, cand_code(e), xi, yi, zi, Rj.x, Rj.y, Rj.z, dx, dy, dz);
+    const double rij2 = r2_exact(dx, dy, dz);
+    const double q2i = xmul(rij2, hi21);
+    const double hj1 = Rj.hinv;
+    const double hj21 = xmul(hj1, hj1);
+    const double q2j = xmul(rij2, hj21);
+    double rij1, qi;
+    if (rij2 > PH_TINY64) {
+      rij1 = 1. / sqrt(rij2);
+      qi = (rij2 * rij1) * hi1;
+    } else {
+      rij1 = 0.;
+      qi = 0.;
+    }
+    const double grkerni = (q2i < R2) ? kern_g<K, double>(q2i, qi) * hfacgrkern : 0.;
+    double grkernj = 0.;
+    bool usej = false;
+    if (q2j < R2) {
+      const double qj = (rij2 * rij1) * hj1;
+      grkernj = kern_g<K, double>(q2j, qj) * hj21 * hj21 * dc.cnormk * (double)Rj.gradh;
+      usej = true;
+    }
+    if (usej_always) usej = true;
+    const double runix = dx * rij1, runiy = dy * rij1, runiz = dz * rij1;
+    const double dvx = vxi - Rj.vx, dvy = vyi - Rj.vy, dvz = vzi - Rj.vz;
+    const double projv = dvx * runix + dvy * runiy + dvz * runiz;
+    const double denij = THERMAL ? (eni - Rj.u) : 0.;
+    const double vsigi = fmax(vwavei - dc.beta * projv, 0.);
+    const double vsigavi = fmax(alphai * vwavei - dc.beta * projv, 0.);
+    vsigmax = fmax(vsigmax, vsigi);
+    double rho1j = 0., rho21j = 0., pro2j = 0., vwavej = 0., vsigavj = 0., autermj = 0., avBtermj = 0.;
+    double rhoj = 0., prj = 0.;
+    if (usej) {
+      rhoj = Rj.rho;
+      rho1j = Rj.rho1;
+      rho21j = rho1j * rho1j;
+      prj = Rj.pr;
+      pro2j = Rj.pro2;
+      vwavej = Rj.vwave;
+      const double mrhoj5 = 0.5 * pm * rho1j;
+      autermj = mrhoj5 * dc.alphau;
+      avBtermj = mrhoj5 * dc.alphaB * rho1j;
+      vsigmax = fmax(vsigmax, fmax(vwavej - dc.beta * projv, 0.));
+      vsigavj = fmax((double)Rj.alpha * vwavej - dc.beta * projv, 0.);
+    }
+    double qrho2i = 0., qrho2j = 0.;
+    if (projv < 0.) {
+      qrho2i = -0.5 * rho1i * vsigavi * projv;
+      if (usej) qrho2j = -0.5 * rho1j * vsigavj * projv;
+    }
+    const double dudtdissi = pm * qrho2i * projv * grkerni;
+    const double gradpi = pm * (pro2i + qrho2i) * grkerni;
+    const double gradpj = usej ? pm * (pro2j + qrho2j) * grkernj : 0.;
+    double dendissterm = 0.;
+    if (THERMAL) {
+      double vsigu;
+      if (dc.gravity) vsigu = fabs(projv);
+      else vsigu = sqrt(fabs(pri - prj) * (2. / (rhoi + rhoj)));
+      dendissterm = vsigu * denij * (auterm * grkerni + autermj * grkernj);
+    }
+    double projsx = 0., projsy = 0., projsz = 0., dudtresist = 0.;
+    if (MHD) {
+      const FRecB &Mj = recB[j];
+      const double Bxj = Mj.Bx, Byj = Mj.By, Bzj = Mj.Bz, psij = Mj.psi;
+      const double dBx = Bxi - Bxj, dBy = Byi - Byj, dBz = Bzi - Bzj;
+      const double projBi = Bxi * runix + Byi * runiy + Bzi * runiz;
+      const double projBj = Bxj * runix + Byj * runiy + Bzj * runiz;
+      const double projdB = dBx * runix + dBy * runiy + dBz * runiz;
+      const double dB2 = dBx * dBx + dBy * dBy + dBz * dBz;
+      divBdiff += -pm * projdB * grkerni;
+      const double ax = dvx - projv * runix, ay = dvy - projv * runiy, az = dvz - projv * runiz;
+      const double vsigB = sqrt(ax * ax + ay * ay + az * az);
+      const double dBdissterm = (avBterm * grkerni + avBtermj * grkernj) * vsigB;
+      if (dc.iresistive_heating > 0) dudtresist = -0.5 * dB2 * dBdissterm;
+      const double pmi = pm * rho21i * grkerni;
+      const double pmj = pm * rho21j * grkernj;
+      const double termi = pmi * projBi;
+      divBsym += termi + pmj * projBj;
+      const double dpsiterm = dc.overcleanfac * (pmi * psii * vwavei + pmj * psij * vwavej);
+      dBx_s += -termi * dvx + dBdissterm * dBx - dpsiterm * runix;
+      dBy_s += -termi * dvy + dBdissterm * dBy - dpsiterm * runiy;
+      dBz_s += -termi * dvz + dBdissterm * dBz - dpsiterm * runiz;
+      projsx = (sxi * runix + sxyi * runiy + sxzi * runiz) * grkerni +
+               (Mj.sxx * runix + Mj.sxy * runiy + Mj.sxz * runiz) * grkernj;
+      projsy = (sxyi * runix + syi * runiy + syzi * runiz) * grkerni +
+               (Mj.sxy * runix + Mj.syy * runiy + Mj.syz * runiz) * grkernj;
+      projsz = (sxzi * runix + syzi * runiy + szi * runiz) * grkerni +
+               (Mj.sxz * runix + Mj.syz * runiy + Mj.szz * runiz) * grkernj;
+    }
+    const double gradp = gradpi + gradpj;
+    fx = fx - runix * gradp - projsx;
+    fy = fy - runiy * gradp - projsy;
+    fz = fz - runiz * gradp - projsz;
+    drhodt += projv * grkerni;
+    if (THERMAL) {
+      dudtdiss += dudtdissi + dudtresist;
+      dendtdiss += dendissterm;
+    }
+    if (DUST) {
+      const FRecD &Dj = recD[j];
+      const double epsj = Dj.eps;
+      const double tsj = Dj.ts;
+      const double epstsj = epsj * tsj;
+      if (epsi > 0. || epsj > 0.) {
+        const double grkernav = 0.5 * (grkerni + grkernj);
+        const double dft = pm * Dj.sqeps * rho1j *
+                           ((tsi - epstsi) * (1. - epsi) / (1. - epsi) + (tsj - epstsj) * (1. - epsj) / (1. - epsj)) *
+                           (pri - prj) * grkernav * rij1;
+        ddust -= dft;
+        if (THERMAL) dudtdust -= sqepsi * dft * denij;
+      }
+      if (epsi < 1.) {
+        const double term = tsi / (1. - epsi) * pm * (pro2i * grkerni + pro2j * grkernj);
+        dvx_s += term * runix; dvy_s += term * runiy; dvz_s += term * runiz;
       }
     }
-#pragma unroll
-    for (int w = 0; w < NM; w++) {
-      unsigned long long m = mk[w];
-      nl += __popcll(m);
-      while (m != 0ull) {
-        const int s = w * 64 + (int)__builtin_ctzll(m);
-        m &= m - 1ull;
-        double dx, dy, dz;
-        sep_of(dc, codes.get(S.leaf[s]), xi, yi, zi, S.x[s], S.y[s], S.z[s], dx, dy, dz);
-        const double rij2 = r2_exact(dx, dy, dz);
-        const double q2i = xmul(rij2, hi21);
-        const double hj1 = S.hinv[s];
-        const double hj21 = xmul(hj1, hj1);
-        const double q2j = xmul(rij2, hj21);
-        double rij1, qi;
-        if (rij2 > PH_TINY64) {
-          rij1 = 1. / sqrt(rij2);
-          qi = (rij2 * rij1) * hi1;
-        } else {
-          rij1 = 0.;
-          qi = 0.;
-        }
-        const double grkerni = (q2i < R2) ? kern_g<K, double>(q2i, qi) * hfacgrkern : 0.;
-        double grkernj = 0.;
-        bool usej = false;
-        if (q2j < R2) {
-          const double qj = (rij2 * rij1) * hj1;
-          grkernj = kern_g<K, double>(q2j, qj) * hj21 * hj21 * dc.cnormk * (double)S.gradh[s];
-          usej = true;
-        }
-        if (usej_always) usej = true;
-        const double runix = dx * rij1, runiy = dy * rij1, runiz = dz * rij1;
-        const double dvx = vxi - S.vx[s], dvy = vyi - S.vy[s], dvz = vzi - S.vz[s];
-        const double projv = dvx * runix + dvy * runiy + dvz * runiz;
-        const double denij = THERMAL ? (eni - S.u[s]) : 0.;
-        const double vsigi = fmax(vwavei - dc.beta * projv, 0.);
-        const double vsigavi = fmax(alphai * vwavei - dc.beta * projv, 0.);
-        vsigmax = fmax(vsigmax, vsigi);
-        double rho1j = 0., rho21j = 0., pro2j = 0., vwavej = 0., vsigavj = 0., autermj = 0., avBtermj = 0.;
-        double rhoj = 0., prj = 0.;
-        if (usej) {
-          rhoj = S.rho[s];
-          rho1j = S.rho1[s];
-          rho21j = rho1j * rho1j;
-          prj = S.pr[s];
-          pro2j = S.pro2[s];
-          vwavej = S.vwave[s];
-          const double mrhoj5 = 0.5 * pm * rho1j;
-          autermj = mrhoj5 * dc.alphau;
-          avBtermj = mrhoj5 * dc.alphaB * rho1j;
-          vsigmax = fmax(vsigmax, fmax(vwavej - dc.beta * projv, 0.));
-          vsigavj = fmax((double)S.alpha[s] * vwavej - dc.beta * projv, 0.);
-        }
-        double qrho2i = 0., qrho2j = 0.;
-        if (projv < 0.) {
-          qrho2i = -0.5 * rho1i * vsigavi * projv;
-          if (usej) qrho2j = -0.5 * rho1j * vsigavj * projv;
-        }
-        const double dudtdissi = pm * qrho2i * projv * grkerni;
-        const double gradpi = pm * (pro2i + qrho2i) * grkerni;
-        const double gradpj = usej ? pm * (pro2j + qrho2j) * grkernj : 0.;
-        double dendissterm = 0.;
-        if (THERMAL) {
-          double vsigu;
-          if (dc.gravity) vsigu = fabs(projv);
-          else vsigu = sqrt(fabs(pri - prj) * (2. / (rhoi + rhoj)));
-          dendissterm = vsigu * denij * (auterm * grkerni + autermj * grkernj);
-        }
-        double projsx = 0., projsy = 0., projsz = 0., dudtresist = 0.;
-        if (MHD) {
-          const double Bxj = S.Bx[s], Byj = S.By[s], Bzj = S.Bz[s], psij = S.psi[s];
-          const double dBx = Bxi - Bxj, dBy = Byi - Byj, dBz = Bzi - Bzj;
-          const double projBi = Bxi * runix + Byi * runiy + Bzi * runiz;
-          const double projBj = Bxj * runix + Byj * runiy + Bzj * runiz;
-          const double projdB = dBx * runix + dBy * runiy + dBz * runiz;
-          const double dB2 = dBx * dBx + dBy * dBy + dBz * dBz;
-          divBdiff += -pm * projdB * grkerni;
-          const double ax = dvx - projv * runix, ay = dvy - projv * runiy, az = dvz - projv * runiz;
-          const double vsigB = sqrt(ax * ax + ay * ay + az * az);
-          const double dBdissterm = (avBterm * grkerni + avBtermj * grkernj) * vsigB;
-          if (dc.iresistive_heating > 0) dudtresist = -0.5 * dB2 * dBdissterm;
-          const double pmi = pm * rho21i * grkerni;
-          const double pmj = pm * rho21j * grkernj;
-          const double termi = pmi * projBi;
-          divBsym += termi + pmj * projBj;
-          const double dpsiterm = dc.overcleanfac * (pmi * psii * vwavei + pmj * psij * vwavej);
-          dBx_s += -termi * dvx + dBdissterm * dBx - dpsiterm * runix;
-          dBy_s += -termi * dvy + dBdissterm * dBy - dpsiterm * runiy;
-          dBz_s += -termi * dvz + dBdissterm * dBz - dpsiterm * runiz;
-          projsx = (sxi * runix + sxyi * runiy + sxzi * runiz) * grkerni +
-                   (S.sxx[s] * runix + S.sxy[s] * runiy + S.sxz[s] * runiz) * grkernj;
-          projsy = (sxyi * runix + syi * runiy + syzi * runiz) * grkerni +
-                   (S.sxy[s] * runix + S.syy[s] * runiy + S.syz[s] * runiz) * grkernj;
-          projsz = (sxzi * runix + syzi * runiy + szi * runiz) * grkerni +
-                   (S.sxz[s] * runix + S.syz[s] * runiy + S.szz[s] * runiz) * grkernj;
-        }
-        const double gradp = gradpi + gradpj;
-        fx = fx - runix * gradp - projsx;
-        fy = fy - runiy * gradp - projsy;
-        fz = fz - runiz * gradp - projsz;
-        drhodt += projv * grkerni;
-        if (THERMAL) {
-          dudtdiss += dudtdissi + dudtresist;
-          dendtdiss += dendissterm;
-        }
-        if (DUST) {
-          const double epsj = S.eps[s];
-          const double tsj = S.ts[s];
-          const double epstsj = epsj * tsj;
-          if (epsi > 0. || epsj > 0.) {
-            const double grkernav = 0.5 * (grkerni + grkernj);
-            const double dft = pm * S.sqeps[s] * rho1j *
-                               ((tsi - epstsi) * (1. - epsi) / (1. - epsi) + (tsj - epstsj) * (1. - epsj) / (1. - epsj)) *
-                               (pri - prj) * grkernav * rij1;
-            ddust -= dft;
-            if (THERMAL) dudtdust -= sqepsi * dft * denij;
-          }
-          if (epsi < 1.) {
-            const double term = tsi / (1. - epsi) * pm * (pro2i * grkerni + pro2j * grkernj);
-            dvx_s += term * runix; dvy_s += term * runiy; dvz_s += term * runiz;
-          }
-        }
-  
-      }
-    }
-    __syncthreads();
   }
   // --- finish_cell_and_store_results ---
   double dtc_o = PH_HUGE64, dtf_o = PH_HUGE64;
+  int bmax = -1;
   if (live) {
       const int o = abs(iparts[p]) - 1;
       if (MHD) {
@@ -455,25 +414,13 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, int ngroups, con
         int b = get_newbin_d(dti, dc.dtmax, (int)out.ibin[o], icall < 2, out.nbinmax, out.istepfrac);
         b = max(b, ibin_neighi - 1);  // Saitoh-Makino limiter
         out.ibin[o] = (int8_t)b;
-        atomicMax(&flags[1], b);
+        bmax = b;
         dtc_o = PH_HUGE64;
         dtf_o = PH_HUGE64;
       }
   }
-  unsigned long long np64 = (unsigned long long)nl;
-  for (int off = 32; off > 0; off >>= 1) np64 += __shfl_xor(np64, off, 64);
-  // global minima (force.F90:3312-3327): positive doubles order as integers
-  unsigned long long a = __double_as_longlong(dtc_o), b = __double_as_longlong(dtf_o);
-  for (int off = 32; off > 0; off >>= 1) {
-    unsigned long long a2 = __shfl_xor(a, off, 64), b2 = __shfl_xor(b, off, 64);
-    a = a < a2 ? a : a2;
-    b = b < b2 ? b : b2;
-  }
-  if (lane == 0) {
-    atomicMin(&dtred[0], a);
-    atomicMin(&dtred[1], b);
-    atomicAdd(&dtred[2], np64);
-  }
+  // global minima (force.F90:3312-3327), pair count, highest bin
+  reduce_force_block(dtc_o, dtf_o, nl, bmax, dtred, flags);
 }
 
 __global__ void k_force_flags(DevConst dc, int nlive, const int32_t *__restrict__ iparts,
@@ -494,12 +441,12 @@ __global__ void k_leaf_hmax(int nleaves, const int32_t *leaves, const int32_t *n
   int c = leaves[l];
   double hm = 0.;
   for (int p = nd_lo[c]; p <= nd_hi[c]; p++) hm = fmax(hm, h[p]);
-  unsigned long long v = __double_as_longlong(hm);
-  int n = c;
-  while (n != 0) {
-    atomicMax(&hmaxn[n], v);
-    n = nd[n].parent;
-  }
+  const unsigned long long v = __double_as_longlong(hm);
+  hmaxn[c] = v;  // each leaf has one thread
+  // A node already at >= v was raised by a thread carrying a value >= v,
+  // which goes on raising the nodes above it: stop there.
+  for (int n = nd[c].parent; n != 0; n = nd[n].parent)
+    if (atomicMax(&hmaxn[n], v) >= v) break;
 }
 __global__ void k_zero_u64(unsigned long long *a, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -608,6 +555,27 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
   rc = walk_groups(c, 1, 1.0, c->fh.p, c->hmaxf.p);
   if (rc) return rc;
   const int ng = c->ngroups;
+  PH_HIPCHECK(c, c->t_nl.ensure(nact + 1));
+  // neighbour lists; a list that does not fit grows the capacity and the
+  // lists are built again (the pair pass below runs once)
+  for (int attempt = 0;; attempt++) {
+    PH_HIPCHECK(c, c->nlist.ensure((size_t)c->caps.flist * nact + 1));
+    int32_t zero = 0;
+    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    if (nact > 0) {
+      LaunchTimer lt(c, "force_list");
+      hipLaunchKernelGGL(k_flist, dim3(xcd_grid(ng)), dim3(64), 0, s, c->dc, c->act.p, nact, ng, c->pleaf.p,
+                         c->leaves.p, c->gcand.p, c->gncand.p, c->caps.gcand, c->nd_lo.p, c->nd_hi.p, c->nd.p,
+                         c->hmaxf.p, rec, c->nlist.p, c->caps.flist, c->t_nl.p, c->counters.p);
+    }
+    PH_HIPCHECK(c, hipGetLastError());
+    int32_t fl = 0;
+    PH_HIPCHECK(c, hipMemcpyAsync(&fl, c->counters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    PH_HIPCHECK(c, hipStreamSynchronize(s));
+    if (!(fl & FFL_LIST)) break;
+    if (attempt >= 6) PH_RET(c, PHSPH_ERR_CAPACITY, "force: neighbour list capacity exceeded");
+    c->caps.flist *= 2;
+  }
   PH_HIPCHECK(c, c->dtred.ensure(3));
   {
     unsigned long long init[3];
@@ -620,12 +588,11 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
     PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, zero2, sizeof(zero2), hipMemcpyHostToDevice, s));
     if (nact > 0) {
       LaunchTimer lt(c, "force");
-      const dim3 gr(xcd_grid(ng)), b(64);
+      const dim3 gr(nblocks(nact, 256)), b(256);
       const bool th = c->p.maxvxyzu >= 4;
 #define FL(KK, M, D, T)                                                                                        \
-  hipLaunchKernelGGL((k_force<KK, M, D, T>), gr, b, 0, s, c->dc, c->act.p, nact, ng, c->pleaf.p, c->leaves.p,    \
-                     c->gcand.p, c->gncand.p, c->caps.gcand, c->nd_lo.p, c->nd_hi.p, c->nd.p, c->iparts.p,     \
-                     rec, recB, recD, a, c->tbin_old.p, icall, c->dtred.p, c->counters.p)
+  hipLaunchKernelGGL((k_force<KK, M, D, T>), gr, b, 0, s, c->dc, c->act.p, nact, c->nlist.p, c->t_nl.p,         \
+                     c->iparts.p, rec, recB, recD, a, c->tbin_old.p, icall, c->dtred.p, c->counters.p)
 #define FLK(KK)                                          \
   if (mhd && dust) { if (th) FL(KK, true, true, true); else FL(KK, true, true, false); }     \
   else if (mhd) { if (th) FL(KK, true, false, true); else FL(KK, true, false, false); }      \
@@ -658,7 +625,7 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
 // ------------------------------------------------------------------------
 // set_active_particles (utils_indtimesteps.f90:114-185)
 // ------------------------------------------------------------------------
-__global__ void k_set_active(int np, const double *__restrict__ xyzh, int ldx, int8_t *__restrict__ iphase,
+__global__ void __launch_bounds__(256) k_set_active(int np, const double *__restrict__ xyzh, int ldx, int8_t *__restrict__ iphase,
                              const int8_t *__restrict__ ibin, int nbinmax, int istepfrac, int32_t *__restrict__ acc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int na = 0, nl = 0, bad = 0;
@@ -676,7 +643,16 @@ __global__ void k_set_active(int np, const double *__restrict__ xyzh, int ldx, i
   na = wave_sum_i(na);
   nl = wave_sum_i(nl);
   bad = wave_sum_i(bad);
+  __shared__ int sh[4][3];
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
+    sh[w][0] = na; sh[w][1] = nl; sh[w][2] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; k++) {
+      na += sh[k][0]; nl += sh[k][1]; bad += sh[k][2];
+    }
     atomicAdd(&acc[0], na);
     atomicAdd(&acc[1], nl);
     atomicAdd(&acc[2], bad);

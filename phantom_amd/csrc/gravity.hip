@@ -106,8 +106,12 @@ __global__ void k_hmax_replay(int nleaves, const int32_t *__restrict__ leaves, c
     hcell = hm;
   }
   if (!set) return;
-  ghmax[c] = (unsigned long long)__double_as_longlong(hset);
-  for (int n = nd[c].parent; n != 0; n = nd[n].parent) atomic_max_pos(&ghmax[n], hset);
+  const unsigned long long v = (unsigned long long)__double_as_longlong(hset);
+  ghmax[c] = v;
+  // a node already at >= v was raised by a thread that carries a value >= v
+  // further up
+  for (int n = nd[c].parent; n != 0; n = nd[n].parent)
+    if (atomicMax(&ghmax[n], v) >= v) break;
 }
 
 // ------------------------------------------------------------------------
@@ -369,7 +373,7 @@ k_force_grav(DevConst dc, const int32_t *__restrict__ act, int nact, const int32
              unsigned long long *__restrict__ dtred, int32_t *__restrict__ flags) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   double dtc_o = PH_HUGE64, dtf_o = PH_HUGE64;
-  int nl = 0;
+  int nl = 0, bmax = -1;
   if (t < nact) {
     const int p = act[t];
     const int l = pleaf[p];
@@ -543,24 +547,12 @@ k_force_grav(DevConst dc, const int32_t *__restrict__ act, int nact, const int32
       int b = get_newbin_d(dti, dc.dtmax, (int)out.ibin[o], icall < 2, out.nbinmax, out.istepfrac);
       b = max(b, ibin_neighi - 1);
       out.ibin[o] = (int8_t)b;
-      atomicMax(&flags[1], b);
+      bmax = b;
       dtc_o = PH_HUGE64;
       dtf_o = PH_HUGE64;
     }
   }
-  unsigned long long np64 = (unsigned long long)nl;
-  for (int off = 32; off > 0; off >>= 1) np64 += __shfl_xor(np64, off, 64);
-  unsigned long long a = __double_as_longlong(dtc_o), b = __double_as_longlong(dtf_o);
-  for (int off = 32; off > 0; off >>= 1) {
-    unsigned long long a2 = __shfl_xor(a, off, 64), b2 = __shfl_xor(b, off, 64);
-    a = a < a2 ? a : a2;
-    b = b < b2 ? b : b2;
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomicMin(&dtred[0], a);
-    atomicMin(&dtred[1], b);
-    atomicAdd(&dtred[2], np64);
-  }
+  reduce_force_block(dtc_o, dtf_o, nl, bmax, dtred, flags);
 }
 
 // ------------------------------------------------------------------------

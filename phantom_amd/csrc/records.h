@@ -52,3 +52,41 @@ __device__ __forceinline__ int get_newbin_d(double dti, double dtmax, int ibin_o
   }
   return ibin_oldi;
 }
+
+// Block-wide reduction of the force results that are global: the timestep
+// minima (positive doubles order as their bit patterns), the interacting
+// pair count and the highest new bin (bmax < 0: none).  Wavefront shuffles,
+// then one set of atomics per block.  Every thread of the block calls it.
+__device__ __forceinline__ void reduce_force_block(double dtc, double dtf, int npairs, int bmax,
+                                                   unsigned long long *dtred, int32_t *flags) {
+  __shared__ unsigned long long s_a[16], s_b[16], s_n[16];
+  __shared__ int s_m[16];
+  unsigned long long a = (unsigned long long)__double_as_longlong(dtc);
+  unsigned long long b = (unsigned long long)__double_as_longlong(dtf);
+  unsigned long long n = (unsigned long long)npairs;
+  int m = bmax;
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long a2 = __shfl_xor(a, off, 64), b2 = __shfl_xor(b, off, 64);
+    a = a < a2 ? a : a2;
+    b = b < b2 ? b : b2;
+    n += __shfl_xor(n, off, 64);
+    m = max(m, __shfl_xor(m, off, 64));
+  }
+  const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_a[w] = a; s_b[w] = b; s_n[w] = n; s_m[w] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < nw; k++) {
+      a = a < s_a[k] ? a : s_a[k];
+      b = b < s_b[k] ? b : s_b[k];
+      n += s_n[k];
+      m = max(m, s_m[k]);
+    }
+    atomicMin(&dtred[0], a);
+    atomicMin(&dtred[1], b);
+    atomicAdd(&dtred[2], n);
+    if (m >= 0) atomicMax(&flags[1], m);
+  }
+}

@@ -1,5 +1,6 @@
-// stream.h -- particle groups and their candidate streams, shared by the
-// density and force pair kernels.
+// stream.h -- particle groups, their candidate leaves and the per-lane
+// traversal of the particles a lane can reach, shared by the density and
+// force list kernels.
 //
 // A group is 64 consecutive entries of the active list (tree order), handled
 // by one wavefront with one particle per lane.  k_group_walk walks the tree
@@ -8,21 +9,18 @@
 // bounding box to the node centre is below size_node + rcut, rcut being the
 // group's search radius (or radkern*hmax of the node itself in the force
 // walk).  Since |x_i - x_j| >= dist(box, xcen_n) - size_n for every particle
-// i in the box and j in node n (also with periodic images), the stream is a
-// superset of every lane's getneigh candidates (kdtree.F90:1288-1292); the
-// neighbour sets themselves are decided per pair by the reference's q^2
-// tests, so they do not change.
+// i in the box and j in node n (also with periodic images), the candidate
+// leaves are a superset of every lane's getneigh candidates
+// (kdtree.F90:1288-1292); the neighbour sets themselves are decided per pair
+// by the reference's q^2 tests, so they do not change.
 //
-// The pair kernels pull the stream through LDS, NL leaves (at most
-// NL*minpart particles) at a time: every staged quantity of a leaf's
-// particles is loaded once per group with coalesced loads.  Each lane tests
-// every staged particle against its own (positions read as LDS broadcasts),
-// keeps the hits as a bit mask, and then evaluates its hits reading the j
-// data from LDS, so no pair ever gathers from global memory and no neighbour
-// list is stored.  Per candidate leaf every lane computes the periodic
-// offset that get_sep (kdtree.F90:1468-1501) gives for its own leaf centre
-// against the candidate's centre, so a separation rounds exactly as the
-// reference's xpartveci - xyzcache (dens.F90:653-670).
+// build_list then narrows the candidates per lane: the wavefront stages a
+// chunk of candidate leaves and their particles in LDS, each lane marks the
+// leaves its own particle can reach and visits only their particles.  Per candidate
+// leaf every lane computes the periodic offset that get_sep
+// (kdtree.F90:1468-1501) gives for its own leaf centre against the
+// candidate's centre, so a separation rounds exactly as the reference's
+// xpartveci - xyzcache (dens.F90:653-670).
 #pragma once
 
 #include "ctx.h"
@@ -47,74 +45,12 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 }
 static inline unsigned xcd_grid(int ngroups) { return (unsigned)(((ngroups + 7) / 8) * 8); }
 
+
+
 __device__ __forceinline__ unsigned long long lanemask_lt() {
   const unsigned lane = __lane_id();
   return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 }
-
-// Metadata of one batch of NL candidate leaves staged in LDS: first tree
-// position, start of each leaf's particles in the batch, centre and size.
-template <int NL>
-struct BatchMeta {
-  int lo[NL];
-  int start[NL + 1];
-  double cx[NL], cy[NL], cz[NL], sz[NL];
-};
-
-// Load the metadata of candidate leaves gc[b0 .. b0+nb) and return the
-// number of staged particles (uniform).  Every lane of the wave must call it.
-template <int NL>
-__device__ __forceinline__ int batch_meta(BatchMeta<NL> &M, const uint32_t *__restrict__ gc, int b0, int nc,
-                                          const phsph_kdnode *__restrict__ nd, const int32_t *__restrict__ nd_lo,
-                                          const int32_t *__restrict__ nd_hi, int lane, int &nb_out) {
-  const int nb = min(NL, nc - b0);
-  int cnt = 0;
-  if (lane < nb) {
-    const int n = (int)gc[b0 + lane];
-    const int lo = nd_lo[n];
-    cnt = min(nd_hi[n] - lo + 1, PH_MINPART);
-    M.lo[lane] = lo;
-    M.cx[lane] = nd[n].xcen[0];
-    M.cy[lane] = nd[n].xcen[1];
-    M.cz[lane] = nd[n].xcen[2];
-    M.sz[lane] = nd[n].size;
-  }
-  int incl = cnt;
-  for (int o = 1; o < NL; o <<= 1) {
-    const int v = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += v;
-  }
-  if (lane < nb) M.start[lane] = incl - cnt;
-  if (lane == nb - 1) M.start[nb] = incl;
-  nb_out = nb;
-  const int total = __shfl(incl, nb - 1, 64);
-  __syncthreads();
-  return total;
-}
-
-// leaf of staged particle s (M.start is increasing, nb <= NL leaves)
-template <int NL>
-__device__ __forceinline__ int batch_leaf_of(const BatchMeta<NL> &M, int nb, int s) {
-  int lo = 0, hi = nb - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (M.start[mid] <= s) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-// per-lane separation codes of up to 16 staged leaves, 5 bits each
-struct SepCodes {
-  unsigned long long lo = 0ull, hi = 0ull;
-  __device__ __forceinline__ void set(int L, int code) {
-    if (L < 12) lo |= (unsigned long long)code << (5 * L);
-    else hi |= (unsigned long long)code << (5 * (L - 12));
-  }
-  __device__ __forceinline__ int get(int L) const {
-    return (int)(((L < 12) ? (lo >> (5 * L)) : (hi >> (5 * (L - 12)))) & 31ull);
-  }
-};
 
 // Separation code of candidate leaf L for a lane whose own leaf has centre
 // (lcx,lcy,lcz) and size lsz: the get_sep offset, and whether any pair of
@@ -166,3 +102,92 @@ __device__ __forceinline__ void sep_of(const DevConst &dc, int code, double xi, 
 
 // host side (density.hip)
 int walk_groups(phsph_ctx *c, int mode, double rfac, const double *d_hsrc, const double *d_hmaxn);
+
+// Neighbour-list building over a group's candidate leaves, CL leaves at a
+// time through LDS.  For each chunk the wavefront loads the leaves' node
+// data and their particles' records with parallel loads into LDS, each lane
+// marks the leaves of the chunk its particle can reach (reach(L) with the
+// leaf's centre, size and hmax in LDS), and then visits only the staged
+// particles of its own reachable leaves, reading them from LDS:
+// visit(j, code, x, y, z, hinv) with the leaf's separation code for the
+// lane.  One wavefront per block (the block barrier orders the staging).
+constexpr int CL = 16;                 // candidate leaves per chunk
+constexpr int CS = CL * PH_MINPART;    // staged particles per chunk
+struct ListStage {
+  int lo[CL], start[CL + 1];
+  double cx[CL], cy[CL], cz[CL], sz[CL], hm[CL];
+  double x[CS], y[CS], z[CS], hinv[CS];
+};
+
+// xyzh-like records: x, y, z and (HINV) 1/h of particle j
+template <class Rec>
+__device__ __forceinline__ void rec_xyzi(const Rec *__restrict__ r, int j, double &x, double &y, double &z,
+                                         double &w) {
+  const double4 v = *reinterpret_cast<const double4 *>(r + j);
+  x = v.x; y = v.y; z = v.z; w = v.w;
+}
+
+template <class Rec, class Reach, class Visit>
+__device__ __forceinline__ void build_list(ListStage &S, const DevConst &dc, const uint32_t *__restrict__ gc, int nc,
+                                           const phsph_kdnode *__restrict__ nd, const int32_t *__restrict__ nd_lo,
+                                           const int32_t *__restrict__ nd_hi, const double *__restrict__ hmaxn,
+                                           const Rec *__restrict__ rec, double lcx, double lcy, double lcz,
+                                           double lsz, Reach reach, Visit visit) {
+  const int lane = threadIdx.x;
+  for (int k0 = 0; k0 < nc; k0 += CL) {
+    const int kn = min(CL, nc - k0);
+    int cnt = 0;
+    if (lane < kn) {
+      const int n = (int)gc[k0 + lane];
+      const int lo = nd_lo[n];
+      cnt = min(nd_hi[n] - lo + 1, PH_MINPART);
+      S.lo[lane] = lo;
+      S.cx[lane] = nd[n].xcen[0];
+      S.cy[lane] = nd[n].xcen[1];
+      S.cz[lane] = nd[n].xcen[2];
+      S.sz[lane] = nd[n].size;
+      S.hm[lane] = hmaxn ? hmaxn[n] : 0.;
+    }
+    int incl = cnt;
+    for (int o = 1; o < CL; o <<= 1) {
+      const int v = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += v;
+    }
+    if (lane < kn) S.start[lane] = incl - cnt;
+    if (lane == kn - 1) S.start[kn] = incl;
+    const int total = __shfl(incl, kn - 1, 64);
+    __syncthreads();
+    for (int s = lane; s < total; s += 64) {
+      int L = 0;  // leaf of slot s: last L with start[L] <= s
+      for (int b = CL / 2; b > 0; b >>= 1)
+        if (L + b < kn && S.start[L + b] <= s) L += b;
+      rec_xyzi(rec, S.lo[L] + (s - S.start[L]), S.x[s], S.y[s], S.z[s], S.hinv[s]);
+    }
+    __syncthreads();
+    unsigned mask = 0u;
+    for (int L = 0; L < kn; L++)
+      if (reach(S.cx[L], S.cy[L], S.cz[L], S.sz[L], S.hm[L])) mask |= 1u << L;
+    while (mask != 0u) {
+      const int L = __builtin_ctz(mask);
+      mask &= mask - 1u;
+      const int code = leaf_sep_code(dc, S.cx[L], S.cy[L], S.cz[L], S.sz[L], lcx, lcy, lcz, lsz);
+      const int s1 = S.start[L + 1], j0 = S.lo[L] - S.start[L];
+      for (int s = S.start[L]; s < s1; s++) visit(j0 + s, code, S.x[s], S.y[s], S.z[s], S.hinv[s]);
+    }
+    __syncthreads();
+  }
+}
+
+// the reach test of a lane: |x_i - xcen| (minimum image) against the leaf
+// size plus the lane's search radius, with a margin for rounding
+__device__ __forceinline__ bool leaf_in_reach(const DevConst &dc, double xi, double yi, double zi, double ccx,
+                                              double ccy, double ccz, double csz, double rad) {
+  double ex = xi - ccx, ey = yi - ccy, ez = zi - ccz;
+  if (dc.periodic) {
+    ex = min_image(ex, dc.dxbound, dc.hdlx);
+    ey = min_image(ey, dc.dybound, dc.hdly);
+    ez = min_image(ez, dc.dzbound, dc.hdlz);
+  }
+  const double sr = (csz + rad) * (1. + 1e-10);
+  return ex * ex + ey * ey + ez * ez < sr * sr;
+}

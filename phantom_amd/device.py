@@ -76,6 +76,7 @@ class DeviceArray:
         assert other.nbytes == self.nbytes
         _chk(hip().hipMemcpy(C.c_void_p(self.ptr), C.c_void_p(other.ptr), self.nbytes, hipMemcpyDeviceToDevice),
              "hipMemcpy D2D")
+        synchronize()  # a device-to-device hipMemcpy may return before it completes
 
     def numpy(self):
         out = np.empty(self.shape, dtype=self.dtype)
