@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a 256 MiB device copy (reference for the HBM byte counters)")
     return ap.parse_args()
 
 
@@ -50,6 +52,8 @@ def log(msg):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        return calibrate_copy()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design note, "
                                    "not an executable path", "n_gpus": args.gpus}))
@@ -135,22 +139,33 @@ def main():
     updates = n * args.steps
     value = updates / elapsed
 
-    fl_dens = work_model.density_flops(nneigh, nits, mhd=bool(p.mhd))
-    fl_force = work_model.force_flops(npairs, mhd=bool(p.mhd), dust=bool(p.use_dustfrac))
     def kmean(name):
         return float(np.mean(kt[name])) * 1e-3 if name in kt else 0.0
 
     # per step: the kernels of one densityiterate and one force call
     t_dens = sum(float(np.sum(kt.get(nm, []))) for nm in ("density_iter", "density_sums")) * 1e-3 / args.steps
     t_force = sum(float(np.sum(kt.get(nm, []))) for nm in ("force_list", "force")) * 1e-3 / args.steps
+
+    # algorithmic flops per launch of each pair kernel (work_model); the
+    # list builders and walks do search work only and are not credited
+    flops = {
+        "density_iter": work_model.density_iter_flops(nneigh, nits),
+        "density_sums": work_model.density_sums_flops(nneigh, mhd=bool(p.mhd)),
+        "force": work_model.force_flops(npairs, mhd=bool(p.mhd), dust=bool(p.use_dustfrac)),
+        "force_list": 0.0, "density_list": 0.0,
+    }
+    step_kernels = [nm for nm in kt if nm not in ("force_prep", "dens_gather")]
+    dominant = max(step_kernels, key=lambda nm: kmean(nm) * len(kt[nm]))
     peak = work_model.PEAK_FP64_TFLOPS if args.precision == "fp64" else work_model.PEAK_FP32_TFLOPS
-    dens_tf = fl_dens / t_dens / 1e12
-    force_tf = fl_force / t_force / 1e12
-    dominant = "force" if t_force >= t_dens else "density"
-    ach = force_tf if dominant == "force" else dens_tf
-    roofline = {"bound": "mfma", "kernel": dominant, "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(ach / peak, 5), "traffic": None,
-                "note": "VALU-bound pair loop; peak is the MI355X FP64 (or FP32) vector/matrix rate"}
+    t_dom = kmean(dominant)
+    ach = flops.get(dominant, 0.0) / t_dom / 1e12 if t_dom > 0 else 0.0
+    roofline = {"bound": "mfma", "achieved": round(ach, 4), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(ach / peak, 6), "traffic": measured_traffic(dominant, args.workload),
+                "kernel": dominant, "launch_ms": round(1e3 * t_dom, 4),
+                "note": "VALU FP64 pair arithmetic (no dense contraction); peak is MI355X FP64 "
+                        "(FP32 in mixed mode); algorithmic flops per work_model, rejected trials "
+                        "and neighbour search not counted"}
+    kernel_tflops = {nm: round(flops[nm] / kmean(nm) / 1e12, 4) for nm in flops if nm in kt and kmean(nm) > 0}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -180,7 +195,7 @@ def main():
                          "density_kernels": 1e3 * t_dens, "force_kernels": 1e3 * t_force,
                          "kernels": {nm: round(1e3 * kmean(nm), 4) for nm in sorted(kt)},
                          "tree_build_first_call": tree_ms},
-        "kernel_tflops": {"density": round(dens_tf, 3), "force": round(force_tf, 3)},
+        "kernel_tflops": kernel_tflops,
         "neighbours": {"dens_mean": stats["actualmean"], "dens_max": stats["maxactual"],
                        "force_pairs_per_particle": npairs / n, "nrhocalc": stats["nrhocalc"]},
         "tree_kernels_ms": {nm: round(ms, 4) for nm, ms in tree_kernels[:3]},
@@ -189,29 +204,63 @@ def main():
     return 0
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 FETCH_SIZE
+    and WRITE_SIZE runs (profiles/traffic.json), each scaled by what the
+    counter reads for a device copy of known size; None when not measured."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    ent = rec.get(workload, {}).get(kernel)
+    return None if ent is None else ent.get("bytes")
+
+
+def calibrate_copy():
+    """A plain device-to-device copy of 256 MiB, three times: the reference
+    point that scales FETCH_SIZE / WRITE_SIZE to bytes."""
+    from phantom_amd import device
+    nbytes = 256 * 1024 * 1024
+    a = device.DeviceArray((nbytes // 8,), np.float64)
+    b = device.DeviceArray((nbytes // 8,), np.float64)
+    for _ in range(3):
+        b.copy_from(a)
+    device.synchronize()
+    print(json.dumps({"copy_bytes_each_way": nbytes, "copies": 3}), flush=True)
+    return 0
+
+
 def cpu_baseline(setup, args):
-    """The oracle (a C restatement of the reference algorithm) on the host
-    cores of the same machine, same particles, density + cons2prim + force."""
+    """The oracle (a C restatement of the reference algorithm, OpenMP over the
+    host cores) on the same particles: maketree once, then density + cons2prim
+    + force, one warm-up and the median of three calls."""
     from oracle import oracle
 
     threads = args.cpu_threads or min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
     os.environ["OMP_NUM_THREADS"] = str(threads)
     p = setup["params"]
     o = oracle.Oracle(p)
-    xyzh = setup["xyzh"].copy()
-    o.maketree(xyzh)
-    alphaind = setup["alphaind"].copy()
-    t0 = time.perf_counter()
-    d = o.densityiterate(xyzh, setup["vxyzu"], Bevol=setup.get("Bevol"), alphaind=alphaind)
-    c = o.cons2prim(xyzh, setup["vxyzu"], d["dvdx"], alphaind, dustevol=setup.get("dustevol"))
-    o.force(xyzh, setup["vxyzu"], d["gradh"], alphaind, c["eos_vars"], divcurlB=d["divcurlB"],
-            Bevol=setup.get("Bevol"), dustfrac=c["dustfrac"])
-    dt = time.perf_counter() - t0
-    n = xyzh.shape[0]
-    log(f"cpu baseline: {dt:.2f} s on {threads} threads")
+    xyzh0 = setup["xyzh"].copy()
+    o.maketree(xyzh0)
+    n = xyzh0.shape[0]
+    times = []
+    for rep in range(4):
+        xyzh = xyzh0.copy()
+        alphaind = setup["alphaind"].copy()
+        t0 = time.perf_counter()
+        d = o.densityiterate(xyzh, setup["vxyzu"], Bevol=setup.get("Bevol"), alphaind=alphaind)
+        c = o.cons2prim(xyzh, setup["vxyzu"], d["dvdx"], alphaind, dustevol=setup.get("dustevol"))
+        o.force(xyzh, setup["vxyzu"], d["gradh"], alphaind, c["eos_vars"], divcurlB=d["divcurlB"],
+                Bevol=setup.get("Bevol"), dustfrac=c["dustfrac"])
+        if rep > 0:
+            times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
+    log(f"cpu baseline: median {dt:.3f} s per call on {threads} threads")
     return {"value": n / dt, "unit": "particle-updates/s", "cores": threads, "kind": "port",
-            "sample": f"one density+cons2prim+force call over all {n} particles of the same workload "
-                      f"(OpenMP, {threads} threads)"}
+            "sample": f"median of 3 density+cons2prim+force calls (after one warm-up) over all {n} particles "
+                      f"of the same workload, OpenMP with {threads} threads"}
 
 
 if __name__ == "__main__":

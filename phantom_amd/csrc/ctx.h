@@ -92,7 +92,7 @@ struct phsph_ctx {
   DBuf<int32_t> gncand;      // count per group
   int ngroups = 0;
   DBuf<int32_t> ntrial;      // trial particles per leaf (statistics)
-  DBuf<uint32_t> nlist;      // neighbour lists, column-major [k*nact + t] (node | code << 27)
+  DBuf<uint32_t> nlist;      // neighbour lists (list_at layout; node | code << 27)
   DBuf<int32_t> t_nl;        // list length per active slot
   DBuf<double> t_hsum, t_hfin;  // density: h of the last summation, h to store
 

@@ -275,9 +275,10 @@ __global__ void k_scan_i32(int32_t *a, int n, int32_t *total) {
 // density, part 1: neighbour list and Newton-Raphson iteration on h.
 // One wavefront per group of 64 active particles, one particle per lane.
 // Each lane visits the candidate leaves of its group that its particle can
-// reach and keeps the particles within its own radius rfac*radkern*h0 (a superset of
-// every kernel support the iteration can reach) in a list, column-major over
-// the active particles.  The iteration itself evaluates only the sums the
+// reach and keeps the particles within its own radius rfac*radkern*h0 (a
+// superset of every kernel support the iteration can reach) in its list
+// (list_at: 64-byte segments per lane).
+// The iteration itself evaluates only the sums the
 // Newton step needs (rho and dOmega); the full set of get_density_sums is
 // taken once, at the h of the last iteration, by part 2.
 // ------------------------------------------------------------------------
@@ -312,16 +313,14 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
   // --- neighbour list over the leaves this particle can reach ---
   int nl = 0;
   build_list(
-      S, dc, gcand + (size_t)g * capg, gncand[g], nd, nd_lo, nd_hi, (const double *)nullptr, tp, lcx, lcy, lcz, lsz,
+      S, dc, gcand + (size_t)g * capg, gncand[g], nd, nd_lo, nd_hi, (const double *)nullptr, tp, xi, yi, zi, lcx,
+      lcy, lcz, lsz,
       [&](double ccx, double ccy, double ccz, double csz, double) {
         return live && leaf_in_reach(dc, xi, yi, zi, ccx, ccy, ccz, csz, rcut);
       },
-      [&](int j, int code, double xj, double yj, double zj, double) {
-        double dx, dy, dz;
-        sep_of(dc, code, xi, yi, zi, xj, yj, zj, dx, dy, dz);
-        const double r2 = r2_exact(dx, dy, dz);
+      [&](int j, int code, double r2, double) {
         if (j != p && r2 < rc2) {
-          if (nl < capl) nlist[(size_t)nl * nact + t] = cand_pack(j, code);
+          if (nl < capl) nlist[list_at(nact, t, nl)] = cand_pack(j, code);
           nl++;
         }
       });
@@ -341,7 +340,7 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
     const double hi1 = 1. / hi, hi21 = xmul(hi1, hi1);
     double srho = 0., sgh = 0.;
     for (int k = 0; k < nl; k++) {
-      const uint32_t e = nlist[(size_t)k * nact + t];
+      const uint32_t e = nlist[list_at(nact, t, k)];
       const double4 r = tp[cand_node(e)];
       double dx, dy, dz;
       sep_of(dc, cand_code(e), xi, yi, zi, r.x, r.y, r.z, dx, dy, dz);
@@ -407,7 +406,7 @@ k_dens_final(DevConst dc, const int32_t *__restrict__ act, int nact, const uint3
              const int32_t *__restrict__ t_nl, const double *__restrict__ t_hsum, const double *__restrict__ t_hfin,
              const int32_t *__restrict__ retry, const int32_t *__restrict__ iparts, DensIn in,
              double *__restrict__ th_out, DensOut out, int32_t *__restrict__ p_nneigh, int32_t *__restrict__ flags) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= nact) return;
   const int p = act[t];
   if (retry[p]) return;
@@ -433,7 +432,7 @@ k_dens_final(DevConst dc, const int32_t *__restrict__ act, int nact, const uint3
   for (int k = 0; k < 6; k++) rr[k] = 0.;
   int nneigh = 1;
   for (int k = 0; k < nl; k++) {
-    const uint32_t e = nlist[(size_t)k * nact + t];
+    const uint32_t e = nlist[list_at(nact, t, k)];
     const int j = cand_node(e);
     const double4 r = in.tp[j];
     double dx, dy, dz;
@@ -733,7 +732,7 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
     rc = walk_groups(c, 0, rfac, c->th_in.p, nullptr);
     if (rc) return rc;
     const int ng = c->ngroups;
-    PH_HIPCHECK(c, c->nlist.ensure((size_t)c->caps.dlist * na + 1));
+    PH_HIPCHECK(c, c->nlist.ensure((size_t)list_cap(c->caps.dlist) * na + 1));
     hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(n, 256)), dim3(256), 0, s, c->tmpi.p, n, 0);
     int32_t zero = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
@@ -751,8 +750,8 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
     {
       LaunchTimer lt(c, "density_sums");
 #define DFIN(KK, MM)                                                                                           \
-  hipLaunchKernelGGL((k_dens_final<KK, MM>), dim3(nblocks(na, 256)), dim3(256), 0, s, c->dc, c->act.p, na,      \
-                     c->nlist.p, c->t_nl.p, c->t_hsum.p, c->t_hfin.p, c->tmpi.p, c->iparts.p, in, c->th.p, out,   \
+  hipLaunchKernelGGL((k_dens_final<KK, MM>), dim3(xcd_grid(nblocks(na, 256))), dim3(256), 0, s, c->dc, c->act.p,  \
+                     na, c->nlist.p, c->t_nl.p, c->t_hsum.p, c->t_hfin.p, c->tmpi.p, c->iparts.p, in, c->th.p, out,   \
                      c->p_nneigh.p, c->counters.p)
       if (c->p.kernel == PHSPH_KERNEL_CUBIC) {
         if (mhd) DFIN(PHSPH_KERNEL_CUBIC, true); else DFIN(PHSPH_KERNEL_CUBIC, false);

@@ -125,16 +125,13 @@ k_flist(DevConst dc, const int32_t *__restrict__ act, int nact, int ngroups, con
   const double hi21 = xmul(hi1, hi1);
   int nl = 0;
   build_list(
-      S, dc, gcand + (size_t)g * capg, gncand[g], nd, nd_lo, nd_hi, hmaxn, rec, lcx, lcy, lcz, lsz,
+      S, dc, gcand + (size_t)g * capg, gncand[g], nd, nd_lo, nd_hi, hmaxn, rec, xi, yi, zi, lcx, lcy, lcz, lsz,
       [&](double ccx, double ccy, double ccz, double csz, double chm) {
         return live && leaf_in_reach(dc, xi, yi, zi, ccx, ccy, ccz, csz, dc.radkern * fmax(hi, chm));
       },
-      [&](int j, int code, double xj, double yj, double zj, double hj1) {
-        double dx, dy, dz;
-        sep_of(dc, code, xi, yi, zi, xj, yj, zj, dx, dy, dz);
-        const double r2 = r2_exact(dx, dy, dz);
+      [&](int j, int code, double r2, double hj1) {
         if (j != p && (xmul(r2, hi21) < R2 || xmul(r2, xmul(hj1, hj1)) < R2)) {
-          if (nl < capl) nlist[(size_t)nl * nact + t] = cand_pack(j, code);
+          if (nl < capl) nlist[list_at(nact, t, nl)] = cand_pack(j, code);
           nl++;
         }
       });
@@ -152,7 +149,7 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const uint32_t *
         const FRecB *__restrict__ recB, const FRecD *__restrict__ recD, ForceIOArgs out,
         const int8_t *__restrict__ tbin_old, int icall, unsigned long long *__restrict__ dtred,
         int32_t *__restrict__ flags) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const bool live = t < nact;
   const int p = live ? act[t] : act[0];
   const double pm = dc.pmass;
@@ -211,7 +208,7 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const uint32_t *
   const int8_t ibinnow_m1 = (int8_t)(out.ibinnow - 1);
   const int nl = live ? t_nl[t] : 0;
   for (int k = 0; k < nl; k++) {
-    const uint32_t e = nlist[(size_t)k * nact + t];
+    const uint32_t e = nlist[list_at(nact, t, k)];
     const int j = cand_node(e);
     const FRec &Rj = rec[j];
     if (ind) {
@@ -559,7 +556,7 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
   // neighbour lists; a list that does not fit grows the capacity and the
   // lists are built again (the pair pass below runs once)
   for (int attempt = 0;; attempt++) {
-    PH_HIPCHECK(c, c->nlist.ensure((size_t)c->caps.flist * nact + 1));
+    PH_HIPCHECK(c, c->nlist.ensure((size_t)list_cap(c->caps.flist) * nact + 1));
     int32_t zero = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
     if (nact > 0) {
@@ -588,7 +585,7 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
     PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, zero2, sizeof(zero2), hipMemcpyHostToDevice, s));
     if (nact > 0) {
       LaunchTimer lt(c, "force");
-      const dim3 gr(nblocks(nact, 256)), b(256);
+      const dim3 gr(xcd_grid(nblocks(nact, 256))), b(256);
       const bool th = c->p.maxvxyzu >= 4;
 #define FL(KK, M, D, T)                                                                                        \
   hipLaunchKernelGGL((k_force<KK, M, D, T>), gr, b, 0, s, c->dc, c->act.p, nact, c->nlist.p, c->t_nl.p,         \

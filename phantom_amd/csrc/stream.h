@@ -103,14 +103,27 @@ __device__ __forceinline__ void sep_of(const DevConst &dc, int code, double xi, 
 // host side (density.hip)
 int walk_groups(phsph_ctx *c, int mode, double rfac, const double *d_hsrc, const double *d_hmaxn);
 
+// Neighbour lists: entry k of active slot t lives in segment k/16 of that
+// slot, a run of 16 entries (64 bytes) that the slot's lane fills on its own;
+// the segments of one index k/16 are consecutive over t.  A lane thus writes
+// whole 64-byte pieces (no partial-line write-backs), and a wavefront reading
+// entry k of 64 consecutive slots reads 64 adjacent segments.
+constexpr int LSEG = 16;
+__device__ __forceinline__ size_t list_at(int nact, int t, int k) {
+  return ((size_t)(k / LSEG) * (size_t)nact + (size_t)t) * LSEG + (size_t)(k % LSEG);
+}
+static inline int list_cap(int cap) { return (cap + LSEG - 1) / LSEG * LSEG; }
+
 // Neighbour-list building over a group's candidate leaves, CL leaves at a
 // time through LDS.  For each chunk the wavefront loads the leaves' node
 // data and their particles' records with parallel loads into LDS, each lane
 // marks the leaves of the chunk its particle can reach (reach(L) with the
 // leaf's centre, size and hmax in LDS), and then visits only the staged
 // particles of its own reachable leaves, reading them from LDS:
-// visit(j, code, x, y, z, hinv) with the leaf's separation code for the
-// lane.  One wavefront per block (the block barrier orders the staging).
+// visit(j, code, r2, w) with the leaf's separation code for the lane, the
+// squared separation from (xi, yi, zi) rounded as sep_of and r2_exact, and
+// the record's fourth word.  One wavefront per block (the block barrier
+// orders the staging).
 constexpr int CL = 16;                 // candidate leaves per chunk
 constexpr int CS = CL * PH_MINPART;    // staged particles per chunk
 struct ListStage {
@@ -131,8 +144,9 @@ template <class Rec, class Reach, class Visit>
 __device__ __forceinline__ void build_list(ListStage &S, const DevConst &dc, const uint32_t *__restrict__ gc, int nc,
                                            const phsph_kdnode *__restrict__ nd, const int32_t *__restrict__ nd_lo,
                                            const int32_t *__restrict__ nd_hi, const double *__restrict__ hmaxn,
-                                           const Rec *__restrict__ rec, double lcx, double lcy, double lcz,
-                                           double lsz, Reach reach, Visit visit) {
+                                           const Rec *__restrict__ rec, double xi, double yi, double zi,
+                                           double lcx, double lcy, double lcz, double lsz, Reach reach,
+                                           Visit visit) {
   const int lane = threadIdx.x;
   for (int k0 = 0; k0 < nc; k0 += CL) {
     const int kn = min(CL, nc - k0);
@@ -171,8 +185,28 @@ __device__ __forceinline__ void build_list(ListStage &S, const DevConst &dc, con
       const int L = __builtin_ctz(mask);
       mask &= mask - 1u;
       const int code = leaf_sep_code(dc, S.cx[L], S.cy[L], S.cz[L], S.sz[L], lcx, lcy, lcz, lsz);
+      // the separation of sep_of with the leaf's offset taken out of the
+      // loop: x_i - (x_j + o), then the minimum image where it may apply
+      // (x_j + 0 = x_j, so the plain case rounds identically)
+      double ox = 0., oy = 0., oz = 0.;
+      const bool wrap = code != SEP_PLAIN;
+      if (wrap && code != SEP_WRAP0) {
+        int sx, sy, sz;
+        off_decode(code, sx, sy, sz);
+        ox = (sx == 0) ? 0. : copysign(dc.dxbound, (double)sx);
+        oy = (sy == 0) ? 0. : copysign(dc.dybound, (double)sy);
+        oz = (sz == 0) ? 0. : copysign(dc.dzbound, (double)sz);
+      }
       const int s1 = S.start[L + 1], j0 = S.lo[L] - S.start[L];
-      for (int s = S.start[L]; s < s1; s++) visit(j0 + s, code, S.x[s], S.y[s], S.z[s], S.hinv[s]);
+      for (int s = S.start[L]; s < s1; s++) {
+        double dx = xsub(xi, xadd(S.x[s], ox)), dy = xsub(yi, xadd(S.y[s], oy)), dz = xsub(zi, xadd(S.z[s], oz));
+        if (wrap) {
+          dx = min_image(dx, dc.dxbound, 0.5 * dc.dxbound);
+          dy = min_image(dy, dc.dybound, 0.5 * dc.dybound);
+          dz = min_image(dz, dc.dzbound, 0.5 * dc.dzbound);
+        }
+        visit(j0 + s, code, r2_exact(dx, dy, dz), S.hinv[s]);
+      }
     }
     __syncthreads();
   }

@@ -36,6 +36,23 @@ F_FINISH_ITER = 30
 F_FINISH = 150
 
 
+def density_iter_flops(nneigh_dens, nits):
+    """The h-rho iterations: rho and Omega sums over the neighbours at every
+    iteration plus the Newton update."""
+    import numpy as np
+    nn = np.asarray(nneigh_dens, dtype=np.float64) - 1.0
+    it = np.asarray(nits, dtype=np.float64)
+    return float(np.sum(nn * it * F_RHO_ITER + it * F_FINISH_ITER))
+
+
+def density_sums_flops(nneigh_dens, mhd=False):
+    """One pass of every density sum at the final h, and store_results."""
+    import numpy as np
+    nn = np.asarray(nneigh_dens, dtype=np.float64) - 1.0
+    f_pair = F_DENS + (F_DENS_MHD if mhd else 0)
+    return float(np.sum(nn * f_pair + F_FINISH))
+
+
 def density_flops(nneigh_dens, nits, mhd=False):
     """nneigh_dens, nits: per-particle arrays (neighbour count including
     self, iterations to converge)."""
