@@ -100,6 +100,9 @@ struct phsph_ctx {
   DBuf<double4> tp4;                     // density: x, y, z, h at entry
   DBuf<double4> tv4, tf4, tB4;           // density: v, f + fext, Bevol and rho(h)
   DBuf<char> frec, frecB, frecD;         // force: packed j records
+  DBuf<char> frec32;                     // force: FP32 j records (mixed precision)
+  DBuf<double4> fpos;                    // force: x, y, z, 1/h (tree order)
+  DBuf<double4> fvel;                    // force: vx, vy, vz, u (mixed precision)
   DBuf<double> fh;                       // force: h (tree order)
   DBuf<int32_t> act;                     // active tree positions
   int nact = 0;

@@ -395,6 +395,70 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
   atomicMax(&leaf_nits[l], (icall == 0) ? 1 : it);
 }
 
+// store_results (dens.F90:1511-1681) at the final h from the sums taken at
+// the h of the last iteration
+template <bool MHD>
+__device__ __forceinline__ void dens_store(const DevConst &dc, int p, double hfin, double rs_rho, double rs_gh,
+                                           double rs_gs, double rs_divv, double rs_divB, const double *dv,
+                                           const double *da, const double *rr, const double *dB,
+                                           const int32_t *__restrict__ iparts, double *__restrict__ th_out,
+                                           const DensOut &out, int32_t *__restrict__ p_nneigh, int nneigh,
+                                           int32_t *__restrict__ flags) {
+  const double pm = dc.pmass;
+  const double hf1 = 1. / hfin, hf21 = hf1 * hf1, hf31 = hf1 * hf21, hf41 = hf21 * hf21;
+  const double rhoi = dc.cnormk * (rs_rho + dc.wab0 * pm) * hf31;
+  const double gradhs = dc.cnormk * (rs_gh + dc.gradh0 * pm) * hf41;
+  const double dhdrhoi = -hfin / (3. * rhoh_d(hfin, dc.hfact, pm));
+  const double omegai = 1. - dhdrhoi * gradhs;
+  const double gradhi = 1. / omegai;
+  const double gradsofti = ((rs_gs + dc.dphidh0 * pm) * hf21) * dhdrhoi;
+  const double hnew = hrho_d(rhoi, dc.hfact, pm);
+  if (hnew < 0.) atomicOr(flags, DFL_NEGH);
+  const int o = abs(iparts[p]) - 1;
+  out.xyzh[(size_t)o * out.ldx + 3] = hnew;
+  th_out[p] = hnew;
+  const float gh32 = (float)gradhi;
+  out.gradh[(size_t)o * out.ldgradh] = gh32;
+  if (dc.gravity && out.ldgradh > 1) out.gradh[(size_t)o * out.ldgradh + 1] = (float)gradsofti;
+  const double rho1i = 1. / rhoi;
+  const double term = dc.cnormk * (double)gh32 * rho1i * hf41;
+  out.divcurlv[(size_t)o * out.lddcv] = (float)(-rs_divv * term);
+  // exact-linear gradients (dens.F90:866-967, 997-1052, 1086-1101)
+  const double rxx = rr[0], rxy = rr[1], rxz = rr[2], ryy = rr[3], ryz = rr[4], rzz = rr[5];
+  const double denom = rxx * ryy * rzz + 2. * rxy * rxz * ryz - rxx * ryz * ryz - ryy * rxz * rxz - rzz * rxy * rxy;
+  const double rm0 = ryy * rzz - ryz * ryz, rm1 = rxz * ryz - rzz * rxy, rm2 = rxy * ryz - rxz * ryy;
+  const double rm3 = rzz * rxx - rxz * rxz, rm4 = rxy * rxz - rxx * ryz, rm5 = rxx * ryy - rxy * rxy;
+  double gv[9], div_a;
+  if (fabs(denom) > PH_TINY64) {
+    const double dd = 1. / denom;
+    for (int cpt = 0; cpt < 3; cpt++) {
+      const double ax = dv[3 * cpt], ay = dv[3 * cpt + 1], az = dv[3 * cpt + 2];
+      gv[3 * cpt] = -((ax * rm0 + ay * rm1 + az * rm2) * dd);
+      gv[3 * cpt + 1] = -((ax * rm1 + ay * rm3 + az * rm4) * dd);
+      gv[3 * cpt + 2] = -((ax * rm2 + ay * rm4 + az * rm5) * dd);
+    }
+    const double gax = (da[0] * rm0 + da[1] * rm1 + da[2] * rm2) * dd;
+    const double gay = (da[3] * rm1 + da[4] * rm3 + da[5] * rm4) * dd;
+    const double gaz = (da[6] * rm2 + da[7] * rm4 + da[8] * rm5) * dd;
+    div_a = -(gax + gay + gaz);
+  } else {
+    for (int k = 0; k < 9; k++) gv[k] = -dv[k] * term;
+    div_a = -term * (da[0] + da[4] + da[8]);
+  }
+  const double d5 = div_a - (gv[0] * gv[0] + gv[4] * gv[4] + gv[8] * gv[8] +
+                             2. * (gv[1] * gv[3] + gv[2] * gv[6] + gv[5] * gv[7]));
+  if (out.alphaind) out.alphaind[(size_t)o * 3 + 2] = (float)d5;
+  if (out.dvdx)
+    for (int k = 0; k < 9; k++) out.dvdx[(size_t)o * 9 + k] = (float)gv[k];
+  if (MHD && out.divcurlB) {
+    out.divcurlB[(size_t)o * 4 + 0] = (float)(-rs_divB * term);
+    out.divcurlB[(size_t)o * 4 + 1] = (float)(-(dB[7] - dB[5]) * term);
+    out.divcurlB[(size_t)o * 4 + 2] = (float)(-(dB[2] - dB[6]) * term);
+    out.divcurlB[(size_t)o * 4 + 3] = (float)(-(dB[3] - dB[1]) * term);
+  }
+  p_nneigh[p] = nneigh;
+}
+
 // ------------------------------------------------------------------------
 // density, part 2: every sum of get_density_sums (dens.F90:578-857) at the h
 // of the last iteration, over the particle's list, and store_results
@@ -475,59 +539,90 @@ k_dens_final(DevConst dc, const int32_t *__restrict__ act, int nact, const uint3
       dB[6] += dBz * runix; dB[7] += dBz * runiy; dB[8] += dBz * runiz;
     }
   }
-  // --- store_results at h = hfin ---
-  const double hf1 = 1. / hfin, hf21 = hf1 * hf1, hf31 = hf1 * hf21, hf41 = hf21 * hf21;
-  const double rhoi = dc.cnormk * (rs_rho + dc.wab0 * pm) * hf31;
-  const double gradhs = dc.cnormk * (rs_gh + dc.gradh0 * pm) * hf41;
-  const double dhdrhoi = -hfin / (3. * rhoh_d(hfin, dc.hfact, pm));
-  const double omegai = 1. - dhdrhoi * gradhs;
-  const double gradhi = 1. / omegai;
-  const double gradsofti = ((rs_gs + dc.dphidh0 * pm) * hf21) * dhdrhoi;
-  const double hnew = hrho_d(rhoi, dc.hfact, pm);
-  if (hnew < 0.) atomicOr(flags, DFL_NEGH);
-  const int o = abs(iparts[p]) - 1;
-  out.xyzh[(size_t)o * out.ldx + 3] = hnew;
-  th_out[p] = hnew;
-  const float gh32 = (float)gradhi;
-  out.gradh[(size_t)o * out.ldgradh] = gh32;
-  if (dc.gravity && out.ldgradh > 1) out.gradh[(size_t)o * out.ldgradh + 1] = (float)gradsofti;
-  const double rho1i = 1. / rhoi;
-  const double term = dc.cnormk * (double)gh32 * rho1i * hf41;
-  out.divcurlv[(size_t)o * out.lddcv] = (float)(-rs_divv * term);
-  // exact-linear gradients (dens.F90:866-967, 997-1052, 1086-1101)
-  const double rxx = rr[0], rxy = rr[1], rxz = rr[2], ryy = rr[3], ryz = rr[4], rzz = rr[5];
-  const double denom = rxx * ryy * rzz + 2. * rxy * rxz * ryz - rxx * ryz * ryz - ryy * rxz * rxz - rzz * rxy * rxy;
-  const double rm0 = ryy * rzz - ryz * ryz, rm1 = rxz * ryz - rzz * rxy, rm2 = rxy * ryz - rxz * ryy;
-  const double rm3 = rzz * rxx - rxz * rxz, rm4 = rxy * rxz - rxx * ryz, rm5 = rxx * ryy - rxy * rxy;
-  double gv[9], div_a;
-  if (fabs(denom) > PH_TINY64) {
-    const double dd = 1. / denom;
-    for (int cpt = 0; cpt < 3; cpt++) {
-      const double ax = dv[3 * cpt], ay = dv[3 * cpt + 1], az = dv[3 * cpt + 2];
-      gv[3 * cpt] = -((ax * rm0 + ay * rm1 + az * rm2) * dd);
-      gv[3 * cpt + 1] = -((ax * rm1 + ay * rm3 + az * rm4) * dd);
-      gv[3 * cpt + 2] = -((ax * rm2 + ay * rm4 + az * rm5) * dd);
+  dens_store<MHD>(dc, p, hfin, rs_rho, rs_gh, rs_gs, rs_divv, rs_divB, dv, da, rr, dB, iparts, th_out, out,
+                  p_nneigh, nneigh, flags);
+}
+
+// Mixed precision (no MHD): the neighbour test q^2 < R^2 on the FP64
+// separation as in k_dens_final (the same neighbours).  The sums that fix
+// h and rho (kernel, dOmega, softening) stay FP64, with r from an FP32
+// reciprocal square root refined in FP64; the velocity and acceleration
+// differences are taken in FP64 (they are small against the values in
+// smooth flow) and the gradient-tensor products in FP32, summed in FP64.
+template <int K>
+__global__ void __launch_bounds__(256)
+k_dens_final_mixed(DevConst dc, const int32_t *__restrict__ act, int nact, const uint32_t *__restrict__ nlist,
+                   const int32_t *__restrict__ t_nl, const double *__restrict__ t_hsum,
+                   const double *__restrict__ t_hfin, const int32_t *__restrict__ retry,
+                   const int32_t *__restrict__ iparts, const double4 *__restrict__ tp,
+                   const double4 *__restrict__ tv, const double4 *__restrict__ tf, double *__restrict__ th_out,
+                   DensOut out, int32_t *__restrict__ p_nneigh, int32_t *__restrict__ flags) {
+  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= nact) return;
+  const int p = act[t];
+  if (retry[p]) return;
+  const double pm = dc.pmass;
+  const double4 ri = tp[p];
+  const double xi = ri.x, yi = ri.y, zi = ri.z;
+  const double hi = t_hsum[t], hfin = t_hfin[t];
+  const int nl = t_nl[t];
+  const bool have_f = tf != nullptr;
+  const double4 vi = tv[p];
+  double4 fi = make_double4(0., 0., 0., 0.);
+  if (have_f) fi = tf[p];
+  const double hi1 = 1. / hi, hi21 = xmul(hi1, hi1);
+  const float pmf = (float)pm;
+  double rs_rho = 0., rs_gh = 0., rs_gs = 0., rs_divv = 0.;
+  double dv[9], da[9], rr[6];
+  for (int k = 0; k < 9; k++) { dv[k] = 0.; da[k] = 0.; }
+  for (int k = 0; k < 6; k++) rr[k] = 0.;
+  int nneigh = 1;
+  for (int k = 0; k < nl; k++) {
+    const uint32_t e = nlist[list_at(nact, t, k)];
+    const int j = cand_node(e);
+    const double4 r = tp[j];
+    double dx, dy, dz;
+    sep_of(dc, cand_code(e), xi, yi, zi, r.x, r.y, r.z, dx, dy, dz);
+    const double r2 = r2_exact(dx, dy, dz);
+    const double q2 = xmul(r2, hi21);
+    if (!(q2 < dc.radkern2)) continue;
+    // r from an FP32 reciprocal square root refined by one FP64 Newton step
+    double rij1 = 0.;
+    const float r2f = (float)r2;
+    if (r2f > 0.f) {
+      const double y = (double)rsqrtf(r2f);
+      rij1 = y * (1.5 - 0.5 * r2 * y * y);
     }
-    const double gax = (da[0] * rm0 + da[1] * rm1 + da[2] * rm2) * dd;
-    const double gay = (da[3] * rm1 + da[4] * rm3 + da[5] * rm4) * dd;
-    const double gaz = (da[6] * rm2 + da[7] * rm4 + da[8] * rm5) * dd;
-    div_a = -(gax + gay + gaz);
-  } else {
-    for (int k = 0; k < 9; k++) gv[k] = -dv[k] * term;
-    div_a = -term * (da[0] + da[4] + da[8]);
+    const double rij = r2 * rij1;
+    const double q = rij * hi1;
+    double wk, gk;
+    kern_wg<K, double>(q2, q, wk, gk);
+    rs_rho += wk * pm;
+    rs_gh += (-q * gk - 3. * wk) * pm;
+    if (dc.gravity) rs_gs += kern_dphidh<K, double>(q2, q) * pm;
+    nneigh++;
+    const float rg = (float)(gk * rij1);
+    const float runix = (float)dx * rg * pmf, runiy = (float)dy * rg * pmf, runiz = (float)dz * rg * pmf;
+    const double4 vj = tv[j];
+    const float dvx = (float)(vi.x - vj.x), dvy = (float)(vi.y - vj.y), dvz = (float)(vi.z - vj.z);
+    rs_divv += (double)(dvx * runix + dvy * runiy + dvz * runiz);
+    dv[0] += (double)(dvx * runix); dv[1] += (double)(dvx * runiy); dv[2] += (double)(dvx * runiz);
+    dv[3] += (double)(dvy * runix); dv[4] += (double)(dvy * runiy); dv[5] += (double)(dvy * runiz);
+    dv[6] += (double)(dvz * runix); dv[7] += (double)(dvz * runiy); dv[8] += (double)(dvz * runiz);
+    if (have_f) {
+      const double4 fj = tf[j];
+      const float dax = (float)(fi.x - fj.x), day = (float)(fi.y - fj.y), daz = (float)(fi.z - fj.z);
+      da[0] += (double)(dax * runix); da[1] += (double)(dax * runiy); da[2] += (double)(dax * runiz);
+      da[3] += (double)(day * runix); da[4] += (double)(day * runiy); da[5] += (double)(day * runiz);
+      da[6] += (double)(daz * runix); da[7] += (double)(daz * runiy); da[8] += (double)(daz * runiz);
+    }
+    const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
+    rr[0] -= (double)(dxf * runix); rr[1] -= (double)(dxf * runiy); rr[2] -= (double)(dxf * runiz);
+    rr[3] -= (double)(dyf * runiy); rr[4] -= (double)(dyf * runiz); rr[5] -= (double)(dzf * runiz);
   }
-  const double d5 = div_a - (gv[0] * gv[0] + gv[4] * gv[4] + gv[8] * gv[8] +
-                             2. * (gv[1] * gv[3] + gv[2] * gv[6] + gv[5] * gv[7]));
-  if (out.alphaind) out.alphaind[(size_t)o * 3 + 2] = (float)d5;
-  if (out.dvdx)
-    for (int k = 0; k < 9; k++) out.dvdx[(size_t)o * 9 + k] = (float)gv[k];
-  if (MHD && out.divcurlB) {
-    out.divcurlB[(size_t)o * 4 + 0] = (float)(-rs_divB * term);
-    out.divcurlB[(size_t)o * 4 + 1] = (float)(-(dB[7] - dB[5]) * term);
-    out.divcurlB[(size_t)o * 4 + 2] = (float)(-(dB[2] - dB[6]) * term);
-    out.divcurlB[(size_t)o * 4 + 3] = (float)(-(dB[3] - dB[1]) * term);
-  }
-  p_nneigh[p] = nneigh;
+  const double dB[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+  dens_store<false>(dc, p, hfin, rs_rho, rs_gh, rs_gs, rs_divv, 0., dv, da, rr, dB, iparts, th_out, out, p_nneigh,
+                    nneigh, flags);
 }
 
 // actual-neighbour statistics and density evaluations (dens.F90:1670-1679)
@@ -688,6 +783,8 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   if (have_f) PH_HIPCHECK(c, c->tf4.ensure(n));
   if (mhd) PH_HIPCHECK(c, c->tB4.ensure(n));
   PH_HIPCHECK(c, c->tmpi.ensure(n));
+  // mixed precision covers the hydro sums; with MHD the pass stays FP64
+  const bool mixed = c->p.precision == PHSPH_PREC_MIXED && !mhd;
   {
     LaunchTimer lt(c, "dens_gather");
     hipLaunchKernelGGL(k_gather_dens, dim3(nblocks(n, 256)), dim3(256), 0, s, c->dc, n, c->iparts.p, c->tx.p,
@@ -753,11 +850,18 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   hipLaunchKernelGGL((k_dens_final<KK, MM>), dim3(xcd_grid(nblocks(na, 256))), dim3(256), 0, s, c->dc, c->act.p,  \
                      na, c->nlist.p, c->t_nl.p, c->t_hsum.p, c->t_hfin.p, c->tmpi.p, c->iparts.p, in, c->th.p, out,   \
                      c->p_nneigh.p, c->counters.p)
-      if (c->p.kernel == PHSPH_KERNEL_CUBIC) {
+#define DMIX(KK)                                                                                                \
+  hipLaunchKernelGGL((k_dens_final_mixed<KK>), dim3(xcd_grid(nblocks(na, 256))), dim3(256), 0, s, c->dc, c->act.p, \
+                     na, c->nlist.p, c->t_nl.p, c->t_hsum.p, c->t_hfin.p, c->tmpi.p, c->iparts.p, c->tp4.p,         \
+                     c->tv4.p, have_f ? c->tf4.p : nullptr, c->th.p, out, c->p_nneigh.p, c->counters.p)
+      if (mixed) {
+        if (c->p.kernel == PHSPH_KERNEL_CUBIC) DMIX(PHSPH_KERNEL_CUBIC); else DMIX(PHSPH_KERNEL_QUINTIC);
+      } else if (c->p.kernel == PHSPH_KERNEL_CUBIC) {
         if (mhd) DFIN(PHSPH_KERNEL_CUBIC, true); else DFIN(PHSPH_KERNEL_CUBIC, false);
       } else {
         if (mhd) DFIN(PHSPH_KERNEL_QUINTIC, true); else DFIN(PHSPH_KERNEL_QUINTIC, false);
       }
+#undef DMIX
 #undef DFIN
     }
     PH_HIPCHECK(c, hipGetLastError());

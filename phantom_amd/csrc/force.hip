@@ -31,7 +31,8 @@ __global__ void k_force_prep(DevConst dc, int nlive, const int32_t *__restrict__
                              double *__restrict__ th, FRec *__restrict__ rec,
                              FRecB *__restrict__ recB, FRecD *__restrict__ recD,
                              const int8_t *__restrict__ ibin_old, int8_t *__restrict__ tbin_old,
-                             float *__restrict__ gsoft) {
+                             float *__restrict__ gsoft, double4 *__restrict__ fpos, double4 *__restrict__ fvel,
+                             FRec32 *__restrict__ rec32) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nlive) return;
   const int o = abs(iparts[p]) - 1;
@@ -82,6 +83,20 @@ __global__ void k_force_prep(DevConst dc, int nlive, const int32_t *__restrict__
     R.pro2 = prj * rho1j * rho1j;
   }
   rec[p] = R;
+  fpos[p] = make_double4(R.x, R.y, R.z, R.hinv);
+  if (rec32) {
+    FRec32 A;
+    A.rho1 = (float)R.rho1;
+    A.pro2 = (float)R.pro2;
+    A.pro2lo = (float)(R.pro2 - (double)A.pro2);
+    A.vwave = (float)R.vwave;
+    A.alpha = R.alpha; A.gradh = R.gradh; A.rho = (float)R.rho;
+    A.pr = (float)R.pr;
+    A.prlo = (float)(R.pr - (double)A.pr);
+    A.pad0 = A.pad1 = A.pad2 = 0.f;
+    rec32[p] = A;
+    fvel[p] = make_double4(R.vx, R.vy, R.vz, R.u);
+  }
   if (dc.dust) {
     FRecD D;
     const double e = dustfrac[o];
@@ -104,7 +119,7 @@ __global__ void __launch_bounds__(64)
 k_flist(DevConst dc, const int32_t *__restrict__ act, int nact, int ngroups, const int32_t *__restrict__ pleaf,
         const int32_t *__restrict__ leaves, const uint32_t *__restrict__ gcand, const int32_t *__restrict__ gncand,
         int capg, const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi,
-        const phsph_kdnode *__restrict__ nd, const double *__restrict__ hmaxn, const FRec *__restrict__ rec,
+        const phsph_kdnode *__restrict__ nd, const double *__restrict__ hmaxn, const double4 *__restrict__ fpos,
         uint32_t *__restrict__ nlist, int capl, int32_t *__restrict__ t_nl, int32_t *__restrict__ flags) {
   __shared__ ListStage S;
   const int g = xcd_block(blockIdx.x, gridDim.x);
@@ -114,18 +129,20 @@ k_flist(DevConst dc, const int32_t *__restrict__ act, int nact, int ngroups, con
   const bool live = t < nact;
   const double R2 = dc.radkern2;
   int p = -1;
-  double xi = 0., yi = 0., zi = 0., hi = 0., lcx = 0., lcy = 0., lcz = 0., lsz = 0.;
+  double xi = 0., yi = 0., zi = 0., hi = 0., hi1 = 1., lcx = 0., lcy = 0., lcz = 0., lsz = 0.;
   if (live) {
     p = act[t];
     const int c = leaves[pleaf[p]];
-    xi = rec[p].x; yi = rec[p].y; zi = rec[p].z; hi = rec[p].h;
+    const double4 r = fpos[p];
+    xi = r.x; yi = r.y; zi = r.z;
+    hi1 = r.w;       // 1/h as the pair kernel computes it
+    hi = 1. / hi1;   // search radius only (the reach test has a margin)
     lcx = nd[c].xcen[0]; lcy = nd[c].xcen[1]; lcz = nd[c].xcen[2]; lsz = nd[c].size;
   }
-  const double hi1 = 1. / fabs(hi);
   const double hi21 = xmul(hi1, hi1);
   int nl = 0;
   build_list(
-      S, dc, gcand + (size_t)g * capg, gncand[g], nd, nd_lo, nd_hi, hmaxn, rec, xi, yi, zi, lcx, lcy, lcz, lsz,
+      S, dc, gcand + (size_t)g * capg, gncand[g], nd, nd_lo, nd_hi, hmaxn, fpos, xi, yi, zi, lcx, lcy, lcz, lsz,
       [&](double ccx, double ccy, double ccz, double csz, double chm) {
         return live && leaf_in_reach(dc, xi, yi, zi, ccx, ccy, ccz, csz, dc.radkern * fmax(hi, chm));
       },
@@ -420,6 +437,164 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const uint32_t *
   reduce_force_block(dtc_o, dtf_o, nl, bmax, dtred, flags);
 }
 
+// Mixed precision, hydro (grad-h, Cullen-Dehnen viscosity, conductivity).
+// The neighbour tests, separations and q^2 are those of k_force (the same
+// neighbour sets and kernel branches).  The j quantities other than
+// position, velocity and u are stored in FP32 (P/rho^2 and P as FP32 pairs
+// carrying FP64 accuracy), 1/r is an FP32 reciprocal square root refined by
+// one FP64 Newton step, and the conductivity signal speed takes its square
+// root and division in FP32; the rest is FP64, because the pressure and
+// viscous terms cancel to a small net force in smooth flow.  j data: 32-byte
+// FP64 position and velocity records and the 48-byte FRec32 instead of the
+// 128-byte record.
+template <int K, bool THERMAL>
+__global__ void __launch_bounds__(256)
+k_force_mixed(DevConst dc, const int32_t *__restrict__ act, int nact, const uint32_t *__restrict__ nlist,
+              const int32_t *__restrict__ t_nl, const int32_t *__restrict__ iparts, const FRec *__restrict__ rec,
+              const double4 *__restrict__ fpos, const double4 *__restrict__ fvel, const FRec32 *__restrict__ rec32,
+              ForceIOArgs out, const int8_t *__restrict__ tbin_old, int icall,
+              unsigned long long *__restrict__ dtred, int32_t *__restrict__ flags) {
+  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const bool live = t < nact;
+  const int p = live ? act[t] : act[0];
+  const double pm = dc.pmass;
+  const double R2 = dc.radkern2;
+  const FRec Ri = rec[p];
+  const double xi = Ri.x, yi = Ri.y, zi = Ri.z;
+  const double hi = Ri.h;
+  // rhoanddhdrho (part.F90:805-816)
+  const double hi1 = 1. / fabs(hi);
+  const double tt = dc.hfact * hi1;
+  const double rhoi = pm * (tt * tt * tt);
+  const double rho1i = 1. / rhoi;
+  const double hi21 = xmul(hi1, hi1), hi41 = hi21 * hi21;
+  double gradhi = (double)Ri.gradh;
+  if (!(gradhi > 0.)) gradhi = 1.;  // force.F90:2616-2621
+  const double hfacgrkern = hi41 * dc.cnormk * gradhi;
+  const double vwavei = Ri.cs;
+  const double pri = Ri.pr, eni = Ri.u;
+  const double pro2i = pri * rho1i * rho1i;
+  const double vxi = Ri.vx, vyi = Ri.vy, vzi = Ri.vz;
+  const double alphai = (double)Ri.alpha;
+  // FP32 copies of the i terms of the conductivity
+  const float pmf = (float)pm, rhoif = (float)rhoi;
+  const float autermf = (float)(0.5 * pm * rho1i * dc.alphau), alphauf = (float)dc.alphau;
+  double fx = 0., fy = 0., fz = 0., drhodt = 0., dudtdiss = 0., dendtdiss = 0.;
+  double vsigmax = 0.;
+  const bool ind = out.ibin != nullptr;
+  int ibin_neighi = 0;
+  const int8_t ibinnow_m1 = (int8_t)(out.ibinnow - 1);
+  const int nl = live ? t_nl[t] : 0;
+  for (int k = 0; k < nl; k++) {
+    const uint32_t e = nlist[list_at(nact, t, k)];
+    const int j = cand_node(e);
+    const double4 pj = fpos[j];
+    if (ind) {  // force.F90:1346-1352
+      ibin_neighi = max(ibin_neighi, (int)tbin_old[j]);
+      const int oj = abs(iparts[j]) - 1;
+      if (out.ibin_wake[oj] < ibinnow_m1) out.ibin_wake[oj] = ibinnow_m1;
+    }
+    double dx, dy, dz;
+    sep_of(dc, cand_code(e), xi, yi, zi, pj.x, pj.y, pj.z, dx, dy, dz);
+    const double rij2 = r2_exact(dx, dy, dz);
+    const double q2i = xmul(rij2, hi21);
+    const double hj1 = pj.w;
+    const double hj21 = xmul(hj1, hj1);
+    const double q2j = xmul(rij2, hj21);
+    const float r2f = (float)rij2;
+    double rij1 = 0.;
+    if (r2f > 0.f) {
+      const double y = (double)rsqrtf(r2f);
+      rij1 = y * (1.5 - 0.5 * rij2 * y * y);
+    }
+    const double rij = rij2 * rij1;
+    const double grkerni = (q2i < R2) ? kern_g<K, double>(q2i, rij * hi1) * hfacgrkern : 0.;
+    const FRec32 A = rec32[j];
+    double grkernj = 0.;
+    bool usej = THERMAL;
+    if (q2j < R2) {
+      grkernj = kern_g<K, double>(q2j, rij * hj1) * hj21 * hj21 * dc.cnormk * (double)A.gradh;
+      usej = true;
+    }
+    const double runix = dx * rij1, runiy = dy * rij1, runiz = dz * rij1;
+    const double4 vj = fvel[j];
+    const double dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
+    const double projv = dvx * runix + dvy * runiy + dvz * runiz;
+    // signal speeds and artificial viscosity (force.F90:1416-1427, 1581-1596)
+    const double vsigi = fmax(vwavei - dc.beta * projv, 0.);
+    const double vsigavi = fmax(alphai * vwavei - dc.beta * projv, 0.);
+    vsigmax = fmax(vsigmax, vsigi);
+    double rho1j = 0., vsigavj = 0., pro2j = 0.;
+    if (usej) {
+      rho1j = (double)A.rho1;
+      pro2j = (double)A.pro2 + (double)A.pro2lo;
+      const double vwavej = (double)A.vwave;
+      vsigmax = fmax(vsigmax, fmax(vwavej - dc.beta * projv, 0.));
+      vsigavj = fmax((double)A.alpha * vwavej - dc.beta * projv, 0.);
+    }
+    double qrho2i = 0., qrho2j = 0.;
+    if (projv < 0.) {
+      qrho2i = -0.5 * rho1i * vsigavi * projv;
+      if (usej) qrho2j = -0.5 * rho1j * vsigavj * projv;
+    }
+    const double gradp = pm * ((pro2i + qrho2i) * grkerni + (usej ? (pro2j + qrho2j) * grkernj : 0.));
+    fx -= runix * gradp;
+    fy -= runiy * gradp;
+    fz -= runiz * gradp;
+    drhodt += projv * grkerni;
+    if (THERMAL) {
+      // conductivity: the square root and division in FP32 on FP64
+      // differences (force.F90:1606-1624)
+      const double dpr = pri - ((double)A.pr + (double)A.prlo);
+      const float vsigu = sqrtf((float)fabs(dpr) * (2.f / (rhoif + A.rho)));
+      const float denij = (float)(eni - vj.w);
+      dudtdiss += pm * qrho2i * projv * grkerni;
+      dendtdiss += (double)(vsigu * denij * (autermf * (float)grkerni + 0.5f * pmf * (float)rho1j * alphauf * (float)grkernj));
+    }
+  }
+  // --- finish_cell_and_store_results (hydro) ---
+  double dtc_o = PH_HUGE64, dtf_o = PH_HUGE64;
+  int bmax = -1;
+  if (live) {
+    const int o = abs(iparts[p]) - 1;
+    const double f2i = fx * fx + fy * fy + fz * fz;
+    double *fo = out.fxyzu + (size_t)o * out.ldf;
+    fo[0] = fx; fo[1] = fy; fo[2] = fz;
+    const double drhodti = pm * drhodt;
+    const double divvi = -drhodti * rho1i;
+    out.divcurlv[(size_t)o * out.lddcv] = (float)divvi;
+    double dtc = dc.dtmax, dtf = PH_BIGNUMBER;
+    double fxyz4 = 0.;
+    if (THERMAL) {
+      const double pdv_work = pri * rho1i * rho1i * drhodti;
+      if (dc.ipdv_heating > 0) fxyz4 += pdv_work;
+      if (dc.ishock_heating > 0) fxyz4 += dudtdiss;
+      fxyz4 += dendtdiss;
+    }
+    const double vsigdtc = fmax(vsigmax, vwavei);
+    if (vsigdtc > PH_TINY64) dtc = dc.C_cour * hi / (vsigdtc * fmax(dc.alpha, 1.0));
+    if (THERMAL) {
+      if (eni + dtc * fxyz4 < PH_EPS64 && eni > PH_EPS64) fxyz4 = fxyz4 / (1. - dtc * fxyz4 / eni);
+      fo[3] = fxyz4;
+    }
+    if (fabs(f2i) > PH_EPS64) dtf = dc.C_force * sqrt(hi / sqrt(f2i));
+    if (!(dtc > PH_TINY64) || !(dtc < PH_HUGE64)) atomicOr(flags, FFL_DTC);
+    dtc_o = dtc;
+    dtf_o = dtf;
+    if (ind) {  // force.F90:3272-3310
+      double dti = dtc;
+      if (dtf < dti + PH_TINY64 && dtf < dc.dtmax) dti = dtf;
+      int b = get_newbin_d(dti, dc.dtmax, (int)out.ibin[o], icall < 2, out.nbinmax, out.istepfrac);
+      b = max(b, ibin_neighi - 1);
+      out.ibin[o] = (int8_t)b;
+      bmax = b;
+      dtc_o = PH_HUGE64;
+      dtf_o = PH_HUGE64;
+    }
+  }
+  reduce_force_block(dtc_o, dtf_o, nl, bmax, dtred, flags);
+}
+
 __global__ void k_force_flags(DevConst dc, int nlive, const int32_t *__restrict__ iparts,
                               const int8_t *__restrict__ iphase, int32_t *actflag) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -528,12 +703,21 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
   FRecD *recD = dust ? (FRecD *)c->frecD.p : nullptr;
   if (a.ibin) PH_HIPCHECK(c, c->tbin_old.ensure(n));
   if (c->p.gravity) PH_HIPCHECK(c, c->gsoft.ensure(n));
+  // mixed precision covers the hydro pair terms; MHD, dust and self-gravity
+  // keep their FP64 pair kernels
+  const bool mixed = c->p.precision == PHSPH_PREC_MIXED && !mhd && !dust && !c->p.gravity;
+  PH_HIPCHECK(c, c->fpos.ensure(n));
+  if (mixed) {
+    PH_HIPCHECK(c, c->frec32.ensure((size_t)n * sizeof(FRec32)));
+    PH_HIPCHECK(c, c->fvel.ensure(n));
+  }
+  FRec32 *rec32 = mixed ? (FRec32 *)c->frec32.p : nullptr;
   {
     LaunchTimer lt(c, "force_prep");
     hipLaunchKernelGGL(k_force_prep, dim3(nblocks(n, 256)), dim3(256), 0, s, c->dc, n, c->iparts.p, c->tx.p,
                        c->ty.p, c->tz.p, a.xyzh, a.ldx, a.vxyzu, a.ldv, a.gradh, a.ldgradh, a.alphaind, a.eos_vars,
                        a.ldeos, a.Bevol, a.dustfrac, c->fh.p, rec, recB, recD, a.ibin ? a.ibin_old : nullptr,
-                       c->tbin_old.p, c->p.gravity ? c->gsoft.p : nullptr);
+                       c->tbin_old.p, c->p.gravity ? c->gsoft.p : nullptr, c->fpos.p, c->fvel.p, rec32);
   }
   PH_HIPCHECK(c, hipGetLastError());
   PH_HIPCHECK(c, c->tmpi.ensure(n));
@@ -563,7 +747,7 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
       LaunchTimer lt(c, "force_list");
       hipLaunchKernelGGL(k_flist, dim3(xcd_grid(ng)), dim3(64), 0, s, c->dc, c->act.p, nact, ng, c->pleaf.p,
                          c->leaves.p, c->gcand.p, c->gncand.p, c->caps.gcand, c->nd_lo.p, c->nd_hi.p, c->nd.p,
-                         c->hmaxf.p, rec, c->nlist.p, c->caps.flist, c->t_nl.p, c->counters.p);
+                         c->hmaxf.p, c->fpos.p, c->nlist.p, c->caps.flist, c->t_nl.p, c->counters.p);
     }
     PH_HIPCHECK(c, hipGetLastError());
     int32_t fl = 0;
@@ -595,7 +779,18 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
   else if (mhd) { if (th) FL(KK, true, false, true); else FL(KK, true, false, false); }      \
   else if (dust) { if (th) FL(KK, false, true, true); else FL(KK, false, true, false); }     \
   else { if (th) FL(KK, false, false, true); else FL(KK, false, false, false); }
-      if (c->p.kernel == PHSPH_KERNEL_CUBIC) { FLK(PHSPH_KERNEL_CUBIC) } else { FLK(PHSPH_KERNEL_QUINTIC) }
+#define FM(KK, T)                                                                                              \
+  hipLaunchKernelGGL((k_force_mixed<KK, T>), gr, b, 0, s, c->dc, c->act.p, nact, c->nlist.p, c->t_nl.p,          \
+                     c->iparts.p, rec, c->fpos.p, c->fvel.p, rec32, a, c->tbin_old.p, icall, c->dtred.p,          \
+                     c->counters.p)
+      if (mixed) {
+        if (c->p.kernel == PHSPH_KERNEL_CUBIC) {
+          if (th) FM(PHSPH_KERNEL_CUBIC, true); else FM(PHSPH_KERNEL_CUBIC, false);
+        } else {
+          if (th) FM(PHSPH_KERNEL_QUINTIC, true); else FM(PHSPH_KERNEL_QUINTIC, false);
+        }
+      } else if (c->p.kernel == PHSPH_KERNEL_CUBIC) { FLK(PHSPH_KERNEL_CUBIC) } else { FLK(PHSPH_KERNEL_QUINTIC) }
+#undef FM
 #undef FLK
 #undef FL
     }

@@ -14,6 +14,15 @@ struct __align__(16) FRec {
   double cs, h;
 };
 static_assert(sizeof(FRec) == 128, "FRec is one 128-byte line");
+// mixed precision (hydro): the j quantities of the dissipation terms in
+// FP32 and P/rho^2 as an unevaluated FP32 pair; the FP64 position, 1/h,
+// velocity and u of j come from 32-byte position and velocity records
+struct __align__(16) FRec32 {
+  float rho1, pro2, pro2lo, vwave;  // pro2 = (double)pro2 + pro2lo to FP64 accuracy
+  float alpha, gradh, rho, pr;
+  float prlo, pad0, pad1, pad2;     // pr = (double)pr + prlo
+};
+static_assert(sizeof(FRec32) == 48, "FRec32 is 48 bytes");
 // MHD: B = Bevol*rho, psi, the stress -m B(x)B/rho^2, and Bevol itself (i)
 struct __align__(16) FRecB {
   double Bx, By, Bz, psi;
