@@ -89,9 +89,15 @@ def main():
     t0 = time.perf_counter()
     ctx.maketree(xyzh)
     tree_ms = 1e3 * (time.perf_counter() - t0)
-    tree_kernels = ctx.timings()
     xyzh0.copy_from(xyzh)  # wrapped positions, initial h
-    log(f"tree built in {tree_ms:.2f} ms (first call)")
+    # a second build on the same context: buffers exist, as in a time step
+    t0 = time.perf_counter()
+    ctx.maketree(xyzh)
+    tree2_ms = 1e3 * (time.perf_counter() - t0)
+    tree_kernels = {}
+    for nm, ms in ctx.timings():
+        tree_kernels[nm] = tree_kernels.get(nm, 0.0) + ms
+    log(f"tree built in {tree_ms:.2f} ms (first call), {tree2_ms:.2f} ms (second)")
 
     kt = {}  # per-launch kernel times (HIP events on the library's stream), by name
     phase = {"density": 0.0, "cons2prim": 0.0, "force": 0.0}
@@ -194,11 +200,11 @@ def main():
                          "force_call": 1e3 * phase["force"] / args.steps,
                          "density_kernels": 1e3 * t_dens, "force_kernels": 1e3 * t_force,
                          "kernels": {nm: round(1e3 * kmean(nm), 4) for nm in sorted(kt)},
-                         "tree_build_first_call": tree_ms},
+                         "tree_build_first_call": tree_ms, "tree_build": tree2_ms},
         "kernel_tflops": kernel_tflops,
         "neighbours": {"dens_mean": stats["actualmean"], "dens_max": stats["maxactual"],
                        "force_pairs_per_particle": npairs / n, "nrhocalc": stats["nrhocalc"]},
-        "tree_kernels_ms": {nm: round(ms, 4) for nm, ms in tree_kernels[:3]},
+        "tree_kernels_ms": {nm: round(ms, 4) for nm, ms in tree_kernels.items()},
     }
     print(json.dumps(out), flush=True)
     return 0

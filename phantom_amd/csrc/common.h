@@ -39,10 +39,12 @@ struct DevConst {
 // multiply, so it does not prevent this).  Where bit-exact agreement is
 // required (tree centres and sizes, neighbour membership tests) every product
 // passes through an empty asm statement that pins the rounded value in a
-// register, so no later add can be fused with it.
+// register, so no later add can be fused with it.  The statement is not
+// volatile: a volatile one would also keep the compiler from moving loads
+// across it and serialise every load in the loops that use it.
 __device__ __forceinline__ double xmul(double a, double b) {
   double p = a * b;
-  __asm__ volatile("" : "+v"(p));
+  __asm__("" : "+v"(p));
   return p;
 }
 __device__ __forceinline__ double xadd(double a, double b) { return a + b; }

@@ -25,7 +25,7 @@
 #define SUB_MAX 1024  // particles per LDS subtree
 #define SUB_BS 256
 #define SUB_QMAX 192  // >= 2*floor(SUB_MAX/(MINPART+1)) nodes per level
-#define BIG_BS 256
+#define BIG_BS 512
 
 enum { CNT_NEXT = 0, CNT_ERR = 1, CNT_FALLBACK = 2, CNT_MAXLEVEL = 3, CNT_MINLEVEL = 4, CNT_NAN = 5,
        CNT_NLEAF = 6, CNT_LIVE = 7, CNT_N = 8 };
@@ -221,8 +221,40 @@ __device__ __forceinline__ int pick_axis(const double *box) {
 }
 
 // ------------------------------------------------------------------------
-// large nodes: one workgroup per node
+// large nodes: one workgroup of BIG_BS threads per node.  The streaming
+// passes take BIG_U elements per thread per iteration with independent
+// loads, so that a node of a million particles keeps enough loads in
+// flight on its one CU.
 // ------------------------------------------------------------------------
+#define BIG_U 8
+#define BIG_CHUNK 2048
+
+// exclusive scan of per-thread counts over the block (block total in *total)
+template <int BS>
+__device__ int block_excl_scan_int(int v, int *shi, int *total) {
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int incl = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o, 64);
+    if (l >= o) incl += u;
+  }
+  __syncthreads();
+  if (l == 63) shi[w] = incl;
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int k = 0; k < BS / 64; k++) {
+    if (k < w) off += shi[k];
+    tot += shi[k];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + incl - v;
+}
+
+__device__ __forceinline__ const double *axis_arr(int a, const double *tx, const double *ty, const double *tz) {
+  return (a == 0) ? tx : (a == 1) ? ty : tz;
+}
+
 __global__ void __launch_bounds__(BIG_BS) k_big_nodes(DevConst dc, const int32_t *frontier, const double *fbox,
                                                       int nfront, double *tx, double *ty, double *tz, double *th,
                                                       int32_t *iparts, int32_t *nd_lo, int32_t *nd_hi,
@@ -230,9 +262,9 @@ __global__ void __launch_bounds__(BIG_BS) k_big_nodes(DevConst dc, const int32_t
                                                       int32_t *tmpB, int level, int maxlevel_indexed,
                                                       int64_t nslots, int32_t *next_front, double *next_box,
                                                       int32_t *counters) {
-  __shared__ double sh[8];
-  __shared__ int shi[8];
-  __shared__ double chunk[3][BIG_BS * 8];
+  __shared__ double sh[BIG_BS / 64];
+  __shared__ int shi[BIG_BS / 64];
+  __shared__ double chunk[4][BIG_CHUNK];
   const int f = blockIdx.x;
   const int nnode = frontier[f];
   const int lo = nd_lo[nnode], hi = nd_hi[nnode];
@@ -244,14 +276,21 @@ __global__ void __launch_bounds__(BIG_BS) k_big_nodes(DevConst dc, const int32_t
   const double fac = xmul(pm, dfac);
   double box[6];
   for (int c = 0; c < 6; c++) box[c] = fbox[(size_t)f * 6 + c];
+  constexpr int STEP = BIG_BS * BIG_U;
 
   // parallel sums + bound
   double s[3] = {0., 0., 0.}, a[3] = {0., 0., 0.}, hmax = 0.;
-  for (int p = lo + tid; p <= hi; p += BIG_BS) {
-    double vx = xmul(fac, tx[p]), vy = xmul(fac, ty[p]), vz = xmul(fac, tz[p]);
-    s[0] += vx; s[1] += vy; s[2] += vz;
-    a[0] += fabs(vx); a[1] += fabs(vy); a[2] += fabs(vz);
-    hmax = fmax(hmax, th[p]);
+  for (int base = lo; base <= hi; base += STEP) {
+#pragma unroll
+    for (int u = 0; u < BIG_U; u++) {
+      const int p = base + u * BIG_BS + tid;
+      if (p <= hi) {
+        const double vx = xmul(fac, tx[p]), vy = xmul(fac, ty[p]), vz = xmul(fac, tz[p]);
+        s[0] += vx; s[1] += vy; s[2] += vz;
+        a[0] += fabs(vx); a[1] += fabs(vy); a[2] += fabs(vz);
+        hmax = fmax(hmax, th[p]);
+      }
+    }
   }
   for (int c = 0; c < 3; c++) {
     s[c] = block_sum<BIG_BS>(s[c], sh);
@@ -262,54 +301,81 @@ __global__ void __launch_bounds__(BIG_BS) k_big_nodes(DevConst dc, const int32_t
   double D = xmul(tot, dfac);
   double com[3];
   for (int c = 0; c < 3; c++) com[c] = s[c] / D;
-  const double u = 1.1102230246251565e-16;
-  const double nu = (double)(n - 1) * u;
+  const double uu = 1.1102230246251565e-16;
+  const double nu = (double)(n - 1) * uu;
   const double gam = nu / (1. - nu);
   const int iaxis = pick_axis(box);
-  double delta = 2. * (2. * gam * a[iaxis] / D + (2. * gam + 4. * u) * fabs(com[iaxis])) + 1e-300;
+  const double *tax = axis_arr(iaxis, tx, ty, tz);
+  const double a_ax = (iaxis == 0) ? a[0] : (iaxis == 1) ? a[1] : a[2];
+  const double c_ax = (iaxis == 0) ? com[0] : (iaxis == 1) ? com[1] : com[2];
+  double delta = 2. * (2. * gam * a_ax / D + (2. * gam + 4. * uu) * fabs(c_ax)) + 1e-300;
   // any particle within delta of the plane?
   int amb = 0;
-  for (int p = lo + tid; p <= hi; p += BIG_BS) {
-    double x = (iaxis == 0) ? tx[p] : (iaxis == 1) ? ty[p] : tz[p];
-    amb |= (fabs(x - com[iaxis]) <= delta);
+  for (int base = lo; base <= hi; base += STEP) {
+#pragma unroll
+    for (int u = 0; u < BIG_U; u++) {
+      const int p = base + u * BIG_BS + tid;
+      if (p <= hi) amb |= (fabs(tax[p] - c_ax) <= delta);
+    }
   }
   int namb;
   block_excl_scan<BIG_BS>(amb, shi, &namb);
   if (namb > 0) {
-    // exact sequential sums (kdtree.F90:655-665), LDS-chunked, one lane adds
+    // exact sequential sums (kdtree.F90:655-665): the products fac*x are
+    // independent and are formed by all threads into LDS; the four running
+    // sums (x, y, z, mass) are four dependent chains, run by lanes 0-3 of
+    // the first wavefront in lockstep, one addition per particle each
     __shared__ double seq[4];
-    double sx = 0., sy = 0., sz = 0., tm = 0.;
-    for (int base = lo; base <= hi; base += BIG_BS * 8) {
-      int cnt = min(BIG_BS * 8, hi - base + 1);
+    double acc = 0.;
+    for (int base = lo; base <= hi; base += BIG_CHUNK) {
+      const int cnt = min(BIG_CHUNK, hi - base + 1);
       __syncthreads();
       for (int k = tid; k < cnt; k += BIG_BS) {
-        chunk[0][k] = tx[base + k]; chunk[1][k] = ty[base + k]; chunk[2][k] = tz[base + k];
+        chunk[0][k] = xmul(fac, tx[base + k]);
+        chunk[1][k] = xmul(fac, ty[base + k]);
+        chunk[2][k] = xmul(fac, tz[base + k]);
+        chunk[3][k] = pm;
       }
       __syncthreads();
-      if (tid == 0) {
-        for (int k = 0; k < cnt; k++) {
-          tm = xadd(tm, pm);
-          sx = xadd(sx, xmul(fac, chunk[0][k]));
-          sy = xadd(sy, xmul(fac, chunk[1][k]));
-          sz = xadd(sz, xmul(fac, chunk[2][k]));
+      if (tid < 4) {
+        // row tid: x, y, z products or the masses; LDS reads issued ahead
+        // of the dependent additions, no branch per element
+        const double *src = chunk[tid];
+        int k = 0;
+        for (; k + 8 <= cnt; k += 8) {
+          double v[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) v[u] = src[k + u];
+#pragma unroll
+          for (int u = 0; u < 8; u++) acc = xadd(acc, v[u]);
         }
+        for (; k < cnt; k++) acc = xadd(acc, src[k]);
       }
     }
-    if (tid == 0) { seq[0] = sx; seq[1] = sy; seq[2] = sz; seq[3] = tm; }
+    if (tid < 4) seq[tid] = acc;
     __syncthreads();
     tot = seq[3];
     D = xmul(tot, dfac);
     for (int c = 0; c < 3; c++) com[c] = seq[c] / D;
     if (tid == 0) atomicAdd(&counters[CNT_FALLBACK], 1);
   }
-  // size: sqrt(max r^2) + epsilon (kdtree.F90:697-782)
+  // size: sqrt(max r^2) + epsilon (kdtree.F90:697-782), and the count of
+  // particles on the left of the pivot
+  const double pivot = (iaxis == 0) ? com[0] : (iaxis == 1) ? com[1] : com[2];
   double r2max = 0.;
-  for (int p = lo + tid; p <= hi; p += BIG_BS) {
-    double r2 = r2_exact(xsub(tx[p], com[0]), xsub(ty[p], com[1]), xsub(tz[p], com[2]));
-    r2max = fmax(r2max, r2);
+  int cl = 0;
+  for (int base = lo; base <= hi; base += STEP) {
+#pragma unroll
+    for (int u = 0; u < BIG_U; u++) {
+      const int p = base + u * BIG_BS + tid;
+      if (p <= hi) {
+        const double r2 = r2_exact(xsub(tx[p], com[0]), xsub(ty[p], com[1]), xsub(tz[p], com[2]));
+        r2max = fmax(r2max, r2);
+        cl += (tax[p] <= pivot);
+      }
+    }
   }
   r2max = block_max<BIG_BS>(r2max, sh);
-  const double pivot = com[iaxis];
   const int il = 2 * nnode, ir = il + 1;
   if (tid == 0) {
     phsph_kdnode e;
@@ -326,49 +392,54 @@ __global__ void __launch_bounds__(BIG_BS) k_big_nodes(DevConst dc, const int32_t
     if (level >= maxlevel_indexed || (int64_t)ir >= nslots) atomicOr(&counters[CNT_ERR], TERR_NODEID);
   }
   if (level >= maxlevel_indexed || (int64_t)ir >= nslots) return;
-  // partition (sort_particles_in_cell): count x <= pivot
-  int cl = 0;
-  for (int p = lo + tid; p <= hi; p += BIG_BS) {
-    double x = (iaxis == 0) ? tx[p] : (iaxis == 1) ? ty[p] : tz[p];
-    cl += (x <= pivot);
-  }
-  double cld = block_sum<BIG_BS>((double)cl, sh);
-  int nl = (int)cld;
+  // partition (sort_particles_in_cell): the k-th element > pivot in the
+  // left part swaps with the k-th element <= pivot counted from the right
+  int nl;
+  block_excl_scan_int<BIG_BS>(cl, shi, &nl);
   if (nl > 0 && nl < n) {
     const int lend = lo + nl - 1;  // last position of the left part
     int carry = 0;
-    for (int base = lo; base <= lend; base += BIG_BS) {
-      int p = base + tid;
-      int b = 0;
-      if (p <= lend) {
-        double x = (iaxis == 0) ? tx[p] : (iaxis == 1) ? ty[p] : tz[p];
-        b = !(x <= pivot);
+    for (int base = lo; base <= lend; base += STEP) {
+      // thread tid takes BIG_U consecutive positions, so ranks follow position
+      const int p0 = base + tid * BIG_U;
+      int b[BIG_U], c = 0;
+#pragma unroll
+      for (int u = 0; u < BIG_U; u++) {
+        const int p = p0 + u;
+        b[u] = (p <= lend) ? !(tax[p] <= pivot) : 0;
+        c += b[u];
       }
       int t;
-      int r = block_excl_scan<BIG_BS>(b, shi, &t);
-      if (b) tmpA[lo + carry + r] = p;
+      int r = block_excl_scan_int<BIG_BS>(c, shi, &t);
+#pragma unroll
+      for (int u = 0; u < BIG_U; u++)
+        if (b[u]) tmpA[lo + carry + r++] = p0 + u;
       carry += t;
     }
     const int nbad = carry;
     carry = 0;
-    for (int base = 0; base < hi - lend; base += BIG_BS) {
-      int p = hi - (base + tid);
-      int b = 0;
-      if (p > lend) {
-        double x = (iaxis == 0) ? tx[p] : (iaxis == 1) ? ty[p] : tz[p];
-        b = (x <= pivot);
+    for (int base = 0; base < hi - lend; base += STEP) {
+      const int q0 = base + tid * BIG_U;  // distance from the right end
+      int b[BIG_U], c = 0;
+#pragma unroll
+      for (int u = 0; u < BIG_U; u++) {
+        const int p = hi - (q0 + u);
+        b[u] = (p > lend) ? (tax[p] <= pivot) : 0;
+        c += b[u];
       }
       int t;
-      int r = block_excl_scan<BIG_BS>(b, shi, &t);
-      if (b) tmpB[lo + carry + r] = p;
+      int r = block_excl_scan_int<BIG_BS>(c, shi, &t);
+#pragma unroll
+      for (int u = 0; u < BIG_U; u++)
+        if (b[u]) tmpB[lo + carry + r++] = hi - (q0 + u);
       carry += t;
     }
     if (carry != nbad && tid == 0) atomicOr(&counters[CNT_ERR], TERR_COUNT);
     __syncthreads();
     for (int k = tid; k < nbad; k += BIG_BS) {
-      int pa = tmpA[lo + k], pb = tmpB[lo + k];
-      double t0 = tx[pa], t1 = ty[pa], t2 = tz[pa], t3 = th[pa];
-      int32_t ti = iparts[pa];
+      const int pa = tmpA[lo + k], pb = tmpB[lo + k];
+      const double t0 = tx[pa], t1 = ty[pa], t2 = tz[pa], t3 = th[pa];
+      const int32_t ti = iparts[pa];
       tx[pa] = tx[pb]; ty[pa] = ty[pb]; tz[pa] = tz[pb]; th[pa] = th[pb]; iparts[pa] = iparts[pb];
       tx[pb] = t0; ty[pb] = t1; tz[pb] = t2; th[pb] = t3; iparts[pb] = ti;
     }
@@ -379,11 +450,18 @@ __global__ void __launch_bounds__(BIG_BS) k_big_nodes(DevConst dc, const int32_t
   const int lA = lo, hA = lo + nl - 1, lB = lo + nl, hB = hi;
   double bmn[2][3], bmx[2][3];
   for (int side = 0; side < 2; side++) {
-    int l0 = side ? lB : lA, h0 = side ? hB : hA;
+    const int l0 = side ? lB : lA, h0 = side ? hB : hA;
     double m0[3] = {PH_HUGE64, PH_HUGE64, PH_HUGE64}, m1[3] = {-PH_HUGE64, -PH_HUGE64, -PH_HUGE64};
-    for (int p = l0 + tid; p <= h0; p += BIG_BS) {
-      m0[0] = fmin(m0[0], tx[p]); m0[1] = fmin(m0[1], ty[p]); m0[2] = fmin(m0[2], tz[p]);
-      m1[0] = fmax(m1[0], tx[p]); m1[1] = fmax(m1[1], ty[p]); m1[2] = fmax(m1[2], tz[p]);
+    for (int base = l0; base <= h0; base += STEP) {
+#pragma unroll
+      for (int u = 0; u < BIG_U; u++) {
+        const int p = base + u * BIG_BS + tid;
+        if (p <= h0) {
+          const double x = tx[p], y = ty[p], z = tz[p];
+          m0[0] = fmin(m0[0], x); m0[1] = fmin(m0[1], y); m0[2] = fmin(m0[2], z);
+          m1[0] = fmax(m1[0], x); m1[1] = fmax(m1[1], y); m1[2] = fmax(m1[2], z);
+        }
+      }
     }
     for (int c = 0; c < 3; c++) {
       bmn[side][c] = block_min<BIG_BS>(m0[c], sh);
