@@ -265,6 +265,113 @@ int phsph_set_active_particles(phsph_ctx *ctx, int32_t np, const double *xyzh,
                                int32_t istepfrac, int32_t *nactive,
                                int32_t *nalive, int32_t *ibinnow);
 
+/*
+ * Leapfrog integrator.  The reference's step (step_leapfrog.f90:91-760) is
+ * a sequence of particle loops around derivs; each loop is one entry point
+ * here, so that a caller keeps the reference's control flow (derivs, the
+ * velocity-error check and its iterations) and moves only the loops to the
+ * device.  Arrays follow the other entry points: Fortran a(ld, np), host or
+ * device pointers, NULL for physics that is off.  Only gas particles are
+ * modelled: iphase (NULL = all active) marks active (>0) and inactive (<0)
+ * gas, and a particle is dead when h < tiny(h) (part.F90:931-941).  Bevol,
+ * dBevol and Bpred have 4 components (MHD), dustevol, ddustevol and dustpred
+ * 1 (one-fluid dust); vpred has the layout of vxyzu.  Counts returned
+ * through pointers may be NULL.  dtmax is params.dtmax.
+ */
+
+/*
+ * init_step (step_leapfrog.f90:44-83) with block timesteps: when
+ * time < tiny(time) every ibin is first set to nbinmax; then
+ * twas(i) = time + 0.5*dtmax/2**ibin(i).  An error without
+ * params.ind_timesteps.
+ */
+int phsph_init_step(phsph_ctx *ctx, int32_t np, double time, int8_t *ibin,
+                    int32_t nbinmax, double *twas);
+
+/*
+ * The predictor loop of step (step_leapfrog.f90:150-237): for each live
+ * particle v,u += hdti*f with hdti = dtsph/2 (global timestep) or
+ * twas(i) - t (block timesteps, where active particles also set
+ * ibin_old(i) = ibin(i)); u is floored to ufloor when ufloor > 0;
+ * Bevol += hdti*dBevol; dustevol = |dustevol + hdti*ddustevol|.
+ * ibin, ibin_old and twas are read only with block timesteps.
+ */
+int phsph_step_predictor(phsph_ctx *ctx, int32_t np, double t, double dtsph,
+                         const double *xyzh, int32_t ldxyzh,
+                         double *vxyzu, int32_t ldvxyzu,
+                         const double *fxyzu, int32_t ldfxyzu,
+                         double *Bevol, const double *dBevol,
+                         double *dustevol, const double *ddustevol,
+                         const int8_t *iphase, const int8_t *ibin,
+                         int8_t *ibin_old, const double *twas, double ufloor,
+                         int32_t *nvfloor);
+
+/* substep_sph (substepping.F90:241-264): x += dt*v for live particles, the
+ * drift of step when there are no external forces or sinks
+ * (step_leapfrog.f90:279). */
+int phsph_substep_sph(phsph_ctx *ctx, int32_t np, double dt, double *xyzh,
+                      int32_t ldxyzh, const double *vxyzu, int32_t ldvxyzu);
+
+/*
+ * The predict_sph loop of step (step_leapfrog.f90:307-400) at
+ * timei = t + dtsph: h -= dtsph*dhdrho*rho*div v; vpred = v + hdti*f with
+ * hdti = dtsph/2 or timei - twas(i); vpred(4) floored to ufloor;
+ * Bpred = Bevol + hdti*dBevol; dustpred = dustevol + hdti*ddustevol; and the
+ * Cullen-Dehnen switch alphaind(1) = alpha_loc if alpha_loc is larger, else
+ * relaxed towards it with decay time h/(0.1 c_s) (implicit step).
+ * divcurlv(1,:) is div v, eos_vars(2,:) the sound speed, alphaind(2,:)
+ * alpha_loc.
+ */
+int phsph_step_predict_sph(phsph_ctx *ctx, int32_t np, double timei,
+                           double dtsph, double *xyzh, int32_t ldxyzh,
+                           const double *vxyzu, int32_t ldvxyzu,
+                           const double *fxyzu, int32_t ldfxyzu,
+                           double *vpred, const double *Bevol,
+                           const double *dBevol, double *Bpred,
+                           const double *dustevol, const double *ddustevol,
+                           double *dustpred, const float *divcurlv,
+                           int32_t lddivcurlv, const double *eos_vars,
+                           int32_t ldeos, float *alphaind,
+                           const int8_t *iphase, const double *twas,
+                           double ufloor, int32_t *nvfloor);
+
+/*
+ * One pass of the corrector loop of step (step_leapfrog.f90:482-625) at
+ * timei = t + dtsph.  Global timestep: v,u += dtsph/2*f, Bevol and dustevol
+ * likewise, and the velocity-error sums the reference passes to
+ * check_velocity_error: errmax = max |v - vpred|^2, v2sum = sum |v|^2 and
+ * npc, the number of particles summed.  Block timesteps: active particles
+ * complete their step (dti = timei - twas + dt(ibin)/2, twas += dti,
+ * ibin_wake = 0), every particle is synchronised to timei, u is floored, and
+ * particles flagged for waking move to bin min(ibin_wake, nbinmax) with twas
+ * at the half step of that bin (nwake counts them); errmax, v2sum and npc are
+ * then 0.  ibin, ibin_wake and twas are used only with block timesteps.
+ */
+int phsph_step_corrector(phsph_ctx *ctx, int32_t np, double timei,
+                         double dtsph, const double *xyzh, int32_t ldxyzh,
+                         double *vxyzu, int32_t ldvxyzu,
+                         const double *fxyzu, int32_t ldfxyzu,
+                         const double *vpred, double *Bevol,
+                         const double *dBevol, double *dustevol,
+                         const double *ddustevol, const int8_t *iphase,
+                         int8_t *ibin, int8_t *ibin_wake, int32_t nbinmax,
+                         double *twas, double ufloor, double *errmax,
+                         double *v2sum, int32_t *npc, int32_t *nwake,
+                         int32_t *nvfloor);
+
+/*
+ * The until_converged loop of step (step_leapfrog.f90:650-696), run when
+ * the velocity error asks for another iteration: vpred = v, Bpred = Bevol,
+ * dustpred = dustevol (block timesteps: active particles only) and, with a
+ * global timestep, v, Bevol, dustevol -= dtsph/2 * their derivatives.  As
+ * in the reference this loop also runs over dead particles.
+ */
+int phsph_step_reset(phsph_ctx *ctx, int32_t np, double dtsph,
+                     double *vxyzu, int32_t ldvxyzu, const double *fxyzu, int32_t ldfxyzu,
+                     double *vpred, double *Bevol, const double *dBevol,
+                     double *Bpred, double *dustevol, const double *ddustevol,
+                     double *dustpred, const int8_t *iphase);
+
 /* get_neighbour_stats (dens.F90:1137-1155) of the last densityiterate. */
 int phsph_get_neighbour_stats(phsph_ctx *ctx, double *trialmean,
                               double *actualmean, int32_t *maxtrial,

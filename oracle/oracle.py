@@ -64,6 +64,16 @@ def lib():
         L.orc_set_poten.restype = None
         L.orc_get_quads.argtypes = [vp, vp, C.c_int64]
         L.orc_kernel_softening.argtypes = [C.c_int32, C.c_double, P(C.c_double), P(C.c_double)]
+        i32, dbl = C.c_int32, C.c_double
+        L.orc_init_step.argtypes = [vp, i32, dbl, vp, i32, vp]
+        L.orc_step_predictor.argtypes = [vp, i32, dbl, dbl, vp, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp,
+                                         dbl, P(i32)]
+        L.orc_substep_sph.argtypes = [vp, i32, dbl, vp, vp, i32]
+        L.orc_step_predict_sph.argtypes = [vp, i32, dbl, dbl, vp, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp,
+                                           i32, vp, i32, vp, vp, vp, dbl, P(i32)]
+        L.orc_step_corrector.argtypes = [vp, i32, dbl, dbl, vp, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp,
+                                         i32, vp, dbl, P(dbl), P(dbl), P(i32), P(i32), P(i32)]
+        L.orc_step_reset.argtypes = [vp, i32, dbl, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -221,6 +231,56 @@ class Oracle:
         lib().orc_set_poten(self._ctx, None)
         return dict(fxyzu=fxyzu, divcurlv=divcurlv, dBevol=dBevol, divBsymm=divBsymm, ddustevol=ddustevol,
                     deltav=deltav, tstop=tstop, dtcourant=dtc.value, dtforce=dtf.value, poten=poten)
+
+    # -- leapfrog integrator loops, same calls as phantom_amd.sph.Context -----
+    @staticmethod
+    def _ld(a):
+        return a.shape[1] if a.ndim == 2 else 1
+
+    def init_step(self, time, ibin, nbinmax, twas):
+        self._check(lib().orc_init_step(self._ctx, ibin.shape[0], time, _ptr(ibin), nbinmax, _ptr(twas)))
+
+    def step_predictor(self, t, dtsph, xyzh, vxyzu, fxyzu, Bevol=None, dBevol=None, dustevol=None,
+                       ddustevol=None, iphase=None, ibin=None, ibin_old=None, twas=None, ufloor=0.0):
+        nfl = C.c_int32()
+        self._check(lib().orc_step_predictor(
+            self._ctx, xyzh.shape[0], t, dtsph, _ptr(xyzh), _ptr(vxyzu), self._ld(vxyzu), _ptr(fxyzu),
+            self._ld(fxyzu), _ptr(Bevol), _ptr(dBevol), _ptr(dustevol), _ptr(ddustevol), _ptr(iphase), _ptr(ibin),
+            _ptr(ibin_old), _ptr(twas), ufloor, C.byref(nfl)))
+        return nfl.value
+
+    def substep_sph(self, dt, xyzh, vxyzu):
+        self._check(lib().orc_substep_sph(self._ctx, xyzh.shape[0], dt, _ptr(xyzh), _ptr(vxyzu), self._ld(vxyzu)))
+
+    def step_predict_sph(self, timei, dtsph, xyzh, vxyzu, fxyzu, vpred, divcurlv, eos_vars, alphaind, Bevol=None,
+                         dBevol=None, Bpred=None, dustevol=None, ddustevol=None, dustpred=None, iphase=None,
+                         twas=None, ufloor=0.0):
+        nfl = C.c_int32()
+        self._check(lib().orc_step_predict_sph(
+            self._ctx, xyzh.shape[0], timei, dtsph, _ptr(xyzh), _ptr(vxyzu), self._ld(vxyzu), _ptr(fxyzu),
+            self._ld(fxyzu), _ptr(vpred), _ptr(Bevol), _ptr(dBevol), _ptr(Bpred), _ptr(dustevol), _ptr(ddustevol),
+            _ptr(dustpred), _ptr(divcurlv), self._ld(divcurlv), _ptr(eos_vars), self._ld(eos_vars), _ptr(alphaind),
+            _ptr(iphase), _ptr(twas), ufloor, C.byref(nfl)))
+        return nfl.value
+
+    def step_corrector(self, timei, dtsph, xyzh, vxyzu, fxyzu, vpred=None, Bevol=None, dBevol=None,
+                       dustevol=None, ddustevol=None, iphase=None, ibin=None, ibin_wake=None, nbinmax=0,
+                       twas=None, ufloor=0.0):
+        em, v2 = C.c_double(), C.c_double()
+        npc, nw, nfl = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(lib().orc_step_corrector(
+            self._ctx, xyzh.shape[0], timei, dtsph, _ptr(xyzh), _ptr(vxyzu), self._ld(vxyzu), _ptr(fxyzu),
+            self._ld(fxyzu), _ptr(vpred), _ptr(Bevol), _ptr(dBevol), _ptr(dustevol), _ptr(ddustevol), _ptr(iphase),
+            _ptr(ibin), _ptr(ibin_wake), nbinmax, _ptr(twas), ufloor, C.byref(em), C.byref(v2), C.byref(npc),
+            C.byref(nw), C.byref(nfl)))
+        return dict(errmax=em.value, v2sum=v2.value, np=npc.value, nwake=nw.value, nvfloor=nfl.value)
+
+    def step_reset(self, dtsph, vxyzu, fxyzu, vpred, Bevol=None, dBevol=None, Bpred=None, dustevol=None,
+                   ddustevol=None, dustpred=None, iphase=None):
+        self._check(lib().orc_step_reset(
+            self._ctx, vxyzu.shape[0], dtsph, _ptr(vxyzu), self._ld(vxyzu), _ptr(fxyzu), self._ld(fxyzu),
+            _ptr(vpred), _ptr(Bevol), _ptr(dBevol), _ptr(Bpred), _ptr(dustevol), _ptr(ddustevol), _ptr(dustpred),
+            _ptr(iphase)))
 
     def get_quads(self):
         """Second moments quads(6) of every node (node ids 1..ncells)."""

@@ -1559,14 +1559,14 @@ int orc_force(orc_ctx *o, int32_t icall, int32_t np, const double *xyzh, const d
   const int thermal = o->p.maxvxyzu >= 4;
   forcein_t in = {xyzh, vxyzu, Bevol, dustfrac, eos_vars, gradh, alphaind, divcurlB, ldv, ldgradh, ldeos};
   double dtcourant = BIGNUMBER, dtforce = BIGNUMBER;
-  int nbinmaxnew = 0;
+  int nbinmaxnew = 0, ncheckbin = 0;
   const int ind = o->p.ind_timesteps != 0;
   if (ind && (!o->ibin || !o->ibin_old || !o->ibin_wake)) {
     snprintf(o->err, sizeof(o->err), "force: individual timesteps need orc_set_bins");
     return PHSPH_ERR_STATE;
   }
   int rc_all = PHSPH_OK;
-#pragma omp parallel reduction(min : dtcourant, dtforce) reduction(max : nbinmaxnew)
+#pragma omp parallel reduction(min : dtcourant, dtforce) reduction(max : nbinmaxnew) reduction(+ : ncheckbin)
   {
     nlist_t l = {0};
 #pragma omp for schedule(dynamic, 64)
@@ -1720,6 +1720,7 @@ int orc_force(orc_ctx *o, int32_t icall, int32_t np, const double *xyzh, const d
           if (b < ibin_neighi - 1) b = (int8_t)(ibin_neighi - 1);
           o->ibin[i - 1] = b;
           if (b > nbinmaxnew) nbinmaxnew = b;
+          ncheckbin++;
         } else {
           if (dtc < dtcourant) dtcourant = dtc;
           double m = dtf;
@@ -1733,7 +1734,8 @@ int orc_force(orc_ctx *o, int32_t icall, int32_t np, const double *xyzh, const d
   }
   if (dtcourant_out) *dtcourant_out = dtcourant;
   if (dtforce_out) *dtforce_out = dtforce;
-  o->nbinmaxnew = nbinmaxnew;
+  /* no active particle: nbinmax is kept (force.F90:748-756) */
+  o->nbinmaxnew = (ncheckbin == 0) ? o->nbinmax : nbinmaxnew;
   return rc_all;
 }
 
@@ -1848,4 +1850,238 @@ void orc_kernel_softening(int32_t kind, double q, double *pot, double *fs) { ker
 void orc_kernel(int32_t kind, double q, double *w, double *g, double *dphidh) {
   get_kernel(kind, q * q, q, w, g);
   *dphidh = dphidh_kern(kind, q * q, q);
+}
+
+/* ------------------------------------------------------------------ */
+/* leapfrog integrator loops (step_leapfrog.f90, substepping.F90)      */
+/* Gas particles only: iphase NULL = all active, >0 active, <0         */
+/* inactive; a particle is dead when h < tiny(h) (part.F90:931-941).   */
+/* ------------------------------------------------------------------ */
+#define AVDECAYCONST 0.1 /* shock_capturing.f90:32 */
+
+static inline int orc_active(const int8_t *iphase, int32_t i) { return iphase == NULL || iphase[i] > 0; }
+
+/* get_dt (utils_indtimesteps.f90:45-51): dtmax/2**ibin */
+static inline double get_dt_bin(double dtmax, int ibin) {
+  double p = 1.;
+  for (int k = 0; k < ibin; k++) p = p * 2.;
+  return dtmax / p;
+}
+
+/* init_step (step_leapfrog.f90:44-83), block timesteps */
+int orc_init_step(orc_ctx *o, int32_t np, double time, int8_t *ibin, int32_t nbinmax, double *twas) {
+  if (!o->p.ind_timesteps) {
+    snprintf(o->err, sizeof(o->err), "init_step: needs individual timesteps");
+    return PHSPH_ERR_INPUT;
+  }
+  if (time < TINY64)
+    for (int32_t i = 0; i < np; i++) ibin[i] = (int8_t)nbinmax;
+  for (int32_t i = 0; i < np; i++) twas[i] = time + 0.5 * get_dt_bin(o->p.dtmax, ibin[i]);
+  return PHSPH_OK;
+}
+
+/* predictor loop of step (step_leapfrog.f90:150-237) */
+int orc_step_predictor(orc_ctx *o, int32_t np, double t, double dtsph, const double *xyzh, double *vxyzu, int32_t ldv,
+                       const double *fxyzu, int32_t ldf, double *Bevol, const double *dBevol, double *dustevol,
+                       const double *ddustevol, const int8_t *iphase, const int8_t *ibin, int8_t *ibin_old,
+                       const double *twas, double ufloor, int32_t *nvfloor) {
+  const int ind = o->p.ind_timesteps, mhd = o->p.mhd, dust = o->p.use_dustfrac && o->p.ndustsmall > 0;
+  const int nv = o->p.maxvxyzu;
+  const double timei = t, hdtsph = 0.5 * dtsph;
+  int32_t nfl = 0;
+  for (int32_t i = 0; i < np; i++) {
+    if (xyzh[4 * (size_t)i + 3] < TINY64) continue;
+    double hdti;
+    if (ind) {
+      if (orc_active(iphase, i)) ibin_old[i] = ibin[i];
+      hdti = twas[i] - timei;
+    } else {
+      hdti = hdtsph;
+    }
+    double *v = vxyzu + (size_t)ldv * i;
+    const double *f = fxyzu + (size_t)ldf * i;
+    for (int k = 0; k < nv; k++) v[k] = v[k] + hdti * f[k];
+    if (ufloor > 0. && nv >= 4 && v[3] < ufloor) {
+      v[3] = ufloor;
+      nfl++;
+    }
+    if (mhd)
+      for (int k = 0; k < 4; k++) Bevol[4 * (size_t)i + k] = Bevol[4 * (size_t)i + k] + hdti * dBevol[4 * (size_t)i + k];
+    if (dust) dustevol[i] = fabs(dustevol[i] + hdti * ddustevol[i]);
+  }
+  if (nvfloor) *nvfloor = nfl;
+  return PHSPH_OK;
+}
+
+/* substep_sph (substepping.F90:241-264) */
+int orc_substep_sph(orc_ctx *o, int32_t np, double dt, double *xyzh, const double *vxyzu, int32_t ldv) {
+  (void)o;
+  for (int32_t i = 0; i < np; i++) {
+    double *x = xyzh + 4 * (size_t)i;
+    if (x[3] < TINY64) continue;
+    const double *v = vxyzu + (size_t)ldv * i;
+    x[0] = x[0] + dt * v[0];
+    x[1] = x[1] + dt * v[1];
+    x[2] = x[2] + dt * v[2];
+  }
+  return PHSPH_OK;
+}
+
+/* predict_sph loop of step (step_leapfrog.f90:307-400), gas, nalpha = 3
+ * (Cullen-Dehnen switch, config.F90:161) */
+int orc_step_predict_sph(orc_ctx *o, int32_t np, double timei, double dtsph, double *xyzh, const double *vxyzu,
+                         int32_t ldv, const double *fxyzu, int32_t ldf, double *vpred, const double *Bevol,
+                         const double *dBevol, double *Bpred, const double *dustevol, const double *ddustevol,
+                         double *dustpred, const float *divcurlv, int32_t lddcv, const double *eos_vars, int32_t ldeos,
+                         float *alphaind, const int8_t *iphase, const double *twas, double ufloor, int32_t *nvfloor) {
+  (void)iphase;
+  const int ind = o->p.ind_timesteps, mhd = o->p.mhd, dust = o->p.use_dustfrac && o->p.ndustsmall > 0;
+  const int nv = o->p.maxvxyzu;
+  const double pm = o->p.massoftype_gas, hdtsph = 0.5 * dtsph;
+  int32_t nfl = 0;
+  for (int32_t i = 0; i < np; i++) {
+    double *x = xyzh + 4 * (size_t)i;
+    if (x[3] < TINY64) continue;
+    /* h prediction (:332) */
+    x[3] = x[3] - dtsph * dhdrho(o, x[3], pm) * rhoh(o, x[3], pm) * (double)divcurlv[(size_t)lddcv * i];
+    double hdti = ind ? timei - twas[i] : hdtsph;
+    const double *v = vxyzu + (size_t)ldv * i;
+    const double *f = fxyzu + (size_t)ldf * i;
+    double *vp = vpred + (size_t)ldv * i;
+    for (int k = 0; k < nv; k++) vp[k] = v[k] + hdti * f[k];
+    if (ufloor > 0. && nv >= 4 && vp[3] < ufloor) {
+      vp[3] = ufloor;
+      nfl++;
+    }
+    if (mhd)
+      for (int k = 0; k < 4; k++) Bpred[4 * (size_t)i + k] = Bevol[4 * (size_t)i + k] + hdti * dBevol[4 * (size_t)i + k];
+    if (dust) dustpred[i] = dustevol[i] + hdti * ddustevol[i];
+    /* viscosity switch (:377-396) */
+    double hi = x[3];
+    double spsoundi = eos_vars[(size_t)ldeos * i + 1];
+    double tdecay1 = AVDECAYCONST * spsoundi / hi;
+    double ddenom = 1. / (1. + dtsph * tdecay1);
+    double alphaloci = (double)alphaind[3 * (size_t)i + 1];
+    double a1 = (double)alphaind[3 * (size_t)i];
+    if (a1 < alphaloci)
+      alphaind[3 * (size_t)i] = (float)alphaloci;
+    else
+      alphaind[3 * (size_t)i] = (float)((a1 + dtsph * alphaloci * tdecay1) * ddenom);
+  }
+  if (nvfloor) *nvfloor = nfl;
+  return PHSPH_OK;
+}
+
+/* one pass of the corrector loop of step (step_leapfrog.f90:482-625) */
+int orc_step_corrector(orc_ctx *o, int32_t np, double timei, double dtsph, const double *xyzh, double *vxyzu,
+                       int32_t ldv, const double *fxyzu, int32_t ldf, const double *vpred, double *Bevol,
+                       const double *dBevol, double *dustevol, const double *ddustevol, const int8_t *iphase,
+                       int8_t *ibin, int8_t *ibin_wake, int32_t nbinmax, double *twas, double ufloor, double *errmax,
+                       double *v2sum, int32_t *npc, int32_t *nwake, int32_t *nvfloor) {
+  const int ind = o->p.ind_timesteps, mhd = o->p.mhd, dust = o->p.use_dustfrac && o->p.ndustsmall > 0;
+  const int nv = o->p.maxvxyzu;
+  const double hdtsph = 0.5 * dtsph, dtmax = o->p.dtmax;
+  /* ibin_dts(ithdt,:) and ibin_dts(ittwas,:) at time_now = timei
+   * (step_leapfrog.f90:156-161, 73-80) */
+  double thdt[MAXBINS + 1], ttwas[MAXBINS + 1];
+  for (int b = 0; b <= MAXBINS; b++) {
+    double dtb = get_dt_bin(dtmax, b);
+    double dt1 = 1.0 / dtb;
+    thdt[b] = 0.5 * dtb;
+    ttwas[b] = ((double)(int64_t)(timei * dt1) + 0.5) * dtb;
+  }
+  double emax = 0., v2 = 0.;
+  int32_t n = 0, nw = 0, nfl = 0;
+  for (int32_t i = 0; i < np; i++) {
+    if (xyzh[4 * (size_t)i + 3] < TINY64) continue;
+    double *v = vxyzu + (size_t)ldv * i;
+    const double *f = fxyzu + (size_t)ldf * i;
+    double *B = mhd ? Bevol + 4 * (size_t)i : NULL;
+    const double *dB = mhd ? dBevol + 4 * (size_t)i : NULL;
+    if (ind) {
+      if (orc_active(iphase, i)) {
+        ibin_wake[i] = 0;
+        double hdti = timei - twas[i];
+        double dti = hdti + thdt[ibin[i]];
+        for (int k = 0; k < nv; k++) v[k] = v[k] + dti * f[k];
+        if (mhd)
+          for (int k = 0; k < 4; k++) B[k] = B[k] + dti * dB[k];
+        if (dust) dustevol[i] = dustevol[i] + dti * ddustevol[i];
+        twas[i] = twas[i] + dti;
+      }
+      double hdti = timei - twas[i];
+      for (int k = 0; k < nv; k++) v[k] = v[k] + hdti * f[k];
+      if (ufloor > 0. && nv >= 4 && v[3] < ufloor) {
+        v[3] = ufloor;
+        nfl++;
+      }
+      if (mhd)
+        for (int k = 0; k < 4; k++) B[k] = B[k] + hdti * dB[k];
+      if (dust) dustevol[i] = dustevol[i] + hdti * ddustevol[i];
+      if (ibin_wake[i] > ibin[i]) {
+        int w = ibin_wake[i] < nbinmax ? ibin_wake[i] : nbinmax;
+        nw++;
+        twas[i] = ttwas[w];
+        ibin[i] = (int8_t)w;
+        ibin_wake[i] = 0;
+      }
+    } else {
+      const double *vp = vpred + (size_t)ldv * i;
+      double vxi = v[0] + hdtsph * f[0];
+      double vyi = v[1] + hdtsph * f[1];
+      double vzi = v[2] + hdtsph * f[2];
+      double eni = 0.;
+      if (nv >= 4) eni = v[3] + hdtsph * f[3];
+      double erri = (vxi - vp[0]) * (vxi - vp[0]) + (vyi - vp[1]) * (vyi - vp[1]) + (vzi - vp[2]) * (vzi - vp[2]);
+      if (erri > emax) emax = erri;
+      v2 = v2 + (vxi * vxi + vyi * vyi + vzi * vzi);
+      n++;
+      v[0] = vxi;
+      v[1] = vyi;
+      v[2] = vzi;
+      if (nv >= 4) v[3] = eni;
+      if (mhd)
+        for (int k = 0; k < 4; k++) B[k] = B[k] + hdtsph * dB[k];
+      if (dust) dustevol[i] = dustevol[i] + hdtsph * ddustevol[i];
+    }
+  }
+  if (errmax) *errmax = emax;
+  if (v2sum) *v2sum = v2;
+  if (npc) *npc = n;
+  if (nwake) *nwake = nw;
+  if (nvfloor) *nvfloor = nfl;
+  return PHSPH_OK;
+}
+
+/* until_converged loop of step (step_leapfrog.f90:650-696) */
+int orc_step_reset(orc_ctx *o, int32_t np, double dtsph, double *vxyzu, int32_t ldv,
+                   const double *fxyzu, int32_t ldf, double *vpred, double *Bevol, const double *dBevol,
+                   double *Bpred, double *dustevol, const double *ddustevol, double *dustpred,
+                   const int8_t *iphase) {
+  const int ind = o->p.ind_timesteps, mhd = o->p.mhd, dust = o->p.use_dustfrac && o->p.ndustsmall > 0;
+  const int nv = o->p.maxvxyzu;
+  const double hdtsph = 0.5 * dtsph;
+  /* this loop has no dead-particle test in the reference */
+  for (int32_t i = 0; i < np; i++) {
+    double *v = vxyzu + (size_t)ldv * i;
+    double *vp = vpred + (size_t)ldv * i;
+    if (ind) {
+      if (!orc_active(iphase, i)) continue;
+      for (int k = 0; k < nv; k++) vp[k] = v[k];
+      if (mhd)
+        for (int k = 0; k < 4; k++) Bpred[4 * (size_t)i + k] = Bevol[4 * (size_t)i + k];
+      if (dust) dustpred[i] = dustevol[i];
+    } else {
+      const double *f = fxyzu + (size_t)ldf * i;
+      for (int k = 0; k < nv; k++) vp[k] = v[k];
+      if (mhd)
+        for (int k = 0; k < 4; k++) Bpred[4 * (size_t)i + k] = Bevol[4 * (size_t)i + k];
+      if (dust) dustpred[i] = dustevol[i];
+      for (int k = 0; k < nv; k++) v[k] = v[k] - hdtsph * f[k];
+      if (mhd)
+        for (int k = 0; k < 4; k++) Bevol[4 * (size_t)i + k] = Bevol[4 * (size_t)i + k] - hdtsph * dBevol[4 * (size_t)i + k];
+      if (dust) dustevol[i] = dustevol[i] - hdtsph * ddustevol[i];
+    }
+  }
+  return PHSPH_OK;
 }

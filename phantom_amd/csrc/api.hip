@@ -483,6 +483,230 @@ int phsph_set_active_particles(phsph_ctx *c, int32_t np, const double *xyzh, int
   return PHSPH_OK;
 }
 
+// ---- leapfrog integrator loops (step_leapfrog.f90, substepping.F90) ----
+
+int phsph_init_step(phsph_ctx *c, int32_t np, double time, int8_t *ibin, int32_t nbinmax, double *twas) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (!c->p.ind_timesteps) PH_RET(c, PHSPH_ERR_STATE, "init_step: needs individual timesteps");
+  if (np < 0 || (np > 0 && (!ibin || !twas))) PH_RET(c, PHSPH_ERR_INPUT, "init_step: missing array");
+  if (nbinmax < 0 || nbinmax > 30) PH_RET(c, PHSPH_ERR_INPUT, "init_step: nbinmax outside 0..30");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  ctx_clear_timings(c);
+  Stager st(c);
+  int8_t *db;
+  double *dt;
+  STAGE(st, ibin, (size_t)np, true, true, &db);
+  STAGE(st, twas, (size_t)np, false, true, &dt);
+  int rc = step_init_run(c, np, time, db, nbinmax, dt);
+  int rc2 = st.finish();
+  return rc ? rc : rc2;
+}
+
+// common argument checks of the step loops: the arrays each physics option
+// needs and the leading dimensions
+static int step_check(phsph_ctx *c, const char *who, int32_t np, int ldx, int ldv, int ldf) {
+  if (np < 0) PH_RET(c, PHSPH_ERR_INPUT, std::string(who) + ": np < 0");
+  if (ldx < 4 || ldv < c->p.maxvxyzu || (ldf >= 0 && ldf < c->p.maxvxyzu))
+    PH_RET(c, PHSPH_ERR_INPUT, std::string(who) + ": leading dimension below the component count");
+  return PHSPH_OK;
+}
+
+int phsph_step_predictor(phsph_ctx *c, int32_t np, double t, double dtsph, const double *xyzh, int32_t ldx,
+                         double *vxyzu, int32_t ldv, const double *fxyzu, int32_t ldf, double *Bevol,
+                         const double *dBevol, double *dustevol, const double *ddustevol, const int8_t *iphase,
+                         const int8_t *ibin, int8_t *ibin_old, const double *twas, double ufloor, int32_t *nvfloor) {
+  if (!c) return PHSPH_ERR_INPUT;
+  int rc = step_check(c, "step_predictor", np, ldx, ldv, ldf);
+  if (rc) return rc;
+  const bool mhd = c->p.mhd, dust = c->p.use_dustfrac && c->p.ndustsmall > 0, ind = c->p.ind_timesteps;
+  if (np > 0 && (!xyzh || !vxyzu || !fxyzu || (mhd && (!Bevol || !dBevol)) || (dust && (!dustevol || !ddustevol)) ||
+                 (ind && (!ibin || !ibin_old || !twas))))
+    PH_RET(c, PHSPH_ERR_INPUT, "step_predictor: missing array");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  ctx_clear_timings(c);
+  Stager st(c);
+  StepArgs a;
+  memset(&a, 0, sizeof(a));
+  double *t8;
+  STAGE(st, const_cast<double *>(xyzh), (size_t)ldx * np, true, false, &t8);
+  a.xyzh = t8; a.ldx = ldx;
+  STAGE(st, vxyzu, (size_t)ldv * np, true, true, &a.vxyzu);
+  a.ldv = ldv;
+  STAGE(st, const_cast<double *>(fxyzu), (size_t)ldf * np, true, false, &t8);
+  a.fxyzu = t8; a.ldf = ldf;
+  STAGE(st, mhd ? Bevol : nullptr, (size_t)4 * np, true, true, &a.Bevol);
+  STAGE(st, const_cast<double *>(mhd ? dBevol : nullptr), (size_t)4 * np, true, false, &t8);
+  a.dBevol = t8;
+  STAGE(st, dust ? dustevol : nullptr, (size_t)np, true, true, &a.dustevol);
+  STAGE(st, const_cast<double *>(dust ? ddustevol : nullptr), (size_t)np, true, false, &t8);
+  a.ddustevol = t8;
+  int8_t *t1;
+  STAGE(st, const_cast<int8_t *>(iphase), (size_t)np, true, false, &t1);
+  a.iphase = t1;
+  if (ind) {
+    STAGE(st, const_cast<int8_t *>(ibin), (size_t)np, true, false, &t1);
+    a.ibin_in = t1;
+    STAGE(st, ibin_old, (size_t)np, true, true, &a.ibin_old);
+    STAGE(st, const_cast<double *>(twas), (size_t)np, true, false, &t8);
+    a.twas = t8;
+  }
+  rc = step_predictor_run(c, np, t, dtsph, a, ufloor, nvfloor);
+  int rc2 = st.finish();
+  return rc ? rc : rc2;
+}
+
+int phsph_substep_sph(phsph_ctx *c, int32_t np, double dt, double *xyzh, int32_t ldx, const double *vxyzu,
+                      int32_t ldv) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (np < 0 || ldx < 4 || ldv < 3) PH_RET(c, PHSPH_ERR_INPUT, "substep_sph: bad size");
+  if (np > 0 && (!xyzh || !vxyzu)) PH_RET(c, PHSPH_ERR_INPUT, "substep_sph: missing array");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  ctx_clear_timings(c);
+  Stager st(c);
+  double *dx, *dv;
+  STAGE(st, xyzh, (size_t)ldx * np, true, true, &dx);
+  STAGE(st, const_cast<double *>(vxyzu), (size_t)ldv * np, true, false, &dv);
+  // the particles move: the tree of the last maketree no longer describes them
+  c->tree_valid = false;
+  int rc = step_drift_run(c, np, dt, dx, ldx, dv, ldv);
+  int rc2 = st.finish();
+  return rc ? rc : rc2;
+}
+
+int phsph_step_predict_sph(phsph_ctx *c, int32_t np, double timei, double dtsph, double *xyzh, int32_t ldx,
+                           const double *vxyzu, int32_t ldv, const double *fxyzu, int32_t ldf, double *vpred,
+                           const double *Bevol, const double *dBevol, double *Bpred, const double *dustevol,
+                           const double *ddustevol, double *dustpred, const float *divcurlv, int32_t lddcv,
+                           const double *eos_vars, int32_t ldeos, float *alphaind, const int8_t *iphase,
+                           const double *twas, double ufloor, int32_t *nvfloor) {
+  if (!c) return PHSPH_ERR_INPUT;
+  int rc = step_check(c, "step_predict_sph", np, ldx, ldv, ldf);
+  if (rc) return rc;
+  if (lddcv < 1 || ldeos < 2) PH_RET(c, PHSPH_ERR_INPUT, "step_predict_sph: bad leading dimension");
+  const bool mhd = c->p.mhd, dust = c->p.use_dustfrac && c->p.ndustsmall > 0, ind = c->p.ind_timesteps;
+  if (np > 0 && (!xyzh || !vxyzu || !fxyzu || !vpred || !divcurlv || !eos_vars || !alphaind ||
+                 (mhd && (!Bevol || !dBevol || !Bpred)) || (dust && (!dustevol || !ddustevol || !dustpred)) ||
+                 (ind && !twas)))
+    PH_RET(c, PHSPH_ERR_INPUT, "step_predict_sph: missing array");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  ctx_clear_timings(c);
+  Stager st(c);
+  StepArgs a;
+  memset(&a, 0, sizeof(a));
+  double *t8;
+  float *tf;
+  int8_t *t1;
+  STAGE(st, xyzh, (size_t)ldx * np, true, true, &a.xyzh);
+  a.ldx = ldx;
+  STAGE(st, const_cast<double *>(vxyzu), (size_t)ldv * np, true, false, &a.vxyzu);
+  a.ldv = ldv;
+  STAGE(st, const_cast<double *>(fxyzu), (size_t)ldf * np, true, false, &t8);
+  a.fxyzu = t8; a.ldf = ldf;
+  STAGE(st, vpred, (size_t)ldv * np, true, true, &a.vpred);
+  STAGE(st, const_cast<double *>(mhd ? Bevol : nullptr), (size_t)4 * np, true, false, &a.Bevol);
+  STAGE(st, const_cast<double *>(mhd ? dBevol : nullptr), (size_t)4 * np, true, false, &t8);
+  a.dBevol = t8;
+  STAGE(st, mhd ? Bpred : nullptr, (size_t)4 * np, true, true, &a.Bpred);
+  STAGE(st, const_cast<double *>(dust ? dustevol : nullptr), (size_t)np, true, false, &a.dustevol);
+  STAGE(st, const_cast<double *>(dust ? ddustevol : nullptr), (size_t)np, true, false, &t8);
+  a.ddustevol = t8;
+  STAGE(st, dust ? dustpred : nullptr, (size_t)np, true, true, &a.dustpred);
+  STAGE(st, const_cast<float *>(divcurlv), (size_t)lddcv * np, true, false, &tf);
+  a.divcurlv = tf; a.lddcv = lddcv;
+  STAGE(st, const_cast<double *>(eos_vars), (size_t)ldeos * np, true, false, &t8);
+  a.eos_vars = t8; a.ldeos = ldeos;
+  STAGE(st, alphaind, (size_t)3 * np, true, true, &a.alphaind);
+  STAGE(st, const_cast<int8_t *>(iphase), (size_t)np, true, false, &t1);
+  a.iphase = t1;
+  if (ind) STAGE(st, const_cast<double *>(twas), (size_t)np, true, false, &a.twas);
+  rc = step_predict_sph_run(c, np, timei, dtsph, a, ufloor, nvfloor);
+  int rc2 = st.finish();
+  return rc ? rc : rc2;
+}
+
+int phsph_step_corrector(phsph_ctx *c, int32_t np, double timei, double dtsph, const double *xyzh, int32_t ldx,
+                         double *vxyzu, int32_t ldv, const double *fxyzu, int32_t ldf, const double *vpred,
+                         double *Bevol, const double *dBevol, double *dustevol, const double *ddustevol,
+                         const int8_t *iphase, int8_t *ibin, int8_t *ibin_wake, int32_t nbinmax, double *twas,
+                         double ufloor, double *errmax, double *v2sum, int32_t *npc, int32_t *nwake,
+                         int32_t *nvfloor) {
+  if (!c) return PHSPH_ERR_INPUT;
+  int rc = step_check(c, "step_corrector", np, ldx, ldv, ldf);
+  if (rc) return rc;
+  const bool mhd = c->p.mhd, dust = c->p.use_dustfrac && c->p.ndustsmall > 0, ind = c->p.ind_timesteps;
+  if (np > 0 && (!xyzh || !vxyzu || !fxyzu || (!ind && !vpred) || (mhd && (!Bevol || !dBevol)) ||
+                 (dust && (!dustevol || !ddustevol)) || (ind && (!ibin || !ibin_wake || !twas))))
+    PH_RET(c, PHSPH_ERR_INPUT, "step_corrector: missing array");
+  if (ind && (nbinmax < 0 || nbinmax > 30)) PH_RET(c, PHSPH_ERR_INPUT, "step_corrector: nbinmax outside 0..30");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  ctx_clear_timings(c);
+  Stager st(c);
+  StepArgs a;
+  memset(&a, 0, sizeof(a));
+  double *t8;
+  int8_t *t1;
+  STAGE(st, const_cast<double *>(xyzh), (size_t)ldx * np, true, false, &a.xyzh);
+  a.ldx = ldx;
+  STAGE(st, vxyzu, (size_t)ldv * np, true, true, &a.vxyzu);
+  a.ldv = ldv;
+  STAGE(st, const_cast<double *>(fxyzu), (size_t)ldf * np, true, false, &t8);
+  a.fxyzu = t8; a.ldf = ldf;
+  STAGE(st, const_cast<double *>(ind ? nullptr : vpred), (size_t)ldv * np, true, false, &a.vpred);
+  STAGE(st, mhd ? Bevol : nullptr, (size_t)4 * np, true, true, &a.Bevol);
+  STAGE(st, const_cast<double *>(mhd ? dBevol : nullptr), (size_t)4 * np, true, false, &t8);
+  a.dBevol = t8;
+  STAGE(st, dust ? dustevol : nullptr, (size_t)np, true, true, &a.dustevol);
+  STAGE(st, const_cast<double *>(dust ? ddustevol : nullptr), (size_t)np, true, false, &t8);
+  a.ddustevol = t8;
+  STAGE(st, const_cast<int8_t *>(iphase), (size_t)np, true, false, &t1);
+  a.iphase = t1;
+  if (ind) {
+    STAGE(st, ibin, (size_t)np, true, true, &a.ibin);
+    STAGE(st, ibin_wake, (size_t)np, true, true, &a.ibin_wake);
+    STAGE(st, twas, (size_t)np, true, true, &a.twas);
+  }
+  rc = step_corrector_run(c, np, timei, dtsph, a, nbinmax, ufloor, errmax, v2sum, npc, nwake, nvfloor);
+  int rc2 = st.finish();
+  return rc ? rc : rc2;
+}
+
+int phsph_step_reset(phsph_ctx *c, int32_t np, double dtsph, double *vxyzu, int32_t ldv, const double *fxyzu,
+                     int32_t ldf, double *vpred, double *Bevol, const double *dBevol, double *Bpred,
+                     double *dustevol, const double *ddustevol, double *dustpred, const int8_t *iphase) {
+  if (!c) return PHSPH_ERR_INPUT;
+  int rc = step_check(c, "step_reset", np, 4, ldv, ldf);
+  if (rc) return rc;
+  const bool mhd = c->p.mhd, dust = c->p.use_dustfrac && c->p.ndustsmall > 0;
+  if (np > 0 && (!vxyzu || !fxyzu || !vpred || (mhd && (!Bevol || !dBevol || !Bpred)) ||
+                 (dust && (!dustevol || !ddustevol || !dustpred))))
+    PH_RET(c, PHSPH_ERR_INPUT, "step_reset: missing array");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  ctx_clear_timings(c);
+  Stager st(c);
+  StepArgs a;
+  memset(&a, 0, sizeof(a));
+  double *t8;
+  int8_t *t1;
+  STAGE(st, vxyzu, (size_t)ldv * np, true, true, &a.vxyzu);
+  a.ldv = ldv;
+  STAGE(st, const_cast<double *>(fxyzu), (size_t)ldf * np, true, false, &t8);
+  a.fxyzu = t8; a.ldf = ldf;
+  STAGE(st, vpred, (size_t)ldv * np, true, true, &a.vpred);
+  STAGE(st, mhd ? Bevol : nullptr, (size_t)4 * np, true, true, &a.Bevol);
+  STAGE(st, const_cast<double *>(mhd ? dBevol : nullptr), (size_t)4 * np, true, false, &t8);
+  a.dBevol = t8;
+  STAGE(st, mhd ? Bpred : nullptr, (size_t)4 * np, true, true, &a.Bpred);
+  STAGE(st, dust ? dustevol : nullptr, (size_t)np, true, true, &a.dustevol);
+  STAGE(st, const_cast<double *>(dust ? ddustevol : nullptr), (size_t)np, true, false, &t8);
+  a.ddustevol = t8;
+  STAGE(st, dust ? dustpred : nullptr, (size_t)np, true, true, &a.dustpred);
+  STAGE(st, const_cast<int8_t *>(iphase), (size_t)np, true, false, &t1);
+  a.iphase = t1;
+  rc = step_reset_run(c, np, dtsph, a);
+  int rc2 = st.finish();
+  return rc ? rc : rc2;
+}
+
 int phsph_get_neighbour_stats(phsph_ctx *c, double *trialmean, double *actualmean, int32_t *maxtrial,
                               int32_t *maxactual, int64_t *nrhocalc, int64_t *nactualtot) {
   if (!c) return PHSPH_ERR_INPUT;

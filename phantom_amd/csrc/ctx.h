@@ -138,6 +138,9 @@ struct phsph_ctx {
   DBuf<int32_t> np2p;
   DBuf<float> gsoft;                  // gradh(2) in tree order
   float *poten_user = nullptr;        // registered potential energy output (host or device)
+
+  // integrator reductions (block partials of the step loops)
+  DBuf<double> step_red;
   float *poten_out = nullptr;         // device view of poten_user during a force call
 
   // staging buffers for host arrays
@@ -233,6 +236,31 @@ struct ForceIOArgs {  // device pointers in the caller's layout
   int nbinmax, istepfrac, ibinnow;
 };
 int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc, double *dtf);
+// step.hip
+struct StepArgs {  // device pointers in the caller's layout
+  double *xyzh; int ldx;
+  double *vxyzu; int ldv;
+  const double *fxyzu; int ldf;
+  double *vpred;
+  double *Bevol; const double *dBevol; double *Bpred;
+  double *dustevol; const double *ddustevol; double *dustpred;
+  const float *divcurlv; int lddcv;
+  const double *eos_vars; int ldeos;
+  float *alphaind;
+  const int8_t *iphase;
+  const int8_t *ibin_in; int8_t *ibin; int8_t *ibin_old; int8_t *ibin_wake;
+  double *twas;
+};
+int step_init_run(phsph_ctx *c, int np, double time, int8_t *ibin, int nbinmax, double *twas);
+int step_predictor_run(phsph_ctx *c, int np, double t, double dtsph, const StepArgs &s, double ufloor,
+                       int32_t *nvfloor);
+int step_drift_run(phsph_ctx *c, int np, double dt, double *xyzh, int ldx, const double *vxyzu, int ldv);
+int step_predict_sph_run(phsph_ctx *c, int np, double timei, double dtsph, const StepArgs &s, double ufloor,
+                         int32_t *nvfloor);
+int step_corrector_run(phsph_ctx *c, int np, double timei, double dtsph, const StepArgs &s, int nbinmax,
+                       double ufloor, double *errmax, double *v2sum, int32_t *npc, int32_t *nwake,
+                       int32_t *nvfloor);
+int step_reset_run(phsph_ctx *c, int np, double dtsph, const StepArgs &s);
 struct FRec;
 int grav_tree(phsph_ctx *c);
 int grav_hmax_replay(phsph_ctx *c, int icall);

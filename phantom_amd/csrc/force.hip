@@ -802,7 +802,8 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
     PH_HIPCHECK(c, hipStreamSynchronize(s));
     const int32_t fl = fl2[0];
     if (fl & FFL_DTC) PH_RET(c, PHSPH_ERR_INPUT, "force: invalid dtc");
-    if (a.ibin) c->nbinmax_new = fl2[1];
+    // with no active particle nbinmax is kept (force.F90:748-756)
+    if (a.ibin) c->nbinmax_new = (nact > 0) ? fl2[1] : a.nbinmax;
     double dtc, dtf;
     memcpy(&dtc, &red[0], 8);
     memcpy(&dtf, &red[1], 8);

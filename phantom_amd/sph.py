@@ -9,6 +9,9 @@ reference's names and argument meaning:
     cons2prim(xyzh, vxyzu, dvdx, alphaind,...) cons2prim.f90:274
     force(icall, xyzh, vxyzu, ...)             force.F90:193
     get_neighbour_stats()                      dens.F90:1137
+    step_predictor / substep_sph / step_predict_sph / step_corrector /
+    step_reset                                 step_leapfrog.f90:150-696,
+                                               substepping.F90:241
 
 Arrays are numpy arrays (host memory) or phantom_amd.device.DeviceArray
 (HBM), in the reference layout: Fortran a(ncomp, n) is a C-contiguous array of shape
@@ -31,7 +34,8 @@ EXPORTED = [
     "phsph_maketree", "phsph_get_tree", "phsph_densityiterate", "phsph_cons2prim", "phsph_force",
     "phsph_get_neighbour_stats", "phsph_get_particle_stats", "phsph_get_force_stats", "phsph_get_timings",
     "phsph_set_bins", "phsph_get_nbinmax", "phsph_set_active_particles", "phsph_set_poten",
-    "phsph_get_quads",
+    "phsph_get_quads", "phsph_init_step", "phsph_step_predictor", "phsph_substep_sph", "phsph_step_predict_sph",
+    "phsph_step_corrector", "phsph_step_reset",
 ]
 
 _lib = None
@@ -76,6 +80,15 @@ def load_library(path=LIB_PATH):
     L.phsph_set_poten.argtypes = [vp, vp]
     L.phsph_get_quads.argtypes = [vp, vp, i64]
     L.phsph_set_active_particles.argtypes = [vp, i32, vp, i32, vp, vp, i32, i32, P(i32), P(i32), P(i32)]
+    L.phsph_init_step.argtypes = [vp, i32, dbl, vp, i32, vp]
+    L.phsph_step_predictor.argtypes = [vp, i32, dbl, dbl, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp,
+                                       dbl, P(i32)]
+    L.phsph_substep_sph.argtypes = [vp, i32, dbl, vp, i32, vp, i32]
+    L.phsph_step_predict_sph.argtypes = [vp, i32, dbl, dbl, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp,
+                                         vp, i32, vp, i32, vp, vp, vp, dbl, P(i32)]
+    L.phsph_step_corrector.argtypes = [vp, i32, dbl, dbl, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp,
+                                       vp, i32, vp, dbl, P(dbl), P(dbl), P(i32), P(i32), P(i32)]
+    L.phsph_step_reset.argtypes = [vp, i32, dbl, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]
     for name in EXPORTED:
         if name != "phsph_default_params" and name != "phsph_last_error":
             getattr(L, name).restype = C.c_int
@@ -336,6 +349,83 @@ class Context:
         self._check(self.lib.phsph_set_active_particles(self._h, n, px, ldx, pph, pb, nbinmax, istepfrac,
                                                         C.byref(na), C.byref(nl), C.byref(bn)))
         return na.value, nl.value, bn.value
+
+    # -- leapfrog integrator loops (step_leapfrog.f90) ------------------------
+    def init_step(self, time, ibin, nbinmax, twas):
+        """init_step (step_leapfrog.f90:44-83): twas = time + dt(ibin)/2,
+        all bins set to nbinmax first when time is 0."""
+        pb, _ = _ptr(ibin)
+        pt, _ = _ptr(twas)
+        self._check(self.lib.phsph_init_step(self._h, ibin.shape[0], time, pb, nbinmax, pt))
+
+    def step_predictor(self, t, dtsph, xyzh, vxyzu, fxyzu, Bevol=None, dBevol=None, dustevol=None,
+                       ddustevol=None, iphase=None, ibin=None, ibin_old=None, twas=None, ufloor=0.0):
+        """Predictor loop of step (step_leapfrog.f90:150-237).  Returns the
+        number of floored thermal energies."""
+        n = xyzh.shape[0]
+        px, ldx = _ptr(xyzh)
+        pv, ldv = _ptr(vxyzu)
+        pf, ldf = _ptr(fxyzu)
+        nfl = C.c_int32()
+        self._check(self.lib.phsph_step_predictor(
+            self._h, n, t, dtsph, px, ldx, pv, ldv, pf, ldf, _ptr(Bevol)[0], _ptr(dBevol)[0], _ptr(dustevol)[0],
+            _ptr(ddustevol)[0], _ptr(iphase)[0], _ptr(ibin)[0], _ptr(ibin_old)[0], _ptr(twas)[0], ufloor,
+            C.byref(nfl)))
+        return nfl.value
+
+    def substep_sph(self, dt, xyzh, vxyzu):
+        """substep_sph (substepping.F90:241-264): x += dt*v."""
+        px, ldx = _ptr(xyzh)
+        pv, ldv = _ptr(vxyzu)
+        self._check(self.lib.phsph_substep_sph(self._h, xyzh.shape[0], dt, px, ldx, pv, ldv))
+
+    def step_predict_sph(self, timei, dtsph, xyzh, vxyzu, fxyzu, vpred, divcurlv, eos_vars, alphaind, Bevol=None,
+                         dBevol=None, Bpred=None, dustevol=None, ddustevol=None, dustpred=None, iphase=None,
+                         twas=None, ufloor=0.0):
+        """predict_sph loop of step (step_leapfrog.f90:307-400).  Returns the
+        number of floored predicted thermal energies."""
+        n = xyzh.shape[0]
+        px, ldx = _ptr(xyzh)
+        pv, ldv = _ptr(vxyzu)
+        pf, ldf = _ptr(fxyzu)
+        pvp, ldvp = _ptr(vpred)
+        if ldvp != ldv:
+            raise ValueError("vpred must have the layout of vxyzu")
+        pdc, lddc = _ptr(divcurlv)
+        pe, lde = _ptr(eos_vars)
+        nfl = C.c_int32()
+        self._check(self.lib.phsph_step_predict_sph(
+            self._h, n, timei, dtsph, px, ldx, pv, ldv, pf, ldf, pvp, _ptr(Bevol)[0], _ptr(dBevol)[0],
+            _ptr(Bpred)[0], _ptr(dustevol)[0], _ptr(ddustevol)[0], _ptr(dustpred)[0], pdc, lddc, pe, lde,
+            _ptr(alphaind)[0], _ptr(iphase)[0], _ptr(twas)[0], ufloor, C.byref(nfl)))
+        return nfl.value
+
+    def step_corrector(self, timei, dtsph, xyzh, vxyzu, fxyzu, vpred=None, Bevol=None, dBevol=None,
+                       dustevol=None, ddustevol=None, iphase=None, ibin=None, ibin_wake=None, nbinmax=0,
+                       twas=None, ufloor=0.0):
+        """One pass of the corrector loop (step_leapfrog.f90:482-625).
+        Returns errmax, v2sum, np (the check_velocity_error inputs), nwake
+        and nvfloor."""
+        n = xyzh.shape[0]
+        px, ldx = _ptr(xyzh)
+        pv, ldv = _ptr(vxyzu)
+        pf, ldf = _ptr(fxyzu)
+        em, v2 = C.c_double(), C.c_double()
+        npc, nw, nfl = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.lib.phsph_step_corrector(
+            self._h, n, timei, dtsph, px, ldx, pv, ldv, pf, ldf, _ptr(vpred)[0], _ptr(Bevol)[0], _ptr(dBevol)[0],
+            _ptr(dustevol)[0], _ptr(ddustevol)[0], _ptr(iphase)[0], _ptr(ibin)[0], _ptr(ibin_wake)[0], nbinmax,
+            _ptr(twas)[0], ufloor, C.byref(em), C.byref(v2), C.byref(npc), C.byref(nw), C.byref(nfl)))
+        return dict(errmax=em.value, v2sum=v2.value, np=npc.value, nwake=nw.value, nvfloor=nfl.value)
+
+    def step_reset(self, dtsph, vxyzu, fxyzu, vpred, Bevol=None, dBevol=None, Bpred=None, dustevol=None,
+                   ddustevol=None, dustpred=None, iphase=None):
+        """until_converged loop of step (step_leapfrog.f90:650-696)."""
+        pv, ldv = _ptr(vxyzu)
+        pf, ldf = _ptr(fxyzu)
+        self._check(self.lib.phsph_step_reset(
+            self._h, vxyzu.shape[0], dtsph, pv, ldv, pf, ldf, _ptr(vpred)[0], _ptr(Bevol)[0], _ptr(dBevol)[0],
+            _ptr(Bpred)[0], _ptr(dustevol)[0], _ptr(ddustevol)[0], _ptr(dustpred)[0], _ptr(iphase)[0]))
 
     def get_force_stats(self):
         n = C.c_int64()
