@@ -29,6 +29,9 @@ WORKLOADS = {
     "mhd_dust_128": lambda: setups.mhd_dust_box(nx=128, jitter=0.05),
     "hydro_box_8M": lambda: setups.uniform_box(nx=256, ny=256, nz=128, box=(-0.5, 0.5, -0.5, 0.5, -0.25, 0.25),
                                                 isothermal=False, seed=8),
+    # self-gravitating n=1.5 polytrope, multipole walk at theta = 0.5
+    "polytrope_1M": lambda: setups.polytrope(n=1048576, theta=0.5),
+    "polytrope_8M": lambda: setups.polytrope(n=8388608, theta=0.5),
 }
 
 
@@ -41,6 +44,9 @@ def parse():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--mode", default="derivs", choices=["derivs", "step"],
+                    help="derivs: densityiterate + cons2prim + force per step (the headline metric); "
+                         "step: whole leapfrog steps (tree, density, force, integrator loops)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 256 MiB device copy (reference for the HBM byte counters)")
     return ap.parse_args()
@@ -60,6 +66,8 @@ def main():
         return 2
     from phantom_amd import device, sph
 
+    if args.mode == "step":
+        return step_bench(args)
     setup = WORKLOADS[args.workload]()
     p = setup["params"]
     p.precision = abi.PREC_MIXED if args.precision == "mixed" else abi.PREC_FP64
@@ -74,7 +82,7 @@ def main():
     dustevol = D.from_numpy(setup["dustevol"]) if "dustevol" in setup else None
     divcurlv = D((n, 1), np.float32)
     divcurlB = D((n, 4), np.float32) if p.mhd else None
-    gradh = D((n, 1), np.float32)
+    gradh = D((n, 2 if p.gravity else 1), np.float32)
     dvdx = D((n, 9), np.float32)
     eos = D((n, 3), np.float64)
     dustfrac = D((n,), np.float64) if p.use_dustfrac else None
@@ -205,6 +213,107 @@ def main():
         "neighbours": {"dens_mean": stats["actualmean"], "dens_max": stats["maxactual"],
                        "force_pairs_per_particle": npairs / n, "nrhocalc": stats["nrhocalc"]},
         "tree_kernels_ms": {nm: round(ms, 4) for nm, ms in tree_kernels.items()},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+class _Recorder:
+    """Forwards the calls of a library context and keeps the kernel times
+    (HIP events on the library stream) of every call."""
+
+    def __init__(self, ctx):
+        self._ctx = ctx
+        self.kt = {}
+        self.on = False
+
+    def __getattr__(self, name):
+        f = getattr(self._ctx, name)
+        if not callable(f) or name.startswith("_") or name == "timings":
+            return f
+
+        def call(*a, **k):
+            r = f(*a, **k)
+            if self.on:
+                for nm, ms in self._ctx.timings():
+                    self.kt.setdefault(nm, []).append(ms)
+            return r
+
+        return call
+
+
+def step_bench(args):
+    """Whole leapfrog steps (step_leapfrog.f90:91-760 through
+    phantom_amd.integrator) with the particle state resident in HBM: per
+    step one tree build, densityiterate, cons2prim, force and the predictor,
+    drift, predict_sph and corrector loops.  dt is the global Courant/force
+    timestep.  Reports updates/s of whole steps, the per-kernel times and
+    the HBM roofline of the integrator loops."""
+    from phantom_amd import device, integrator, sph
+
+    setup = WORKLOADS[args.workload]()
+    p = setup["params"]
+    p.precision = abi.PREC_MIXED if args.precision == "mixed" else abi.PREC_FP64
+    n = setup["xyzh"].shape[0]
+    log(f"workload {args.workload} n={n} precision={args.precision} mode=step")
+    s = integrator.State(p, setup["xyzh"], setup["vxyzu"], setup["alphaind"], Bevol=setup.get("Bevol"),
+                         dustevol=setup.get("dustevol"), device=True)
+    ctx = sph.Context(p)
+    rec = _Recorder(ctx)
+    integrator.init_global(rec, s)
+    t = 0.0
+
+    def one():
+        nonlocal t
+        dt = min(s.dtforce, s.dtcourant, p.dtmax)
+        r = integrator.step(rec, s, t, dt)
+        t = t + dt
+        return r
+
+    for k in range(args.warmup):
+        one()
+        log(f"warmup step {k + 1}/{args.warmup} done")
+    device.synchronize()
+    rec.on = True
+    its = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        its.append(one()["its"])
+    device.synchronize()
+    elapsed = time.perf_counter() - t0
+    log(f"timed {args.steps} steps in {elapsed:.3f} s")
+    kt = rec.kt
+
+    def kmean(name):
+        return float(np.mean(kt[name])) * 1e-3 if name in kt else 0.0
+
+    def ktot(name):
+        return float(np.sum(kt.get(name, []))) * 1e-3 / args.steps
+
+    loops = {}
+    for nm in ("step_predictor", "substep_sph", "step_predict_sph", "step_corrector"):
+        if nm not in kt:
+            continue
+        b = n * work_model.step_loop_bytes(nm, nv=p.maxvxyzu, mhd=bool(p.mhd),
+                                           dust=bool(p.use_dustfrac and p.ndustsmall), ind=bool(p.ind_timesteps))
+        gbs = b / kmean(nm) / 1e9
+        loops[nm] = {"bound": "hbm", "bytes": b, "launch_ms": round(1e3 * kmean(nm), 4),
+                     "achieved": round(gbs, 1), "peak": work_model.PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(gbs / work_model.PEAK_HBM_GBS, 4)}
+    out = {
+        "metric": "leapfrog step particle updates/s",
+        "value": n * args.steps / elapsed,
+        "unit": "particle-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-pair/f64-accum",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "n_particles": n, "mode": "step", "precision": args.precision,
+                   "parallelism": "single GPU"},
+        "corrector_iterations": its,
+        "kernels_ms_per_step": {nm: round(1e3 * ktot(nm), 4) for nm in sorted(kt)},
+        "integrator_loops": loops,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -38,21 +38,21 @@ def build(verbose=False):
 
 
 def build_fortran(verbose=False):
-    """The iso_c_binding module and the Fortran driver that calls the
-    library the way Phantom's deriv.f90 does (skipped without amdflang)."""
+    """The iso_c_binding module and the Fortran drivers that call the
+    library the way Phantom's deriv.f90 and step do (skipped without
+    amdflang)."""
     if not os.path.exists(FC):
         if verbose:
             print("amdflang not found: Fortran interface not built", flush=True)
         return None
     mod = os.path.join(FORTRAN, "phsph.F90")
-    drv = os.path.join(FORTRAN, "shim_driver.f90")
     obj = os.path.join(FORTRAN, "phsph.o")
-    exe = os.path.join(FORTRAN, "shim_driver")
-    cmds = [
-        [FC, "-O2", "-c", mod, "-J", FORTRAN, "-o", obj],
-        [FC, "-O2", "-I", FORTRAN, "-J", FORTRAN, drv, obj, "-L", HERE, "-lphsph",
-         "-Wl,-rpath,$ORIGIN/..", "-o", exe],
-    ]
+    cmds = [[FC, "-O2", "-c", mod, "-J", FORTRAN, "-o", obj]]
+    for name in ("shim_driver", "step_driver"):
+        drv = os.path.join(FORTRAN, name + ".f90")
+        exe = os.path.join(FORTRAN, name)
+        cmds.append([FC, "-O2", "-I", FORTRAN, "-J", FORTRAN, drv, obj, "-L", HERE, "-lphsph",
+                     "-Wl,-rpath,$ORIGIN/..", "-o", exe])
     for c in cmds:
         if verbose:
             print(" ".join(c), flush=True)

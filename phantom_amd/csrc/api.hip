@@ -7,6 +7,7 @@
 // no transfer.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "ctx.h"
 
@@ -168,6 +169,31 @@ int phsph_create(const phsph_params *p, int32_t device, phsph_ctx **out) {
   }
   c->p = *p;
   c->dc = make_devconst(*p);
+  // initial list capacities and the list budget (KiB) may be set from the
+  // environment, e.g. PHSPH_CAPS="gcand=64,dlist=32,flist=32,p2p=16,budget_kib=1":
+  // larger values avoid second search passes, smaller ones exercise them
+  if (const char *env = getenv("PHSPH_CAPS")) {
+    std::string spec(env);
+    size_t pos = 0;
+    while (pos < spec.size()) {
+      size_t end = spec.find(',', pos);
+      if (end == std::string::npos) end = spec.size();
+      const std::string item = spec.substr(pos, end - pos);
+      const size_t eq = item.find('=');
+      if (eq != std::string::npos) {
+        const std::string key = item.substr(0, eq);
+        const int v = atoi(item.c_str() + eq + 1);
+        if (v >= 1 && v <= (1 << 24)) {
+          if (key == "gcand") c->caps.gcand = v;
+          else if (key == "dlist") c->caps.dlist = v;
+          else if (key == "flist") c->caps.flist = v;
+          else if (key == "p2p") c->caps.p2p = v;
+          else if (key == "budget_kib") c->list_budget = (size_t)v << 10;
+        }
+      }
+      pos = end + 1;
+    }
+  }
   hipError_t e = hipSetDevice(device);
   // a blocking stream: work the caller queued on the null stream (copies
   // into device arrays it passes) completes before the library's kernels

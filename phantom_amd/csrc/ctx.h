@@ -61,6 +61,7 @@ struct phsph_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   Caps caps;
+  size_t list_budget = (size_t)16 << 30;  // bytes a uniform list capacity may take
 
   // ---- tree (valid after maketree) ----
   int np = 0;            // particles passed to maketree
@@ -90,6 +91,11 @@ struct phsph_ctx {
   // ---- neighbour search ----
   DBuf<uint32_t> gcand;      // candidate leaves per group [ngroups*caps.gcand]
   DBuf<int32_t> gncand;      // count per group
+  DBuf<int64_t> goff;        // exact per-group offsets into gcand (after an overflow)
+  const int64_t *goff_p = nullptr;  // goff.p when the last walk used exact offsets
+  DBuf<int64_t> loff;        // per-group offsets of the force lists (after an overflow)
+  DBuf<int32_t> gmax;        // longest list per group
+  DBuf<int64_t> poff;        // per-leaf offsets of the near-field lists (after an overflow)
   int ngroups = 0;
   DBuf<int32_t> ntrial;      // trial particles per leaf (statistics)
   DBuf<uint32_t> nlist;      // neighbour lists (list_at layout; node | code << 27)
@@ -104,7 +110,8 @@ struct phsph_ctx {
   DBuf<double4> fpos;                    // force: x, y, z, 1/h (tree order)
   DBuf<double4> fvel;                    // force: vx, vy, vz, u (mixed precision)
   DBuf<double> fh;                       // force: h (tree order)
-  DBuf<int32_t> act;                     // active tree positions
+  DBuf<int32_t> act;                     // active tree positions (-1: group padding)
+  std::vector<int32_t> act_host;         // host copy used to pad sparse groups
   int nact = 0;
   DBuf<int32_t> act_dens;                // active positions of the last densityiterate
   int nact_dens = 0;
@@ -214,6 +221,7 @@ struct DensIOArgs {  // device pointers in the caller's layout
 int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a);
 int dens_trial_stats(phsph_ctx *c);
 int compact_active(phsph_ctx *c, const int32_t *flag, int n);
+int pad_groups(phsph_ctx *c);
 int dens_particle_stats(phsph_ctx *c, int32_t *d_nneigh, int32_t *d_nits);
 // force.hip
 struct ForceIOArgs {  // device pointers in the caller's layout

@@ -28,9 +28,10 @@
 #include "stream.h"
 
 #include <algorithm>
+#include <vector>
 #include <cstdlib>
 
-enum { DFL_CAND = 1, DFL_LIST = 2, DFL_REWALK = 4, DFL_NOCONV = 8, DFL_NEGH = 16 };
+enum { DFL_CAND = 1, DFL_LIST = 2, DFL_REWALK = 4, DFL_NOCONV = 8, DFL_NEGH = 16, DFL_STACK = 32 };
 
 struct DensIn {
   const double4 *tp;  // x, y, z (treecache) and h at entry
@@ -106,16 +107,18 @@ k_group_walk(DevConst dc, int mode, double rfac, const phsph_kdnode *__restrict_
              const double *__restrict__ hmaxn, const int32_t *__restrict__ leaf_flag,
              const int32_t *__restrict__ act, int nact, const double *__restrict__ tx,
              const double *__restrict__ ty, const double *__restrict__ tz, const double *__restrict__ hsrc,
-             int ngroups, uint32_t *__restrict__ gcand, int32_t *__restrict__ gncand, int capg,
-             int32_t *__restrict__ flags) {
+             int ngroups, uint32_t *__restrict__ gcand, int32_t *__restrict__ gncand, const int64_t *goff,
+             int capg, int32_t *__restrict__ flags) {
   __shared__ int stk[WALK_STACK];
   const int g = xcd_block(blockIdx.x, gridDim.x);
   if (g >= ngroups) return;
+  const size_t gbase = cand_base(goff, g, capg);
+  const int gcap = cand_cap(goff, g, capg);
   const int lane = threadIdx.x;
   const int t = g * GROUP + lane;
   double xl = PH_HUGE64, yl = PH_HUGE64, zl = PH_HUGE64, xh = -PH_HUGE64, yh = -PH_HUGE64, zh = -PH_HUGE64;
   double hm = 0.;
-  if (t < nact) {
+  if (t < nact && act[t] >= 0) {
     const int p = act[t];
     xl = xh = tx[p];
     yl = yh = ty[p];
@@ -163,7 +166,7 @@ k_group_walk(DevConst dc, int mode, double rfac, const phsph_kdnode *__restrict_
     const unsigned long long ml = __ballot(open && isleaf);
     if (open && isleaf) {
       const int k = nc + __popcll(ml & lanemask_lt());
-      if (k < capg) gcand[(size_t)g * capg + k] = (uint32_t)node;
+      if (k < gcap) gcand[gbase + k] = (uint32_t)node;
     }
     nc += __popcll(ml);
     const bool push = open && !isleaf;
@@ -183,7 +186,8 @@ k_group_walk(DevConst dc, int mode, double rfac, const phsph_kdnode *__restrict_
   }
   if (lane == 0) {
     gncand[g] = nc;
-    if (nc > capg || ovf) atomicOr(flags, DFL_CAND);
+    if (nc > gcap) atomicOr(flags, DFL_CAND);
+    if (ovf) atomicOr(flags, DFL_STACK);
   }
 }
 
@@ -286,7 +290,7 @@ template <int K>
 __global__ void __launch_bounds__(64)
 k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, int nact, int ngroups,
             const int32_t *__restrict__ pleaf, const int32_t *__restrict__ leaves,
-            const uint32_t *__restrict__ gcand, const int32_t *__restrict__ gncand, int capg,
+            const uint32_t *__restrict__ gcand, const int32_t *__restrict__ gncand, const int64_t *goff, int capg,
             const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi,
             const phsph_kdnode *__restrict__ nd, const double4 *__restrict__ tp, uint32_t *__restrict__ nlist,
             int capl, int32_t *__restrict__ t_nl, double *__restrict__ t_hsum, double *__restrict__ t_hfin,
@@ -297,7 +301,7 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
   if (g >= ngroups) return;
   const int lane = threadIdx.x;
   const int t = g * GROUP + lane;
-  const bool live = t < nact;
+  const bool live = t < nact && act[t] >= 0;  // act[t] < 0: padding of a partial group
   int p = -1, l = 0;
   double xi = 0., yi = 0., zi = 0., h0 = 1., lcx = 0., lcy = 0., lcz = 0., lsz = 0.;
   if (live) {
@@ -313,7 +317,7 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
   // --- neighbour list over the leaves this particle can reach ---
   int nl = 0;
   build_list(
-      S, dc, gcand + (size_t)g * capg, gncand[g], nd, nd_lo, nd_hi, (const double *)nullptr, tp, xi, yi, zi, lcx,
+      S, dc, gcand + cand_base(goff, g, capg), gncand[g], nd, nd_lo, nd_hi, (const double *)nullptr, tp, xi, yi, zi, lcx,
       lcy, lcz, lsz,
       [&](double ccx, double ccy, double ccz, double csz, double) {
         return live && leaf_in_reach(dc, xi, yi, zi, ccx, ccy, ccz, csz, rcut);
@@ -473,7 +477,7 @@ k_dens_final(DevConst dc, const int32_t *__restrict__ act, int nact, const uint3
   const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= nact) return;
   const int p = act[t];
-  if (retry[p]) return;
+  if (p < 0 || retry[p]) return;
   const double pm = dc.pmass;
   const double4 ri = in.tp[p];
   const double xi = ri.x, yi = ri.y, zi = ri.z;
@@ -560,7 +564,7 @@ k_dens_final_mixed(DevConst dc, const int32_t *__restrict__ act, int nact, const
   const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= nact) return;
   const int p = act[t];
-  if (retry[p]) return;
+  if (p < 0 || retry[p]) return;
   const double pm = dc.pmass;
   const double4 ri = tp[p];
   const double xi = ri.x, yi = ri.y, zi = ri.z;
@@ -736,8 +740,15 @@ int walk_groups(phsph_ctx *c, int mode, double rfac, const double *d_hsrc, const
   hipStream_t s = c->stream;
   const int ng = (c->nact + GROUP - 1) / GROUP;
   c->ngroups = ng;
-  for (int attempt = 0; attempt < 6; attempt++) {
-    PH_HIPCHECK(c, c->gcand.ensure((size_t)(ng + 1) * c->caps.gcand));
+  c->goff_p = nullptr;
+  while (c->caps.gcand > 16 && !list_fits(c, (size_t)(ng + 1) * c->caps.gcand)) c->caps.gcand /= 2;
+  // a uniform capacity per group, doubled (and kept) while it fits in the
+  // list budget; beyond that every group is walked again into a list sized
+  // from the counts of the previous walk
+  bool exact = false;
+  for (int attempt = 0; attempt < 12; attempt++) {
+    const int64_t *goff = exact ? c->goff.p : nullptr;
+    if (!exact) PH_HIPCHECK(c, c->gcand.ensure((size_t)(ng + 1) * c->caps.gcand));
     PH_HIPCHECK(c, c->gncand.ensure(ng + 1));
     int32_t zero = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
@@ -745,14 +756,38 @@ int walk_groups(phsph_ctx *c, int mode, double rfac, const double *d_hsrc, const
       LaunchTimer lt(c, mode == 0 ? "walk_density" : "walk_force");
       hipLaunchKernelGGL(k_group_walk, dim3(xcd_grid(ng)), dim3(64), 0, s, c->dc, mode, rfac, c->nd.p, d_hmaxn,
                          c->leaf_flag.p, c->act.p, c->nact, c->tx.p, c->ty.p, c->tz.p, d_hsrc, ng, c->gcand.p,
-                         c->gncand.p, c->caps.gcand, c->counters.p);
+                         c->gncand.p, goff, c->caps.gcand, c->counters.p);
     }
     PH_HIPCHECK(c, hipGetLastError());
     int32_t fl = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(&fl, c->counters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     PH_HIPCHECK(c, hipStreamSynchronize(s));
-    if (!(fl & DFL_CAND)) return PHSPH_OK;
-    c->caps.gcand *= 2;
+    if (getenv("PHSPH_DEBUG")) {
+      size_t fr = 0, tot = 0;
+      (void)hipMemGetInfo(&fr, &tot);
+      fprintf(stderr, "phsph walk: mode %d groups %d cap %d exact %d flags %d free %.1f GiB\n", mode, ng,
+              c->caps.gcand, (int)exact, fl, fr / 1073741824.0);
+    }
+    if (fl & DFL_STACK) PH_RET(c, PHSPH_ERR_CAPACITY, "neighbour walk: node stack exceeded");
+    if (!(fl & DFL_CAND)) {
+      c->goff_p = goff;
+      return PHSPH_OK;
+    }
+    if (exact) break;
+    if (list_fits(c, (size_t)(ng + 1) * c->caps.gcand * 2)) {
+      c->caps.gcand *= 2;
+      continue;
+    }
+    // exact offsets from the counts (gncand holds every group's full count)
+    exact = true;
+    PH_HIPCHECK(c, c->goff.ensure((size_t)ng + 1));
+    hipLaunchKernelGGL(k_scan_offsets<int32_t>, dim3(1), dim3(1024), 0, s, c->gncand.p, ng, (int64_t)1, 1,
+                       c->goff.p);
+    PH_HIPCHECK(c, hipGetLastError());
+    int64_t total = 0;
+    PH_HIPCHECK(c, hipMemcpyAsync(&total, c->goff.p + ng, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PH_HIPCHECK(c, hipStreamSynchronize(s));
+    PH_HIPCHECK(c, c->gcand.ensure((size_t)total + 1));
   }
   PH_RET(c, PHSPH_ERR_CAPACITY, "neighbour walk: candidate list capacity exceeded");
 }
@@ -769,6 +804,40 @@ int compact_active(phsph_ctx *c, const int32_t *flag, int n) {
   PH_HIPCHECK(c, hipMemcpyAsync(&na, c->counters.p + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   PH_HIPCHECK(c, hipStreamSynchronize(s));
   c->nact = na;
+  return PHSPH_OK;
+}
+
+// Groups for a sparse active set.  With every live particle active, 64
+// consecutive tree positions form a compact group.  When only some are (block
+// timesteps, retried particles), 64 consecutive active particles can lie far
+// apart and their group's candidate leaves cover the union of their
+// neighbourhoods; so a group is closed early where two consecutive active
+// particles are more than PAD_GAP tree positions apart, and the rest of its
+// 64 slots are padding (act = -1).  The padded length becomes c->nact.
+constexpr int PAD_GAP = 64;
+int pad_groups(phsph_ctx *c) {
+  const int na = c->nact;
+  if (na == 0 || na >= c->nlive) return PHSPH_OK;
+  hipStream_t s = c->stream;
+  std::vector<int32_t> a((size_t)na);
+  PH_HIPCHECK(c, hipMemcpyAsync(a.data(), c->act.p, sizeof(int32_t) * na, hipMemcpyDeviceToHost, s));
+  PH_HIPCHECK(c, hipStreamSynchronize(s));
+  std::vector<int32_t> &o = c->act_host;
+  o.clear();
+  int lane = 0;
+  for (int t = 0; t < na; t++) {
+    if (lane == GROUP || (lane > 0 && a[t] - a[t - 1] > PAD_GAP)) {
+      if (lane < GROUP) o.insert(o.end(), (size_t)(GROUP - lane), -1);
+      lane = 0;
+    }
+    o.push_back(a[t]);
+    lane++;
+  }
+  if (o.size() == (size_t)na) return PHSPH_OK;  // no group was closed early
+  PH_HIPCHECK(c, c->act.ensure(o.size() + 1));
+  PH_HIPCHECK(c, hipMemcpyAsync(c->act.p, o.data(), sizeof(int32_t) * o.size(), hipMemcpyHostToDevice, s));
+  PH_HIPCHECK(c, hipStreamSynchronize(s));
+  c->nact = (int)o.size();
   return PHSPH_OK;
 }
 
@@ -819,17 +888,24 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   // Particles whose h outgrows their neighbour list, or whose list does not
   // fit, are handed back and run again (alone) with a wider search or a
   // longer list.
-  PH_HIPCHECK(c, c->t_nl.ensure(n + 1));
-  PH_HIPCHECK(c, c->t_hsum.ensure(n + 1));
-  PH_HIPCHECK(c, c->t_hfin.ensure(n + 1));
   double rfac = 1.1;
+  rc = pad_groups(c);
+  if (rc) return rc;
+  // list capacity: it grows for the particles run again (the list memory is
+  // that of the retried particles); a grown capacity is kept for later calls
+  // while the lists of every active particle fit in the list budget
+  while (c->caps.dlist > 16 && !list_fits(c, (size_t)list_cap(c->caps.dlist) * nact)) c->caps.dlist /= 2;
+  int dlist = c->caps.dlist;
   for (int attempt = 0; attempt < 12; attempt++) {
-    const int na = c->nact;
+    const int na = c->nact;  // active slots, padding included
     if (na == 0) break;
+    PH_HIPCHECK(c, c->t_nl.ensure((size_t)na + 1));
+    PH_HIPCHECK(c, c->t_hsum.ensure((size_t)na + 1));
+    PH_HIPCHECK(c, c->t_hfin.ensure((size_t)na + 1));
     rc = walk_groups(c, 0, rfac, c->th_in.p, nullptr);
     if (rc) return rc;
     const int ng = c->ngroups;
-    PH_HIPCHECK(c, c->nlist.ensure((size_t)list_cap(c->caps.dlist) * na + 1));
+    PH_HIPCHECK(c, c->nlist.ensure((size_t)list_cap(dlist) * na + 1));
     hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(n, 256)), dim3(256), 0, s, c->tmpi.p, n, 0);
     int32_t zero = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
@@ -837,8 +913,8 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
       LaunchTimer lt(c, "density_iter");
 #define DITER(KK)                                                                                              \
   hipLaunchKernelGGL((k_dens_iter<KK>), dim3(xcd_grid(ng)), dim3(64), 0, s, c->dc, icall, rfac, c->act.p, na, ng, \
-                     c->pleaf.p, c->leaves.p, c->gcand.p, c->gncand.p, c->caps.gcand, c->nd_lo.p, c->nd_hi.p,     \
-                     c->nd.p, c->tp4.p, c->nlist.p, c->caps.dlist, c->t_nl.p, c->t_hsum.p, c->t_hfin.p,           \
+                     c->pleaf.p, c->leaves.p, c->gcand.p, c->gncand.p, c->goff_p, c->caps.gcand, c->nd_lo.p,      \
+                     c->nd_hi.p, c->nd.p, c->tp4.p, c->nlist.p, dlist, c->t_nl.p, c->t_hsum.p, c->t_hfin.p,       \
                      c->p_nits.p, c->leaf_nits.p, c->tmpi.p, grav ? c->htraj.p : nullptr,                       \
                      grav ? c->hconv.p : nullptr, c->counters.p)
       if (c->p.kernel == PHSPH_KERNEL_CUBIC) DITER(PHSPH_KERNEL_CUBIC); else DITER(PHSPH_KERNEL_QUINTIC);
@@ -868,9 +944,12 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
     int32_t fl = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(&fl, c->counters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     PH_HIPCHECK(c, hipStreamSynchronize(s));
-    if (dbg)
-      fprintf(stderr, "phsph densityiterate: attempt %d particles %d groups %d rfac %g list %d flags %d\n", attempt,
-              na, ng, rfac, c->caps.dlist, fl);
+    if (dbg) {
+      size_t fr = 0, tot = 0;
+      (void)hipMemGetInfo(&fr, &tot);
+      fprintf(stderr, "phsph densityiterate: attempt %d particles %d groups %d rfac %g list %d flags %d free %.1f GiB\n",
+              attempt, na, ng, rfac, dlist, fl, fr / 1073741824.0);
+    }
     if (fl & DFL_NOCONV) PH_RET(c, PHSPH_ERR_NOCONVERGE, "densityiterate: could not converge in density");
     if (fl & DFL_NEGH) PH_RET(c, PHSPH_ERR_INPUT, "densityiterate: setting negative h from hrho");
     if (!(fl & (DFL_REWALK | DFL_LIST))) {
@@ -878,11 +957,15 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
       break;
     }
     if (fl & DFL_REWALK) rfac *= 1.5;
-    if (fl & DFL_LIST) c->caps.dlist *= 2;
+    if (fl & DFL_LIST) dlist *= 2;
     rc = compact_active(c, c->tmpi.p, n);
+    if (rc) return rc;
+    rc = pad_groups(c);
     if (rc) return rc;
   }
   if (c->nact != 0) PH_RET(c, PHSPH_ERR_CAPACITY, "densityiterate: neighbour search could not be widened enough");
+  while (dlist > c->caps.dlist && !list_fits(c, (size_t)list_cap(dlist) * nact)) dlist /= 2;
+  if (dlist > c->caps.dlist) c->caps.dlist = dlist;
   if (grav) {
     rc = grav_hmax_replay(c, icall);
     if (rc) return rc;

@@ -118,15 +118,16 @@ __global__ void k_force_prep(DevConst dc, int nlive, const int32_t *__restrict__
 __global__ void __launch_bounds__(64)
 k_flist(DevConst dc, const int32_t *__restrict__ act, int nact, int ngroups, const int32_t *__restrict__ pleaf,
         const int32_t *__restrict__ leaves, const uint32_t *__restrict__ gcand, const int32_t *__restrict__ gncand,
-        int capg, const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi,
+        const int64_t *goff, int capg, const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi,
         const phsph_kdnode *__restrict__ nd, const double *__restrict__ hmaxn, const double4 *__restrict__ fpos,
-        uint32_t *__restrict__ nlist, int capl, int32_t *__restrict__ t_nl, int32_t *__restrict__ flags) {
+        uint32_t *__restrict__ nlist, const int64_t *loff, int ucap, int32_t *__restrict__ t_nl,
+        int32_t *__restrict__ flags) {
   __shared__ ListStage S;
   const int g = xcd_block(blockIdx.x, gridDim.x);
   if (g >= ngroups) return;
   const int lane = threadIdx.x;
   const int t = g * GROUP + lane;
-  const bool live = t < nact;
+  const bool live = t < nact && act[t] >= 0;  // act[t] < 0: padding of a partial group
   const double R2 = dc.radkern2;
   int p = -1;
   double xi = 0., yi = 0., zi = 0., hi = 0., hi1 = 1., lcx = 0., lcy = 0., lcz = 0., lsz = 0.;
@@ -140,21 +141,32 @@ k_flist(DevConst dc, const int32_t *__restrict__ act, int nact, int ngroups, con
     lcx = nd[c].xcen[0]; lcy = nd[c].xcen[1]; lcz = nd[c].xcen[2]; lsz = nd[c].size;
   }
   const double hi21 = xmul(hi1, hi1);
+  const int capl = list_cap_g(loff, g * GROUP, ucap);
   int nl = 0;
   build_list(
-      S, dc, gcand + (size_t)g * capg, gncand[g], nd, nd_lo, nd_hi, hmaxn, fpos, xi, yi, zi, lcx, lcy, lcz, lsz,
+      S, dc, gcand + cand_base(goff, g, capg), gncand[g], nd, nd_lo, nd_hi, hmaxn, fpos, xi, yi, zi, lcx, lcy, lcz,
+      lsz,
       [&](double ccx, double ccy, double ccz, double csz, double chm) {
         return live && leaf_in_reach(dc, xi, yi, zi, ccx, ccy, ccz, csz, dc.radkern * fmax(hi, chm));
       },
       [&](int j, int code, double r2, double hj1) {
         if (j != p && (xmul(r2, hi21) < R2 || xmul(r2, xmul(hj1, hj1)) < R2)) {
-          if (nl < capl) nlist[list_at(nact, t, nl)] = cand_pack(j, code);
+          if (nl < capl) nlist[list_at_g(loff, nact, t, nl)] = cand_pack(j, code);
           nl++;
         }
       });
+  if (t < nact) t_nl[t] = nl;  // 0 for padding
   if (!live) return;
-  t_nl[t] = nl;
   if (nl > capl) atomicOr(flags, FFL_LIST);
+}
+
+// longest list of each group of 64 slots (t_nl over the group's live slots)
+__global__ void k_group_max(const int32_t *__restrict__ t_nl, int nact, int ngroups, int32_t *__restrict__ gmax) {
+  const int g = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int t = g * GROUP + (threadIdx.x & 63);
+  int v = (g < ngroups && t < nact) ? t_nl[t] : 0;
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  if (g < ngroups && (threadIdx.x & 63) == 0) gmax[g] = v;
 }
 
 // Force, part 2: the pair terms (force.F90:1227-2056) over each particle's
@@ -162,12 +174,12 @@ k_flist(DevConst dc, const int32_t *__restrict__ act, int nact, int ngroups, con
 template <int K, bool MHD, bool DUST, bool THERMAL>
 __global__ void __launch_bounds__(256)
 k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const uint32_t *__restrict__ nlist,
-        const int32_t *__restrict__ t_nl, const int32_t *__restrict__ iparts, const FRec *__restrict__ rec,
+        const int64_t *loff, const int32_t *__restrict__ t_nl, const int32_t *__restrict__ iparts, const FRec *__restrict__ rec,
         const FRecB *__restrict__ recB, const FRecD *__restrict__ recD, ForceIOArgs out,
         const int8_t *__restrict__ tbin_old, int icall, unsigned long long *__restrict__ dtred,
         int32_t *__restrict__ flags) {
   const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const bool live = t < nact;
+  const bool live = t < nact && act[t] >= 0;
   const int p = live ? act[t] : act[0];
   const double pm = dc.pmass;
   const double R2 = dc.radkern2;
@@ -225,7 +237,7 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const uint32_t *
   const int8_t ibinnow_m1 = (int8_t)(out.ibinnow - 1);
   const int nl = live ? t_nl[t] : 0;
   for (int k = 0; k < nl; k++) {
-    const uint32_t e = nlist[list_at(nact, t, k)];
+    const uint32_t e = nlist[list_at_g(loff, nact, t, k)];
     const int j = cand_node(e);
     const FRec &Rj = rec[j];
     if (ind) {
@@ -450,12 +462,12 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const uint32_t *
 template <int K, bool THERMAL>
 __global__ void __launch_bounds__(256)
 k_force_mixed(DevConst dc, const int32_t *__restrict__ act, int nact, const uint32_t *__restrict__ nlist,
-              const int32_t *__restrict__ t_nl, const int32_t *__restrict__ iparts, const FRec *__restrict__ rec,
+              const int64_t *loff, const int32_t *__restrict__ t_nl, const int32_t *__restrict__ iparts, const FRec *__restrict__ rec,
               const double4 *__restrict__ fpos, const double4 *__restrict__ fvel, const FRec32 *__restrict__ rec32,
               ForceIOArgs out, const int8_t *__restrict__ tbin_old, int icall,
               unsigned long long *__restrict__ dtred, int32_t *__restrict__ flags) {
   const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const bool live = t < nact;
+  const bool live = t < nact && act[t] >= 0;
   const int p = live ? act[t] : act[0];
   const double pm = dc.pmass;
   const double R2 = dc.radkern2;
@@ -486,7 +498,7 @@ k_force_mixed(DevConst dc, const int32_t *__restrict__ act, int nact, const uint
   const int8_t ibinnow_m1 = (int8_t)(out.ibinnow - 1);
   const int nl = live ? t_nl[t] : 0;
   for (int k = 0; k < nl; k++) {
-    const uint32_t e = nlist[list_at(nact, t, k)];
+    const uint32_t e = nlist[list_at_g(loff, nact, t, k)];
     const int j = cand_node(e);
     const double4 pj = fpos[j];
     if (ind) {  // force.F90:1346-1352
@@ -725,8 +737,10 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
                      c->tmpi.p);
   int rc = compact_active(c, c->tmpi.p, n);
   if (rc) return rc;
-  const int nact = c->nact;
   if (c->p.gravity) return grav_force(c, icall, a, rec, c->gsoft.p, dtc_out, dtf_out);
+  rc = pad_groups(c);
+  if (rc) return rc;
+  const int nact = c->nact;  // active slots, padding included
   // node hmax of the final smoothing lengths for the h_j search radius
   PH_HIPCHECK(c, c->hmaxf.ensure(c->ncells + 2));
   hipLaunchKernelGGL(k_zero_u64, dim3(nblocks(c->ncells + 2, 256)), dim3(256), 0, s,
@@ -737,25 +751,47 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
   if (rc) return rc;
   const int ng = c->ngroups;
   PH_HIPCHECK(c, c->t_nl.ensure(nact + 1));
-  // neighbour lists; a list that does not fit grows the capacity and the
-  // lists are built again (the pair pass below runs once)
-  for (int attempt = 0;; attempt++) {
-    PH_HIPCHECK(c, c->nlist.ensure((size_t)list_cap(c->caps.flist) * nact + 1));
+  // neighbour lists at a uniform capacity, doubled (and kept) while the lists
+  // of every active particle fit in the list budget; beyond that the lists
+  // are built again with a capacity per group of 64 (its longest list), so
+  // that a few particles with many neighbours do not size the whole buffer.
+  // The pair pass below runs once.
+  const int64_t *loff = nullptr;
+  while (c->caps.flist > 16 && !list_fits(c, (size_t)list_cap(c->caps.flist) * nact)) c->caps.flist /= 2;
+  for (int attempt = 0; attempt < 12; attempt++) {
+    const int ucap = list_cap(c->caps.flist);
+    if (!loff) PH_HIPCHECK(c, c->nlist.ensure((size_t)ucap * nact + 1));
     int32_t zero = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
     if (nact > 0) {
       LaunchTimer lt(c, "force_list");
       hipLaunchKernelGGL(k_flist, dim3(xcd_grid(ng)), dim3(64), 0, s, c->dc, c->act.p, nact, ng, c->pleaf.p,
-                         c->leaves.p, c->gcand.p, c->gncand.p, c->caps.gcand, c->nd_lo.p, c->nd_hi.p, c->nd.p,
-                         c->hmaxf.p, c->fpos.p, c->nlist.p, c->caps.flist, c->t_nl.p, c->counters.p);
+                         c->leaves.p, c->gcand.p, c->gncand.p, c->goff_p, c->caps.gcand, c->nd_lo.p, c->nd_hi.p,
+                         c->nd.p, c->hmaxf.p, c->fpos.p, c->nlist.p, loff, ucap, c->t_nl.p, c->counters.p);
     }
     PH_HIPCHECK(c, hipGetLastError());
     int32_t fl = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(&fl, c->counters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     PH_HIPCHECK(c, hipStreamSynchronize(s));
     if (!(fl & FFL_LIST)) break;
-    if (attempt >= 6) PH_RET(c, PHSPH_ERR_CAPACITY, "force: neighbour list capacity exceeded");
-    c->caps.flist *= 2;
+    if (loff) PH_RET(c, PHSPH_ERR_CAPACITY, "force: neighbour list capacity exceeded");
+    if (list_fits(c, (size_t)list_cap(2 * c->caps.flist) * nact)) {  // a larger uniform capacity, kept
+      c->caps.flist *= 2;
+      continue;
+    }
+    // per-group capacity: the group's longest list rounded to 16, times 64 slots
+    PH_HIPCHECK(c, c->gmax.ensure((size_t)ng + 1));
+    PH_HIPCHECK(c, c->loff.ensure((size_t)ng + 1));
+    hipLaunchKernelGGL(k_group_max, dim3(nblocks((size_t)ng * 64, 256)), dim3(256), 0, s, c->t_nl.p, nact, ng,
+                       c->gmax.p);
+    hipLaunchKernelGGL(k_scan_offsets<int32_t>, dim3(1), dim3(1024), 0, s, c->gmax.p, ng, (int64_t)GROUP, LSEG,
+                       c->loff.p);
+    PH_HIPCHECK(c, hipGetLastError());
+    int64_t total = 0;
+    PH_HIPCHECK(c, hipMemcpyAsync(&total, c->loff.p + ng, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PH_HIPCHECK(c, hipStreamSynchronize(s));
+    PH_HIPCHECK(c, c->nlist.ensure((size_t)total + 1));
+    loff = c->loff.p;
   }
   PH_HIPCHECK(c, c->dtred.ensure(3));
   {
@@ -772,7 +808,7 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
       const dim3 gr(xcd_grid(nblocks(nact, 256))), b(256);
       const bool th = c->p.maxvxyzu >= 4;
 #define FL(KK, M, D, T)                                                                                        \
-  hipLaunchKernelGGL((k_force<KK, M, D, T>), gr, b, 0, s, c->dc, c->act.p, nact, c->nlist.p, c->t_nl.p,         \
+  hipLaunchKernelGGL((k_force<KK, M, D, T>), gr, b, 0, s, c->dc, c->act.p, nact, c->nlist.p, loff, c->t_nl.p,   \
                      c->iparts.p, rec, recB, recD, a, c->tbin_old.p, icall, c->dtred.p, c->counters.p)
 #define FLK(KK)                                          \
   if (mhd && dust) { if (th) FL(KK, true, true, true); else FL(KK, true, true, false); }     \
@@ -780,7 +816,7 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
   else if (dust) { if (th) FL(KK, false, true, true); else FL(KK, false, true, false); }     \
   else { if (th) FL(KK, false, false, true); else FL(KK, false, false, false); }
 #define FM(KK, T)                                                                                              \
-  hipLaunchKernelGGL((k_force_mixed<KK, T>), gr, b, 0, s, c->dc, c->act.p, nact, c->nlist.p, c->t_nl.p,          \
+  hipLaunchKernelGGL((k_force_mixed<KK, T>), gr, b, 0, s, c->dc, c->act.p, nact, c->nlist.p, loff, c->t_nl.p,    \
                      c->iparts.p, rec, c->fpos.p, c->fvel.p, rec32, a, c->tbin_old.p, icall, c->dtred.p,          \
                      c->counters.p)
       if (mixed) {

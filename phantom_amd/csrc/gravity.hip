@@ -220,7 +220,7 @@ __global__ void __launch_bounds__(64)
 k_grav_walk(DevConst dc, int nleaves, const int32_t *__restrict__ leaves, const int32_t *__restrict__ leaf_flag,
             const phsph_kdnode *__restrict__ nd, const unsigned long long *__restrict__ ghmax,
             const double *__restrict__ quads, double *__restrict__ fnode_out, uint32_t *__restrict__ p2p,
-            int32_t *__restrict__ np2p, int capp, int32_t *__restrict__ flags) {
+            int32_t *__restrict__ np2p, const int64_t *poff, int ucapp, int32_t *__restrict__ flags) {
   __shared__ uint32_t stk[GSTACK];       // source id << 6 | level
   __shared__ double fb[GMAXDEPTH + 1][LENFGRAV];
   __shared__ int branch[GMAXDEPTH + 1];
@@ -232,6 +232,9 @@ k_grav_walk(DevConst dc, int nleaves, const int32_t *__restrict__ leaves, const 
     if (threadIdx.x == 0) np2p[l] = 0;
     return;
   }
+  // near-field list of this leaf: uniform capacity, or exact offsets
+  const size_t pbase = poff ? (size_t)poff[l] : (size_t)l * ucapp;
+  const int capp = poff ? (int)(poff[l + 1] - poff[l]) : ucapp;
   const int lane = threadIdx.x;
   for (int k = lane; k < (GMAXDEPTH + 1) * LENFGRAV; k += 64) (&fb[0][0])[k] = 0.;
   if (lane == 0) {
@@ -312,7 +315,7 @@ k_grav_walk(DevConst dc, int nleaves, const int32_t *__restrict__ leaves, const 
       const int k = nc + __popcll(mp & lanemask_lt());
       int sc = SEP_PLAIN;
       if (dc.periodic) sc = (code == SEP_PLAIN) ? SEP_WRAP0 : code;
-      if (k < capp) p2p[(size_t)l * capp + k] = cand_pack(src, sc);
+      if (k < capp) p2p[pbase + k] = cand_pack(src, sc);
     }
     nc += __popcll(mp);
     // pushes: exclusive scan of the per-lane counts
@@ -359,6 +362,7 @@ struct GravArgs {
   const double *fnode;     // 20 coefficients per leaf (leaf index)
   const uint32_t *p2p;     // near-field leaves per leaf
   const int32_t *np2p;
+  const int64_t *poff;     // per-leaf offsets into p2p, or null: l*capp
   int capp;
   const float *gsoft;      // gradh(2) in tree order
   float *poten;            // caller order (may be null)
@@ -402,9 +406,10 @@ k_force_grav(DevConst dc, const int32_t *__restrict__ act, int nact, const int32
     const bool ind = out.ibin != nullptr;
     int ibin_neighi = 0;
     const int8_t ibinnow_m1 = (int8_t)(out.ibinnow - 1);
-    const int ne = min(G.np2p[l], G.capp);
+    const size_t pbase = G.poff ? (size_t)G.poff[l] : (size_t)l * G.capp;
+    const int ne = min(G.np2p[l], G.poff ? (int)(G.poff[l + 1] - G.poff[l]) : G.capp);
     for (int e = 0; e < ne; e++) {
-      const uint32_t en = G.p2p[(size_t)l * G.capp + e];
+      const uint32_t en = G.p2p[pbase + e];
       const int cn = cand_node(en), code = cand_code(en);
       const int j1 = nd_hi[cn];
       for (int j = nd_lo[cn]; j <= j1; j++) {
@@ -599,14 +604,20 @@ int grav_force(phsph_ctx *c, int icall, const ForceIOArgs &a, const FRec *rec, c
   const int nl = c->nleaves, nact = c->nact;
   PH_HIPCHECK(c, c->fnode.ensure((size_t)(nl + 1) * LENFGRAV));
   PH_HIPCHECK(c, c->np2p.ensure(nl + 1));
-  for (int attempt = 0;; attempt++) {
-    PH_HIPCHECK(c, c->p2p.ensure((size_t)(nl + 1) * c->caps.p2p));
+  // near-field lists at a uniform capacity, doubled (and kept) while it fits
+  // in the list budget; beyond that the walk runs again with each leaf's list
+  // sized from its count
+  const int64_t *poff = nullptr;
+  while (c->caps.p2p > 16 && !list_fits(c, (size_t)(nl + 1) * c->caps.p2p)) c->caps.p2p /= 2;
+  for (int attempt = 0; attempt < 12; attempt++) {
+    if (!poff) PH_HIPCHECK(c, c->p2p.ensure((size_t)(nl + 1) * c->caps.p2p));
     int32_t zero = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
     if (nl > 0) {
       LaunchTimer lt(c, "grav_walk");
       hipLaunchKernelGGL(k_grav_walk, dim3(nl), dim3(64), 0, s, c->dc, nl, c->leaves.p, c->leaf_flag.p, c->nd.p,
-                         c->ghmax.p, c->quads.p, c->fnode.p, c->p2p.p, c->np2p.p, c->caps.p2p, c->counters.p);
+                         c->ghmax.p, c->quads.p, c->fnode.p, c->p2p.p, c->np2p.p, poff, c->caps.p2p,
+                         c->counters.p);
     }
     PH_HIPCHECK(c, hipGetLastError());
     int32_t fl = 0;
@@ -615,8 +626,19 @@ int grav_force(phsph_ctx *c, int icall, const ForceIOArgs &a, const FRec *rec, c
     if (fl & GFL_DEPTH) PH_RET(c, PHSPH_ERR_CAPACITY, "force: tree deeper than 32 levels for the gravity walk");
     if (fl & GFL_STACK) PH_RET(c, PHSPH_ERR_CAPACITY, "force: gravity walk stack overflow");
     if (!(fl & GFL_P2P)) break;
-    if (attempt >= 5) PH_RET(c, PHSPH_ERR_CAPACITY, "force: near-field list capacity exceeded");
-    c->caps.p2p *= 2;
+    if (poff) PH_RET(c, PHSPH_ERR_CAPACITY, "force: near-field list capacity exceeded");
+    if (list_fits(c, (size_t)(nl + 1) * c->caps.p2p * 2)) {  // a larger uniform capacity, kept
+      c->caps.p2p *= 2;
+      continue;
+    }
+    PH_HIPCHECK(c, c->poff.ensure((size_t)nl + 1));
+    hipLaunchKernelGGL(k_scan_offsets<int32_t>, dim3(1), dim3(1024), 0, s, c->np2p.p, nl, (int64_t)1, 1, c->poff.p);
+    PH_HIPCHECK(c, hipGetLastError());
+    int64_t total = 0;
+    PH_HIPCHECK(c, hipMemcpyAsync(&total, c->poff.p + nl, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PH_HIPCHECK(c, hipStreamSynchronize(s));
+    PH_HIPCHECK(c, c->p2p.ensure((size_t)total + 1));
+    poff = c->poff.p;
   }
   PH_HIPCHECK(c, c->dtred.ensure(3));
   unsigned long long init[3];
@@ -627,7 +649,7 @@ int grav_force(phsph_ctx *c, int icall, const ForceIOArgs &a, const FRec *rec, c
   PH_HIPCHECK(c, hipMemcpyAsync(c->dtred.p, init, sizeof(init), hipMemcpyHostToDevice, s));
   int32_t zero2[2] = {0, 0};
   PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, zero2, sizeof(zero2), hipMemcpyHostToDevice, s));
-  GravArgs G{c->fnode.p, c->p2p.p, c->np2p.p, c->caps.p2p, gsoft, c->poten_out};
+  GravArgs G{c->fnode.p, c->p2p.p, c->np2p.p, poff, c->caps.p2p, gsoft, c->poten_out};
   if (nact > 0) {
     LaunchTimer lt(c, "force");
     const dim3 gr(nblocks(nact, 128)), b(128);

@@ -26,7 +26,7 @@
 #include "ctx.h"
 
 constexpr int GROUP = 64;                         // particles per group = wavefront size
-constexpr int WALK_STACK = 2048;                  // node stack of the group walk (LDS)
+constexpr int WALK_STACK = 4096;                  // node stack of the group walk (LDS)
 constexpr int HTRAJ = 16;                         // h after each density iteration kept per particle (self-gravity)
 
 // separation code of a candidate leaf for a lane: 0..26 periodic offset
@@ -101,6 +101,9 @@ __device__ __forceinline__ void sep_of(const DevConst &dc, int code, double xi, 
 }
 
 // host side (density.hip)
+// Candidate leaves of each group of 64 active particles, into c->gcand at
+// c->goff_p (exact per-group offsets after an overflow, else nullptr and the
+// uniform capacity c->caps.gcand).
 int walk_groups(phsph_ctx *c, int mode, double rfac, const double *d_hsrc, const double *d_hmaxn);
 
 // Neighbour lists: entry k of active slot t lives in segment k/16 of that
@@ -112,6 +115,63 @@ constexpr int LSEG = 16;
 __device__ __forceinline__ size_t list_at(int nact, int t, int k) {
   return ((size_t)(k / LSEG) * (size_t)nact + (size_t)t) * LSEG + (size_t)(k % LSEG);
 }
+// The same lists with a capacity per group of 64 slots, for lengths that vary
+// strongly between groups: group g owns entries off[g] .. off[g+1]-1 (a
+// multiple of 64*16), its slots' segments interleaved as above.  With
+// off == nullptr every slot has the uniform capacity of list_at.
+__device__ __forceinline__ size_t list_at_g(const int64_t *off, int nact, int t, int k) {
+  if (!off) return list_at(nact, t, k);
+  return (size_t)off[t / GROUP] + ((size_t)(k / LSEG) * GROUP + (size_t)(t % GROUP)) * LSEG + (size_t)(k % LSEG);
+}
+__device__ __forceinline__ int list_cap_g(const int64_t *off, int t, int ucap) {
+  return off ? (int)((off[t / GROUP + 1] - off[t / GROUP]) / GROUP) : ucap;
+}
+// candidate leaves of group g: uniform capacity capg, or exact offsets
+__device__ __forceinline__ size_t cand_base(const int64_t *goff, int g, int capg) {
+  return goff ? (size_t)goff[g] : (size_t)g * capg;
+}
+__device__ __forceinline__ int cand_cap(const int64_t *goff, int g, int capg) {
+  return goff ? (int)(goff[g + 1] - goff[g]) : capg;
+}
+
+// Device memory a uniform list capacity may take (phsph_ctx::list_budget,
+// 16 GiB by default) before the lists switch to exact per-group sizes;
+// uniform capacities grow by doubling and are kept for later calls while
+// they fit.
+static inline bool list_fits(const phsph_ctx *c, size_t entries) { return entries * sizeof(uint32_t) <= c->list_budget; }
+
+// exclusive prefix sum of n counts into off[0..n] (off[n] = total), one block
+template <class T>
+__global__ void __launch_bounds__(1024) k_scan_offsets(const T *__restrict__ cnt, int n, int64_t scale, int round,
+                                                       int64_t *__restrict__ off) {
+  __shared__ int64_t part[1024];
+  __shared__ int64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += 1024) {
+    const int i = base + threadIdx.x;
+    int64_t v = 0;
+    if (i < n) {
+      int64_t c = (int64_t)cnt[i];
+      if (round > 1) c = (c + round - 1) / round * round;
+      v = c * scale;
+    }
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const int64_t a = (threadIdx.x >= (unsigned)o) ? part[threadIdx.x - o] : 0;
+      __syncthreads();
+      part[threadIdx.x] += a;
+      __syncthreads();
+    }
+    if (i < n) off[i] = carry + part[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += part[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) off[n] = carry;
+}
+
 static inline int list_cap(int cap) { return (cap + LSEG - 1) / LSEG * LSEG; }
 
 // Neighbour-list building over a group's candidate leaves, CL leaves at a

@@ -6,6 +6,8 @@
 !   phsph_densityiterate -> densityiterate (src/main/dens.F90:117)        !
 !   phsph_cons2prim      -> cons2prim_everything (cons2prim.f90:274)      !
 !   phsph_force          -> force          (src/main/force.F90:193)       !
+!   phsph_step_*, phsph_substep_sph -> the particle loops of step         !
+!                          (step_leapfrog.f90:150-696, substepping.F90:241)!
 ! Implicit module inputs of the reference (massoftype, hfact, tolh, the   !
 ! shock-capturing and timestep options, the box) travel in phsph_params, !
 ! set once with phsph_init.  A non-zero status from the library is passed !
@@ -43,6 +45,8 @@ module phsph
  public :: phsph_maketree, phsph_densityiterate, phsph_cons2prim, phsph_force
  public :: phsph_neighbour_stats
  public :: phsph_set_bins, phsph_get_nbinmax, phsph_set_active_particles, phsph_set_poten
+ public :: phsph_init_step, phsph_step_predictor, phsph_substep_sph, phsph_step_predict_sph
+ public :: phsph_step_corrector, phsph_step_reset
 
  interface
     subroutine c_default_params(p, kernel) bind(c, name='phsph_default_params')
@@ -122,9 +126,73 @@ module phsph
        integer(c_int32_t), value :: np, ldx, nbinmax, istepfrac
        integer(c_int32_t), intent(out) :: nactive, nalive, ibinnow
     end function
+    integer(c_int) function c_init_step(ctx, np, time, ibin, nbinmax, twas) bind(c, name='phsph_init_step')
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx, ibin, twas
+       integer(c_int32_t), value :: np, nbinmax
+       real(c_double), value :: time
+    end function
+    integer(c_int) function c_step_predictor(ctx, np, t, dtsph, xyzh, ldx, vxyzu, ldv, fxyzu, ldf, Bevol, dBevol, &
+                                             dustevol, ddustevol, iphase, ibin, ibin_old, twas, ufloor, nvfloor) &
+                                             bind(c, name='phsph_step_predictor')
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx, xyzh, vxyzu, fxyzu, Bevol, dBevol, dustevol, ddustevol, iphase, ibin, ibin_old, twas
+       integer(c_int32_t), value :: np, ldx, ldv, ldf
+       real(c_double), value :: t, dtsph, ufloor
+       integer(c_int32_t), intent(out) :: nvfloor
+    end function
+    integer(c_int) function c_substep_sph(ctx, np, dt, xyzh, ldx, vxyzu, ldv) bind(c, name='phsph_substep_sph')
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx, xyzh, vxyzu
+       integer(c_int32_t), value :: np, ldx, ldv
+       real(c_double), value :: dt
+    end function
+    integer(c_int) function c_step_predict_sph(ctx, np, timei, dtsph, xyzh, ldx, vxyzu, ldv, fxyzu, ldf, vpred, &
+                                               Bevol, dBevol, Bpred, dustevol, ddustevol, dustpred, divcurlv, &
+                                               lddcv, eos_vars, ldeos, alphaind, iphase, twas, ufloor, nvfloor) &
+                                               bind(c, name='phsph_step_predict_sph')
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx, xyzh, vxyzu, fxyzu, vpred, Bevol, dBevol, Bpred, dustevol, ddustevol, dustpred
+       type(c_ptr), value :: divcurlv, eos_vars, alphaind, iphase, twas
+       integer(c_int32_t), value :: np, ldx, ldv, ldf, lddcv, ldeos
+       real(c_double), value :: timei, dtsph, ufloor
+       integer(c_int32_t), intent(out) :: nvfloor
+    end function
+    integer(c_int) function c_step_corrector(ctx, np, timei, dtsph, xyzh, ldx, vxyzu, ldv, fxyzu, ldf, vpred, &
+                                             Bevol, dBevol, dustevol, ddustevol, iphase, ibin, ibin_wake, nbinmax, &
+                                             twas, ufloor, errmax, v2sum, npc, nwake, nvfloor) &
+                                             bind(c, name='phsph_step_corrector')
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx, xyzh, vxyzu, fxyzu, vpred, Bevol, dBevol, dustevol, ddustevol, iphase
+       type(c_ptr), value :: ibin, ibin_wake, twas
+       integer(c_int32_t), value :: np, ldx, ldv, ldf, nbinmax
+       real(c_double), value :: timei, dtsph, ufloor
+       real(c_double), intent(out) :: errmax, v2sum
+       integer(c_int32_t), intent(out) :: npc, nwake, nvfloor
+    end function
+    integer(c_int) function c_step_reset(ctx, np, dtsph, vxyzu, ldv, fxyzu, ldf, vpred, Bevol, dBevol, Bpred, &
+                                         dustevol, ddustevol, dustpred, iphase) bind(c, name='phsph_step_reset')
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx, vxyzu, fxyzu, vpred, Bevol, dBevol, Bpred, dustevol, ddustevol, dustpred, iphase
+       integer(c_int32_t), value :: np, ldv, ldf
+       real(c_double), value :: dtsph
+    end function
  end interface
 
 contains
+
+!-- c_loc of an array, or NULL for a zero-size (switched-off) one
+type(c_ptr) function ptr_or_null(a) result(pp)
+ real(c_double), intent(in), target :: a(..)
+ pp = c_null_ptr
+ if (size(a) > 0) pp = c_loc(a)
+end function ptr_or_null
+
+type(c_ptr) function ptr_or_null_i1(a) result(pp)
+ integer(c_int8_t), intent(in), target :: a(:)
+ pp = c_null_ptr
+ if (size(a) > 0) pp = c_loc(a)
+end function ptr_or_null_i1
 
 !-- reference defaults for a kernel (options.f90, shock_capturing.f90, timestep.f90)
 function phsph_default(kernel) result(p)
@@ -284,5 +352,106 @@ integer function phsph_set_active_particles(npart, xyzh, iphase, ibin, nbinmax, 
                      c_loc(ibin), int(nbinmax, c_int32_t), int(istepfrac, c_int32_t), na, nl, bn)
  nactive = na; nalive = nl; ibinnow = int(bn, kind=1)
 end function phsph_set_active_particles
+
+!-- init_step (step_leapfrog.f90:44): twas = time + dt(ibin)/2 (block timesteps)
+integer function phsph_init_step(npart, time, ibin, nbinmax, twas) result(ierr)
+ integer,           intent(in)            :: npart, nbinmax
+ real(c_double),    intent(in)            :: time
+ integer(c_int8_t), intent(inout), target :: ibin(:)
+ real(c_double),    intent(inout), target :: twas(:)
+ ierr = c_init_step(ctx, int(npart, c_int32_t), time, c_loc(ibin), int(nbinmax, c_int32_t), c_loc(twas))
+end function phsph_init_step
+
+!-- predictor loop of step (step_leapfrog.f90:150-237).  Zero-size arrays
+!   stand for physics that is off (Bevol/dBevol for MHD, dustevol/ddustevol
+!   for dust, ibin/ibin_old/twas for block timesteps)
+integer function phsph_step_predictor(npart, t, dtsph, xyzh, vxyzu, fxyzu, Bevol, dBevol, dustevol, ddustevol, &
+                                      iphase, ibin, ibin_old, twas, ufloor, nvfloor) result(ierr)
+ integer,           intent(in)            :: npart
+ real(c_double),    intent(in)            :: t, dtsph, ufloor
+ real(c_double),    intent(in),    target :: xyzh(:,:), fxyzu(:,:), dBevol(:,:), ddustevol(:,:), twas(:)
+ real(c_double),    intent(inout), target :: vxyzu(:,:), Bevol(:,:), dustevol(:,:)
+ integer(c_int8_t), intent(in),    target :: iphase(:), ibin(:)
+ integer(c_int8_t), intent(inout), target :: ibin_old(:)
+ integer,           intent(out)           :: nvfloor
+ integer(c_int32_t) :: nf
+ ierr = c_step_predictor(ctx, int(npart, c_int32_t), t, dtsph, c_loc(xyzh), int(size(xyzh,1), c_int32_t), &
+                         c_loc(vxyzu), int(size(vxyzu,1), c_int32_t), c_loc(fxyzu), int(size(fxyzu,1), c_int32_t), &
+                         ptr_or_null(Bevol), ptr_or_null(dBevol), ptr_or_null(dustevol), ptr_or_null(ddustevol), &
+                         ptr_or_null_i1(iphase), ptr_or_null_i1(ibin), ptr_or_null_i1(ibin_old), ptr_or_null(twas), &
+                         ufloor, nf)
+ nvfloor = nf
+end function phsph_step_predictor
+
+!-- substep_sph (substepping.F90:241): x += dt*v
+integer function phsph_substep_sph(dt, npart, xyzh, vxyzu) result(ierr)
+ real(c_double), intent(in)            :: dt
+ integer,        intent(in)            :: npart
+ real(c_double), intent(inout), target :: xyzh(:,:)
+ real(c_double), intent(in),    target :: vxyzu(:,:)
+ ierr = c_substep_sph(ctx, int(npart, c_int32_t), dt, c_loc(xyzh), int(size(xyzh,1), c_int32_t), c_loc(vxyzu), &
+                      int(size(vxyzu,1), c_int32_t))
+end function phsph_substep_sph
+
+!-- predict_sph loop of step (step_leapfrog.f90:307-400)
+integer function phsph_step_predict_sph(npart, timei, dtsph, xyzh, vxyzu, fxyzu, vpred, Bevol, dBevol, Bpred, &
+                                        dustevol, ddustevol, dustpred, divcurlv, eos_vars, alphaind, iphase, &
+                                        twas, ufloor, nvfloor) result(ierr)
+ integer,           intent(in)            :: npart
+ real(c_double),    intent(in)            :: timei, dtsph, ufloor
+ real(c_double),    intent(inout), target :: xyzh(:,:), vpred(:,:), Bpred(:,:), dustpred(:,:)
+ real(c_double),    intent(in),    target :: vxyzu(:,:), fxyzu(:,:), Bevol(:,:), dBevol(:,:), dustevol(:,:)
+ real(c_double),    intent(in),    target :: ddustevol(:,:), eos_vars(:,:), twas(:)
+ real(c_float),     intent(in),    target :: divcurlv(:,:)
+ real(c_float),     intent(inout), target :: alphaind(:,:)
+ integer(c_int8_t), intent(in),    target :: iphase(:)
+ integer,           intent(out)           :: nvfloor
+ integer(c_int32_t) :: nf
+ ierr = c_step_predict_sph(ctx, int(npart, c_int32_t), timei, dtsph, c_loc(xyzh), int(size(xyzh,1), c_int32_t), &
+                           c_loc(vxyzu), int(size(vxyzu,1), c_int32_t), c_loc(fxyzu), int(size(fxyzu,1), c_int32_t), &
+                           c_loc(vpred), ptr_or_null(Bevol), ptr_or_null(dBevol), ptr_or_null(Bpred), &
+                           ptr_or_null(dustevol), ptr_or_null(ddustevol), ptr_or_null(dustpred), c_loc(divcurlv), &
+                           int(size(divcurlv,1), c_int32_t), c_loc(eos_vars), int(size(eos_vars,1), c_int32_t), &
+                           c_loc(alphaind), ptr_or_null_i1(iphase), ptr_or_null(twas), ufloor, nf)
+ nvfloor = nf
+end function phsph_step_predict_sph
+
+!-- one pass of the corrector loop (step_leapfrog.f90:482-625); errmax,
+!   v2sum and npc are the inputs of check_velocity_error
+integer function phsph_step_corrector(npart, timei, dtsph, xyzh, vxyzu, fxyzu, vpred, Bevol, dBevol, dustevol, &
+                                      ddustevol, iphase, ibin, ibin_wake, nbinmax, twas, ufloor, errmax, v2sum, &
+                                      npc, nwake, nvfloor) result(ierr)
+ integer,           intent(in)            :: npart, nbinmax
+ real(c_double),    intent(in)            :: timei, dtsph, ufloor
+ real(c_double),    intent(in),    target :: xyzh(:,:), fxyzu(:,:), vpred(:,:), dBevol(:,:), ddustevol(:,:)
+ real(c_double),    intent(inout), target :: vxyzu(:,:), Bevol(:,:), dustevol(:,:), twas(:)
+ integer(c_int8_t), intent(in),    target :: iphase(:)
+ integer(c_int8_t), intent(inout), target :: ibin(:), ibin_wake(:)
+ real(c_double),    intent(out)           :: errmax, v2sum
+ integer,           intent(out)           :: npc, nwake, nvfloor
+ integer(c_int32_t) :: n1, n2, n3
+ ierr = c_step_corrector(ctx, int(npart, c_int32_t), timei, dtsph, c_loc(xyzh), int(size(xyzh,1), c_int32_t), &
+                         c_loc(vxyzu), int(size(vxyzu,1), c_int32_t), c_loc(fxyzu), int(size(fxyzu,1), c_int32_t), &
+                         ptr_or_null(vpred), ptr_or_null(Bevol), ptr_or_null(dBevol), ptr_or_null(dustevol), &
+                         ptr_or_null(ddustevol), ptr_or_null_i1(iphase), ptr_or_null_i1(ibin), &
+                         ptr_or_null_i1(ibin_wake), int(nbinmax, c_int32_t), ptr_or_null(twas), ufloor, errmax, &
+                         v2sum, n1, n2, n3)
+ npc = n1; nwake = n2; nvfloor = n3
+end function phsph_step_corrector
+
+!-- until_converged loop of step (step_leapfrog.f90:650-696)
+integer function phsph_step_reset(npart, dtsph, vxyzu, fxyzu, vpred, Bevol, dBevol, Bpred, dustevol, ddustevol, &
+                                  dustpred, iphase) result(ierr)
+ integer,           intent(in)            :: npart
+ real(c_double),    intent(in)            :: dtsph
+ real(c_double),    intent(inout), target :: vxyzu(:,:), vpred(:,:), Bevol(:,:), Bpred(:,:), dustevol(:,:)
+ real(c_double),    intent(inout), target :: dustpred(:,:)
+ real(c_double),    intent(in),    target :: fxyzu(:,:), dBevol(:,:), ddustevol(:,:)
+ integer(c_int8_t), intent(in),    target :: iphase(:)
+ ierr = c_step_reset(ctx, int(npart, c_int32_t), dtsph, c_loc(vxyzu), int(size(vxyzu,1), c_int32_t), c_loc(fxyzu), &
+                     int(size(fxyzu,1), c_int32_t), c_loc(vpred), ptr_or_null(Bevol), ptr_or_null(dBevol), &
+                     ptr_or_null(Bpred), ptr_or_null(dustevol), ptr_or_null(ddustevol), ptr_or_null(dustpred), &
+                     ptr_or_null_i1(iphase))
+end function phsph_step_reset
 
 end module phsph

@@ -75,3 +75,28 @@ BYTES_FORCE = 8 * (4 + 4 + 2) + 4 * (1 + 1) + 8 * 4 + 4  # xyzh, vxyzu, P, cs; g
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+
+
+def step_loop_bytes(loop, nv=4, mhd=False, dust=False, ind=False):
+    """Compulsory HBM bytes per particle of the integrator loops
+    (step.hip): every array the loop reads is read once and every array it
+    updates is written once, in the caller's FP64 / FP32 / int8 layout."""
+    b = 0
+    if loop == "step_predictor":
+        b = 8 + 8 * nv * 3                      # h; v read + write; f
+        b += (32 * 2 + 32) if mhd else 0        # Bevol read + write, dBevol
+        b += (8 * 2 + 8) if dust else 0
+        b += (8 + 1 + 1 + 1) if ind else 0      # twas, ibin, ibin_old, iphase
+    elif loop == "substep_sph":
+        b = 8 + 24 * 2 + 24                     # h; x read + write; v
+    elif loop == "step_predict_sph":
+        b = 8 * 2 + 4 + 8 * nv * 3 + 8 + 4 * 2 + 4  # h rw, div v, v, f, vpred, c_s, alpha(1:2) r, alpha(1) w
+        b += 32 * 3 if mhd else 0
+        b += 8 * 3 if dust else 0
+        b += 8 if ind else 0
+    elif loop == "step_corrector":
+        b = 8 + 8 * nv * 3                      # h; v read + write; f
+        b += (8 * 2 + 1 * 2 + 1 * 2 + 1) if ind else 24  # twas, ibin, ibin_wake, iphase | vpred(1:3)
+        b += (32 * 2 + 32) if mhd else 0
+        b += (8 * 2 + 8) if dust else 0
+    return b

@@ -1,0 +1,75 @@
+"""The second search passes: with tiny initial capacities every group's
+candidate leaves, density and force neighbour lists and gravity near-field
+lists overflow.  With the default list budget the library doubles its
+uniform capacities; with a 1 KiB budget it takes its exact-size paths
+(per-group candidate offsets, per-group force list capacities, per-leaf
+near-field offsets; density retries with longer lists in both).  Either way
+it must agree with the oracle to the parity tolerances: neighbour and
+iteration counts bit-exact, FP64 results within 1e-12 of the rms."""
+import os
+
+import numpy as np
+import pytest
+
+from phantom_amd import setups
+from tests import parity
+from tests.test_gpu_gravity import _run_gpu as run_gpu_grav, _run_oracle as run_oracle_grav
+
+pytestmark = pytest.mark.gpu
+
+SPECS = {"doubling": "gcand=4,dlist=16,flist=16,p2p=2", "exact": "gcand=4,dlist=16,flist=16,p2p=2,budget_kib=1"}
+
+
+def _with_caps(spec, fn, *a):
+    old = os.environ.get("PHSPH_CAPS")
+    os.environ["PHSPH_CAPS"] = spec
+    try:
+        return fn(*a)
+    finally:
+        if old is None:
+            del os.environ["PHSPH_CAPS"]
+        else:
+            os.environ["PHSPH_CAPS"] = old
+
+
+CASES = {
+    "shock_tube_ny8": lambda: setups.shock_tube(ny=8),
+    "mhd_dust_box": lambda: setups.mhd_dust_box(nx=12, jitter=0.05),
+    "box_bins": lambda: setups.with_bins(setups.uniform_box(nx=16, seed=4), nbinmax=3, istepfrac=4),
+}
+
+
+@pytest.mark.parametrize("mode", list(SPECS))
+@pytest.mark.parametrize("name", list(CASES))
+def test_tiny_capacities_hydro(name, mode, gpu):
+    setup = CASES[name]()
+    print(f"\n[case {name}] n={setup['xyzh'].shape[0]} caps {SPECS[mode]}", flush=True)
+    g = _with_caps(SPECS[mode], parity.run_gpu, setup)
+    o = parity.run_oracle(setup)
+    np.testing.assert_array_equal(g["nneigh"], o["dens"]["nneigh"])
+    np.testing.assert_array_equal(g["nits"], o["dens"]["nits"])
+    assert parity.rel_rms(g["xyzh"][:, 3], o["xyzh"][:, 3]) <= 1e-12
+    ncomp = 4 if setup["params"].maxvxyzu >= 4 else 3
+    for c in range(ncomp):
+        assert parity.rel_rms(g["force"]["fxyzu"][:, c], o["force"]["fxyzu"][:, c]) <= 1e-12, c
+    if setup["params"].mhd:
+        for c in range(4):
+            assert parity.rel_rms(g["force"]["dBevol"][:, c], o["force"]["dBevol"][:, c]) <= 1e-12, c
+    if setup.get("bins") is not None:
+        np.testing.assert_array_equal(g["force"]["ibin"], o["force"]["ibin"])
+    g["ctx"].close()
+
+
+@pytest.mark.parametrize("mode", list(SPECS))
+def test_tiny_capacities_gravity(mode, gpu):
+    setup = setups.polytrope(n=3000, theta=0.5)
+    print(f"\n[case polytrope] n={setup['xyzh'].shape[0]} caps {SPECS[mode]}", flush=True)
+    g = _with_caps(SPECS[mode], run_gpu_grav, setup)
+    o = run_oracle_grav(setup)
+    np.testing.assert_array_equal(g["nneigh"], o["dens"]["nneigh"])
+    assert parity.rel_rms(g["xyzh"][:, 3], o["xyzh"][:, 3]) <= 1e-12
+    for c in range(4):
+        assert parity.rel_rms(g["force"]["fxyzu"][:, c], o["force"]["fxyzu"][:, c]) <= 1e-12, c
+    pg = np.asarray(g["force"]["poten"], np.float64)
+    po = np.asarray(o["force"]["poten"], np.float64)
+    assert parity.rel_rms(pg, po) <= 2e-7
