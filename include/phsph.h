@@ -13,6 +13,11 @@
  *   phsph_force          <- force           (src/main/force.F90:193-263)
  *   phsph_get_neighbour_stats <- get_neighbour_stats (src/main/dens.F90:1137)
  *
+ * and the particle loops of the leapfrog step around it
+ * (src/main/step_leapfrog.f90:91-760, src/main/substepping.F90:241-264):
+ * phsph_step_predictor, phsph_substep_sph, phsph_step_predict_sph,
+ * phsph_step_corrector, phsph_step_reset, phsph_init_step.
+ *
  * Conventions
  *  - Every particle array is a Fortran column-major array a(ncomp, maxp)
  *    passed as a pointer to a(1,1) plus its leading dimension `ld` (>= ncomp).

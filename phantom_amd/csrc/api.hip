@@ -225,6 +225,7 @@ int phsph_set_params(phsph_ctx *c, const phsph_params *p) {
   if (msg) PH_RET(c, PHSPH_ERR_INPUT, msg);
   c->p = *p;
   c->dc = make_devconst(*p);
+  c->dlist_reuse = false;
   return PHSPH_OK;
 }
 
@@ -232,6 +233,7 @@ const char *phsph_last_error(const phsph_ctx *c) { return c ? c->err.c_str() : "
 
 int phsph_maketree(phsph_ctx *c, double *xyzh, int32_t ld, int32_t np, const int8_t *iphase,
                    int32_t *leaf_is_active, int64_t *ncells) {
+  if (c) c->dlist_reuse = false;
   if (!c) return PHSPH_ERR_INPUT;
   if (!xyzh || np <= 0 || ld < 4) PH_RET(c, PHSPH_ERR_INPUT, "maketree: bad xyzh/np/ld");
   if ((int64_t)np >= (1 << 27)) PH_RET(c, PHSPH_ERR_CAPACITY, "maketree: at most 2^27-1 particles");

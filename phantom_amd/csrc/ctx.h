@@ -96,6 +96,17 @@ struct phsph_ctx {
   DBuf<int64_t> loff;        // per-group offsets of the force lists (after an overflow)
   DBuf<int32_t> gmax;        // longest list per group
   DBuf<int64_t> poff;        // per-leaf offsets of the near-field lists (after an overflow)
+  DBuf<double> hmax0;        // node hmax of h at densityiterate entry (bit patterns of positive doubles)
+  // the density lists also serve as force lists while positions, h and the
+  // active set are those of the last densityiterate
+  bool dlist_reuse = false;
+  int dlist_nact = 0, dlist_cap = 0;
+  int dlist_nretry = 0;             // particles of the first attempt run again
+  DBuf<int32_t> dretry0;            // their flags, by tree position
+  DBuf<uint32_t> nlist_r;           // lists of retries (density) and of those particles (force)
+  DBuf<int32_t> t_nl_r;
+  DBuf<double> t_hsum_r, t_hfin_r;
+  bool force_reused_lists = false;  // the last force call used the density lists
   int ngroups = 0;
   DBuf<int32_t> ntrial;      // trial particles per leaf (statistics)
   DBuf<uint32_t> nlist;      // neighbour lists (list_at layout; node | code << 27)
@@ -244,6 +255,7 @@ struct ForceIOArgs {  // device pointers in the caller's layout
   int nbinmax, istepfrac, ibinnow;
 };
 int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc, double *dtf);
+int leaf_hmax_run(phsph_ctx *c, const double *d_h, double *d_hmaxn);
 // step.hip
 struct StepArgs {  // device pointers in the caller's layout
   double *xyzh; int ldx;

@@ -101,6 +101,8 @@ __global__ void k_walk(DevConst dc, double rfac, const phsph_kdnode *__restrict_
 // group walk: candidate leaves of 64 consecutive active particles
 // mode 0: rcut = rfac*radkern*max(h of the group)
 // mode 1: rcut = max(that, radkern*hmax of the node)      (force, get_hj)
+// mode 0 with hmaxn: rcut = rfac*radkern*max(h of the group, hmax of the node)
+//   (density lists that also hold every j whose own radius reaches i)
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(64)
 k_group_walk(DevConst dc, int mode, double rfac, const phsph_kdnode *__restrict__ nd,
@@ -156,7 +158,7 @@ k_group_walk(DevConst dc, int mode, double rfac, const phsph_kdnode *__restrict_
       const double qx = fmax(fabs(dx) - ex, 0.), qy = fmax(fabs(dy) - ey, 0.), qz = fmax(fabs(dz) - ez, 0.);
       const double d2 = qx * qx + qy * qy + qz * qz;
       double rc_ = rcg;
-      if (mode == 1) rc_ = fmax(rc_, dc.radkern * hmaxn[node]);
+      if (hmaxn) rc_ = fmax(rc_, (mode == 1 ? 1. : rfac) * dc.radkern * hmaxn[node]);
       const double s = (e.size + rc_) * (1. + 1e-10);
       open = d2 < s * s;
       isleaf = leaf_flag[node] != 0;
@@ -292,8 +294,9 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
             const int32_t *__restrict__ pleaf, const int32_t *__restrict__ leaves,
             const uint32_t *__restrict__ gcand, const int32_t *__restrict__ gncand, const int64_t *goff, int capg,
             const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi,
-            const phsph_kdnode *__restrict__ nd, const double4 *__restrict__ tp, uint32_t *__restrict__ nlist,
-            int capl, int32_t *__restrict__ t_nl, double *__restrict__ t_hsum, double *__restrict__ t_hfin,
+            const phsph_kdnode *__restrict__ nd, const double4 *__restrict__ tp, const double *__restrict__ hmax0,
+            double lmargin, uint32_t *__restrict__ nlist, int capl, int32_t *__restrict__ t_nl,
+            double *__restrict__ t_hsum, double *__restrict__ t_hfin,
             int32_t *__restrict__ p_nits, int32_t *__restrict__ leaf_nits, int32_t *__restrict__ retry,
             double *__restrict__ htraj, double *__restrict__ hconv, int32_t *__restrict__ flags) {
   __shared__ ListStage S;
@@ -313,17 +316,20 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
     lcx = nd[c].xcen[0]; lcy = nd[c].xcen[1]; lcz = nd[c].xcen[2]; lsz = nd[c].size;
   }
   const double rcut = rfac * dc.radkern * h0;
-  const double rc2 = rcut * rcut;
+  // list radius: rcut (times a margin), and with hmax0 also every j whose own
+  // radius rfac*radkern*h0_j reaches i, so that the list holds the force
+  // neighbours (q_i < R or q_j < R at the converged h) as well
+  const double rl = rfac * lmargin * dc.radkern;
+  const double rcl = rl * h0, rc2 = rcl * rcl;
   // --- neighbour list over the leaves this particle can reach ---
   int nl = 0;
   build_list(
-      S, dc, gcand + cand_base(goff, g, capg), gncand[g], nd, nd_lo, nd_hi, (const double *)nullptr, tp, xi, yi, zi, lcx,
-      lcy, lcz, lsz,
-      [&](double ccx, double ccy, double ccz, double csz, double) {
-        return live && leaf_in_reach(dc, xi, yi, zi, ccx, ccy, ccz, csz, rcut);
+      S, dc, gcand + cand_base(goff, g, capg), gncand[g], nd, nd_lo, nd_hi, hmax0, tp, xi, yi, zi, lcx, lcy, lcz, lsz,
+      [&](double ccx, double ccy, double ccz, double csz, double chm) {
+        return live && leaf_in_reach(dc, xi, yi, zi, ccx, ccy, ccz, csz, fmax(rcl, rl * chm));
       },
-      [&](int j, int code, double r2, double) {
-        if (j != p && r2 < rc2) {
+      [&](int j, int code, double r2, double hj0) {
+        if (j != p && (r2 < rc2 || (hmax0 && r2 < (rl * hj0) * (rl * hj0)))) {
           if (nl < capl) nlist[list_at(nact, t, nl)] = cand_pack(j, code);
           nl++;
         }
@@ -815,6 +821,21 @@ int compact_active(phsph_ctx *c, const int32_t *flag, int n) {
 // particles are more than PAD_GAP tree positions apart, and the rest of its
 // 64 slots are padding (act = -1).  The padded length becomes c->nact.
 constexpr int PAD_GAP = 64;
+
+// The first attempt's lists hold, for a particle i, every j within
+// hf*radkern*max(h0_i, h0_j).  They contain the force neighbours (q_i < R or
+// q_j < R at the final h) only if no final h exceeds hf*h0: flag otherwise
+// (a particle whose h grew past its first search).
+__global__ void k_check_hbound(int nact, const int32_t *__restrict__ act, const double *__restrict__ h0,
+                               const double *__restrict__ hfin, double hf, int32_t *__restrict__ flag) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  bool bad = false;
+  if (t < nact) {
+    const int p = act[t];
+    bad = !(hfin[p] * (1. + 1e-12) <= hf * h0[p]);
+  }
+  if (__ballot(bad) != 0ull && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
 int pad_groups(phsph_ctx *c) {
   const int na = c->nact;
   if (na == 0 || na >= c->nlive) return PHSPH_OK;
@@ -888,7 +909,22 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   // Particles whose h outgrows their neighbour list, or whose list does not
   // fit, are handed back and run again (alone) with a wider search or a
   // longer list.
-  double rfac = 1.1;
+  const double rfac0 = 1.1;  // first search radius in units of radkern*h at entry
+  double rfac = rfac0;
+  // Lists the force pass can use as its neighbour lists (no self-gravity):
+  // they also cover every j whose radius reaches i, with a margin for the
+  // converged h (within tolh of the h of the last summation, which is at
+  // most rfac*h0 without a retry).  The lists of the first attempt are kept;
+  // particles run again there are listed by dretry0 and get force lists of
+  // their own.
+  const bool reusable = !grav && c->p.tolh <= 1e-3;
+  const double lmargin = reusable ? 1. + 10. * c->p.tolh : 1.;
+  c->dlist_reuse = false;
+  if (reusable) {
+    PH_HIPCHECK(c, c->hmax0.ensure(c->ncells + 2));
+    rc = leaf_hmax_run(c, c->th_in.p, c->hmax0.p);
+    if (rc) return rc;
+  }
   rc = pad_groups(c);
   if (rc) return rc;
   // list capacity: it grows for the particles run again (the list memory is
@@ -899,13 +935,19 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   for (int attempt = 0; attempt < 12; attempt++) {
     const int na = c->nact;  // active slots, padding included
     if (na == 0) break;
-    PH_HIPCHECK(c, c->t_nl.ensure((size_t)na + 1));
-    PH_HIPCHECK(c, c->t_hsum.ensure((size_t)na + 1));
-    PH_HIPCHECK(c, c->t_hfin.ensure((size_t)na + 1));
-    rc = walk_groups(c, 0, rfac, c->th_in.p, nullptr);
+    // the first attempt writes the lists the force pass may use; retries
+    // write buffers of their own so that those stay intact
+    DBuf<uint32_t> &LB = attempt == 0 ? c->nlist : c->nlist_r;
+    DBuf<int32_t> &TNL = attempt == 0 ? c->t_nl : c->t_nl_r;
+    DBuf<double> &THS = attempt == 0 ? c->t_hsum : c->t_hsum_r;
+    DBuf<double> &THF = attempt == 0 ? c->t_hfin : c->t_hfin_r;
+    PH_HIPCHECK(c, TNL.ensure((size_t)na + 1));
+    PH_HIPCHECK(c, THS.ensure((size_t)na + 1));
+    PH_HIPCHECK(c, THF.ensure((size_t)na + 1));
+    rc = walk_groups(c, 0, rfac * lmargin, c->th_in.p, reusable ? c->hmax0.p : nullptr);
     if (rc) return rc;
     const int ng = c->ngroups;
-    PH_HIPCHECK(c, c->nlist.ensure((size_t)list_cap(dlist) * na + 1));
+    PH_HIPCHECK(c, LB.ensure((size_t)list_cap(dlist) * na + 1));
     hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(n, 256)), dim3(256), 0, s, c->tmpi.p, n, 0);
     int32_t zero = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
@@ -914,7 +956,8 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
 #define DITER(KK)                                                                                              \
   hipLaunchKernelGGL((k_dens_iter<KK>), dim3(xcd_grid(ng)), dim3(64), 0, s, c->dc, icall, rfac, c->act.p, na, ng, \
                      c->pleaf.p, c->leaves.p, c->gcand.p, c->gncand.p, c->goff_p, c->caps.gcand, c->nd_lo.p,      \
-                     c->nd_hi.p, c->nd.p, c->tp4.p, c->nlist.p, dlist, c->t_nl.p, c->t_hsum.p, c->t_hfin.p,       \
+                     c->nd_hi.p, c->nd.p, c->tp4.p, reusable ? c->hmax0.p : nullptr, lmargin, LB.p, dlist,          \
+                     TNL.p, THS.p, THF.p,                                                                           \
                      c->p_nits.p, c->leaf_nits.p, c->tmpi.p, grav ? c->htraj.p : nullptr,                       \
                      grav ? c->hconv.p : nullptr, c->counters.p)
       if (c->p.kernel == PHSPH_KERNEL_CUBIC) DITER(PHSPH_KERNEL_CUBIC); else DITER(PHSPH_KERNEL_QUINTIC);
@@ -924,11 +967,11 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
       LaunchTimer lt(c, "density_sums");
 #define DFIN(KK, MM)                                                                                           \
   hipLaunchKernelGGL((k_dens_final<KK, MM>), dim3(xcd_grid(nblocks(na, 256))), dim3(256), 0, s, c->dc, c->act.p,  \
-                     na, c->nlist.p, c->t_nl.p, c->t_hsum.p, c->t_hfin.p, c->tmpi.p, c->iparts.p, in, c->th.p, out,   \
+                     na, LB.p, TNL.p, THS.p, THF.p, c->tmpi.p, c->iparts.p, in, c->th.p, out,                       \
                      c->p_nneigh.p, c->counters.p)
 #define DMIX(KK)                                                                                                \
   hipLaunchKernelGGL((k_dens_final_mixed<KK>), dim3(xcd_grid(nblocks(na, 256))), dim3(256), 0, s, c->dc, c->act.p, \
-                     na, c->nlist.p, c->t_nl.p, c->t_hsum.p, c->t_hfin.p, c->tmpi.p, c->iparts.p, c->tp4.p,         \
+                     na, LB.p, TNL.p, THS.p, THF.p, c->tmpi.p, c->iparts.p, c->tp4.p,                               \
                      c->tv4.p, have_f ? c->tf4.p : nullptr, c->th.p, out, c->p_nneigh.p, c->counters.p)
       if (mixed) {
         if (c->p.kernel == PHSPH_KERNEL_CUBIC) DMIX(PHSPH_KERNEL_CUBIC); else DMIX(PHSPH_KERNEL_QUINTIC);
@@ -952,6 +995,16 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
     }
     if (fl & DFL_NOCONV) PH_RET(c, PHSPH_ERR_NOCONVERGE, "densityiterate: could not converge in density");
     if (fl & DFL_NEGH) PH_RET(c, PHSPH_ERR_INPUT, "densityiterate: setting negative h from hrho");
+    if (attempt == 0) {
+      // the lists of the first attempt, and which particles they do not
+      // cover (those run again)
+      c->dlist_nact = na;
+      c->dlist_nretry = 0;
+      if (reusable) {
+        PH_HIPCHECK(c, c->dretry0.ensure((size_t)n + 1));
+        PH_HIPCHECK(c, hipMemcpyAsync(c->dretry0.p, c->tmpi.p, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+      }
+    }
     if (!(fl & (DFL_REWALK | DFL_LIST))) {
       c->nact = 0;
       break;
@@ -960,10 +1013,22 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
     if (fl & DFL_LIST) dlist *= 2;
     rc = compact_active(c, c->tmpi.p, n);
     if (rc) return rc;
+    if (attempt == 0) c->dlist_nretry = c->nact;
     rc = pad_groups(c);
     if (rc) return rc;
   }
   if (c->nact != 0) PH_RET(c, PHSPH_ERR_CAPACITY, "densityiterate: neighbour search could not be widened enough");
+  c->dlist_reuse = reusable;
+  if (reusable && nact > 0) {
+    int32_t zero = 0, bad = 0;
+    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p + 2, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_check_hbound, dim3(nblocks(nact, 256)), dim3(256), 0, s, nact, c->act_dens.p, c->th_in.p,
+                       c->th.p, rfac0 * lmargin, c->counters.p + 2);
+    PH_HIPCHECK(c, hipGetLastError());
+    PH_HIPCHECK(c, hipMemcpyAsync(&bad, c->counters.p + 2, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    PH_HIPCHECK(c, hipStreamSynchronize(s));
+    if (bad) c->dlist_reuse = false;
+  }
   while (dlist > c->caps.dlist && !list_fits(c, (size_t)list_cap(dlist) * nact)) dlist /= 2;
   if (dlist > c->caps.dlist) c->caps.dlist = dlist;
   if (grav) {

@@ -173,13 +173,14 @@ __global__ void k_group_max(const int32_t *__restrict__ t_nl, int nact, int ngro
 // list and finish_cell_and_store_results.  One thread per active particle.
 template <int K, bool MHD, bool DUST, bool THERMAL>
 __global__ void __launch_bounds__(256)
-k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const uint32_t *__restrict__ nlist,
-        const int64_t *loff, const int32_t *__restrict__ t_nl, const int32_t *__restrict__ iparts, const FRec *__restrict__ rec,
+k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *__restrict__ skip,
+        const uint32_t *__restrict__ nlist, const int64_t *loff, const int32_t *__restrict__ t_nl, const int32_t *__restrict__ iparts, const FRec *__restrict__ rec,
         const FRecB *__restrict__ recB, const FRecD *__restrict__ recD, ForceIOArgs out,
         const int8_t *__restrict__ tbin_old, int icall, unsigned long long *__restrict__ dtred,
         int32_t *__restrict__ flags) {
   const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const bool live = t < nact && act[t] >= 0;
+  // act[t] < 0: padding; skip: particles whose lists are elsewhere
+  const bool live = t < nact && act[t] >= 0 && !(skip && skip[act[t]]);
   const int p = live ? act[t] : act[0];
   const double pm = dc.pmass;
   const double R2 = dc.radkern2;
@@ -236,17 +237,11 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const uint32_t *
   int ibin_neighi = 0;
   const int8_t ibinnow_m1 = (int8_t)(out.ibinnow - 1);
   const int nl = live ? t_nl[t] : 0;
+  int npair = 0;  // interacting pairs (entries that pass the neighbour test)
   for (int k = 0; k < nl; k++) {
     const uint32_t e = nlist[list_at_g(loff, nact, t, k)];
     const int j = cand_node(e);
     const FRec &Rj = rec[j];
-    if (ind) {
-      // force.F90:1346-1352: every neighbour of an active particle is
-      // flagged to be woken; the limiter needs the largest neighbour bin
-      ibin_neighi = max(ibin_neighi, (int)tbin_old[j]);
-      const int oj = abs(iparts[j]) - 1;
-      if (out.ibin_wake[oj] < ibinnow_m1) out.ibin_wake[oj] = ibinnow_m1;
-    }
     double dx, dy, dz;
     sep_of(dc, cand_code(e), xi, yi, zi, Rj.x, Rj.y, Rj.z, dx, dy, dz);
     const double rij2 = r2_exact(dx, dy, dz);
@@ -254,6 +249,16 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const uint32_t *
     const double hj1 = Rj.hinv;
     const double hj21 = xmul(hj1, hj1);
     const double q2j = xmul(rij2, hj21);
+    // a density list also holds particles that are not force neighbours
+    if (!(q2i < R2 || q2j < R2)) continue;
+    npair++;
+    if (ind) {
+      // force.F90:1346-1352: every neighbour of an active particle is
+      // flagged to be woken; the limiter needs the largest neighbour bin
+      ibin_neighi = max(ibin_neighi, (int)tbin_old[j]);
+      const int oj = abs(iparts[j]) - 1;
+      if (out.ibin_wake[oj] < ibinnow_m1) out.ibin_wake[oj] = ibinnow_m1;
+    }
     double rij1, qi;
     if (rij2 > PH_TINY64) {
       rij1 = 1. / sqrt(rij2);
@@ -446,7 +451,7 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const uint32_t *
       }
   }
   // global minima (force.F90:3312-3327), pair count, highest bin
-  reduce_force_block(dtc_o, dtf_o, nl, bmax, dtred, flags);
+  reduce_force_block(dtc_o, dtf_o, npair, bmax, dtred, flags);
 }
 
 // Mixed precision, hydro (grad-h, Cullen-Dehnen viscosity, conductivity).
@@ -461,13 +466,14 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const uint32_t *
 // 128-byte record.
 template <int K, bool THERMAL>
 __global__ void __launch_bounds__(256)
-k_force_mixed(DevConst dc, const int32_t *__restrict__ act, int nact, const uint32_t *__restrict__ nlist,
-              const int64_t *loff, const int32_t *__restrict__ t_nl, const int32_t *__restrict__ iparts, const FRec *__restrict__ rec,
+k_force_mixed(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *__restrict__ skip,
+              const uint32_t *__restrict__ nlist, const int64_t *loff, const int32_t *__restrict__ t_nl, const int32_t *__restrict__ iparts, const FRec *__restrict__ rec,
               const double4 *__restrict__ fpos, const double4 *__restrict__ fvel, const FRec32 *__restrict__ rec32,
               ForceIOArgs out, const int8_t *__restrict__ tbin_old, int icall,
               unsigned long long *__restrict__ dtred, int32_t *__restrict__ flags) {
   const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const bool live = t < nact && act[t] >= 0;
+  // act[t] < 0: padding; skip: particles whose lists are elsewhere
+  const bool live = t < nact && act[t] >= 0 && !(skip && skip[act[t]]);
   const int p = live ? act[t] : act[0];
   const double pm = dc.pmass;
   const double R2 = dc.radkern2;
@@ -497,15 +503,11 @@ k_force_mixed(DevConst dc, const int32_t *__restrict__ act, int nact, const uint
   int ibin_neighi = 0;
   const int8_t ibinnow_m1 = (int8_t)(out.ibinnow - 1);
   const int nl = live ? t_nl[t] : 0;
+  int npair = 0;  // interacting pairs (entries that pass the neighbour test)
   for (int k = 0; k < nl; k++) {
     const uint32_t e = nlist[list_at_g(loff, nact, t, k)];
     const int j = cand_node(e);
     const double4 pj = fpos[j];
-    if (ind) {  // force.F90:1346-1352
-      ibin_neighi = max(ibin_neighi, (int)tbin_old[j]);
-      const int oj = abs(iparts[j]) - 1;
-      if (out.ibin_wake[oj] < ibinnow_m1) out.ibin_wake[oj] = ibinnow_m1;
-    }
     double dx, dy, dz;
     sep_of(dc, cand_code(e), xi, yi, zi, pj.x, pj.y, pj.z, dx, dy, dz);
     const double rij2 = r2_exact(dx, dy, dz);
@@ -513,6 +515,14 @@ k_force_mixed(DevConst dc, const int32_t *__restrict__ act, int nact, const uint
     const double hj1 = pj.w;
     const double hj21 = xmul(hj1, hj1);
     const double q2j = xmul(rij2, hj21);
+    // a density list also holds particles that are not force neighbours
+    if (!(q2i < R2 || q2j < R2)) continue;
+    npair++;
+    if (ind) {  // force.F90:1346-1352
+      ibin_neighi = max(ibin_neighi, (int)tbin_old[j]);
+      const int oj = abs(iparts[j]) - 1;
+      if (out.ibin_wake[oj] < ibinnow_m1) out.ibin_wake[oj] = ibinnow_m1;
+    }
     const float r2f = (float)rij2;
     double rij1 = 0.;
     if (r2f > 0.f) {
@@ -604,7 +614,7 @@ k_force_mixed(DevConst dc, const int32_t *__restrict__ act, int nact, const uint
       dtf_o = PH_HUGE64;
     }
   }
-  reduce_force_block(dtc_o, dtf_o, nl, bmax, dtred, flags);
+  reduce_force_block(dtc_o, dtf_o, npair, bmax, dtred, flags);
 }
 
 __global__ void k_force_flags(DevConst dc, int nlive, const int32_t *__restrict__ iparts,
@@ -635,6 +645,34 @@ __global__ void k_leaf_hmax(int nleaves, const int32_t *leaves, const int32_t *n
 __global__ void k_zero_u64(unsigned long long *a, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0ull;
+}
+
+// node hmax (as the bit patterns of the positive doubles) of h in tree order
+int leaf_hmax_run(phsph_ctx *c, const double *d_h, double *d_hmaxn) {
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(k_zero_u64, dim3(nblocks(c->ncells + 2, 256)), dim3(256), 0, s, (unsigned long long *)d_hmaxn,
+                     (int64_t)(c->ncells + 2));
+  if (c->nleaves > 0)
+    hipLaunchKernelGGL(k_leaf_hmax, dim3(nblocks(c->nleaves, 256)), dim3(256), 0, s, c->nleaves, c->leaves.p,
+                       c->nd_lo.p, c->nd_hi.p, c->nd.p, d_h, (unsigned long long *)d_hmaxn);
+  PH_HIPCHECK(c, hipGetLastError());
+  return PHSPH_OK;
+}
+
+// The density lists serve as force lists only if the active set, every
+// position and every h are those the density pass left: flag otherwise.
+__global__ void k_check_reuse(int nlive, int nact, const int32_t *__restrict__ act,
+                              const int32_t *__restrict__ act_dens, const double4 *__restrict__ fpos,
+                              const double *__restrict__ fh, const double4 *__restrict__ tp,
+                              const double *__restrict__ th, int32_t *__restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool bad = false;
+  if (i < nact && act[i] != act_dens[i]) bad = true;
+  if (i < nlive) {
+    const double4 f = fpos[i], d = tp[i];
+    if (f.x != d.x || f.y != d.y || f.z != d.z || fh[i] != th[i]) bad = true;
+  }
+  if (__ballot(bad) != 0ull && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
 }
 
 // ------------------------------------------------------------------------
@@ -701,6 +739,107 @@ int cons2prim_run(phsph_ctx *c, int np, const double *xyzh, int ldx, const doubl
 }
 
 // ------------------------------------------------------------------------
+// Force neighbour lists of the active slots c->act[0..c->nact) into L (list
+// lengths into TNL): a uniform capacity, doubled (and kept) while the lists of
+// every active particle fit in the list budget; beyond that the lists are
+// built again with a capacity per group of 64 (its longest list), so that a
+// few particles with many neighbours do not size the whole buffer.  Needs
+// c->hmaxf of the final h.
+static int force_lists(phsph_ctx *c, DBuf<uint32_t> &L, DBuf<int32_t> &TNL, const int64_t **loff_out,
+                       const char *tname) {
+  hipStream_t s = c->stream;
+  const int nact = c->nact;
+  int rc = walk_groups(c, 1, 1.0, c->fh.p, c->hmaxf.p);
+  if (rc) return rc;
+  const int ng = c->ngroups;
+  PH_HIPCHECK(c, TNL.ensure((size_t)nact + 1));
+  const int64_t *loff = nullptr;
+  while (c->caps.flist > 16 && !list_fits(c, (size_t)list_cap(c->caps.flist) * nact)) c->caps.flist /= 2;
+  for (int attempt = 0; attempt < 12; attempt++) {
+    const int ucap = list_cap(c->caps.flist);
+    if (!loff) PH_HIPCHECK(c, L.ensure((size_t)ucap * nact + 1));
+    int32_t zero = 0;
+    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p + 3, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    if (nact > 0) {
+      LaunchTimer lt(c, tname);
+      hipLaunchKernelGGL(k_flist, dim3(xcd_grid(ng)), dim3(64), 0, s, c->dc, c->act.p, nact, ng, c->pleaf.p,
+                         c->leaves.p, c->gcand.p, c->gncand.p, c->goff_p, c->caps.gcand, c->nd_lo.p, c->nd_hi.p,
+                         c->nd.p, c->hmaxf.p, c->fpos.p, L.p, loff, ucap, TNL.p, c->counters.p + 3);
+    }
+    PH_HIPCHECK(c, hipGetLastError());
+    int32_t fl = 0;
+    PH_HIPCHECK(c, hipMemcpyAsync(&fl, c->counters.p + 3, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    PH_HIPCHECK(c, hipStreamSynchronize(s));
+    if (!(fl & FFL_LIST)) break;
+    if (loff) PH_RET(c, PHSPH_ERR_CAPACITY, "force: neighbour list capacity exceeded");
+    if (list_fits(c, (size_t)list_cap(2 * c->caps.flist) * nact)) {  // a larger uniform capacity, kept
+      c->caps.flist *= 2;
+      continue;
+    }
+    // per-group capacity: the group's longest list rounded to 16, times 64 slots
+    PH_HIPCHECK(c, c->gmax.ensure((size_t)ng + 1));
+    PH_HIPCHECK(c, c->loff.ensure((size_t)ng + 1));
+    hipLaunchKernelGGL(k_group_max, dim3(nblocks((size_t)ng * 64, 256)), dim3(256), 0, s, TNL.p, nact, ng,
+                       c->gmax.p);
+    hipLaunchKernelGGL(k_scan_offsets<int32_t>, dim3(1), dim3(1024), 0, s, c->gmax.p, ng, (int64_t)GROUP, LSEG,
+                       c->loff.p);
+    PH_HIPCHECK(c, hipGetLastError());
+    int64_t total = 0;
+    PH_HIPCHECK(c, hipMemcpyAsync(&total, c->loff.p + ng, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PH_HIPCHECK(c, hipStreamSynchronize(s));
+    PH_HIPCHECK(c, L.ensure((size_t)total + 1));
+    loff = c->loff.p;
+  }
+  *loff_out = loff;
+  return PHSPH_OK;
+}
+
+struct ForcePass {
+  const FRec *rec;
+  const FRecB *recB;
+  const FRecD *recD;
+  const FRec32 *rec32;
+  bool mixed, mhd, dust;
+};
+
+// One launch of the pair kernel over the active slots c->act[0..c->nact)
+// with lists L (layout loff) and lengths TNL; particles flagged in skip are
+// left out.  dt minima, pair count and flags accumulate in c->dtred and
+// c->counters.
+static int force_pass(phsph_ctx *c, int icall, const ForceIOArgs &a, const ForcePass &f, const uint32_t *L,
+                      const int64_t *loff, const int32_t *TNL, const int32_t *skip) {
+  hipStream_t s = c->stream;
+  const int nact = c->nact;
+  if (nact == 0) return PHSPH_OK;
+  LaunchTimer lt(c, "force");
+  const dim3 gr(xcd_grid(nblocks(nact, 256))), b(256);
+  const bool th = c->p.maxvxyzu >= 4;
+  const bool mhd = f.mhd, dust = f.dust;
+#define FL(KK, M, D, T)                                                                                        \
+  hipLaunchKernelGGL((k_force<KK, M, D, T>), gr, b, 0, s, c->dc, c->act.p, nact, skip, L, loff, TNL, c->iparts.p, \
+                     f.rec, f.recB, f.recD, a, c->tbin_old.p, icall, c->dtred.p, c->counters.p)
+#define FLK(KK)                                          \
+  if (mhd && dust) { if (th) FL(KK, true, true, true); else FL(KK, true, true, false); }     \
+  else if (mhd) { if (th) FL(KK, true, false, true); else FL(KK, true, false, false); }      \
+  else if (dust) { if (th) FL(KK, false, true, true); else FL(KK, false, true, false); }     \
+  else { if (th) FL(KK, false, false, true); else FL(KK, false, false, false); }
+#define FM(KK, T)                                                                                              \
+  hipLaunchKernelGGL((k_force_mixed<KK, T>), gr, b, 0, s, c->dc, c->act.p, nact, skip, L, loff, TNL, c->iparts.p,  \
+                     f.rec, c->fpos.p, c->fvel.p, f.rec32, a, c->tbin_old.p, icall, c->dtred.p, c->counters.p)
+  if (f.mixed) {
+    if (c->p.kernel == PHSPH_KERNEL_CUBIC) {
+      if (th) FM(PHSPH_KERNEL_CUBIC, true); else FM(PHSPH_KERNEL_CUBIC, false);
+    } else {
+      if (th) FM(PHSPH_KERNEL_QUINTIC, true); else FM(PHSPH_KERNEL_QUINTIC, false);
+    }
+  } else if (c->p.kernel == PHSPH_KERNEL_CUBIC) { FLK(PHSPH_KERNEL_CUBIC) } else { FLK(PHSPH_KERNEL_QUINTIC) }
+#undef FM
+#undef FLK
+#undef FL
+  PH_HIPCHECK(c, hipGetLastError());
+  return PHSPH_OK;
+}
+
 int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, double *dtf_out) {
   (void)icall;
   hipStream_t s = c->stream;
@@ -738,60 +877,31 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
   int rc = compact_active(c, c->tmpi.p, n);
   if (rc) return rc;
   if (c->p.gravity) return grav_force(c, icall, a, rec, c->gsoft.p, dtc_out, dtf_out);
+  const int nact_real = c->nact;
+  // the lists of the last densityiterate, if they still describe these particles
+  bool reuse = c->dlist_reuse && c->nact == c->nact_dens;
+  if (reuse) {
+    int32_t zero = 0;
+    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p + 2, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    const int m = std::max(n, c->nact);
+    hipLaunchKernelGGL(k_check_reuse, dim3(nblocks(m, 256)), dim3(256), 0, s, n, c->nact, c->act.p, c->act_dens.p,
+                       c->fpos.p, c->fh.p, c->tp4.p, c->th.p, c->counters.p + 2);
+    PH_HIPCHECK(c, hipGetLastError());
+    int32_t bad = 0;
+    PH_HIPCHECK(c, hipMemcpyAsync(&bad, c->counters.p + 2, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    PH_HIPCHECK(c, hipStreamSynchronize(s));
+    reuse = (bad == 0);
+  }
   rc = pad_groups(c);
   if (rc) return rc;
-  const int nact = c->nact;  // active slots, padding included
+  if (reuse && c->nact != c->dlist_nact) reuse = false;
+  c->force_reused_lists = reuse;
   // node hmax of the final smoothing lengths for the h_j search radius
-  PH_HIPCHECK(c, c->hmaxf.ensure(c->ncells + 2));
-  hipLaunchKernelGGL(k_zero_u64, dim3(nblocks(c->ncells + 2, 256)), dim3(256), 0, s,
-                     (unsigned long long *)c->hmaxf.p, (int64_t)(c->ncells + 2));
-  hipLaunchKernelGGL(k_leaf_hmax, dim3(nblocks(c->nleaves, 256)), dim3(256), 0, s, c->nleaves, c->leaves.p,
-                     c->nd_lo.p, c->nd_hi.p, c->nd.p, c->fh.p, (unsigned long long *)c->hmaxf.p);
-  rc = walk_groups(c, 1, 1.0, c->fh.p, c->hmaxf.p);
-  if (rc) return rc;
-  const int ng = c->ngroups;
-  PH_HIPCHECK(c, c->t_nl.ensure(nact + 1));
-  // neighbour lists at a uniform capacity, doubled (and kept) while the lists
-  // of every active particle fit in the list budget; beyond that the lists
-  // are built again with a capacity per group of 64 (its longest list), so
-  // that a few particles with many neighbours do not size the whole buffer.
-  // The pair pass below runs once.
-  const int64_t *loff = nullptr;
-  while (c->caps.flist > 16 && !list_fits(c, (size_t)list_cap(c->caps.flist) * nact)) c->caps.flist /= 2;
-  for (int attempt = 0; attempt < 12; attempt++) {
-    const int ucap = list_cap(c->caps.flist);
-    if (!loff) PH_HIPCHECK(c, c->nlist.ensure((size_t)ucap * nact + 1));
-    int32_t zero = 0;
-    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
-    if (nact > 0) {
-      LaunchTimer lt(c, "force_list");
-      hipLaunchKernelGGL(k_flist, dim3(xcd_grid(ng)), dim3(64), 0, s, c->dc, c->act.p, nact, ng, c->pleaf.p,
-                         c->leaves.p, c->gcand.p, c->gncand.p, c->goff_p, c->caps.gcand, c->nd_lo.p, c->nd_hi.p,
-                         c->nd.p, c->hmaxf.p, c->fpos.p, c->nlist.p, loff, ucap, c->t_nl.p, c->counters.p);
-    }
-    PH_HIPCHECK(c, hipGetLastError());
-    int32_t fl = 0;
-    PH_HIPCHECK(c, hipMemcpyAsync(&fl, c->counters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    PH_HIPCHECK(c, hipStreamSynchronize(s));
-    if (!(fl & FFL_LIST)) break;
-    if (loff) PH_RET(c, PHSPH_ERR_CAPACITY, "force: neighbour list capacity exceeded");
-    if (list_fits(c, (size_t)list_cap(2 * c->caps.flist) * nact)) {  // a larger uniform capacity, kept
-      c->caps.flist *= 2;
-      continue;
-    }
-    // per-group capacity: the group's longest list rounded to 16, times 64 slots
-    PH_HIPCHECK(c, c->gmax.ensure((size_t)ng + 1));
-    PH_HIPCHECK(c, c->loff.ensure((size_t)ng + 1));
-    hipLaunchKernelGGL(k_group_max, dim3(nblocks((size_t)ng * 64, 256)), dim3(256), 0, s, c->t_nl.p, nact, ng,
-                       c->gmax.p);
-    hipLaunchKernelGGL(k_scan_offsets<int32_t>, dim3(1), dim3(1024), 0, s, c->gmax.p, ng, (int64_t)GROUP, LSEG,
-                       c->loff.p);
-    PH_HIPCHECK(c, hipGetLastError());
-    int64_t total = 0;
-    PH_HIPCHECK(c, hipMemcpyAsync(&total, c->loff.p + ng, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    PH_HIPCHECK(c, hipStreamSynchronize(s));
-    PH_HIPCHECK(c, c->nlist.ensure((size_t)total + 1));
-    loff = c->loff.p;
+  const bool need_lists = !reuse || c->dlist_nretry > 0;
+  if (need_lists) {
+    PH_HIPCHECK(c, c->hmaxf.ensure(c->ncells + 2));
+    rc = leaf_hmax_run(c, c->fh.p, c->hmaxf.p);
+    if (rc) return rc;
   }
   PH_HIPCHECK(c, c->dtred.ensure(3));
   {
@@ -801,54 +911,58 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
     init[1] = init[0];
     init[2] = 0ull;
     PH_HIPCHECK(c, hipMemcpyAsync(c->dtred.p, init, sizeof(init), hipMemcpyHostToDevice, s));
-    int32_t zero2[2] = {0, 0};
-    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, zero2, sizeof(zero2), hipMemcpyHostToDevice, s));
-    if (nact > 0) {
-      LaunchTimer lt(c, "force");
-      const dim3 gr(xcd_grid(nblocks(nact, 256))), b(256);
-      const bool th = c->p.maxvxyzu >= 4;
-#define FL(KK, M, D, T)                                                                                        \
-  hipLaunchKernelGGL((k_force<KK, M, D, T>), gr, b, 0, s, c->dc, c->act.p, nact, c->nlist.p, loff, c->t_nl.p,   \
-                     c->iparts.p, rec, recB, recD, a, c->tbin_old.p, icall, c->dtred.p, c->counters.p)
-#define FLK(KK)                                          \
-  if (mhd && dust) { if (th) FL(KK, true, true, true); else FL(KK, true, true, false); }     \
-  else if (mhd) { if (th) FL(KK, true, false, true); else FL(KK, true, false, false); }      \
-  else if (dust) { if (th) FL(KK, false, true, true); else FL(KK, false, true, false); }     \
-  else { if (th) FL(KK, false, false, true); else FL(KK, false, false, false); }
-#define FM(KK, T)                                                                                              \
-  hipLaunchKernelGGL((k_force_mixed<KK, T>), gr, b, 0, s, c->dc, c->act.p, nact, c->nlist.p, loff, c->t_nl.p,    \
-                     c->iparts.p, rec, c->fpos.p, c->fvel.p, rec32, a, c->tbin_old.p, icall, c->dtred.p,          \
-                     c->counters.p)
-      if (mixed) {
-        if (c->p.kernel == PHSPH_KERNEL_CUBIC) {
-          if (th) FM(PHSPH_KERNEL_CUBIC, true); else FM(PHSPH_KERNEL_CUBIC, false);
-        } else {
-          if (th) FM(PHSPH_KERNEL_QUINTIC, true); else FM(PHSPH_KERNEL_QUINTIC, false);
-        }
-      } else if (c->p.kernel == PHSPH_KERNEL_CUBIC) { FLK(PHSPH_KERNEL_CUBIC) } else { FLK(PHSPH_KERNEL_QUINTIC) }
-#undef FM
-#undef FLK
-#undef FL
-    }
-    PH_HIPCHECK(c, hipGetLastError());
-    int32_t fl2[2] = {0, 0};
-    unsigned long long red[3];
-    PH_HIPCHECK(c, hipMemcpyAsync(fl2, c->counters.p, sizeof(fl2), hipMemcpyDeviceToHost, s));
-    PH_HIPCHECK(c, hipMemcpyAsync(red, c->dtred.p, sizeof(red), hipMemcpyDeviceToHost, s));
-    PH_HIPCHECK(c, hipStreamSynchronize(s));
-    const int32_t fl = fl2[0];
-    if (fl & FFL_DTC) PH_RET(c, PHSPH_ERR_INPUT, "force: invalid dtc");
-    // with no active particle nbinmax is kept (force.F90:748-756)
-    if (a.ibin) c->nbinmax_new = (nact > 0) ? fl2[1] : a.nbinmax;
-    double dtc, dtf;
-    memcpy(&dtc, &red[0], 8);
-    memcpy(&dtf, &red[1], 8);
-    c->force_npairs = (int64_t)red[2];
-    if (a.ibin) dtc = dtf = PH_BIGNUMBER;  // per-particle bins instead (force.F90:3312)
-    if (dtc_out) *dtc_out = (nact > 0) ? dtc : PH_BIGNUMBER;
-    if (dtf_out) *dtf_out = (nact > 0) ? fmin(dtf, PH_BIGNUMBER) : PH_BIGNUMBER;
-    return PHSPH_OK;
   }
+  int32_t zero2[2] = {0, 0};
+  int32_t fl_first[2] = {0, 0};  // flags and bin maximum of a first pass
+  PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, zero2, sizeof(zero2), hipMemcpyHostToDevice, s));
+  ForcePass fp{rec, recB, recD, rec32, mixed, mhd, dust};
+  if (reuse) {
+    // the density lists, without the particles the density pass ran again
+    rc = force_pass(c, icall, a, fp, c->nlist.p, nullptr, c->t_nl.p, c->dlist_nretry > 0 ? c->dretry0.p : nullptr);
+    if (rc) return rc;
+    if (c->dlist_nretry > 0) {  // those particles get lists of their own
+      // the walk and the compaction below use the flag counters: keep this
+      // pass's flags and bin maximum
+      PH_HIPCHECK(c, hipMemcpyAsync(fl_first, c->counters.p, sizeof(fl_first), hipMemcpyDeviceToHost, s));
+      PH_HIPCHECK(c, hipStreamSynchronize(s));
+      rc = compact_active(c, c->dretry0.p, n);
+      if (rc) return rc;
+      rc = pad_groups(c);
+      if (rc) return rc;
+      const int64_t *loff = nullptr;
+      rc = force_lists(c, c->nlist_r, c->t_nl_r, &loff, "force_list_rerun");
+      if (rc) return rc;
+      PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, zero2, sizeof(zero2), hipMemcpyHostToDevice, s));
+      rc = force_pass(c, icall, a, fp, c->nlist_r.p, loff, c->t_nl_r.p, nullptr);
+      if (rc) return rc;
+    }
+  } else {
+    c->dlist_reuse = false;  // the lists built here take the density lists' place
+    const int64_t *loff = nullptr;
+    rc = force_lists(c, c->nlist, c->t_nl, &loff, "force_list");
+    if (rc) return rc;
+    rc = force_pass(c, icall, a, fp, c->nlist.p, loff, c->t_nl.p, nullptr);
+    if (rc) return rc;
+  }
+  int32_t fl2[2] = {0, 0};
+  unsigned long long red[3];
+  PH_HIPCHECK(c, hipMemcpyAsync(fl2, c->counters.p, sizeof(fl2), hipMemcpyDeviceToHost, s));
+  PH_HIPCHECK(c, hipMemcpyAsync(red, c->dtred.p, sizeof(red), hipMemcpyDeviceToHost, s));
+  PH_HIPCHECK(c, hipStreamSynchronize(s));
+  fl2[0] |= fl_first[0];
+  fl2[1] = std::max(fl2[1], fl_first[1]);
+  const int32_t fl = fl2[0];
+  if (fl & FFL_DTC) PH_RET(c, PHSPH_ERR_INPUT, "force: invalid dtc");
+  // with no active particle nbinmax is kept (force.F90:748-756)
+  if (a.ibin) c->nbinmax_new = (nact_real > 0) ? fl2[1] : a.nbinmax;
+  double dtc, dtf;
+  memcpy(&dtc, &red[0], 8);
+  memcpy(&dtf, &red[1], 8);
+  c->force_npairs = (int64_t)red[2];
+  if (a.ibin) dtc = dtf = PH_BIGNUMBER;  // per-particle bins instead (force.F90:3312)
+  if (dtc_out) *dtc_out = (nact_real > 0) ? dtc : PH_BIGNUMBER;
+  if (dtf_out) *dtf_out = (nact_real > 0) ? fmin(dtf, PH_BIGNUMBER) : PH_BIGNUMBER;
+  return PHSPH_OK;
 }
 
 // ------------------------------------------------------------------------
