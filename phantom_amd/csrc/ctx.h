@@ -111,6 +111,7 @@ struct phsph_ctx {
   DBuf<int32_t> ntrial;      // trial particles per leaf (statistics)
   DBuf<uint32_t> nlist;      // neighbour lists (list_at layout; node | code << 27)
   DBuf<int32_t> t_nl;        // list length per active slot
+  DBuf<int32_t> t_nlj;       // density lists: entries at the back (only within the neighbour's radius)
   DBuf<double> t_hsum, t_hfin;  // density: h of the last summation, h to store
 
   // ---- tree-ordered particle mirror ----

@@ -296,6 +296,7 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
             const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi,
             const phsph_kdnode *__restrict__ nd, const double4 *__restrict__ tp, const double *__restrict__ hmax0,
             double lmargin, uint32_t *__restrict__ nlist, int capl, int32_t *__restrict__ t_nl,
+            int32_t *__restrict__ t_nlj,
             double *__restrict__ t_hsum, double *__restrict__ t_hfin,
             int32_t *__restrict__ p_nits, int32_t *__restrict__ leaf_nits, int32_t *__restrict__ retry,
             double *__restrict__ htraj, double *__restrict__ hconv, int32_t *__restrict__ flags) {
@@ -322,20 +323,25 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
   const double rl = rfac * lmargin * dc.radkern;
   const double rcl = rl * h0, rc2 = rcl * rcl;
   // --- neighbour list over the leaves this particle can reach ---
-  int nl = 0;
+  // entries within its own radius fill the list from the front (the density
+  // sums read only those), entries only within the neighbour's radius fill
+  // it from the back (for the force pass)
+  int nl = 0, nlj = 0;
   build_list(
       S, dc, gcand + cand_base(goff, g, capg), gncand[g], nd, nd_lo, nd_hi, hmax0, tp, xi, yi, zi, lcx, lcy, lcz, lsz,
       [&](double ccx, double ccy, double ccz, double csz, double chm) {
         return live && leaf_in_reach(dc, xi, yi, zi, ccx, ccy, ccz, csz, fmax(rcl, rl * chm));
       },
       [&](int j, int code, double r2, double hj0) {
-        if (j != p && (r2 < rc2 || (hmax0 && r2 < (rl * hj0) * (rl * hj0)))) {
-          if (nl < capl) nlist[list_at(nact, t, nl)] = cand_pack(j, code);
-          nl++;
+        const bool own = r2 < rc2;
+        if (j != p && (own || (hmax0 && r2 < (rl * hj0) * (rl * hj0)))) {
+          if (nl + nlj < capl) nlist[list_at(nact, t, own ? nl : capl - 1 - nlj)] = cand_pack(j, code);
+          nl += own ? 1 : 0;
+          nlj += own ? 0 : 1;
         }
       });
   if (!live) return;
-  if (nl > capl) {
+  if (nl + nlj > capl) {
     atomicOr(flags, DFL_LIST);
     retry[p] = 1;
     return;
@@ -399,6 +405,7 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
   }
   if (hconv) hconv[p] = hi;  // the h of the last summation (cell%h once converged)
   t_nl[t] = nl;
+  if (t_nlj) t_nlj[t] = nlj;
   t_hsum[t] = hi;
   t_hfin[t] = hfin;
   p_nits[p] = (icall == 0) ? 1 : it;
@@ -942,9 +949,12 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
     DBuf<double> &THS = attempt == 0 ? c->t_hsum : c->t_hsum_r;
     DBuf<double> &THF = attempt == 0 ? c->t_hfin : c->t_hfin_r;
     PH_HIPCHECK(c, TNL.ensure((size_t)na + 1));
+    if (attempt == 0 && reusable) PH_HIPCHECK(c, c->t_nlj.ensure((size_t)na + 1));
     PH_HIPCHECK(c, THS.ensure((size_t)na + 1));
     PH_HIPCHECK(c, THF.ensure((size_t)na + 1));
-    rc = walk_groups(c, 0, rfac * lmargin, c->th_in.p, reusable ? c->hmax0.p : nullptr);
+    // only the first attempt's lists serve the force pass
+    const bool widen = attempt == 0 && reusable;
+    rc = walk_groups(c, 0, rfac * lmargin, c->th_in.p, widen ? c->hmax0.p : nullptr);
     if (rc) return rc;
     const int ng = c->ngroups;
     PH_HIPCHECK(c, LB.ensure((size_t)list_cap(dlist) * na + 1));
@@ -956,8 +966,8 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
 #define DITER(KK)                                                                                              \
   hipLaunchKernelGGL((k_dens_iter<KK>), dim3(xcd_grid(ng)), dim3(64), 0, s, c->dc, icall, rfac, c->act.p, na, ng, \
                      c->pleaf.p, c->leaves.p, c->gcand.p, c->gncand.p, c->goff_p, c->caps.gcand, c->nd_lo.p,      \
-                     c->nd_hi.p, c->nd.p, c->tp4.p, reusable ? c->hmax0.p : nullptr, lmargin, LB.p, dlist,          \
-                     TNL.p, THS.p, THF.p,                                                                           \
+                     c->nd_hi.p, c->nd.p, c->tp4.p, widen ? c->hmax0.p : nullptr, lmargin, LB.p, dlist,             \
+                     TNL.p, widen ? c->t_nlj.p : nullptr, THS.p, THF.p,                                             \
                      c->p_nits.p, c->leaf_nits.p, c->tmpi.p, grav ? c->htraj.p : nullptr,                       \
                      grav ? c->hconv.p : nullptr, c->counters.p)
       if (c->p.kernel == PHSPH_KERNEL_CUBIC) DITER(PHSPH_KERNEL_CUBIC); else DITER(PHSPH_KERNEL_QUINTIC);
@@ -999,6 +1009,7 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
       // the lists of the first attempt, and which particles they do not
       // cover (those run again)
       c->dlist_nact = na;
+      c->dlist_cap = dlist;
       c->dlist_nretry = 0;
       if (reusable) {
         PH_HIPCHECK(c, c->dretry0.ensure((size_t)n + 1));

@@ -174,7 +174,9 @@ __global__ void k_group_max(const int32_t *__restrict__ t_nl, int nact, int ngro
 template <int K, bool MHD, bool DUST, bool THERMAL>
 __global__ void __launch_bounds__(256)
 k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *__restrict__ skip,
-        const uint32_t *__restrict__ nlist, const int64_t *loff, const int32_t *__restrict__ t_nl, const int32_t *__restrict__ iparts, const FRec *__restrict__ rec,
+        const uint32_t *__restrict__ nlist, const int64_t *loff, const int32_t *__restrict__ t_nl,
+        const int32_t *__restrict__ t_nlj, int capb, const int32_t *__restrict__ iparts,
+        const double4 *__restrict__ fpos, const FRec *__restrict__ rec,
         const FRecB *__restrict__ recB, const FRecD *__restrict__ recD, ForceIOArgs out,
         const int8_t *__restrict__ tbin_old, int icall, unsigned long long *__restrict__ dtred,
         int32_t *__restrict__ flags) {
@@ -236,22 +238,28 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *_
   const bool ind = out.ibin != nullptr;
   int ibin_neighi = 0;
   const int8_t ibinnow_m1 = (int8_t)(out.ibinnow - 1);
+  // the list's front entries, then (density lists) its back entries
   const int nl = live ? t_nl[t] : 0;
+  const int ntot = nl + ((live && t_nlj) ? t_nlj[t] : 0);
   int npair = 0;  // interacting pairs (entries that pass the neighbour test)
-  for (int k = 0; k < nl; k++) {
+  for (int kk = 0; kk < ntot; kk++) {
+    const int k = kk < nl ? kk : capb - 1 - (kk - nl);
     const uint32_t e = nlist[list_at_g(loff, nact, t, k)];
     const int j = cand_node(e);
-    const FRec &Rj = rec[j];
+    // the neighbour test on the 32-byte position record (x, y, z, 1/h); the
+    // full record is read for neighbours only
+    const double4 pj = fpos[j];
     double dx, dy, dz;
-    sep_of(dc, cand_code(e), xi, yi, zi, Rj.x, Rj.y, Rj.z, dx, dy, dz);
+    sep_of(dc, cand_code(e), xi, yi, zi, pj.x, pj.y, pj.z, dx, dy, dz);
     const double rij2 = r2_exact(dx, dy, dz);
     const double q2i = xmul(rij2, hi21);
-    const double hj1 = Rj.hinv;
+    const double hj1 = pj.w;
     const double hj21 = xmul(hj1, hj1);
     const double q2j = xmul(rij2, hj21);
     // a density list also holds particles that are not force neighbours
     if (!(q2i < R2 || q2j < R2)) continue;
     npair++;
+    const FRec &Rj = rec[j];
     if (ind) {
       // force.F90:1346-1352: every neighbour of an active particle is
       // flagged to be woken; the limiter needs the largest neighbour bin
@@ -467,7 +475,8 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *_
 template <int K, bool THERMAL>
 __global__ void __launch_bounds__(256)
 k_force_mixed(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *__restrict__ skip,
-              const uint32_t *__restrict__ nlist, const int64_t *loff, const int32_t *__restrict__ t_nl, const int32_t *__restrict__ iparts, const FRec *__restrict__ rec,
+              const uint32_t *__restrict__ nlist, const int64_t *loff, const int32_t *__restrict__ t_nl,
+              const int32_t *__restrict__ t_nlj, int capb, const int32_t *__restrict__ iparts, const FRec *__restrict__ rec,
               const double4 *__restrict__ fpos, const double4 *__restrict__ fvel, const FRec32 *__restrict__ rec32,
               ForceIOArgs out, const int8_t *__restrict__ tbin_old, int icall,
               unsigned long long *__restrict__ dtred, int32_t *__restrict__ flags) {
@@ -502,9 +511,12 @@ k_force_mixed(DevConst dc, const int32_t *__restrict__ act, int nact, const int3
   const bool ind = out.ibin != nullptr;
   int ibin_neighi = 0;
   const int8_t ibinnow_m1 = (int8_t)(out.ibinnow - 1);
+  // the list's front entries, then (density lists) its back entries
   const int nl = live ? t_nl[t] : 0;
+  const int ntot = nl + ((live && t_nlj) ? t_nlj[t] : 0);
   int npair = 0;  // interacting pairs (entries that pass the neighbour test)
-  for (int k = 0; k < nl; k++) {
+  for (int kk = 0; kk < ntot; kk++) {
+    const int k = kk < nl ? kk : capb - 1 - (kk - nl);
     const uint32_t e = nlist[list_at_g(loff, nact, t, k)];
     const int j = cand_node(e);
     const double4 pj = fpos[j];
@@ -807,7 +819,7 @@ struct ForcePass {
 // left out.  dt minima, pair count and flags accumulate in c->dtred and
 // c->counters.
 static int force_pass(phsph_ctx *c, int icall, const ForceIOArgs &a, const ForcePass &f, const uint32_t *L,
-                      const int64_t *loff, const int32_t *TNL, const int32_t *skip) {
+                      const int64_t *loff, const int32_t *TNL, const int32_t *TNLJ, int capb, const int32_t *skip) {
   hipStream_t s = c->stream;
   const int nact = c->nact;
   if (nact == 0) return PHSPH_OK;
@@ -816,16 +828,17 @@ static int force_pass(phsph_ctx *c, int icall, const ForceIOArgs &a, const Force
   const bool th = c->p.maxvxyzu >= 4;
   const bool mhd = f.mhd, dust = f.dust;
 #define FL(KK, M, D, T)                                                                                        \
-  hipLaunchKernelGGL((k_force<KK, M, D, T>), gr, b, 0, s, c->dc, c->act.p, nact, skip, L, loff, TNL, c->iparts.p, \
-                     f.rec, f.recB, f.recD, a, c->tbin_old.p, icall, c->dtred.p, c->counters.p)
+  hipLaunchKernelGGL((k_force<KK, M, D, T>), gr, b, 0, s, c->dc, c->act.p, nact, skip, L, loff, TNL, TNLJ, capb,   \
+                     c->iparts.p, c->fpos.p, f.rec, f.recB, f.recD, a, c->tbin_old.p, icall, c->dtred.p, c->counters.p)
 #define FLK(KK)                                          \
   if (mhd && dust) { if (th) FL(KK, true, true, true); else FL(KK, true, true, false); }     \
   else if (mhd) { if (th) FL(KK, true, false, true); else FL(KK, true, false, false); }      \
   else if (dust) { if (th) FL(KK, false, true, true); else FL(KK, false, true, false); }     \
   else { if (th) FL(KK, false, false, true); else FL(KK, false, false, false); }
 #define FM(KK, T)                                                                                              \
-  hipLaunchKernelGGL((k_force_mixed<KK, T>), gr, b, 0, s, c->dc, c->act.p, nact, skip, L, loff, TNL, c->iparts.p,  \
-                     f.rec, c->fpos.p, c->fvel.p, f.rec32, a, c->tbin_old.p, icall, c->dtred.p, c->counters.p)
+  hipLaunchKernelGGL((k_force_mixed<KK, T>), gr, b, 0, s, c->dc, c->act.p, nact, skip, L, loff, TNL, TNLJ, capb,  \
+                     c->iparts.p, f.rec, c->fpos.p, c->fvel.p, f.rec32, a, c->tbin_old.p, icall, c->dtred.p,         \
+                     c->counters.p)
   if (f.mixed) {
     if (c->p.kernel == PHSPH_KERNEL_CUBIC) {
       if (th) FM(PHSPH_KERNEL_CUBIC, true); else FM(PHSPH_KERNEL_CUBIC, false);
@@ -918,7 +931,8 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
   ForcePass fp{rec, recB, recD, rec32, mixed, mhd, dust};
   if (reuse) {
     // the density lists, without the particles the density pass ran again
-    rc = force_pass(c, icall, a, fp, c->nlist.p, nullptr, c->t_nl.p, c->dlist_nretry > 0 ? c->dretry0.p : nullptr);
+    rc = force_pass(c, icall, a, fp, c->nlist.p, nullptr, c->t_nl.p, c->t_nlj.p, c->dlist_cap,
+                    c->dlist_nretry > 0 ? c->dretry0.p : nullptr);
     if (rc) return rc;
     if (c->dlist_nretry > 0) {  // those particles get lists of their own
       // the walk and the compaction below use the flag counters: keep this
@@ -933,7 +947,7 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
       rc = force_lists(c, c->nlist_r, c->t_nl_r, &loff, "force_list_rerun");
       if (rc) return rc;
       PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, zero2, sizeof(zero2), hipMemcpyHostToDevice, s));
-      rc = force_pass(c, icall, a, fp, c->nlist_r.p, loff, c->t_nl_r.p, nullptr);
+      rc = force_pass(c, icall, a, fp, c->nlist_r.p, loff, c->t_nl_r.p, nullptr, 0, nullptr);
       if (rc) return rc;
     }
   } else {
@@ -941,7 +955,7 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
     const int64_t *loff = nullptr;
     rc = force_lists(c, c->nlist, c->t_nl, &loff, "force_list");
     if (rc) return rc;
-    rc = force_pass(c, icall, a, fp, c->nlist.p, loff, c->t_nl.p, nullptr);
+    rc = force_pass(c, icall, a, fp, c->nlist.p, loff, c->t_nl.p, nullptr, 0, nullptr);
     if (rc) return rc;
   }
   int32_t fl2[2] = {0, 0};
