@@ -289,7 +289,7 @@ __global__ void k_scan_i32(int32_t *a, int n, int32_t *total) {
 // taken once, at the h of the last iteration, by part 2.
 // ------------------------------------------------------------------------
 template <int K>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64 * LWPB)
 k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, int nact, int ngroups,
             const int32_t *__restrict__ pleaf, const int32_t *__restrict__ leaves,
             const uint32_t *__restrict__ gcand, const int32_t *__restrict__ gncand, const int64_t *goff, int capg,
@@ -300,10 +300,11 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
             double *__restrict__ t_hsum, double *__restrict__ t_hfin,
             int32_t *__restrict__ p_nits, int32_t *__restrict__ leaf_nits, int32_t *__restrict__ retry,
             double *__restrict__ htraj, double *__restrict__ hconv, int32_t *__restrict__ flags) {
-  __shared__ ListStage S;
-  const int g = xcd_block(blockIdx.x, gridDim.x);
+  __shared__ ListStage Sw[LWPB];
+  ListStage &S = Sw[threadIdx.x >> 6];
+  const int g = list_group(gridDim.x);
   if (g >= ngroups) return;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const int t = g * GROUP + lane;
   const bool live = t < nact && act[t] >= 0;  // act[t] < 0: padding of a partial group
   int p = -1, l = 0;
@@ -964,7 +965,8 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
     {
       LaunchTimer lt(c, "density_iter");
 #define DITER(KK)                                                                                              \
-  hipLaunchKernelGGL((k_dens_iter<KK>), dim3(xcd_grid(ng)), dim3(64), 0, s, c->dc, icall, rfac, c->act.p, na, ng, \
+  hipLaunchKernelGGL((k_dens_iter<KK>), dim3(list_grid(ng)), dim3(64 * LWPB), 0, s, c->dc, icall, rfac, c->act.p,   \
+                     na, ng,                                                                                        \
                      c->pleaf.p, c->leaves.p, c->gcand.p, c->gncand.p, c->goff_p, c->caps.gcand, c->nd_lo.p,      \
                      c->nd_hi.p, c->nd.p, c->tp4.p, widen ? c->hmax0.p : nullptr, lmargin, LB.p, dlist,             \
                      TNL.p, widen ? c->t_nlj.p : nullptr, THS.p, THF.p,                                             \

@@ -45,6 +45,21 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 }
 static inline unsigned xcd_grid(int ngroups) { return (unsigned)(((ngroups + 7) / 8) * 8); }
 
+// The list kernels run LWPB independent wavefronts per block, one group each,
+// so that more waves share a CU; each wavefront stages through its own LDS
+// and synchronises only itself.
+constexpr int LWPB = 4;
+static inline unsigned list_grid(int ngroups) { return xcd_grid((ngroups + LWPB - 1) / LWPB); }
+__device__ __forceinline__ int list_group(int nb) {
+  return xcd_block(blockIdx.x, nb) * LWPB + (int)(threadIdx.x >> 6);
+}
+// orders a wavefront's LDS stores before its later LDS loads
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 
 
 __device__ __forceinline__ unsigned long long lanemask_lt() {
@@ -182,8 +197,8 @@ static inline int list_cap(int cap) { return (cap + LSEG - 1) / LSEG * LSEG; }
 // particles of its own reachable leaves, reading them from LDS:
 // visit(j, code, r2, w) with the leaf's separation code for the lane, the
 // squared separation from (xi, yi, zi) rounded as sep_of and r2_exact, and
-// the record's fourth word.  One wavefront per block (the block barrier
-// orders the staging).
+// the record's fourth word.  S is the wavefront's own stage; only the
+// wavefront synchronises (wave_lds_sync).
 constexpr int CL = 16;                 // candidate leaves per chunk
 constexpr int CS = CL * PH_MINPART;    // staged particles per chunk
 struct ListStage {
@@ -207,7 +222,7 @@ __device__ __forceinline__ void build_list(ListStage &S, const DevConst &dc, con
                                            const Rec *__restrict__ rec, double xi, double yi, double zi,
                                            double lcx, double lcy, double lcz, double lsz, Reach reach,
                                            Visit visit) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   for (int k0 = 0; k0 < nc; k0 += CL) {
     const int kn = min(CL, nc - k0);
     int cnt = 0;
@@ -230,14 +245,14 @@ __device__ __forceinline__ void build_list(ListStage &S, const DevConst &dc, con
     if (lane < kn) S.start[lane] = incl - cnt;
     if (lane == kn - 1) S.start[kn] = incl;
     const int total = __shfl(incl, kn - 1, 64);
-    __syncthreads();
+    wave_lds_sync();
     for (int s = lane; s < total; s += 64) {
       int L = 0;  // leaf of slot s: last L with start[L] <= s
       for (int b = CL / 2; b > 0; b >>= 1)
         if (L + b < kn && S.start[L + b] <= s) L += b;
       rec_xyzi(rec, S.lo[L] + (s - S.start[L]), S.x[s], S.y[s], S.z[s], S.hinv[s]);
     }
-    __syncthreads();
+    wave_lds_sync();
     unsigned mask = 0u;
     for (int L = 0; L < kn; L++)
       if (reach(S.cx[L], S.cy[L], S.cz[L], S.sz[L], S.hm[L])) mask |= 1u << L;
@@ -268,7 +283,7 @@ __device__ __forceinline__ void build_list(ListStage &S, const DevConst &dc, con
         visit(j0 + s, code, r2_exact(dx, dy, dz), S.hinv[s]);
       }
     }
-    __syncthreads();
+    wave_lds_sync();
   }
 }
 
