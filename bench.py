@@ -95,12 +95,12 @@ def main():
 
     ctx = sph.Context(p)
     t0 = time.perf_counter()
-    ctx.maketree(xyzh)
+    ctx.maketree(xyzh, leaf=False)
     tree_ms = 1e3 * (time.perf_counter() - t0)
     xyzh0.copy_from(xyzh)  # wrapped positions, initial h
     # a second build on the same context: buffers exist, as in a time step
     t0 = time.perf_counter()
-    ctx.maketree(xyzh)
+    ctx.maketree(xyzh, leaf=False)
     tree2_ms = 1e3 * (time.perf_counter() - t0)
     tree_kernels = {}
     for nm, ms in ctx.timings():

@@ -137,8 +137,10 @@ class Oracle:
         if rc != 0:
             raise OracleError(rc, lib().orc_last_error(self._ctx).decode())
 
-    def maketree(self, xyzh, iphase=None):
-        """xyzh (n,4) float64 C-contiguous, modified in place (periodic wrap)."""
+    def maketree(self, xyzh, iphase=None, leaf=True):
+        """xyzh (n,4) float64 C-contiguous, modified in place (periodic wrap).
+        leaf is accepted for the call surface of phantom_amd.sph.Context; the
+        tree is read with get_tree."""
         assert xyzh.dtype == np.float64 and xyzh.flags.c_contiguous and xyzh.shape[1] == 4
         self.n = xyzh.shape[0]
         self._check(lib().orc_maketree(self._ctx, _ptr(xyzh), self.n, _ptr(iphase)))

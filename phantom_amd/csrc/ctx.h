@@ -38,6 +38,24 @@ struct DBuf {
   }
 };
 
+// One node of the tree build's large-node phase (more particles than one
+// workgroup builds in LDS): its range, split decision and chunk span.
+struct BigNode {
+  double com[3];   // centre of mass (parallel sums, or sequential after amb)
+  double tot;      // node mass
+  double hmax;
+  double delta;    // bound on |sequential - parallel| centre on the split axis
+  double pivot;    // com[iaxis]
+  int lo, hi;      // particle range (tree positions, inclusive)
+  int nnode;       // node id
+  int iaxis;       // split axis
+  int chunk0, nchunk;  // chunks of this node (0 for a small node)
+  int nl;          // particles in the left child (0: no children)
+  int part;        // the two-pointer partition moves particles
+  int nbad;        // swaps of the partition
+  int pad_;
+};
+
 struct TimedLaunch {
   std::string name;
   hipEvent_t t0, t1;
@@ -85,6 +103,11 @@ struct phsph_ctx {
   DBuf<int32_t> tmpi, tmpi2;  // scratch (tree build)
   DBuf<int32_t> frontier_a, frontier_b;  // node ids of a tree level
   DBuf<double> fbox_a, fbox_b;           // their bounding boxes (6 each)
+  DBuf<BigNode> bnode;       // large nodes of a level (one per frontier entry)
+  DBuf<int32_t> ch_node;     // chunk -> frontier entry
+  DBuf<int32_t> ch_i;        // per-chunk counts (4 each)
+  DBuf<int32_t> ch_off;      // per-chunk partition offsets (2 each)
+  DBuf<double> ch_red;       // per-chunk partial sums and bounds (12 each)
   DBuf<int32_t> counters;    // small device counters
   DBuf<double> dred;         // small device reductions
 

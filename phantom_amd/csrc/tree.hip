@@ -25,10 +25,9 @@
 #define SUB_MAX 1024  // particles per LDS subtree
 #define SUB_BS 256
 #define SUB_QMAX 192  // >= 2*floor(SUB_MAX/(MINPART+1)) nodes per level
-#define BIG_BS 512
 
 enum { CNT_NEXT = 0, CNT_ERR = 1, CNT_FALLBACK = 2, CNT_MAXLEVEL = 3, CNT_MINLEVEL = 4, CNT_NAN = 5,
-       CNT_NLEAF = 6, CNT_LIVE = 7, CNT_N = 8 };
+       CNT_NLEAF = 6, CNT_LIVE = 7, CNT_NCHUNK = 8, CNT_NBIG = 9, CNT_SEQMAX = 10, CNT_N = 11 };
 enum { TERR_NODEID = 1, TERR_QUEUE = 2, TERR_COUNT = 4 };
 
 // ------------------------------------------------------------------------
@@ -221,13 +220,20 @@ __device__ __forceinline__ int pick_axis(const double *box) {
 }
 
 // ------------------------------------------------------------------------
-// large nodes: one workgroup of BIG_BS threads per node.  The streaming
-// passes take BIG_U elements per thread per iteration with independent
-// loads, so that a node of a million particles keeps enough loads in
-// flight on its one CU.
+// large nodes (more than SUB_MAX particles).  Every large node of a level is
+// cut into chunks of BN_CH consecutive tree positions, and each pass of
+// construct_node runs over all chunks of all large nodes of the level at
+// once, one workgroup per chunk.  One wavefront per node combines its chunk
+// results in a fixed order.  Per level: plan -> sums -> centre -> ambiguity
+// test -> sequential sums (ambiguous nodes only) -> size and left count ->
+// node record -> partition (count, scan, scatter, swap) -> child boxes ->
+// children.
 // ------------------------------------------------------------------------
-#define BIG_U 8
-#define BIG_CHUNK 2048
+#define BN_CH 2048
+#define BN_BS 256
+#define BN_U (BN_CH / BN_BS)
+#define BN_SEQ 1024     // positions per LDS stage of the sequential sums
+#define BN_SEQ_BS 512   // wavefront 0 sums, the others stage
 
 // exclusive scan of per-thread counts over the block (block total in *total)
 template <int BS>
@@ -251,234 +257,553 @@ __device__ int block_excl_scan_int(int v, int *shi, int *total) {
   return off + incl - v;
 }
 
+// K per-thread values reduced over the block in a fixed order (kind 0: sum,
+// 1: max, 2: min); the result is valid in thread 0
+template <int BS, int K>
+__device__ void block_reduce(double (&v)[K], const int (&kind)[K], double *sh) {
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    double x = v[k];
+    for (int o = 32; o > 0; o >>= 1) {
+      const double y = __shfl_xor(x, o, 64);
+      x = (kind[k] == 0) ? x + y : (kind[k] == 1) ? fmax(x, y) : fmin(x, y);
+    }
+    v[k] = x;
+  }
+  __syncthreads();
+  if (l == 0)
+    for (int k = 0; k < K; k++) sh[w * K + k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 0; k < K; k++) {
+      double x = sh[k];
+      for (int ww = 1; ww < BS / 64; ww++) {
+        const double y = sh[ww * K + k];
+        x = (kind[k] == 0) ? x + y : (kind[k] == 1) ? fmax(x, y) : fmin(x, y);
+      }
+      v[k] = x;
+    }
+}
+
 __device__ __forceinline__ const double *axis_arr(int a, const double *tx, const double *ty, const double *tz) {
   return (a == 0) ? tx : (a == 1) ? ty : tz;
 }
 
-__global__ void __launch_bounds__(BIG_BS) k_big_nodes(DevConst dc, const int32_t *frontier, const double *fbox,
-                                                      int nfront, double *tx, double *ty, double *tz, double *th,
-                                                      int32_t *iparts, int32_t *nd_lo, int32_t *nd_hi,
-                                                      phsph_kdnode *nd, int32_t *leaf_flag, int32_t *tmpA,
-                                                      int32_t *tmpB, int level, int maxlevel_indexed,
-                                                      int64_t nslots, int32_t *next_front, double *next_box,
-                                                      int32_t *counters) {
-  __shared__ double sh[BIG_BS / 64];
-  __shared__ int shi[BIG_BS / 64];
-  __shared__ double chunk[4][BIG_CHUNK];
-  const int f = blockIdx.x;
-  const int nnode = frontier[f];
-  const int lo = nd_lo[nnode], hi = nd_hi[nnode];
-  const int n = hi - lo + 1;
-  if (n <= SUB_MAX) return;  // handled by the subtree kernel
-  const int tid = threadIdx.x;
+// chunk c of the level: its node (frontier entry) and positions [p0, p1];
+// false for a block past the level's chunk count
+__device__ __forceinline__ bool bn_chunk(const BigNode *bn, const int32_t *ch_node, const int32_t *counters,
+                                         int maxch, int c, int &f, int &p0, int &p1) {
+  const int nch = counters[CNT_NCHUNK];
+  if (nch > maxch || c >= nch) return false;
+  f = ch_node[c];
+  p0 = bn[f].lo + (c - bn[f].chunk0) * BN_CH;
+  p1 = min(bn[f].hi, p0 + BN_CH - 1);
+  return true;
+}
+
+// the level's large nodes, their chunks and split axes
+__global__ void __launch_bounds__(1024) k_bn_plan(const int32_t *frontier, const double *fbox, int nfront,
+                                                  const int32_t *nd_lo, const int32_t *nd_hi, BigNode *bn,
+                                                  int32_t *ch_node, int maxch, int32_t *counters) {
+  __shared__ int shi[16];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nfront; base += 1024) {
+    const int f = base + threadIdx.x;
+    int nch = 0, lo = 0, hi = -1, nnode = 0;
+    if (f < nfront) {
+      nnode = frontier[f];
+      lo = nd_lo[nnode];
+      hi = nd_hi[nnode];
+      const int n = hi - lo + 1;
+      nch = (n > SUB_MAX) ? (n + BN_CH - 1) / BN_CH : 0;
+    }
+    int tot;
+    const int off = carry + block_excl_scan_int<1024>(nch, shi, &tot);
+    if (f < nfront) {
+      BigNode b;
+      for (int k = 0; k < 3; k++) b.com[k] = 0.;
+      b.tot = 0.; b.hmax = 0.; b.delta = 0.; b.pivot = 0.;
+      b.lo = lo; b.hi = hi; b.nnode = nnode;
+      b.iaxis = pick_axis(fbox + (size_t)f * 6);
+      b.chunk0 = off; b.nchunk = nch;
+      b.nl = 0; b.part = 0; b.nbad = 0; b.pad_ = 0;
+      bn[f] = b;
+      for (int k = 0; k < nch; k++)
+        if (off + k < maxch) ch_node[off + k] = f;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    counters[CNT_NCHUNK] = carry;
+    if (carry > maxch) atomicOr(&counters[CNT_ERR], TERR_QUEUE);
+  }
+}
+
+// parallel sums of m*x over a chunk, their absolute values, and hmax
+__global__ void __launch_bounds__(BN_BS) k_bn_sums(DevConst dc, const BigNode *bn, const int32_t *ch_node,
+                                                   int maxch, const int32_t *counters, const double *tx,
+                                                   const double *ty, const double *tz, const double *th,
+                                                   double *ch_red) {
+  __shared__ double sh[BN_BS / 64 * 7];
+  const int c = blockIdx.x;
+  int f, p0, p1;
+  if (!bn_chunk(bn, ch_node, counters, maxch, c, f, p0, p1)) return;
   const double pm = dc.pmass;
   const double dfac = (pm > 0.) ? 1. / pm : 1.;
   const double fac = xmul(pm, dfac);
-  double box[6];
-  for (int c = 0; c < 6; c++) box[c] = fbox[(size_t)f * 6 + c];
-  constexpr int STEP = BIG_BS * BIG_U;
+  double v[7] = {0., 0., 0., 0., 0., 0., 0.};
+#pragma unroll
+  for (int u = 0; u < BN_U; u++) {
+    const int p = p0 + u * BN_BS + threadIdx.x;
+    if (p <= p1) {
+      const double vx = xmul(fac, tx[p]), vy = xmul(fac, ty[p]), vz = xmul(fac, tz[p]);
+      v[0] += vx; v[1] += vy; v[2] += vz;
+      v[3] += fabs(vx); v[4] += fabs(vy); v[5] += fabs(vz);
+      v[6] = fmax(v[6], th[p]);
+    }
+  }
+  const int kind[7] = {0, 0, 0, 0, 0, 0, 1};
+  block_reduce<BN_BS, 7>(v, kind, sh);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 7; k++) ch_red[(size_t)c * 12 + k] = v[k];
+}
 
-  // parallel sums + bound
-  double s[3] = {0., 0., 0.}, a[3] = {0., 0., 0.}, hmax = 0.;
-  for (int base = lo; base <= hi; base += STEP) {
-#pragma unroll
-    for (int u = 0; u < BIG_U; u++) {
-      const int p = base + u * BIG_BS + tid;
-      if (p <= hi) {
-        const double vx = xmul(fac, tx[p]), vy = xmul(fac, ty[p]), vz = xmul(fac, tz[p]);
-        s[0] += vx; s[1] += vy; s[2] += vz;
-        a[0] += fabs(vx); a[1] += fabs(vy); a[2] += fabs(vz);
-        hmax = fmax(hmax, th[p]);
-      }
-    }
+// centre of mass from the parallel sums, and the bound on its distance from
+// the sequential one on the split axis (|S_seq - S| <= 2 gamma_{n-1} sum|a_i|)
+__global__ void k_bn_centre(DevConst dc, BigNode *bn, int maxch, const int32_t *counters, const double *ch_red) {
+  const int f = blockIdx.x, lane = threadIdx.x;
+  if (counters[CNT_NCHUNK] > maxch) return;
+  BigNode *b = bn + f;
+  const int nchunk = b->nchunk, chunk0 = b->chunk0;
+  if (nchunk == 0) return;
+  double s[3] = {0., 0., 0.}, a[3] = {0., 0., 0.}, hm = 0.;
+  for (int k = lane; k < nchunk; k += 64) {
+    const double *r = ch_red + (size_t)(chunk0 + k) * 12;
+    s[0] += r[0]; s[1] += r[1]; s[2] += r[2];
+    a[0] += r[3]; a[1] += r[4]; a[2] += r[5];
+    hm = fmax(hm, r[6]);
   }
-  for (int c = 0; c < 3; c++) {
-    s[c] = block_sum<BIG_BS>(s[c], sh);
-    a[c] = block_sum<BIG_BS>(a[c], sh);
+  for (int k = 0; k < 3; k++) {
+    s[k] = wave_sum(s[k]);
+    a[k] = wave_sum(a[k]);
   }
-  hmax = block_max<BIG_BS>(hmax, sh);
-  double tot = xmul((double)n, pm);
-  double D = xmul(tot, dfac);
-  double com[3];
-  for (int c = 0; c < 3; c++) com[c] = s[c] / D;
-  const double uu = 1.1102230246251565e-16;
-  const double nu = (double)(n - 1) * uu;
-  const double gam = nu / (1. - nu);
-  const int iaxis = pick_axis(box);
-  const double *tax = axis_arr(iaxis, tx, ty, tz);
-  const double a_ax = (iaxis == 0) ? a[0] : (iaxis == 1) ? a[1] : a[2];
-  const double c_ax = (iaxis == 0) ? com[0] : (iaxis == 1) ? com[1] : com[2];
-  double delta = 2. * (2. * gam * a_ax / D + (2. * gam + 4. * uu) * fabs(c_ax)) + 1e-300;
-  // any particle within delta of the plane?
+  hm = wave_max(hm);
+  if (lane == 0) {
+    const int n = b->hi - b->lo + 1;
+    const double pm = dc.pmass;
+    const double dfac = (pm > 0.) ? 1. / pm : 1.;
+    const double tot = xmul((double)n, pm);
+    const double D = xmul(tot, dfac);
+    double com[3];
+    for (int k = 0; k < 3; k++) com[k] = s[k] / D;
+    const double uu = 1.1102230246251565e-16;
+    const double nu = (double)(n - 1) * uu;
+    const double gam = nu / (1. - nu);
+    const int ia = b->iaxis;
+    const double a_ax = a[ia], c_ax = com[ia];
+    for (int k = 0; k < 3; k++) b->com[k] = com[k];
+    b->tot = tot;
+    b->hmax = hm;
+    b->delta = 2. * (2. * gam * a_ax / D + (2. * gam + 4. * uu) * fabs(c_ax)) + 1e-300;
+    b->pivot = c_ax;
+  }
+}
+
+// does any particle of the chunk lie within delta of the split plane?
+__global__ void __launch_bounds__(BN_BS) k_bn_amb(const BigNode *bn, const int32_t *ch_node, int maxch,
+                                                  const int32_t *counters, const double *tx, const double *ty,
+                                                  const double *tz, int32_t *ch_i) {
+  const int c = blockIdx.x;
+  int f, p0, p1;
+  if (!bn_chunk(bn, ch_node, counters, maxch, c, f, p0, p1)) return;
+  const double *tax = axis_arr(bn[f].iaxis, tx, ty, tz);
+  const double c_ax = bn[f].pivot, delta = bn[f].delta;
   int amb = 0;
-  for (int base = lo; base <= hi; base += STEP) {
 #pragma unroll
-    for (int u = 0; u < BIG_U; u++) {
-      const int p = base + u * BIG_BS + tid;
-      if (p <= hi) amb |= (fabs(tax[p] - c_ax) <= delta);
+  for (int u = 0; u < BN_U; u++) {
+    const int p = p0 + u * BN_BS + threadIdx.x;
+    if (p <= p1) amb |= (fabs(tax[p] - c_ax) <= delta);
+  }
+  amb = __syncthreads_or(amb);
+  if (threadIdx.x == 0) ch_i[(size_t)c * 4 + 0] = amb;
+}
+
+// cnt additions of one running sum in order, from an LDS row, with the
+// next 16 values loaded while the current 16 are added (two register sets
+// in turn).  Reads up to 15 values past cnt: the rows are BN_SEQ_LD long.
+__device__ __forceinline__ double seq_chain(double acc, const double *src, int cnt) {
+  int k = 0;
+  if (cnt >= 16) {
+    double v[16], w[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = src[u];
+    while (true) {
+#pragma unroll
+      for (int u = 0; u < 16; u++) w[u] = src[k + 16 + u];
+#pragma unroll
+      for (int u = 0; u < 16; u++) acc = xadd(acc, v[u]);
+      k += 16;
+      if (k + 16 > cnt) break;
+#pragma unroll
+      for (int u = 0; u < 16; u++) v[u] = src[k + 16 + u];
+#pragma unroll
+      for (int u = 0; u < 16; u++) acc = xadd(acc, w[u]);
+      k += 16;
+      if (k + 16 > cnt) break;
     }
   }
-  int namb;
-  block_excl_scan<BIG_BS>(amb, shi, &namb);
-  if (namb > 0) {
-    // exact sequential sums (kdtree.F90:655-665): the products fac*x are
-    // independent and are formed by all threads into LDS; the four running
-    // sums (x, y, z, mass) are four dependent chains, run by lanes 0-3 of
-    // the first wavefront in lockstep, one addition per particle each
-    __shared__ double seq[4];
-    double acc = 0.;
-    for (int base = lo; base <= hi; base += BIG_CHUNK) {
-      const int cnt = min(BIG_CHUNK, hi - base + 1);
-      __syncthreads();
-      for (int k = tid; k < cnt; k += BIG_BS) {
-        chunk[0][k] = xmul(fac, tx[base + k]);
-        chunk[1][k] = xmul(fac, ty[base + k]);
-        chunk[2][k] = xmul(fac, tz[base + k]);
-        chunk[3][k] = pm;
-      }
-      __syncthreads();
-      if (tid < 4) {
-        // row tid: x, y, z products or the masses; LDS reads issued ahead
-        // of the dependent additions, no branch per element
-        const double *src = chunk[tid];
-        int k = 0;
-        for (; k + 8 <= cnt; k += 8) {
-          double v[8];
-#pragma unroll
-          for (int u = 0; u < 8; u++) v[u] = src[k + u];
-#pragma unroll
-          for (int u = 0; u < 8; u++) acc = xadd(acc, v[u]);
-        }
-        for (; k < cnt; k++) acc = xadd(acc, src[k]);
-      }
+  for (; k < cnt; k++) acc = xadd(acc, src[k]);
+  return acc;
+}
+
+// exact sequential sums (kdtree.F90:655-665) of the nodes with a particle
+// within delta of the plane.  The four running sums (x, y, z, mass) are four
+// dependent chains in lanes 0-3 of wavefront 0; the other wavefronts stage
+// the products fac*x of the next BN_SEQ positions into LDS meanwhile.  The
+// rows are padded so that the four lanes read distinct LDS banks.
+#define BN_SEQ_LD (BN_SEQ + 20)
+__global__ void __launch_bounds__(BN_SEQ_BS) k_bn_seq(DevConst dc, BigNode *bn, int maxch, int32_t *counters,
+                                                      const int32_t *ch_i, const double *tx, const double *ty,
+                                                      const double *tz) {
+  __shared__ double buf[2][3][BN_SEQ_LD];
+  __shared__ double mrow[BN_SEQ_LD];
+  __shared__ double seq[4];
+  const int f = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+  if (counters[CNT_NCHUNK] > maxch) return;
+  const int nchunk = bn[f].nchunk, chunk0 = bn[f].chunk0;
+  if (nchunk == 0) return;
+  int amb = 0;
+  for (int k = tid; k < nchunk; k += BN_SEQ_BS) amb |= ch_i[(size_t)(chunk0 + k) * 4 + 0];
+  amb = __syncthreads_or(amb);
+  if (!amb) return;
+  const int lo = bn[f].lo, hi = bn[f].hi, iaxis = bn[f].iaxis;
+  const double pm = dc.pmass;
+  const double dfac = (pm > 0.) ? 1. / pm : 1.;
+  const double fac = xmul(pm, dfac);
+  const int n = hi - lo + 1;
+  const int nst = (n + BN_SEQ - 1) / BN_SEQ;
+  auto fill = [&](int st) {
+    const int base = lo + st * BN_SEQ;
+    const int cnt = min(BN_SEQ, hi - base + 1);
+    double(*dst)[BN_SEQ_LD] = buf[st & 1];
+    for (int k = tid - 64; k < cnt; k += BN_SEQ_BS - 64) {
+      dst[0][k] = xmul(fac, tx[base + k]);
+      dst[1][k] = xmul(fac, ty[base + k]);
+      dst[2][k] = xmul(fac, tz[base + k]);
     }
-    if (tid < 4) seq[tid] = acc;
+  };
+  for (int k = tid; k < BN_SEQ_LD; k += BN_SEQ_BS) mrow[k] = pm;
+  if (tid >= 64) fill(0);
+  __syncthreads();
+  double acc = 0.;
+  for (int st = 0; st < nst; st++) {
+    if (tid >= 64) {
+      if (st + 1 < nst) fill(st + 1);
+    } else if (lane < 4) {
+      const int cnt = min(BN_SEQ, hi - (lo + st * BN_SEQ) + 1);
+      acc = seq_chain(acc, lane < 3 ? buf[st & 1][lane] : mrow, cnt);
+    }
     __syncthreads();
-    tot = seq[3];
-    D = xmul(tot, dfac);
-    for (int c = 0; c < 3; c++) com[c] = seq[c] / D;
-    if (tid == 0) atomicAdd(&counters[CNT_FALLBACK], 1);
   }
-  // size: sqrt(max r^2) + epsilon (kdtree.F90:697-782), and the count of
-  // particles on the left of the pivot
-  const double pivot = (iaxis == 0) ? com[0] : (iaxis == 1) ? com[1] : com[2];
-  double r2max = 0.;
-  int cl = 0;
-  for (int base = lo; base <= hi; base += STEP) {
+  if (tid < 4) seq[tid] = acc;
+  __syncthreads();
+  if (tid == 0) {
+    const double tot = seq[3];
+    const double D = xmul(tot, dfac);
+    BigNode *b = bn + f;
+    for (int k = 0; k < 3; k++) b->com[k] = seq[k] / D;
+    b->tot = tot;
+    b->pivot = b->com[iaxis];
+    atomicAdd(&counters[CNT_FALLBACK], 1);
+    atomicMax(&counters[CNT_SEQMAX], n);
+  }
+}
+
+// size (kdtree.F90:697-782) and the count of particles on the left of the
+// pivot, per chunk
+__global__ void __launch_bounds__(BN_BS) k_bn_size(const BigNode *bn, const int32_t *ch_node, int maxch,
+                                                   const int32_t *counters, const double *tx, const double *ty,
+                                                   const double *tz, double *ch_red, int32_t *ch_i) {
+  __shared__ double sh[BN_BS / 64 * 2];
+  const int c = blockIdx.x;
+  int f, p0, p1;
+  if (!bn_chunk(bn, ch_node, counters, maxch, c, f, p0, p1)) return;
+  const double c0 = bn[f].com[0], c1 = bn[f].com[1], c2 = bn[f].com[2], pivot = bn[f].pivot;
+  const double *tax = axis_arr(bn[f].iaxis, tx, ty, tz);
+  double v[2] = {0., 0.};
 #pragma unroll
-    for (int u = 0; u < BIG_U; u++) {
-      const int p = base + u * BIG_BS + tid;
-      if (p <= hi) {
-        const double r2 = r2_exact(xsub(tx[p], com[0]), xsub(ty[p], com[1]), xsub(tz[p], com[2]));
-        r2max = fmax(r2max, r2);
-        cl += (tax[p] <= pivot);
-      }
+  for (int u = 0; u < BN_U; u++) {
+    const int p = p0 + u * BN_BS + threadIdx.x;
+    if (p <= p1) {
+      v[0] = fmax(v[0], r2_exact(xsub(tx[p], c0), xsub(ty[p], c1), xsub(tz[p], c2)));
+      v[1] += (tax[p] <= pivot) ? 1. : 0.;
     }
   }
-  r2max = block_max<BIG_BS>(r2max, sh);
-  const int il = 2 * nnode, ir = il + 1;
-  if (tid == 0) {
+  const int kind[2] = {1, 0};
+  block_reduce<BN_BS, 2>(v, kind, sh);
+  if (threadIdx.x == 0) {
+    ch_red[(size_t)c * 12 + 0] = v[0];
+    ch_i[(size_t)c * 4 + 1] = (int)v[1];
+  }
+}
+
+// the node record, and the split: left count or, with everything on one
+// side, half and half in place (kdtree.F90:856-865)
+__global__ void k_bn_node(BigNode *bn, int maxch, int32_t *counters, const double *ch_red, const int32_t *ch_i,
+                          phsph_kdnode *nd, int32_t *leaf_flag, int level, int maxlevel_indexed, int64_t nslots) {
+  const int f = blockIdx.x, lane = threadIdx.x;
+  if (counters[CNT_NCHUNK] > maxch) return;
+  BigNode *b = bn + f;
+  const int nchunk = b->nchunk, chunk0 = b->chunk0;
+  if (nchunk == 0) return;
+  double r2 = 0.;
+  int cl = 0;
+  for (int k = lane; k < nchunk; k += 64) {
+    r2 = fmax(r2, ch_red[(size_t)(chunk0 + k) * 12 + 0]);
+    cl += ch_i[(size_t)(chunk0 + k) * 4 + 1];
+  }
+  r2 = wave_max(r2);
+  cl = wave_sum_i(cl);
+  if (lane == 0) {
+    const int nnode = b->nnode, il = 2 * nnode, ir = il + 1;
+    const int n = b->hi - b->lo + 1;
     phsph_kdnode e;
-    e.xcen[0] = com[0]; e.xcen[1] = com[1]; e.xcen[2] = com[2];
-    e.size = xadd(sqrt(r2max), PH_EPS64);
-    e.hmax = hmax;
-    e.mass = tot;
+    e.xcen[0] = b->com[0]; e.xcen[1] = b->com[1]; e.xcen[2] = b->com[2];
+    e.size = xadd(sqrt(r2), PH_EPS64);
+    e.hmax = b->hmax;
+    e.mass = b->tot;
     e.leftchild = il;
     e.rightchild = ir;
     e.parent = nnode / 2;
     e.idum = 0;
     nd[nnode] = e;
     leaf_flag[nnode] = 0;
-    if (level >= maxlevel_indexed || (int64_t)ir >= nslots) atomicOr(&counters[CNT_ERR], TERR_NODEID);
+    b->nbad = 0;
+    if (level >= maxlevel_indexed || (int64_t)ir >= nslots) {
+      atomicOr(&counters[CNT_ERR], TERR_NODEID);
+      b->nl = 0;
+      b->part = 0;
+    } else if (cl > 0 && cl < n) {
+      b->nl = cl;
+      b->part = 1;
+    } else {
+      b->nl = n / 2;
+      b->part = 0;
+    }
   }
-  if (level >= maxlevel_indexed || (int64_t)ir >= nslots) return;
-  // partition (sort_particles_in_cell): the k-th element > pivot in the
-  // left part swaps with the k-th element <= pivot counted from the right
-  int nl;
-  block_excl_scan_int<BIG_BS>(cl, shi, &nl);
-  if (nl > 0 && nl < n) {
-    const int lend = lo + nl - 1;  // last position of the left part
-    int carry = 0;
-    for (int base = lo; base <= lend; base += STEP) {
-      // thread tid takes BIG_U consecutive positions, so ranks follow position
-      const int p0 = base + tid * BIG_U;
-      int b[BIG_U], c = 0;
+}
+
+// sort_particles_in_cell as a permutation: the k-th element > pivot in the
+// left part [lo, lo+nl-1] swaps with the k-th element <= pivot counted from
+// the right end of the right part.  Per chunk, the counts of both kinds.
+__global__ void __launch_bounds__(BN_BS) k_bn_pcount(const BigNode *bn, const int32_t *ch_node, int maxch,
+                                                     const int32_t *counters, const double *tx, const double *ty,
+                                                     const double *tz, int32_t *ch_i) {
+  __shared__ double sh[BN_BS / 64 * 2];
+  const int c = blockIdx.x;
+  int f, p0, p1;
+  if (!bn_chunk(bn, ch_node, counters, maxch, c, f, p0, p1)) return;
+  if (!bn[f].part) return;
+  const int lend = bn[f].lo + bn[f].nl - 1;
+  const double pivot = bn[f].pivot;
+  const double *tax = axis_arr(bn[f].iaxis, tx, ty, tz);
+  double v[2] = {0., 0.};
 #pragma unroll
-      for (int u = 0; u < BIG_U; u++) {
-        const int p = p0 + u;
-        b[u] = (p <= lend) ? !(tax[p] <= pivot) : 0;
-        c += b[u];
-      }
-      int t;
-      int r = block_excl_scan_int<BIG_BS>(c, shi, &t);
-#pragma unroll
-      for (int u = 0; u < BIG_U; u++)
-        if (b[u]) tmpA[lo + carry + r++] = p0 + u;
-      carry += t;
+  for (int u = 0; u < BN_U; u++) {
+    const int p = p0 + u * BN_BS + threadIdx.x;
+    if (p <= p1) {
+      const bool le = tax[p] <= pivot;
+      if (p <= lend) v[0] += le ? 0. : 1.;
+      else v[1] += le ? 1. : 0.;
     }
-    const int nbad = carry;
-    carry = 0;
-    for (int base = 0; base < hi - lend; base += STEP) {
-      const int q0 = base + tid * BIG_U;  // distance from the right end
-      int b[BIG_U], c = 0;
-#pragma unroll
-      for (int u = 0; u < BIG_U; u++) {
-        const int p = hi - (q0 + u);
-        b[u] = (p > lend) ? (tax[p] <= pivot) : 0;
-        c += b[u];
-      }
-      int t;
-      int r = block_excl_scan_int<BIG_BS>(c, shi, &t);
-#pragma unroll
-      for (int u = 0; u < BIG_U; u++)
-        if (b[u]) tmpB[lo + carry + r++] = hi - (q0 + u);
-      carry += t;
+  }
+  const int kind[2] = {0, 0};
+  block_reduce<BN_BS, 2>(v, kind, sh);
+  if (threadIdx.x == 0) {
+    ch_i[(size_t)c * 4 + 2] = (int)v[0];
+    ch_i[(size_t)c * 4 + 3] = (int)v[1];
+  }
+}
+
+// per node: chunk offsets of both kinds in position order
+__global__ void k_bn_pscan(BigNode *bn, int maxch, int32_t *counters, const int32_t *ch_i, int32_t *ch_off) {
+  const int f = blockIdx.x, lane = threadIdx.x;
+  if (counters[CNT_NCHUNK] > maxch) return;
+  BigNode *b = bn + f;
+  if (b->nchunk == 0 || !b->part) return;
+  const int nchunk = b->nchunk, chunk0 = b->chunk0;
+  int carry_l = 0, carry_r = 0;
+  for (int base = 0; base < nchunk; base += 64) {
+    const int k = base + lane;
+    int vl = 0, vr = 0;
+    if (k < nchunk) {
+      vl = ch_i[(size_t)(chunk0 + k) * 4 + 2];
+      vr = ch_i[(size_t)(chunk0 + k) * 4 + 3];
     }
-    if (carry != nbad && tid == 0) atomicOr(&counters[CNT_ERR], TERR_COUNT);
-    __syncthreads();
-    for (int k = tid; k < nbad; k += BIG_BS) {
+    int il = vl, ir = vr;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int ul = __shfl_up(il, o, 64), ur = __shfl_up(ir, o, 64);
+      if (lane >= o) { il += ul; ir += ur; }
+    }
+    if (k < nchunk) {
+      ch_off[(size_t)(chunk0 + k) * 2 + 0] = carry_l + il - vl;
+      ch_off[(size_t)(chunk0 + k) * 2 + 1] = carry_r + ir - vr;
+    }
+    carry_l += __shfl(il, 63, 64);
+    carry_r += __shfl(ir, 63, 64);
+  }
+  if (lane == 0) {
+    b->nbad = min(carry_l, carry_r);
+    if (carry_l != carry_r) atomicOr(&counters[CNT_ERR], TERR_COUNT);
+  }
+}
+
+// ranks of the misplaced elements: the k-th left one goes to tmpA[lo+k],
+// the k-th right one counted from the right end to tmpB[lo+k]
+__global__ void __launch_bounds__(BN_BS) k_bn_pscatter(const BigNode *bn, const int32_t *ch_node, int maxch,
+                                                       const int32_t *counters, const double *tx,
+                                                       const double *ty, const double *tz, const int32_t *ch_off,
+                                                       int32_t *tmpA, int32_t *tmpB) {
+  __shared__ int shi[BN_BS / 64];
+  const int c = blockIdx.x;
+  int f, p0, p1;
+  if (!bn_chunk(bn, ch_node, counters, maxch, c, f, p0, p1)) return;
+  if (!bn[f].part) return;
+  const int lo = bn[f].lo, lend = lo + bn[f].nl - 1, nbad = bn[f].nbad;
+  const double pivot = bn[f].pivot;
+  const double *tax = axis_arr(bn[f].iaxis, tx, ty, tz);
+  const int q0 = p0 + threadIdx.x * BN_U;  // BN_U consecutive positions per thread: ranks follow positions
+  int fl[BN_U], fr[BN_U], nl_ = 0, nr_ = 0;
+#pragma unroll
+  for (int u = 0; u < BN_U; u++) {
+    const int p = q0 + u;
+    fl[u] = 0;
+    fr[u] = 0;
+    if (p <= p1) {
+      const bool le = tax[p] <= pivot;
+      if (p <= lend) fl[u] = le ? 0 : 1;
+      else fr[u] = le ? 1 : 0;
+    }
+    nl_ += fl[u];
+    nr_ += fr[u];
+  }
+  int tl, tr;
+  int rl = ch_off[(size_t)c * 2 + 0] + block_excl_scan_int<BN_BS>(nl_, shi, &tl);
+  int rr = ch_off[(size_t)c * 2 + 1] + block_excl_scan_int<BN_BS>(nr_, shi, &tr);
+#pragma unroll
+  for (int u = 0; u < BN_U; u++) {
+    if (fl[u]) {
+      const int k = rl++;
+      if (k < nbad) tmpA[lo + k] = q0 + u;
+    }
+    if (fr[u]) {
+      const int k = nbad - 1 - rr++;
+      if (k >= 0) tmpB[lo + k] = q0 + u;
+    }
+  }
+}
+
+// the swaps: chunk c of a node takes pairs k0 .. k0+BN_CH-1
+__global__ void __launch_bounds__(BN_BS) k_bn_swap(const BigNode *bn, const int32_t *ch_node, int maxch,
+                                                   const int32_t *counters, const int32_t *tmpA,
+                                                   const int32_t *tmpB, double *tx, double *ty, double *tz,
+                                                   double *th, int32_t *iparts) {
+  const int c = blockIdx.x;
+  int f, p0, p1;
+  if (!bn_chunk(bn, ch_node, counters, maxch, c, f, p0, p1)) return;
+  if (!bn[f].part) return;
+  const int lo = bn[f].lo, hi = bn[f].hi, nbad = bn[f].nbad;
+  const int k0 = p0 - lo;
+#pragma unroll
+  for (int u = 0; u < BN_U; u++) {
+    const int k = k0 + u * BN_BS + threadIdx.x;
+    if (k < nbad) {
       const int pa = tmpA[lo + k], pb = tmpB[lo + k];
-      const double t0 = tx[pa], t1 = ty[pa], t2 = tz[pa], t3 = th[pa];
-      const int32_t ti = iparts[pa];
-      tx[pa] = tx[pb]; ty[pa] = ty[pb]; tz[pa] = tz[pb]; th[pa] = th[pb]; iparts[pa] = iparts[pb];
-      tx[pb] = t0; ty[pb] = t1; tz[pb] = t2; th[pb] = t3; iparts[pb] = ti;
-    }
-    __syncthreads();
-  } else {
-    nl = n / 2;  // all on one side: split in half in place (kdtree.F90:856-865)
-  }
-  const int lA = lo, hA = lo + nl - 1, lB = lo + nl, hB = hi;
-  double bmn[2][3], bmx[2][3];
-  for (int side = 0; side < 2; side++) {
-    const int l0 = side ? lB : lA, h0 = side ? hB : hA;
-    double m0[3] = {PH_HUGE64, PH_HUGE64, PH_HUGE64}, m1[3] = {-PH_HUGE64, -PH_HUGE64, -PH_HUGE64};
-    for (int base = l0; base <= h0; base += STEP) {
-#pragma unroll
-      for (int u = 0; u < BIG_U; u++) {
-        const int p = base + u * BIG_BS + tid;
-        if (p <= h0) {
-          const double x = tx[p], y = ty[p], z = tz[p];
-          m0[0] = fmin(m0[0], x); m0[1] = fmin(m0[1], y); m0[2] = fmin(m0[2], z);
-          m1[0] = fmax(m1[0], x); m1[1] = fmax(m1[1], y); m1[2] = fmax(m1[2], z);
-        }
+      if (pa >= lo && pa <= hi && pb >= lo && pb <= hi) {
+        const double t0 = tx[pa], t1 = ty[pa], t2 = tz[pa], t3 = th[pa];
+        const int32_t ti = iparts[pa];
+        tx[pa] = tx[pb]; ty[pa] = ty[pb]; tz[pa] = tz[pb]; th[pa] = th[pb]; iparts[pa] = iparts[pb];
+        tx[pb] = t0; ty[pb] = t1; tz[pb] = t2; th[pb] = t3; iparts[pb] = ti;
       }
     }
-    for (int c = 0; c < 3; c++) {
-      bmn[side][c] = block_min<BIG_BS>(m0[c], sh);
-      bmx[side][c] = block_max<BIG_BS>(m1[c], sh);
+  }
+}
+
+// bounding boxes of both children, per chunk
+__global__ void __launch_bounds__(BN_BS) k_bn_box(const BigNode *bn, const int32_t *ch_node, int maxch,
+                                                  const int32_t *counters, const double *tx, const double *ty,
+                                                  const double *tz, double *ch_red) {
+  __shared__ double sh[BN_BS / 64 * 12];
+  const int c = blockIdx.x;
+  int f, p0, p1;
+  if (!bn_chunk(bn, ch_node, counters, maxch, c, f, p0, p1)) return;
+  if (bn[f].nl == 0) return;
+  const int hA = bn[f].lo + bn[f].nl - 1;
+  double v[12];
+  for (int k = 0; k < 3; k++) {
+    v[k] = PH_HUGE64; v[3 + k] = -PH_HUGE64;
+    v[6 + k] = PH_HUGE64; v[9 + k] = -PH_HUGE64;
+  }
+#pragma unroll
+  for (int u = 0; u < BN_U; u++) {
+    const int p = p0 + u * BN_BS + threadIdx.x;
+    if (p <= p1) {
+      const double r[3] = {tx[p], ty[p], tz[p]};
+      const int o = (p <= hA) ? 0 : 6;
+      for (int k = 0; k < 3; k++) {
+        v[o + k] = fmin(v[o + k], r[k]);
+        v[o + 3 + k] = fmax(v[o + 3 + k], r[k]);
+      }
     }
   }
-  if (tid == 0) {
+  const int kind[12] = {2, 2, 2, 1, 1, 1, 2, 2, 2, 1, 1, 1};
+  block_reduce<BN_BS, 12>(v, kind, sh);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 12; k++) ch_red[(size_t)c * 12 + k] = v[k];
+}
+
+// children: ranges, boxes, next frontier
+__global__ void k_bn_finish(const BigNode *bn, int maxch, int32_t *counters, const double *ch_red,
+                            int32_t *nd_lo, int32_t *nd_hi, int32_t *next_front, double *next_box) {
+  const int f = blockIdx.x, lane = threadIdx.x;
+  if (counters[CNT_NCHUNK] > maxch) return;
+  const BigNode *b = bn + f;
+  const int nchunk = b->nchunk, chunk0 = b->chunk0, nl = b->nl;
+  if (nchunk == 0 || nl == 0) return;
+  double v[12];
+  for (int k = 0; k < 3; k++) {
+    v[k] = PH_HUGE64; v[3 + k] = -PH_HUGE64;
+    v[6 + k] = PH_HUGE64; v[9 + k] = -PH_HUGE64;
+  }
+  for (int j = lane; j < nchunk; j += 64) {
+    const double *r = ch_red + (size_t)(chunk0 + j) * 12;
+    for (int k = 0; k < 3; k++) {
+      v[k] = fmin(v[k], r[k]); v[3 + k] = fmax(v[3 + k], r[3 + k]);
+      v[6 + k] = fmin(v[6 + k], r[6 + k]); v[9 + k] = fmax(v[9 + k], r[9 + k]);
+    }
+  }
+  for (int k = 0; k < 3; k++) {
+    v[k] = wave_min(v[k]); v[3 + k] = wave_max(v[3 + k]);
+    v[6 + k] = wave_min(v[6 + k]); v[9 + k] = wave_max(v[9 + k]);
+  }
+  if (lane == 0) {
+    const int il = 2 * b->nnode, ir = il + 1;
+    const int lA = b->lo, hA = b->lo + nl - 1, lB = b->lo + nl, hB = b->hi;
     nd_lo[il] = lA; nd_hi[il] = hA; nd_lo[ir] = lB; nd_hi[ir] = hB;
-    int idx = atomicAdd(&counters[CNT_NEXT], 2);
+    const int idx = atomicAdd(&counters[CNT_NEXT], 2);
     next_front[idx] = il;
     next_front[idx + 1] = ir;
-    for (int c = 0; c < 3; c++) {
-      next_box[(size_t)idx * 6 + c] = bmn[0][c];
-      next_box[(size_t)idx * 6 + 3 + c] = bmx[0][c];
-      next_box[(size_t)(idx + 1) * 6 + c] = bmn[1][c];
-      next_box[(size_t)(idx + 1) * 6 + 3 + c] = bmx[1][c];
+    for (int k = 0; k < 6; k++) {
+      next_box[(size_t)idx * 6 + k] = v[k];
+      next_box[(size_t)(idx + 1) * 6 + k] = v[6 + k];
     }
+    const int nbig = (hA - lA + 1 > SUB_MAX ? 1 : 0) + (hB - lB + 1 > SUB_MAX ? 1 : 0);
+    if (nbig) atomicAdd(&counters[CNT_NBIG], nbig);
   }
 }
 
@@ -684,13 +1009,23 @@ int tree_build(phsph_ctx *c, double *d_xyzh, int ld, int np, const int8_t *d_iph
   PH_HIPCHECK(c, c->frontier_b.ensure((size_t)np + nb + 2));
   PH_HIPCHECK(c, c->fbox_a.ensure((size_t)(np + 2) * 6));
   PH_HIPCHECK(c, c->fbox_b.ensure((size_t)(np + 2) * 6));
+  // large nodes are disjoint sets of more than SUB_MAX particles: at most
+  // np/(SUB_MAX+1) per level, twice that in a frontier, and their chunks at
+  // most np/BN_CH + one per node
+  const int bn_cap = 2 * (np / (SUB_MAX + 1)) + 4;
+  const int maxch = np / BN_CH + np / (SUB_MAX + 1) + 2;
+  PH_HIPCHECK(c, c->bnode.ensure(bn_cap));
+  PH_HIPCHECK(c, c->ch_node.ensure(maxch));
+  PH_HIPCHECK(c, c->ch_i.ensure((size_t)maxch * 4));
+  PH_HIPCHECK(c, c->ch_off.ensure((size_t)maxch * 2));
+  PH_HIPCHECK(c, c->ch_red.ensure((size_t)maxch * 12));
   int32_t *live = c->tmpi.p;
   int32_t *bcount = c->frontier_b.p;  // block counts of the compaction
 
   // the reference seeds the bounds with particle 1 before wrapping it
   double x0[3];
   PH_HIPCHECK(c, hipMemcpyAsync(x0, d_xyzh, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
-  int32_t hc[CNT_N] = {0, 0, 0, 0, 1 << 30, 0, 0, 0};
+  int32_t hc[CNT_N] = {0, 0, 0, 0, 1 << 30, 0, 0, 0, 0, 0, 0};
   PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, hc, sizeof(hc), hipMemcpyHostToDevice, s));
   PH_HIPCHECK(c, hipStreamSynchronize(s));
   {
@@ -719,30 +1054,65 @@ int tree_build(phsph_ctx *c, double *d_xyzh, int ld, int np, const int8_t *d_iph
   int32_t one = 1;
   PH_HIPCHECK(c, hipMemcpyAsync(c->frontier_a.p, &one, sizeof(int32_t), hipMemcpyHostToDevice, s));
   PH_HIPCHECK(c, hipMemcpyAsync(c->fbox_a.p, rootbox, 6 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  const bool dbg = getenv("PHSPH_DEBUG") != nullptr;
   int nfront = 1;
+  int nbig = (c->nlive > SUB_MAX) ? 1 : 0;  // large nodes in the frontier
   int level = 0;
   int32_t *fa = c->frontier_a.p, *fb = c->frontier_b.p;
   double *ba = c->fbox_a.p, *bb = c->fbox_b.p;
   while (nfront > 0) {
     int32_t zero = 0;
     PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p + CNT_NEXT, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
-    {
+    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p + CNT_NBIG, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p + CNT_SEQMAX, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    if (nbig < nfront) {
       LaunchTimer lt(c, "tree_subtrees");
       hipLaunchKernelGGL(k_subtrees, dim3(nfront), dim3(SUB_BS), 0, s, c->dc, fa, ba, nfront, c->tx.p, c->ty.p,
                          c->tz.p, c->th.p, c->iparts.p, c->nd_lo.p, c->nd_hi.p, c->nd.p, c->leaf_flag.p, level,
                          c->maxlevel_indexed, c->nslots, c->counters.p);
     }
-    {
+    if (nbig > 0) {
+      if (nfront > bn_cap) PH_RET(c, PHSPH_ERR_STATE, "maketree: large-node frontier beyond its capacity");
       LaunchTimer lt(c, "tree_bignodes");
-      hipLaunchKernelGGL(k_big_nodes, dim3(nfront), dim3(BIG_BS), 0, s, c->dc, fa, ba, nfront, c->tx.p, c->ty.p,
-                         c->tz.p, c->th.p, c->iparts.p, c->nd_lo.p, c->nd_hi.p, c->nd.p, c->leaf_flag.p,
-                         c->tmpi.p, c->tmpi2.p, level, c->maxlevel_indexed, c->nslots, fb, bb, c->counters.p);
+      // chunks <= sum over large nodes of ceil(n/BN_CH) <= nlive/BN_CH + nbig
+      const int gch = std::min(maxch, (c->nlive + BN_CH - 1) / BN_CH + nbig);
+      BigNode *bn = c->bnode.p;
+      int32_t *cnt_d = c->counters.p;
+      const int32_t *chn = c->ch_node.p;
+      hipLaunchKernelGGL(k_bn_plan, dim3(1), dim3(1024), 0, s, fa, ba, nfront, c->nd_lo.p, c->nd_hi.p, bn,
+                         c->ch_node.p, maxch, cnt_d);
+      hipLaunchKernelGGL(k_bn_sums, dim3(gch), dim3(BN_BS), 0, s, c->dc, bn, chn, maxch, cnt_d, c->tx.p, c->ty.p,
+                         c->tz.p, c->th.p, c->ch_red.p);
+      hipLaunchKernelGGL(k_bn_centre, dim3(nfront), dim3(64), 0, s, c->dc, bn, maxch, cnt_d, c->ch_red.p);
+      hipLaunchKernelGGL(k_bn_amb, dim3(gch), dim3(BN_BS), 0, s, bn, chn, maxch, cnt_d, c->tx.p, c->ty.p, c->tz.p,
+                         c->ch_i.p);
+      hipLaunchKernelGGL(k_bn_seq, dim3(nfront), dim3(BN_SEQ_BS), 0, s, c->dc, bn, maxch, cnt_d, c->ch_i.p,
+                         c->tx.p, c->ty.p, c->tz.p);
+      hipLaunchKernelGGL(k_bn_size, dim3(gch), dim3(BN_BS), 0, s, bn, chn, maxch, cnt_d, c->tx.p, c->ty.p,
+                         c->tz.p, c->ch_red.p, c->ch_i.p);
+      hipLaunchKernelGGL(k_bn_node, dim3(nfront), dim3(64), 0, s, bn, maxch, cnt_d, c->ch_red.p, c->ch_i.p,
+                         c->nd.p, c->leaf_flag.p, level, c->maxlevel_indexed, c->nslots);
+      hipLaunchKernelGGL(k_bn_pcount, dim3(gch), dim3(BN_BS), 0, s, bn, chn, maxch, cnt_d, c->tx.p, c->ty.p,
+                         c->tz.p, c->ch_i.p);
+      hipLaunchKernelGGL(k_bn_pscan, dim3(nfront), dim3(64), 0, s, bn, maxch, cnt_d, c->ch_i.p, c->ch_off.p);
+      hipLaunchKernelGGL(k_bn_pscatter, dim3(gch), dim3(BN_BS), 0, s, bn, chn, maxch, cnt_d, c->tx.p, c->ty.p,
+                         c->tz.p, c->ch_off.p, c->tmpi.p, c->tmpi2.p);
+      hipLaunchKernelGGL(k_bn_swap, dim3(gch), dim3(BN_BS), 0, s, bn, chn, maxch, cnt_d, c->tmpi.p, c->tmpi2.p,
+                         c->tx.p, c->ty.p, c->tz.p, c->th.p, c->iparts.p);
+      hipLaunchKernelGGL(k_bn_box, dim3(gch), dim3(BN_BS), 0, s, bn, chn, maxch, cnt_d, c->tx.p, c->ty.p, c->tz.p,
+                         c->ch_red.p);
+      hipLaunchKernelGGL(k_bn_finish, dim3(nfront), dim3(64), 0, s, bn, maxch, cnt_d, c->ch_red.p, c->nd_lo.p,
+                         c->nd_hi.p, fb, bb);
     }
     PH_HIPCHECK(c, hipGetLastError());
     PH_HIPCHECK(c, hipMemcpyAsync(cnt, c->counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
     PH_HIPCHECK(c, hipStreamSynchronize(s));
+    if (dbg)
+      fprintf(stderr, "maketree level %d: frontier %d, large %d, sequential sums so far %d, longest this level %d\n",
+              level, nfront, nbig, cnt[CNT_FALLBACK], cnt[CNT_SEQMAX]);
     if (cnt[CNT_ERR]) break;
     nfront = cnt[CNT_NEXT];
+    nbig = cnt[CNT_NBIG];
     std::swap(fa, fb);
     std::swap(ba, bb);
     level++;

@@ -97,7 +97,7 @@ def derivs(ctx, icall, s, timei=0.0, dt=0.0):
     Returns dtnew (deriv.f90:186-191)."""
     p = s.params
     if icall == 1:
-        ctx.maketree(s.xyzh, iphase=s.iphase)
+        ctx.maketree(s.xyzh, iphase=s.iphase, leaf=False)
         ctx.densityiterate(1, s.xyzh, s.vpred, fxyzu=s.fxyzu, Bevol=s.Bpred, iphase=s.iphase,
                            divcurlv=s.divcurlv, divcurlB=s.divcurlB, alphaind=s.alphaind, gradh=s.gradh,
                            dvdx=s.dvdx)

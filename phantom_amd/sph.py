@@ -159,22 +159,22 @@ class Context:
         self.params = params
 
     # -- maketree -----------------------------------------------------------
-    def maketree(self, xyzh, iphase=None):
+    def maketree(self, xyzh, iphase=None, leaf=True):
         """Build the tree; xyzh is wrapped in place (periodic).  Returns
-        (leaf_is_active (ncells,), ncells)."""
+        (leaf_is_active (ncells,), ncells); with leaf=False the leaf flags
+        are not copied out and (None, ncells) is returned."""
         n = xyzh.shape[0]
         px, ldx = _ptr(xyzh)
         pph, _ = _ptr(iphase)
         maxp = max(self.params.maxp, n)
-        leaf = np.zeros(2 * maxp + 1, dtype=np.int32)
+        leaf_arr = np.zeros(2 * maxp + 1, dtype=np.int32) if leaf else None
         ncells = C.c_int64()
-        self._check(self.lib.phsph_maketree(self._h, px, ldx, n, pph, leaf.ctypes.data_as(C.c_void_p),
+        self._check(self.lib.phsph_maketree(self._h, px, ldx, n, pph,
+                                            leaf_arr.ctypes.data_as(C.c_void_p) if leaf else None,
                                             C.byref(ncells)))
         self.np = n
         self._ncells = ncells.value
-        h = xyzh[:, 3] if isinstance(xyzh, np.ndarray) else xyzh.numpy()[:, 3]
-        self._nlive = int((h >= 2.2250738585072014e-308).sum())
-        return leaf[:ncells.value], ncells.value
+        return (leaf_arr[:ncells.value] if leaf else None), ncells.value
 
     def get_tree(self):
         """Mirror of node(1:ncells), inoderange(2,ncells) (1-based positions),
@@ -185,11 +185,13 @@ class Context:
         nc = self._ncells
         node = np.zeros(nc, dtype=node_dt)
         inoderange = np.zeros((nc, 2), dtype=np.int32)
-        inodeparts = np.zeros(self._nlive, dtype=np.int32)
+        # the library fills the first nlive entries; particle indices are never 0
+        inodeparts = np.zeros(self.np, dtype=np.int32)
         self._check(self.lib.phsph_get_tree(self._h, node.ctypes.data_as(C.c_void_p),
                                             inoderange.ctypes.data_as(C.c_void_p),
                                             inodeparts.ctypes.data_as(C.c_void_p), nc, C.byref(ml)))
-        return dict(node=node, inoderange=inoderange, inodeparts=inodeparts, maxlevel=ml.value, ncells=nc)
+        nlive = int(np.count_nonzero(inodeparts))
+        return dict(node=node, inoderange=inoderange, inodeparts=inodeparts[:nlive], maxlevel=ml.value, ncells=nc)
 
     def get_quads(self):
         """node(1:ncells)%quads: second moments about each node's centre of
