@@ -20,7 +20,7 @@
 //    > pivot in the left part swaps with the k-th element <= pivot counted
 //    from the right end of the right part, everything else stays.
 //  * Children are numbered 2n, 2n+1 (Gafton & Rosswog indexing, :820-823).
-#include "ctx.h"
+#include "stream.h"
 
 #define SUB_MAX 1024  // particles per LDS subtree
 #define SUB_BS 256
@@ -808,9 +808,31 @@ __global__ void k_bn_finish(const BigNode *bn, int maxch, int32_t *counters, con
 }
 
 // ------------------------------------------------------------------------
-// small nodes: one workgroup builds the whole subtree in LDS, one node per
-// thread per level, with the reference's sequential arithmetic
+// small nodes: one workgroup builds the whole subtree in LDS, level by
+// level, with the reference's sequential arithmetic.  A node of more than
+// SUB_WAVE_MIN particles is built by one wavefront: its maxima, minima and
+// the partition run across the lanes (the partition as the permutation of
+// sort_particles_in_cell, see k_bn_pcount), and its four running sums are
+// four dependent chains in lanes 0-3.  Smaller nodes take one thread each.
 // ------------------------------------------------------------------------
+#define SUB_WAVE_MIN 64
+
+// the running sum of one node's row in position order: lanes 0-2 add
+// fac*x, fac*y, fac*z, lane 3 adds the particle mass (kdtree.F90:655-665)
+__device__ __forceinline__ double sub_chain(const double *row, int a0, int a1, bool ism, double pm, double fac) {
+  double acc = 0.;
+  int k = a0;
+  for (; k + 16 <= a1 + 1; k += 16) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = row[k + u];
+#pragma unroll
+    for (int u = 0; u < 16; u++) acc = xadd(acc, ism ? pm : xmul(fac, v[u]));
+  }
+  for (; k <= a1; k++) acc = xadd(acc, ism ? pm : xmul(fac, row[k]));
+  return acc;
+}
+
 __global__ void __launch_bounds__(SUB_BS) k_subtrees(DevConst dc, const int32_t *frontier, const double *fbox,
                                                      int nfront, double *tx, double *ty, double *tz, double *th,
                                                      int32_t *iparts, int32_t *nd_lo, int32_t *nd_hi,
@@ -818,6 +840,7 @@ __global__ void __launch_bounds__(SUB_BS) k_subtrees(DevConst dc, const int32_t 
                                                      int maxlevel_indexed, int64_t nslots, int32_t *counters) {
   __shared__ double sx[SUB_MAX], sy[SUB_MAX], sz[SUB_MAX], shh[SUB_MAX];
   __shared__ int32_t sip[SUB_MAX];
+  __shared__ int32_t sidx[SUB_MAX];  // partition pairs of the wavefront-built nodes
   __shared__ int32_t qn[2][SUB_QMAX], ql[2][SUB_QMAX], qh[2][SUB_QMAX];
   __shared__ double qb[2][SUB_QMAX][6];
   __shared__ int qcount[2];
@@ -826,7 +849,7 @@ __global__ void __launch_bounds__(SUB_BS) k_subtrees(DevConst dc, const int32_t 
   const int lo = nd_lo[root], hi = nd_hi[root];
   const int n = hi - lo + 1;
   if (n > SUB_MAX) return;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   for (int k = tid; k < n; k += SUB_BS) {
     sx[k] = tx[lo + k]; sy[k] = ty[lo + k]; sz[k] = tz[lo + k]; shh[k] = th[lo + k]; sip[k] = iparts[lo + k];
   }
@@ -839,23 +862,139 @@ __global__ void __launch_bounds__(SUB_BS) k_subtrees(DevConst dc, const int32_t 
   __syncthreads();
   const double pm = dc.pmass;
   const double dfac = (pm > 0.) ? 1. / pm : 1.;
+  const double fac = xmul(pm, dfac);
   int cur = 0;
   int level = level0;
   int maxlev = 0, minlev = 1 << 30;
   while (qcount[cur] > 0) {
     const int nq = min(qcount[cur], SUB_QMAX);
     const int nxt = cur ^ 1;
+    // nodes of more than SUB_WAVE_MIN particles: one wavefront each
+    for (int t = wv; t < nq; t += SUB_BS / 64) {
+      const int a0 = ql[cur][t], a1 = qh[cur][t];
+      const int npn = a1 - a0 + 1;
+      if (npn <= SUB_WAVE_MIN) continue;
+      const int nnode = qn[cur][t];
+      double hmax = 0.;
+      for (int k = a0 + lane; k <= a1; k += 64) hmax = (shh[k] > hmax) ? shh[k] : hmax;
+      hmax = wave_max(hmax);
+      double acc = 0.;
+      if (lane < 4) acc = sub_chain(lane == 0 ? sx : (lane == 1 ? sy : sz), a0, a1, lane == 3, pm, fac);
+      const double xc = __shfl(acc, 0, 64), yc = __shfl(acc, 1, 64), zc = __shfl(acc, 2, 64);
+      const double tot = __shfl(acc, 3, 64);
+      const double D = xmul(tot, dfac);
+      const double com0 = xc / D, com1 = yc / D, com2 = zc / D;
+      double r2max = 0.;
+      for (int k = a0 + lane; k <= a1; k += 64) {
+        const double r2 = r2_exact(xsub(sx[k], com0), xsub(sy[k], com1), xsub(sz[k], com2));
+        r2max = (r2 > r2max) ? r2 : r2max;
+      }
+      r2max = wave_max(r2max);
+      const int il = 2 * nnode, ir = il + 1;
+      const bool bad_id = (level >= maxlevel_indexed || (int64_t)ir >= nslots);
+      if (lane == 0) {
+        phsph_kdnode e;
+        e.xcen[0] = com0; e.xcen[1] = com1; e.xcen[2] = com2;
+        e.size = xadd(sqrt(r2max), PH_EPS64);
+        e.hmax = hmax;
+        e.mass = tot;
+        e.leftchild = il;
+        e.rightchild = ir;
+        e.parent = nnode / 2;
+        e.idum = 0;
+        nd[nnode] = e;
+        nd_lo[nnode] = lo + a0;
+        nd_hi[nnode] = lo + a1;
+        leaf_flag[nnode] = 0;
+        if (bad_id) atomicOr(&counters[CNT_ERR], TERR_NODEID);
+      }
+      if (bad_id) continue;
+      const double *box = qb[cur][t];
+      const int iaxis = pick_axis(box);
+      const double pivot = (iaxis == 0) ? com0 : (iaxis == 1) ? com1 : com2;
+      const double *sa = (iaxis == 0) ? sx : (iaxis == 1) ? sy : sz;
+      int cl = 0;
+      for (int base = a0; base <= a1; base += 64) {
+        const int k = base + lane;
+        cl += __popcll(__ballot(k <= a1 && sa[k] <= pivot));
+      }
+      int nl = npn / 2;  // everything on one side: halves in place (kdtree.F90:856-865)
+      if (cl > 0 && cl < npn) {
+        // the k-th element > pivot of the left part [a0, a0+cl-1] swaps
+        // with the k-th element <= pivot counted from the right end
+        const int lend = a0 + cl - 1;
+        const unsigned long long lt = lanemask_lt();
+        int nbad = 0;
+        for (int base = a0; base <= lend; base += 64) {
+          const int k = base + lane;
+          const bool b = k <= lend && !(sa[k] <= pivot);
+          const unsigned long long m = __ballot(b);
+          if (b) sidx[a0 + nbad + __popcll(m & lt)] = k;
+          nbad += __popcll(m);
+        }
+        int nr = 0;
+        for (int base = lend + 1; base <= a1; base += 64) {
+          const int k = base + lane;
+          const bool b = k <= a1 && sa[k] <= pivot;
+          const unsigned long long m = __ballot(b);
+          const int j = nbad - 1 - (nr + __popcll(m & lt));
+          if (b && j >= 0) sidx[a0 + nbad + j] = k;
+          nr += __popcll(m);
+        }
+        if (nr != nbad && lane == 0) atomicOr(&counters[CNT_ERR], TERR_COUNT);
+        const int npair = min(nbad, nr);
+        wave_lds_sync();
+        for (int j = lane; j < npair; j += 64) {
+          const int pa = sidx[a0 + j], pb = sidx[a0 + nbad + j];
+          const double t0 = sx[pa], t1 = sy[pa], t2 = sz[pa], t3 = shh[pa];
+          const int32_t ti = sip[pa];
+          sx[pa] = sx[pb]; sy[pa] = sy[pb]; sz[pa] = sz[pb]; shh[pa] = shh[pb]; sip[pa] = sip[pb];
+          sx[pb] = t0; sy[pb] = t1; sz[pb] = t2; shh[pb] = t3; sip[pb] = ti;
+        }
+        wave_lds_sync();
+        nl = cl;
+      }
+      const int lA = a0, hA = a0 + nl - 1, lB = a0 + nl, hB = a1;
+      double bA[6], bB[6];
+      for (int c = 0; c < 3; c++) {
+        bA[c] = PH_HUGE64; bA[3 + c] = -PH_HUGE64;
+        bB[c] = PH_HUGE64; bB[3 + c] = -PH_HUGE64;
+      }
+      for (int k = lA + lane; k <= hA; k += 64) {
+        bA[0] = fmin(bA[0], sx[k]); bA[1] = fmin(bA[1], sy[k]); bA[2] = fmin(bA[2], sz[k]);
+        bA[3] = fmax(bA[3], sx[k]); bA[4] = fmax(bA[4], sy[k]); bA[5] = fmax(bA[5], sz[k]);
+      }
+      for (int k = lB + lane; k <= hB; k += 64) {
+        bB[0] = fmin(bB[0], sx[k]); bB[1] = fmin(bB[1], sy[k]); bB[2] = fmin(bB[2], sz[k]);
+        bB[3] = fmax(bB[3], sx[k]); bB[4] = fmax(bB[4], sy[k]); bB[5] = fmax(bB[5], sz[k]);
+      }
+      for (int c = 0; c < 3; c++) {
+        bA[c] = wave_min(bA[c]); bA[3 + c] = wave_max(bA[3 + c]);
+        bB[c] = wave_min(bB[c]); bB[3 + c] = wave_max(bB[3 + c]);
+      }
+      if (lane == 0) {
+        const int slot = atomicAdd(&qcount[nxt], 2);
+        if (slot + 1 >= SUB_QMAX) {
+          atomicOr(&counters[CNT_ERR], TERR_QUEUE);
+        } else {
+          qn[nxt][slot] = il; ql[nxt][slot] = lA; qh[nxt][slot] = hA;
+          qn[nxt][slot + 1] = ir; ql[nxt][slot + 1] = lB; qh[nxt][slot + 1] = hB;
+          for (int c = 0; c < 6; c++) { qb[nxt][slot][c] = bA[c]; qb[nxt][slot + 1][c] = bB[c]; }
+        }
+      }
+    }
+    // nodes of at most SUB_WAVE_MIN particles: one thread each
     for (int t = tid; t < nq; t += SUB_BS) {
       const int nnode = qn[cur][t];
       const int a0 = ql[cur][t], a1 = qh[cur][t];
       const int npn = a1 - a0 + 1;
+      if (npn > SUB_WAVE_MIN) continue;
       // construct_node: sequential sums in node order
       int nodeisactive = 0;
       for (int k = a0; k <= a1; k++)
         if (sip[k] > 0) { nodeisactive = 1; break; }
       double hmax = 0., xc = 0., yc = 0., zc = 0., tot = 0.;
       for (int k = a0; k <= a1; k++) {
-        double fac = xmul(pm, dfac);
         hmax = (shh[k] > hmax) ? shh[k] : hmax;
         tot = xadd(tot, pm);
         xc = xadd(xc, xmul(fac, sx[k]));
