@@ -176,6 +176,14 @@ struct phsph_ctx {
   DBuf<unsigned long long> ghmax;     // node hmax as the force walk sees it
   DBuf<double> htraj, hconv;          // h per density iteration, converged h
   DBuf<double> fnode;                 // far-field expansion per leaf (20)
+  // node-by-node dual walk: destination nodes of a level (ping-pong), their
+  // candidate source lists (shared by siblings) and incoming expansions
+  DBuf<int32_t> gw_node[2], gw_ccnt[2];
+  DBuf<int64_t> gw_coff[2];
+  DBuf<double> gw_H[2];
+  DBuf<uint32_t> gw_cand[2];
+  DBuf<int32_t> gw_act;               // node has an active leaf below (block timesteps)
+  DBuf<unsigned long long> gw_used;   // candidate entries taken by the level
   DBuf<uint32_t> p2p;                 // near-field leaves per leaf
   DBuf<int32_t> np2p;
   DBuf<float> gsoft;                  // gradh(2) in tree order

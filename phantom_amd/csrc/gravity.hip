@@ -28,7 +28,7 @@ constexpr int LENFGRAV = 20;
 constexpr int GMAXDEPTH = 32;   // kdtree.F90:42
 constexpr int GSTACK = 2048;    // (source, level) stack of the dual walk (LDS)
 
-enum { GFL_STACK = 1, GFL_DEPTH = 2, GFL_P2P = 4, GFL_TRAJ = 8, GFL_DTC = 32 };
+enum { GFL_STACK = 1, GFL_DEPTH = 2, GFL_P2P = 4, GFL_TRAJ = 8, GFL_CAND = 16, GFL_DTC = 32 };
 
 // ids 2n, 2n+1 are the children of n: a slot is a node when every ancestor
 // is an internal node (leaf_flag == 0)
@@ -214,96 +214,253 @@ __device__ __forceinline__ double node_sep(const DevConst &dc, const double *a, 
 }
 
 // ------------------------------------------------------------------------
-// dual-tree walk, one wavefront per leaf
+// dual-tree walk, node by node (getneigh_dual, kdtree.F90:1357-1460;
+// node_interaction :1653-1694; open_nodes :1591-1645).
+//
+// The reference walks once per leaf: (source, destination) pairs start at
+// (root, root) and the destination runs down the leaf's branch.  The pairs
+// whose destination is a node A depend only on the path to A, so here every
+// node processes them once, level by level.  A node's candidate sources are
+// those its parent opened: a source equal to the destination, or not well
+// separated from it, is opened into its children (a leaf source stays
+// itself) for both children of the destination.  Well-separated sources add
+// their expansion (compute_M2L) to the node's coefficients.  A node's
+// incoming expansion is its parent's, plus the parent's own coefficients,
+// shifted to the node's centre (propagate_fnode_to_node): the same
+// operations as the reference's per-leaf downward pass.  At a leaf the
+// candidates are resolved completely, as the reference does once its
+// destination is the leaf: sources that are not well separated are opened
+// until they are leaves, which form the leaf's near-field list.
 // ------------------------------------------------------------------------
-__global__ void __launch_bounds__(64)
-k_grav_walk(DevConst dc, int nleaves, const int32_t *__restrict__ leaves, const int32_t *__restrict__ leaf_flag,
-            const phsph_kdnode *__restrict__ nd, const unsigned long long *__restrict__ ghmax,
-            const double *__restrict__ quads, double *__restrict__ fnode_out, uint32_t *__restrict__ p2p,
-            int32_t *__restrict__ np2p, const int64_t *poff, int ucapp, int32_t *__restrict__ flags) {
-  __shared__ uint32_t stk[GSTACK];       // source id << 6 | level
-  __shared__ double fb[GMAXDEPTH + 1][LENFGRAV];
-  __shared__ int branch[GMAXDEPTH + 1];
-  __shared__ int s_nparents;
-  const int l = blockIdx.x;
+struct GwLevel {
+  const int32_t *node;   // destination node ids
+  const int64_t *coff;   // candidate list offset
+  const int32_t *ccnt;   // candidate list length
+  const double *H;       // incoming expansion (LENFGRAV per entry)
+  const uint32_t *cand;  // candidate source node ids
+};
+struct GwNext {
+  int32_t *node;
+  int64_t *coff;
+  int32_t *ccnt;
+  double *H;
+  uint32_t *cand;
+  int64_t cand_cap;
+  unsigned long long *cand_used;  // candidate entries taken (may exceed cand_cap)
+  int32_t *nent;                  // entries of the next level
+};
+
+// node_interaction: separation (dst - src) and whether the pair is well
+// separated
+__device__ __forceinline__ bool gw_wellsep(const DevConst &dc, const phsph_kdnode *__restrict__ nd,
+                                           const unsigned long long *__restrict__ ghmax, int dst, int src,
+                                           double &dx, double &dy, double &dz, double &r2, int &code) {
+  r2 = node_sep(dc, nd[dst].xcen, nd[src].xcen, dx, dy, dz, code);
+  const double R = dc.radkern;
+  const double rcut_dst = __longlong_as_double((long long)ghmax[dst]) * R;
+  const double rcut_src = __longlong_as_double((long long)ghmax[src]) * R;
+  const double rcut = fmax(rcut_dst, rcut_src);
+  const double ss = nd[dst].size + nd[src].size;
+  const double rcut2 = (ss + rcut) * (ss + rcut);
+  return (dc.tree_acc2 * r2 > ss * ss) && (r2 > rcut2);
+}
+
+__device__ __forceinline__ bool gw_active(int n, const int32_t *__restrict__ leaf_flag,
+                                          const int32_t *__restrict__ act) {
+  const int lf = leaf_flag[n];
+  return (lf != 0) ? (lf > 0) : (act == nullptr || act[n] != 0);
+}
+
+// ancestors of active leaves (block timesteps): the walk skips the rest
+__global__ void k_gw_mark(int nleaves, const int32_t *__restrict__ leaves, const int32_t *__restrict__ leaf_flag,
+                          const phsph_kdnode *__restrict__ nd, int32_t *act) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= nleaves) return;
-  const int icell = leaves[l];
-  if (leaf_flag[icell] <= 0) {
-    if (threadIdx.x == 0) np2p[l] = 0;
+  const int c = leaves[l];
+  if (leaf_flag[c] <= 0) return;
+  for (int n = nd[c].parent; n != 0; n = nd[n].parent)
+    if (atomicOr(&act[n], 1) != 0) break;
+}
+
+__global__ void k_gw_fill(int32_t *a, int64_t n, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// level 0: the root with itself as its one candidate
+__global__ void k_gw_init(int32_t *node, int64_t *coff, int32_t *ccnt, double *H, uint32_t *cand) {
+  const int k = threadIdx.x;
+  if (k < LENFGRAV) H[k] = 0.;
+  if (k == 0) {
+    node[0] = 1;
+    coff[0] = 0;
+    ccnt[0] = 1;
+    cand[0] = 1u;
+  }
+}
+
+__global__ void __launch_bounds__(64)
+k_gw_level(DevConst dc, int nent, GwLevel in, GwNext out, const int32_t *__restrict__ leaf_flag,
+           const int32_t *__restrict__ act, const phsph_kdnode *__restrict__ nd,
+           const unsigned long long *__restrict__ ghmax, const double *__restrict__ quads,
+           const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ pleaf, double *__restrict__ fnode_out,
+           uint32_t *__restrict__ p2p, int32_t *__restrict__ np2p, const int64_t *poff, int ucapp,
+           int32_t *__restrict__ flags) {
+  __shared__ uint32_t stk[GSTACK];
+  const int e = blockIdx.x;
+  if (e >= nent) return;
+  const int lane = threadIdx.x;
+  const int A = in.node[e];
+  const int64_t c0 = in.coff[e];
+  const int nc = in.ccnt[e];
+  const int lfA = leaf_flag[A];
+  double facc[LENFGRAV];
+  for (int k = 0; k < LENFGRAV; k++) facc[k] = 0.;
+  if (lfA == 0) {
+    // internal destination: expansions of the well-separated sources, and
+    // the opened sources as the children's candidates
+    int ntot = 0;
+    for (int b = 0; b < nc; b += 64) {
+      const int k = b + lane;
+      int npush = 0;
+      if (k < nc) {
+        const int src = (int)in.cand[c0 + k];
+        bool opened = (src == A);
+        if (!opened) {
+          double dx, dy, dz, r2;
+          int code;
+          if (gw_wellsep(dc, nd, ghmax, A, src, dx, dy, dz, r2, code)) {
+            double f[LENFGRAV];
+            m2l(dx, dy, dz, 1. / sqrt(r2), nd[src].mass, quads + 6 * (size_t)src, f);
+            for (int q = 0; q < LENFGRAV; q++) facc[q] += f[q];
+          } else {
+            opened = true;
+          }
+        }
+        if (opened)
+          npush = (leaf_flag[src] != 0) ? 1 : ((nd[src].leftchild != 0 ? 1 : 0) + (nd[src].rightchild != 0 ? 1 : 0));
+      }
+      ntot += wave_sum_i(npush);
+    }
+    unsigned long long base = 0ull;
+    if (lane == 0) base = atomicAdd(out.cand_used, (unsigned long long)ntot);
+    base = __shfl(base, 0, 64);
+    const bool fits = (int64_t)(base + (unsigned long long)ntot) <= out.cand_cap;
+    if (fits) {
+      int w = 0;
+      for (int b = 0; b < nc; b += 64) {
+        const int k = b + lane;
+        int npush = 0;
+        uint32_t e0 = 0u, e1 = 0u;
+        if (k < nc) {
+          const int src = (int)in.cand[c0 + k];
+          bool opened = (src == A);
+          if (!opened) {
+            double dx, dy, dz, r2;
+            int code;
+            opened = !gw_wellsep(dc, nd, ghmax, A, src, dx, dy, dz, r2, code);
+          }
+          if (opened) {
+            if (leaf_flag[src] != 0) {
+              e0 = (uint32_t)src;
+              npush = 1;
+            } else {
+              const int il = nd[src].leftchild, ir = nd[src].rightchild;
+              if (il != 0) { e0 = (uint32_t)il; npush = 1; }
+              if (ir != 0) {
+                if (npush == 0) e0 = (uint32_t)ir;
+                else e1 = (uint32_t)ir;
+                npush++;
+              }
+            }
+          }
+        }
+        int incl = npush;
+        for (int o = 1; o < 64; o <<= 1) {
+          const int v = __shfl_up(incl, o, 64);
+          if (lane >= o) incl += v;
+        }
+        const size_t at = (size_t)base + w + incl - npush;
+        if (npush > 0) out.cand[at] = e0;
+        if (npush > 1) out.cand[at + 1] = e1;
+        w += __shfl(incl, 63, 64);
+      }
+    }
+    double fs[LENFGRAV];
+    for (int q = 0; q < LENFGRAV; q++) fs[q] = in.H[(size_t)e * LENFGRAV + q] + wave_sum(facc[q]);
+    // the children's incoming expansions: lane 0 the left child, lane 1 the right
+    const int il = nd[A].leftchild, ir = nd[A].rightchild;
+    const bool ul = il != 0 && gw_active(il, leaf_flag, act);
+    const bool ur = ir != 0 && gw_active(ir, leaf_flag, act);
+    int slot = 0;
+    if (lane == 0 && (ul || ur)) slot = atomicAdd(out.nent, (ul ? 1 : 0) + (ur ? 1 : 0));
+    slot = __shfl(slot, 0, 64);
+    if (lane < 2) {
+      const bool mine = (lane == 0) ? ul : ur;
+      if (mine) {
+        const int ch = (lane == 0) ? il : ir;
+        const int at = slot + ((lane == 1 && ul) ? 1 : 0);
+        double dx, dy, dz;
+        int cd;
+        node_sep(dc, nd[ch].xcen, nd[A].xcen, dx, dy, dz, cd);
+        double fc[LENFGRAV];
+        l2l(fc, fs, dx, dy, dz);
+        for (int q = 0; q < LENFGRAV; q++) out.H[(size_t)at * LENFGRAV + q] = fc[q];
+        out.node[at] = ch;
+        out.coff[at] = (int64_t)base;
+        out.ccnt[at] = fits ? ntot : 0;
+      }
+    }
+    if (!fits && lane == 0) atomicOr(flags, GFL_CAND);
     return;
   }
-  // near-field list of this leaf: uniform capacity, or exact offsets
-  const size_t pbase = poff ? (size_t)poff[l] : (size_t)l * ucapp;
-  const int capp = poff ? (int)(poff[l + 1] - poff[l]) : ucapp;
-  const int lane = threadIdx.x;
-  for (int k = lane; k < (GMAXDEPTH + 1) * LENFGRAV; k += 64) (&fb[0][0])[k] = 0.;
-  if (lane == 0) {
-    int np_ = 1, j = icell;
-    branch[1] = icell;
-    bool ok = true;
-    while (nd[j].parent != 0) {
-      j = nd[j].parent;
-      if (np_ + 1 > GMAXDEPTH) { ok = false; break; }
-      branch[++np_] = j;
-    }
-    if (!ok) atomicOr(flags, GFL_DEPTH);
-    s_nparents = ok ? np_ : 0;
-    stk[0] = (1u << 6) | (uint32_t)np_;
-  }
-  __syncthreads();
-  const int nparents = s_nparents;
-  if (nparents == 0) return;
-  const double R = dc.radkern;
-  int top = 1, nc = 0;
+  if (lfA < 0) return;  // inactive leaf: no walk
+  // active leaf: resolve its candidates completely
+  const int li = pleaf[nd_lo[A]];
+  const size_t pbase = poff ? (size_t)poff[li] : (size_t)li * ucapp;
+  const int capp = poff ? (int)(poff[li + 1] - poff[li]) : ucapp;
+  int top = 0, npp = 0, b = 0;
   bool ovf = false;
-  while (top > 0) {
-    const int take = min(top, 64);
+  while (true) {
+    int take;
     uint32_t ent = 0u;
-    if (lane < take) ent = stk[top - take + lane];
-    __syncthreads();
-    top -= take;
-    bool stackit = false, p2p_hit = false;
-    int code = 13, src = 0, lev = 0, npush = 0, c0 = 0, c1 = 0;
+    if (b < nc) {
+      take = min(64, nc - b);
+      if (lane < take) ent = in.cand[c0 + b + lane];
+      b += take;
+    } else if (top > 0) {
+      take = min(top, 64);
+      if (lane < take) ent = stk[top - take + lane];
+      top -= take;
+    } else {
+      break;
+    }
+    wave_lds_sync();
+    bool p2p_hit = false;
+    int code = SEP_PLAIN, npush = 0;
+    uint32_t e0 = 0u, e1 = 0u;
+    const int src = (int)ent;
     if (lane < take) {
-      src = (int)(ent >> 6);
-      lev = (int)(ent & 63u);
-      const int dst = branch[lev];
-      if (dst == src) {
-        stackit = true;
-      } else {
-        // node_interaction (kdtree.F90:1653-1694)
-        double dx, dy, dz;
-        const double r2 = node_sep(dc, nd[dst].xcen, nd[src].xcen, dx, dy, dz, code);
-        const double rcut_dst = __longlong_as_double((long long)ghmax[dst]) * R;
-        const double rcut_src = __longlong_as_double((long long)ghmax[src]) * R;
-        const double rcut = fmax(rcut_dst, rcut_src);
-        const double ss = nd[dst].size + nd[src].size;
-        const double rcut2 = (ss + rcut) * (ss + rcut);
-        const bool wellsep = (dc.tree_acc2 * r2 > ss * ss) && (r2 > rcut2);
-        if (wellsep) {
+      bool opened = (src == A);
+      if (!opened) {
+        double dx, dy, dz, r2;
+        if (gw_wellsep(dc, nd, ghmax, A, src, dx, dy, dz, r2, code)) {
           double f[LENFGRAV];
           m2l(dx, dy, dz, 1. / sqrt(r2), nd[src].mass, quads + 6 * (size_t)src, f);
-          for (int k = 0; k < LENFGRAV; k++) atomicAdd(&fb[lev][k], f[k]);
+          for (int q = 0; q < LENFGRAV; q++) facc[q] += f[q];
         } else {
-          stackit = true;
+          opened = true;
         }
       }
-      if (stackit) {  // open_nodes (kdtree.F90:1591-1645)
-        const bool dstleaf = !(lev - 1 > 0);
-        const int levn = dstleaf ? lev : lev - 1;
+      if (opened) {
         if (leaf_flag[src] != 0) {
-          if (dstleaf) {
-            p2p_hit = true;
-          } else {
-            c0 = (src << 6) | levn;
-            npush = 1;
-          }
+          p2p_hit = true;
         } else {
           const int il = nd[src].leftchild, ir = nd[src].rightchild;
-          if (il != 0) { c0 = (il << 6) | levn; npush = 1; }
+          if (il != 0) { e0 = (uint32_t)il; npush = 1; }
           if (ir != 0) {
-            if (npush == 0) c0 = (ir << 6) | levn;
-            else c1 = (ir << 6) | levn;
+            if (npush == 0) e0 = (uint32_t)ir;
+            else e1 = (uint32_t)ir;
             npush++;
           }
         }
@@ -312,13 +469,12 @@ k_grav_walk(DevConst dc, int nleaves, const int32_t *__restrict__ leaves, const 
     // near-field leaves, with the separation code of the pair
     const unsigned long long mp = __ballot(p2p_hit);
     if (p2p_hit) {
-      const int k = nc + __popcll(mp & lanemask_lt());
+      const int k = npp + __popcll(mp & lanemask_lt());
       int sc = SEP_PLAIN;
       if (dc.periodic) sc = (code == SEP_PLAIN) ? SEP_WRAP0 : code;
       if (k < capp) p2p[pbase + k] = cand_pack(src, sc);
     }
-    nc += __popcll(mp);
-    // pushes: exclusive scan of the per-lane counts
+    npp += __popcll(mp);
     int incl = npush;
     for (int o = 1; o < 64; o <<= 1) {
       const int v = __shfl_up(incl, o, 64);
@@ -330,28 +486,90 @@ k_grav_walk(DevConst dc, int nleaves, const int32_t *__restrict__ leaves, const 
       break;
     }
     const int at = top + incl - npush;
-    if (npush > 0) stk[at] = (uint32_t)c0;
-    if (npush > 1) stk[at + 1] = (uint32_t)c1;
+    if (npush > 0) stk[at] = e0;
+    if (npush > 1) stk[at + 1] = e1;
     top += total;
-    __syncthreads();
+    wave_lds_sync();
   }
-  __syncthreads();
+  for (int q = 0; q < LENFGRAV; q++) {
+    const double v = in.H[(size_t)e * LENFGRAV + q] + wave_sum(facc[q]);
+    if (lane == q) fnode_out[(size_t)li * LENFGRAV + q] = v;
+  }
   if (lane == 0) {
-    np2p[l] = nc;
+    np2p[li] = npp;
     if (ovf) atomicOr(flags, GFL_STACK);
-    if (nc > capp) atomicOr(flags, GFL_P2P);
-    // downward pass: shift each level's expansion to the next centre
-    double facc[LENFGRAV], ftmp[LENFGRAV];
-    for (int k = 0; k < LENFGRAV; k++) facc[k] = 0.;
-    for (int i = nparents; i >= 2; i--) {
-      double dx, dy, dz;
-      int cd;
-      node_sep(dc, nd[branch[i - 1]].xcen, nd[branch[i]].xcen, dx, dy, dz, cd);
-      for (int k = 0; k < LENFGRAV; k++) ftmp[k] = facc[k] + fb[i][k];
-      l2l(facc, ftmp, dx, dy, dz);
-    }
-    for (int k = 0; k < LENFGRAV; k++) fnode_out[(size_t)l * LENFGRAV + k] = facc[k] + fb[1][k];
+    if (npp > capp) atomicOr(flags, GFL_P2P);
   }
+}
+
+// the walk over all levels; flags of the walk in counters[0]
+static int grav_walk(phsph_ctx *c, const int64_t *poff) {
+  hipStream_t s = c->stream;
+  const int nl = c->nleaves;
+  const size_t ecap = (size_t)nl + 2;  // nodes of one level <= leaves
+  for (int k = 0; k < 2; k++) {
+    PH_HIPCHECK(c, c->gw_node[k].ensure(ecap));
+    PH_HIPCHECK(c, c->gw_ccnt[k].ensure(ecap));
+    PH_HIPCHECK(c, c->gw_coff[k].ensure(ecap));
+    PH_HIPCHECK(c, c->gw_H[k].ensure(ecap * LENFGRAV));
+    PH_HIPCHECK(c, c->gw_cand[k].ensure((size_t)64 * ecap + 4096));
+  }
+  PH_HIPCHECK(c, c->gw_used.ensure(1));
+  hipLaunchKernelGGL(k_gw_fill, dim3(nblocks(nl + 1, 256)), dim3(256), 0, s, c->np2p.p, (int64_t)nl + 1, 0);
+  const int32_t *act = nullptr;
+  if (c->dc.ind_timesteps) {
+    PH_HIPCHECK(c, c->gw_act.ensure(c->nslots));
+    hipLaunchKernelGGL(k_gw_fill, dim3(nblocks(c->nslots, 256)), dim3(256), 0, s, c->gw_act.p, c->nslots, 0);
+    hipLaunchKernelGGL(k_gw_mark, dim3(nblocks(nl, 256)), dim3(256), 0, s, nl, c->leaves.p, c->leaf_flag.p,
+                       c->nd.p, c->gw_act.p);
+    act = c->gw_act.p;
+  }
+  hipLaunchKernelGGL(k_gw_init, dim3(1), dim3(64), 0, s, c->gw_node[0].p, c->gw_coff[0].p, c->gw_ccnt[0].p,
+                     c->gw_H[0].p, c->gw_cand[0].p);
+  PH_HIPCHECK(c, hipGetLastError());
+  // the root takes part if it is an active leaf or has one below
+  int32_t root_flag = 0, root_act = 1;
+  PH_HIPCHECK(c, hipMemcpyAsync(&root_flag, c->leaf_flag.p + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  if (act) PH_HIPCHECK(c, hipMemcpyAsync(&root_act, act + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  PH_HIPCHECK(c, hipStreamSynchronize(s));
+  int nent = (root_flag != 0) ? (root_flag > 0 ? 1 : 0) : (root_act != 0 ? 1 : 0);
+  int cur = 0, level = 0;
+  while (nent > 0) {
+    if (level >= GMAXDEPTH) {  // a branch of more than GMAXDEPTH nodes
+      int32_t fl = GFL_DEPTH;
+      PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &fl, sizeof(int32_t), hipMemcpyHostToDevice, s));
+      break;
+    }
+    const int nxt = cur ^ 1;
+    int32_t zero = 0;
+    unsigned long long zero64 = 0ull;
+    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p + 1, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    PH_HIPCHECK(c, hipMemcpyAsync(c->gw_used.p, &zero64, sizeof(zero64), hipMemcpyHostToDevice, s));
+    GwLevel in{c->gw_node[cur].p, c->gw_coff[cur].p, c->gw_ccnt[cur].p, c->gw_H[cur].p, c->gw_cand[cur].p};
+    GwNext out{c->gw_node[nxt].p, c->gw_coff[nxt].p, c->gw_ccnt[nxt].p, c->gw_H[nxt].p, c->gw_cand[nxt].p,
+               (int64_t)c->gw_cand[nxt].cap, c->gw_used.p, c->counters.p + 1};
+    hipLaunchKernelGGL(k_gw_level, dim3(nent), dim3(64), 0, s, c->dc, nent, in, out, c->leaf_flag.p, act, c->nd.p,
+                       c->ghmax.p, c->quads.p, c->nd_lo.p, c->pleaf.p, c->fnode.p, c->p2p.p, c->np2p.p, poff,
+                       c->caps.p2p, c->counters.p);
+    PH_HIPCHECK(c, hipGetLastError());
+    int32_t cnt[2];
+    unsigned long long used = 0ull;
+    PH_HIPCHECK(c, hipMemcpyAsync(cnt, c->counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    PH_HIPCHECK(c, hipMemcpyAsync(&used, c->gw_used.p, sizeof(used), hipMemcpyDeviceToHost, s));
+    PH_HIPCHECK(c, hipStreamSynchronize(s));
+    if (cnt[0] & GFL_CAND) {
+      // the level's candidate lists did not fit: a larger buffer, the level again
+      PH_HIPCHECK(c, c->gw_cand[nxt].ensure((size_t)used + used / 4 + 1024));
+      int32_t fl = cnt[0] & ~GFL_CAND;
+      PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &fl, sizeof(int32_t), hipMemcpyHostToDevice, s));
+      continue;
+    }
+    if (cnt[0] & GFL_STACK) break;
+    nent = cnt[1];
+    cur = nxt;
+    level++;
+  }
+  return PHSPH_OK;
 }
 
 // ------------------------------------------------------------------------
@@ -615,9 +833,8 @@ int grav_force(phsph_ctx *c, int icall, const ForceIOArgs &a, const FRec *rec, c
     PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
     if (nl > 0) {
       LaunchTimer lt(c, "grav_walk");
-      hipLaunchKernelGGL(k_grav_walk, dim3(nl), dim3(64), 0, s, c->dc, nl, c->leaves.p, c->leaf_flag.p, c->nd.p,
-                         c->ghmax.p, c->quads.p, c->fnode.p, c->p2p.p, c->np2p.p, poff, c->caps.p2p,
-                         c->counters.p);
+      int rc = grav_walk(c, poff);
+      if (rc) return rc;
     }
     PH_HIPCHECK(c, hipGetLastError());
     int32_t fl = 0;
