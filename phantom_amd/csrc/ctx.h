@@ -146,7 +146,8 @@ struct phsph_ctx {
   DBuf<double4> fvel;                    // force: vx, vy, vz, u (mixed precision)
   DBuf<double> fh;                       // force: h (tree order)
   DBuf<int32_t> act;                     // active tree positions (-1: group padding)
-  std::vector<int32_t> act_host;         // host copy used to pad sparse groups
+  DBuf<int32_t> act2;                    // padded active list being built
+  DBuf<int32_t> grp_brk, grp_rs, grp_gr, grp_bc;  // group formation: breaks, run starts, run groups, block sums
   int nact = 0;
   DBuf<int32_t> act_dens;                // active positions of the last densityiterate
   int nact_dens = 0;

@@ -251,30 +251,39 @@ __global__ void k_compact_fill(const int32_t *flag, int n, const int32_t *boff, 
   for (int k = 0; k < w; k++) off += shi[k];
   if (b) out[off + __popcll(m & ((1ull << l) - 1ull))] = i;
 }
-__global__ void k_scan_i32(int32_t *a, int n, int32_t *total) {
-  __shared__ int carry;
-  __shared__ int tmp[256];
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int base = 0; base < n; base += 256) {
-    int i = base + threadIdx.x;
-    int v = (i < n) ? a[i] : 0;
-    tmp[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-      int t = (threadIdx.x >= o) ? tmp[threadIdx.x - o] : 0;
-      __syncthreads();
-      tmp[threadIdx.x] += t;
-      __syncthreads();
+// exclusive scan of a[0..n) in place, total to *total: one block of 1024
+// threads, four consecutive items per thread
+__global__ void __launch_bounds__(1024) k_scan_i32(int32_t *a, int n, int32_t *total) {
+  __shared__ int wsum[16];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  int carry = 0;
+  for (int base = 0; base < n; base += 4096) {
+    const int i0 = base + tid * 4;
+    int v[4];
+    for (int k = 0; k < 4; k++) v[k] = (i0 + k < n) ? a[i0 + k] : 0;
+    const int t = v[0] + v[1] + v[2] + v[3];
+    int incl = t;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += u;
     }
-    int incl = tmp[threadIdx.x];
-    int tot = tmp[255];
-    if (i < n) a[i] = carry + incl - v;
+    if (lane == 63) wsum[w] = incl;
     __syncthreads();
-    if (threadIdx.x == 0) carry += tot;
+    int woff = 0, btot = 0;
+    for (int k = 0; k < 16; k++) {
+      const int x = wsum[k];
+      if (k < w) woff += x;
+      btot += x;
+    }
+    int run = carry + woff + incl - t;
+    for (int k = 0; k < 4; k++) {
+      if (i0 + k < n) a[i0 + k] = run;
+      run += v[k];
+    }
+    carry += btot;
     __syncthreads();
   }
-  if (threadIdx.x == 0) *total = carry;
+  if (tid == 0) *total = carry;
 }
 
 // ------------------------------------------------------------------------
@@ -812,7 +821,7 @@ int compact_active(phsph_ctx *c, const int32_t *flag, int n) {
   PH_HIPCHECK(c, c->tmpi2.ensure((size_t)nb + 1));
   PH_HIPCHECK(c, c->act.ensure(n + 1));
   hipLaunchKernelGGL(k_compact_count, dim3(nb), dim3(256), 0, s, flag, n, c->tmpi2.p);
-  hipLaunchKernelGGL(k_scan_i32, dim3(1), dim3(256), 0, s, c->tmpi2.p, (int)nb, c->counters.p + 1);
+  hipLaunchKernelGGL(k_scan_i32, dim3(1), dim3(1024), 0, s, c->tmpi2.p, (int)nb, c->counters.p + 1);
   hipLaunchKernelGGL(k_compact_fill, dim3(nb), dim3(256), 0, s, flag, n, c->tmpi2.p, c->act.p);
   int32_t na = 0;
   PH_HIPCHECK(c, hipMemcpyAsync(&na, c->counters.p + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -821,14 +830,142 @@ int compact_active(phsph_ctx *c, const int32_t *flag, int n) {
   return PHSPH_OK;
 }
 
-// Groups for a sparse active set.  With every live particle active, 64
-// consecutive tree positions form a compact group.  When only some are (block
-// timesteps, retried particles), 64 consecutive active particles can lie far
-// apart and their group's candidate leaves cover the union of their
-// neighbourhoods; so a group is closed early where two consecutive active
-// particles are more than PAD_GAP tree positions apart, and the rest of its
-// 64 slots are padding (act = -1).  The padded length becomes c->nact.
+// Groups of the active list.  64 consecutive active particles form a group,
+// except that a group closes early (the rest of its 64 slots are padding,
+// act = -1) where the next active particle lies far from the previous one:
+// more than PAD_GAP tree positions after it (sparse active sets: block
+// timesteps, retried particles), or in a leaf whose centre is more than
+// GROUP_JUMP times the two leaves' sizes away.  Tree order is depth-first,
+// so consecutive leaves are neighbours except where the order crosses from
+// one subtree to the next; a group across such a jump has a bounding box
+// spanning both sides, and its candidate leaves cover the space between.
+// The padded length becomes c->nact.
 constexpr int PAD_GAP = 64;
+constexpr double GROUP_JUMP = 4.;
+
+__global__ void k_fill_act(int n, int32_t *a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = -1;
+}
+
+__global__ void k_grp_breaks(int na, const int32_t *__restrict__ act, const int32_t *__restrict__ pleaf,
+                             const int32_t *__restrict__ leaves, const phsph_kdnode *__restrict__ nd,
+                             int32_t *__restrict__ brk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= na) return;
+  int b = 1;
+  if (i > 0) {
+    const int p = act[i], q = act[i - 1];
+    b = (p - q > PAD_GAP) ? 1 : 0;
+    if (!b) {
+      const int lp = pleaf[p], lq = pleaf[q];
+      if (lp != lq) {
+        const phsph_kdnode &A = nd[leaves[lp]];
+        const phsph_kdnode &B = nd[leaves[lq]];
+        const double dx = A.xcen[0] - B.xcen[0], dy = A.xcen[1] - B.xcen[1], dz = A.xcen[2] - B.xcen[2];
+        const double sl = GROUP_JUMP * (A.size + B.size);
+        b = (dx * dx + dy * dy + dz * dz > sl * sl) ? 1 : 0;
+      }
+    }
+  }
+  brk[i] = b;
+}
+
+// groups of each run (block sums): a run is rs[r] .. rs[r+1]-1
+__global__ void k_grp_bsum(int na, const int32_t *__restrict__ rs, const int32_t *__restrict__ nruns_p,
+                           int32_t *__restrict__ bsum) {
+  __shared__ int shi[4];
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nruns = *nruns_p;
+  int g = 0;
+  if (r < nruns) {
+    const int len = ((r + 1 < nruns) ? rs[r + 1] : na) - rs[r];
+    g = (len + GROUP - 1) / GROUP;
+  }
+  const int v = wave_sum_i(g);
+  if ((threadIdx.x & 63) == 0) shi[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) bsum[blockIdx.x] = shi[0] + shi[1] + shi[2] + shi[3];
+}
+// first group of each run
+__global__ void k_grp_place(int na, const int32_t *__restrict__ rs, const int32_t *__restrict__ nruns_p,
+                            const int32_t *__restrict__ boff, int32_t *__restrict__ gr) {
+  __shared__ int shi[4];
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nruns = *nruns_p;
+  int g = 0;
+  if (r < nruns) {
+    const int len = ((r + 1 < nruns) ? rs[r + 1] : na) - rs[r];
+    g = (len + GROUP - 1) / GROUP;
+  }
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int incl = g;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o, 64);
+    if (l >= o) incl += u;
+  }
+  if (l == 63) shi[w] = incl;
+  __syncthreads();
+  int off = boff[blockIdx.x];
+  for (int k = 0; k < w; k++) off += shi[k];
+  if (r < nruns) gr[r] = off + incl - g;
+}
+// each active particle to its slot: its run by bisection over the run starts
+__global__ void k_grp_scatter(int na, const int32_t *__restrict__ act, const int32_t *__restrict__ rs,
+                              const int32_t *__restrict__ nruns_p, const int32_t *__restrict__ gr,
+                              int32_t *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= na) return;
+  int lo = 0, hi = *nruns_p - 1;  // largest r with rs[r] <= i (rs[0] = 0)
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (rs[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  const int o = i - rs[lo];
+  out[(size_t)GROUP * (gr[lo] + o / GROUP) + (o % GROUP)] = act[i];
+}
+
+int pad_groups(phsph_ctx *c) {
+  const int na = c->nact;
+  if (na == 0) return PHSPH_OK;
+  hipStream_t s = c->stream;
+  const unsigned nb = nblocks(na, 256);
+  PH_HIPCHECK(c, c->grp_brk.ensure(na + 1));
+  PH_HIPCHECK(c, c->grp_rs.ensure(na + 1));
+  PH_HIPCHECK(c, c->grp_gr.ensure(na + 1));
+  PH_HIPCHECK(c, c->grp_bc.ensure((size_t)nb + 1));
+  int32_t *nruns = c->counters.p + 6, *ngrp = c->counters.p + 7;
+  hipLaunchKernelGGL(k_grp_breaks, dim3(nb), dim3(256), 0, s, na, c->act.p, c->pleaf.p, c->leaves.p, c->nd.p,
+                     c->grp_brk.p);
+  hipLaunchKernelGGL(k_compact_count, dim3(nb), dim3(256), 0, s, c->grp_brk.p, na, c->grp_bc.p);
+  hipLaunchKernelGGL(k_scan_i32, dim3(1), dim3(1024), 0, s, c->grp_bc.p, (int)nb, nruns);
+  hipLaunchKernelGGL(k_compact_fill, dim3(nb), dim3(256), 0, s, c->grp_brk.p, na, c->grp_bc.p, c->grp_rs.p);
+  PH_HIPCHECK(c, hipGetLastError());
+  int32_t nr = 0;
+  PH_HIPCHECK(c, hipMemcpyAsync(&nr, nruns, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  PH_HIPCHECK(c, hipStreamSynchronize(s));
+  if (nr <= 1) return PHSPH_OK;  // one run: 64 consecutive active particles per group
+  hipLaunchKernelGGL(k_grp_bsum, dim3(nb), dim3(256), 0, s, na, c->grp_rs.p, nruns, c->grp_bc.p);
+  hipLaunchKernelGGL(k_scan_i32, dim3(1), dim3(1024), 0, s, c->grp_bc.p, (int)nb, ngrp);
+  hipLaunchKernelGGL(k_grp_place, dim3(nb), dim3(256), 0, s, na, c->grp_rs.p, nruns, c->grp_bc.p, c->grp_gr.p);
+  PH_HIPCHECK(c, hipGetLastError());
+  int32_t ng = 0;
+  PH_HIPCHECK(c, hipMemcpyAsync(&ng, ngrp, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  PH_HIPCHECK(c, hipStreamSynchronize(s));
+  const int64_t slots = (int64_t)GROUP * ng;
+  if (slots == na) return PHSPH_OK;  // no group closed early
+  if (slots > INT32_MAX - 1) PH_RET(c, PHSPH_ERR_CAPACITY, "active list: too many padded groups");
+  PH_HIPCHECK(c, c->act2.ensure((size_t)slots + 1));
+  hipLaunchKernelGGL(k_fill_act, dim3(nblocks((int)slots, 256)), dim3(256), 0, s, (int)slots, c->act2.p);
+  hipLaunchKernelGGL(k_grp_scatter, dim3(nb), dim3(256), 0, s, na, c->act.p, c->grp_rs.p, nruns, c->grp_gr.p,
+                     c->act2.p);
+  PH_HIPCHECK(c, hipGetLastError());
+  std::swap(c->act.p, c->act2.p);
+  std::swap(c->act.cap, c->act2.cap);
+  c->nact = (int)slots;
+  return PHSPH_OK;
+}
 
 // The first attempt's lists hold, for a particle i, every j within
 // hf*radkern*max(h0_i, h0_j).  They contain the force neighbours (q_i < R or
@@ -844,32 +981,6 @@ __global__ void k_check_hbound(int nact, const int32_t *__restrict__ act, const 
   }
   if (__ballot(bad) != 0ull && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
 }
-int pad_groups(phsph_ctx *c) {
-  const int na = c->nact;
-  if (na == 0 || na >= c->nlive) return PHSPH_OK;
-  hipStream_t s = c->stream;
-  std::vector<int32_t> a((size_t)na);
-  PH_HIPCHECK(c, hipMemcpyAsync(a.data(), c->act.p, sizeof(int32_t) * na, hipMemcpyDeviceToHost, s));
-  PH_HIPCHECK(c, hipStreamSynchronize(s));
-  std::vector<int32_t> &o = c->act_host;
-  o.clear();
-  int lane = 0;
-  for (int t = 0; t < na; t++) {
-    if (lane == GROUP || (lane > 0 && a[t] - a[t - 1] > PAD_GAP)) {
-      if (lane < GROUP) o.insert(o.end(), (size_t)(GROUP - lane), -1);
-      lane = 0;
-    }
-    o.push_back(a[t]);
-    lane++;
-  }
-  if (o.size() == (size_t)na) return PHSPH_OK;  // no group was closed early
-  PH_HIPCHECK(c, c->act.ensure(o.size() + 1));
-  PH_HIPCHECK(c, hipMemcpyAsync(c->act.p, o.data(), sizeof(int32_t) * o.size(), hipMemcpyHostToDevice, s));
-  PH_HIPCHECK(c, hipStreamSynchronize(s));
-  c->nact = (int)o.size();
-  return PHSPH_OK;
-}
-
 int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   hipStream_t s = c->stream;
   const int n = c->nlive;

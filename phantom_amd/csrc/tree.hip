@@ -155,33 +155,39 @@ __global__ void k_count_blocks(const int32_t *live, int np, int32_t *bcount) {
   block_excl_scan<256>(b, shi, &tot);
   if (threadIdx.x == 0) bcount[blockIdx.x] = tot;
 }
-// exclusive scan of block counts (single block, sequential chunks)
-__global__ void k_scan_blocks(int32_t *bcount, int nb, int32_t *total) {
-  __shared__ int carry;
-  __shared__ int tmp[256];
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int base = 0; base < nb; base += 256) {
-    int i = base + threadIdx.x;
-    int v = (i < nb) ? bcount[i] : 0;
-    int tot;
-    // counts exceed 1: plain Hillis-Steele scan
-    tmp[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-      int t = (threadIdx.x >= o) ? tmp[threadIdx.x - o] : 0;
-      __syncthreads();
-      tmp[threadIdx.x] += t;
-      __syncthreads();
+// exclusive scan of block counts in place, total to *total: one block of
+// 1024 threads, four consecutive counts per thread
+__global__ void __launch_bounds__(1024) k_scan_blocks(int32_t *bcount, int nb, int32_t *total) {
+  __shared__ int wsum[16];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  int carry = 0;
+  for (int base = 0; base < nb; base += 4096) {
+    const int i0 = base + tid * 4;
+    int v[4];
+    for (int k = 0; k < 4; k++) v[k] = (i0 + k < nb) ? bcount[i0 + k] : 0;
+    const int t = v[0] + v[1] + v[2] + v[3];
+    int incl = t;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += u;
     }
-    int incl = tmp[threadIdx.x];
-    tot = tmp[255];
-    if (i < nb) bcount[i] = carry + incl - v;
+    if (lane == 63) wsum[w] = incl;
     __syncthreads();
-    if (threadIdx.x == 0) carry += tot;
+    int woff = 0, btot = 0;
+    for (int k = 0; k < 16; k++) {
+      const int x = wsum[k];
+      if (k < w) woff += x;
+      btot += x;
+    }
+    int run = carry + woff + incl - t;
+    for (int k = 0; k < 4; k++) {
+      if (i0 + k < nb) bcount[i0 + k] = run;
+      run += v[k];
+    }
+    carry += btot;
     __syncthreads();
   }
-  if (threadIdx.x == 0) *total = carry;
+  if (tid == 0) *total = carry;
 }
 // scatter live particles into the tree cache (construct_root_node :429-456)
 __global__ void k_fill_cache(DevConst dc, const double *xyzh, int ld, int np, const int32_t *live,
@@ -1172,7 +1178,7 @@ int tree_build(phsph_ctx *c, double *d_xyzh, int ld, int np, const int8_t *d_iph
     hipLaunchKernelGGL(k_root_wrap, dim3(nb), dim3(256), 0, s, c->dc, d_xyzh, ld, np, live, c->dred.p,
                        c->counters.p);
     hipLaunchKernelGGL(k_count_blocks, dim3(nb), dim3(256), 0, s, live, np, bcount);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(256), 0, s, bcount, (int)nb, c->counters.p + CNT_LIVE);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, bcount, (int)nb, c->counters.p + CNT_LIVE);
     hipLaunchKernelGGL(k_fill_cache, dim3(nb), dim3(256), 0, s, c->dc, d_xyzh, ld, np, live, bcount, d_iphase,
                        c->tx.p, c->ty.p, c->tz.p, c->th.p, c->iparts.p);
   }
@@ -1274,7 +1280,7 @@ int tree_build(phsph_ctx *c, double *d_xyzh, int ld, int np, const int8_t *d_iph
   {
     LaunchTimer lt(c, "tree_leaves");
     hipLaunchKernelGGL(k_leaf_count, dim3(nbl), dim3(256), 0, s, c->leaf_flag.p, c->ncells, c->tmpi2.p);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(256), 0, s, c->tmpi2.p, (int)nbl,
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, c->tmpi2.p, (int)nbl,
                        c->counters.p + CNT_NLEAF);
     hipLaunchKernelGGL(k_leaf_fill, dim3(nbl), dim3(256), 0, s, c->leaf_flag.p, c->ncells, c->tmpi2.p,
                        c->leaves.p);
