@@ -38,6 +38,11 @@ struct DBuf {
   }
 };
 
+// Nodes of at most this many particles are built inside one workgroup;
+// larger nodes take the chunked large-node phase (which also forms their
+// second moments under self-gravity).
+constexpr int TREE_SUB_MAX = 1024;
+
 // One node of the tree build's large-node phase (more particles than one
 // workgroup builds in LDS): its range, split decision and chunk span.
 struct BigNode {

@@ -5,7 +5,8 @@
 // 2909-2927).
 //
 //  * k_quads: second moments of every node about its centre of mass
-//    (kdtree.F90:724-750), one wavefront per node.
+//    (kdtree.F90:724-750), one wavefront per node of at most TREE_SUB_MAX
+//    particles (the tree build forms those of larger nodes chunk by chunk).
 //  * k_hmax_replay: the reference raises a leaf's hmax (and its ancestors')
 //    whenever a density iteration relists the cell (set_hmaxcell,
 //    neigh_kdtree.f90:115-131; dens.F90:338-373).  The density kernel keeps
@@ -47,6 +48,7 @@ k_quads(DevConst dc, int64_t ncells, const phsph_kdnode *__restrict__ nd, const 
         const double *__restrict__ ty, const double *__restrict__ tz, double *__restrict__ quads) {
   const int64_t n = (int64_t)blockIdx.x + 1;
   if (n > ncells || !node_valid((int)n, leaf_flag)) return;
+  if (nd_hi[n] - nd_lo[n] + 1 > TREE_SUB_MAX) return;  // formed by the tree build's large-node phase
   const int lane = threadIdx.x;
   const double c0 = nd[n].xcen[0], c1 = nd[n].xcen[1], c2 = nd[n].xcen[2];
   const double pm = dc.pmass;

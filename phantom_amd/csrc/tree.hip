@@ -22,7 +22,7 @@
 //  * Children are numbered 2n, 2n+1 (Gafton & Rosswog indexing, :820-823).
 #include "stream.h"
 
-#define SUB_MAX 1024  // particles per LDS subtree
+#define SUB_MAX TREE_SUB_MAX  // particles per LDS subtree
 #define SUB_BS 256
 #define SUB_QMAX 192  // >= 2*floor(SUB_MAX/(MINPART+1)) nodes per level
 
@@ -531,37 +531,48 @@ __global__ void __launch_bounds__(BN_SEQ_BS) k_bn_seq(DevConst dc, BigNode *bn, 
 }
 
 // size (kdtree.F90:697-782) and the count of particles on the left of the
-// pivot, per chunk
-__global__ void __launch_bounds__(BN_BS) k_bn_size(const BigNode *bn, const int32_t *ch_node, int maxch,
-                                                   const int32_t *counters, const double *tx, const double *ty,
-                                                   const double *tz, double *ch_red, int32_t *ch_i) {
-  __shared__ double sh[BN_BS / 64 * 2];
+// pivot, per chunk; under self-gravity also the chunk's share of the second
+// moments about the centre of mass (kdtree.F90:724-750)
+__global__ void __launch_bounds__(BN_BS) k_bn_size(DevConst dc, const BigNode *bn, const int32_t *ch_node,
+                                                   int maxch, const int32_t *counters, const double *tx,
+                                                   const double *ty, const double *tz, double *ch_red,
+                                                   int32_t *ch_i, int quads) {
+  __shared__ double sh[BN_BS / 64 * 8];
   const int c = blockIdx.x;
   int f, p0, p1;
   if (!bn_chunk(bn, ch_node, counters, maxch, c, f, p0, p1)) return;
   const double c0 = bn[f].com[0], c1 = bn[f].com[1], c2 = bn[f].com[2], pivot = bn[f].pivot;
   const double *tax = axis_arr(bn[f].iaxis, tx, ty, tz);
-  double v[2] = {0., 0.};
+  const double pm = dc.pmass;
+  double v[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
 #pragma unroll
   for (int u = 0; u < BN_U; u++) {
     const int p = p0 + u * BN_BS + threadIdx.x;
     if (p <= p1) {
       v[0] = fmax(v[0], r2_exact(xsub(tx[p], c0), xsub(ty[p], c1), xsub(tz[p], c2)));
       v[1] += (tax[p] <= pivot) ? 1. : 0.;
+      if (quads) {
+        const double dx = tx[p] - c0, dy = ty[p] - c1, dz = tz[p] - c2;
+        v[2] += pm * (dx * dx); v[3] += pm * (dx * dy); v[4] += pm * (dx * dz);
+        v[5] += pm * (dy * dy); v[6] += pm * (dy * dz); v[7] += pm * (dz * dz);
+      }
     }
   }
-  const int kind[2] = {1, 0};
-  block_reduce<BN_BS, 2>(v, kind, sh);
+  const int kind[8] = {1, 0, 0, 0, 0, 0, 0, 0};
+  block_reduce<BN_BS, 8>(v, kind, sh);
   if (threadIdx.x == 0) {
     ch_red[(size_t)c * 12 + 0] = v[0];
     ch_i[(size_t)c * 4 + 1] = (int)v[1];
+    if (quads)
+      for (int k = 0; k < 6; k++) ch_red[(size_t)c * 12 + 1 + k] = v[2 + k];
   }
 }
 
 // the node record, and the split: left count or, with everything on one
 // side, half and half in place (kdtree.F90:856-865)
 __global__ void k_bn_node(BigNode *bn, int maxch, int32_t *counters, const double *ch_red, const int32_t *ch_i,
-                          phsph_kdnode *nd, int32_t *leaf_flag, int level, int maxlevel_indexed, int64_t nslots) {
+                          phsph_kdnode *nd, int32_t *leaf_flag, int level, int maxlevel_indexed, int64_t nslots,
+                          double *quads) {
   const int f = blockIdx.x, lane = threadIdx.x;
   if (counters[CNT_NCHUNK] > maxch) return;
   BigNode *b = bn + f;
@@ -575,6 +586,14 @@ __global__ void k_bn_node(BigNode *bn, int maxch, int32_t *counters, const doubl
   }
   r2 = wave_max(r2);
   cl = wave_sum_i(cl);
+  if (quads) {  // second moments: chunk shares in chunk order per lane, then across lanes
+    double q[6] = {0., 0., 0., 0., 0., 0.};
+    for (int k = lane; k < nchunk; k += 64)
+      for (int j = 0; j < 6; j++) q[j] += ch_red[(size_t)(chunk0 + k) * 12 + 1 + j];
+    for (int j = 0; j < 6; j++) q[j] = wave_sum(q[j]);
+    if (lane == 0)
+      for (int j = 0; j < 6; j++) quads[6 * (size_t)b->nnode + j] = q[j];
+  }
   if (lane == 0) {
     const int nnode = b->nnode, il = 2 * nnode, ir = il + 1;
     const int n = b->hi - b->lo + 1;
@@ -1164,6 +1183,12 @@ int tree_build(phsph_ctx *c, double *d_xyzh, int ld, int np, const int8_t *d_iph
   PH_HIPCHECK(c, c->ch_i.ensure((size_t)maxch * 4));
   PH_HIPCHECK(c, c->ch_off.ensure((size_t)maxch * 2));
   PH_HIPCHECK(c, c->ch_red.ensure((size_t)maxch * 12));
+  // under self-gravity the large nodes' second moments are formed here
+  double *quads = nullptr;
+  if (c->p.gravity) {
+    PH_HIPCHECK(c, c->quads.ensure((size_t)c->nslots * 6));
+    quads = c->quads.p;
+  }
   int32_t *live = c->tmpi.p;
   int32_t *bcount = c->frontier_b.p;  // block counts of the compaction
 
@@ -1233,10 +1258,10 @@ int tree_build(phsph_ctx *c, double *d_xyzh, int ld, int np, const int8_t *d_iph
                          c->ch_i.p);
       hipLaunchKernelGGL(k_bn_seq, dim3(nfront), dim3(BN_SEQ_BS), 0, s, c->dc, bn, maxch, cnt_d, c->ch_i.p,
                          c->tx.p, c->ty.p, c->tz.p);
-      hipLaunchKernelGGL(k_bn_size, dim3(gch), dim3(BN_BS), 0, s, bn, chn, maxch, cnt_d, c->tx.p, c->ty.p,
-                         c->tz.p, c->ch_red.p, c->ch_i.p);
+      hipLaunchKernelGGL(k_bn_size, dim3(gch), dim3(BN_BS), 0, s, c->dc, bn, chn, maxch, cnt_d, c->tx.p, c->ty.p,
+                         c->tz.p, c->ch_red.p, c->ch_i.p, quads ? 1 : 0);
       hipLaunchKernelGGL(k_bn_node, dim3(nfront), dim3(64), 0, s, bn, maxch, cnt_d, c->ch_red.p, c->ch_i.p,
-                         c->nd.p, c->leaf_flag.p, level, c->maxlevel_indexed, c->nslots);
+                         c->nd.p, c->leaf_flag.p, level, c->maxlevel_indexed, c->nslots, quads);
       hipLaunchKernelGGL(k_bn_pcount, dim3(gch), dim3(BN_BS), 0, s, bn, chn, maxch, cnt_d, c->tx.p, c->ty.p,
                          c->tz.p, c->ch_i.p);
       hipLaunchKernelGGL(k_bn_pscan, dim3(nfront), dim3(64), 0, s, bn, maxch, cnt_d, c->ch_i.p, c->ch_off.p);
