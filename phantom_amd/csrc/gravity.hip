@@ -643,8 +643,10 @@ k_force_grav(DevConst dc, const int32_t *__restrict__ act, int nact, const int32
         const double hj21 = xmul(hj1, hj1);
         const double q2j = xmul(rij2, hj21);
         if (!(q2i < R2 || q2j < R2)) {
-          // trial neighbour outside both kernels: 1/r^2 (force.F90:1991-2053)
-          const double rij1 = 1. / sqrt(rij2);
+          // trial neighbour outside both kernels: 1/r^2 (force.F90:1991-2053).
+          // 1/sqrt(r^2) as one reciprocal square root (within 2 ulp of the
+          // reference's sqrt and division); the neighbour tests above use r^2
+          const double rij1 = rsqrt(rij2);
           const double fgravj = rij1 * rij1 * rij1 * pm;
           fx = fx - dx * fgravj;
           fy = fy - dy * fgravj;
