@@ -13,12 +13,13 @@
 //    each particle's h after every iteration; this kernel replays the cell's
 //    compute_hmax sequence (dens.F90:1275-1289) to give the force walk the
 //    same node hmax, which decides which node pairs are well separated.
-//  * k_grav_walk: one wavefront per leaf runs getneigh_dual breadth-parallel:
-//    up to 64 (source, level) pairs per step; well-separated pairs add their
-//    quadrupole expansion (compute_M2L) to the level's coefficients in LDS,
-//    the rest open the source; leaf-leaf pairs that are not well separated
-//    become the leaf's near-field list.  Lane 0 then shifts the per-level
-//    expansions down the branch to the leaf (propagate_fnode_to_node).
+//  * k_gw_level: getneigh_dual node by node, one level of destination nodes
+//    per launch, one wavefront per node: well-separated candidate sources
+//    add their quadrupole expansion (compute_M2L) to the node's
+//    coefficients, the rest are opened for the node's children; the
+//    children's incoming expansions are shifted from the node's
+//    (propagate_fnode_to_node).  At a leaf the sources are opened down to
+//    leaves, which form its near-field list.
 //  * k_force_grav: one thread per active particle over its leaf's
 //    near-field list: SPH pair terms plus softened gravity for neighbours,
 //    1/r^2 for the other trial particles, then the far field by Taylor
