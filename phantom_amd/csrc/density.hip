@@ -522,48 +522,104 @@ k_dens_final(DevConst dc, const int32_t *__restrict__ act, int nact, const uint3
   for (int k = 0; k < 9; k++) { dv[k] = 0.; da[k] = 0.; dB[k] = 0.; }
   for (int k = 0; k < 6; k++) rr[k] = 0.;
   int nneigh = 1;
-  for (int k = 0; k < nl; k++) {
-    const uint32_t e = nlist[list_at(nact, t, k)];
-    const int j = cand_node(e);
-    const double4 r = in.tp[j];
-    double dx, dy, dz;
-    sep_of(dc, cand_code(e), xi, yi, zi, r.x, r.y, r.z, dx, dy, dz);
-    const double r2 = r2_exact(dx, dy, dz);
-    const double q2 = xmul(r2, hi21);
-    if (!(q2 < dc.radkern2)) continue;
-    const double rij = sqrt(r2);
-    const double q = rij * hi1;
-    double wk, gk;
-    kern_wg<K, double>(q2, q, wk, gk);
-    rs_rho += wk * pm;
-    rs_gh += (-q * gk - 3. * wk) * pm;
-    if (dc.gravity) rs_gs += kern_dphidh<K, double>(q2, q) * pm;
-    nneigh++;
-    const double rij1 = 1. / (rij + PH_EPS64);
-    const double rg = rij1 * gk;
-    const double runix = dx * rg * pm, runiy = dy * rg * pm, runiz = dz * rg * pm;
-    const double4 vj = in.tv[j];
-    const double dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
-    rs_divv += dvx * runix + dvy * runiy + dvz * runiz;
-    dv[0] += dvx * runix; dv[1] += dvx * runiy; dv[2] += dvx * runiz;
-    dv[3] += dvy * runix; dv[4] += dvy * runiy; dv[5] += dvy * runiz;
-    dv[6] += dvz * runix; dv[7] += dvz * runiy; dv[8] += dvz * runiz;
-    if (have_f) {
-      const double4 fj = in.tf[j];
-      const double dax = fi.x - fj.x, day = fi.y - fj.y, daz = fi.z - fj.z;
-      da[0] += dax * runix; da[1] += dax * runiy; da[2] += dax * runiz;
-      da[3] += day * runix; da[4] += day * runiy; da[5] += day * runiz;
-      da[6] += daz * runix; da[7] += daz * runiy; da[8] += daz * runiz;
-    }
-    rr[0] -= dx * runix; rr[1] -= dx * runiy; rr[2] -= dx * runiz;
-    rr[3] -= dy * runiy; rr[4] -= dy * runiz; rr[5] -= dz * runiz;
-    if (MHD) {  // B_j = Bevol_j * rho_j with rho_j from the h it had on entry
+  if constexpr (MHD) {
+    for (int k = 0; k < nl; k++) {
+      const uint32_t e = nlist[list_at(nact, t, k)];
+      const int j = cand_node(e);
+      const double4 r = in.tp[j];
+      double dx, dy, dz;
+      sep_of(dc, cand_code(e), xi, yi, zi, r.x, r.y, r.z, dx, dy, dz);
+      const double r2 = r2_exact(dx, dy, dz);
+      const double q2 = xmul(r2, hi21);
+      if (!(q2 < dc.radkern2)) continue;
+      const double rij = sqrt(r2);
+      const double q = rij * hi1;
+      double wk, gk;
+      kern_wg<K, double>(q2, q, wk, gk);
+      rs_rho += wk * pm;
+      rs_gh += (-q * gk - 3. * wk) * pm;
+      if (dc.gravity) rs_gs += kern_dphidh<K, double>(q2, q) * pm;
+      nneigh++;
+      const double rij1 = 1. / (rij + PH_EPS64);
+      const double rg = rij1 * gk;
+      const double runix = dx * rg * pm, runiy = dy * rg * pm, runiz = dz * rg * pm;
+      const double4 vj = in.tv[j];
+      const double dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
+      rs_divv += dvx * runix + dvy * runiy + dvz * runiz;
+      dv[0] += dvx * runix; dv[1] += dvx * runiy; dv[2] += dvx * runiz;
+      dv[3] += dvy * runix; dv[4] += dvy * runiy; dv[5] += dvy * runiz;
+      dv[6] += dvz * runix; dv[7] += dvz * runiy; dv[8] += dvz * runiz;
+      if (have_f) {
+        const double4 fj = in.tf[j];
+        const double dax = fi.x - fj.x, day = fi.y - fj.y, daz = fi.z - fj.z;
+        da[0] += dax * runix; da[1] += dax * runiy; da[2] += dax * runiz;
+        da[3] += day * runix; da[4] += day * runiy; da[5] += day * runiz;
+        da[6] += daz * runix; da[7] += daz * runiy; da[8] += daz * runiz;
+      }
+      rr[0] -= dx * runix; rr[1] -= dx * runiy; rr[2] -= dx * runiz;
+      rr[3] -= dy * runiy; rr[4] -= dy * runiz; rr[5] -= dz * runiz;
+      // B_j = Bevol_j * rho_j with rho_j from the h it had on entry
       const double4 B = in.tB[j];
       const double dBx = Bxi - B.x * B.w, dBy = Byi - B.y * B.w, dBz = Bzi - B.z * B.w;
       rs_divB += dBx * runix + dBy * runiy + dBz * runiz;
       dB[0] += dBx * runix; dB[1] += dBx * runiy; dB[2] += dBx * runiz;
       dB[3] += dBy * runix; dB[4] += dBy * runiy; dB[5] += dBy * runiz;
       dB[6] += dBz * runix; dB[7] += dBz * runiy; dB[8] += dBz * runiz;
+    }
+  } else {
+    // the records of entry k+1 are loaded while entry k is summed (velocity
+    // and force also for entries that turn out to lie outside the kernel)
+    const double4 zero4 = make_double4(0., 0., 0., 0.);
+    uint32_t e = 0u;
+    double4 r = zero4, vj = zero4, fj = zero4;
+    if (nl > 0) {
+      e = nlist[list_at(nact, t, 0)];
+      const int j = cand_node(e);
+      r = in.tp[j];
+      vj = in.tv[j];
+      if (have_f) fj = in.tf[j];
+    }
+    for (int k = 0; k < nl; k++) {
+      uint32_t en = 0u;
+      double4 rn = zero4, vn = zero4, fn = zero4;
+      if (k + 1 < nl) {
+        en = nlist[list_at(nact, t, k + 1)];
+        const int jn = cand_node(en);
+        rn = in.tp[jn];
+        vn = in.tv[jn];
+        if (have_f) fn = in.tf[jn];
+      }
+      double dx, dy, dz;
+      sep_of(dc, cand_code(e), xi, yi, zi, r.x, r.y, r.z, dx, dy, dz);
+      const double r2 = r2_exact(dx, dy, dz);
+      const double q2 = xmul(r2, hi21);
+      if (q2 < dc.radkern2) {
+        const double rij = sqrt(r2);
+        const double q = rij * hi1;
+        double wk, gk;
+        kern_wg<K, double>(q2, q, wk, gk);
+        rs_rho += wk * pm;
+        rs_gh += (-q * gk - 3. * wk) * pm;
+        if (dc.gravity) rs_gs += kern_dphidh<K, double>(q2, q) * pm;
+        nneigh++;
+        const double rij1 = 1. / (rij + PH_EPS64);
+        const double rg = rij1 * gk;
+        const double runix = dx * rg * pm, runiy = dy * rg * pm, runiz = dz * rg * pm;
+        const double dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
+        rs_divv += dvx * runix + dvy * runiy + dvz * runiz;
+        dv[0] += dvx * runix; dv[1] += dvx * runiy; dv[2] += dvx * runiz;
+        dv[3] += dvy * runix; dv[4] += dvy * runiy; dv[5] += dvy * runiz;
+        dv[6] += dvz * runix; dv[7] += dvz * runiy; dv[8] += dvz * runiz;
+        if (have_f) {
+          const double dax = fi.x - fj.x, day = fi.y - fj.y, daz = fi.z - fj.z;
+          da[0] += dax * runix; da[1] += dax * runiy; da[2] += dax * runiz;
+          da[3] += day * runix; da[4] += day * runiy; da[5] += day * runiz;
+          da[6] += daz * runix; da[7] += daz * runiy; da[8] += daz * runiz;
+        }
+        rr[0] -= dx * runix; rr[1] -= dx * runiy; rr[2] -= dx * runiz;
+        rr[3] -= dy * runiy; rr[4] -= dy * runiz; rr[5] -= dz * runiz;
+      }
+      e = en; r = rn; vj = vn; fj = fn;
     }
   }
   dens_store<MHD>(dc, p, hfin, rs_rho, rs_gh, rs_gs, rs_divv, rs_divB, dv, da, rr, dB, iparts, th_out, out,
