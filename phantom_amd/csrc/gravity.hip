@@ -629,18 +629,36 @@ k_force_grav(DevConst dc, const int32_t *__restrict__ act, int nact, const int32
     const int8_t ibinnow_m1 = (int8_t)(out.ibinnow - 1);
     const size_t pbase = G.poff ? (size_t)G.poff[l] : (size_t)l * G.capp;
     const int ne = min(G.np2p[l], G.poff ? (int)(G.poff[l + 1] - G.poff[l]) : G.capp);
+    // list entries are read two leaves ahead and leaf ranges one ahead, and
+    // each particle's position record while the previous one is summed
+    uint32_t en1 = (ne > 0) ? G.p2p[pbase] : 0u;
+    uint32_t en2 = (ne > 1) ? G.p2p[pbase + 1] : 0u;
+    int nlo = 0, nhi = -1;
+    if (ne > 0) {
+      nlo = nd_lo[cand_node(en1)];
+      nhi = nd_hi[cand_node(en1)];
+    }
     for (int e = 0; e < ne; e++) {
-      const uint32_t en = G.p2p[pbase + e];
-      const int cn = cand_node(en), code = cand_code(en);
-      const int j1 = nd_hi[cn];
-      for (int j = nd_lo[cn]; j <= j1; j++) {
+      const int code = cand_code(en1);
+      const int j0 = nlo, j1 = nhi;
+      en1 = en2;
+      if (e + 2 < ne) en2 = G.p2p[pbase + e + 2];
+      if (e + 1 < ne) {
+        nlo = nd_lo[cand_node(en1)];
+        nhi = nd_hi[cand_node(en1)];
+      }
+      double4 pn = make_double4(0., 0., 0., 0.);
+      if (j0 <= j1) pn = *reinterpret_cast<const double4 *>(rec + j0);
+      for (int j = j0; j <= j1; j++) {
+        const double4 pj = pn;
+        if (j + 1 <= j1) pn = *reinterpret_cast<const double4 *>(rec + j + 1);
         if (j == p) continue;
         const FRec &Rj = rec[j];
         double dx, dy, dz;
-        sep_of(dc, code, xi, yi, zi, Rj.x, Rj.y, Rj.z, dx, dy, dz);
+        sep_of(dc, code, xi, yi, zi, pj.x, pj.y, pj.z, dx, dy, dz);
         const double rij2 = r2_exact(dx, dy, dz);
         const double q2i = xmul(rij2, hi21);
-        const double hj1 = Rj.hinv;
+        const double hj1 = pj.w;
         const double hj21 = xmul(hj1, hj1);
         const double q2j = xmul(rij2, hj21);
         if (!(q2i < R2 || q2j < R2)) {
