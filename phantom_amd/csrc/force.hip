@@ -243,13 +243,24 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *_
   const int nl = live ? t_nl[t] : 0;
   const int ntot = nl + ((live && t_nlj) ? t_nlj[t] : 0);
   int npair = 0;  // interacting pairs (entries that pass the neighbour test)
+  // entry kk+1 and its position record are read while entry kk is summed
+  auto slot = [&](int kk) { return kk < nl ? kk : capb - 1 - (kk - nl); };
+  uint32_t en = 0u;
+  double4 pn = make_double4(0., 0., 0., 0.);
+  if (ntot > 0) {
+    en = nlist[list_at_g(loff, nact, t, slot(0))];
+    pn = fpos[cand_node(en)];
+  }
   for (int kk = 0; kk < ntot; kk++) {
-    const int k = kk < nl ? kk : capb - 1 - (kk - nl);
-    const uint32_t e = nlist[list_at_g(loff, nact, t, k)];
+    const uint32_t e = en;
     const int j = cand_node(e);
     // the neighbour test on the 32-byte position record (x, y, z, 1/h); the
     // full record is read for neighbours only
-    const double4 pj = fpos[j];
+    const double4 pj = pn;
+    if (kk + 1 < ntot) {
+      en = nlist[list_at_g(loff, nact, t, slot(kk + 1))];
+      pn = fpos[cand_node(en)];
+    }
     double dx, dy, dz;
     sep_of(dc, cand_code(e), xi, yi, zi, pj.x, pj.y, pj.z, dx, dy, dz);
     const double rij2 = r2_exact(dx, dy, dz);
@@ -516,11 +527,22 @@ k_force_mixed(DevConst dc, const int32_t *__restrict__ act, int nact, const int3
   const int nl = live ? t_nl[t] : 0;
   const int ntot = nl + ((live && t_nlj) ? t_nlj[t] : 0);
   int npair = 0;  // interacting pairs (entries that pass the neighbour test)
+  // entry kk+1 and its position record are read while entry kk is summed
+  auto slot = [&](int kk) { return kk < nl ? kk : capb - 1 - (kk - nl); };
+  uint32_t en = 0u;
+  double4 pn = make_double4(0., 0., 0., 0.);
+  if (ntot > 0) {
+    en = nlist[list_at_g(loff, nact, t, slot(0))];
+    pn = fpos[cand_node(en)];
+  }
   for (int kk = 0; kk < ntot; kk++) {
-    const int k = kk < nl ? kk : capb - 1 - (kk - nl);
-    const uint32_t e = nlist[list_at_g(loff, nact, t, k)];
+    const uint32_t e = en;
     const int j = cand_node(e);
-    const double4 pj = fpos[j];
+    const double4 pj = pn;
+    if (kk + 1 < ntot) {
+      en = nlist[list_at_g(loff, nact, t, slot(kk + 1))];
+      pn = fpos[cand_node(en)];
+    }
     double dx, dy, dz;
     sep_of(dc, cand_code(e), xi, yi, zi, pj.x, pj.y, pj.z, dx, dy, dz);
     const double rij2 = r2_exact(dx, dy, dz);
