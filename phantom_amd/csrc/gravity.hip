@@ -29,6 +29,7 @@
 constexpr int LENFGRAV = 20;
 constexpr int GMAXDEPTH = 32;   // kdtree.F90:42
 constexpr int GSTACK = 2048;    // (source, level) stack of the dual walk (LDS)
+constexpr int GW_MASKS = 64;    // candidates whose first-pass result is kept: 64 * GW_MASKS
 
 enum { GFL_STACK = 1, GFL_DEPTH = 2, GFL_P2P = 4, GFL_TRAJ = 8, GFL_CAND = 16, GFL_DTC = 32 };
 
@@ -310,6 +311,7 @@ k_gw_level(DevConst dc, int nent, GwLevel in, GwNext out, const int32_t *__restr
            uint32_t *__restrict__ p2p, int32_t *__restrict__ np2p, const int64_t *poff, int ucapp,
            int32_t *__restrict__ flags) {
   __shared__ uint32_t stk[GSTACK];
+  __shared__ unsigned long long omask[GW_MASKS];  // opened candidates of the first pass, 64 per word
   const int e = blockIdx.x;
   if (e >= nent) return;
   const int lane = threadIdx.x;
@@ -343,8 +345,11 @@ k_gw_level(DevConst dc, int nent, GwLevel in, GwNext out, const int32_t *__restr
         if (opened)
           npush = (leaf_flag[src] != 0) ? 1 : ((nd[src].leftchild != 0 ? 1 : 0) + (nd[src].rightchild != 0 ? 1 : 0));
       }
+      const unsigned long long m = __ballot(npush > 0);
+      if (lane == 0 && (b >> 6) < GW_MASKS) omask[b >> 6] = m;
       ntot += wave_sum_i(npush);
     }
+    wave_lds_sync();
     unsigned long long base = 0ull;
     if (lane == 0) base = atomicAdd(out.cand_used, (unsigned long long)ntot);
     base = __shfl(base, 0, 64);
@@ -357,11 +362,16 @@ k_gw_level(DevConst dc, int nent, GwLevel in, GwNext out, const int32_t *__restr
         uint32_t e0 = 0u, e1 = 0u;
         if (k < nc) {
           const int src = (int)in.cand[c0 + k];
-          bool opened = (src == A);
-          if (!opened) {
-            double dx, dy, dz, r2;
-            int code;
-            opened = !gw_wellsep(dc, nd, ghmax, A, src, dx, dy, dz, r2, code);
+          bool opened;
+          if ((b >> 6) < GW_MASKS) {
+            opened = (omask[b >> 6] >> lane) & 1ull;
+          } else {
+            opened = (src == A);
+            if (!opened) {
+              double dx, dy, dz, r2;
+              int code;
+              opened = !gw_wellsep(dc, nd, ghmax, A, src, dx, dy, dz, r2, code);
+            }
           }
           if (opened) {
             if (leaf_flag[src] != 0) {
