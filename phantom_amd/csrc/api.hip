@@ -126,6 +126,23 @@ struct Stager {
     if (_rc) return _rc;                                              \
   } while (0)
 
+struct SetWords {
+  uint32_t w[32];
+};
+__global__ void k_set_words(uint32_t *dst, SetWords b, int nw) {
+  const int i = threadIdx.x;
+  if (i < nw) dst[i] = b.w[i];
+}
+hipError_t dev_set(phsph_ctx *c, void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return hipSuccess;
+  if (bytes % 4 != 0 || bytes > sizeof(SetWords) || ((uintptr_t)dst & 3u) != 0)
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream);
+  SetWords b;
+  memcpy(b.w, src, bytes);
+  hipLaunchKernelGGL(k_set_words, dim3(1), dim3(32), 0, c->stream, (uint32_t *)dst, b, (int)(bytes / 4));
+  return hipGetLastError();
+}
+
 __global__ void k_zero_i32_api(int32_t *a, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0;

@@ -247,6 +247,10 @@ struct LaunchTimer {
 };
 
 void ctx_clear_timings(phsph_ctx *c);
+// copies a few host bytes (a multiple of 4, at most 128) to device memory on
+// the library stream with a one-block kernel whose arguments carry them:
+// no host staging buffer, and a capturable kernel node instead of a copy
+hipError_t dev_set(phsph_ctx *c, void *dst, const void *src, size_t bytes);
 DevConst make_devconst(const phsph_params &p);
 
 static inline unsigned nblocks(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }

@@ -788,7 +788,7 @@ int dens_trial_stats(phsph_ctx *c) {
   hipStream_t s = c->stream;
   PH_HIPCHECK(c, c->ntrial.ensure(c->nleaves + 1));
   int32_t zero = 0;
-  PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+  PH_HIPCHECK(c, dev_set(c, c->counters.p, &zero, sizeof(int32_t)));
   if (c->nleaves > 0)
     hipLaunchKernelGGL(k_walk, dim3(nblocks(c->nleaves, 128)), dim3(128), 0, s, c->dc, 1.0, c->nd.p,
                        c->leaf_flag.p, c->nd_lo.p, c->nd_hi.p, c->leaves.p, c->nleaves, c->ntrial.p,
@@ -796,7 +796,7 @@ int dens_trial_stats(phsph_ctx *c) {
   PH_HIPCHECK(c, hipGetLastError());
   PH_HIPCHECK(c, c->stats_i.ensure(8));
   long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  PH_HIPCHECK(c, hipMemcpyAsync(c->stats_i.p, z, sizeof(z), hipMemcpyHostToDevice, s));
+  PH_HIPCHECK(c, dev_set(c, c->stats_i.p, z, sizeof(z)));
   if (c->nact_dens > 0)
     hipLaunchKernelGGL(k_dens_stats, dim3(stats_grid(c->nact_dens)), dim3(256), 0, s, c->nact_dens,
                        c->act_dens.p, c->pleaf.p, nullptr, nullptr, c->ntrial.p, c->stats_i.p);
@@ -830,7 +830,7 @@ int walk_groups(phsph_ctx *c, int mode, double rfac, const double *d_hsrc, const
     if (!exact) PH_HIPCHECK(c, c->gcand.ensure((size_t)(ng + 1) * c->caps.gcand));
     PH_HIPCHECK(c, c->gncand.ensure(ng + 1));
     int32_t zero = 0;
-    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    PH_HIPCHECK(c, dev_set(c, c->counters.p, &zero, sizeof(int32_t)));
     if (ng > 0) {
       LaunchTimer lt(c, mode == 0 ? "walk_density" : "walk_force");
       hipLaunchKernelGGL(k_group_walk, dim3(xcd_grid(ng)), dim3(64), 0, s, c->dc, mode, rfac, c->nd.p, d_hmaxn,
@@ -1128,7 +1128,7 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
     PH_HIPCHECK(c, LB.ensure((size_t)list_cap(dlist) * na + 1));
     hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(n, 256)), dim3(256), 0, s, c->tmpi.p, n, 0);
     int32_t zero = 0;
-    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    PH_HIPCHECK(c, dev_set(c, c->counters.p, &zero, sizeof(int32_t)));
     {
       LaunchTimer lt(c, "density_iter");
 #define DITER(KK)                                                                                              \
@@ -1201,7 +1201,7 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   c->dlist_reuse = reusable;
   if (reusable && nact > 0) {
     int32_t zero = 0, bad = 0;
-    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p + 2, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    PH_HIPCHECK(c, dev_set(c, c->counters.p + 2, &zero, sizeof(int32_t)));
     hipLaunchKernelGGL(k_check_hbound, dim3(nblocks(nact, 256)), dim3(256), 0, s, nact, c->act_dens.p, c->th_in.p,
                        c->th.p, rfac0 * lmargin, c->counters.p + 2);
     PH_HIPCHECK(c, hipGetLastError());
@@ -1218,7 +1218,7 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   // statistics (dens.F90:1670-1679); trial counts are computed on request
   PH_HIPCHECK(c, c->stats_i.ensure(8));
   long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  PH_HIPCHECK(c, hipMemcpyAsync(c->stats_i.p, z, sizeof(z), hipMemcpyHostToDevice, s));
+  PH_HIPCHECK(c, dev_set(c, c->stats_i.p, z, sizeof(z)));
   if (nact > 0)
     hipLaunchKernelGGL(k_dens_stats, dim3(stats_grid(nact)), dim3(256), 0, s, nact, c->act_dens.p, c->pleaf.p,
                        c->p_nneigh.p, c->leaf_nits.p, nullptr, c->stats_i.p);

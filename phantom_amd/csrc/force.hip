@@ -794,7 +794,7 @@ static int force_lists(phsph_ctx *c, DBuf<uint32_t> &L, DBuf<int32_t> &TNL, cons
     const int ucap = list_cap(c->caps.flist);
     if (!loff) PH_HIPCHECK(c, L.ensure((size_t)ucap * nact + 1));
     int32_t zero = 0;
-    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p + 3, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    PH_HIPCHECK(c, dev_set(c, c->counters.p + 3, &zero, sizeof(int32_t)));
     if (nact > 0) {
       LaunchTimer lt(c, tname);
       hipLaunchKernelGGL(k_flist, dim3(list_grid(ng)), dim3(64 * LWPB), 0, s, c->dc, c->act.p, nact, ng, c->pleaf.p,
@@ -918,7 +918,7 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
   bool reuse = c->dlist_reuse && c->nact == c->nact_dens;
   if (reuse) {
     int32_t zero = 0;
-    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p + 2, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    PH_HIPCHECK(c, dev_set(c, c->counters.p + 2, &zero, sizeof(int32_t)));
     const int m = std::max(n, c->nact);
     hipLaunchKernelGGL(k_check_reuse, dim3(nblocks(m, 256)), dim3(256), 0, s, n, c->nact, c->act.p, c->act_dens.p,
                        c->fpos.p, c->fh.p, c->tp4.p, c->th.p, c->counters.p + 2);
@@ -946,11 +946,11 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
     memcpy(&init[0], &big, 8);
     init[1] = init[0];
     init[2] = 0ull;
-    PH_HIPCHECK(c, hipMemcpyAsync(c->dtred.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+    PH_HIPCHECK(c, dev_set(c, c->dtred.p, init, sizeof(init)));
   }
   int32_t zero2[2] = {0, 0};
   int32_t fl_first[2] = {0, 0};  // flags and bin maximum of a first pass
-  PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, zero2, sizeof(zero2), hipMemcpyHostToDevice, s));
+  PH_HIPCHECK(c, dev_set(c, c->counters.p, zero2, sizeof(zero2)));
   ForcePass fp{rec, recB, recD, rec32, mixed, mhd, dust};
   if (reuse) {
     // the density lists, without the particles the density pass ran again
@@ -969,7 +969,7 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
       const int64_t *loff = nullptr;
       rc = force_lists(c, c->nlist_r, c->t_nl_r, &loff, "force_list_rerun");
       if (rc) return rc;
-      PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, zero2, sizeof(zero2), hipMemcpyHostToDevice, s));
+      PH_HIPCHECK(c, dev_set(c, c->counters.p, zero2, sizeof(zero2)));
       rc = force_pass(c, icall, a, fp, c->nlist_r.p, loff, c->t_nl_r.p, nullptr, 0, nullptr);
       if (rc) return rc;
     }
@@ -1044,7 +1044,7 @@ int set_active_run(phsph_ctx *c, int np, const double *xyzh, int ldx, int8_t *ip
   hipStream_t s = c->stream;
   PH_HIPCHECK(c, c->counters.ensure(64));
   int32_t z[3] = {0, 0, 0};
-  PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, z, sizeof(z), hipMemcpyHostToDevice, s));
+  PH_HIPCHECK(c, dev_set(c, c->counters.p, z, sizeof(z)));
   if (np > 0) {
     LaunchTimer lt(c, "set_active");
     hipLaunchKernelGGL(k_set_active, dim3(nblocks(np, 256)), dim3(256), 0, s, np, xyzh, ldx, iphase, ibin, nbinmax,

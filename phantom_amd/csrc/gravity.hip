@@ -550,14 +550,14 @@ static int grav_walk(phsph_ctx *c, const int64_t *poff) {
   while (nent > 0) {
     if (level >= GMAXDEPTH) {  // a branch of more than GMAXDEPTH nodes
       int32_t fl = GFL_DEPTH;
-      PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &fl, sizeof(int32_t), hipMemcpyHostToDevice, s));
+      PH_HIPCHECK(c, dev_set(c, c->counters.p, &fl, sizeof(int32_t)));
       break;
     }
     const int nxt = cur ^ 1;
     int32_t zero = 0;
     unsigned long long zero64 = 0ull;
-    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p + 1, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
-    PH_HIPCHECK(c, hipMemcpyAsync(c->gw_used.p, &zero64, sizeof(zero64), hipMemcpyHostToDevice, s));
+    PH_HIPCHECK(c, dev_set(c, c->counters.p + 1, &zero, sizeof(int32_t)));
+    PH_HIPCHECK(c, dev_set(c, c->gw_used.p, &zero64, sizeof(zero64)));
     GwLevel in{c->gw_node[cur].p, c->gw_coff[cur].p, c->gw_ccnt[cur].p, c->gw_H[cur].p, c->gw_cand[cur].p};
     GwNext out{c->gw_node[nxt].p, c->gw_coff[nxt].p, c->gw_ccnt[nxt].p, c->gw_H[nxt].p, c->gw_cand[nxt].p,
                (int64_t)c->gw_cand[nxt].cap, c->gw_used.p, c->counters.p + 1};
@@ -574,7 +574,7 @@ static int grav_walk(phsph_ctx *c, const int64_t *poff) {
       // the level's candidate lists did not fit: a larger buffer, the level again
       PH_HIPCHECK(c, c->gw_cand[nxt].ensure((size_t)used + used / 4 + 1024));
       int32_t fl = cnt[0] & ~GFL_CAND;
-      PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &fl, sizeof(int32_t), hipMemcpyHostToDevice, s));
+      PH_HIPCHECK(c, dev_set(c, c->counters.p, &fl, sizeof(int32_t)));
       continue;
     }
     if (cnt[0] & GFL_STACK) break;
@@ -837,7 +837,7 @@ int grav_hmax_replay(phsph_ctx *c, int icall) {
   if (icall == 0 || c->nleaves == 0) return PHSPH_OK;
   hipStream_t s = c->stream;
   int32_t zero = 0;
-  PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p + 4, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+  PH_HIPCHECK(c, dev_set(c, c->counters.p + 4, &zero, sizeof(int32_t)));
   hipLaunchKernelGGL(k_hmax_replay, dim3(nblocks(c->nleaves, 128)), dim3(128), 0, s, c->nleaves, c->leaves.p,
                      c->leaf_flag.p, c->nd_lo.p, c->nd_hi.p, c->nd.p, c->p_nits.p, c->htraj.p, c->hconv.p,
                      c->ghmax.p, c->counters.p + 4);
@@ -863,7 +863,7 @@ int grav_force(phsph_ctx *c, int icall, const ForceIOArgs &a, const FRec *rec, c
   for (int attempt = 0; attempt < 12; attempt++) {
     if (!poff) PH_HIPCHECK(c, c->p2p.ensure((size_t)(nl + 1) * c->caps.p2p));
     int32_t zero = 0;
-    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    PH_HIPCHECK(c, dev_set(c, c->counters.p, &zero, sizeof(int32_t)));
     if (nl > 0) {
       LaunchTimer lt(c, "grav_walk");
       int rc = grav_walk(c, poff);
@@ -896,9 +896,9 @@ int grav_force(phsph_ctx *c, int icall, const ForceIOArgs &a, const FRec *rec, c
   memcpy(&init[0], &big, 8);
   init[1] = init[0];
   init[2] = 0ull;
-  PH_HIPCHECK(c, hipMemcpyAsync(c->dtred.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+  PH_HIPCHECK(c, dev_set(c, c->dtred.p, init, sizeof(init)));
   int32_t zero2[2] = {0, 0};
-  PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, zero2, sizeof(zero2), hipMemcpyHostToDevice, s));
+  PH_HIPCHECK(c, dev_set(c, c->counters.p, zero2, sizeof(zero2)));
   GravArgs G{c->fnode.p, c->p2p.p, c->np2p.p, poff, c->caps.p2p, gsoft, c->poten_out};
   if (nact > 0) {
     LaunchTimer lt(c, "force");

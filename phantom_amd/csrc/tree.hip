@@ -1196,7 +1196,7 @@ int tree_build(phsph_ctx *c, double *d_xyzh, int ld, int np, const int8_t *d_iph
   double x0[3];
   PH_HIPCHECK(c, hipMemcpyAsync(x0, d_xyzh, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
   int32_t hc[CNT_N] = {0, 0, 0, 0, 1 << 30, 0, 0, 0, 0, 0, 0};
-  PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p, hc, sizeof(hc), hipMemcpyHostToDevice, s));
+  PH_HIPCHECK(c, dev_set(c, c->counters.p, hc, sizeof(hc)));
   PH_HIPCHECK(c, hipStreamSynchronize(s));
   {
     LaunchTimer lt(c, "tree_root");
@@ -1219,10 +1219,10 @@ int tree_build(phsph_ctx *c, double *d_xyzh, int ld, int np, const int8_t *d_iph
   hipLaunchKernelGGL(k_root_bounds, dim3(1), dim3(256), 0, s, c->dred.p, (int)nb, x0[0], x0[1], x0[2], rootbox);
   hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(c->nslots, 256)), dim3(256), 0, s, c->leaf_flag.p, c->nslots, 0);
   int32_t root_init[2] = {0, c->nlive - 1};
-  PH_HIPCHECK(c, hipMemcpyAsync(c->nd_lo.p + 1, &root_init[0], sizeof(int32_t), hipMemcpyHostToDevice, s));
-  PH_HIPCHECK(c, hipMemcpyAsync(c->nd_hi.p + 1, &root_init[1], sizeof(int32_t), hipMemcpyHostToDevice, s));
+  PH_HIPCHECK(c, dev_set(c, c->nd_lo.p + 1, &root_init[0], sizeof(int32_t)));
+  PH_HIPCHECK(c, dev_set(c, c->nd_hi.p + 1, &root_init[1], sizeof(int32_t)));
   int32_t one = 1;
-  PH_HIPCHECK(c, hipMemcpyAsync(c->frontier_a.p, &one, sizeof(int32_t), hipMemcpyHostToDevice, s));
+  PH_HIPCHECK(c, dev_set(c, c->frontier_a.p, &one, sizeof(int32_t)));
   PH_HIPCHECK(c, hipMemcpyAsync(c->fbox_a.p, rootbox, 6 * sizeof(double), hipMemcpyDeviceToDevice, s));
   const bool dbg = getenv("PHSPH_DEBUG") != nullptr;
   int nfront = 1;
@@ -1232,9 +1232,10 @@ int tree_build(phsph_ctx *c, double *d_xyzh, int ld, int np, const int8_t *d_iph
   double *ba = c->fbox_a.p, *bb = c->fbox_b.p;
   while (nfront > 0) {
     int32_t zero = 0;
-    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p + CNT_NEXT, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
-    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p + CNT_NBIG, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
-    PH_HIPCHECK(c, hipMemcpyAsync(c->counters.p + CNT_SEQMAX, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    PH_HIPCHECK(c, dev_set(c, c->counters.p + CNT_NEXT, &zero, sizeof(int32_t)));
+    static_assert(CNT_SEQMAX == CNT_NBIG + 1, "adjacent level counters");
+    const int32_t zero2[2] = {0, 0};
+    PH_HIPCHECK(c, dev_set(c, c->counters.p + CNT_NBIG, zero2, sizeof(zero2)));
     if (nbig < nfront) {
       LaunchTimer lt(c, "tree_subtrees");
       hipLaunchKernelGGL(k_subtrees, dim3(nfront), dim3(SUB_BS), 0, s, c->dc, fa, ba, nfront, c->tx.p, c->ty.p,
