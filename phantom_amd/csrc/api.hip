@@ -187,7 +187,7 @@ int phsph_create(const phsph_params *p, int32_t device, phsph_ctx **out) {
   c->p = *p;
   c->dc = make_devconst(*p);
   // initial list capacities and the list budget (KiB) may be set from the
-  // environment, e.g. PHSPH_CAPS="gcand=64,dlist=32,flist=32,p2p=16,budget_kib=1":
+  // environment, e.g. PHSPH_CAPS="gcand=64,dlist=32,flist=32,p2p=16,gnear=8,budget_kib=1":
   // larger values avoid second search passes, smaller ones exercise them
   if (const char *env = getenv("PHSPH_CAPS")) {
     std::string spec(env);
@@ -205,6 +205,7 @@ int phsph_create(const phsph_params *p, int32_t device, phsph_ctx **out) {
           else if (key == "dlist") c->caps.dlist = v;
           else if (key == "flist") c->caps.flist = v;
           else if (key == "p2p") c->caps.p2p = v;
+          else if (key == "gnear") c->caps.gnear = v;
           else if (key == "budget_kib") c->list_budget = (size_t)v << 10;
         }
       }

@@ -73,6 +73,7 @@ struct Caps {
   int p2p = 256;     // near-field leaves per leaf (self-gravity)
   int dlist = 256;   // density neighbour list per particle
   int flist = 256;   // force neighbour list per particle
+  int gnear = 128;   // neighbours per particle in the self-gravity force pass
 };
 
 struct phsph_ctx {
@@ -182,6 +183,7 @@ struct phsph_ctx {
   DBuf<unsigned long long> ghmax;     // node hmax as the force walk sees it
   DBuf<double> htraj, hconv;          // h per density iteration, converged h
   DBuf<double> fnode;                 // far-field expansion per leaf (20)
+  DBuf<uint32_t> gnear;               // self-gravity force: neighbours per active particle
   // node-by-node dual walk: destination nodes of a level (ping-pong), their
   // candidate source lists (shared by siblings) and incoming expansions
   DBuf<int32_t> gw_node[2], gw_ccnt[2];

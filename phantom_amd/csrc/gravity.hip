@@ -31,7 +31,7 @@ constexpr int GMAXDEPTH = 32;   // kdtree.F90:42
 constexpr int GSTACK = 2048;    // (source, level) stack of the dual walk (LDS)
 constexpr int GW_MASKS = 64;    // candidates whose first-pass result is kept: 64 * GW_MASKS
 
-enum { GFL_STACK = 1, GFL_DEPTH = 2, GFL_P2P = 4, GFL_TRAJ = 8, GFL_CAND = 16, GFL_DTC = 32 };
+enum { GFL_STACK = 1, GFL_DEPTH = 2, GFL_P2P = 4, GFL_TRAJ = 8, GFL_CAND = 16, GFL_DTC = 32, GFL_NEAR = 64 };
 
 // ids 2n, 2n+1 are the children of n: a slot is a node when every ancestor
 // is an internal node (leaf_flag == 0)
@@ -597,6 +597,9 @@ struct GravArgs {
   int capp;
   const float *gsoft;      // gradh(2) in tree order
   float *poten;            // caller order (may be null)
+  uint32_t *nbuf;          // neighbours per active particle (list_at layout)
+  int capn;                // its capacity per particle
+  int32_t *nmax;           // longest neighbour list met (after an overflow)
 };
 
 template <int K, bool THERMAL>
@@ -639,8 +642,11 @@ k_force_grav(DevConst dc, const int32_t *__restrict__ act, int nact, const int32
     const int8_t ibinnow_m1 = (int8_t)(out.ibinnow - 1);
     const size_t pbase = G.poff ? (size_t)G.poff[l] : (size_t)l * G.capp;
     const int ne = min(G.np2p[l], G.poff ? (int)(G.poff[l + 1] - G.poff[l]) : G.capp);
-    // list entries are read two leaves ahead and leaf ranges one ahead, and
-    // each particle's position record while the previous one is summed
+    int nn = 0;  // neighbours met in the first pass
+    // first pass, over the near-field leaves: the 1/r^2 terms of the trial
+    // particles outside both kernels, and the list of the neighbours.  List
+    // entries are read two leaves ahead and leaf ranges one ahead, and each
+    // particle's position record while the previous one is summed
     uint32_t en1 = (ne > 0) ? G.p2p[pbase] : 0u;
     uint32_t en2 = (ne > 1) ? G.p2p[pbase + 1] : 0u;
     int nlo = 0, nhi = -1;
@@ -663,7 +669,6 @@ k_force_grav(DevConst dc, const int32_t *__restrict__ act, int nact, const int32
         const double4 pj = pn;
         if (j + 1 <= j1) pn = *reinterpret_cast<const double4 *>(rec + j + 1);
         if (j == p) continue;
-        const FRec &Rj = rec[j];
         double dx, dy, dz;
         sep_of(dc, code, xi, yi, zi, pj.x, pj.y, pj.z, dx, dy, dz);
         const double rij2 = r2_exact(dx, dy, dz);
@@ -683,82 +688,104 @@ k_force_grav(DevConst dc, const int32_t *__restrict__ act, int nact, const int32
           pot = pot + pm * (-rij1);
           continue;
         }
-        nl++;
-        if (ind) {  // force.F90:1346-1352
-          ibin_neighi = max(ibin_neighi, (int)tbin_old[j]);
-          const int o = abs(iparts[j]) - 1;
-          if (out.ibin_wake[o] < ibinnow_m1) out.ibin_wake[o] = ibinnow_m1;
-        }
-        double rij1, qi;
-        if (rij2 > PH_TINY64) {
-          rij1 = 1. / sqrt(rij2);
-          qi = (rij2 * rij1) * hi1;
-        } else {
-          rij1 = 0.;
-          qi = 0.;
-        }
-        double grkerni = 0., phii, fgravi;
-        if (q2i < R2) {
-          grkerni = kern_g<K, double>(q2i, qi) * hfacgrkern;
-          double fmi;
-          kern_soft<K>(q2i, qi, phii, fmi);
-          phii = phii * hi1;
-          fmi = fmi * hi21;
-          fgravi = fmi + gradsofti * grkerni;
-        } else {
-          phii = -rij1;
-          fgravi = rij1 * rij1;
-        }
-        double grkernj = 0., fgravj;
-        bool usej = false;
-        if (q2j < R2) {
-          const double qj = (rij2 * rij1) * hj1;
-          grkernj = kern_g<K, double>(q2j, qj) * hj21 * hj21 * dc.cnormk * (double)Rj.gradh;
-          double phij, fmj;
-          kern_soft<K>(q2j, qj, phij, fmj);
-          fmj = fmj * hj21;
-          fgravj = fmj + (double)G.gsoft[j] * grkernj;
-          usej = true;
-        } else {
-          fgravj = rij1 * rij1;
-        }
-        const double fgrav = 0.5 * (pm * fgravi + pm * fgravj);
-        const double runix = dx * rij1, runiy = dy * rij1, runiz = dz * rij1;
-        const double dvx = vxi - Rj.vx, dvy = vyi - Rj.vy, dvz = vzi - Rj.vz;
-        const double projv = dvx * runix + dvy * runiy + dvz * runiz;
-        const double denij = THERMAL ? (eni - Rj.u) : 0.;
-        const double vsigi = fmax(vwavei - dc.beta * projv, 0.);
-        const double vsigavi = fmax(alphai * vwavei - dc.beta * projv, 0.);
-        vsigmax = fmax(vsigmax, vsigi);
-        double rho1j = 0., pro2j = 0., vwavej = 0., vsigavj = 0., autermj = 0.;
-        if (usej) {
-          rho1j = Rj.rho1;
-          pro2j = Rj.pro2;
-          vwavej = Rj.vwave;
-          autermj = 0.5 * pm * rho1j * dc.alphau;
-          vsigmax = fmax(vsigmax, fmax(vwavej - dc.beta * projv, 0.));
-          vsigavj = fmax((double)Rj.alpha * vwavej - dc.beta * projv, 0.);
-        }
-        double qrho2i = 0., qrho2j = 0.;
-        if (projv < 0.) {
-          qrho2i = -0.5 * rho1i * vsigavi * projv;
-          if (usej) qrho2j = -0.5 * rho1j * vsigavj * projv;
-        }
-        const double dudtdissi = pm * qrho2i * projv * grkerni;
-        const double gradpi = pm * (pro2i + qrho2i) * grkerni;
-        const double gradpj = usej ? pm * (pro2j + qrho2j) * grkernj : 0.;
-        if (THERMAL) {
-          const double vsigu = fabs(projv);  // with self-gravity (force.F90:1606-1624)
-          dudtdiss += dudtdissi;
-          dendtdiss += vsigu * denij * (auterm * grkerni + autermj * grkernj);
-        }
-        const double gradp = gradpi + gradpj;
-        fx = fx - runix * (gradp + fgrav);
-        fy = fy - runiy * (gradp + fgrav);
-        fz = fz - runiz * (gradp + fgrav);
-        pot = pot + pm * phii;
-        drhodt += projv * grkerni;
+        // a neighbour: its pair terms are summed in the second pass, where
+        // every lane of the wavefront has a neighbour to sum
+        if (nn < G.capn) G.nbuf[list_at(nact, t, nn)] = cand_pack(j, code);
+        nn++;
       }
+    }
+    if (nn > G.capn) {
+      atomicOr(flags, GFL_NEAR);
+      atomicMax(G.nmax, nn);
+    }
+    // second pass: the neighbours (force.F90:1286-1990)
+    const int nk = min(nn, G.capn);
+    for (int kk = 0; kk < nk; kk++) {
+      const uint32_t ek = G.nbuf[list_at(nact, t, kk)];
+      const int j = cand_node(ek), code = cand_code(ek);
+      const FRec &Rj = rec[j];
+      double dx, dy, dz;
+      sep_of(dc, code, xi, yi, zi, Rj.x, Rj.y, Rj.z, dx, dy, dz);
+      const double rij2 = r2_exact(dx, dy, dz);
+      const double q2i = xmul(rij2, hi21);
+      const double hj1 = Rj.hinv;
+      const double hj21 = xmul(hj1, hj1);
+      const double q2j = xmul(rij2, hj21);
+      nl++;
+      if (ind) {  // force.F90:1346-1352
+        ibin_neighi = max(ibin_neighi, (int)tbin_old[j]);
+        const int o = abs(iparts[j]) - 1;
+        if (out.ibin_wake[o] < ibinnow_m1) out.ibin_wake[o] = ibinnow_m1;
+      }
+      double rij1, qi;
+      if (rij2 > PH_TINY64) {
+        rij1 = 1. / sqrt(rij2);
+        qi = (rij2 * rij1) * hi1;
+      } else {
+        rij1 = 0.;
+        qi = 0.;
+      }
+      double grkerni = 0., phii, fgravi;
+      if (q2i < R2) {
+        grkerni = kern_g<K, double>(q2i, qi) * hfacgrkern;
+        double fmi;
+        kern_soft<K>(q2i, qi, phii, fmi);
+        phii = phii * hi1;
+        fmi = fmi * hi21;
+        fgravi = fmi + gradsofti * grkerni;
+      } else {
+        phii = -rij1;
+        fgravi = rij1 * rij1;
+      }
+      double grkernj = 0., fgravj;
+      bool usej = false;
+      if (q2j < R2) {
+        const double qj = (rij2 * rij1) * hj1;
+        grkernj = kern_g<K, double>(q2j, qj) * hj21 * hj21 * dc.cnormk * (double)Rj.gradh;
+        double phij, fmj;
+        kern_soft<K>(q2j, qj, phij, fmj);
+        fmj = fmj * hj21;
+        fgravj = fmj + (double)G.gsoft[j] * grkernj;
+        usej = true;
+      } else {
+        fgravj = rij1 * rij1;
+      }
+      const double fgrav = 0.5 * (pm * fgravi + pm * fgravj);
+      const double runix = dx * rij1, runiy = dy * rij1, runiz = dz * rij1;
+      const double dvx = vxi - Rj.vx, dvy = vyi - Rj.vy, dvz = vzi - Rj.vz;
+      const double projv = dvx * runix + dvy * runiy + dvz * runiz;
+      const double denij = THERMAL ? (eni - Rj.u) : 0.;
+      const double vsigi = fmax(vwavei - dc.beta * projv, 0.);
+      const double vsigavi = fmax(alphai * vwavei - dc.beta * projv, 0.);
+      vsigmax = fmax(vsigmax, vsigi);
+      double rho1j = 0., pro2j = 0., vwavej = 0., vsigavj = 0., autermj = 0.;
+      if (usej) {
+        rho1j = Rj.rho1;
+        pro2j = Rj.pro2;
+        vwavej = Rj.vwave;
+        autermj = 0.5 * pm * rho1j * dc.alphau;
+        vsigmax = fmax(vsigmax, fmax(vwavej - dc.beta * projv, 0.));
+        vsigavj = fmax((double)Rj.alpha * vwavej - dc.beta * projv, 0.);
+      }
+      double qrho2i = 0., qrho2j = 0.;
+      if (projv < 0.) {
+        qrho2i = -0.5 * rho1i * vsigavi * projv;
+        if (usej) qrho2j = -0.5 * rho1j * vsigavj * projv;
+      }
+      const double dudtdissi = pm * qrho2i * projv * grkerni;
+      const double gradpi = pm * (pro2i + qrho2i) * grkerni;
+      const double gradpj = usej ? pm * (pro2j + qrho2j) * grkernj : 0.;
+      if (THERMAL) {
+        const double vsigu = fabs(projv);  // with self-gravity (force.F90:1606-1624)
+        dudtdiss += dudtdissi;
+        dendtdiss += vsigu * denij * (auterm * grkerni + autermj * grkernj);
+      }
+      const double gradp = gradpi + gradpj;
+      fx = fx - runix * (gradp + fgrav);
+      fy = fy - runiy * (gradp + fgrav);
+      fz = fz - runiz * (gradp + fgrav);
+      pot = pot + pm * phii;
+      drhodt += projv * grkerni;
     }
     // --- finish_cell_and_store_results with gravity (force.F90:2909-2927) ---
     const int o = abs(iparts[p]) - 1;
@@ -896,31 +923,45 @@ int grav_force(phsph_ctx *c, int icall, const ForceIOArgs &a, const FRec *rec, c
   memcpy(&init[0], &big, 8);
   init[1] = init[0];
   init[2] = 0ull;
-  PH_HIPCHECK(c, dev_set(c, c->dtred.p, init, sizeof(init)));
-  int32_t zero2[2] = {0, 0};
-  PH_HIPCHECK(c, dev_set(c, c->counters.p, zero2, sizeof(zero2)));
-  GravArgs G{c->fnode.p, c->p2p.p, c->np2p.p, poff, c->caps.p2p, gsoft, c->poten_out};
-  if (nact > 0) {
-    LaunchTimer lt(c, "force");
-    const dim3 gr(nblocks(nact, 128)), b(128);
-    const bool th = c->p.maxvxyzu >= 4;
+  int32_t fl2[2] = {0, 0};
+  unsigned long long red[3];
+  // the second pass's neighbour lists: a capacity per particle, kept for
+  // later calls; after an overflow the call runs again at the longest list
+  int capn = c->caps.gnear;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    PH_HIPCHECK(c, c->gnear.ensure((size_t)list_cap(capn) * (size_t)(nact > 0 ? nact : 1)));
+    PH_HIPCHECK(c, dev_set(c, c->dtred.p, init, sizeof(init)));
+    const int32_t zero3[3] = {0, 0, 0};
+    PH_HIPCHECK(c, dev_set(c, c->counters.p, zero3, 2 * sizeof(int32_t)));
+    PH_HIPCHECK(c, dev_set(c, c->counters.p + 5, zero3, sizeof(int32_t)));
+    GravArgs G{c->fnode.p, c->p2p.p, c->np2p.p, poff, c->caps.p2p, gsoft, c->poten_out,
+               c->gnear.p, list_cap(capn), c->counters.p + 5};
+    if (nact > 0) {
+      LaunchTimer lt(c, "force");
+      const dim3 gr(nblocks(nact, 128)), b(128);
+      const bool th = c->p.maxvxyzu >= 4;
 #define GL(KK, T)                                                                                               \
   hipLaunchKernelGGL((k_force_grav<KK, T>), gr, b, 0, s, c->dc, c->act.p, nact, c->pleaf.p, c->leaves.p,        \
                      c->nd_lo.p, c->nd_hi.p, c->nd.p, c->iparts.p, rec, G, a, c->tbin_old.p, icall, c->dtred.p,  \
                      c->counters.p)
-    if (c->p.kernel == PHSPH_KERNEL_CUBIC) {
-      if (th) GL(PHSPH_KERNEL_CUBIC, true); else GL(PHSPH_KERNEL_CUBIC, false);
-    } else {
-      if (th) GL(PHSPH_KERNEL_QUINTIC, true); else GL(PHSPH_KERNEL_QUINTIC, false);
-    }
+      if (c->p.kernel == PHSPH_KERNEL_CUBIC) {
+        if (th) GL(PHSPH_KERNEL_CUBIC, true); else GL(PHSPH_KERNEL_CUBIC, false);
+      } else {
+        if (th) GL(PHSPH_KERNEL_QUINTIC, true); else GL(PHSPH_KERNEL_QUINTIC, false);
+      }
 #undef GL
+    }
+    PH_HIPCHECK(c, hipGetLastError());
+    int32_t nmax = 0;
+    PH_HIPCHECK(c, hipMemcpyAsync(fl2, c->counters.p, sizeof(fl2), hipMemcpyDeviceToHost, s));
+    PH_HIPCHECK(c, hipMemcpyAsync(red, c->dtred.p, sizeof(red), hipMemcpyDeviceToHost, s));
+    PH_HIPCHECK(c, hipMemcpyAsync(&nmax, c->counters.p + 5, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    PH_HIPCHECK(c, hipStreamSynchronize(s));
+    if (!(fl2[0] & GFL_NEAR)) break;
+    if (attempt == 1) PH_RET(c, PHSPH_ERR_CAPACITY, "force: neighbour list capacity exceeded");
+    capn = nmax;
+    if (list_fits(c, (size_t)list_cap(capn) * (size_t)nact)) c->caps.gnear = list_cap(capn);
   }
-  PH_HIPCHECK(c, hipGetLastError());
-  int32_t fl2[2] = {0, 0};
-  unsigned long long red[3];
-  PH_HIPCHECK(c, hipMemcpyAsync(fl2, c->counters.p, sizeof(fl2), hipMemcpyDeviceToHost, s));
-  PH_HIPCHECK(c, hipMemcpyAsync(red, c->dtred.p, sizeof(red), hipMemcpyDeviceToHost, s));
-  PH_HIPCHECK(c, hipStreamSynchronize(s));
   if (fl2[0] & GFL_DTC) PH_RET(c, PHSPH_ERR_INPUT, "force: invalid dtc");
   if (a.ibin) c->nbinmax_new = fl2[1];
   double dtc, dtf;

@@ -3,7 +3,9 @@ candidate leaves, density and force neighbour lists and gravity near-field
 lists overflow.  With the default list budget the library doubles its
 uniform capacities; with a 1 KiB budget it takes its exact-size paths
 (per-group candidate offsets, per-group force list capacities, per-leaf
-near-field offsets; density retries with longer lists in both).  Either way
+near-field offsets; density retries with longer lists in both), and the
+self-gravity force pass's neighbour lists overflow and run again at the
+longest list.  Either way
 it must agree with the oracle to the parity tolerances: neighbour and
 iteration counts bit-exact, FP64 results within 1e-12 of the rms."""
 import os
@@ -17,7 +19,8 @@ from tests.test_gpu_gravity import _run_gpu as run_gpu_grav, _run_oracle as run_
 
 pytestmark = pytest.mark.gpu
 
-SPECS = {"doubling": "gcand=4,dlist=16,flist=16,p2p=2", "exact": "gcand=4,dlist=16,flist=16,p2p=2,budget_kib=1"}
+SPECS = {"doubling": "gcand=4,dlist=16,flist=16,p2p=2,gnear=4",
+         "exact": "gcand=4,dlist=16,flist=16,p2p=2,gnear=4,budget_kib=1"}
 
 
 def _with_caps(spec, fn, *a):
