@@ -531,19 +531,21 @@ k_force_mixed(DevConst dc, const int32_t *__restrict__ act, int nact, const int3
   int npair = 0;  // interacting pairs (entries that pass the neighbour test)
   // entry kk+1 and its position record are read while entry kk is summed
   auto slot = [&](int kk) { return kk < nl ? kk : capb - 1 - (kk - nl); };
-  uint32_t en = 0u;
+  uint32_t en = 0u, en2 = 0u;
   double4 pn = make_double4(0., 0., 0., 0.);
   if (ntot > 0) {
     en = nlist[list_at_g(loff, nact, t, slot(0))];
     pn = fpos[cand_node(en)];
   }
+  if (ntot > 1) en2 = nlist[list_at_g(loff, nact, t, slot(1))];
   for (int kk = 0; kk < ntot; kk++) {
     const uint32_t e = en;
     const int j = cand_node(e);
     const double4 pj = pn;
     if (kk + 1 < ntot) {
-      en = nlist[list_at_g(loff, nact, t, slot(kk + 1))];
+      en = en2;
       pn = fpos[cand_node(en)];
+      if (kk + 2 < ntot) en2 = nlist[list_at_g(loff, nact, t, slot(kk + 2))];
     }
     double dx, dy, dz;
     sep_of(dc, cand_code(e), xi, yi, zi, pj.x, pj.y, pj.z, dx, dy, dz);
