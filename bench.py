@@ -3,10 +3,16 @@
 One step = densityiterate(icall=1) + cons2prim + force over every particle
 of the workload, on arrays resident in HBM.  Before each step the smoothing
 lengths are reset to their initial values (one device-to-device copy of
-xyzh), so every step does the same work; the kd-tree is built once before
-the timed region and its build time is reported separately.
+xyzh), so every step does the same work; the kd-tree is built outside the
+timed region and its build time is reported separately, with the combined
+tree + density + force figure.
 
-Prints one JSON line (see the repository's measurement notes for fields).
+The default workload is the 8,388,608-particle adiabatic hydro box (cubic
+kernel, hfact 1.2) in FP64: `value`.  The same box then runs in mixed
+precision (the ratio and the per-quantity errors of mixed against FP64),
+and the 1,097,505-particle 3D shock tube runs in both precisions.
+
+Prints one JSON line.
 """
 import argparse
 import json
@@ -38,9 +44,11 @@ WORKLOADS = {
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="shock_tube_1M", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="hydro_box_8M", choices=sorted(WORKLOADS))
+    ap.add_argument("--quick", action="store_true",
+                    help="the named workload and precision only (no mixed/FP64 pair, no 1M shock tube)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -64,15 +72,141 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design note, "
                                    "not an executable path", "n_gpus": args.gpus}))
         return 2
-    from phantom_amd import device, sph
-
     if args.mode == "step":
         return step_bench(args)
-    setup = WORKLOADS[args.workload]()
+    main_run = run_derivs(args.workload, args.precision, args.steps, args.warmup, keep=not args.quick)
+    p = main_run["params"]
+    n = main_run["n"]
+    kt = main_run["kt"]
+
+    def kmean(name):
+        return float(np.mean(kt[name])) * 1e-3 if name in kt else 0.0
+
+    # algorithmic flops per launch of each pair kernel (work_model); the
+    # list builders and walks do search work only and are not credited
+    flops = {
+        "density_iter": work_model.density_iter_flops(main_run["nneigh"], main_run["nits"]),
+        "density_sums": work_model.density_sums_flops(main_run["nneigh"], mhd=bool(p.mhd)),
+        "force": work_model.force_flops(main_run["npairs"], mhd=bool(p.mhd), dust=bool(p.use_dustfrac)),
+        "force_list": 0.0, "density_list": 0.0,
+    }
+    step_kernels = [nm for nm in kt if nm not in ("force_prep", "dens_gather")]
+    dominant = max(step_kernels, key=lambda nm: kmean(nm) * len(kt[nm]))
+    dens_prec, force_prec = main_run["precision_used"]
+    kprec = force_prec if dominant.startswith("force") else dens_prec
+    peak = work_model.PEAK_FP64_TFLOPS if kprec == abi.PREC_FP64 else work_model.PEAK_FP32_TFLOPS
+    t_dom = kmean(dominant)
+    ach = flops.get(dominant, 0.0) / t_dom / 1e12 if t_dom > 0 else 0.0
+    roofline = {"bound": "valu", "achieved": round(ach, 4), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(ach / peak, 6), "traffic": measured_traffic(dominant, args.workload),
+                "kernel": dominant, "launch_ms": round(1e3 * t_dom, 4),
+                "note": "vector ALU pair arithmetic (no dense contraction, no MFMA); peak is MI355X vector "
+                        + ("FP64" if peak == work_model.PEAK_FP64_TFLOPS else "FP32")
+                        + "; algorithmic flops per work_model (interacting pairs only: rejected trials and the "
+                          "neighbour search are not credited); launch time from HIP events on the library stream"}
+    kernel_tflops = {nm: round(flops[nm] / kmean(nm) / 1e12, 4) for nm in flops if nm in kt and kmean(nm) > 0}
+
+    extra = {}
+    if not args.quick:
+        # the same workload in the other precision: speed ratio and the error
+        # of mixed precision against the FP64 path (which equals the oracle
+        # to 1e-12 in the parity tests)
+        other = "mixed" if args.precision == "fp64" else "fp64"
+        o_run = run_derivs(args.workload, other, args.steps, args.warmup, keep=True)
+        fp, mx = (main_run, o_run) if args.precision == "fp64" else (o_run, main_run)
+        extra["mixed"] = precision_pair(fp, mx)
+        # the 1M shock tube, the configuration the metric's 1M half is quoted on
+        if args.workload != "shock_tube_1M":
+            t64 = run_derivs("shock_tube_1M", "fp64", args.steps, args.warmup, keep=True)
+            t32 = run_derivs("shock_tube_1M", "mixed", args.steps, args.warmup, keep=True)
+            extra["shock_tube_1M"] = dict(n_particles=t64["n"], fp64=summary(t64), **precision_pair(t64, t32))
+    tree_ms = main_run["tree_ms"]
+    combined_ms = tree_ms + main_run["ms_step"]
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(main_run["setup"], args)
+
+    out = {
+        "metric": "density+force particle updates/s",
+        "value": main_run["value"],
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_run["ms_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-pair/f64-accum",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "n_particles": n, "kernel": "cubic" if p.kernel == 0 else "quintic",
+                   "precision": args.precision, "precision_used": {"density": dens_prec, "force": force_prec},
+                   "mhd": bool(p.mhd), "dust": bool(p.use_dustfrac), "start_h": main_run["start_h"],
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "tree": {"build_ms": round(tree_ms, 4), "first_call_ms": round(main_run["tree_first_ms"], 4),
+                 "kernels_ms": {nm: round(ms, 4) for nm, ms in main_run["tree_kernels"].items()},
+                 "combined_ms_per_step": round(combined_ms, 4),
+                 "combined_updates_per_s": n / (combined_ms * 1e-3)},
+        "breakdown_ms": dict(main_run["breakdown"], kernels={nm: round(1e3 * kmean(nm), 4) for nm in sorted(kt)}),
+        "kernel_tflops": kernel_tflops,
+        "neighbours": {"dens_mean": main_run["stats"]["actualmean"], "dens_max": main_run["stats"]["maxactual"],
+                       "force_pairs_per_particle": main_run["npairs"] / n, "nrhocalc": main_run["stats"]["nrhocalc"]},
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def summary(r):
+    return {"value": r["value"], "ms_per_step": r["ms_step"], "tree_build_ms": round(r["tree_ms"], 4),
+            "precision_used": {"density": r["precision_used"][0], "force": r["precision_used"][1]}}
+
+
+def precision_pair(fp, mx):
+    """Mixed against FP64 on the same particles: updates/s of each, their
+    ratio and the error of mixed precision in rho (from h through
+    rho = m (hfact/h)^3) and in dv/dt.  rho: max over particles of the
+    relative error.  dv/dt: max over particles and components of
+    |a_mixed - a_fp64| / rms|a_fp64| (per-particle relative errors are not
+    meaningful where the acceleration passes through zero), plus the
+    99.9th percentile of the per-particle relative error |da|/|a|."""
+    out = {"mixed": summary(mx), "fp64_value": fp["value"], "mixed_value": mx["value"],
+           "mixed_over_fp64": mx["value"] / fp["value"]}
+    if fp.get("h") is not None and mx.get("h") is not None:
+        h64, h32 = fp["h"], mx["h"]
+        rho64, rho32 = (1.0 / h64) ** 3, (1.0 / h32) ** 3
+        a64, a32 = fp["f"][:, :3], mx["f"][:, :3]
+        rms = float(np.sqrt(np.mean(np.sum(a64 * a64, axis=1))))
+        da = np.sqrt(np.sum((a32 - a64) ** 2, axis=1))
+        am = np.sqrt(np.sum(a64 * a64, axis=1))
+        rel = da / np.where(am > 0, am, 1.0)
+        out["max_rel_err_rho"] = float(np.max(np.abs(rho32 - rho64) / rho64))
+        out["max_err_dvdt_over_rms"] = float(np.max(np.abs(a32 - a64)) / rms) if rms > 0 else 0.0
+        out["p999_rel_err_dvdt"] = float(np.quantile(rel, 0.999))
+        if fp["f"].shape[1] >= 4:
+            u64, u32 = fp["f"][:, 3], mx["f"][:, 3]
+            urms = float(np.sqrt(np.mean(u64 * u64)))
+            out["max_err_dudt_over_rms"] = float(np.max(np.abs(u32 - u64)) / urms) if urms > 0 else 0.0
+        out["nneigh_identical"] = bool(np.array_equal(fp["nneigh"], mx["nneigh"]))
+    return out
+
+
+def run_derivs(workload, precision, steps, warmup, keep=False):
+    """Time `steps` density + cons2prim + force steps of one workload in one
+    precision on HBM-resident arrays (after `warmup` untimed steps).  Every
+    step starts from the same positions and h (one device copy before it).
+    The tree is built outside the timed region: its time is that of the
+    second build on the context (buffers allocated, as in a run)."""
+    from phantom_amd import device, sph
+
+    setup = WORKLOADS[workload]()
     p = setup["params"]
-    p.precision = abi.PREC_MIXED if args.precision == "mixed" else abi.PREC_FP64
+    p.precision = abi.PREC_MIXED if precision == "mixed" else abi.PREC_FP64
     n = setup["xyzh"].shape[0]
-    log(f"workload {args.workload} n={n} precision={args.precision}")
+    log(f"workload {workload} n={n} precision={precision}")
     D = device.DeviceArray
     xyzh0 = D.from_numpy(setup["xyzh"])
     xyzh = D.from_numpy(setup["xyzh"])
@@ -96,7 +230,7 @@ def main():
     ctx = sph.Context(p)
     t0 = time.perf_counter()
     ctx.maketree(xyzh, leaf=False)
-    tree_ms = 1e3 * (time.perf_counter() - t0)
+    tree_first_ms = 1e3 * (time.perf_counter() - t0)
     start_h = "setup"
     if p.gravity:
         # the polytrope's analytic density falls to zero at the surface, so
@@ -108,14 +242,19 @@ def main():
         setup["xyzh"] = xyzh.numpy()
         start_h = "converged"
     xyzh0.copy_from(xyzh)  # wrapped positions, h at the start of every step
-    # a second build on the same context: buffers exist, as in a time step
-    t0 = time.perf_counter()
-    ctx.maketree(xyzh, leaf=False)
-    tree2_ms = 1e3 * (time.perf_counter() - t0)
+    # further builds on the same context: buffers exist, as in a time step
+    tms = []
     tree_kernels = {}
-    for nm, ms in ctx.timings():
-        tree_kernels[nm] = tree_kernels.get(nm, 0.0) + ms
-    log(f"tree built in {tree_ms:.2f} ms (first call), {tree2_ms:.2f} ms (second)")
+    for _ in range(3):
+        device.synchronize()
+        t0 = time.perf_counter()
+        ctx.maketree(xyzh, leaf=False)
+        tms.append(1e3 * (time.perf_counter() - t0))
+        tree_kernels = {}
+        for nm, ms in ctx.timings():
+            tree_kernels[nm] = tree_kernels.get(nm, 0.0) + ms
+    tree_ms = float(np.median(tms))
+    log(f"tree built in {tree_first_ms:.2f} ms (first call), {tree_ms:.2f} ms (median of 3 later builds)")
 
     kt = {}  # per-launch kernel times (HIP events on the library's stream), by name
     phase = {"density": 0.0, "cons2prim": 0.0, "force": 0.0}
@@ -145,87 +284,37 @@ def main():
             phase["cons2prim"] += c - b
             phase["force"] += d - c
 
-    for k in range(args.warmup):
+    for k in range(warmup):
         step(False)
-        log(f"warmup step {k + 1}/{args.warmup} done")
+        log(f"warmup step {k + 1}/{warmup} done")
     device.synchronize()
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         step(True)
     device.synchronize()
     elapsed = time.perf_counter() - t0
-    log(f"timed {args.steps} steps in {elapsed:.3f} s")
-
-    stats = ctx.get_neighbour_stats()
-    nneigh, nits = ctx.get_particle_stats(n)
-    npairs = ctx.get_force_stats()["npairs"]
-    ms_step = 1e3 * elapsed / args.steps
-    updates = n * args.steps
-    value = updates / elapsed
-
-    def kmean(name):
-        return float(np.mean(kt[name])) * 1e-3 if name in kt else 0.0
-
-    # per step: the kernels of one densityiterate and one force call
-    t_dens = sum(float(np.sum(kt.get(nm, []))) for nm in ("density_iter", "density_sums")) * 1e-3 / args.steps
-    t_force = sum(float(np.sum(kt.get(nm, []))) for nm in ("force_list", "force")) * 1e-3 / args.steps
-
-    # algorithmic flops per launch of each pair kernel (work_model); the
-    # list builders and walks do search work only and are not credited
-    flops = {
-        "density_iter": work_model.density_iter_flops(nneigh, nits),
-        "density_sums": work_model.density_sums_flops(nneigh, mhd=bool(p.mhd)),
-        "force": work_model.force_flops(npairs, mhd=bool(p.mhd), dust=bool(p.use_dustfrac)),
-        "force_list": 0.0, "density_list": 0.0,
-    }
-    step_kernels = [nm for nm in kt if nm not in ("force_prep", "dens_gather")]
-    dominant = max(step_kernels, key=lambda nm: kmean(nm) * len(kt[nm]))
-    peak = work_model.PEAK_FP64_TFLOPS if args.precision == "fp64" else work_model.PEAK_FP32_TFLOPS
-    t_dom = kmean(dominant)
-    ach = flops.get(dominant, 0.0) / t_dom / 1e12 if t_dom > 0 else 0.0
-    roofline = {"bound": "mfma", "achieved": round(ach, 4), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(ach / peak, 6), "traffic": measured_traffic(dominant, args.workload),
-                "kernel": dominant, "launch_ms": round(1e3 * t_dom, 4),
-                "note": "VALU FP64 pair arithmetic (no dense contraction); peak is MI355X FP64 "
-                        "(FP32 in mixed mode); algorithmic flops per work_model, rejected trials "
-                        "and neighbour search not counted"}
-    kernel_tflops = {nm: round(flops[nm] / kmean(nm) / 1e12, 4) for nm in flops if nm in kt and kmean(nm) > 0}
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(setup, args)
-
-    out = {
-        "metric": "density+force particle updates/s",
-        "value": value,
-        "unit": "particle-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32-pair/f64-accum",
-        "data": "synthetic",
-        "config": {"workload": args.workload, "n_particles": n, "kernel": "cubic" if p.kernel == 0 else "quintic",
-                   "precision": args.precision, "mhd": bool(p.mhd), "dust": bool(p.use_dustfrac),
-                   "start_h": start_h, "parallelism": "single GPU"},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "breakdown_ms": {"density_call": 1e3 * phase["density"] / args.steps,
-                         "cons2prim_call": 1e3 * phase["cons2prim"] / args.steps,
-                         "force_call": 1e3 * phase["force"] / args.steps,
-                         "density_kernels": 1e3 * t_dens, "force_kernels": 1e3 * t_force,
-                         "kernels": {nm: round(1e3 * kmean(nm), 4) for nm in sorted(kt)},
-                         "tree_build_first_call": tree_ms, "tree_build": tree2_ms},
-        "kernel_tflops": kernel_tflops,
-        "neighbours": {"dens_mean": stats["actualmean"], "dens_max": stats["maxactual"],
-                       "force_pairs_per_particle": npairs / n, "nrhocalc": stats["nrhocalc"]},
-        "tree_kernels_ms": {nm: round(ms, 4) for nm, ms in tree_kernels.items()},
-    }
-    print(json.dumps(out), flush=True)
-    return 0
+    log(f"timed {steps} steps in {elapsed:.3f} s ({n * steps / elapsed / 1e6:.1f} M updates/s)")
+    t_dens = sum(float(np.sum(kt.get(nm, []))) for nm in ("density_iter", "density_sums")) * 1e-3 / steps
+    t_force = sum(float(np.sum(kt.get(nm, []))) for nm in ("force_list", "force")) * 1e-3 / steps
+    out = dict(
+        n=n, params=p, setup=setup, start_h=start_h, kt=kt, value=n * steps / elapsed,
+        ms_step=1e3 * elapsed / steps, tree_ms=tree_ms, tree_first_ms=tree_first_ms, tree_kernels=tree_kernels,
+        stats=ctx.get_neighbour_stats(), npairs=ctx.get_force_stats()["npairs"],
+        precision_used=ctx.get_precision(),
+        breakdown={"density_call": 1e3 * phase["density"] / steps,
+                   "cons2prim_call": 1e3 * phase["cons2prim"] / steps,
+                   "force_call": 1e3 * phase["force"] / steps,
+                   "density_kernels": 1e3 * t_dens, "force_kernels": 1e3 * t_force})
+    out["nneigh"], out["nits"] = ctx.get_particle_stats(n)
+    if keep:
+        out["h"] = xyzh.numpy()[:, 3].copy()
+        out["f"] = fxyzu.numpy()
+    ctx.close()
+    for a in (xyzh0, xyzh, vxyzu, alphaind, Bevol, dustevol, divcurlv, divcurlB, gradh, dvdx, eos, dustfrac, fxyzu,
+              dBevol, divBsymm, ddustevol, deltav, tstop):
+        if a is not None:
+            a.free()
+    return out
 
 
 class _Recorder:
@@ -357,18 +446,46 @@ def calibrate_copy():
     return 0
 
 
+def host_cores():
+    """CPUs this process may run on (sched_getaffinity) and the machine's
+    physical cores and sockets as lscpu reports them."""
+    import subprocess
+    info = {"affinity_cpus": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count()}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        kv = {}
+        for line in txt.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        cps, sk = int(kv.get("Core(s) per socket", "0")), int(kv.get("Socket(s)", "0"))
+        info.update(model=kv.get("Model name"), sockets=sk, cores_per_socket=cps, physical_cores=cps * sk,
+                    threads_per_core=int(kv.get("Thread(s) per core", "0")))
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return info
+
+
 def cpu_baseline(setup, args):
     """The oracle (a C restatement of the reference algorithm, OpenMP over the
-    host cores) on the same particles: maketree once, then density + cons2prim
-    + force, one warm-up and the median of three calls."""
+    host cores) on the same particles: maketree once, then density +
+    cons2prim + force, one warm-up and the median of three calls.  Threads:
+    the process's OpenMP thread count (OMP_NUM_THREADS: the CPU share of one
+    GPU on a shared host), else every CPU of its affinity mask."""
     from oracle import oracle
 
-    threads = args.cpu_threads or min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
+    cores = host_cores()
+    env_threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = args.cpu_threads or env_threads or cores["affinity_cpus"]
+    threads = min(threads, cores["affinity_cpus"])
     os.environ["OMP_NUM_THREADS"] = str(threads)
     p = setup["params"]
+    p.precision = abi.PREC_FP64
     o = oracle.Oracle(p)
     xyzh0 = setup["xyzh"].copy()
+    t0 = time.perf_counter()
     o.maketree(xyzh0)
+    tree_s = time.perf_counter() - t0
     n = xyzh0.shape[0]
     times = []
     for rep in range(4):
@@ -381,11 +498,13 @@ def cpu_baseline(setup, args):
                 Bevol=setup.get("Bevol"), dustfrac=c["dustfrac"])
         if rep > 0:
             times.append(time.perf_counter() - t0)
+        log(f"cpu baseline call {rep + 1}/4: {time.perf_counter() - t0:.3f} s")
     dt = float(np.median(times))
     log(f"cpu baseline: median {dt:.3f} s per call on {threads} threads")
     return {"value": n / dt, "unit": "particle-updates/s", "cores": threads, "kind": "port",
             "sample": f"median of 3 density+cons2prim+force calls (after one warm-up) over all {n} particles "
-                      f"of the same workload, OpenMP with {threads} threads"}
+                      f"of the same workload, OpenMP with {threads} threads; maketree (serial) {tree_s:.2f} s",
+            "host": dict(cores, omp_num_threads_env=env_threads)}
 
 
 if __name__ == "__main__":

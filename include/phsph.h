@@ -400,6 +400,16 @@ int phsph_get_particle_stats(phsph_ctx *ctx, int32_t np, int32_t *nneigh,
 int phsph_get_force_stats(phsph_ctx *ctx, int64_t *npairs);
 
 /*
+ * Precision (PHSPH_PREC_*) the pair sums of the last phsph_densityiterate
+ * and phsph_force actually ran in; -1 before the first call.  With
+ * params.precision = PHSPH_PREC_MIXED the hydro sums run in mixed
+ * precision; the density sums with MHD, and the force pair terms with MHD,
+ * dust or self-gravity, have FP64 kernels only and report PHSPH_PREC_FP64.
+ * No reference counterpart (the reference is FP64 throughout).
+ */
+int phsph_get_precision(phsph_ctx *ctx, int32_t *dens_prec, int32_t *force_prec);
+
+/*
  * Kernel timing of the last call on the context's stream, measured with
  * HIP events around each launch: up to `cap` (name, milliseconds) entries
  * are written; returns the number of entries available.

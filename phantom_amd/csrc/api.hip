@@ -425,6 +425,7 @@ int phsph_force(phsph_ctx *c, int32_t icall, int32_t np, const double *xyzh, int
   Stager st(c);
   ForceIOArgs a;
   memset(&a, 0, sizeof(a));
+  a.np = np;
   double *t;
   float *tf;
   int8_t *ph;
@@ -793,6 +794,13 @@ int phsph_get_particle_stats(phsph_ctx *c, int32_t np, int32_t *nneigh, int32_t 
 int phsph_get_force_stats(phsph_ctx *c, int64_t *npairs) {
   if (!c) return PHSPH_ERR_INPUT;
   if (npairs) *npairs = c->force_npairs;
+  return PHSPH_OK;
+}
+
+int phsph_get_precision(phsph_ctx *c, int32_t *dens_prec, int32_t *force_prec) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (dens_prec) *dens_prec = c->dens_prec_used;
+  if (force_prec) *force_prec = c->force_prec_used;
   return PHSPH_OK;
 }
 

@@ -169,6 +169,9 @@ struct phsph_ctx {
   int maxtrial = -1, maxactual = -1;
   int64_t nrhocalc = -1, nactualtot = -1;
   int64_t force_npairs = -1;  // interacting pairs of the last force call
+  // precision the pair sums of the last densityiterate / force ran in
+  // (PHSPH_PREC_*; -1 before the first call)
+  int32_t dens_prec_used = -1, force_prec_used = -1;
 
   // block timesteps registered by phsph_set_bins (host or device pointers)
   int8_t *bins_ibin = nullptr, *bins_ibin_wake = nullptr;
@@ -184,6 +187,9 @@ struct phsph_ctx {
   DBuf<double> htraj, hconv;          // h per density iteration, converged h
   DBuf<double> fnode;                 // far-field expansion per leaf (20)
   DBuf<uint32_t> gnear;               // self-gravity force: neighbours per active particle
+  DBuf<int32_t> gnear_n;              // their count per active particle
+  DBuf<int64_t> gnear_off;            // per-group offsets into gnear (beyond the list budget)
+  DBuf<int8_t> ibin_save;             // ibin at force entry, restored before a re-run
   // node-by-node dual walk: destination nodes of a level (ping-pong), their
   // candidate source lists (shared by siblings) and incoming expansions
   DBuf<int32_t> gw_node[2], gw_ccnt[2];
@@ -297,6 +303,7 @@ struct ForceIOArgs {  // device pointers in the caller's layout
   // block timesteps (null ibin: global timestep)
   int8_t *ibin; const int8_t *ibin_old; int8_t *ibin_wake;
   int nbinmax, istepfrac, ibinnow;
+  int np;  // length of the caller's particle arrays
 };
 int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc, double *dtf);
 int leaf_hmax_run(phsph_ctx *c, const double *d_h, double *d_hmaxn);

@@ -1050,6 +1050,7 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   PH_HIPCHECK(c, c->tmpi.ensure(n));
   // mixed precision covers the hydro sums; with MHD the pass stays FP64
   const bool mixed = c->p.precision == PHSPH_PREC_MIXED && !mhd;
+  c->dens_prec_used = mixed ? PHSPH_PREC_MIXED : PHSPH_PREC_FP64;
   {
     LaunchTimer lt(c, "dens_gather");
     hipLaunchKernelGGL(k_gather_dens, dim3(nblocks(n, 256)), dim3(256), 0, s, c->dc, n, c->iparts.p, c->tx.p,

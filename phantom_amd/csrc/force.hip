@@ -161,15 +161,6 @@ k_flist(DevConst dc, const int32_t *__restrict__ act, int nact, int ngroups, con
   if (nl > capl) atomicOr(flags, FFL_LIST);
 }
 
-// longest list of each group of 64 slots (t_nl over the group's live slots)
-__global__ void k_group_max(const int32_t *__restrict__ t_nl, int nact, int ngroups, int32_t *__restrict__ gmax) {
-  const int g = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-  const int t = g * GROUP + (threadIdx.x & 63);
-  int v = (g < ngroups && t < nact) ? t_nl[t] : 0;
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-  if (g < ngroups && (threadIdx.x & 63) == 0) gmax[g] = v;
-}
-
 // Force, part 2: the pair terms (force.F90:1227-2056) over each particle's
 // list and finish_cell_and_store_results.  One thread per active particle.
 template <int K, bool MHD, bool DUST, bool THERMAL>
@@ -897,6 +888,7 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
   // mixed precision covers the hydro pair terms; MHD, dust and self-gravity
   // keep their FP64 pair kernels
   const bool mixed = c->p.precision == PHSPH_PREC_MIXED && !mhd && !dust && !c->p.gravity;
+  c->force_prec_used = mixed ? PHSPH_PREC_MIXED : PHSPH_PREC_FP64;
   PH_HIPCHECK(c, c->fpos.ensure(n));
   if (mixed) {
     PH_HIPCHECK(c, c->frec32.ensure((size_t)n * sizeof(FRec32)));

@@ -597,8 +597,10 @@ struct GravArgs {
   int capp;
   const float *gsoft;      // gradh(2) in tree order
   float *poten;            // caller order (may be null)
-  uint32_t *nbuf;          // neighbours per active particle (list_at layout)
-  int capn;                // its capacity per particle
+  uint32_t *nbuf;          // neighbours per active particle (list_at_g layout)
+  int capn;                // its uniform capacity per particle (noff == null)
+  const int64_t *noff;     // per-group offsets into nbuf, or null
+  int32_t *nnb;            // neighbours met per active particle
   int32_t *nmax;           // longest neighbour list met (after an overflow)
 };
 
@@ -643,6 +645,7 @@ k_force_grav(DevConst dc, const int32_t *__restrict__ act, int nact, const int32
     const size_t pbase = G.poff ? (size_t)G.poff[l] : (size_t)l * G.capp;
     const int ne = min(G.np2p[l], G.poff ? (int)(G.poff[l + 1] - G.poff[l]) : G.capp);
     int nn = 0;  // neighbours met in the first pass
+    const int capl = list_cap_g(G.noff, t, G.capn);
     // first pass, over the near-field leaves: the 1/r^2 terms of the trial
     // particles outside both kernels, and the list of the neighbours.  List
     // entries are read two leaves ahead and leaf ranges one ahead, and each
@@ -690,18 +693,19 @@ k_force_grav(DevConst dc, const int32_t *__restrict__ act, int nact, const int32
         }
         // a neighbour: its pair terms are summed in the second pass, where
         // every lane of the wavefront has a neighbour to sum
-        if (nn < G.capn) G.nbuf[list_at(nact, t, nn)] = cand_pack(j, code);
+        if (nn < capl) G.nbuf[list_at_g(G.noff, nact, t, nn)] = cand_pack(j, code);
         nn++;
       }
     }
-    if (nn > G.capn) {
+    G.nnb[t] = nn;
+    if (nn > capl) {
       atomicOr(flags, GFL_NEAR);
       atomicMax(G.nmax, nn);
     }
     // second pass: the neighbours (force.F90:1286-1990)
-    const int nk = min(nn, G.capn);
+    const int nk = min(nn, capl);
     for (int kk = 0; kk < nk; kk++) {
-      const uint32_t ek = G.nbuf[list_at(nact, t, kk)];
+      const uint32_t ek = G.nbuf[list_at_g(G.noff, nact, t, kk)];
       const int j = cand_node(ek), code = cand_code(ek);
       const FRec &Rj = rec[j];
       double dx, dy, dz;
@@ -926,16 +930,27 @@ int grav_force(phsph_ctx *c, int icall, const ForceIOArgs &a, const FRec *rec, c
   int32_t fl2[2] = {0, 0};
   unsigned long long red[3];
   // the second pass's neighbour lists: a capacity per particle, kept for
-  // later calls; after an overflow the call runs again at the longest list
+  // later calls; after an overflow the call runs again at the longest list,
+  // or with per-group capacities when that does not fit in the list budget.
+  // The kernel moves ibin to the new bin in place, so a re-run starts from
+  // the bins the call was given.
   int capn = c->caps.gnear;
+  const int64_t *noff = nullptr;
+  PH_HIPCHECK(c, c->gnear_n.ensure((size_t)(nact > 0 ? nact : 1)));
+  if (a.ibin && a.np > 0) {
+    PH_HIPCHECK(c, c->ibin_save.ensure((size_t)a.np));
+    PH_HIPCHECK(c, hipMemcpyAsync(c->ibin_save.p, a.ibin, (size_t)a.np, hipMemcpyDeviceToDevice, s));
+  }
   for (int attempt = 0; attempt < 2; attempt++) {
-    PH_HIPCHECK(c, c->gnear.ensure((size_t)list_cap(capn) * (size_t)(nact > 0 ? nact : 1)));
+    if (attempt > 0 && a.ibin && a.np > 0)
+      PH_HIPCHECK(c, hipMemcpyAsync(a.ibin, c->ibin_save.p, (size_t)a.np, hipMemcpyDeviceToDevice, s));
+    if (!noff) PH_HIPCHECK(c, c->gnear.ensure((size_t)list_cap(capn) * (size_t)(nact > 0 ? nact : 1)));
     PH_HIPCHECK(c, dev_set(c, c->dtred.p, init, sizeof(init)));
     const int32_t zero3[3] = {0, 0, 0};
     PH_HIPCHECK(c, dev_set(c, c->counters.p, zero3, 2 * sizeof(int32_t)));
     PH_HIPCHECK(c, dev_set(c, c->counters.p + 5, zero3, sizeof(int32_t)));
     GravArgs G{c->fnode.p, c->p2p.p, c->np2p.p, poff, c->caps.p2p, gsoft, c->poten_out,
-               c->gnear.p, list_cap(capn), c->counters.p + 5};
+               c->gnear.p, list_cap(capn), noff, c->gnear_n.p, c->counters.p + 5};
     if (nact > 0) {
       LaunchTimer lt(c, "force");
       const dim3 gr(nblocks(nact, 128)), b(128);
@@ -960,7 +975,25 @@ int grav_force(phsph_ctx *c, int icall, const ForceIOArgs &a, const FRec *rec, c
     if (!(fl2[0] & GFL_NEAR)) break;
     if (attempt == 1) PH_RET(c, PHSPH_ERR_CAPACITY, "force: neighbour list capacity exceeded");
     capn = nmax;
-    if (list_fits(c, (size_t)list_cap(capn) * (size_t)nact)) c->caps.gnear = list_cap(capn);
+    if (list_fits(c, (size_t)list_cap(capn) * (size_t)nact)) {
+      c->caps.gnear = list_cap(capn);
+      continue;
+    }
+    // one particle with many neighbours would size every list: give each
+    // group of 64 the longest list among its own particles instead
+    const int ng = (nact + GROUP - 1) / GROUP;
+    PH_HIPCHECK(c, c->gmax.ensure((size_t)ng));
+    PH_HIPCHECK(c, c->gnear_off.ensure((size_t)ng + 1));
+    hipLaunchKernelGGL(k_group_max, dim3(nblocks((size_t)ng * 64, 256)), dim3(256), 0, s, c->gnear_n.p, nact, ng,
+                       c->gmax.p);
+    hipLaunchKernelGGL(k_scan_offsets<int32_t>, dim3(1), dim3(1024), 0, s, c->gmax.p, ng, (int64_t)GROUP, LSEG,
+                       c->gnear_off.p);
+    PH_HIPCHECK(c, hipGetLastError());
+    int64_t total = 0;
+    PH_HIPCHECK(c, hipMemcpyAsync(&total, c->gnear_off.p + ng, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PH_HIPCHECK(c, hipStreamSynchronize(s));
+    PH_HIPCHECK(c, c->gnear.ensure((size_t)total + 1));
+    noff = c->gnear_off.p;
   }
   if (fl2[0] & GFL_DTC) PH_RET(c, PHSPH_ERR_INPUT, "force: invalid dtc");
   if (a.ibin) c->nbinmax_new = fl2[1];

@@ -189,6 +189,16 @@ __global__ void __launch_bounds__(1024) k_scan_offsets(const T *__restrict__ cnt
 
 static inline int list_cap(int cap) { return (cap + LSEG - 1) / LSEG * LSEG; }
 
+// longest list of each group of 64 slots (t_nl over the group's live slots)
+static __global__ __attribute__((unused)) void k_group_max(const int32_t *__restrict__ t_nl, int nact, int ngroups,
+                                   int32_t *__restrict__ gmax) {
+  const int g = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int t = g * GROUP + (threadIdx.x & 63);
+  int v = (g < ngroups && t < nact) ? t_nl[t] : 0;
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  if (g < ngroups && (threadIdx.x & 63) == 0) gmax[g] = v;
+}
+
 // Neighbour-list building over a group's candidate leaves, CL leaves at a
 // time through LDS.  For each chunk the wavefront loads the leaves' node
 // data and their particles' records with parallel loads into LDS, each lane

@@ -174,8 +174,9 @@ def step(ctx, s, t, dtsph, ufloor=0.0, tolv=1.e-2, maxits=MAXITS, derivs_fn=None
         nvfloorc += r["nvfloor"]
         converged, dterr_i, _ = check_velocity_error(r["errmax"], r["v2sum"], r["np"], its, tolv, dtsph,
                                                      s.dtcourant, s.dtforce, ind)
-        if its == 1:
-            dterr = dterr_i
+        # timestep:dterr is overwritten by every call (step_leapfrog.f90:807),
+        # so after an iterated step it is huge and does not limit dt
+        dterr = dterr_i
         if not converged:
             ctx.step_reset(dtsph, s.vxyzu, s.fxyzu, s.vpred, Bevol=s.Bevol, dBevol=s.dBevol, Bpred=s.Bpred,
                            dustevol=s.dustevol, ddustevol=s.ddustevol, dustpred=s.dustpred, iphase=s.iphase)

@@ -332,3 +332,40 @@ def polytrope(n=8000, nindex=1.5, seed=42, kernel=abi.KERNEL_CUBIC, jitter=0.1, 
     alphaind = np.zeros((npart, 3), dtype=np.float32)
     return dict(xyzh=xyzh, vxyzu=vxyzu, alphaind=alphaind, params=p,
                 info=dict(workload="polytrope", n=npart, nindex=nindex, seed=seed, K=K, rho_c=rho_c))
+
+
+def two_stars(n_each=10000, seed=11, separation=20.0, theta=0.5, kernel=abi.KERNEL_CUBIC):
+    """The two-star configuration of Phantom's symmetric-FMM momentum test
+    (test_gravity.f90:634-736): two uniform random spheres of radius 1, the
+    second centred at (separation, 0, 0), n_each particles each, total mass
+    1 in equal masses, hfact 1.2, zero thermal energy (ideal gas with
+    u = 0, so pure gravity), open boundaries, tree_accuracy theta.  The
+    initial h is hfact times the mean interparticle spacing in a sphere."""
+    rng = np.random.default_rng(seed)
+    pts = []
+    for x0 in (0.0, separation):
+        r = rng.uniform(0.0, 1.0, n_each) ** (1.0 / 3.0)
+        mu = rng.uniform(-1.0, 1.0, n_each)
+        ph = rng.uniform(0.0, 2.0 * math.pi, n_each)
+        s = np.sqrt(1.0 - mu * mu)
+        pts.append(np.stack([x0 + r * s * np.cos(ph), r * s * np.sin(ph), r * mu], axis=1))
+    pos = np.concatenate(pts)
+    n = pos.shape[0]
+    p = abi.default_params(kernel)
+    p.hfact = 1.2
+    p.periodic = 0
+    p.gravity = 1
+    p.tree_accuracy = theta
+    p.ieos = 2
+    p.gamma = 5.0 / 3.0
+    p.maxvxyzu = 4
+    p.massoftype_gas = 1.0 / n
+    p.maxp = n
+    p.xmin, p.xmax, p.ymin, p.ymax, p.zmin, p.zmax = -2.0, separation + 2.0, -2.0, 2.0, -2.0, 2.0
+    xyzh = np.empty((n, 4))
+    xyzh[:, :3] = pos
+    xyzh[:, 3] = p.hfact * (4.0 / 3.0 * math.pi / n_each) ** (1.0 / 3.0)
+    vxyzu = np.zeros((n, 4))
+    alphaind = np.zeros((n, 3), dtype=np.float32)
+    return dict(xyzh=xyzh, vxyzu=vxyzu, alphaind=alphaind, params=p,
+                info=dict(workload="two_stars", n=n, seed=seed, separation=separation))

@@ -35,7 +35,7 @@ EXPORTED = [
     "phsph_get_neighbour_stats", "phsph_get_particle_stats", "phsph_get_force_stats", "phsph_get_timings",
     "phsph_set_bins", "phsph_get_nbinmax", "phsph_set_active_particles", "phsph_set_poten",
     "phsph_get_quads", "phsph_init_step", "phsph_step_predictor", "phsph_substep_sph", "phsph_step_predict_sph",
-    "phsph_step_corrector", "phsph_step_reset",
+    "phsph_step_corrector", "phsph_step_reset", "phsph_get_precision",
 ]
 
 _lib = None
@@ -74,6 +74,7 @@ def load_library(path=LIB_PATH):
     L.phsph_get_neighbour_stats.argtypes = [vp, P(dbl), P(dbl), P(i32), P(i32), P(i64), P(i64)]
     L.phsph_get_particle_stats.argtypes = [vp, i32, vp, vp]
     L.phsph_get_force_stats.argtypes = [vp, P(i64)]
+    L.phsph_get_precision.argtypes = [vp, P(i32), P(i32)]
     L.phsph_get_timings.argtypes = [vp, P(C.c_char_p), P(dbl), i32]
     L.phsph_set_bins.argtypes = [vp, vp, vp, vp, i32, i32, i32]
     L.phsph_get_nbinmax.argtypes = [vp, P(i32)]
@@ -428,6 +429,13 @@ class Context:
         self._check(self.lib.phsph_step_reset(
             self._h, vxyzu.shape[0], dtsph, pv, ldv, pf, ldf, _ptr(vpred)[0], _ptr(Bevol)[0], _ptr(dBevol)[0],
             _ptr(Bpred)[0], _ptr(dustevol)[0], _ptr(ddustevol)[0], _ptr(dustpred)[0], _ptr(iphase)[0]))
+
+    def get_precision(self):
+        """(density, force) precision the last calls ran in (abi.PREC_*,
+        -1 before the first call)."""
+        d, f = C.c_int32(), C.c_int32()
+        self._check(self.lib.phsph_get_precision(self._h, C.byref(d), C.byref(f)))
+        return d.value, f.value
 
     def get_force_stats(self):
         n = C.c_int64()

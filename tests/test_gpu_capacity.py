@@ -15,7 +15,7 @@ import pytest
 
 from phantom_amd import setups
 from tests import parity
-from tests.test_gpu_gravity import _run_gpu as run_gpu_grav, _run_oracle as run_oracle_grav
+from tests.test_gpu_gravity import _moving, _run_gpu as run_gpu_grav, _run_oracle as run_oracle_grav
 
 pytestmark = pytest.mark.gpu
 
@@ -76,3 +76,22 @@ def test_tiny_capacities_gravity(mode, gpu):
     pg = np.asarray(g["force"]["poten"], np.float64)
     po = np.asarray(o["force"]["poten"], np.float64)
     assert parity.rel_rms(pg, po) <= 2e-7
+
+
+@pytest.mark.parametrize("mode", list(SPECS))
+def test_tiny_capacities_gravity_bins(mode, gpu):
+    """The self-gravity force pass runs again after its neighbour lists
+    overflow; with block timesteps that run must start from the bins the
+    call was given (the kernel moves ibin in place), so ibin, ibin_wake and
+    nbinmax equal the oracle's.  With the 1 KiB budget the re-run takes
+    per-group list capacities."""
+    setup = setups.with_bins(_moving(setups.polytrope(n=3000, theta=0.5)), nbinmax=3, istepfrac=2, dtmax=0.05)
+    print(f"\n[case polytrope bins] n={setup['xyzh'].shape[0]} caps {SPECS[mode]}", flush=True)
+    g = _with_caps(SPECS[mode], run_gpu_grav, setup)
+    o = run_oracle_grav(setup)
+    np.testing.assert_array_equal(g["nneigh"], o["dens"]["nneigh"])
+    for c in range(4):
+        assert parity.rel_rms(g["force"]["fxyzu"][:, c], o["force"]["fxyzu"][:, c]) <= 1e-12, c
+    np.testing.assert_array_equal(g["force"]["ibin"], o["force"]["ibin"])
+    np.testing.assert_array_equal(g["force"]["ibin_wake"], o["force"]["ibin_wake"])
+    assert g["force"]["nbinmax"] == o["force"]["nbinmax"]

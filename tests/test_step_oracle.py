@@ -228,3 +228,47 @@ def test_wake_moves_to_the_bin_half_step():
     # timei is a multiple of every bin's step: twas is half a step ahead
     for k, b in zip(range(10, 14), [2, 3, 3, 1]):
         assert s.twas[k] == timei + 0.5 * dtmax / 2 ** b
+
+
+def _drag_force(k):
+    """derivs replacement: f = -k v at the predicted velocity, so the
+    corrected velocity differs from the prediction and the corrector has to
+    iterate when k dt is large."""
+
+    def fn(ctx, icall, s, timei, dt):
+        act = np.ones(s.n, bool) if s.iphase is None else (s.iphase > 0)
+        s.fxyzu[act, :3] = -k * s.vpred[act, :3]
+        if s.fxyzu.shape[1] >= 4:
+            s.fxyzu[act, 3] = 0.0
+        s.dtcourant = s.dtforce = 1.e29
+        return s.params.dtmax
+
+    return fn
+
+
+def test_dterr_after_iterated_step():
+    """check_velocity_error sets timestep:dterr on every call, and finite
+    only on the first iteration (step_leapfrog.f90:807-838): an iterated
+    step leaves it huge, so the next dt is dtmax, not the first iteration's
+    dterr.  A step that converges at once keeps the finite dterr."""
+    p = _params(maxvxyzu=4, dtmax=0.1)
+    dt = 0.1
+    huge = np.finfo(np.float64).max
+    # strong drag: the first corrector iteration misses tolv
+    s = _free_state(p)
+    o = stepping.OracleBackend(p)
+    fn = _drag_force(5.0)
+    integrator.init_global(o, s, derivs_fn=fn)
+    _, dtnext, out = integrator.evolve_global(o, s, 1, dt=dt, derivs_fn=fn)
+    assert out[0]["its"] >= 2
+    assert out[0]["dterr"] == huge
+    assert dtnext == p.dtmax + np.finfo(np.float64).eps
+    # weak drag: converged at the first iteration with a finite dterr, which
+    # sets dt = dt sqrt(tolv/errmax)
+    s = _free_state(p)
+    fn = _drag_force(0.05)
+    integrator.init_global(o, s, derivs_fn=fn)
+    _, dtnext, out = integrator.evolve_global(o, s, 1, dt=dt, derivs_fn=fn)
+    assert out[0]["its"] == 1
+    assert out[0]["dterr"] < huge
+    assert dtnext == min(out[0]["dterr"], p.dtmax + np.finfo(np.float64).eps)
