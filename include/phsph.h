@@ -400,6 +400,61 @@ int phsph_get_particle_stats(phsph_ctx *ctx, int32_t np, int32_t *nneigh,
 int phsph_get_force_stats(phsph_ctx *ctx, int64_t *npairs);
 
 /*
+ * Adjoints with respect to positions.  Phantom has no adjoint: these are
+ * the vector-Jacobian products of its forward sums, for gradient-based
+ * uses of the hot path (relaxation of initial conditions to a target
+ * density, sensitivities).  Hydrodynamics only (no MHD, dust or
+ * self-gravity).  The h-rho constraint rho_i = m (hfact/h_i)^3 of
+ * densityiterate (dens.F90:1382-1466) is differentiated implicitly, so h
+ * must be converged (a tight tolh makes the gradient that of the discrete
+ * solution).  Neighbour lists are recomputed from the current tree; no
+ * pair data is stored between the forward and the adjoint.  Arrays follow
+ * the other entry points (host or device, a(ld, np)); gradients dxyz are
+ * (3, np) with leading dimension ldd.
+ */
+
+/*
+ * Adjoint of densityiterate's density summation (get_density_sums,
+ * dens.F90:578-857): dxyz_k = sum_i [rhobar_i drho_i/dx_k + hbar_i dh_i/dx_k]
+ * with rho_i = m (hfact/h_i)^3 at the converged h.  xyzh: the positions and
+ * h after phsph_maketree + phsph_densityiterate(icall 1) on them.  rhobar,
+ * hbar: (np) adjoints of rho and h (either may be NULL = 0).
+ */
+int phsph_density_vjp(phsph_ctx *ctx, int32_t np, const double *xyzh, int32_t ldxyzh,
+                      const double *rhobar, const double *hbar, double *dxyz, int32_t lddxyz);
+
+/*
+ * The pressure acceleration of force (force.F90:1595-1596 without
+ * viscosity, conductivity or other physics: a_i = -sum_j m [P_i/(Omega_i
+ * rho_i^2) dW_i + P_j/(Omega_j rho_j^2) dW_j] r_hat, with Omega in FP64)
+ * for a barotropic closure P = polyk rho (ieos 1) or P = (gamma-1) u rho
+ * with u held fixed (ieos 2, u from vxyzu), and its adjoint
+ * dxyz = d/dx sum_i fbar_i . a_i through the separations, h(x) and
+ * Omega(x).  xyzh as for phsph_density_vjp.  fxyz(ldfxyz, np) (may be
+ * NULL) receives a; fbar (ldfbar >= 3, may be NULL: forward only).
+ */
+int phsph_force_vjp(phsph_ctx *ctx, int32_t np, const double *xyzh, int32_t ldxyzh,
+                    const double *vxyzu, int32_t ldvxyzu, const double *fbar, int32_t ldfbar,
+                    double *fxyz, int32_t ldfxyz, double *dxyz, int32_t lddxyz);
+
+/*
+ * One leapfrog step density -> force -> kick (step_leapfrog.f90:150-625
+ * with the pressure acceleration above): v_half = v + dt/2 a(x),
+ * x' = x + dt v_half, then maketree + densityiterate at x' and
+ * v' = v_half + dt/2 a(x').  Forward: xyzh_new (positions and converged h
+ * at x'), vxyzu_new (u copied), either may be NULL.  Adjoint, when xbar or
+ * vbar is given ((3, np), ld ldxbar / ldvbar; NULL = 0): dxyz and dvxyz
+ * (ld lddxyz) receive dL/dx and dL/dv of L = xbar.x' + vbar.v'.  The tree
+ * and h at x and x' are rebuilt for the adjoint instead of stored.  xyzh's
+ * h is the start of the density iteration at x.  The context's tree is
+ * left at x (after an adjoint) or x' (forward only).
+ */
+int phsph_step_vjp(phsph_ctx *ctx, int32_t np, double dt, const double *xyzh, int32_t ldxyzh,
+                   const double *vxyzu, int32_t ldvxyzu, double *xyzh_new, double *vxyzu_new,
+                   const double *xbar, int32_t ldxbar, const double *vbar, int32_t ldvbar,
+                   double *dxyz, double *dvxyz, int32_t lddxyz);
+
+/*
  * Precision (PHSPH_PREC_*) the pair sums of the last phsph_densityiterate
  * and phsph_force actually ran in; -1 before the first call.  With
  * params.precision = PHSPH_PREC_MIXED the hydro sums run in mixed

@@ -797,6 +797,69 @@ int phsph_get_force_stats(phsph_ctx *c, int64_t *npairs) {
   return PHSPH_OK;
 }
 
+int phsph_density_vjp(phsph_ctx *c, int32_t np, const double *xyzh, int32_t ldx, const double *rhobar,
+                      const double *hbar, double *dxyz, int32_t ldd) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (!xyzh || !dxyz || np <= 0 || ldx < 4 || ldd < 3) PH_RET(c, PHSPH_ERR_INPUT, "density_vjp: bad arguments");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  ctx_clear_timings(c);
+  Stager st(c);
+  double *dX, *dR, *dH, *dD;
+  STAGE(st, const_cast<double *>(xyzh), (size_t)ldx * np, true, false, &dX);
+  STAGE(st, const_cast<double *>(rhobar), (size_t)np, true, false, &dR);
+  STAGE(st, const_cast<double *>(hbar), (size_t)np, true, false, &dH);
+  STAGE(st, dxyz, (size_t)ldd * np, false, true, &dD);
+  int rc = vjp_density_run(c, np, dX, ldx, dR, dH, dD, ldd);
+  int rc2 = st.finish();
+  return rc ? rc : rc2;
+}
+
+int phsph_force_vjp(phsph_ctx *c, int32_t np, const double *xyzh, int32_t ldx, const double *vxyzu, int32_t ldv,
+                    const double *fbar, int32_t ldfb, double *fxyz, int32_t ldf, double *dxyz, int32_t ldd) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (!xyzh || np <= 0 || ldx < 4 || (fxyz && ldf < 3) || (fbar && (ldfb < 3 || !dxyz || ldd < 3)))
+    PH_RET(c, PHSPH_ERR_INPUT, "force_vjp: bad arguments");
+  if (vxyzu && ldv < c->p.maxvxyzu) PH_RET(c, PHSPH_ERR_INPUT, "force_vjp: bad ldvxyzu");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  ctx_clear_timings(c);
+  Stager st(c);
+  double *dX, *dV, *dFb, *dF, *dD;
+  STAGE(st, const_cast<double *>(xyzh), (size_t)ldx * np, true, false, &dX);
+  STAGE(st, const_cast<double *>(vxyzu), (size_t)ldv * np, true, false, &dV);
+  STAGE(st, const_cast<double *>(fbar), (size_t)ldfb * np, true, false, &dFb);
+  STAGE(st, fxyz, (size_t)ldf * np, false, true, &dF);
+  STAGE(st, fbar ? dxyz : nullptr, (size_t)ldd * np, false, true, &dD);
+  int rc = vjp_force_run(c, np, dX, ldx, dV, ldv, dFb, ldfb, dF, ldf, dD, ldd);
+  int rc2 = st.finish();
+  return rc ? rc : rc2;
+}
+
+int phsph_step_vjp(phsph_ctx *c, int32_t np, double dt, const double *xyzh, int32_t ldx, const double *vxyzu,
+                   int32_t ldv, double *xyzh_new, double *vxyzu_new, const double *xbar, int32_t ldxb,
+                   const double *vbar, int32_t ldvb, double *dxyz, double *dvxyz, int32_t ldd) {
+  if (!c) return PHSPH_ERR_INPUT;
+  const bool back = xbar || vbar;
+  if (!xyzh || !vxyzu || np <= 0 || ldx < 4 || ldv < c->p.maxvxyzu || (xbar && ldxb < 3) || (vbar && ldvb < 3) ||
+      (back && (!dxyz || !dvxyz || ldd < 3)))
+    PH_RET(c, PHSPH_ERR_INPUT, "step_vjp: bad arguments");
+  if ((int64_t)np >= (1 << 27)) PH_RET(c, PHSPH_ERR_CAPACITY, "step_vjp: at most 2^27-1 particles");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  ctx_clear_timings(c);
+  Stager st(c);
+  double *dX, *dV, *dXn, *dVn, *dXb, *dVb, *dDx, *dDv;
+  STAGE(st, const_cast<double *>(xyzh), (size_t)ldx * np, true, false, &dX);
+  STAGE(st, const_cast<double *>(vxyzu), (size_t)ldv * np, true, false, &dV);
+  STAGE(st, xyzh_new, (size_t)ldx * np, true, true, &dXn);
+  STAGE(st, vxyzu_new, (size_t)ldv * np, true, true, &dVn);
+  STAGE(st, const_cast<double *>(xbar), (size_t)ldxb * np, true, false, &dXb);
+  STAGE(st, const_cast<double *>(vbar), (size_t)ldvb * np, true, false, &dVb);
+  STAGE(st, back ? dxyz : nullptr, (size_t)ldd * np, false, true, &dDx);
+  STAGE(st, back ? dvxyz : nullptr, (size_t)ldd * np, false, true, &dDv);
+  int rc = vjp_step_run(c, np, dt, dX, ldx, dV, ldv, dXn, ldx, dVn, ldv, dXb, ldxb, dVb, ldvb, dDx, ldd, dDv, ldd);
+  int rc2 = st.finish();
+  return rc ? rc : rc2;
+}
+
 int phsph_get_precision(phsph_ctx *c, int32_t *dens_prec, int32_t *force_prec) {
   if (!c) return PHSPH_ERR_INPUT;
   if (dens_prec) *dens_prec = c->dens_prec_used;

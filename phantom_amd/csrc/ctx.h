@@ -203,6 +203,17 @@ struct phsph_ctx {
   DBuf<float> gsoft;                  // gradh(2) in tree order
   float *poten_user = nullptr;        // registered potential energy output (host or device)
 
+  // adjoints (vjp.hip): per-particle state and weights in tree order, the
+  // recomputed neighbour lists, and the state of the step adjoint
+  DBuf<double> vjp_c;                 // P/rho of the barotropic closure
+  DBuf<double4> vjp_st;               // Omega, A, d2(rho_sum)/dh2, rho
+  DBuf<double2> vjp_gdb, vjp_ahb;     // gather weights; Abar, hbar
+  DBuf<double4> vjp_abar, vjp_grad;   // adjoint of the acceleration; pair part of the gradient
+  DBuf<uint32_t> vjp_nl;
+  DBuf<int32_t> vjp_tnl;
+  DBuf<double> vjp_x0, vjp_x1, vjp_vh, vjp_a, vjp_t3, vjp_t3b;
+  DBuf<float> vjp_dcv, vjp_alpha, vjp_gradh, vjp_dvdx;
+
   // integrator reductions (block partials of the step loops)
   DBuf<double> step_red;
   float *poten_out = nullptr;         // device view of poten_user during a force call
@@ -339,6 +350,14 @@ int grav_force(phsph_ctx *c, int icall, const ForceIOArgs &a, const FRec *rec, c
                double *dtf_out);
 int set_active_run(phsph_ctx *c, int np, const double *xyzh, int ldx, int8_t *iphase, const int8_t *ibin,
                    int nbinmax, int istepfrac, int32_t *nactive, int32_t *nalive);
+// vjp.hip
+int vjp_density_run(phsph_ctx *c, int np, const double *xyzh, int ldx, const double *rhobar, const double *hbar,
+                    double *dxyz, int ldd);
+int vjp_force_run(phsph_ctx *c, int np, const double *xyzh, int ldx, const double *vxyzu, int ldv,
+                  const double *fbar, int ldfb, double *accel, int lda, double *dxyz, int ldd);
+int vjp_step_run(phsph_ctx *c, int np, double dt, const double *xyzh, int ldx, const double *vxyzu, int ldv,
+                 double *xyzh_new, int ldxn, double *vxyzu_new, int ldvn, const double *xbar, int ldxb,
+                 const double *vbar, int ldvb, double *dx, int lddx, double *dv, int lddv);
 int cons2prim_run(phsph_ctx *c, int np, const double *xyzh, int ldx, const double *vxyzu, int ldv,
                   const float *dvdx, float *alphaind, const double *dustevol, double *dustfrac,
                   double *eos, int ldeos);

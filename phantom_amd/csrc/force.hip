@@ -824,6 +824,11 @@ static int force_lists(phsph_ctx *c, DBuf<uint32_t> &L, DBuf<int32_t> &TNL, cons
   return PHSPH_OK;
 }
 
+int force_lists_all(phsph_ctx *c, DBuf<uint32_t> &L, DBuf<int32_t> &TNL, const int64_t **loff,
+                    const char *tname) {
+  return force_lists(c, L, TNL, loff, tname);
+}
+
 struct ForcePass {
   const FRec *rec;
   const FRecB *recB;

@@ -369,3 +369,37 @@ def two_stars(n_each=10000, seed=11, separation=20.0, theta=0.5, kernel=abi.KERN
     alphaind = np.zeros((n, 3), dtype=np.float32)
     return dict(xyzh=xyzh, vxyzu=vxyzu, alphaind=alphaind, params=p,
                 info=dict(workload="two_stars", n=n, seed=seed, separation=separation))
+
+
+def polytrope_profile(nindex=1.5, nt=2000):
+    """Tabulated (r, rho, P) of the M = R = G = 1 polytrope of index nindex
+    (Lane-Emden), the table a stellar relaxation starts from: rho =
+    rho_c theta^n(xi1 r), P = K rho^(1+1/n)."""
+    xi, th, dth = lane_emden(nindex)
+    xi1 = xi[-1]
+    rho_c = 1.0 / (4.0 * math.pi * (1.0 / xi1) ** 3 * xi1 * xi1 * (-dth[-1]))
+    alpha = 1.0 / xi1
+    K = 4.0 * math.pi * alpha * alpha * rho_c ** (1.0 - 1.0 / nindex) / (nindex + 1.0)
+    r = np.linspace(0.0, 1.0, nt)
+    # theta floored just above zero at the surface so that the entropy
+    # P/rho^gamma = K is defined at every table point
+    rho = rho_c * np.maximum(np.interp(r * xi1, xi, th), 1e-6) ** nindex
+    return r, rho, K * rho ** (1.0 + 1.0 / nindex)
+
+
+def relaxation_target(n=1048576, nindex=3.0, sigma=0.02, seed=2024, kernel=abi.KERNEL_CUBIC):
+    """The differentiable-relaxation workload: an n = 3 polytrope (M = R = 1)
+    sampled by the radial stretch map of polytrope() plus seeded Gaussian
+    noise of sigma * h on every coordinate; hydrodynamics only (the adjoint
+    kernels have no self-gravity), ideal gas with gamma = 1 + 1/n and
+    u = K rho^(1/n) n held per particle, open boundaries.  Returns the setup
+    and the target profile table (r, rho_t) of the same polytrope."""
+    s = polytrope(n=n, nindex=nindex, seed=seed, kernel=kernel, gamma=1.0 + 1.0 / nindex)
+    p = s["params"]
+    p.gravity = 0
+    rng = np.random.default_rng(seed + 1)
+    h = s["xyzh"][:, 3]
+    s["xyzh"][:, :3] += rng.normal(0.0, 1.0, size=(h.shape[0], 3)) * (sigma * h)[:, None]
+    r, rho, _ = polytrope_profile(nindex)
+    s["target"] = (r, rho)
+    return s

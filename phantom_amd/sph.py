@@ -35,7 +35,8 @@ EXPORTED = [
     "phsph_get_neighbour_stats", "phsph_get_particle_stats", "phsph_get_force_stats", "phsph_get_timings",
     "phsph_set_bins", "phsph_get_nbinmax", "phsph_set_active_particles", "phsph_set_poten",
     "phsph_get_quads", "phsph_init_step", "phsph_step_predictor", "phsph_substep_sph", "phsph_step_predict_sph",
-    "phsph_step_corrector", "phsph_step_reset", "phsph_get_precision",
+    "phsph_step_corrector", "phsph_step_reset", "phsph_get_precision", "phsph_density_vjp", "phsph_force_vjp",
+    "phsph_step_vjp",
 ]
 
 _lib = None
@@ -75,6 +76,9 @@ def load_library(path=LIB_PATH):
     L.phsph_get_particle_stats.argtypes = [vp, i32, vp, vp]
     L.phsph_get_force_stats.argtypes = [vp, P(i64)]
     L.phsph_get_precision.argtypes = [vp, P(i32), P(i32)]
+    L.phsph_density_vjp.argtypes = [vp, i32, vp, i32, vp, vp, vp, i32]
+    L.phsph_force_vjp.argtypes = [vp, i32, vp, i32, vp, i32, vp, i32, vp, i32, vp, i32]
+    L.phsph_step_vjp.argtypes = [vp, i32, dbl, vp, i32, vp, i32, vp, vp, vp, i32, vp, i32, vp, vp, i32]
     L.phsph_get_timings.argtypes = [vp, P(C.c_char_p), P(dbl), i32]
     L.phsph_set_bins.argtypes = [vp, vp, vp, vp, i32, i32, i32]
     L.phsph_get_nbinmax.argtypes = [vp, P(i32)]
@@ -429,6 +433,58 @@ class Context:
         self._check(self.lib.phsph_step_reset(
             self._h, vxyzu.shape[0], dtsph, pv, ldv, pf, ldf, _ptr(vpred)[0], _ptr(Bevol)[0], _ptr(dBevol)[0],
             _ptr(Bpred)[0], _ptr(dustevol)[0], _ptr(ddustevol)[0], _ptr(dustpred)[0], _ptr(iphase)[0]))
+
+    # -- adjoints --------------------------------------------------------------
+    def density_vjp(self, xyzh, rhobar=None, hbar=None, dxyz=None):
+        """dL/dx for L = sum rhobar.rho + hbar.h at the converged h of the
+        last densityiterate on these positions (tree current)."""
+        n = xyzh.shape[0]
+        if dxyz is None:
+            dxyz = _zeros_like_layout(xyzh, (n, 3), np.float64)
+        px, ldx = _ptr(xyzh)
+        pr, _ = _ptr(rhobar)
+        ph, _ = _ptr(hbar)
+        pd, ldd = _ptr(dxyz)
+        self._check(self.lib.phsph_density_vjp(self._h, n, px, ldx, pr, ph, pd, ldd))
+        return dxyz
+
+    def force_vjp(self, xyzh, vxyzu=None, fbar=None, fxyz=None, dxyz=None, want_accel=True):
+        """The pressure acceleration (barotropic closure) and, with fbar, its
+        adjoint dL/dx of L = sum fbar.a.  Returns (a, dxyz)."""
+        n = xyzh.shape[0]
+        if want_accel and fxyz is None:
+            fxyz = _zeros_like_layout(xyzh, (n, 3), np.float64)
+        if fbar is not None and dxyz is None:
+            dxyz = _zeros_like_layout(xyzh, (n, 3), np.float64)
+        px, ldx = _ptr(xyzh)
+        pv, ldv = _ptr(vxyzu)
+        pb, ldb = _ptr(fbar)
+        pf, ldf = _ptr(fxyz)
+        pd, ldd = _ptr(dxyz)
+        self._check(self.lib.phsph_force_vjp(self._h, n, px, ldx, pv, ldv, pb, ldb, pf, ldf, pd, ldd))
+        return fxyz, dxyz
+
+    def step_vjp(self, dt, xyzh, vxyzu, xbar=None, vbar=None, want_state=True):
+        """One kick-drift-kick step with the pressure acceleration; returns
+        (xyzh_new, vxyzu_new, dL/dx, dL/dv) with L = xbar.x' + vbar.v' (the
+        gradients are None without xbar and vbar)."""
+        n = xyzh.shape[0]
+        xn = _zeros_like_layout(xyzh, xyzh.shape, np.float64) if want_state else None
+        vn = _zeros_like_layout(xyzh, vxyzu.shape, np.float64) if want_state else None
+        back = xbar is not None or vbar is not None
+        dx = _zeros_like_layout(xyzh, (n, 3), np.float64) if back else None
+        dv = _zeros_like_layout(xyzh, (n, 3), np.float64) if back else None
+        px, ldx = _ptr(xyzh)
+        pv, ldv = _ptr(vxyzu)
+        pxb, ldxb = _ptr(xbar)
+        pvb, ldvb = _ptr(vbar)
+        pxn, _ = _ptr(xn)
+        pvn, _ = _ptr(vn)
+        pdx, ldd = _ptr(dx)
+        pdv, _ = _ptr(dv)
+        self._check(self.lib.phsph_step_vjp(self._h, n, float(dt), px, ldx, pv, ldv, pxn, pvn, pxb, ldxb, pvb, ldvb,
+                                            pdx, pdv, ldd if back else 3))
+        return xn, vn, dx, dv
 
     def get_precision(self):
         """(density, force) precision the last calls ran in (abi.PREC_*,

@@ -46,13 +46,17 @@ class OracleBackend(oracle.Oracle):
               tstop=None):
         L = oracle.lib()
         dtc, dtf = C.c_double(), C.c_double()
-        L.orc_set_poten(self._ctx, None)
+        L.orc_set_poten(self._ctx, _p(getattr(self, "_poten", None)))
         self._check(L.orc_force(self._ctx, icall, xyzh.shape[0], _p(xyzh), _p(vxyzu), _ld(vxyzu), _p(fxyzu),
                                 _ld(fxyzu), _p(divcurlv), _ld(divcurlv), _p(divcurlB), _p(Bevol), _p(dBevol),
                                 _p(divBsymm), _p(dustfrac), _p(ddustevol), _p(deltav), _p(tstop), _p(gradh),
                                 _ld(gradh), _p(alphaind), _p(eos_vars), _ld(eos_vars), _p(iphase), C.byref(dtc),
                                 C.byref(dtf)))
         return dict(fxyzu=fxyzu, dtcourant=dtc.value, dtforce=dtf.value)
+
+    def set_poten(self, poten):
+        """part:poten, written by force while params.gravity is set."""
+        self._poten = poten
 
     def get_nbinmax(self):
         return self.nbinmax()
