@@ -1226,7 +1226,9 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   PH_HIPCHECK(c, hipGetLastError());
   long long acc[8];
   PH_HIPCHECK(c, hipMemcpyAsync(acc, c->stats_i.p, sizeof(acc), hipMemcpyDeviceToHost, s));
-  // the active list again holds every active particle of this call
+  // the active list again holds every active particle of this call (the
+  // padding of a retry set may have left c->act with a smaller buffer)
+  PH_HIPCHECK(c, c->act.ensure((size_t)nact + 1));
   if (nact > 0)
     PH_HIPCHECK(c, hipMemcpyAsync(c->act.p, c->act_dens.p, sizeof(int32_t) * nact, hipMemcpyDeviceToDevice, s));
   PH_HIPCHECK(c, hipStreamSynchronize(s));

@@ -90,8 +90,9 @@ def test_density_vjp_vs_reference_and_fd(cloud):
     a = _solve(ctx, xyzh, v)
     g = ctx.density_vjp(a, rhobar=w)
     # the dense reference through autograd
+    # (its unclamped Newton iteration starts from the library's converged h)
     x = torch.tensor(xyzh[:, :3], requires_grad=True)
-    rho, _ = R.density(x, torch.tensor(xyzh[:, 3]), p.massoftype_gas, p.hfact, R.Kernel(True))
+    rho, _ = R.density(x, torch.tensor(a[:, 3]), p.massoftype_gas, p.hfact, R.Kernel(True))
     gr = torch.autograd.grad((torch.tensor(w) * rho).sum(), x)[0].numpy()
     np.testing.assert_allclose(_rho(p, a), rho.detach().numpy(), rtol=1e-12)
     err_ref = _max_rel(g, gr)
@@ -120,7 +121,7 @@ def test_pressure_accel_forward(cloud):
     a = _solve(ctx, xyzh, v)
     acc, _ = ctx.force_vjp(a, v)
     x = torch.tensor(xyzh[:, :3], requires_grad=True)
-    _, h = R.density(x, torch.tensor(xyzh[:, 3]), p.massoftype_gas, p.hfact, R.Kernel(True))
+    _, h = R.density(x, torch.tensor(a[:, 3]), p.massoftype_gas, p.hfact, R.Kernel(True))
     ar = R.pressure_accel(x, h, p.massoftype_gas, p.hfact, R.Kernel(True), p.polyk).detach().numpy()
     assert _max_rel(acc, ar) <= 1e-10
     assert np.abs(acc.sum(0)).max() <= 1e-12 * np.abs(acc).sum(0).max()
@@ -140,7 +141,7 @@ def test_force_vjp_vs_reference_and_fd(cloud):
     _, g = ctx.force_vjp(a, v, fbar=wb)
     x = torch.tensor(xyzh[:, :3], requires_grad=True)
     K = R.Kernel(True)
-    _, h = R.density(x, torch.tensor(xyzh[:, 3]), p.massoftype_gas, p.hfact, K)
+    _, h = R.density(x, torch.tensor(a[:, 3]), p.massoftype_gas, p.hfact, K)
     ar = R.pressure_accel(x, h, p.massoftype_gas, p.hfact, K, p.polyk)
     gr = torch.autograd.grad((torch.tensor(wb) * ar).sum(), x)[0].numpy()
     err_ref = _max_rel(g, gr)
@@ -176,7 +177,8 @@ def test_step_vjp_vs_reference_and_fd(ieos, gpu):
     vt = torch.tensor(v[:, :3], requires_grad=True)
     K = R.Kernel(True)
     cfac = p.polyk if ieos == 1 else torch.tensor((p.gamma - 1.0) * v[:, 3])
-    x1, v1 = R.step(x, vt, torch.tensor(xyzh[:, 3]), dt, p.massoftype_gas, p.hfact, K, cfac)
+    hc = _solve(ctx, xyzh, v)[:, 3]  # the reference's Newton starts from converged h
+    x1, v1 = R.step(x, vt, torch.tensor(hc), dt, p.massoftype_gas, p.hfact, K, cfac)
     assert _max_rel(xn[:, :3], x1.detach().numpy()) <= 1e-12
     assert _max_rel(vn[:, :3], v1.detach().numpy()) <= 1e-10
     L = (torch.tensor(xb) * x1).sum() + (torch.tensor(vb) * v1).sum()
@@ -209,7 +211,7 @@ def test_torch_autograd_harness(cloud):
     torch, R = _torch_ref()
     p, xyzh, v, ctx = cloud
     K = R.Kernel(True)
-    h0 = torch.tensor(xyzh[:, 3])
+    h0 = torch.tensor(_solve(ctx, xyzh, v)[:, 3])
     target = adjoint.profile_target(np.linspace(0.0, 2.0, 200), 2.0 * np.exp(-np.linspace(0.0, 2.0, 200) ** 2))
     x = torch.tensor(xyzh[:, :3], requires_grad=True)
     loss = adjoint.density_loss(x, h0, ctx, target)
