@@ -51,7 +51,8 @@ extern "C" {
 
 /* ---- status codes --------------------------------------------------- */
 #define PHSPH_OK              0
-#define PHSPH_ERR_INPUT      -1  /* invalid argument, h <= 0, eps outside [0,1] */
+#define PHSPH_ERR_INPUT      -1  /* invalid argument or parameter; negative h
+                                    from hrho (dens.F90:1598); invalid dtc */
 #define PHSPH_ERR_NAN        -2  /* NaN in particle positions (kdtree.F90:391) */
 #define PHSPH_ERR_NOCONVERGE -3  /* h-rho iteration hit maxdensits (dens.F90:1443) */
 #define PHSPH_ERR_CAPACITY   -4  /* tree depth / list capacity exceeded */
@@ -84,7 +85,8 @@ typedef struct phsph_params {
   int32_t ieos;                /* eos:ieos, 1 (isothermal) or 2 (ideal gas)   */
   int32_t maxp;                /* dim:maxp; sets ncellsmax = 2*maxp           */
   int32_t precision;           /* PHSPH_PREC_*                                */
-  int32_t idrag;               /* dust:idrag, 2 (const K) or 3 (const t_s)    */
+  int32_t idrag;               /* dust:idrag, 1 (Epstein/Stokes), 2 (const K) */
+                               /* or 3 (const t_s)                             */
   int32_t ishock_heating;      /* eos:ishock_heating                          */
   int32_t ipdv_heating;        /* eos:ipdv_heating                            */
   int32_t iresistive_heating;  /* eos:iresistive_heating                      */
@@ -106,6 +108,14 @@ typedef struct phsph_params {
   double overcleanfac;         /* timestep:overcleanfac                       */
   double K_code;               /* dust:K_code(1)                              */
   double tree_accuracy;        /* kdtree:tree_accuracy (opening angle)        */
+  /* idrag = 1 (dust.f90:188-245): grain size and intrinsic density in code
+   * units (part:grainsize(1) = grainsizecgs/udist, graindens(1) =
+   * graindenscgs/unit_density) and the effective surface density of the
+   * mean free path lambda = seff/rho_gas (dust:seff of init_drag,
+   * dust.f90:94-97: pi/sqrt(2)*5/64 * (2 m_p/umass) / (sigma_H2/udist^2)) */
+  double grainsize;
+  double graindens;
+  double seff;
 } phsph_params;
 
 /* Reference tree node, layout of type(kdnode) in a non-GRAVITY build
@@ -120,6 +130,23 @@ typedef struct phsph_kdnode {
   int32_t parent;
   int32_t idum;
 } phsph_kdnode;
+
+/* Reference tree node of a GRAVITY build (dtype_kdtree.F90:53-68 with the
+ * GRAVITY fields, 120 bytes): the 64-byte node followed by tobecached,
+ * cached (a default-kind logical, 4 bytes) and quads(6). */
+typedef struct phsph_kdnode_grav {
+  double  xcen[3];
+  double  size;
+  double  hmax;
+  double  mass;
+  int32_t leftchild;
+  int32_t rightchild;
+  int32_t parent;
+  int32_t idum;
+  int32_t tobecached;
+  int32_t cached;
+  double  quads[6];
+} phsph_kdnode_grav;
 
 typedef struct phsph_ctx phsph_ctx;
 
@@ -166,6 +193,14 @@ int phsph_get_tree(phsph_ctx *ctx, phsph_kdnode *node, int32_t *inoderange,
  * (kdtree.F90:724-750, dtype_kdtree.F90:66).  Unreachable node slots hold
  * unspecified values.  Only with params.gravity. */
 int phsph_get_quads(phsph_ctx *ctx, double *quads, int64_t cap_nodes);
+
+/* The node array of a GRAVITY build of the reference (maketree's node(:)
+ * out-argument, dtype_kdtree.F90:53-68) for node ids
+ * 1..min(cap_nodes, ncells): the fields of phsph_get_tree's nodes (hmax as
+ * raised by the density relists), tobecached = 1 and cached = .false. as
+ * construct_node leaves them (kdtree.F90:788-789), and quads as
+ * phsph_get_quads.  Only with params.gravity. */
+int phsph_get_tree_grav(phsph_ctx *ctx, phsph_kdnode_grav *node, int64_t cap_nodes);
 
 /*
  * densityiterate (dens.F90:117-152).  icall 0 = one summation pass without

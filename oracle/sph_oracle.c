@@ -182,6 +182,11 @@ void orc_default_params(phsph_params *p, int32_t kernel) {
   p->psidecayfac = 1.; p->overcleanfac = 1.;
   p->K_code = 1.;
   p->tree_accuracy = 0.5;
+  /* grainsizecgs, graindenscgs (dust.f90:38-39) with udist = umass = 1 and
+   * seff of init_drag (dust.f90:94-97) in those units */
+  p->grainsize = 0.1;
+  p->graindens = 3.;
+  p->seff = PI_PH / sqrt(2.) * 5. / 64. * ((2. * 1.67262158e-24) / 1.) / (2.367e-15 / (1. * 1.));
 }
 
 orc_ctx *orc_create(const phsph_params *p) {
@@ -1242,9 +1247,39 @@ int orc_cons2prim(orc_ctx *o, int32_t np, const double *xyzh, const double *vxyz
 /* force                                                               */
 /* ------------------------------------------------------------------ */
 
-/* get_ts (dust.f90:161-276) for idrag = 2 (constant K) and 3 (constant
- * stopping time) */
-static double get_ts(const orc_ctx *o, double rhogas, double rhodust) {
+/* get_ts (dust.f90:161-276) with dv2 = 0 (the one-fluid calls,
+ * force.F90:1793,2357) for idrag = 1 (Epstein/Stokes, dust.f90:188-245;
+ * cste_mu and coeff_gei_1 of init_drag, dust.f90:78-79), 2 (constant K)
+ * and 3 (constant stopping time) */
+static double get_ts(const orc_ctx *o, double rhogas, double rhodust, double spsoundgas) {
+  if (o->p.idrag == 1) {
+    const double sgrain = o->p.grainsize, densgrain = o->p.graindens;
+    const double cste_mu = sqrt(2. / (PI_PH * o->p.gamma)), coeff_gei_1 = sqrt(8. / (PI_PH * o->p.gamma));
+    double rhosum = rhogas + rhodust;
+    double lambda = o->p.seff / rhogas;
+    double kn_eff = (sgrain > 0.) ? 9. * lambda / (4. * sgrain) : HUGE64;
+    double dragcoeff = 0., f = 0., abs_dv = 0., dv2 = 0.;
+    if (kn_eff >= 1.) {
+      dragcoeff = (densgrain > TINY64) ? coeff_gei_1 * spsoundgas / (densgrain * sgrain) : HUGE64;
+      f = 1.;  /* dv2 = 0: no Kwok correction */
+      (void)dv2;
+    } else {
+      double viscmol_nu = cste_mu * lambda * spsoundgas;
+      double re_dust = 2. * sgrain * abs_dv / viscmol_nu;
+      if (re_dust <= 1.) {
+        dragcoeff = 4.5 * viscmol_nu / (densgrain * sgrain * sgrain);
+        f = 1.;
+      } else if (re_dust <= 800.) {
+        dragcoeff = 9. / (densgrain * sgrain * pow(re_dust, 0.6));
+        f = abs_dv;
+      } else {
+        dragcoeff = 0.163075 / (densgrain * sgrain);
+        f = abs_dv;
+      }
+    }
+    double ts1 = (dragcoeff == HUGE64) ? HUGE64 : dragcoeff * f * rhosum;
+    return (ts1 > 0.) ? 1. / ts1 : HUGE64;
+  }
   if (o->p.idrag == 2) {
     double rhosum = rhogas + rhodust;
     if (o->p.K_code > 0.) return rhogas * rhodust / (o->p.K_code * rhosum);
@@ -1520,7 +1555,7 @@ static void compute_forces(const orc_ctx *o, int32_t i, const fpart_t *pi, const
       fsum[F_DIVBDIFF] += divBdiffterm;
     }
     if (dust) {
-      double tsj = get_ts(o, rhogasj, rhoj * epsj);
+      double tsj = get_ts(o, rhogasj, rhoj * epsj, csj);
       if (o->p.ilimitdustflux) { double lim = (1. / hj1) / csj; if (lim < tsj) tsj = lim; }
       double epstsj = epsj * tsj;
       if (epsi > 0. || epsj > 0.) {
@@ -1609,7 +1644,7 @@ int orc_force(orc_ctx *o, int32_t icall, int32_t np, const double *xyzh, const d
         get_stress(o, q->pr, q->cs, rhoi, rho1i, pm, q->Bx_e * rhoi, q->By_e * rhoi, q->Bz_e * rhoi, &q->pro2,
                    &q->vwave, s);
         q->sxx = s[0]; q->sxy = s[1]; q->sxz = s[2]; q->syy = s[3]; q->syz = s[4]; q->szz = s[5];
-        q->tstop = dust ? get_ts(o, q->rhogas, rhoi * q->eps) : 0.;
+        q->tstop = dust ? get_ts(o, q->rhogas, rhoi * q->eps, q->cs) : 0.;
         q->alpha = (double)alphaind[3 * (size_t)(i - 1)];
         double g = (double)gradh[(size_t)ldgradh * (i - 1)];
         q->gradh = (g > 0.) ? g : 1.;

@@ -6,6 +6,7 @@ shock_capturing.f90:47-65, timestep.f90:52-62, eos.f90:1888-1905,
 dust.f90:26-33, kdtree.F90:46).
 """
 import ctypes as C
+import math
 
 KERNEL_CUBIC = 0
 KERNEL_QUINTIC = 1
@@ -40,7 +41,7 @@ _DBL_FIELDS = [
     "xmin", "xmax", "ymin", "ymax", "zmin", "zmax", "hfact", "tolh",
     "massoftype_gas", "gamma", "polyk", "alpha", "alphamax", "alphau",
     "alphaB", "beta", "C_cour", "C_force", "dtmax", "psidecayfac",
-    "overcleanfac", "K_code", "tree_accuracy",
+    "overcleanfac", "K_code", "tree_accuracy", "grainsize", "graindens", "seff",
 ]
 
 
@@ -62,6 +63,29 @@ class KdNode(C.Structure):
 
 
 assert C.sizeof(KdNode) == 64
+
+
+class KdNodeGrav(C.Structure):
+    """type(kdnode) of a GRAVITY build (dtype_kdtree.F90:53-68)."""
+    _fields_ = KdNode._fields_ + [("tobecached", C.c_int32), ("cached", C.c_int32),
+                                  ("quads", C.c_double * 6)]
+
+
+assert C.sizeof(KdNodeGrav) == 120
+
+
+PI_PH = 3.1415926536            # physcon.f90:23
+MASS_PROTON_CGS = 1.67262158e-24  # physcon.f90:41
+CROSS_SECTION_H2_CGS = 2.367e-15  # physcon.f90:43
+
+
+def seff_code_units(umass, udist):
+    """Effective surface density of the gas mean free path, lambda =
+    seff/rho_gas, in code units (init_drag, dust.f90:94-97)."""
+    cste_seff = PI_PH / math.sqrt(2.0) * 5.0 / 64.0
+    mass_mol_gas = (2.0 * MASS_PROTON_CGS) / umass
+    cross_section_gas = CROSS_SECTION_H2_CGS / (udist * udist)
+    return cste_seff * mass_mol_gas / cross_section_gas
 
 
 def default_params(kernel=KERNEL_CUBIC, **overrides):
@@ -102,6 +126,9 @@ def default_params(kernel=KERNEL_CUBIC, **overrides):
     p.overcleanfac = 1.0
     p.K_code = 1.0
     p.tree_accuracy = 0.5
+    p.grainsize = 0.1   # grainsizecgs (dust.f90:38) in units with udist = 1 cm
+    p.graindens = 3.0   # graindenscgs (dust.f90:39) in units with unit_density = 1 g/cm^3
+    p.seff = seff_code_units(1.0, 1.0)
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise KeyError(k)

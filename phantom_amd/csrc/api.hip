@@ -27,6 +27,12 @@ DevConst make_devconst(const phsph_params &p) {
   d.iresistive_heating = p.iresistive_heating;
   d.ilimitdustflux = p.ilimitdustflux;
   d.idrag = p.idrag;
+  // init_drag (dust.f90:78-79)
+  d.cste_mu = sqrt(2. / (PH_PI * p.gamma));
+  d.coeff_gei_1 = sqrt(8. / (PH_PI * p.gamma));
+  d.grainsize = p.grainsize;
+  d.graindens = p.graindens;
+  d.seff = p.seff;
   if (p.kernel == PHSPH_KERNEL_CUBIC) {  // kernel_cubic.f90:23-30
     d.radkern = 2.; d.radkern2 = 4.; d.cnormk = 1. / PH_PI;
     d.wab0 = 1.; d.gradh0 = -3.; d.dphidh0 = 1.4;
@@ -62,8 +68,8 @@ static const char *validate(const phsph_params *p) {
   if (!(p->tolh > 0.)) return "tolh must be > 0";
   if (!(p->xmax > p->xmin && p->ymax > p->ymin && p->zmax > p->zmin)) return "empty box";
   if (p->use_dustfrac && (p->ndustsmall < 0 || p->ndustsmall > 1)) return "ndustsmall must be 0 or 1";
-  if (p->use_dustfrac && p->ndustsmall == 1 && p->idrag != 2 && p->idrag != 3)
-    return "one-fluid dust supports idrag 2 (constant K) and 3 (constant stopping time)";
+  if (p->use_dustfrac && p->ndustsmall == 1 && p->idrag != 1 && p->idrag != 2 && p->idrag != 3)
+    return "one-fluid dust supports idrag 1 (Epstein/Stokes), 2 (constant K) and 3 (constant stopping time)";
   if (p->gravity && (p->mhd || (p->use_dustfrac && p->ndustsmall > 0)))
     return "self-gravity is available for hydrodynamics (not with MHD or dust)";
   // read only with gravity, in 0..1 (neigh_kdtree.f90:345)
@@ -148,6 +154,15 @@ __global__ void k_zero_i32_api(int32_t *a, int n) {
   if (i < n) a[i] = 0;
 }
 
+// effective surface density of the gas mean free path in code units
+// (init_drag, dust.f90:94-97)
+static double seff_code_units(double umass, double udist) {
+  const double cste_seff = PH_PI / sqrt(2.) * 5. / 64.;
+  const double mass_mol_gas = (2. * 1.67262158e-24) / umass;   // physcon.f90:41
+  const double cross_section_gas = 2.367e-15 / (udist * udist);  // physcon.f90:43
+  return cste_seff * mass_mol_gas / cross_section_gas;
+}
+
 extern "C" {
 
 void phsph_default_params(phsph_params *p, int32_t kernel) {
@@ -171,6 +186,9 @@ void phsph_default_params(phsph_params *p, int32_t kernel) {
   p->psidecayfac = 1.; p->overcleanfac = 1.;
   p->K_code = 1.;
   p->tree_accuracy = 0.5;
+  p->grainsize = 0.1;  // grainsizecgs, graindenscgs (dust.f90:38-39) with udist = umass = 1
+  p->graindens = 3.;
+  p->seff = seff_code_units(1., 1.);
 }
 
 int phsph_create(const phsph_params *p, int32_t device, phsph_ctx **out) {
@@ -323,6 +341,36 @@ int phsph_get_tree(phsph_ctx *c, phsph_kdnode *node, int32_t *inoderange, int32_
   if (inodeparts)
     PH_HIPCHECK(c, hipMemcpy(inodeparts, c->iparts.p, (size_t)c->nlive * sizeof(int32_t), hipMemcpyDeviceToHost));
   if (maxlevel) *maxlevel = c->maxlevel;
+  return PHSPH_OK;
+}
+
+int phsph_get_tree_grav(phsph_ctx *c, phsph_kdnode_grav *node, int64_t cap_nodes) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (!c->tree_valid) PH_RET(c, PHSPH_ERR_STATE, "get_tree_grav: no tree");
+  if (!c->p.gravity || !c->quads.p) PH_RET(c, PHSPH_ERR_STATE, "get_tree_grav: self-gravity is off");
+  if (!node) PH_RET(c, PHSPH_ERR_INPUT, "get_tree_grav: missing array");
+  const int64_t n = std::min<int64_t>(cap_nodes, c->ncells);
+  if (n <= 0) return PHSPH_OK;
+  std::vector<phsph_kdnode> nd(n);
+  std::vector<double> q((size_t)n * 6);
+  int rc = phsph_get_tree(c, nd.data(), nullptr, nullptr, n, nullptr);
+  if (rc) return rc;
+  rc = phsph_get_quads(c, q.data(), n);
+  if (rc) return rc;
+  for (int64_t k = 0; k < n; k++) {
+    phsph_kdnode_grav &g = node[k];
+    for (int d = 0; d < 3; d++) g.xcen[d] = nd[k].xcen[d];
+    g.size = nd[k].size;
+    g.hmax = nd[k].hmax;
+    g.mass = nd[k].mass;
+    g.leftchild = nd[k].leftchild;
+    g.rightchild = nd[k].rightchild;
+    g.parent = nd[k].parent;
+    g.idum = nd[k].idum;
+    g.tobecached = 1;  // construct_node (kdtree.F90:788-789)
+    g.cached = 0;
+    for (int m = 0; m < 6; m++) g.quads[m] = q[(size_t)k * 6 + m];
+  }
   return PHSPH_OK;
 }
 

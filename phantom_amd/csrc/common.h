@@ -31,6 +31,7 @@ struct DevConst {
   double alpha, alphamax, alphau, alphaB, beta;
   double C_cour, C_force, dtmax, psidecayfac, overcleanfac, K_code;
   double tree_acc2;  // tree_accuracy^2 (opening angle of the gravity walk)
+  double grainsize, graindens, seff, cste_mu, coeff_gei_1;  // idrag = 1 (dust.f90:78-97)
 };
 
 // ---- exact (non-contracted) FP64 arithmetic ----------------------------

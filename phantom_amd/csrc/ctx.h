@@ -158,6 +158,7 @@ struct phsph_ctx {
   DBuf<int32_t> act_dens;                // active positions of the last densityiterate
   int nact_dens = 0;
   bool trial_stale = true;               // trial statistics not yet computed
+  int dens_icall = 1;                    // icall of the last densityiterate
 
   // per-particle / per-leaf outputs of density kept for statistics
   DBuf<int32_t> p_nneigh, p_nits, leaf_nits;
@@ -185,6 +186,7 @@ struct phsph_ctx {
   DBuf<double> quads;                 // node second moments, 6 per slot
   DBuf<unsigned long long> ghmax;     // node hmax as the force walk sees it
   DBuf<double> htraj, hconv;          // h per density iteration, converged h
+  DBuf<double> trial_hmax;            // hmax of each leaf at its last neighbour list (trial statistics)
   DBuf<double> fnode;                 // far-field expansion per leaf (20)
   DBuf<uint32_t> gnear;               // self-gravity force: neighbours per active particle
   DBuf<int32_t> gnear_n;              // their count per active particle

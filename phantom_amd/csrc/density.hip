@@ -51,10 +51,31 @@ struct DensOut {
 // ------------------------------------------------------------------------
 // per-leaf walk of the reference (trial-neighbour statistics only)
 // ------------------------------------------------------------------------
+// hmax of each leaf at the last neighbour list the reference makes for it:
+// the relisted hmax of the replay, else the tree's (get_cell_location)
+__global__ void k_trial_hmax(int nleaves, const int32_t *__restrict__ leaves, const int32_t *__restrict__ leaf_flag,
+                             const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi,
+                             const phsph_kdnode *__restrict__ nd, const int32_t *__restrict__ p_nits,
+                             const double *__restrict__ htraj, const double *__restrict__ hconv,
+                             double *__restrict__ lhmax, int32_t *__restrict__ flags) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nleaves) return;
+  const int c = leaves[l];
+  double h = nd[c].hmax;
+  if (leaf_flag[c] > 0) {
+    double hset = 0.;
+    const int set = leaf_relist_hmax(nd_lo[c], nd_hi[c], h, p_nits, htraj, hconv, &hset);
+    if (set < 0) atomicOr(flags, DFL_NOCONV);
+    if (set > 0) h = hset;
+  }
+  lhmax[l] = h;
+}
+
+// lhmax: hmax of each leaf by leaf index (nullptr: the node's)
 __global__ void k_walk(DevConst dc, double rfac, const phsph_kdnode *__restrict__ nd,
                        const int32_t *__restrict__ leaf_flag, const int32_t *__restrict__ nd_lo,
                        const int32_t *__restrict__ nd_hi, const int32_t *__restrict__ leaves, int nleaves,
-                       int32_t *__restrict__ ntrial, int32_t *__restrict__ flags) {
+                       const double *__restrict__ lhmax, int32_t *__restrict__ ntrial, int32_t *__restrict__ flags) {
   int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= nleaves) return;
   const int c = leaves[l];
@@ -65,7 +86,7 @@ __global__ void k_walk(DevConst dc, double rfac, const phsph_kdnode *__restrict_
   const double xp0 = nd[c].xcen[0], xp1 = nd[c].xcen[1], xp2 = nd[c].xcen[2];
   const double xsizei = nd[c].size;
   // rcut = radkern*hmax (get_cell_location, neigh_kdtree.f90:363)
-  const double rcuti = xmul(xmul(rfac, dc.radkern), nd[c].hmax);
+  const double rcuti = xmul(xmul(rfac, dc.radkern), lhmax ? lhmax[l] : nd[c].hmax);
   int stack[64];
   int ns = 1;
   stack[0] = 1;
@@ -782,17 +803,27 @@ int dens_particle_stats(phsph_ctx *c, int32_t *d_nneigh, int32_t *d_nits) {
 }
 
 // trial neighbours of the reference's per-leaf walk at radkern*hmax, over
-// the active particles of the last densityiterate (computed on request)
+// the active particles of the last densityiterate (computed on request).
+// The hmax is that of the leaf's last neighbour list: the tree's, or the
+// one the relists of the iteration raised it to (dens.F90:338-373, the
+// count of the last get_neighbour_list is the cell's nneightry,
+// dens.F90:1267).
 int dens_trial_stats(phsph_ctx *c) {
   if (!c->trial_stale) return PHSPH_OK;
   hipStream_t s = c->stream;
   PH_HIPCHECK(c, c->ntrial.ensure(c->nleaves + 1));
+  PH_HIPCHECK(c, c->trial_hmax.ensure(c->nleaves + 1));
   int32_t zero = 0;
   PH_HIPCHECK(c, dev_set(c, c->counters.p, &zero, sizeof(int32_t)));
+  const bool replay = c->dens_icall != 0 && c->htraj.p != nullptr;
+  if (c->nleaves > 0 && replay)
+    hipLaunchKernelGGL(k_trial_hmax, dim3(nblocks(c->nleaves, 128)), dim3(128), 0, s, c->nleaves, c->leaves.p,
+                       c->leaf_flag.p, c->nd_lo.p, c->nd_hi.p, c->nd.p, c->p_nits.p, c->htraj.p, c->hconv.p,
+                       c->trial_hmax.p, c->counters.p);
   if (c->nleaves > 0)
     hipLaunchKernelGGL(k_walk, dim3(nblocks(c->nleaves, 128)), dim3(128), 0, s, c->dc, 1.0, c->nd.p,
-                       c->leaf_flag.p, c->nd_lo.p, c->nd_hi.p, c->leaves.p, c->nleaves, c->ntrial.p,
-                       c->counters.p);
+                       c->leaf_flag.p, c->nd_lo.p, c->nd_hi.p, c->leaves.p, c->nleaves,
+                       replay ? c->trial_hmax.p : nullptr, c->ntrial.p, c->counters.p);
   PH_HIPCHECK(c, hipGetLastError());
   PH_HIPCHECK(c, c->stats_i.ensure(8));
   long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -806,6 +837,8 @@ int dens_trial_stats(phsph_ctx *c) {
   PH_HIPCHECK(c, hipMemcpyAsync(&fl, c->counters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   PH_HIPCHECK(c, hipStreamSynchronize(s));
   if (fl & DFL_CAND) PH_RET(c, PHSPH_ERR_CAPACITY, "neighbour statistics: walk stack overflow");
+  if (fl & DFL_NOCONV)
+    PH_RET(c, PHSPH_ERR_CAPACITY, "neighbour statistics: a cell iterated more often than the h record holds");
   if (c->nact_dens > 0) {
     c->trialmean = (double)acc[2] / (double)c->nact_dens;
     c->maxtrial = (int)acc[4];
@@ -1074,10 +1107,11 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   DensIn in{c->tp4.p, c->tv4.p, have_f ? c->tf4.p : nullptr, c->tB4.p};
   DensOut out{a.xyzh, a.ldx, a.divcurlv, a.lddcv, a.divcurlB, a.alphaind, a.gradh, a.ldgradh, a.dvdx};
   const bool grav = c->p.gravity != 0;
-  if (grav) {
-    PH_HIPCHECK(c, c->htraj.ensure((size_t)n * HTRAJ + 1));
-    PH_HIPCHECK(c, c->hconv.ensure(n + 1));
-  }
+  // h after every Newton update: the relists of the reference are replayed
+  // from it (node hmax for the gravity walk, trial-neighbour statistics)
+  PH_HIPCHECK(c, c->htraj.ensure((size_t)n * HTRAJ + 1));
+  PH_HIPCHECK(c, c->hconv.ensure(n + 1));
+  c->dens_icall = icall;
   hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(c->nleaves, 256)), dim3(256), 0, s, c->leaf_nits.p, c->nleaves, 0);
   hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(n, 256)), dim3(256), 0, s, c->p_nits.p, n, -1);
   hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(n, 256)), dim3(256), 0, s, c->p_nneigh.p, n, -1);
@@ -1138,8 +1172,7 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
                      c->pleaf.p, c->leaves.p, c->gcand.p, c->gncand.p, c->goff_p, c->caps.gcand, c->nd_lo.p,      \
                      c->nd_hi.p, c->nd.p, c->tp4.p, widen ? c->hmax0.p : nullptr, lmargin, LB.p, dlist,             \
                      TNL.p, widen ? c->t_nlj.p : nullptr, THS.p, THF.p,                                             \
-                     c->p_nits.p, c->leaf_nits.p, c->tmpi.p, grav ? c->htraj.p : nullptr,                       \
-                     grav ? c->hconv.p : nullptr, c->counters.p)
+                     c->p_nits.p, c->leaf_nits.p, c->tmpi.p, c->htraj.p, c->hconv.p, c->counters.p)
       if (c->p.kernel == PHSPH_KERNEL_CUBIC) DITER(PHSPH_KERNEL_CUBIC); else DITER(PHSPH_KERNEL_QUINTIC);
 #undef DITER
     }

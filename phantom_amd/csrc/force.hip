@@ -102,7 +102,7 @@ __global__ void k_force_prep(DevConst dc, int nlive, const int32_t *__restrict__
     const double e = dustfrac[o];
     D.eps = e;
     D.sqeps = sqrt(e / (1. - e));
-    double tsj = get_ts_d(dc, rhoj * (1. - e), rhoj * e);
+    double tsj = get_ts_d(dc, rhoj * (1. - e), rhoj * e, csj);
     if (dc.ilimitdustflux) tsj = fmin(tsj, hj / csj);
     D.ts = tsj;
     D.pad = 0.;
@@ -212,7 +212,7 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *_
   double epsi = 0., tsi = 0., epstsi = 0., sqepsi = 0.;
   if (DUST) {
     epsi = recD[p].eps;
-    tsi = get_ts_d(dc, rhoi * (1. - epsi), rhoi * epsi);
+    tsi = get_ts_d(dc, rhoi * (1. - epsi), rhoi * epsi, csi);
     if (dc.ilimitdustflux) tsi = fmin(tsi, hi / csi);
     epstsi = epsi * tsi;
     sqepsi = sqrt(epsi / (1. - epsi));

@@ -83,31 +83,12 @@ __global__ void k_hmax_replay(int nleaves, const int32_t *__restrict__ leaves, c
   if (l >= nleaves) return;
   const int c = leaves[l];
   if (leaf_flag[c] <= 0) return;
-  const int lo = nd_lo[c], hi = nd_hi[c];
-  int nits = 0;
-  for (int p = lo; p <= hi; p++) nits = max(nits, p_nits[p]);
-  if (nits <= 1) return;  // converged at the first iteration: no relist
-  if (nits - 1 > HTRAJ) {
+  double hset = 0.;
+  const int set = leaf_relist_hmax(nd_lo[c], nd_hi[c], __longlong_as_double((long long)ghmax[c]), p_nits, htraj,
+                                   hconv, &hset);
+  if (set < 0) {
     atomicOr(flags, GFL_TRAJ);
     return;
-  }
-  double hcell = __longlong_as_double((long long)ghmax[c]);
-  bool set = false;
-  double hset = 0.;
-  for (int k = 1; k < nits; k++) {  // not yet converged after iteration k
-    double hmx = 0.;
-    for (int p = lo; p <= hi; p++) {
-      const int np_ = p_nits[p];
-      if (np_ < 1) continue;
-      const double v = (k < np_) ? htraj[(size_t)p * HTRAJ + k - 1] : hconv[p];
-      hmx = fmax(hmx, v);
-    }
-    const double hm = 1.01 * hmx;
-    if (hm > hcell) {
-      set = true;
-      hset = hm;
-    }
-    hcell = hm;
   }
   if (!set) return;
   const unsigned long long v = (unsigned long long)__double_as_longlong(hset);

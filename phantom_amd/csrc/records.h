@@ -36,8 +36,35 @@ struct __align__(16) FRecD {
   double eps, sqeps, ts, pad;
 };
 
-// get_ts (dust.f90:161-276), idrag 2 (constant K) and 3 (constant t_s)
-__device__ __forceinline__ double get_ts_d(const DevConst &dc, double rhogas, double rhodust) {
+// get_ts (dust.f90:161-276) with dv2 = 0, as the one-fluid force calls it
+// (force.F90:1793): idrag 1 (Epstein or Stokes by the Knudsen number, no
+// supersonic correction at dv2 = 0), 2 (constant K) and 3 (constant t_s)
+__device__ __forceinline__ double get_ts_d(const DevConst &dc, double rhogas, double rhodust, double spsoundgas) {
+  if (dc.idrag == 1) {
+    const double rhosum = rhogas + rhodust;
+    const double lambda = dc.seff / rhogas;
+    const double kn_eff = (dc.grainsize > 0.) ? 9. * lambda / (4. * dc.grainsize) : PH_HUGE64;
+    double dragcoeff, f;
+    if (kn_eff >= 1.) {  // Epstein
+      dragcoeff = (dc.graindens > PH_TINY64) ? dc.coeff_gei_1 * spsoundgas / (dc.graindens * dc.grainsize) : PH_HUGE64;
+      f = 1.;
+    } else {  // Stokes at |dv| = 0: Re_dust = 0 (NaN when the viscosity is 0)
+      const double viscmol_nu = dc.cste_mu * lambda * spsoundgas;
+      const double re_dust = 2. * dc.grainsize * 0. / viscmol_nu;
+      if (re_dust <= 1.) {
+        dragcoeff = 4.5 * viscmol_nu / (dc.graindens * dc.grainsize * dc.grainsize);
+        f = 1.;
+      } else if (re_dust <= 800.) {
+        dragcoeff = 9. / (dc.graindens * dc.grainsize * pow(re_dust, 0.6));
+        f = 0.;
+      } else {
+        dragcoeff = 0.163075 / (dc.graindens * dc.grainsize);
+        f = 0.;
+      }
+    }
+    const double ts1 = (dragcoeff == PH_HUGE64) ? PH_HUGE64 : dragcoeff * f * rhosum;
+    return (ts1 > 0.) ? 1. / ts1 : PH_HUGE64;
+  }
   if (dc.idrag == 2) {
     if (dc.K_code > 0.) return rhogas * rhodust / (dc.K_code * (rhogas + rhodust));
     return PH_HUGE64;

@@ -115,6 +115,40 @@ __device__ __forceinline__ void sep_of(const DevConst &dc, int code, double xi, 
   dz = min_image(dz, dc.dzbound, 0.5 * dc.dzbound);
 }
 
+// The hmax a leaf ends densityiterate with where the reference lists its
+// neighbours again (compute_hmax, dens.F90:1275-1289, then set_hmaxcell):
+// the cell's iterations are replayed from each particle's h after every
+// Newton update (htraj, HTRAJ per particle) and at convergence (hconv); the
+// cell's hmax after iteration k is 1.01 max h, and a relist happens when it
+// exceeds the previous hmax while the cell is not yet converged.  Returns 1
+// with *hset the hmax of the last relist, 0 when the leaf is never relisted,
+// -1 when a cell needed more than HTRAJ + 1 iterations.
+__device__ __forceinline__ int leaf_relist_hmax(int lo, int hi, double hcell, const int32_t *__restrict__ p_nits,
+                                                const double *__restrict__ htraj, const double *__restrict__ hconv,
+                                                double *hset) {
+  int nits = 0;
+  for (int p = lo; p <= hi; p++) nits = max(nits, p_nits[p]);
+  if (nits <= 1) return 0;  // converged at the first iteration: no relist
+  if (nits - 1 > HTRAJ) return -1;
+  int set = 0;
+  for (int k = 1; k < nits; k++) {  // not yet converged after iteration k
+    double hmx = 0.;
+    for (int p = lo; p <= hi; p++) {
+      const int np_ = p_nits[p];
+      if (np_ < 1) continue;
+      const double v = (k < np_) ? htraj[(size_t)p * HTRAJ + k - 1] : hconv[p];
+      hmx = fmax(hmx, v);
+    }
+    const double hm = 1.01 * hmx;
+    if (hm > hcell) {
+      set = 1;
+      *hset = hm;
+    }
+    hcell = hm;
+  }
+  return set;
+}
+
 // host side (density.hip)
 // Candidate leaves of each group of 64 active particles, into c->gcand at
 // c->goff_p (exact per-group offsets after an overflow, else nullptr and the

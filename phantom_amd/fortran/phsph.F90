@@ -31,6 +31,7 @@ module phsph
     real(c_double)     :: alpha, alphamax, alphau, alphaB, beta
     real(c_double)     :: C_cour, C_force, dtmax, psidecayfac, overcleanfac, K_code
     real(c_double)     :: tree_accuracy
+    real(c_double)     :: grainsize, graindens, seff
  end type phsph_params
 
  ! type(kdnode) of a non-GRAVITY build (dtype_kdtree.F90:53-68)

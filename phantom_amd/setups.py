@@ -249,6 +249,46 @@ def with_bins(setup, nbinmax=3, istepfrac=2, seed=5, dtmax=0.016):
     return s
 
 
+def with_levels(setup, levels, nbinmax, istepfrac, seed=5, dtmax=0.016):
+    """Block-timestep version of a setup with a chosen population per bin:
+    levels = [(ibin, fraction), ...] (fractions summing to 1) assigns each
+    bin round(fraction * n) particles drawn by a seeded permutation, so the
+    active set of substep istepfrac (mod(istepfrac, 2**(nbinmax - ibin))
+    == 0, utils_indtimesteps.f90:114-185) has an exact, spatially random
+    fraction of the particles.  ibin_wake = 0; iphase and ibinnow as
+    set_active_particles leaves them."""
+    s = dict(setup)
+    p = s["params"]
+    p.ind_timesteps = 1
+    p.dtmax = dtmax
+    n = s["xyzh"].shape[0]
+    perm = np.random.default_rng(seed).permutation(n)
+    ibin_old = np.zeros(n, np.int8)
+    start = 0
+    for k, (b, frac) in enumerate(levels):
+        stop = n if k == len(levels) - 1 else start + int(round(frac * n))
+        ibin_old[perm[start:stop]] = b
+        start = stop
+    active = (istepfrac % (2 ** (nbinmax - ibin_old.astype(np.int64)))) == 0
+    s["iphase"] = np.where(active, 1, -1).astype(np.int8)
+    ibinnow = nbinmax
+    for k in range(nbinmax):
+        if istepfrac % 2 ** (nbinmax - k) == 0:
+            ibinnow = k
+            break
+    s["bins"] = dict(ibin=ibin_old.copy(), ibin_old=ibin_old, ibin_wake=np.zeros(n, np.int8), nbinmax=nbinmax,
+                     istepfrac=istepfrac, ibinnow=ibinnow)
+    return s
+
+
+# Four bins whose active sets are 1, 1/2, 1/8 and 1/64 of the particles at
+# substeps istepfrac = 64, 32, 8 and 1 of a nbinmax = 6 hierarchy.
+LEVELS_4 = [(0, 32 / 64), (1, 24 / 64), (3, 7 / 64), (6, 1 / 64)]
+# Ten bins 0..9 holding 2^-(k+1) of the particles (bin 9: 2^-9): the active
+# fraction at istepfrac = 2^m of a nbinmax = 9 hierarchy is 2^-(9-m).
+LEVELS_10 = [(k, 2.0 ** -(k + 1)) for k in range(9)] + [(9, 2.0 ** -9)]
+
+
 def lane_emden(nindex, dxi=1e-4):
     """Lane-Emden solution theta(xi) for polytropic index nindex, integrated
     with RK4 from the centre to the first zero xi1.  Returns (xi, theta,

@@ -36,7 +36,7 @@ EXPORTED = [
     "phsph_set_bins", "phsph_get_nbinmax", "phsph_set_active_particles", "phsph_set_poten",
     "phsph_get_quads", "phsph_init_step", "phsph_step_predictor", "phsph_substep_sph", "phsph_step_predict_sph",
     "phsph_step_corrector", "phsph_step_reset", "phsph_get_precision", "phsph_density_vjp", "phsph_force_vjp",
-    "phsph_step_vjp",
+    "phsph_step_vjp", "phsph_get_tree_grav",
 ]
 
 _lib = None
@@ -84,6 +84,7 @@ def load_library(path=LIB_PATH):
     L.phsph_get_nbinmax.argtypes = [vp, P(i32)]
     L.phsph_set_poten.argtypes = [vp, vp]
     L.phsph_get_quads.argtypes = [vp, vp, i64]
+    L.phsph_get_tree_grav.argtypes = [vp, vp, i64]
     L.phsph_set_active_particles.argtypes = [vp, i32, vp, i32, vp, vp, i32, i32, P(i32), P(i32), P(i32)]
     L.phsph_init_step.argtypes = [vp, i32, dbl, vp, i32, vp]
     L.phsph_step_predictor.argtypes = [vp, i32, dbl, dbl, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp,
@@ -197,6 +198,17 @@ class Context:
                                             inodeparts.ctypes.data_as(C.c_void_p), nc, C.byref(ml)))
         nlive = int(np.count_nonzero(inodeparts))
         return dict(node=node, inoderange=inoderange, inodeparts=inodeparts[:nlive], maxlevel=ml.value, ncells=nc)
+
+    def get_tree_grav(self):
+        """node(1:ncells) in the GRAVITY layout of type(kdnode) (with
+        tobecached, cached and quads) as a numpy structured array."""
+        nodes = (abi.KdNodeGrav * max(self._ncells, 1))()
+        self._check(self.lib.phsph_get_tree_grav(self._h, nodes, self._ncells))
+        dt = np.dtype([("xcen", "<f8", (3,)), ("size", "<f8"), ("hmax", "<f8"), ("mass", "<f8"),
+                       ("leftchild", "<i4"), ("rightchild", "<i4"), ("parent", "<i4"), ("idum", "<i4"),
+                       ("tobecached", "<i4"), ("cached", "<i4"), ("quads", "<f8", (6,))])
+        assert dt.itemsize == C.sizeof(abi.KdNodeGrav)
+        return np.frombuffer(nodes, dtype=dt, count=self._ncells).copy()
 
     def get_quads(self):
         """node(1:ncells)%quads: second moments about each node's centre of

@@ -48,6 +48,16 @@ def rel_rms(a, b):
     return float(np.max(np.abs(a - b)) / scale)
 
 
+def rel_err(a, b):
+    """max over particles of |a - b| / max(|b|, rms(b)): the per-particle
+    relative error, measured against the rms where b passes through zero."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = np.maximum(np.abs(b), np.sqrt(np.mean(b * b)))
+    scale = np.where(scale > 0.0, scale, 1.0)
+    return float(np.max(np.abs(a - b) / scale))
+
+
 def reachable_nodes(tree):
     """Indices (0-based) of the nodes reachable from the root."""
     node = tree["node"]

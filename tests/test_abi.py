@@ -38,7 +38,7 @@ def test_params_struct_matches_header():
         if m:
             fields += [f.strip() for f in m.group(2).split(",")]
     assert fields == [n for n, _ in abi.Params._fields_]
-    assert C.sizeof(abi.Params) == 16 * 4 + 23 * 8
+    assert C.sizeof(abi.Params) == 16 * 4 + 26 * 8
 
 
 @pytest.mark.parametrize("kernel", [abi.KERNEL_CUBIC, abi.KERNEL_QUINTIC])

@@ -1,0 +1,198 @@
+"""Parity at the sizes of the benchmark configurations, GPU against the
+oracle on identical particles:
+
+* the 64^3 isothermal box (262,144 particles, cubic kernel), with
+  icall = 1 and with icall = 0 (one density pass, no iteration);
+* the 128^3 MHD + one-fluid dust box (2,097,152 particles) on block
+  timesteps, at the substeps where 1, 1/2, 1/8 and 1/64 of the particles
+  are active (four bins of a nbinmax = 6 hierarchy);
+* the self-gravitating n = 1.5 polytrope with about 1M particles at
+  tree_accuracy 0.5, on ten bins, at the substeps where 1, 2^-3 and 2^-9 of
+  the particles are active;
+* mixed precision against the FP64 oracle on the 1,097,505-particle shock
+  tube and the 8,388,608-particle adiabatic box (and the FP64 GPU path
+  against the oracle on the same box).
+
+Tolerances: neighbour counts, iteration counts, neighbour statistics
+(trial and actual), bins and wake flags bit-exact; FP64 h within 1e-12 of
+its rms, acceleration components within 1e-12 of the rms of |f|; du/dt,
+dB/dt, dε/dt and Δv within 1e-12 relative per particle (relative to the rms
+where the quantity passes through zero); float32 outputs to float
+rounding.  Mixed
+precision: rho within 1e-6 relative per particle, dv/dt within 1e-6 of the
+rms of |dv/dt|, counts bit-exact; the errors are printed.  Only active
+particles are compared under block timesteps (inactive particles keep their
+inputs)."""
+import numpy as np
+import pytest
+
+from phantom_amd import abi, setups
+from tests import parity
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+TOL_MIXED = 1e-6
+
+
+def _moving(s, seed=7, sigma=0.1):
+    rng = np.random.default_rng(seed)
+    s["vxyzu"][:, :3] = rng.normal(0.0, sigma, size=(s["vxyzu"].shape[0], 3))
+    return s
+
+
+def _c3(istepfrac):
+    return lambda: setups.with_levels(setups.mhd_dust_box(nx=128, jitter=0.05), setups.LEVELS_4, nbinmax=6,
+                                      istepfrac=istepfrac)
+
+
+def _c4(istepfrac):
+    return lambda: setups.with_levels(_moving(setups.polytrope(n=1048576, theta=0.5)), setups.LEVELS_10,
+                                      nbinmax=9, istepfrac=istepfrac, dtmax=0.05)
+
+
+CASES = {
+    "C1_box64_isothermal": lambda: setups.uniform_box(nx=64),
+    "C3_mhd_dust_128_active_1": _c3(64),
+    "C3_mhd_dust_128_active_1/2": _c3(32),
+    "C3_mhd_dust_128_active_1/8": _c3(8),
+    "C3_mhd_dust_128_active_1/64": _c3(1),
+    "C4_polytrope_1M_active_1": _c4(512),
+    "C4_polytrope_1M_active_2^-3": _c4(64),
+    "C4_polytrope_1M_active_2^-9": _c4(1),
+}
+
+
+def _active(setup):
+    iph = setup.get("iphase")
+    n = setup["xyzh"].shape[0]
+    return np.ones(n, bool) if iph is None else iph > 0
+
+
+def _fscale(f):
+    return float(np.sqrt(np.mean(np.sum(f[:, :3] ** 2, axis=1))))
+
+
+def _compare(name, setup, g, o):
+    p = setup["params"]
+    act = _active(setup)
+    nact = int(act.sum())
+    if "bins" in setup:
+        frac = nact / act.size
+        print(f"[{name}] active {nact} of {act.size} ({frac:.6f})", flush=True)
+    # counts and statistics
+    np.testing.assert_array_equal(g["nneigh"][act], o["dens"]["nneigh"][act])
+    np.testing.assert_array_equal(g["nits"][act], o["dens"]["nits"][act])
+    assert np.all(g["nneigh"][~act] == 0)
+    gs, os_ = g["stats"], o["stats"]
+    for key in ("nactualtot", "maxactual", "nrhocalc", "maxtrial"):
+        assert gs[key] == os_[key], key
+    assert gs["trialmean"] == pytest.approx(os_["trialmean"], rel=1e-14)
+    # density
+    assert parity.rel_rms(g["xyzh"][act, 3], o["xyzh"][act, 3]) <= TOL
+    for key in ("gradh", "divcurlv", "dvdx"):
+        a, b = np.asarray(g["dens"][key], np.float64)[act], np.asarray(o["dens"][key], np.float64)[act]
+        assert parity.rel_rms(a, b) <= 2e-7, key
+    # forces
+    gf, of = g["force"], o["force"]
+    fg, fo = np.asarray(gf["fxyzu"], np.float64)[act], np.asarray(of["fxyzu"], np.float64)[act]
+    fs = _fscale(fo)
+    errs = [float(np.max(np.abs(fg[:, c] - fo[:, c])) / fs) for c in range(3)]
+    print(f"[{name}] force err {max(errs):.2e} of rms|f|", flush=True)
+    assert max(errs) <= TOL
+    # per-particle relative errors, against the rms where the quantity
+    # passes through zero (du/dt with heating and cooling terms, dB/dt, ...)
+    if p.maxvxyzu >= 4:
+        e = parity.rel_err(fg[:, 3], fo[:, 3])
+        print(f"[{name}] du/dt err {e:.2e} (max/rms {parity.rel_rms(fg[:, 3], fo[:, 3]):.2e})", flush=True)
+        assert e <= TOL
+    if p.mhd:
+        for c in range(4):
+            assert parity.rel_err(gf["dBevol"][act, c], of["dBevol"][act, c]) <= TOL, ("dBevol", c)
+        assert parity.rel_rms(gf["divBsymm"][act], of["divBsymm"][act]) <= 2e-7
+    if p.use_dustfrac:
+        assert parity.rel_err(gf["ddustevol"][act], of["ddustevol"][act]) <= TOL
+        for c in range(3):
+            assert parity.rel_err(gf["deltav"][act, c], of["deltav"][act, c]) <= TOL
+    if p.gravity:
+        assert parity.rel_rms(gf["poten"][act], of["poten"][act]) <= 2e-7
+    if "bins" in setup:
+        np.testing.assert_array_equal(gf["ibin"], of["ibin"])
+        np.testing.assert_array_equal(gf["ibin_wake"], of["ibin_wake"])
+        assert gf["nbinmax"] == of["nbinmax"]
+    else:
+        assert abs(gf["dtcourant"] - of["dtcourant"]) <= 1e-12 * of["dtcourant"]
+        assert abs(gf["dtforce"] - of["dtforce"]) <= 1e-12 * of["dtforce"]
+        if not p.mhd:
+            f = np.asarray(gf["fxyzu"])[:, :3]
+            assert np.abs(f.sum(axis=0)).max() <= 1e-12 * np.abs(f).sum(axis=0).max()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_config_parity(name, gpu):
+    setup = CASES[name]()
+    print(f"\n[case {name}] n={setup['xyzh'].shape[0]}", flush=True)
+    g = parity.run_gpu(setup)
+    g["ctx"].close()
+    o = parity.run_oracle(setup)
+    _compare(name, setup, g, o)
+
+
+def test_box64_icall0(gpu):
+    """densityiterate(icall=0): one summation at the entry h, h stored from
+    the Newton update of that summation (dens.F90:341)."""
+    setup = setups.uniform_box(nx=64)
+    print("\n[case C1_box64_icall0] n=262144", flush=True)
+    g = parity.run_gpu(setup, icall=0)
+    g["ctx"].close()
+    o = parity.run_oracle(setup, icall=0)
+    np.testing.assert_array_equal(g["nneigh"], o["dens"]["nneigh"])
+    assert np.all(g["nits"] == 1)
+    assert g["stats"]["nrhocalc"] == o["stats"]["nrhocalc"]
+    assert parity.rel_rms(g["xyzh"][:, 3], o["xyzh"][:, 3]) <= TOL
+    fg, fo = np.asarray(g["force"]["fxyzu"]), np.asarray(o["force"]["fxyzu"])
+    assert np.max(np.abs(fg[:, :3] - fo[:, :3])) <= TOL * _fscale(fo)
+
+
+def _mixed_errors(name, g, o):
+    """max relative error of rho (rho = m (hfact/h)^3) per particle and max
+    error of each dv/dt component over the rms of |dv/dt|"""
+    h_g, h_o = g["xyzh"][:, 3], o["xyzh"][:, 3]
+    rho_g, rho_o = h_g ** -3.0, h_o ** -3.0
+    e_rho = float(np.max(np.abs(rho_g - rho_o) / rho_o))
+    fg, fo = np.asarray(g["force"]["fxyzu"]), np.asarray(o["force"]["fxyzu"])
+    e_f = float(np.max(np.abs(fg[:, :3] - fo[:, :3])) / _fscale(fo))
+    e_u = parity.rel_rms(fg[:, 3], fo[:, 3]) if fo.shape[1] > 3 else 0.0
+    print(f"[{name}] mixed vs FP64 oracle: rho max rel {e_rho:.3e}, dv/dt max/rms {e_f:.3e}, "
+          f"du/dt max/rms {e_u:.3e}", flush=True)
+    return e_rho, e_f
+
+
+@pytest.mark.parametrize("name", ["shock_tube_1M", "hydro_box_8M"])
+def test_mixed_full_size(name, gpu):
+    if name == "shock_tube_1M":
+        mk = lambda: setups.shock_tube()  # noqa: E731
+    else:
+        mk = lambda: setups.uniform_box(nx=256, ny=256, nz=128, box=(-0.5, 0.5, -0.5, 0.5, -0.25, 0.25),  # noqa
+                                        isothermal=False, seed=8)
+    setup = mk()
+    n = setup["xyzh"].shape[0]
+    print(f"\n[case mixed_{name}] n={n}", flush=True)
+    o = parity.run_oracle(setup)
+    if name == "hydro_box_8M":
+        # the FP64 path at this size as well
+        g64 = parity.run_gpu(setup)
+        g64["ctx"].close()
+        _compare("fp64_" + name, setup, g64, o)
+        del g64
+    sm = mk()
+    sm["params"].precision = abi.PREC_MIXED
+    gm = parity.run_gpu(sm)
+    prec = gm["ctx"].get_precision()
+    gm["ctx"].close()
+    assert prec == (abi.PREC_MIXED, abi.PREC_MIXED)
+    np.testing.assert_array_equal(gm["nneigh"], o["dens"]["nneigh"])
+    np.testing.assert_array_equal(gm["nits"], o["dens"]["nits"])
+    e_rho, e_f = _mixed_errors(name, gm, o)
+    assert e_rho <= TOL_MIXED
+    assert e_f <= TOL_MIXED

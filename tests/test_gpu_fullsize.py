@@ -53,6 +53,8 @@ def test_counts(full):
     gs, os_ = g["stats"], o["stats"]
     assert gs["nactualtot"] == os_["nactualtot"]
     assert gs["nrhocalc"] == os_["nrhocalc"]
+    assert gs["maxtrial"] == os_["maxtrial"]
+    assert gs["trialmean"] == pytest.approx(os_["trialmean"], rel=1e-14)
 
 
 def test_density(full):

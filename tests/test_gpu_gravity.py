@@ -105,6 +105,24 @@ def test_quads(pair):
     assert np.abs(g["quads"][used] - o["quads"][used]).max() <= 1e-13 * scale
 
 
+def test_gravity_node_layout(pair):
+    """node(:) of a GRAVITY build: the non-GRAVITY fields of get_tree, quads
+    of get_quads, tobecached = 1 and cached = .false. (kdtree.F90:788-789)."""
+    name, setup, g, o = pair
+    ctx = sph.Context(setup["params"])
+    xyzh = setup["xyzh"].copy()
+    ctx.maketree(xyzh, iphase=setup.get("iphase"))
+    tree = ctx.get_tree()
+    quads = ctx.get_quads()
+    ng = ctx.get_tree_grav()
+    ctx.close()
+    used = parity.reachable_nodes(tree)
+    for f in ("xcen", "size", "hmax", "mass", "leftchild", "rightchild", "parent"):
+        np.testing.assert_array_equal(ng[f][used], tree["node"][f][used], err_msg=f)
+    np.testing.assert_array_equal(ng["quads"][used], quads[used])
+    assert np.all(ng["tobecached"] == 1) and np.all(ng["cached"] == 0)
+
+
 def test_density_and_node_hmax(pair):
     """h, neighbour and iteration counts as without gravity; the node hmax
     after densityiterate follows the relists of every cell (set_hmaxcell)."""

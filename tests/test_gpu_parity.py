@@ -19,7 +19,22 @@ def _cases():
         "shock_tube_ny8_quintic": lambda: setups.shock_tube(ny=8),
         "mhd_dust_16": lambda: setups.mhd_dust_box(nx=16, jitter=0.05),
         "random_8k": lambda: setups.random_box(n=8000, seed=3),
+        "mhd_dust_16_epstein_stokes": lambda: _epstein_stokes(setups.mhd_dust_box(nx=16, jitter=0.05)),
     }
+
+
+def _epstein_stokes(s):
+    """Physical drag (idrag = 1) with the mean free path chosen so that the
+    Knudsen number 9 lambda / (4 s) crosses 1 inside the box's range of gas
+    densities: both the Epstein and the Stokes regime are taken."""
+    p = s["params"]
+    p.idrag = 1
+    p.grainsize = 1e-3
+    p.graindens = 3.0
+    eps = s["dustevol"] ** 2 / (1.0 + s["dustevol"] ** 2)
+    rhogas_mid = float(np.median(p.massoftype_gas * (p.hfact / s["xyzh"][:, 3]) ** 3 * (1.0 - eps)))
+    p.seff = 4.0 / 9.0 * p.grainsize * rhogas_mid
+    return s
 
 
 def _lattice(nx):
@@ -64,6 +79,9 @@ def test_neighbour_counts_bit_exact(pair):
     assert gs["nactualtot"] == os_["nactualtot"]
     assert gs["maxactual"] == os_["maxactual"]
     assert gs["nrhocalc"] == os_["nrhocalc"]
+    # trial neighbours of each cell's last neighbour list (dens.F90:1137-1155)
+    assert gs["maxtrial"] == os_["maxtrial"]
+    assert gs["trialmean"] == pytest.approx(os_["trialmean"], rel=1e-14)
 
 
 def test_density_fp64(pair):

@@ -500,3 +500,46 @@ def test_fmm_two_stars_momentum():
     # the stars attract each other: a force of order G m1 m2 / d^2 per star
     star1 = r["xyzh"][:, 0] < 10.0
     assert abs(m * f[star1, 0].sum() - 0.25 / 400.0) < 0.05 * 0.25 / 400.0
+
+
+# ------------------------------------------------------- Epstein / Stokes drag
+def test_epstein_stokes_stopping_time():
+    """idrag = 1 (dust.f90:188-245) with the one-fluid dv2 = 0: on a box
+    whose gas densities put the Knudsen number 9 lambda/(4 s) (lambda =
+    seff/rho_gas) on both sides of 1,
+
+    * Epstein particles follow the textbook law t_s = rho_grain s / (rho
+      v_th) with v_th = sqrt(8/(pi gamma)) c_s: t_s rho c_s is one constant;
+    * Stokes particles have t_s = rho_grain s^2 / (4.5 nu rho) with the
+      molecular viscosity nu proportional to lambda c_s, so t_s rho c_s /
+      rho_gas is one constant (Stokes drag does not depend on the gas
+      density at fixed c_s);
+    * the two laws meet at Kn = 1: the Epstein constant equals the Stokes
+      constant times the switching gas density 9 seff / (4 s)."""
+    s = setups.mhd_dust_box(nx=14, jitter=0.05)
+    p = s["params"]
+    p.mhd = 0
+    s.pop("Bevol")
+    p.idrag = 1
+    p.grainsize = 1e-3
+    p.graindens = 3.0
+    eps0 = s["dustevol"] ** 2 / (1.0 + s["dustevol"] ** 2)
+    rho_est = p.massoftype_gas * (p.hfact / s["xyzh"][:, 3]) ** 3
+    p.seff = 4.0 / 9.0 * p.grainsize * float(np.median(rho_est * (1.0 - eps0)))
+    r = oracle.run_all(s)
+    h = r["xyzh"][:, 3]
+    rho = p.massoftype_gas * (p.hfact / h) ** 3
+    eps = r["c2p"]["dustfrac"]
+    rhog = rho * (1.0 - eps)
+    cs = r["c2p"]["eos_vars"][:, 1]
+    ts = r["force"]["tstop"]
+    kn = 9.0 * (p.seff / rhog) / (4.0 * p.grainsize)
+    ep, st = kn >= 1.0, kn < 1.0
+    assert ep.sum() > 100 and st.sum() > 100
+    c_ep = ts[ep] * rho[ep] * cs[ep]
+    c_st = ts[st] * rho[st] * cs[st] / rhog[st]
+    assert np.ptp(c_ep) <= 1e-12 * c_ep.mean()
+    assert np.ptp(c_st) <= 1e-12 * c_st.mean()
+    v_th = math.sqrt(8.0 / (abi.PI_PH * p.gamma))
+    assert c_ep.mean() == pytest.approx(p.graindens * p.grainsize / v_th, rel=1e-12)
+    assert c_ep.mean() == pytest.approx(c_st.mean() * 9.0 * p.seff / (4.0 * p.grainsize), rel=1e-12)
