@@ -400,6 +400,31 @@ int phsph_step_corrector(phsph_ctx *ctx, int32_t np, double timei,
                          int32_t *nvfloor);
 
 /*
+ * phsph_step_corrector without allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during the
+ * call (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, results and errors as
+ * phsph_step_corrector, except that every array must already be device
+ * memory (PHSPH_ERR_INPUT otherwise: nothing is staged) and np is at most
+ * 16,777,216 (PHSPH_ERR_CAPACITY above).  The reduction partials live in a
+ * fixed 2 MiB device array of the library, one per GPU, held until the
+ * process ends and looked up by phsph_create; the call makes no hipMalloc
+ * or hipFree and records no kernel timings.  Calls that overlap on one GPU
+ * (contexts on different streams) share that array and race.
+ */
+int phsph_step_corrector_noalloc(phsph_ctx *ctx, int32_t np, double timei,
+                                 double dtsph, const double *xyzh, int32_t ldxyzh,
+                                 double *vxyzu, int32_t ldvxyzu,
+                                 const double *fxyzu, int32_t ldfxyzu,
+                                 const double *vpred, double *Bevol,
+                                 const double *dBevol, double *dustevol,
+                                 const double *ddustevol, const int8_t *iphase,
+                                 int8_t *ibin, int8_t *ibin_wake, int32_t nbinmax,
+                                 double *twas, double ufloor, double *errmax,
+                                 double *v2sum, int32_t *npc, int32_t *nwake,
+                                 int32_t *nvfloor);
+
+/*
  * The until_converged loop of step (step_leapfrog.f90:650-696), run when
  * the velocity error asks for another iteration: vpred = v, Bpred = Bevol,
  * dustpred = dustevol (block timesteps: active particles only) and, with a

@@ -240,7 +240,9 @@ int phsph_create(const phsph_params *p, int32_t device, phsph_ctx **out) {
     return PHSPH_ERR_HIP;
   }
   *out = c;
-  return PHSPH_OK;
+  // device address of the scratch array of phsph_step_corrector_noalloc
+  // on this device (once per device, with the device current)
+  return step_scratch_init(c);
 }
 
 int phsph_destroy(phsph_ctx *c) {
@@ -763,6 +765,49 @@ int phsph_step_corrector(phsph_ctx *c, int32_t np, double timei, double dtsph, c
   rc = step_corrector_run(c, np, timei, dtsph, a, nbinmax, ufloor, errmax, v2sum, npc, nwake, nvfloor);
   int rc2 = st.finish();
   return rc ? rc : rc2;
+}
+
+int phsph_step_corrector_noalloc(phsph_ctx *c, int32_t np, double timei, double dtsph, const double *xyzh,
+                                 int32_t ldx, double *vxyzu, int32_t ldv, const double *fxyzu, int32_t ldf,
+                                 const double *vpred, double *Bevol, const double *dBevol, double *dustevol,
+                                 const double *ddustevol, const int8_t *iphase, int8_t *ibin, int8_t *ibin_wake,
+                                 int32_t nbinmax, double *twas, double ufloor, double *errmax, double *v2sum,
+                                 int32_t *npc, int32_t *nwake, int32_t *nvfloor) {
+  if (!c) return PHSPH_ERR_INPUT;
+  int rc = step_check(c, "step_corrector_noalloc", np, ldx, ldv, ldf);
+  if (rc) return rc;
+  const bool mhd = c->p.mhd, dust = c->p.use_dustfrac && c->p.ndustsmall > 0, ind = c->p.ind_timesteps;
+  if (np > 0 && (!xyzh || !vxyzu || !fxyzu || (!ind && !vpred) || (mhd && (!Bevol || !dBevol)) ||
+                 (dust && (!dustevol || !ddustevol)) || (ind && (!ibin || !ibin_wake || !twas))))
+    PH_RET(c, PHSPH_ERR_INPUT, "step_corrector_noalloc: missing array");
+  if (ind && (nbinmax < 0 || nbinmax > 30))
+    PH_RET(c, PHSPH_ERR_INPUT, "step_corrector_noalloc: nbinmax outside 0..30");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  StepArgs a;
+  memset(&a, 0, sizeof(a));
+  a.xyzh = const_cast<double *>(xyzh); a.ldx = ldx;
+  a.vxyzu = vxyzu; a.ldv = ldv;
+  a.fxyzu = fxyzu; a.ldf = ldf;
+  a.vpred = ind ? nullptr : const_cast<double *>(vpred);
+  a.Bevol = mhd ? Bevol : nullptr;
+  a.dBevol = mhd ? dBevol : nullptr;
+  a.dustevol = dust ? dustevol : nullptr;
+  a.ddustevol = dust ? ddustevol : nullptr;
+  a.iphase = iphase;
+  if (ind) {
+    a.ibin = ibin;
+    a.ibin_wake = ibin_wake;
+    a.twas = twas;
+  }
+  // no staging: every array the kernel reads or writes must be device memory
+  if (np > 0) {
+    const void *need[] = {a.xyzh, a.vxyzu, a.fxyzu, a.vpred, a.Bevol, a.dBevol, a.dustevol, a.ddustevol, a.iphase,
+                          a.ibin, a.ibin_wake, a.twas};
+    for (const void *q : need)
+      if (q && !is_device_ptr(q))
+        PH_RET(c, PHSPH_ERR_INPUT, "step_corrector_noalloc: every array must be device memory");
+  }
+  return step_corrector_noalloc_run(c, np, timei, dtsph, a, nbinmax, ufloor, errmax, v2sum, npc, nwake, nvfloor);
 }
 
 int phsph_step_reset(phsph_ctx *c, int32_t np, double dtsph, double *vxyzu, int32_t ldv, const double *fxyzu,

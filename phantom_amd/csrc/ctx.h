@@ -345,6 +345,10 @@ int step_corrector_run(phsph_ctx *c, int np, double timei, double dtsph, const S
                        double ufloor, double *errmax, double *v2sum, int32_t *npc, int32_t *nwake,
                        int32_t *nvfloor);
 int step_reset_run(phsph_ctx *c, int np, double dtsph, const StepArgs &s);
+int step_scratch_init(phsph_ctx *c);
+int step_corrector_noalloc_run(phsph_ctx *c, int np, double timei, double dtsph, const StepArgs &s, int nbinmax,
+                               double ufloor, double *errmax, double *v2sum, int32_t *npc, int32_t *nwake,
+                               int32_t *nvfloor);
 struct FRec;
 int grav_tree(phsph_ctx *c);
 int grav_hmax_replay(phsph_ctx *c, int icall);

@@ -12,6 +12,8 @@
 // state is bit-identical to a sequential evaluation.
 #include "ctx.h"
 
+#include <mutex>
+
 namespace {
 
 constexpr int SBS = 256;          // threads per block of the step kernels
@@ -412,6 +414,60 @@ int step_corrector_run(phsph_ctx *c, int np, double timei, double dtsph, const S
   if (rc) return rc;
   PH_HIPCHECK(c, hipStreamSynchronize(c->stream));
   if (res.nbad > 0) PH_RET(c, PHSPH_ERR_INPUT, "step_corrector: ibin outside 0..30");
+  if (errmax) *errmax = res.emax;
+  if (v2sum) *v2sum = res.v2;
+  if (npc) *npc = res.np;
+  if (nwake) *nwake = res.nwake;
+  if (nvfloor) *nvfloor = res.nfloor;
+  return PHSPH_OK;
+}
+
+// ---- the corrector without allocation --------------------------------------
+// Scratch memory of phsph_step_corrector_noalloc: the per-block partials of
+// the corrector's reductions and their combined result, in a fixed device
+// array instead of context buffers.  Each GPU holds its own copy for the
+// life of the process; every call on a GPU shares it.
+constexpr int NOALLOC_MAX_BLOCKS = 65536;  // np <= NOALLOC_MAX_BLOCKS * SBS = 16,777,216
+static __device__ __align__(256) StepPart scratch[NOALLOC_MAX_BLOCKS + 1];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static StepPart *g_scratch[NOALLOC_MAX_DEVICES];  // device address of scratch per device
+static std::mutex g_scratch_mu;
+
+// Looks up the device address of scratch on the current device (the
+// caller's context device), once per device; called by phsph_create.
+int step_scratch_init(phsph_ctx *c) {
+  if (c->device < 0 || c->device >= NOALLOC_MAX_DEVICES) PH_RET(c, PHSPH_ERR_INPUT, "device index beyond 63");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch[c->device]) return PHSPH_OK;
+  void *ptr = nullptr;
+  PH_HIPCHECK(c, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  g_scratch[c->device] = static_cast<StepPart *>(ptr);
+  return PHSPH_OK;
+}
+
+int step_corrector_noalloc_run(phsph_ctx *c, int np, double timei, double dtsph, const StepArgs &s, int nbinmax,
+                               double ufloor, double *errmax, double *v2sum, int32_t *npc, int32_t *nwake,
+                               int32_t *nvfloor) {
+  const int nb = np > 0 ? (int)nblocks(np, SBS) : 1;
+  if (nb > NOALLOC_MAX_BLOCKS)
+    PH_RET(c, PHSPH_ERR_CAPACITY, "step_corrector_noalloc: np above 16,777,216 (the scratch limit)");
+  StepPart *parts;
+  {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    parts = (c->device >= 0 && c->device < NOALLOC_MAX_DEVICES) ? g_scratch[c->device] : nullptr;
+  }
+  if (!parts) PH_RET(c, PHSPH_ERR_INPUT, "step_corrector_noalloc: scratch not set up on this device");
+  StepPart *out = parts + nb, res;
+  const BinTable tab = make_bin_table(c->p.dtmax, timei);
+  // every block writes its partial and k_step_reduce overwrites out, so
+  // nothing left by an earlier call is read
+  hipLaunchKernelGGL(k_corrector, dim3(nb), dim3(SBS), 0, c->stream, s, make_flags(c, ufloor, nbinmax), tab, timei,
+                     0.5 * dtsph, np, parts);
+  PH_HIPCHECK(c, hipGetLastError());
+  int rc = finish_reduce(c, nb, parts, out, &res);
+  if (rc) return rc;
+  PH_HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (res.nbad > 0) PH_RET(c, PHSPH_ERR_INPUT, "step_corrector_noalloc: ibin outside 0..30");
   if (errmax) *errmax = res.emax;
   if (v2sum) *v2sum = res.v2;
   if (npc) *npc = res.np;

@@ -36,7 +36,7 @@ EXPORTED = [
     "phsph_set_bins", "phsph_get_nbinmax", "phsph_set_active_particles", "phsph_set_poten",
     "phsph_get_quads", "phsph_init_step", "phsph_step_predictor", "phsph_substep_sph", "phsph_step_predict_sph",
     "phsph_step_corrector", "phsph_step_reset", "phsph_get_precision", "phsph_density_vjp", "phsph_force_vjp",
-    "phsph_step_vjp", "phsph_get_tree_grav",
+    "phsph_step_vjp", "phsph_step_corrector_noalloc", "phsph_get_tree_grav",
 ]
 
 _lib = None
@@ -94,6 +94,7 @@ def load_library(path=LIB_PATH):
                                          vp, i32, vp, i32, vp, vp, vp, dbl, P(i32)]
     L.phsph_step_corrector.argtypes = [vp, i32, dbl, dbl, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp,
                                        vp, i32, vp, dbl, P(dbl), P(dbl), P(i32), P(i32), P(i32)]
+    L.phsph_step_corrector_noalloc.argtypes = L.phsph_step_corrector.argtypes
     L.phsph_step_reset.argtypes = [vp, i32, dbl, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]
     for name in EXPORTED:
         if name != "phsph_default_params" and name != "phsph_last_error":
@@ -432,6 +433,24 @@ class Context:
         em, v2 = C.c_double(), C.c_double()
         npc, nw, nfl = C.c_int32(), C.c_int32(), C.c_int32()
         self._check(self.lib.phsph_step_corrector(
+            self._h, n, timei, dtsph, px, ldx, pv, ldv, pf, ldf, _ptr(vpred)[0], _ptr(Bevol)[0], _ptr(dBevol)[0],
+            _ptr(dustevol)[0], _ptr(ddustevol)[0], _ptr(iphase)[0], _ptr(ibin)[0], _ptr(ibin_wake)[0], nbinmax,
+            _ptr(twas)[0], ufloor, C.byref(em), C.byref(v2), C.byref(npc), C.byref(nw), C.byref(nfl)))
+        return dict(errmax=em.value, v2sum=v2.value, np=npc.value, nwake=nw.value, nvfloor=nfl.value)
+
+    def step_corrector_noalloc(self, timei, dtsph, xyzh, vxyzu, fxyzu, vpred=None, Bevol=None, dBevol=None,
+                               dustevol=None, ddustevol=None, iphase=None, ibin=None, ibin_wake=None, nbinmax=0,
+                               twas=None, ufloor=0.0):
+        """step_corrector through phsph_step_corrector_noalloc: every array
+        must be a device array (phantom_amd.device.DeviceArray); the library
+        allocates nothing during the call."""
+        n = xyzh.shape[0]
+        px, ldx = _ptr(xyzh)
+        pv, ldv = _ptr(vxyzu)
+        pf, ldf = _ptr(fxyzu)
+        em, v2 = C.c_double(), C.c_double()
+        npc, nw, nfl = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.lib.phsph_step_corrector_noalloc(
             self._h, n, timei, dtsph, px, ldx, pv, ldv, pf, ldf, _ptr(vpred)[0], _ptr(Bevol)[0], _ptr(dBevol)[0],
             _ptr(dustevol)[0], _ptr(ddustevol)[0], _ptr(iphase)[0], _ptr(ibin)[0], _ptr(ibin_wake)[0], nbinmax,
             _ptr(twas)[0], ufloor, C.byref(em), C.byref(v2), C.byref(npc), C.byref(nw), C.byref(nfl)))
