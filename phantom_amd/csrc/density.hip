@@ -386,9 +386,23 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
     it++;
     const double hi1 = 1. / hi, hi21 = xmul(hi1, hi1);
     double srho = 0., sgh = 0.;
+    // software pipeline: the position record of entry k+1 and the list
+    // entry k+2 are loaded while entry k is summed
+    uint32_t en = 0u, en2 = 0u;
+    double4 rn = make_double4(0., 0., 0., 0.);
+    if (nl > 0) {
+      en = nlist[list_at(nact, t, 0)];
+      rn = tp[cand_node(en)];
+    }
+    if (nl > 1) en2 = nlist[list_at(nact, t, 1)];
     for (int k = 0; k < nl; k++) {
-      const uint32_t e = nlist[list_at(nact, t, k)];
-      const double4 r = tp[cand_node(e)];
+      const uint32_t e = en;
+      const double4 r = rn;
+      if (k + 1 < nl) {
+        en = en2;
+        rn = tp[cand_node(en)];
+        if (k + 2 < nl) en2 = nlist[list_at(nact, t, k + 2)];
+      }
       double dx, dy, dz;
       sep_of(dc, cand_code(e), xi, yi, zi, r.x, r.y, r.z, dx, dy, dz);
       const double r2 = r2_exact(dx, dy, dz);

@@ -234,24 +234,31 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *_
   const int nl = live ? t_nl[t] : 0;
   const int ntot = nl + ((live && t_nlj) ? t_nlj[t] : 0);
   int npair = 0;  // interacting pairs (entries that pass the neighbour test)
-  // entry kk+1 and its position record are read while entry kk is summed
+  // Hydro: the whole 128-byte record of entry kk+1 (its first 32 bytes are
+  // the position record) and list entry kk+2 are read while entry kk is
+  // summed, so the record of a pair is in registers when it is needed.
+  // MHD and dust (more registers live): entry kk+1's 32-byte position
+  // record ahead, the full record after the neighbour test.
+  constexpr bool WHOLE = !MHD && !DUST;
   auto slot = [&](int kk) { return kk < nl ? kk : capb - 1 - (kk - nl); };
   uint32_t en = 0u, en2 = 0u;
   double4 pn = make_double4(0., 0., 0., 0.);
+  FRec Rn;
   if (ntot > 0) {
     en = nlist[list_at_g(loff, nact, t, slot(0))];
-    pn = fpos[cand_node(en)];
+    if constexpr (WHOLE) Rn = rec[cand_node(en)];
+    else pn = fpos[cand_node(en)];
   }
   if (ntot > 1) en2 = nlist[list_at_g(loff, nact, t, slot(1))];
   for (int kk = 0; kk < ntot; kk++) {
     const uint32_t e = en;
     const int j = cand_node(e);
-    // the neighbour test on the 32-byte position record (x, y, z, 1/h); the
-    // full record is read for neighbours only
-    const double4 pj = pn;
+    const FRec Rw = Rn;
+    const double4 pj = WHOLE ? make_double4(Rw.x, Rw.y, Rw.z, Rw.hinv) : pn;
     if (kk + 1 < ntot) {
       en = en2;
-      pn = fpos[cand_node(en)];
+      if constexpr (WHOLE) Rn = rec[cand_node(en)];
+      else pn = fpos[cand_node(en)];
       if (kk + 2 < ntot) en2 = nlist[list_at_g(loff, nact, t, slot(kk + 2))];
     }
     double dx, dy, dz;
@@ -264,7 +271,7 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *_
     // a density list also holds particles that are not force neighbours
     if (!(q2i < R2 || q2j < R2)) continue;
     npair++;
-    const FRec &Rj = rec[j];
+    const FRec &Rj = WHOLE ? Rw : rec[j];
     if (ind) {
       // force.F90:1346-1352: every neighbour of an active particle is
       // flagged to be woken; the limiter needs the largest neighbour bin

@@ -317,14 +317,19 @@ __device__ __forceinline__ void build_list(ListStage &S, const DevConst &dc, con
         oz = (sz == 0) ? 0. : copysign(dc.dzbound, (double)sz);
       }
       const int s1 = S.start[L + 1], j0 = S.lo[L] - S.start[L];
-      for (int s = S.start[L]; s < s1; s++) {
-        double dx = xsub(xi, xadd(S.x[s], ox)), dy = xsub(yi, xadd(S.y[s], oy)), dz = xsub(zi, xadd(S.z[s], oz));
-        if (wrap) {
+      if (!wrap) {  // x_i - (x_j + 0) = x_i - x_j: no offset add
+        for (int s = S.start[L]; s < s1; s++) {
+          const double dx = xsub(xi, S.x[s]), dy = xsub(yi, S.y[s]), dz = xsub(zi, S.z[s]);
+          visit(j0 + s, code, r2_exact(dx, dy, dz), S.hinv[s]);
+        }
+      } else {
+        for (int s = S.start[L]; s < s1; s++) {
+          double dx = xsub(xi, xadd(S.x[s], ox)), dy = xsub(yi, xadd(S.y[s], oy)), dz = xsub(zi, xadd(S.z[s], oz));
           dx = min_image(dx, dc.dxbound, 0.5 * dc.dxbound);
           dy = min_image(dy, dc.dybound, 0.5 * dc.dybound);
           dz = min_image(dz, dc.dzbound, 0.5 * dc.dzbound);
+          visit(j0 + s, code, r2_exact(dx, dy, dz), S.hinv[s]);
         }
-        visit(j0 + s, code, r2_exact(dx, dy, dz), S.hinv[s]);
       }
     }
     wave_lds_sync();
