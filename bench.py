@@ -152,6 +152,7 @@ def main():
                  "combined_updates_per_s": n / (combined_ms * 1e-3)},
         "breakdown_ms": dict(main_run["breakdown"], kernels={nm: round(1e3 * kmean(nm), 4) for nm in sorted(kt)}),
         "kernel_tflops": kernel_tflops,
+        "tuning": main_run["tuning"],
         "neighbours": {"dens_mean": main_run["stats"]["actualmean"], "dens_max": main_run["stats"]["maxactual"],
                        "force_pairs_per_particle": main_run["npairs"] / n, "nrhocalc": main_run["stats"]["nrhocalc"]},
     }
@@ -296,7 +297,7 @@ def run_derivs(workload, precision, steps, warmup, keep=False):
         n=n, params=p, setup=setup, start_h=start_h, kt=kt, value=n * steps / elapsed,
         ms_step=1e3 * elapsed / steps, tree_ms=tree_ms, tree_first_ms=tree_first_ms, tree_kernels=tree_kernels,
         stats=ctx.get_neighbour_stats(), npairs=ctx.get_force_stats()["npairs"],
-        precision_used=ctx.get_precision(),
+        precision_used=ctx.get_precision(), tuning=ctx.get_tuning(),
         breakdown={"density_call": 1e3 * phase["density"] / steps,
                    "cons2prim_call": 1e3 * phase["cons2prim"] / steps,
                    "force_call": 1e3 * phase["force"] / steps,

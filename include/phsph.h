@@ -437,6 +437,24 @@ int phsph_step_reset(phsph_ctx *ctx, int32_t np, double dtsph,
                      double *Bpred, double *dustevol, const double *ddustevol,
                      double *dustpred, const int8_t *iphase);
 
+/*
+ * Configuration of the neighbour-list kernels, picked by the library's
+ * autotuner (no counterpart in the reference): candidate leaves staged per
+ * chunk `cl` (8 or 16), wavefronts per block `lw` (2 or 4), and `r2` = 1 to
+ * store each list entry's squared separation for the h iteration (0:
+ * recomputed from the positions).  Every configuration gives bit-identical
+ * results.  By default the first densityiterate of a context with at least
+ * 32768 active particles times each configuration once on the caller's
+ * own particles and keeps the fastest.  phsph_set_tuning fixes a
+ * configuration (cl = 0 asks for a new tuning on the next large call);
+ * phsph_get_tuning returns the current one, whether it was tuned (tuned =
+ * 1) and, if `ms`/`cfg` are given (room for 8), the time of each
+ * configuration tried (cfg: cl, lw, r2 per configuration) and their count.
+ */
+int phsph_set_tuning(phsph_ctx *ctx, int32_t cl, int32_t lw, int32_t r2);
+int phsph_get_tuning(phsph_ctx *ctx, int32_t *cl, int32_t *lw, int32_t *r2, int32_t *tuned,
+                     float *ms, int32_t *cfg, int32_t *ntried);
+
 /* get_neighbour_stats (dens.F90:1137-1155) of the last densityiterate. */
 int phsph_get_neighbour_stats(phsph_ctx *ctx, double *trialmean,
                               double *actualmean, int32_t *maxtrial,

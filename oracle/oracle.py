@@ -57,6 +57,8 @@ def lib():
         L.orc_set_bins.restype = None
         L.orc_get_nbinmax.argtypes = [vp]
         L.orc_get_nbinmax.restype = C.c_int32
+        L.orc_get_max_force_trial.argtypes = [vp]
+        L.orc_get_max_force_trial.restype = C.c_int64
         L.orc_set_active_particles.argtypes = [C.c_int32, vp, vp, vp, C.c_int32, C.c_int32, P(C.c_int32),
                                                P(C.c_int32)]
         L.orc_set_active_particles.restype = C.c_int32
@@ -201,6 +203,10 @@ class Oracle:
     def nbinmax(self):
         return lib().orc_get_nbinmax(self._ctx)
 
+    def max_force_trial(self):
+        """Longest cell neighbour list of the last force call."""
+        return lib().orc_get_max_force_trial(self._ctx)
+
     def cons2prim(self, xyzh, vxyzu, dvdx, alphaind, dustevol=None, iphase=None):
         n = xyzh.shape[0]
         eos_vars = np.zeros((n, 3))
@@ -313,5 +319,6 @@ def run_all(setup, icall=1):
                 Bevol=setup.get("Bevol"), dustfrac=c["dustfrac"], iphase=setup.get("iphase"))
     if bins is not None:
         f.update(ibin=ibin, ibin_wake=ibin_wake, nbinmax=o.nbinmax())
+    f["max_force_trial"] = o.max_force_trial()
     return dict(oracle=o, xyzh=xyzh, tree=tree, dens=d, c2p=c, force=f, stats=o.neighbour_stats(),
                 alphaind=alphaind)

@@ -63,6 +63,8 @@ typedef struct {
   /* neighbour statistics (dens.F90:106-108) */
   int64_t nneightry, nneighact, maxneightry, ncalc, nptot;
   int32_t maxneighact;
+  /* longest neighbour list of the last force call (cells' getneigh lists) */
+  int64_t maxforcetrial;
   /* block timesteps (part:ibin, ibin_old, ibin_wake; timestep_ind module) */
   int8_t *ibin, *ibin_wake;
   const int8_t *ibin_old;
@@ -1601,7 +1603,8 @@ int orc_force(orc_ctx *o, int32_t icall, int32_t np, const double *xyzh, const d
     return PHSPH_ERR_STATE;
   }
   int rc_all = PHSPH_OK;
-#pragma omp parallel reduction(min : dtcourant, dtforce) reduction(max : nbinmaxnew) reduction(+ : ncheckbin)
+  int64_t maxtry = 0;
+#pragma omp parallel reduction(min : dtcourant, dtforce) reduction(max : nbinmaxnew, maxtry) reduction(+ : ncheckbin)
   {
     nlist_t l = {0};
 #pragma omp for schedule(dynamic, 64)
@@ -1656,6 +1659,7 @@ int orc_force(orc_ctx *o, int32_t icall, int32_t np, const double *xyzh, const d
       int rc = o->p.gravity ? getneigh_dual(o, (int32_t)icell, &l, fnode)
                             : getneigh(o, nd->xcen, nd->size, o->radkern * nd->hmax, 1, &l);
       if (rc) { rc_all = rc; continue; }
+      if ((int64_t)l.n > maxtry) maxtry = l.n;
       for (int k = 0; k < npcell; k++) {
         const fpart_t *q = &parts[k];
         int32_t i = gidx[k];
@@ -1771,6 +1775,7 @@ int orc_force(orc_ctx *o, int32_t icall, int32_t np, const double *xyzh, const d
   if (dtforce_out) *dtforce_out = dtforce;
   /* no active particle: nbinmax is kept (force.F90:748-756) */
   o->nbinmaxnew = (ncheckbin == 0) ? o->nbinmax : nbinmaxnew;
+  o->maxforcetrial = maxtry;
   return rc_all;
 }
 
@@ -1805,6 +1810,9 @@ void orc_set_bins(orc_ctx *o, int8_t *ibin, const int8_t *ibin_old, int8_t *ibin
 }
 
 int32_t orc_get_nbinmax(const orc_ctx *o) { return o->nbinmaxnew; }
+/* the longest cell neighbour list of the last force call: at most 1000
+ * (force.F90:58, maxcellcache) means every position came from the cache */
+int64_t orc_get_max_force_trial(const orc_ctx *o) { return o->maxforcetrial; }
 
 /* register the potential energy output of force (part:poten, real*4) */
 void orc_set_poten(orc_ctx *o, float *poten) { o->poten = poten; }

@@ -346,6 +346,38 @@ int phsph_get_tree(phsph_ctx *c, phsph_kdnode *node, int32_t *inoderange, int32_
   return PHSPH_OK;
 }
 
+int phsph_set_tuning(phsph_ctx *c, int32_t cl, int32_t lw, int32_t r2) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (cl == 0) {
+    c->tune_mode = TUNE_PENDING;
+    return PHSPH_OK;
+  }
+  if ((cl != 8 && cl != 16) || (lw != 2 && lw != 4) || (r2 != 0 && r2 != 1))
+    PH_RET(c, PHSPH_ERR_INPUT, "set_tuning: cl must be 8 or 16, lw 2 or 4, r2 0 or 1");
+  c->tune = ListTune{cl, lw, r2};
+  c->tune_mode = TUNE_FIXED;
+  return PHSPH_OK;
+}
+
+int phsph_get_tuning(phsph_ctx *c, int32_t *cl, int32_t *lw, int32_t *r2, int32_t *tuned, float *ms, int32_t *cfg,
+                     int32_t *ntried) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (cl) *cl = c->tune.cl;
+  if (lw) *lw = c->tune.lw;
+  if (r2) *r2 = c->tune.r2;
+  if (tuned) *tuned = c->tune_mode == TUNE_DONE ? 1 : 0;
+  if (ntried) *ntried = c->tune_ntried;
+  for (int k = 0; k < c->tune_ntried; k++) {
+    if (ms) ms[k] = c->tune_ms[k];
+    if (cfg) {
+      cfg[3 * k] = c->tune_tried[k].cl;
+      cfg[3 * k + 1] = c->tune_tried[k].lw;
+      cfg[3 * k + 2] = c->tune_tried[k].r2;
+    }
+  }
+  return PHSPH_OK;
+}
+
 int phsph_get_tree_grav(phsph_ctx *c, phsph_kdnode_grav *node, int64_t cap_nodes) {
   if (!c) return PHSPH_ERR_INPUT;
   if (!c->tree_valid) PH_RET(c, PHSPH_ERR_STATE, "get_tree_grav: no tree");

@@ -76,6 +76,17 @@ struct Caps {
   int gnear = 128;   // neighbours per particle in the self-gravity force pass
 };
 
+// list-kernel configuration picked by the autotuner (density.hip):
+// candidate leaves staged per chunk (8 or 16), wavefronts per block (2 or
+// 4), squared separations stored with the density lists (1) or recomputed
+// from the positions in the h iteration (0)
+struct ListTune {
+  int cl = 16, lw = 4, r2 = 0;
+};
+enum { TUNE_FIXED = 0, TUNE_PENDING = 1, TUNE_DONE = 2 };
+constexpr int TUNE_MIN_ACTIVE = 32768;  // smallest density call the autotuner times
+constexpr int TUNE_MAX_CONFIGS = 8;
+
 struct phsph_ctx {
   phsph_params p;
   DBuf<double> hmaxf;                 // node hmax of the final h (force walk)
@@ -133,6 +144,7 @@ struct phsph_ctx {
   int dlist_nretry = 0;             // particles of the first attempt run again
   DBuf<int32_t> dretry0;            // their flags, by tree position
   DBuf<uint32_t> nlist_r;           // lists of retries (density) and of those particles (force)
+  DBuf<double> nr2, nr2_r;          // squared separations of the own entries of the density lists
   DBuf<int32_t> t_nl_r;
   DBuf<double> t_hsum_r, t_hfin_r;
   bool force_reused_lists = false;  // the last force call used the density lists
@@ -158,6 +170,12 @@ struct phsph_ctx {
   DBuf<int32_t> act_dens;                // active positions of the last densityiterate
   int nact_dens = 0;
   bool trial_stale = true;               // trial statistics not yet computed
+  // autotuner state (density.hip: tune_density_iter)
+  ListTune tune;
+  int tune_mode = TUNE_PENDING;
+  int tune_ntried = 0, tune_nact = 0;
+  ListTune tune_tried[TUNE_MAX_CONFIGS];
+  float tune_ms[TUNE_MAX_CONFIGS] = {0.f};
   int dens_icall = 1;                    // icall of the last densityiterate
 
   // per-particle / per-leaf outputs of density kept for statistics

@@ -318,21 +318,22 @@ __global__ void __launch_bounds__(1024) k_scan_i32(int32_t *a, int n, int32_t *t
 // Newton step needs (rho and dOmega); the full set of get_density_sums is
 // taken once, at the h of the last iteration, by part 2.
 // ------------------------------------------------------------------------
-template <int K>
-__global__ void __launch_bounds__(64 * LWPB)
+template <int K, int CL, int LW, bool STORE_R2>
+__global__ void __launch_bounds__(64 * LW)
 k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, int nact, int ngroups,
             const int32_t *__restrict__ pleaf, const int32_t *__restrict__ leaves,
             const uint32_t *__restrict__ gcand, const int32_t *__restrict__ gncand, const int64_t *goff, int capg,
             const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi,
             const phsph_kdnode *__restrict__ nd, const double4 *__restrict__ tp, const double *__restrict__ hmax0,
-            double lmargin, uint32_t *__restrict__ nlist, int capl, int32_t *__restrict__ t_nl,
+            double lmargin, uint32_t *__restrict__ nlist, double *__restrict__ nr2, int capl,
+            int32_t *__restrict__ t_nl,
             int32_t *__restrict__ t_nlj,
             double *__restrict__ t_hsum, double *__restrict__ t_hfin,
             int32_t *__restrict__ p_nits, int32_t *__restrict__ leaf_nits, int32_t *__restrict__ retry,
             double *__restrict__ htraj, double *__restrict__ hconv, int32_t *__restrict__ flags) {
-  __shared__ ListStage Sw[LWPB];
-  ListStage &S = Sw[threadIdx.x >> 6];
-  const int g = list_group(gridDim.x);
+  __shared__ ListStage<CL> Sw[LW];
+  ListStage<CL> &S = Sw[threadIdx.x >> 6];
+  const int g = list_group<LW>(gridDim.x);
   if (g >= ngroups) return;
   const int lane = threadIdx.x & 63;
   const int t = g * GROUP + lane;
@@ -366,7 +367,11 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
       [&](int j, int code, double r2, double hj0) {
         const bool own = r2 < rc2;
         if (j != p && (own || (hmax0 && r2 < (rl * hj0) * (rl * hj0)))) {
-          if (nl + nlj < capl) nlist[list_at(nact, t, own ? nl : capl - 1 - nlj)] = cand_pack(j, code);
+          if (nl + nlj < capl) {
+            const size_t a = list_at(nact, t, own ? nl : capl - 1 - nlj);
+            nlist[a] = cand_pack(j, code);
+            if (STORE_R2 && own) nr2[a] = r2;  // the separation the iteration needs, exact as get_density_sums takes it
+          }
           nl += own ? 1 : 0;
           nlj += own ? 0 : 1;
         }
@@ -386,26 +391,7 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
     it++;
     const double hi1 = 1. / hi, hi21 = xmul(hi1, hi1);
     double srho = 0., sgh = 0.;
-    // software pipeline: the position record of entry k+1 and the list
-    // entry k+2 are loaded while entry k is summed
-    uint32_t en = 0u, en2 = 0u;
-    double4 rn = make_double4(0., 0., 0., 0.);
-    if (nl > 0) {
-      en = nlist[list_at(nact, t, 0)];
-      rn = tp[cand_node(en)];
-    }
-    if (nl > 1) en2 = nlist[list_at(nact, t, 1)];
-    for (int k = 0; k < nl; k++) {
-      const uint32_t e = en;
-      const double4 r = rn;
-      if (k + 1 < nl) {
-        en = en2;
-        rn = tp[cand_node(en)];
-        if (k + 2 < nl) en2 = nlist[list_at(nact, t, k + 2)];
-      }
-      double dx, dy, dz;
-      sep_of(dc, cand_code(e), xi, yi, zi, r.x, r.y, r.z, dx, dy, dz);
-      const double r2 = r2_exact(dx, dy, dz);
+    auto term = [&](double r2) {
       const double q2 = xmul(r2, hi21);
       if (q2 < R2) {
         const double q = sqrt(r2) * hi1;
@@ -413,6 +399,40 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
         kern_wg<K, double>(q2, q, wk, gk);
         srho += wk * pm;
         sgh += (-q * gk - 3. * wk) * pm;
+      }
+    };
+    if constexpr (STORE_R2) {
+      // the squared separations the search stored with the list, two
+      // entries (16 bytes of the slot's segment) per load, read ahead
+      double2 rn = make_double2(0., 0.);
+      if (nl > 0) rn = *reinterpret_cast<const double2 *>(nr2 + list_at(nact, t, 0));
+      for (int k = 0; k < nl; k += 2) {
+        const double2 rr = rn;
+        if (k + 2 < nl) rn = *reinterpret_cast<const double2 *>(nr2 + list_at(nact, t, k + 2));
+        term(rr.x);
+        if (k + 1 < nl) term(rr.y);
+      }
+    } else {
+      // the separation again from the positions: the record of entry k+1
+      // and the list entry k+2 are read while entry k is summed
+      uint32_t en = 0u, en2 = 0u;
+      double4 rn = make_double4(0., 0., 0., 0.);
+      if (nl > 0) {
+        en = nlist[list_at(nact, t, 0)];
+        rn = tp[cand_node(en)];
+      }
+      if (nl > 1) en2 = nlist[list_at(nact, t, 1)];
+      for (int k = 0; k < nl; k++) {
+        const uint32_t e = en;
+        const double4 r = rn;
+        if (k + 1 < nl) {
+          en = en2;
+          rn = tp[cand_node(en)];
+          if (k + 2 < nl) en2 = nlist[list_at(nact, t, k + 2)];
+        }
+        double dx, dy, dz;
+        sep_of(dc, cand_code(e), xi, yi, zi, r.x, r.y, r.z, dx, dy, dz);
+        term(r2_exact(dx, dy, dz));
       }
     }
     const double hi31 = hi1 * hi21, hi41 = hi21 * hi21;
@@ -816,6 +836,98 @@ int dens_particle_stats(phsph_ctx *c, int32_t *d_nneigh, int32_t *d_nits) {
   return PHSPH_OK;
 }
 
+// ------------------------------------------------------------------------
+// k_dens_iter configurations and the autotuner
+// ------------------------------------------------------------------------
+struct IterLaunch {
+  phsph_ctx *c;
+  hipStream_t s;
+  int ng, icall;
+  double rfac;
+  int na;
+  double lmargin;
+  int dlist;
+  bool widen;
+  uint32_t *lb;
+  double *r2b;
+  int32_t *tnl;
+  double *ths, *thf;
+};
+
+template <int K, int CL, int LW, bool R2>
+static void launch_iter_t(const IterLaunch &L) {
+  phsph_ctx *c = L.c;
+  hipLaunchKernelGGL((k_dens_iter<K, CL, LW, R2>), dim3(list_grid<LW>(L.ng)), dim3(64 * LW), 0, L.s, c->dc, L.icall,
+                     L.rfac, c->act.p, L.na, L.ng, c->pleaf.p, c->leaves.p, c->gcand.p, c->gncand.p, c->goff_p,
+                     c->caps.gcand, c->nd_lo.p, c->nd_hi.p, c->nd.p, c->tp4.p, L.widen ? c->hmax0.p : nullptr,
+                     L.lmargin, L.lb, L.r2b, L.dlist, L.tnl, L.widen ? c->t_nlj.p : nullptr, L.ths, L.thf,
+                     c->p_nits.p, c->leaf_nits.p, c->tmpi.p, c->htraj.p, c->hconv.p, c->counters.p);
+}
+
+template <int K>
+static void launch_iter_k(const ListTune &t, const IterLaunch &L) {
+  if (t.cl == 8) {
+    if (t.lw == 2) { if (t.r2) launch_iter_t<K, 8, 2, true>(L); else launch_iter_t<K, 8, 2, false>(L); }
+    else           { if (t.r2) launch_iter_t<K, 8, 4, true>(L); else launch_iter_t<K, 8, 4, false>(L); }
+  } else {
+    if (t.lw == 2) { if (t.r2) launch_iter_t<K, 16, 2, true>(L); else launch_iter_t<K, 16, 2, false>(L); }
+    else           { if (t.r2) launch_iter_t<K, 16, 4, true>(L); else launch_iter_t<K, 16, 4, false>(L); }
+  }
+}
+
+void launch_dens_iter(const ListTune &t, const IterLaunch &L) {
+  if (L.c->p.kernel == PHSPH_KERNEL_CUBIC) launch_iter_k<PHSPH_KERNEL_CUBIC>(t, L);
+  else launch_iter_k<PHSPH_KERNEL_QUINTIC>(t, L);
+}
+
+// The autotuner: on the first density call of a context with at least
+// TUNE_MIN_ACTIVE active particles, k_dens_iter runs once in every
+// configuration (candidate leaves staged per chunk, wavefronts per block,
+// squared separations stored with the lists or recomputed from the
+// positions), timed with HIP events on the library stream, and the fastest
+// is kept for the context.  Every configuration visits the same leaves in
+// the same order and writes the same lists and sums, so the outputs of the
+// last run are those of any other.
+int tune_density_iter(phsph_ctx *c, const IterLaunch &L) {
+  // the separations buffer is needed by the configurations that store them
+  PH_HIPCHECK(c, c->nr2.ensure((size_t)list_cap(L.dlist) * L.na + 2));
+  IterLaunch L2 = L;
+  L2.r2b = c->nr2.p;
+  hipEvent_t e0, e1;
+  PH_HIPCHECK(c, hipEventCreate(&e0));
+  PH_HIPCHECK(c, hipEventCreate(&e1));
+  float best = 1e30f;
+  int nt = 0;
+  ListTune bt = c->tune;
+  for (int cl : {8, 16})
+    for (int lw : {2, 4})
+      for (int r2 : {0, 1}) {
+        const ListTune t{cl, lw, r2};
+        PH_HIPCHECK(c, hipEventRecord(e0, L.s));
+        launch_dens_iter(t, L2);
+        PH_HIPCHECK(c, hipEventRecord(e1, L.s));
+        PH_HIPCHECK(c, hipEventSynchronize(e1));
+        float ms = 0.f;
+        PH_HIPCHECK(c, hipEventElapsedTime(&ms, e0, e1));
+        if (nt < TUNE_MAX_CONFIGS) {
+          c->tune_tried[nt] = t;
+          c->tune_ms[nt] = ms;
+          nt++;
+        }
+        if (ms < best) {
+          best = ms;
+          bt = t;
+        }
+      }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  c->tune_ntried = nt;
+  c->tune = bt;
+  c->tune_mode = TUNE_DONE;
+  c->tune_nact = L.na;
+  return PHSPH_OK;
+}
+
 // trial neighbours of the reference's per-leaf walk at radkern*hmax, over
 // the active particles of the last densityiterate (computed on request).
 // The hmax is that of the leaf's last neighbour list: the tree's, or the
@@ -1154,8 +1266,10 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   // list capacity: it grows for the particles run again (the list memory is
   // that of the retried particles); a grown capacity is kept for later calls
   // while the lists of every active particle fit in the list budget
-  while (c->caps.dlist > 16 && !list_fits(c, (size_t)list_cap(c->caps.dlist) * nact)) c->caps.dlist /= 2;
+  // 12 bytes per entry: the entry and its squared separation
+  while (c->caps.dlist > 16 && !list_fits(c, (size_t)list_cap(c->caps.dlist) * nact * 3)) c->caps.dlist /= 2;
   int dlist = c->caps.dlist;
+  bool tuned_now = false;
   for (int attempt = 0; attempt < 12; attempt++) {
     const int na = c->nact;  // active slots, padding included
     if (na == 0) break;
@@ -1175,20 +1289,21 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
     if (rc) return rc;
     const int ng = c->ngroups;
     PH_HIPCHECK(c, LB.ensure((size_t)list_cap(dlist) * na + 1));
+    DBuf<double> &R2B = attempt == 0 ? c->nr2 : c->nr2_r;
+    if (c->tune.r2 || c->tune_mode == TUNE_PENDING) PH_HIPCHECK(c, R2B.ensure((size_t)list_cap(dlist) * na + 2));
     hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(n, 256)), dim3(256), 0, s, c->tmpi.p, n, 0);
     int32_t zero = 0;
     PH_HIPCHECK(c, dev_set(c, c->counters.p, &zero, sizeof(int32_t)));
     {
-      LaunchTimer lt(c, "density_iter");
-#define DITER(KK)                                                                                              \
-  hipLaunchKernelGGL((k_dens_iter<KK>), dim3(list_grid(ng)), dim3(64 * LWPB), 0, s, c->dc, icall, rfac, c->act.p,   \
-                     na, ng,                                                                                        \
-                     c->pleaf.p, c->leaves.p, c->gcand.p, c->gncand.p, c->goff_p, c->caps.gcand, c->nd_lo.p,      \
-                     c->nd_hi.p, c->nd.p, c->tp4.p, widen ? c->hmax0.p : nullptr, lmargin, LB.p, dlist,             \
-                     TNL.p, widen ? c->t_nlj.p : nullptr, THS.p, THF.p,                                             \
-                     c->p_nits.p, c->leaf_nits.p, c->tmpi.p, c->htraj.p, c->hconv.p, c->counters.p)
-      if (c->p.kernel == PHSPH_KERNEL_CUBIC) DITER(PHSPH_KERNEL_CUBIC); else DITER(PHSPH_KERNEL_QUINTIC);
-#undef DITER
+      const IterLaunch L{c, s, ng, icall, rfac, na, lmargin, dlist, widen, LB.p, R2B.p, TNL.p, THS.p, THF.p};
+      if (attempt == 0 && c->tune_mode == TUNE_PENDING && na >= TUNE_MIN_ACTIVE) {
+        tuned_now = true;
+        rc = tune_density_iter(c, L);  // times every configuration, keeps the fastest
+        if (rc) return rc;
+      } else {
+        LaunchTimer lt(c, "density_iter");
+        launch_dens_iter(c->tune, L);
+      }
     }
     {
       LaunchTimer lt(c, "density_sums");
@@ -1221,6 +1336,10 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
               attempt, na, ng, rfac, dlist, fl, fr / 1073741824.0);
     }
     if (fl & DFL_NOCONV) PH_RET(c, PHSPH_ERR_NOCONVERGE, "densityiterate: could not converge in density");
+    // a tuning attempt whose lists overflowed (the first call on a context
+    // grows the list capacity) times unrepresentative work: tune again on
+    // the next large call
+    if (attempt == 0 && tuned_now && (fl & DFL_LIST)) c->tune_mode = TUNE_PENDING;
     if (fl & DFL_NEGH) PH_RET(c, PHSPH_ERR_INPUT, "densityiterate: setting negative h from hrho");
     if (attempt == 0) {
       // the lists of the first attempt, and which particles they do not
@@ -1257,7 +1376,7 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
     PH_HIPCHECK(c, hipStreamSynchronize(s));
     if (bad) c->dlist_reuse = false;
   }
-  while (dlist > c->caps.dlist && !list_fits(c, (size_t)list_cap(dlist) * nact)) dlist /= 2;
+  while (dlist > c->caps.dlist && !list_fits(c, (size_t)list_cap(dlist) * nact * 3)) dlist /= 2;
   if (dlist > c->caps.dlist) c->caps.dlist = dlist;
   if (grav) {
     rc = grav_hmax_replay(c, icall);

@@ -115,16 +115,17 @@ __global__ void k_force_prep(DevConst dc, int nlive, const int32_t *__restrict__
 // rounded FP64 tests.  One wavefront per group of 64 active particles; each
 // lane visits the candidate leaves it can reach, a leaf's particles lying
 // within its size of its centre and having h at most the leaf's hmax.
-__global__ void __launch_bounds__(64 * LWPB)
+template <int CL, int LW>
+__global__ void __launch_bounds__(64 * LW)
 k_flist(DevConst dc, const int32_t *__restrict__ act, int nact, int ngroups, const int32_t *__restrict__ pleaf,
         const int32_t *__restrict__ leaves, const uint32_t *__restrict__ gcand, const int32_t *__restrict__ gncand,
         const int64_t *goff, int capg, const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi,
         const phsph_kdnode *__restrict__ nd, const double *__restrict__ hmaxn, const double4 *__restrict__ fpos,
         uint32_t *__restrict__ nlist, const int64_t *loff, int ucap, int32_t *__restrict__ t_nl,
         int32_t *__restrict__ flags) {
-  __shared__ ListStage Sw[LWPB];
-  ListStage &S = Sw[threadIdx.x >> 6];
-  const int g = list_group(gridDim.x);
+  __shared__ ListStage<CL> Sw[LW];
+  ListStage<CL> &S = Sw[threadIdx.x >> 6];
+  const int g = list_group<LW>(gridDim.x);
   if (g >= ngroups) return;
   const int lane = threadIdx.x & 63;
   const int t = g * GROUP + lane;
@@ -799,9 +800,14 @@ static int force_lists(phsph_ctx *c, DBuf<uint32_t> &L, DBuf<int32_t> &TNL, cons
     PH_HIPCHECK(c, dev_set(c, c->counters.p + 3, &zero, sizeof(int32_t)));
     if (nact > 0) {
       LaunchTimer lt(c, tname);
-      hipLaunchKernelGGL(k_flist, dim3(list_grid(ng)), dim3(64 * LWPB), 0, s, c->dc, c->act.p, nact, ng, c->pleaf.p,
-                         c->leaves.p, c->gcand.p, c->gncand.p, c->goff_p, c->caps.gcand, c->nd_lo.p, c->nd_hi.p,
-                         c->nd.p, c->hmaxf.p, c->fpos.p, L.p, loff, ucap, TNL.p, c->counters.p + 3);
+#define FLIST(CC, LL)                                                                                         \
+  hipLaunchKernelGGL((k_flist<CC, LL>), dim3(list_grid<LL>(ng)), dim3(64 * LL), 0, s, c->dc, c->act.p, nact, ng,   \
+                     c->pleaf.p, c->leaves.p, c->gcand.p, c->gncand.p, c->goff_p, c->caps.gcand, c->nd_lo.p,       \
+                     c->nd_hi.p, c->nd.p, c->hmaxf.p, c->fpos.p, L.p, loff, ucap, TNL.p, c->counters.p + 3)
+      // the chunk length and wavefronts per block the density autotuner chose
+      if (c->tune.cl == 8) { if (c->tune.lw == 2) FLIST(8, 2); else FLIST(8, 4); }
+      else { if (c->tune.lw == 2) FLIST(16, 2); else FLIST(16, 4); }
+#undef FLIST
     }
     PH_HIPCHECK(c, hipGetLastError());
     int32_t fl = 0;

@@ -45,13 +45,16 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 }
 static inline unsigned xcd_grid(int ngroups) { return (unsigned)(((ngroups + 7) / 8) * 8); }
 
-// The list kernels run LWPB independent wavefronts per block, one group each,
+// The list kernels run LW independent wavefronts per block, one group each,
 // so that more waves share a CU; each wavefront stages through its own LDS
-// and synchronises only itself.
-constexpr int LWPB = 4;
-static inline unsigned list_grid(int ngroups) { return xcd_grid((ngroups + LWPB - 1) / LWPB); }
+// and synchronises only itself.  LW (2 or 4) and the chunk length CL of
+// build_list (8 or 16 candidate leaves) are picked per context by the
+// autotuner (tune.h); every choice gives bit-identical results.
+template <int LW>
+static inline unsigned list_grid(int ngroups) { return xcd_grid((ngroups + LW - 1) / LW); }
+template <int LW>
 __device__ __forceinline__ int list_group(int nb) {
-  return xcd_block(blockIdx.x, nb) * LWPB + (int)(threadIdx.x >> 6);
+  return xcd_block(blockIdx.x, nb) * LW + (int)(threadIdx.x >> 6);
 }
 // orders a wavefront's LDS stores before its later LDS loads
 __device__ __forceinline__ void wave_lds_sync() {
@@ -243,9 +246,10 @@ static __global__ __attribute__((unused)) void k_group_max(const int32_t *__rest
 // squared separation from (xi, yi, zi) rounded as sep_of and r2_exact, and
 // the record's fourth word.  S is the wavefront's own stage; only the
 // wavefront synchronises (wave_lds_sync).
-constexpr int CL = 16;                 // candidate leaves per chunk
-constexpr int CS = CL * PH_MINPART;    // staged particles per chunk
+template <int CLn>
 struct ListStage {
+  static constexpr int CL = CLn;                // candidate leaves per chunk
+  static constexpr int CS = CLn * PH_MINPART;  // staged particles per chunk
   int lo[CL], start[CL + 1];
   double cx[CL], cy[CL], cz[CL], sz[CL], hm[CL];
   double x[CS], y[CS], z[CS], hinv[CS];
@@ -259,8 +263,8 @@ __device__ __forceinline__ void rec_xyzi(const Rec *__restrict__ r, int j, doubl
   x = v.x; y = v.y; z = v.z; w = v.w;
 }
 
-template <class Rec, class Reach, class Visit>
-__device__ __forceinline__ void build_list(ListStage &S, const DevConst &dc, const uint32_t *__restrict__ gc, int nc,
+template <int CL, class Rec, class Reach, class Visit>
+__device__ __forceinline__ void build_list(ListStage<CL> &S, const DevConst &dc, const uint32_t *__restrict__ gc, int nc,
                                            const phsph_kdnode *__restrict__ nd, const int32_t *__restrict__ nd_lo,
                                            const int32_t *__restrict__ nd_hi, const double *__restrict__ hmaxn,
                                            const Rec *__restrict__ rec, double xi, double yi, double zi,

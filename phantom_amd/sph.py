@@ -36,7 +36,8 @@ EXPORTED = [
     "phsph_set_bins", "phsph_get_nbinmax", "phsph_set_active_particles", "phsph_set_poten",
     "phsph_get_quads", "phsph_init_step", "phsph_step_predictor", "phsph_substep_sph", "phsph_step_predict_sph",
     "phsph_step_corrector", "phsph_step_reset", "phsph_get_precision", "phsph_density_vjp", "phsph_force_vjp",
-    "phsph_step_vjp", "phsph_step_corrector_noalloc", "phsph_get_tree_grav",
+    "phsph_step_vjp", "phsph_step_corrector_noalloc", "phsph_get_tree_grav", "phsph_set_tuning",
+    "phsph_get_tuning",
 ]
 
 _lib = None
@@ -85,6 +86,8 @@ def load_library(path=LIB_PATH):
     L.phsph_set_poten.argtypes = [vp, vp]
     L.phsph_get_quads.argtypes = [vp, vp, i64]
     L.phsph_get_tree_grav.argtypes = [vp, vp, i64]
+    L.phsph_set_tuning.argtypes = [vp, i32, i32, i32]
+    L.phsph_get_tuning.argtypes = [vp, P(i32), P(i32), P(i32), P(i32), vp, vp, P(i32)]
     L.phsph_set_active_particles.argtypes = [vp, i32, vp, i32, vp, vp, i32, i32, P(i32), P(i32), P(i32)]
     L.phsph_init_step.argtypes = [vp, i32, dbl, vp, i32, vp]
     L.phsph_step_predictor.argtypes = [vp, i32, dbl, dbl, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp,
@@ -199,6 +202,24 @@ class Context:
                                             inodeparts.ctypes.data_as(C.c_void_p), nc, C.byref(ml)))
         nlive = int(np.count_nonzero(inodeparts))
         return dict(node=node, inoderange=inoderange, inodeparts=inodeparts[:nlive], maxlevel=ml.value, ncells=nc)
+
+    def set_tuning(self, cl=0, lw=0, r2=0):
+        """Fix the list-kernel configuration (cl 8|16, lw 2|4, r2 0|1), or
+        with cl = 0 ask the autotuner to time them on the next large call."""
+        self._check(self.lib.phsph_set_tuning(self._h, cl, lw, r2))
+
+    def get_tuning(self):
+        """Current configuration, whether it was tuned, and the time (ms) of
+        each configuration the autotuner tried."""
+        cl, lw, r2, tuned, nt = (C.c_int32() for _ in range(5))
+        ms = np.zeros(8, np.float32)
+        cfg = np.zeros(24, np.int32)
+        self._check(self.lib.phsph_get_tuning(self._h, C.byref(cl), C.byref(lw), C.byref(r2), C.byref(tuned),
+                                              ms.ctypes.data_as(C.c_void_p), cfg.ctypes.data_as(C.c_void_p),
+                                              C.byref(nt)))
+        tried = [dict(cl=int(cfg[3 * k]), lw=int(cfg[3 * k + 1]), r2=int(cfg[3 * k + 2]), ms=float(ms[k]))
+                 for k in range(nt.value)]
+        return dict(cl=cl.value, lw=lw.value, r2=r2.value, tuned=bool(tuned.value), tried=tried)
 
     def get_tree_grav(self):
         """node(1:ncells) in the GRAVITY layout of type(kdnode) (with

@@ -17,8 +17,11 @@ Tolerances: neighbour counts, iteration counts, neighbour statistics
 (trial and actual), bins and wake flags bit-exact; FP64 h within 1e-12 of
 its rms, acceleration components within 1e-12 of the rms of |f|; du/dt,
 dB/dt, dε/dt and Δv within 1e-12 relative per particle (relative to the rms
-where the quantity passes through zero); float32 outputs to float
-rounding.  Mixed
+where the quantity passes through zero); float32 outputs to float rounding.
+With MHD and dust the force pass is also compared alone, on identical
+inputs, at 1e-12 per particle; chained through the density iteration, du/dt
+of the dust box holds to 1e-10 (its dust heating term amplifies the
+rounding-level differences of h, see _compare).  Mixed
 precision: rho within 1e-6 relative per particle, dv/dt within 1e-6 of the
 rms of |dv/dt|, counts bit-exact; the errors are printed.  Only active
 particles are compared under block timesteps (inactive particles keep their
@@ -26,6 +29,7 @@ inputs)."""
 import numpy as np
 import pytest
 
+from oracle import oracle
 from phantom_amd import abi, setups
 from tests import parity
 
@@ -88,6 +92,11 @@ def _compare(name, setup, g, o):
     for key in ("nactualtot", "maxactual", "nrhocalc", "maxtrial"):
         assert gs[key] == os_[key], key
     assert gs["trialmean"] == pytest.approx(os_["trialmean"], rel=1e-14)
+    if p.periodic:
+        # every neighbour position the reference reads comes from its
+        # 1000-entry cell caches (dens.F90:99, force.F90:58), whose
+        # rounding the GPU reproduces
+        assert os_["maxtrial"] <= 1000 and o["force"]["max_force_trial"] <= 1000
     # density
     assert parity.rel_rms(g["xyzh"][act, 3], o["xyzh"][act, 3]) <= TOL
     for key in ("gradh", "divcurlv", "dvdx"):
@@ -105,7 +114,17 @@ def _compare(name, setup, g, o):
     if p.maxvxyzu >= 4:
         e = parity.rel_err(fg[:, 3], fo[:, 3])
         print(f"[{name}] du/dt err {e:.2e} (max/rms {parity.rel_rms(fg[:, 3], fo[:, 3]):.2e})", flush=True)
-        assert e <= TOL
+        if p.use_dustfrac:
+            # The dust box has a uniform pressure by construction (u set
+            # from the local gas density), and the one-fluid dust heating
+            # is a sum of (P_i - P_j) (u_i - u_j) terms (force.F90:1814-1821):
+            # differences of P of ~1e-3 P carry the 1e-16 rounding
+            # differences of h between the two summation orders amplified
+            # ~1e3 times, so the chained comparison holds to 1e-10; the force
+            # kernel itself is compared at 1e-12 on identical inputs below.
+            assert e <= 1e-10
+        else:
+            assert e <= TOL
     if p.mhd:
         for c in range(4):
             assert parity.rel_err(gf["dBevol"][act, c], of["dBevol"][act, c]) <= TOL, ("dBevol", c)
@@ -116,6 +135,21 @@ def _compare(name, setup, g, o):
             assert parity.rel_err(gf["deltav"][act, c], of["deltav"][act, c]) <= TOL
     if p.gravity:
         assert parity.rel_rms(gf["poten"][act], of["poten"][act]) <= 2e-7
+    if p.use_dustfrac or p.mhd:
+        # the force pass alone: the oracle's force on the GPU's density and
+        # cons2prim outputs, per particle to 1e-12
+        f2 = _oracle_force_on(setup, g)
+        a = act
+        for c in range(np.asarray(gf["fxyzu"]).shape[1]):
+            e = parity.rel_err(np.asarray(gf["fxyzu"])[a, c], np.asarray(f2["fxyzu"])[a, c])
+            assert e <= TOL, ("fxyzu on identical inputs", c, e)
+        if p.mhd:
+            for c in range(4):
+                assert parity.rel_err(gf["dBevol"][a, c], f2["dBevol"][a, c]) <= TOL, ("dBevol", c)
+        if p.use_dustfrac:
+            assert parity.rel_err(gf["ddustevol"][a], f2["ddustevol"][a]) <= TOL
+            for c in range(3):
+                assert parity.rel_err(gf["deltav"][a, c], f2["deltav"][a, c]) <= TOL
     if "bins" in setup:
         np.testing.assert_array_equal(gf["ibin"], of["ibin"])
         np.testing.assert_array_equal(gf["ibin_wake"], of["ibin_wake"])
@@ -126,6 +160,23 @@ def _compare(name, setup, g, o):
         if not p.mhd:
             f = np.asarray(gf["fxyzu"])[:, :3]
             assert np.abs(f.sum(axis=0)).max() <= 1e-12 * np.abs(f).sum(axis=0).max()
+
+
+def _oracle_force_on(setup, g):
+    """The oracle's force on the particles, h, density outputs and
+    cons2prim outputs the GPU produced (tree rebuilt at those positions)."""
+    p = setup["params"]
+    o = oracle.Oracle(p)
+    xyzh = g["xyzh"].copy()
+    iphase = setup.get("iphase")
+    o.maketree(xyzh, iphase=iphase)
+    bins = setup.get("bins")
+    if bins is not None:
+        o.set_bins(bins["ibin"].copy(), bins["ibin_old"], bins["ibin_wake"].copy(), bins["nbinmax"],
+                   bins["istepfrac"], bins["ibinnow"])
+    return o.force(xyzh, setup["vxyzu"], g["dens"]["gradh"], g["alphaind"], g["c2p"]["eos_vars"],
+                   divcurlB=g["dens"]["divcurlB"], Bevol=setup.get("Bevol"), dustfrac=g["c2p"]["dustfrac"],
+                   iphase=iphase)
 
 
 @pytest.mark.parametrize("name", list(CASES))

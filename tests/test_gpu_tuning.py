@@ -1,0 +1,87 @@
+"""The autotuner of the neighbour-list kernels: on the first large density
+call it times every configuration (candidate leaves per chunk, wavefronts
+per block, squared separations stored or recomputed) and keeps the fastest.
+Every configuration must give bit-identical outputs, so the choice never
+changes a result: each one is fixed in turn and the whole derivative chain
+(tree, density, cons2prim, force) compared bit for bit with the first; the
+oracle agreement of the tuned run is the parity tests' own."""
+import numpy as np
+import pytest
+
+from phantom_amd import setups, sph
+from tests import parity
+
+pytestmark = pytest.mark.gpu
+
+CONFIGS = [(cl, lw, r2) for cl in (8, 16) for lw in (2, 4) for r2 in (0, 1)]
+
+
+def _chain(setup, cfg=None, calls=1):
+    """tree + density (calls times from the same start: the first call on a
+    context may grow the list capacity, which postpones tuning) + cons2prim
+    + force"""
+    p = setup["params"]
+    ctx = sph.Context(p)
+    if cfg is not None:
+        ctx.set_tuning(*cfg)
+    for _ in range(calls):
+        xyzh = setup["xyzh"].copy()
+        alphaind = setup["alphaind"].copy()
+        ctx.maketree(xyzh)
+        d = ctx.densityiterate(1, xyzh, setup["vxyzu"], alphaind=alphaind)
+    nneigh, nits = ctx.get_particle_stats(xyzh.shape[0])
+    c = ctx.cons2prim(xyzh, setup["vxyzu"], d["dvdx"], alphaind)
+    f = ctx.force(1, xyzh, setup["vxyzu"], d["gradh"], alphaind, c["eos_vars"])
+    tune = ctx.get_tuning()
+    ctx.close()
+    out = dict(h=xyzh[:, 3], gradh=d["gradh"], divv=d["divcurlv"], dvdx=d["dvdx"], nneigh=nneigh, nits=nits,
+               f=f["fxyzu"], dtc=f["dtcourant"], dtf=f["dtforce"])
+    return out, tune
+
+
+@pytest.fixture(scope="module", params=["box64_cubic", "shock_tube_ny24_quintic"])
+def case(request, gpu):
+    if request.param == "box64_cubic":
+        s = setups.uniform_box(nx=64, isothermal=False)
+    else:
+        s = setups.shock_tube(ny=24)
+    print(f"\n[case {request.param}] n={s['xyzh'].shape[0]}", flush=True)
+    return request.param, s
+
+
+def test_autotune_picks_fastest(case):
+    name, s = case
+    _, tune = _chain(s, calls=2)
+    assert tune["tuned"]
+    tried = tune["tried"]
+    assert len(tried) == len(CONFIGS)
+    assert {(t["cl"], t["lw"], t["r2"]) for t in tried} == set(CONFIGS)
+    assert all(t["ms"] > 0 for t in tried)
+    best = min(tried, key=lambda t: t["ms"])
+    assert (tune["cl"], tune["lw"], tune["r2"]) == (best["cl"], best["lw"], best["r2"])
+    print(f"[{name}] tuned " + ", ".join(f"cl{t['cl']}/lw{t['lw']}/r2{t['r2']}:{t['ms']:.3f}ms" for t in tried),
+          flush=True)
+
+
+def test_configurations_bit_identical(case):
+    name, s = case
+    ref, t0 = _chain(s, CONFIGS[0])
+    assert not t0["tuned"] and (t0["cl"], t0["lw"], t0["r2"]) == CONFIGS[0]
+    for cfg in CONFIGS[1:]:
+        out, _ = _chain(s, cfg)
+        for k in ref:
+            np.testing.assert_array_equal(np.asarray(out[k]), np.asarray(ref[k]), err_msg=f"{cfg} {k}")
+    # and the tuned run against the oracle's neighbour counts
+    o = parity.run_oracle(s)
+    np.testing.assert_array_equal(ref["nneigh"], o["dens"]["nneigh"])
+
+
+def test_set_tuning_rejects_unknown(gpu):
+    ctx = sph.Context(setups.uniform_box(nx=8)["params"])
+    with pytest.raises(sph.PhsphError):
+        ctx.set_tuning(12, 4, 0)
+    with pytest.raises(sph.PhsphError):
+        ctx.set_tuning(16, 3, 0)
+    ctx.set_tuning(0)  # tune again on the next large call
+    assert not ctx.get_tuning()["tuned"]
+    ctx.close()
