@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", default="derivs", choices=["derivs", "step"],
+    ap.add_argument("--mode", default="derivs", choices=["derivs", "step", "substep"],
                     help="derivs: densityiterate + cons2prim + force per step (the headline metric); "
                          "step: whole leapfrog steps (tree, density, force, integrator loops)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -74,6 +74,9 @@ def main():
         return 2
     if args.mode == "step":
         return step_bench(args)
+    if args.mode == "substep":
+        print(json.dumps(substep_bench()), flush=True)
+        return 0
     main_run = run_derivs(args.workload, args.precision, args.steps, args.warmup, keep=not args.quick)
     p = main_run["params"]
     n = main_run["n"]
@@ -120,6 +123,7 @@ def main():
             t64 = run_derivs("shock_tube_1M", "fp64", args.steps, args.warmup, keep=True)
             t32 = run_derivs("shock_tube_1M", "mixed", args.steps, args.warmup, keep=True)
             extra["shock_tube_1M"] = dict(n_particles=t64["n"], fp64=summary(t64), **precision_pair(t64, t32))
+        extra["substep"] = substep_bench()
     tree_ms = main_run["tree_ms"]
     combined_ms = tree_ms + main_run["ms_step"]
 
@@ -413,6 +417,89 @@ def step_bench(args):
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def substep_bench(nx=128, reps=3):
+    """Time of one block-timestep substep (evol_prestep, step, time update:
+    evolve.f90:200-257) on the 128^3 MHD + one-fluid dust box with four
+    populated bins (nbinmax 6), at the substeps whose active fractions are
+    1, 1/2, 1/8 and 1/64 (istepfrac 64, 32, 8, 1).  Three drivers of the
+    same substep, state resident in HBM: the host loop over the individual
+    entry points (integrator.evolve_ind), the library's own loop with
+    direct launches, and the library's loop with the fixed loop chains
+    replayed as HIP graphs (phsph_step_graph).  Each time is the median of
+    `reps` runs from the same saved state, after one untimed run."""
+    from phantom_amd import device, integrator, sph
+
+    setup = setups.with_levels(setups.mhd_dust_box(nx=nx, jitter=0.05), setups.LEVELS_4, nbinmax=6, istepfrac=0)
+    p = setup["params"]
+    n = setup["xyzh"].shape[0]
+    log(f"substep timing: mhd_dust_box_{nx} n={n}, bins {setups.LEVELS_4}")
+    s = integrator.State(p, setup["xyzh"], setup["vxyzu"], setup["alphaind"], Bevol=setup.get("Bevol"),
+                         dustevol=setup.get("dustevol"))
+    s.enable_bins(0)
+    ctx = sph.Context(p)
+    integrator.init_ind(ctx, s)
+    b = setup["bins"]["ibin_old"]
+    s.ibin[:] = b
+    s.ibin_old[:] = b
+    s.ibin_wake[:] = 0
+    s.iphase[:] = 1
+    s.nbinmax = 6
+    s.twas[:] = 0.5 * (p.dtmax / 2.0 ** b.astype(np.float64))
+    save = s.copy(device=True)
+    work = save.copy()
+    del s
+
+    def restore():
+        for k in integrator.State.ARRAYS:
+            a = getattr(save, k)
+            if a is not None:
+                getattr(work, k).copy_from(a)
+        for k in integrator.State.SCALARS:
+            setattr(work, k, getattr(save, k))
+
+    def host(t):
+        return integrator.evolve_ind(ctx, work, 1, time=t, tlast=0.0)[2][0]
+
+    def library(t):
+        return integrator.evolve_graph(ctx, work, 1, time=t, tlast=0.0, graphs=False)[3][0]
+
+    def graphs(t):
+        return integrator.evolve_graph(ctx, work, 1, time=t, tlast=0.0, graphs=True)[3][0]
+
+    res = {}
+    for m, frac in ((64, "1"), (32, "1/2"), (8, "1/8"), (1, "1/64")):
+        row = {}
+        for name, fn in (("host", host), ("library", library), ("graphs", graphs)):
+            ts = []
+            for r in range(reps + 1):
+                restore()
+                work.istepfrac = m - 1
+                t = (m - 1) / 64.0 * p.dtmax
+                device.synchronize()
+                t0 = time.perf_counter()
+                o = fn(t)
+                device.synchronize()
+                ts.append(time.perf_counter() - t0)
+            row[f"{name}_ms"] = round(1e3 * float(np.median(ts[1:])), 3)
+            row["nactive"] = o["nactive"]
+            row["its"] = o["its"]
+            if name == "graphs":
+                # kernel times of the last run (HIP events; the graph-replayed
+                # loop chains are not among them)
+                kt = {}
+                for nm, ms in ctx.timings():
+                    kt[nm] = kt.get(nm, 0.0) + ms
+                row["kernels_ms"] = {nm: round(v, 3) for nm, v in sorted(kt.items(), key=lambda x: -x[1])[:8]}
+        row["active_fraction"] = row["nactive"] / n
+        log(f"substep at active fraction {frac}: {row}")
+        res[frac] = row
+    ctx.close()
+    return {"workload": f"mhd_dust_box_{nx}", "n_particles": n, "nbinmax": 6, "levels": setups.LEVELS_4,
+            "fractions": res,
+            "note": "ms per substep (set_active_particles, predictor, drift, predict_sph, tree, density, "
+                    "cons2prim, force, corrector); state resident in HBM"}
 
 
 def measured_traffic(kernel, workload):

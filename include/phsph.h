@@ -438,22 +438,86 @@ int phsph_step_reset(phsph_ctx *ctx, int32_t np, double dtsph,
                      double *dustpred, const int8_t *iphase);
 
 /*
+ * Whole steps on device-resident state.  phsph_step_graph runs `nsub`
+ * steps of the reference's evolve loop with the control flow inside the
+ * library (no host round trip per loop):
+ *  - block timesteps (params.ind_timesteps): per substep evol_prestep's
+ *    istepfrac and set_active_particles (evolve.f90:200-222,
+ *    utils_indtimesteps.f90:114-185), step (step_leapfrog.f90:91-760) at
+ *    dt = dtmax/2**nbinmax, then update_time_and_dt's time and
+ *    change_nbinmax (evolve_utils.F90, utils_indtimesteps.f90:186-223);
+ *  - global timestep: step at clock->dt, then dt = min(dtforce, dtcourant,
+ *    dterr, dtmax + epsilon(dtmax)) (evolve.f90:105-160).
+ * step is predictor, substep_sph drift, predict_sph, derivs (maketree,
+ * densityiterate, cons2prim, force: deriv.f90:114-189), then the corrector
+ * with check_velocity_error (step_leapfrog.f90:769-843) and, while it
+ * fails, the reset and derivs(icall = 2), up to `maxits` iterations
+ * (PHSPH_ERR_NOCONVERGE when they do not converge and maxits > 1).
+ * With `graphs` = 1 the loops without data-dependent control (predictor,
+ * drift and predict_sph with their reductions; the corrector with its
+ * reduction) are captured as HIP graphs, replayed with the step's times
+ * through hipGraphExecUpdate; with 0 the same kernels are launched
+ * directly.  Both give bit-identical states, equal to those of the
+ * individual entry points called in the same order.
+ * Every array must be device memory (PHSPH_ERR_INPUT otherwise); arrays of
+ * physics that is off may be NULL.  The state must hold the derivatives of
+ * the current time (the first derivs and init_step of the reference's
+ * initial.F90 are made through the individual entry points).  `clock` is
+ * read and updated in place; `out` (room for nsub) receives one summary
+ * per step and may be NULL.  No reference counterpart as one call: it is
+ * the reference's evolve loop restricted to the SPH path (no sinks,
+ * external forces, cooling or dumps).
+ */
+typedef struct phsph_step_state {
+  double *xyzh, *vxyzu, *fxyzu, *vpred;   /* xyzh(4,:), vxyzu/fxyzu/vpred(ld,:) */
+  double *Bevol, *dBevol, *Bpred;         /* (4,:), MHD */
+  double *dustevol, *ddustevol, *dustpred, *dustfrac, *deltav, *tstop; /* one-fluid dust */
+  float *divcurlv, *divcurlB, *divBsymm, *alphaind, *gradh, *dvdx;
+  double *eos_vars;                       /* (3,:) */
+  float *poten;                           /* self-gravity potential, may be NULL */
+  int8_t *iphase, *ibin, *ibin_old, *ibin_wake;  /* block timesteps; iphase NULL: all active */
+  double *twas;
+  int32_t np, ldxyzh, ldvxyzu, ldfxyzu, lddivcurlv, ldgradh, ldeos, pad;
+} phsph_step_state;
+
+typedef struct phsph_step_clock {
+  double time, tlast;        /* evolve.f90 time and tlast (block timesteps) */
+  double dt;                 /* global timestep: dt of the next step */
+  double dtcourant, dtforce; /* of the last force call (global timestep) */
+  int32_t nbinmax, istepfrac, ibinnow, pad;
+} phsph_step_clock;
+
+typedef struct phsph_step_summary {
+  double dt, dtnew, dterr;   /* step dt, derivs' dtnew, timestep:dterr */
+  int32_t its, nactive, nwake, nvfloorp, nvfloorps, nvfloorc;
+  int32_t ngraph, ninst;     /* graph launches of the step; of them, freshly instantiated */
+} phsph_step_summary;
+
+int phsph_step_graph(phsph_ctx *ctx, const phsph_step_state *state,
+                     phsph_step_clock *clock, int32_t nsub, double ufloor,
+                     double tolv, int32_t maxits, int32_t graphs,
+                     phsph_step_summary *out);
+
+/*
  * Configuration of the neighbour-list kernels, picked by the library's
  * autotuner (no counterpart in the reference): candidate leaves staged per
- * chunk `cl` (8 or 16), wavefronts per block `lw` (2 or 4), and `r2` = 1 to
+ * chunk `cl` (8 or 16), wavefronts per block `lw` (2 or 4), `r2` = 1 to
  * store each list entry's squared separation for the h iteration (0:
- * recomputed from the positions).  Every configuration gives bit-identical
- * results.  By default the first densityiterate of a context with at least
- * 32768 active particles times each configuration once on the caller's
- * own particles and keeps the fastest.  phsph_set_tuning fixes a
- * configuration (cl = 0 asks for a new tuning on the next large call);
- * phsph_get_tuning returns the current one, whether it was tuned (tuned =
- * 1) and, if `ms`/`cfg` are given (room for 8), the time of each
- * configuration tried (cfg: cl, lw, r2 per configuration) and their count.
+ * recomputed from the positions), and `q` = 1 to run the density list
+ * kernel as persistent wavefronts drawing groups of active particles from
+ * per-XCD work queues (0: one group per wavefront).  Every configuration
+ * gives bit-identical results.  By default the first densityiterate of a
+ * context with at least 32768 active particles times each configuration
+ * once on the caller's own particles and keeps the fastest.
+ * phsph_set_tuning fixes a configuration (cl = 0 asks for a new tuning on
+ * the next large call); phsph_get_tuning returns the current one, whether
+ * it was tuned (tuned = 1) and, if `ms`/`cfg` are given (room for 16), the
+ * time of each configuration tried (cfg: cl, lw, r2, q per configuration)
+ * and their count.
  */
-int phsph_set_tuning(phsph_ctx *ctx, int32_t cl, int32_t lw, int32_t r2);
-int phsph_get_tuning(phsph_ctx *ctx, int32_t *cl, int32_t *lw, int32_t *r2, int32_t *tuned,
-                     float *ms, int32_t *cfg, int32_t *ntried);
+int phsph_set_tuning(phsph_ctx *ctx, int32_t cl, int32_t lw, int32_t r2, int32_t q);
+int phsph_get_tuning(phsph_ctx *ctx, int32_t *cl, int32_t *lw, int32_t *r2, int32_t *q,
+                     int32_t *tuned, float *ms, int32_t *cfg, int32_t *ntried);
 
 /* get_neighbour_stats (dens.F90:1137-1155) of the last densityiterate. */
 int phsph_get_neighbour_stats(phsph_ctx *ctx, double *trialmean,

@@ -74,6 +74,33 @@ class KdNodeGrav(C.Structure):
 assert C.sizeof(KdNodeGrav) == 120
 
 
+class StepState(C.Structure):
+    """phsph_step_state: device pointers of the arrays one step touches."""
+    _fields_ = [(k, C.c_void_p) for k in (
+        "xyzh", "vxyzu", "fxyzu", "vpred", "Bevol", "dBevol", "Bpred", "dustevol", "ddustevol", "dustpred",
+        "dustfrac", "deltav", "tstop", "divcurlv", "divcurlB", "divBsymm", "alphaind", "gradh", "dvdx", "eos_vars",
+        "poten", "iphase", "ibin", "ibin_old", "ibin_wake", "twas")] + \
+        [(k, C.c_int32) for k in ("np", "ldxyzh", "ldvxyzu", "ldfxyzu", "lddivcurlv", "ldgradh", "ldeos", "pad")]
+
+
+class StepClock(C.Structure):
+    """phsph_step_clock: evolve loop time and bin bookkeeping."""
+    _fields_ = [("time", C.c_double), ("tlast", C.c_double), ("dt", C.c_double), ("dtcourant", C.c_double),
+                ("dtforce", C.c_double), ("nbinmax", C.c_int32), ("istepfrac", C.c_int32), ("ibinnow", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+class StepSummary(C.Structure):
+    """phsph_step_summary: one step's dt, dtnew, dterr and counters."""
+    _fields_ = [("dt", C.c_double), ("dtnew", C.c_double), ("dterr", C.c_double), ("its", C.c_int32),
+                ("nactive", C.c_int32), ("nwake", C.c_int32), ("nvfloorp", C.c_int32), ("nvfloorps", C.c_int32),
+                ("nvfloorc", C.c_int32), ("ngraph", C.c_int32), ("ninst", C.c_int32)]
+
+
+assert C.sizeof(StepState) == 26 * 8 + 8 * 4
+assert C.sizeof(StepClock) == 5 * 8 + 4 * 4
+assert C.sizeof(StepSummary) == 3 * 8 + 8 * 4
+
 PI_PH = 3.1415926536            # physcon.f90:23
 MASS_PROTON_CGS = 1.67262158e-24  # physcon.f90:41
 CROSS_SECTION_H2_CGS = 2.367e-15  # physcon.f90:43

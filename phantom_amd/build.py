@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libphsph.so")
-SOURCES = ["api.hip", "tree.hip", "density.hip", "force.hip", "gravity.hip", "step.hip", "vjp.hip"]
+SOURCES = ["api.hip", "tree.hip", "density.hip", "force.hip", "gravity.hip", "step.hip", "substep.hip", "vjp.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PHSPH_ARCH", "gfx950")

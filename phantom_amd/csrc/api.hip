@@ -234,6 +234,7 @@ int phsph_create(const phsph_params *p, int32_t device, phsph_ctx **out) {
   // a blocking stream: work the caller queued on the null stream (copies
   // into device arrays it passes) completes before the library's kernels
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamDefault);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device);
   if (e != hipSuccess) {
     c->err = std::string("HIP error ") + hipGetErrorString(e);
     *out = c;
@@ -251,6 +252,11 @@ int phsph_destroy(phsph_ctx *c) {
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     ctx_clear_timings(c);
+    for (hipGraphExec_t &x : c->step_graph)
+      if (x) {
+        hipGraphExecDestroy(x);
+        x = nullptr;
+      }
     hipStreamDestroy(c->stream);
   }
   delete c;  // device buffers are released by their destructors
@@ -346,33 +352,35 @@ int phsph_get_tree(phsph_ctx *c, phsph_kdnode *node, int32_t *inoderange, int32_
   return PHSPH_OK;
 }
 
-int phsph_set_tuning(phsph_ctx *c, int32_t cl, int32_t lw, int32_t r2) {
+int phsph_set_tuning(phsph_ctx *c, int32_t cl, int32_t lw, int32_t r2, int32_t q) {
   if (!c) return PHSPH_ERR_INPUT;
   if (cl == 0) {
     c->tune_mode = TUNE_PENDING;
     return PHSPH_OK;
   }
-  if ((cl != 8 && cl != 16) || (lw != 2 && lw != 4) || (r2 != 0 && r2 != 1))
-    PH_RET(c, PHSPH_ERR_INPUT, "set_tuning: cl must be 8 or 16, lw 2 or 4, r2 0 or 1");
-  c->tune = ListTune{cl, lw, r2};
+  if ((cl != 8 && cl != 16) || (lw != 2 && lw != 4) || (r2 != 0 && r2 != 1) || (q != 0 && q != 1))
+    PH_RET(c, PHSPH_ERR_INPUT, "set_tuning: cl must be 8 or 16, lw 2 or 4, r2 and q 0 or 1");
+  c->tune = ListTune{cl, lw, r2, q};
   c->tune_mode = TUNE_FIXED;
   return PHSPH_OK;
 }
 
-int phsph_get_tuning(phsph_ctx *c, int32_t *cl, int32_t *lw, int32_t *r2, int32_t *tuned, float *ms, int32_t *cfg,
-                     int32_t *ntried) {
+int phsph_get_tuning(phsph_ctx *c, int32_t *cl, int32_t *lw, int32_t *r2, int32_t *q, int32_t *tuned, float *ms,
+                     int32_t *cfg, int32_t *ntried) {
   if (!c) return PHSPH_ERR_INPUT;
   if (cl) *cl = c->tune.cl;
   if (lw) *lw = c->tune.lw;
   if (r2) *r2 = c->tune.r2;
+  if (q) *q = c->tune.q;
   if (tuned) *tuned = c->tune_mode == TUNE_DONE ? 1 : 0;
   if (ntried) *ntried = c->tune_ntried;
   for (int k = 0; k < c->tune_ntried; k++) {
     if (ms) ms[k] = c->tune_ms[k];
     if (cfg) {
-      cfg[3 * k] = c->tune_tried[k].cl;
-      cfg[3 * k + 1] = c->tune_tried[k].lw;
-      cfg[3 * k + 2] = c->tune_tried[k].r2;
+      cfg[4 * k] = c->tune_tried[k].cl;
+      cfg[4 * k + 1] = c->tune_tried[k].lw;
+      cfg[4 * k + 2] = c->tune_tried[k].r2;
+      cfg[4 * k + 3] = c->tune_tried[k].q;
     }
   }
   return PHSPH_OK;
@@ -840,6 +848,49 @@ int phsph_step_corrector_noalloc(phsph_ctx *c, int32_t np, double timei, double 
         PH_RET(c, PHSPH_ERR_INPUT, "step_corrector_noalloc: every array must be device memory");
   }
   return step_corrector_noalloc_run(c, np, timei, dtsph, a, nbinmax, ufloor, errmax, v2sum, npc, nwake, nvfloor);
+}
+
+int phsph_step_graph(phsph_ctx *c, const phsph_step_state *st, phsph_step_clock *clock, int32_t nsub,
+                     double ufloor, double tolv, int32_t maxits, int32_t graphs, phsph_step_summary *out) {
+  if (!c) return PHSPH_ERR_INPUT;
+  if (!st || !clock || nsub < 0 || maxits < 1) PH_RET(c, PHSPH_ERR_INPUT, "step_graph: missing state or clock");
+  const phsph_step_state &S = *st;
+  const int np = S.np;
+  const bool mhd = c->p.mhd, dust = c->p.use_dustfrac && c->p.ndustsmall > 0, ind = c->p.ind_timesteps;
+  const bool grav = c->p.gravity;
+  if (np <= 0 || (int64_t)np >= (1 << 27)) PH_RET(c, PHSPH_ERR_INPUT, "step_graph: np outside 1..2^27-1");
+  if (S.ldxyzh < 4 || S.ldvxyzu < c->p.maxvxyzu || S.ldfxyzu < c->p.maxvxyzu || S.ldvxyzu < 3 ||
+      S.lddivcurlv < 1 || S.ldgradh < (grav ? 2 : 1) || S.ldeos < 2)
+    PH_RET(c, PHSPH_ERR_INPUT, "step_graph: leading dimension below the component count");
+  if (ind && (clock->nbinmax < 0 || clock->nbinmax > 30))
+    PH_RET(c, PHSPH_ERR_INPUT, "step_graph: nbinmax outside 0..30");
+  // every array the step reads or writes, device memory only
+  const void *need[] = {S.xyzh, S.vxyzu, S.fxyzu, S.vpred, S.divcurlv, S.alphaind, S.gradh, S.dvdx, S.eos_vars};
+  for (const void *q : need)
+    if (!is_device_ptr(q)) PH_RET(c, PHSPH_ERR_INPUT, "step_graph: every array must be device memory");
+  if (mhd) {
+    const void *m[] = {S.Bevol, S.dBevol, S.Bpred, S.divcurlB, S.divBsymm};
+    for (const void *q : m)
+      if (!is_device_ptr(q)) PH_RET(c, PHSPH_ERR_INPUT, "step_graph: MHD needs device Bevol, dBevol, Bpred, divcurlB, divBsymm");
+  }
+  if (dust) {
+    const void *d[] = {S.dustevol, S.ddustevol, S.dustpred, S.dustfrac, S.deltav, S.tstop};
+    for (const void *q : d)
+      if (!is_device_ptr(q)) PH_RET(c, PHSPH_ERR_INPUT, "step_graph: dust needs its device arrays");
+  }
+  if (ind) {
+    const void *b[] = {S.iphase, S.ibin, S.ibin_old, S.ibin_wake, S.twas};
+    for (const void *q : b)
+      if (!is_device_ptr(q)) PH_RET(c, PHSPH_ERR_INPUT, "step_graph: block timesteps need device iphase and bins");
+  } else if (S.iphase && !is_device_ptr(S.iphase)) {
+    PH_RET(c, PHSPH_ERR_INPUT, "step_graph: every array must be device memory");
+  }
+  if (grav && S.poten && !is_device_ptr(S.poten)) PH_RET(c, PHSPH_ERR_INPUT, "step_graph: poten must be device memory");
+  PH_HIPCHECK(c, hipSetDevice(c->device));
+  phsph_step_clock k = *clock;
+  int rc = step_graph_run(c, S, k, nsub, ufloor, tolv, maxits, graphs != 0, out);
+  *clock = k;
+  return rc;
 }
 
 int phsph_step_reset(phsph_ctx *c, int32_t np, double dtsph, double *vxyzu, int32_t ldv, const double *fxyzu,

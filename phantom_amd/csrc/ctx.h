@@ -79,13 +79,15 @@ struct Caps {
 // list-kernel configuration picked by the autotuner (density.hip):
 // candidate leaves staged per chunk (8 or 16), wavefronts per block (2 or
 // 4), squared separations stored with the density lists (1) or recomputed
-// from the positions in the h iteration (0)
+// from the positions in the h iteration (0), and the density list kernel
+// launched one group per wavefront (0) or as persistent wavefronts drawing
+// groups from per-XCD work queues (1)
 struct ListTune {
-  int cl = 16, lw = 4, r2 = 0;
+  int cl = 16, lw = 4, r2 = 0, q = 0;
 };
 enum { TUNE_FIXED = 0, TUNE_PENDING = 1, TUNE_DONE = 2 };
 constexpr int TUNE_MIN_ACTIVE = 32768;  // smallest density call the autotuner times
-constexpr int TUNE_MAX_CONFIGS = 8;
+constexpr int TUNE_MAX_CONFIGS = 16;
 
 struct phsph_ctx {
   phsph_params p;
@@ -234,8 +236,17 @@ struct phsph_ctx {
   DBuf<double> vjp_x0, vjp_x1, vjp_vh, vjp_a, vjp_t3, vjp_t3b;
   DBuf<float> vjp_dcv, vjp_alpha, vjp_gradh, vjp_dvdx;
 
+  // work-queue counters of the persistent list kernels (8 per kernel)
+  DBuf<int32_t> wq;
+  int ncu = 0;  // compute units of the device
+
   // integrator reductions (block partials of the step loops)
   DBuf<double> step_red;
+  // whole steps (substep.hip): the captured loop chains (0: predictor,
+  // drift, predict_sph; 1: corrector), how often they were instantiated,
+  // updated in place and launched
+  hipGraphExec_t step_graph[2] = {nullptr, nullptr};
+  int64_t step_graph_inst = 0, step_graph_upd = 0, step_graph_launch = 0;
   float *poten_out = nullptr;         // device view of poten_user during a force call
 
   // staging buffers for host arrays
@@ -363,6 +374,24 @@ int step_corrector_run(phsph_ctx *c, int np, double timei, double dtsph, const S
                        double ufloor, double *errmax, double *v2sum, int32_t *npc, int32_t *nwake,
                        int32_t *nvfloor);
 int step_reset_run(phsph_ctx *c, int np, double dtsph, const StepArgs &s);
+// Enqueue-only forms of the step loops for whole steps (substep.hip): no
+// timer, no synchronisation and no allocation, so they can be captured
+// into a HIP graph; step_chain_prepare sizes the reduction buffer first.
+// The reductions of the predictor, predict_sph and the corrector land in
+// three consecutive result slots that step_chain_read copies back.
+struct StepSums {
+  double emax, v2;
+  int np, nwake, nfloor, nbad;
+};
+int step_chain_prepare(phsph_ctx *c, int np);
+void step_chain_pre(phsph_ctx *c, int np, double t, double dtsph, const StepArgs &pred, const StepArgs &psph,
+                    double ufloor);
+void step_chain_post(phsph_ctx *c, int np, double timei, double dtsph, const StepArgs &corr, int nbinmax,
+                     double ufloor);
+int step_chain_read(phsph_ctx *c, int np, StepSums *out3);
+// substep.hip
+int step_graph_run(phsph_ctx *c, const phsph_step_state &S, phsph_step_clock &k, int nsub, double ufloor,
+                   double tolv, int maxits, bool graphs, phsph_step_summary *out);
 int step_scratch_init(phsph_ctx *c);
 int step_corrector_noalloc_run(phsph_ctx *c, int np, double timei, double dtsph, const StepArgs &s, int nbinmax,
                                double ufloor, double *errmax, double *v2sum, int32_t *npc, int32_t *nwake,

@@ -317,9 +317,24 @@ __global__ void __launch_bounds__(1024) k_scan_i32(int32_t *a, int n, int32_t *t
 // The iteration itself evaluates only the sums the
 // Newton step needs (rho and dOmega); the full set of get_density_sums is
 // taken once, at the h of the last iteration, by part 2.
+// one neighbour's terms of the h iteration's density and gradh sums
+// (get_density_sums, dens.F90:646-692) at 1/h = hi1, 1/h^2 = hi21
+template <int K>
+__device__ __forceinline__ void dens_iter_term(double r2, double hi1, double hi21, double R2, double pm,
+                                               double &srho, double &sgh) {
+  const double q2 = xmul(r2, hi21);
+  if (q2 < R2) {
+    const double q = sqrt(r2) * hi1;
+    double wk, gk;
+    kern_wg<K, double>(q2, q, wk, gk);
+    srho += wk * pm;
+    sgh += (-q * gk - 3. * wk) * pm;
+  }
+}
+
 // ------------------------------------------------------------------------
-template <int K, int CL, int LW, bool STORE_R2>
-__global__ void __launch_bounds__(64 * LW)
+template <int K, int CL, int LW, bool STORE_R2, bool QUEUE>
+__global__ void __launch_bounds__(64 * LW) __attribute__((amdgpu_waves_per_eu(5)))
 k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act, int nact, int ngroups,
             const int32_t *__restrict__ pleaf, const int32_t *__restrict__ leaves,
             const uint32_t *__restrict__ gcand, const int32_t *__restrict__ gncand, const int64_t *goff, int capg,
@@ -330,11 +345,12 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
             int32_t *__restrict__ t_nlj,
             double *__restrict__ t_hsum, double *__restrict__ t_hfin,
             int32_t *__restrict__ p_nits, int32_t *__restrict__ leaf_nits, int32_t *__restrict__ retry,
-            double *__restrict__ htraj, double *__restrict__ hconv, int32_t *__restrict__ flags) {
+            double *__restrict__ htraj, double *__restrict__ hconv, int32_t *__restrict__ flags,
+            int32_t *__restrict__ wq) {
   __shared__ ListStage<CL> Sw[LW];
   ListStage<CL> &S = Sw[threadIdx.x >> 6];
-  const int g = list_group<LW>(gridDim.x);
-  if (g >= ngroups) return;
+  // one group of 64 active particles: its lists and h iteration
+  auto group = [&](const int g) {
   const int lane = threadIdx.x & 63;
   const int t = g * GROUP + lane;
   const bool live = t < nact && act[t] >= 0;  // act[t] < 0: padding of a partial group
@@ -359,6 +375,13 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
   // sums read only those), entries only within the neighbour's radius fill
   // it from the back (for the force pass)
   int nl = 0, nlj = 0;
+  // The sums of the first Newton iteration (h = h0) are taken while the list
+  // is built: the own entries arrive in list order with the separations a
+  // pass over the list computes, so these are that pass's sums.
+  const double pm = dc.pmass;
+  const double R2 = dc.radkern2;
+  const double h01 = 1. / h0, h021 = xmul(h01, h01);
+  double srho0 = 0., sgh0 = 0.;
   build_list(
       S, dc, gcand + cand_base(goff, g, capg), gncand[g], nd, nd_lo, nd_hi, hmax0, tp, xi, yi, zi, lcx, lcy, lcz, lsz,
       [&](double ccx, double ccy, double ccz, double csz, double chm) {
@@ -372,6 +395,7 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
             nlist[a] = cand_pack(j, code);
             if (STORE_R2 && own) nr2[a] = r2;  // the separation the iteration needs, exact as get_density_sums takes it
           }
+          if (own) dens_iter_term<K>(r2, h01, h021, R2, pm, srho0, sgh0);
           nl += own ? 1 : 0;
           nlj += own ? 0 : 1;
         }
@@ -383,25 +407,17 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
     return;
   }
   // --- Newton-Raphson on h (finish_cell, dens.F90:1382-1466) ---
-  const double pm = dc.pmass;
-  const double R2 = dc.radkern2;
   double hi = h0, hfin = h0;
   int it = 0;
   for (;;) {
     it++;
     const double hi1 = 1. / hi, hi21 = xmul(hi1, hi1);
     double srho = 0., sgh = 0.;
-    auto term = [&](double r2) {
-      const double q2 = xmul(r2, hi21);
-      if (q2 < R2) {
-        const double q = sqrt(r2) * hi1;
-        double wk, gk;
-        kern_wg<K, double>(q2, q, wk, gk);
-        srho += wk * pm;
-        sgh += (-q * gk - 3. * wk) * pm;
-      }
-    };
-    if constexpr (STORE_R2) {
+    auto term = [&](double r2) { dens_iter_term<K>(r2, hi1, hi21, R2, pm, srho, sgh); };
+    if (it == 1) {
+      srho = srho0;
+      sgh = sgh0;
+    } else if constexpr (STORE_R2) {
       // the squared separations the search stored with the list, two
       // entries (16 bytes of the slot's segment) per load, read ahead
       double2 rn = make_double2(0., 0.);
@@ -475,6 +491,31 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
   t_hfin[t] = hfin;
   p_nits[p] = (icall == 0) ? 1 : it;
   atomicMax(&leaf_nits[l], (icall == 0) ? 1 : it);
+  };
+  if constexpr (!QUEUE) {
+    // one group per wavefront, XCD-contiguous
+    const int g = list_group<LW>(gridDim.x);
+    if (g < ngroups) group(g);
+  } else {
+    // persistent wavefronts: the groups are split into 8 contiguous ranges,
+    // one per XCD (blocks are dispatched to the XCDs round robin); each
+    // wavefront takes the next group of its XCD's range from that range's
+    // counter and, once the range is exhausted, of the other ranges in
+    // turn.  Counters only grow, so every wavefront ends after at most
+    // 8 exhausted draws plus the groups it ran.
+    const int x = blockIdx.x & 7, per = (ngroups + 7) >> 3;
+    for (int k = 0; k < 8; k++) {
+      const int r = (x + k) & 7;
+      for (;;) {
+        int tk = 0;
+        if ((threadIdx.x & 63) == 0) tk = atomicAdd(&wq[r], 1);
+        tk = __shfl(tk, 0, 64);
+        const int g = r * per + tk;
+        if (tk >= per || g >= ngroups) break;
+        group(g);
+      }
+    }
+  }
 }
 
 // store_results (dens.F90:1511-1681) at the final h from the sums taken at
@@ -854,24 +895,44 @@ struct IterLaunch {
   double *ths, *thf;
 };
 
-template <int K, int CL, int LW, bool R2>
+template <int K, int CL, int LW, bool R2, bool Q>
 static void launch_iter_t(const IterLaunch &L) {
   phsph_ctx *c = L.c;
-  hipLaunchKernelGGL((k_dens_iter<K, CL, LW, R2>), dim3(list_grid<LW>(L.ng)), dim3(64 * LW), 0, L.s, c->dc, L.icall,
+  unsigned grid = list_grid<LW>(L.ng);
+  if (Q) {
+    // persistent: as many blocks as are resident at once (never more than
+    // one group per wavefront), their queue counters cleared first
+    int bpc = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, reinterpret_cast<const void *>(
+                                                                &k_dens_iter<K, CL, LW, R2, true>),
+                                                     64 * LW, 0) != hipSuccess || bpc < 1)
+      bpc = 1;
+    const unsigned res = xcd_grid(bpc * (c->ncu > 0 ? c->ncu : 256));
+    if (res < grid) grid = res;
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, L.s, c->wq.p, 8, 0);
+  }
+  hipLaunchKernelGGL((k_dens_iter<K, CL, LW, R2, Q>), dim3(grid), dim3(64 * LW), 0, L.s, c->dc, L.icall,
                      L.rfac, c->act.p, L.na, L.ng, c->pleaf.p, c->leaves.p, c->gcand.p, c->gncand.p, c->goff_p,
                      c->caps.gcand, c->nd_lo.p, c->nd_hi.p, c->nd.p, c->tp4.p, L.widen ? c->hmax0.p : nullptr,
                      L.lmargin, L.lb, L.r2b, L.dlist, L.tnl, L.widen ? c->t_nlj.p : nullptr, L.ths, L.thf,
-                     c->p_nits.p, c->leaf_nits.p, c->tmpi.p, c->htraj.p, c->hconv.p, c->counters.p);
+                     c->p_nits.p, c->leaf_nits.p, c->tmpi.p, c->htraj.p, c->hconv.p, c->counters.p, c->wq.p);
+}
+
+template <int K, int CL, int LW>
+static void launch_iter_r(const ListTune &t, const IterLaunch &L) {
+  if (t.r2) {
+    if (t.q) launch_iter_t<K, CL, LW, true, true>(L); else launch_iter_t<K, CL, LW, true, false>(L);
+  } else {
+    if (t.q) launch_iter_t<K, CL, LW, false, true>(L); else launch_iter_t<K, CL, LW, false, false>(L);
+  }
 }
 
 template <int K>
 static void launch_iter_k(const ListTune &t, const IterLaunch &L) {
   if (t.cl == 8) {
-    if (t.lw == 2) { if (t.r2) launch_iter_t<K, 8, 2, true>(L); else launch_iter_t<K, 8, 2, false>(L); }
-    else           { if (t.r2) launch_iter_t<K, 8, 4, true>(L); else launch_iter_t<K, 8, 4, false>(L); }
+    if (t.lw == 2) launch_iter_r<K, 8, 2>(t, L); else launch_iter_r<K, 8, 4>(t, L);
   } else {
-    if (t.lw == 2) { if (t.r2) launch_iter_t<K, 16, 2, true>(L); else launch_iter_t<K, 16, 2, false>(L); }
-    else           { if (t.r2) launch_iter_t<K, 16, 4, true>(L); else launch_iter_t<K, 16, 4, false>(L); }
+    if (t.lw == 2) launch_iter_r<K, 16, 2>(t, L); else launch_iter_r<K, 16, 4>(t, L);
   }
 }
 
@@ -901,8 +962,9 @@ int tune_density_iter(phsph_ctx *c, const IterLaunch &L) {
   ListTune bt = c->tune;
   for (int cl : {8, 16})
     for (int lw : {2, 4})
-      for (int r2 : {0, 1}) {
-        const ListTune t{cl, lw, r2};
+      for (int r2 : {0, 1})
+        for (int q : {0, 1}) {
+        const ListTune t{cl, lw, r2, q};
         PH_HIPCHECK(c, hipEventRecord(e0, L.s));
         launch_dens_iter(t, L2);
         PH_HIPCHECK(c, hipEventRecord(e1, L.s));
@@ -1207,6 +1269,7 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   if (have_f) PH_HIPCHECK(c, c->tf4.ensure(n));
   if (mhd) PH_HIPCHECK(c, c->tB4.ensure(n));
   PH_HIPCHECK(c, c->tmpi.ensure(n));
+  PH_HIPCHECK(c, c->wq.ensure(16));
   // mixed precision covers the hydro sums; with MHD the pass stays FP64
   const bool mixed = c->p.precision == PHSPH_PREC_MIXED && !mhd;
   c->dens_prec_used = mixed ? PHSPH_PREC_MIXED : PHSPH_PREC_FP64;

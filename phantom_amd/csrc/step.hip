@@ -486,3 +486,49 @@ int step_reset_run(phsph_ctx *c, int np, double dtsph, const StepArgs &s) {
   PH_HIPCHECK(c, hipStreamSynchronize(c->stream));
   return PHSPH_OK;
 }
+
+// ---- enqueue-only chains of whole steps ------------------------------------
+// The reduction buffer holds the block partials (reused by each loop in
+// stream order) and three result slots: predictor, predict_sph, corrector.
+
+static_assert(sizeof(StepSums) == sizeof(StepPart), "StepSums mirrors StepPart");
+
+int step_chain_prepare(phsph_ctx *c, int np) {
+  const int nb = np > 0 ? (int)nblocks(np, SBS) : 1;
+  const size_t words = (size_t)(nb + 3) * (sizeof(StepPart) / sizeof(double));
+  PH_HIPCHECK(c, c->step_red.ensure(words));
+  return PHSPH_OK;
+}
+
+void step_chain_pre(phsph_ctx *c, int np, double t, double dtsph, const StepArgs &pred, const StepArgs &psph,
+                    double ufloor) {
+  const int nb = np > 0 ? (int)nblocks(np, SBS) : 1;
+  StepPart *parts = reinterpret_cast<StepPart *>(c->step_red.p), *out = parts + nb;
+  hipStream_t s = c->stream;
+  const StepFlags fl = make_flags(c, ufloor, 0);
+  hipLaunchKernelGGL(k_predictor, dim3(nb), dim3(SBS), 0, s, pred, fl, t, 0.5 * dtsph, np, parts);
+  hipLaunchKernelGGL(k_step_reduce, dim3(1), dim3(SBS), 0, s, parts, nb, out);
+  if (np > 0)
+    hipLaunchKernelGGL(k_drift, dim3(nb), dim3(SBS), 0, s, psph.xyzh, psph.ldx, psph.vxyzu, psph.ldv, dtsph, np);
+  const double timei = t + dtsph;
+  hipLaunchKernelGGL(k_predict_sph, dim3(nb), dim3(SBS), 0, s, psph, fl, timei, dtsph, 0.5 * dtsph, np, parts);
+  hipLaunchKernelGGL(k_step_reduce, dim3(1), dim3(SBS), 0, s, parts, nb, out + 1);
+}
+
+void step_chain_post(phsph_ctx *c, int np, double timei, double dtsph, const StepArgs &corr, int nbinmax,
+                     double ufloor) {
+  const int nb = np > 0 ? (int)nblocks(np, SBS) : 1;
+  StepPart *parts = reinterpret_cast<StepPart *>(c->step_red.p), *out = parts + nb;
+  const BinTable tab = make_bin_table(c->p.dtmax, timei);
+  hipLaunchKernelGGL(k_corrector, dim3(nb), dim3(SBS), 0, c->stream, corr, make_flags(c, ufloor, nbinmax), tab,
+                     timei, 0.5 * dtsph, np, parts);
+  hipLaunchKernelGGL(k_step_reduce, dim3(1), dim3(SBS), 0, c->stream, parts, nb, out + 2);
+}
+
+int step_chain_read(phsph_ctx *c, int np, StepSums *out3) {
+  const int nb = np > 0 ? (int)nblocks(np, SBS) : 1;
+  const StepPart *out = reinterpret_cast<const StepPart *>(c->step_red.p) + nb;
+  PH_HIPCHECK(c, hipMemcpyAsync(out3, out, 3 * sizeof(StepPart), hipMemcpyDeviceToHost, c->stream));
+  PH_HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return PHSPH_OK;
+}

@@ -49,7 +49,8 @@ static inline unsigned xcd_grid(int ngroups) { return (unsigned)(((ngroups + 7) 
 // so that more waves share a CU; each wavefront stages through its own LDS
 // and synchronises only itself.  LW (2 or 4) and the chunk length CL of
 // build_list (8 or 16 candidate leaves) are picked per context by the
-// autotuner (tune.h); every choice gives bit-identical results.
+// autotuner (tune_density_iter, density.hip); every choice gives
+// bit-identical results.
 template <int LW>
 static inline unsigned list_grid(int ngroups) { return xcd_grid((ngroups + LW - 1) / LW); }
 template <int LW>

@@ -40,6 +40,26 @@ module phsph
     integer(c_int32_t) :: leftchild, rightchild, parent, idum
  end type phsph_kdnode
 
+ ! phsph_step_state, phsph_step_clock, phsph_step_summary (include/phsph.h):
+ ! whole steps on device-resident arrays (c_ptr of HBM allocations)
+ type, bind(c), public :: phsph_step_state_t
+    type(c_ptr) :: xyzh, vxyzu, fxyzu, vpred, Bevol, dBevol, Bpred
+    type(c_ptr) :: dustevol, ddustevol, dustpred, dustfrac, deltav, tstop
+    type(c_ptr) :: divcurlv, divcurlB, divBsymm, alphaind, gradh, dvdx, eos_vars, poten
+    type(c_ptr) :: iphase, ibin, ibin_old, ibin_wake, twas
+    integer(c_int32_t) :: np, ldxyzh, ldvxyzu, ldfxyzu, lddivcurlv, ldgradh, ldeos, pad
+ end type phsph_step_state_t
+
+ type, bind(c), public :: phsph_step_clock_t
+    real(c_double)     :: time, tlast, dt, dtcourant, dtforce
+    integer(c_int32_t) :: nbinmax, istepfrac, ibinnow, pad
+ end type phsph_step_clock_t
+
+ type, bind(c), public :: phsph_step_summary_t
+    real(c_double)     :: dt, dtnew, dterr
+    integer(c_int32_t) :: its, nactive, nwake, nvfloorp, nvfloorps, nvfloorc, ngraph, ninst
+ end type phsph_step_summary_t
+
  type(c_ptr), save :: ctx = c_null_ptr
 
  public :: phsph_default, phsph_init, phsph_finalise, phsph_errmsg
@@ -47,7 +67,7 @@ module phsph
  public :: phsph_neighbour_stats
  public :: phsph_set_bins, phsph_get_nbinmax, phsph_set_active_particles, phsph_set_poten
  public :: phsph_init_step, phsph_step_predictor, phsph_substep_sph, phsph_step_predict_sph
- public :: phsph_step_corrector, phsph_step_reset
+ public :: phsph_step_corrector, phsph_step_reset, phsph_step_graph
 
  interface
     subroutine c_default_params(p, kernel) bind(c, name='phsph_default_params')
@@ -170,6 +190,16 @@ module phsph
        real(c_double), value :: timei, dtsph, ufloor
        real(c_double), intent(out) :: errmax, v2sum
        integer(c_int32_t), intent(out) :: npc, nwake, nvfloor
+    end function
+    integer(c_int) function c_step_graph(ctx, st, clk, nsub, ufloor, tolv, maxits, graphs, summaries) &
+                                         bind(c, name='phsph_step_graph')
+       import :: c_int, c_int32_t, c_double, c_ptr, phsph_step_state_t, phsph_step_clock_t, phsph_step_summary_t
+       type(c_ptr), value :: ctx
+       type(phsph_step_state_t), intent(in) :: st
+       type(phsph_step_clock_t), intent(inout) :: clk
+       integer(c_int32_t), value :: nsub, maxits, graphs
+       real(c_double), value :: ufloor, tolv
+       type(phsph_step_summary_t), intent(out) :: summaries(*)
     end function
     integer(c_int) function c_step_reset(ctx, np, dtsph, vxyzu, ldv, fxyzu, ldf, vpred, Bevol, dBevol, Bpred, &
                                          dustevol, ddustevol, dustpred, iphase) bind(c, name='phsph_step_reset')
@@ -454,5 +484,20 @@ integer function phsph_step_reset(npart, dtsph, vxyzu, fxyzu, vpred, Bevol, dBev
                      ptr_or_null(Bpred), ptr_or_null(dustevol), ptr_or_null(ddustevol), ptr_or_null(dustpred), &
                      ptr_or_null_i1(iphase))
 end function phsph_step_reset
+
+!-- nsub steps of the evolve loop inside the library on device-resident state
+!   (evolve.f90:105-257 with step, step_leapfrog.f90:91-843)
+integer function phsph_step_graph(st, clk, nsub, ufloor, tolv, maxits, graphs, summaries) result(ierr)
+ type(phsph_step_state_t), intent(in) :: st
+ type(phsph_step_clock_t), intent(inout) :: clk
+ integer, intent(in) :: nsub, maxits
+ real(c_double), intent(in) :: ufloor, tolv
+ logical, intent(in) :: graphs
+ type(phsph_step_summary_t), intent(out) :: summaries(nsub)
+ integer(c_int32_t) :: ig
+ ig = 0
+ if (graphs) ig = 1
+ ierr = c_step_graph(ctx, st, clk, int(nsub, c_int32_t), ufloor, tolv, int(maxits, c_int32_t), ig, summaries)
+end function phsph_step_graph
 
 end module phsph

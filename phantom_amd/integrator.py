@@ -79,6 +79,36 @@ class State:
         self.dtcourant = BIGNUMBER
         self.dtforce = BIGNUMBER
 
+    ARRAYS = ("xyzh", "vxyzu", "vpred", "fxyzu", "alphaind", "divcurlv", "gradh", "dvdx", "eos_vars", "Bevol",
+              "Bpred", "dBevol", "divcurlB", "divBsymm", "dustevol", "dustpred", "ddustevol", "dustfrac", "deltav",
+              "tstop", "iphase", "ibin", "ibin_old", "ibin_wake", "twas")
+    SCALARS = ("nbinmax", "istepfrac", "ibinnow", "dtcourant", "dtforce")
+
+    def copy(self, device=None):
+        """A copy of the state: device=True puts every array in HBM, False
+        on the host, None keeps each where it is."""
+        from .device import DeviceArray
+        c = object.__new__(State)
+        c.params, c.n = self.params, self.n
+        for k in self.ARRAYS:
+            a = getattr(self, k)
+            on_dev = a is not None and not isinstance(a, np.ndarray)
+            want = on_dev if device is None else device
+            if a is None:
+                b = None
+            elif want:
+                if on_dev:
+                    b = DeviceArray(a.shape, a.dtype)
+                    b.copy_from(a)
+                else:
+                    b = DeviceArray.from_numpy(a)
+            else:
+                b = a.numpy() if on_dev else a.copy()
+            setattr(c, k, b)
+        for k in self.SCALARS:
+            setattr(c, k, getattr(self, k))
+        return c
+
     def enable_bins(self, nbinmax, ibin=None):
         """Allocate the block-timestep arrays (host side; the library stages
         them).  ibin defaults to nbinmax for every particle."""
@@ -273,3 +303,47 @@ def evolve_ind(ctx, s, nsub, time=0.0, tlast=0.0, **kw):
             s.istepfrac = 0
             tlast = time
     return time, tlast, out
+
+
+def step_state(s):
+    """The phsph_step_state of a state whose arrays are all in HBM."""
+    from . import abi
+    st = abi.StepState()
+
+    def ptr(a):
+        if a is None:
+            return None
+        if isinstance(a, np.ndarray):
+            raise ValueError("step_graph needs device arrays (State.copy(device=True))")
+        return a.ptr
+
+    for k in State.ARRAYS:
+        setattr(st, k, ptr(getattr(s, k)))
+    st.poten = None
+    st.np = s.n
+    st.ldxyzh = 4
+    st.ldvxyzu = s.vxyzu.shape[1]
+    st.ldfxyzu = s.fxyzu.shape[1]
+    st.lddivcurlv = s.divcurlv.shape[1]
+    st.ldgradh = s.gradh.shape[1]
+    st.ldeos = s.eos_vars.shape[1]
+    return st
+
+
+def evolve_graph(ctx, s, nsub, time=0.0, tlast=0.0, dt=None, graphs=True, ufloor=0.0, tolv=1.e-2, maxits=MAXITS):
+    """nsub steps of evolve_ind (block timesteps) or evolve_global inside
+    the library (phsph_step_graph), on a state whose arrays are all in HBM.
+    Returns (time, tlast, dt, the list of step summaries); dt is the next
+    global timestep (global timestepping) and the state's bin counters and
+    dt constraints are updated as evolve_ind / evolve_global leave them."""
+    from . import abi
+    p = s.params
+    k = abi.StepClock()
+    k.time, k.tlast = time, tlast
+    k.dtcourant, k.dtforce = s.dtcourant, s.dtforce
+    k.dt = min(s.dtforce, s.dtcourant, p.dtmax) if dt is None else dt
+    k.nbinmax, k.istepfrac, k.ibinnow = s.nbinmax, s.istepfrac, s.ibinnow
+    out = ctx.step_graph(step_state(s), k, nsub, ufloor=ufloor, tolv=tolv, maxits=maxits, graphs=graphs)
+    s.nbinmax, s.istepfrac, s.ibinnow = k.nbinmax, k.istepfrac, k.ibinnow
+    s.dtcourant, s.dtforce = k.dtcourant, k.dtforce
+    return k.time, k.tlast, k.dt, out

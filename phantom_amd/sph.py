@@ -37,7 +37,7 @@ EXPORTED = [
     "phsph_get_quads", "phsph_init_step", "phsph_step_predictor", "phsph_substep_sph", "phsph_step_predict_sph",
     "phsph_step_corrector", "phsph_step_reset", "phsph_get_precision", "phsph_density_vjp", "phsph_force_vjp",
     "phsph_step_vjp", "phsph_step_corrector_noalloc", "phsph_get_tree_grav", "phsph_set_tuning",
-    "phsph_get_tuning",
+    "phsph_get_tuning", "phsph_step_graph",
 ]
 
 _lib = None
@@ -86,8 +86,8 @@ def load_library(path=LIB_PATH):
     L.phsph_set_poten.argtypes = [vp, vp]
     L.phsph_get_quads.argtypes = [vp, vp, i64]
     L.phsph_get_tree_grav.argtypes = [vp, vp, i64]
-    L.phsph_set_tuning.argtypes = [vp, i32, i32, i32]
-    L.phsph_get_tuning.argtypes = [vp, P(i32), P(i32), P(i32), P(i32), vp, vp, P(i32)]
+    L.phsph_set_tuning.argtypes = [vp, i32, i32, i32, i32]
+    L.phsph_get_tuning.argtypes = [vp, P(i32), P(i32), P(i32), P(i32), P(i32), vp, vp, P(i32)]
     L.phsph_set_active_particles.argtypes = [vp, i32, vp, i32, vp, vp, i32, i32, P(i32), P(i32), P(i32)]
     L.phsph_init_step.argtypes = [vp, i32, dbl, vp, i32, vp]
     L.phsph_step_predictor.argtypes = [vp, i32, dbl, dbl, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp,
@@ -98,6 +98,7 @@ def load_library(path=LIB_PATH):
     L.phsph_step_corrector.argtypes = [vp, i32, dbl, dbl, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp,
                                        vp, i32, vp, dbl, P(dbl), P(dbl), P(i32), P(i32), P(i32)]
     L.phsph_step_corrector_noalloc.argtypes = L.phsph_step_corrector.argtypes
+    L.phsph_step_graph.argtypes = [vp, vp, vp, i32, dbl, dbl, i32, i32, vp]
     L.phsph_step_reset.argtypes = [vp, i32, dbl, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]
     for name in EXPORTED:
         if name != "phsph_default_params" and name != "phsph_last_error":
@@ -203,23 +204,24 @@ class Context:
         nlive = int(np.count_nonzero(inodeparts))
         return dict(node=node, inoderange=inoderange, inodeparts=inodeparts[:nlive], maxlevel=ml.value, ncells=nc)
 
-    def set_tuning(self, cl=0, lw=0, r2=0):
-        """Fix the list-kernel configuration (cl 8|16, lw 2|4, r2 0|1), or
-        with cl = 0 ask the autotuner to time them on the next large call."""
-        self._check(self.lib.phsph_set_tuning(self._h, cl, lw, r2))
+    def set_tuning(self, cl=0, lw=0, r2=0, q=0):
+        """Fix the list-kernel configuration (cl 8|16, lw 2|4, r2 0|1, q
+        0|1), or with cl = 0 ask the autotuner to time them on the next
+        large call."""
+        self._check(self.lib.phsph_set_tuning(self._h, cl, lw, r2, q))
 
     def get_tuning(self):
         """Current configuration, whether it was tuned, and the time (ms) of
         each configuration the autotuner tried."""
-        cl, lw, r2, tuned, nt = (C.c_int32() for _ in range(5))
-        ms = np.zeros(8, np.float32)
-        cfg = np.zeros(24, np.int32)
-        self._check(self.lib.phsph_get_tuning(self._h, C.byref(cl), C.byref(lw), C.byref(r2), C.byref(tuned),
-                                              ms.ctypes.data_as(C.c_void_p), cfg.ctypes.data_as(C.c_void_p),
-                                              C.byref(nt)))
-        tried = [dict(cl=int(cfg[3 * k]), lw=int(cfg[3 * k + 1]), r2=int(cfg[3 * k + 2]), ms=float(ms[k]))
-                 for k in range(nt.value)]
-        return dict(cl=cl.value, lw=lw.value, r2=r2.value, tuned=bool(tuned.value), tried=tried)
+        cl, lw, r2, q, tuned, nt = (C.c_int32() for _ in range(6))
+        ms = np.zeros(16, np.float32)
+        cfg = np.zeros(64, np.int32)
+        self._check(self.lib.phsph_get_tuning(self._h, C.byref(cl), C.byref(lw), C.byref(r2), C.byref(q),
+                                              C.byref(tuned), ms.ctypes.data_as(C.c_void_p),
+                                              cfg.ctypes.data_as(C.c_void_p), C.byref(nt)))
+        tried = [dict(cl=int(cfg[4 * k]), lw=int(cfg[4 * k + 1]), r2=int(cfg[4 * k + 2]), q=int(cfg[4 * k + 3]),
+                      ms=float(ms[k])) for k in range(nt.value)]
+        return dict(cl=cl.value, lw=lw.value, r2=r2.value, q=q.value, tuned=bool(tuned.value), tried=tried)
 
     def get_tree_grav(self):
         """node(1:ncells) in the GRAVITY layout of type(kdnode) (with
@@ -485,6 +487,16 @@ class Context:
         self._check(self.lib.phsph_step_reset(
             self._h, vxyzu.shape[0], dtsph, pv, ldv, pf, ldf, _ptr(vpred)[0], _ptr(Bevol)[0], _ptr(dBevol)[0],
             _ptr(Bpred)[0], _ptr(dustevol)[0], _ptr(ddustevol)[0], _ptr(dustpred)[0], _ptr(iphase)[0]))
+
+    def step_graph(self, state, clock, nsub, ufloor=0.0, tolv=1.e-2, maxits=30, graphs=True):
+        """nsub steps of the evolve loop inside the library
+        (phsph_step_graph) on device-resident state.  state: abi.StepState,
+        clock: abi.StepClock (updated in place).  Returns one dict per step
+        (dt, dtnew, dterr, its, nactive, nwake, nvfloorp/ps/c)."""
+        out = (abi.StepSummary * max(nsub, 1))()
+        self._check(self.lib.phsph_step_graph(self._h, C.byref(state), C.byref(clock), nsub, ufloor, tolv, maxits,
+                                              1 if graphs else 0, out))
+        return [{k: getattr(out[i], k) for k, _ in abi.StepSummary._fields_} for i in range(nsub)]
 
     # -- adjoints --------------------------------------------------------------
     def density_vjp(self, xyzh, rhobar=None, hbar=None, dxyz=None):

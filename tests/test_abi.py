@@ -41,6 +41,23 @@ def test_params_struct_matches_header():
     assert C.sizeof(abi.Params) == 16 * 4 + 26 * 8
 
 
+def _struct_fields(txt, name):
+    body = re.sub(r"/\*.*?\*/", "", txt[txt.index(f"typedef struct {name}"):txt.index(f"}} {name};")], flags=re.S)
+    fields = []
+    for line in body.splitlines()[1:]:
+        m = re.match(r"\s*(?:int8_t|int32_t|float|double)\s*(.*);", line)
+        if m:
+            fields += [f.strip().lstrip("*").strip() for f in m.group(1).split(",")]
+    return fields
+
+
+@pytest.mark.parametrize("name,mirror", [("phsph_step_state", abi.StepState), ("phsph_step_clock", abi.StepClock),
+                                         ("phsph_step_summary", abi.StepSummary)])
+def test_step_structs_match_header(name, mirror):
+    txt = open(os.path.join(ROOT, "include", "phsph.h")).read()
+    assert _struct_fields(txt, name) == [n for n, _ in mirror._fields_]
+
+
 @pytest.mark.parametrize("kernel", [abi.KERNEL_CUBIC, abi.KERNEL_QUINTIC])
 def test_default_params_agree(kernel):
     L = sph.load_library()

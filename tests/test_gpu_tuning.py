@@ -1,6 +1,8 @@
 """The autotuner of the neighbour-list kernels: on the first large density
 call it times every configuration (candidate leaves per chunk, wavefronts
-per block, squared separations stored or recomputed) and keeps the fastest.
+per block, squared separations stored or recomputed, one group per
+wavefront or persistent wavefronts on per-XCD work queues) and keeps the
+fastest.
 Every configuration must give bit-identical outputs, so the choice never
 changes a result: each one is fixed in turn and the whole derivative chain
 (tree, density, cons2prim, force) compared bit for bit with the first; the
@@ -13,7 +15,7 @@ from tests import parity
 
 pytestmark = pytest.mark.gpu
 
-CONFIGS = [(cl, lw, r2) for cl in (8, 16) for lw in (2, 4) for r2 in (0, 1)]
+CONFIGS = [(cl, lw, r2, q) for cl in (8, 16) for lw in (2, 4) for r2 in (0, 1) for q in (0, 1)]
 
 
 def _chain(setup, cfg=None, calls=1):
@@ -55,18 +57,18 @@ def test_autotune_picks_fastest(case):
     assert tune["tuned"]
     tried = tune["tried"]
     assert len(tried) == len(CONFIGS)
-    assert {(t["cl"], t["lw"], t["r2"]) for t in tried} == set(CONFIGS)
+    assert {(t["cl"], t["lw"], t["r2"], t["q"]) for t in tried} == set(CONFIGS)
     assert all(t["ms"] > 0 for t in tried)
     best = min(tried, key=lambda t: t["ms"])
-    assert (tune["cl"], tune["lw"], tune["r2"]) == (best["cl"], best["lw"], best["r2"])
-    print(f"[{name}] tuned " + ", ".join(f"cl{t['cl']}/lw{t['lw']}/r2{t['r2']}:{t['ms']:.3f}ms" for t in tried),
-          flush=True)
+    assert (tune["cl"], tune["lw"], tune["r2"], tune["q"]) == (best["cl"], best["lw"], best["r2"], best["q"])
+    print(f"[{name}] tuned " + ", ".join(f"cl{t['cl']}/lw{t['lw']}/r2{t['r2']}/q{t['q']}:{t['ms']:.3f}ms"
+                                         for t in tried), flush=True)
 
 
 def test_configurations_bit_identical(case):
     name, s = case
     ref, t0 = _chain(s, CONFIGS[0])
-    assert not t0["tuned"] and (t0["cl"], t0["lw"], t0["r2"]) == CONFIGS[0]
+    assert not t0["tuned"] and (t0["cl"], t0["lw"], t0["r2"], t0["q"]) == CONFIGS[0]
     for cfg in CONFIGS[1:]:
         out, _ = _chain(s, cfg)
         for k in ref:
@@ -82,6 +84,8 @@ def test_set_tuning_rejects_unknown(gpu):
         ctx.set_tuning(12, 4, 0)
     with pytest.raises(sph.PhsphError):
         ctx.set_tuning(16, 3, 0)
+    with pytest.raises(sph.PhsphError):
+        ctx.set_tuning(16, 4, 0, 2)
     ctx.set_tuning(0)  # tune again on the next large call
     assert not ctx.get_tuning()["tuned"]
     ctx.close()
