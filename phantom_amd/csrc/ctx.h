@@ -206,6 +206,7 @@ struct phsph_ctx {
   DBuf<double> quads;                 // node second moments, 6 per slot
   DBuf<unsigned long long> ghmax;     // node hmax as the force walk sees it
   DBuf<double> htraj, hconv;          // h per density iteration, converged h
+  int64_t htraj_stride = 0;           // htraj: iteration k of particle p at (k-1)*htraj_stride + p
   DBuf<double> trial_hmax;            // hmax of each leaf at its last neighbour list (trial statistics)
   DBuf<double> fnode;                 // far-field expansion per leaf (20)
   DBuf<uint32_t> gnear;               // self-gravity force: neighbours per active particle

@@ -56,7 +56,7 @@ struct DensOut {
 __global__ void k_trial_hmax(int nleaves, const int32_t *__restrict__ leaves, const int32_t *__restrict__ leaf_flag,
                              const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi,
                              const phsph_kdnode *__restrict__ nd, const int32_t *__restrict__ p_nits,
-                             const double *__restrict__ htraj, const double *__restrict__ hconv,
+                             const double *__restrict__ htraj, size_t hstride, const double *__restrict__ hconv,
                              double *__restrict__ lhmax, int32_t *__restrict__ flags) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= nleaves) return;
@@ -64,7 +64,7 @@ __global__ void k_trial_hmax(int nleaves, const int32_t *__restrict__ leaves, co
   double h = nd[c].hmax;
   if (leaf_flag[c] > 0) {
     double hset = 0.;
-    const int set = leaf_relist_hmax(nd_lo[c], nd_hi[c], h, p_nits, htraj, hconv, &hset);
+    const int set = leaf_relist_hmax(nd_lo[c], nd_hi[c], h, p_nits, htraj, hstride, hconv, &hset);
     if (set < 0) atomicOr(flags, DFL_NOCONV);
     if (set > 0) h = hset;
   }
@@ -345,7 +345,7 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
             int32_t *__restrict__ t_nlj,
             double *__restrict__ t_hsum, double *__restrict__ t_hfin,
             int32_t *__restrict__ p_nits, int32_t *__restrict__ leaf_nits, int32_t *__restrict__ retry,
-            double *__restrict__ htraj, double *__restrict__ hconv, int32_t *__restrict__ flags,
+            double *__restrict__ htraj, size_t hstride, double *__restrict__ hconv, int32_t *__restrict__ flags,
             int32_t *__restrict__ wq) {
   __shared__ ListStage<CL> Sw[LW];
   ListStage<CL> &S = Sw[threadIdx.x >> 6];
@@ -477,7 +477,8 @@ k_dens_iter(DevConst dc, int icall, double rfac, const int32_t *__restrict__ act
       return;
     }
     hi = hnew;
-    if (htraj && it <= HTRAJ) htraj[(size_t)p * HTRAJ + it - 1] = hi;
+    // iteration-major: the lanes still iterating write adjacent words
+    if (htraj && it <= HTRAJ) htraj[(size_t)(it - 1) * hstride + p] = hi;
     if (dc.radkern * hi * (1. + 1e-12) > rcut) {  // outgrew the list: run again with a wider one
       atomicOr(flags, DFL_REWALK);
       retry[p] = 1;
@@ -915,7 +916,8 @@ static void launch_iter_t(const IterLaunch &L) {
                      L.rfac, c->act.p, L.na, L.ng, c->pleaf.p, c->leaves.p, c->gcand.p, c->gncand.p, c->goff_p,
                      c->caps.gcand, c->nd_lo.p, c->nd_hi.p, c->nd.p, c->tp4.p, L.widen ? c->hmax0.p : nullptr,
                      L.lmargin, L.lb, L.r2b, L.dlist, L.tnl, L.widen ? c->t_nlj.p : nullptr, L.ths, L.thf,
-                     c->p_nits.p, c->leaf_nits.p, c->tmpi.p, c->htraj.p, c->hconv.p, c->counters.p, c->wq.p);
+                     c->p_nits.p, c->leaf_nits.p, c->tmpi.p, c->htraj.p, (size_t)c->htraj_stride, c->hconv.p,
+                     c->counters.p, c->wq.p);
 }
 
 template <int K, int CL, int LW>
@@ -1006,7 +1008,7 @@ int dens_trial_stats(phsph_ctx *c) {
   const bool replay = c->dens_icall != 0 && c->htraj.p != nullptr;
   if (c->nleaves > 0 && replay)
     hipLaunchKernelGGL(k_trial_hmax, dim3(nblocks(c->nleaves, 128)), dim3(128), 0, s, c->nleaves, c->leaves.p,
-                       c->leaf_flag.p, c->nd_lo.p, c->nd_hi.p, c->nd.p, c->p_nits.p, c->htraj.p, c->hconv.p,
+                       c->leaf_flag.p, c->nd_lo.p, c->nd_hi.p, c->nd.p, c->p_nits.p, c->htraj.p, (size_t)c->htraj_stride, c->hconv.p,
                        c->trial_hmax.p, c->counters.p);
   if (c->nleaves > 0)
     hipLaunchKernelGGL(k_walk, dim3(nblocks(c->nleaves, 128)), dim3(128), 0, s, c->dc, 1.0, c->nd.p,
@@ -1299,6 +1301,7 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   // h after every Newton update: the relists of the reference are replayed
   // from it (node hmax for the gravity walk, trial-neighbour statistics)
   PH_HIPCHECK(c, c->htraj.ensure((size_t)n * HTRAJ + 1));
+  c->htraj_stride = n;
   PH_HIPCHECK(c, c->hconv.ensure(n + 1));
   c->dens_icall = icall;
   hipLaunchKernelGGL(k_zero_i32, dim3(nblocks(c->nleaves, 256)), dim3(256), 0, s, c->leaf_nits.p, c->nleaves, 0);

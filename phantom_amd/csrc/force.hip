@@ -166,6 +166,7 @@ k_flist(DevConst dc, const int32_t *__restrict__ act, int nact, int ngroups, con
 // list and finish_cell_and_store_results.  One thread per active particle.
 template <int K, bool MHD, bool DUST, bool THERMAL>
 __global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu((MHD || DUST) ? 2 : (THERMAL ? 3 : 4))))
 k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *__restrict__ skip,
         const uint32_t *__restrict__ nlist, const int64_t *loff, const int32_t *__restrict__ t_nl,
         const int32_t *__restrict__ t_nlj, int capb, const int32_t *__restrict__ iparts,
@@ -235,32 +236,89 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *_
   const int nl = live ? t_nl[t] : 0;
   const int ntot = nl + ((live && t_nlj) ? t_nlj[t] : 0);
   int npair = 0;  // interacting pairs (entries that pass the neighbour test)
-  // Hydro: the whole 128-byte record of entry kk+1 (its first 32 bytes are
-  // the position record) and list entry kk+2 are read while entry kk is
-  // summed, so the record of a pair is in registers when it is needed.
-  // MHD and dust (more registers live): entry kk+1's 32-byte position
-  // record ahead, the full record after the neighbour test.
+  // Three ways to read the neighbour records ahead (same pairs, same order,
+  // same arithmetic):
+  //  - hydro, cubic kernel (TEST_FIRST): entry kk+1's neighbour test is made
+  //    from its 32-byte position record while entry kk is summed, and the
+  //    other 96 bytes of its record are read only when it is a force
+  //    neighbour (a density list also holds particles that are not); entry
+  //    kk+2's position record and list entry kk+3 are read at the same time;
+  //  - hydro, quintic kernel (WHOLE): the whole 128-byte record of entry kk+1
+  //    and list entry kk+2 are read while entry kk is summed (its pairs cost
+  //    more arithmetic, and the second neighbour test costs more than the
+  //    reads it saves);
+  //  - MHD and dust (more registers live): entry kk+1's position record
+  //    ahead, the rest of the record after the neighbour test.
   constexpr bool WHOLE = !MHD && !DUST;
+  constexpr bool TEST_FIRST = WHOLE && K == PHSPH_KERNEL_CUBIC;
   auto slot = [&](int kk) { return kk < nl ? kk : capb - 1 - (kk - nl); };
-  uint32_t en = 0u, en2 = 0u;
-  double4 pn = make_double4(0., 0., 0., 0.);
+  auto is_pair = [&](uint32_t e, const double4 &pj) {
+    double dx, dy, dz;
+    sep_of(dc, cand_code(e), xi, yi, zi, pj.x, pj.y, pj.z, dx, dy, dz);
+    const double rij2 = r2_exact(dx, dy, dz);
+    return xmul(rij2, hi21) < R2 || xmul(rij2, xmul(pj.w, pj.w)) < R2;
+  };
+  auto tail_of = [&](int j) { return *reinterpret_cast<const FTail *>(&rec[j].vx); };
+  uint32_t en = 0u, en2 = 0u, en3 = 0u;
+  double4 pn = make_double4(0., 0., 0., 0.), pn2 = pn;
+  FTail Tn;
   FRec Rn;
+  bool okn = false;
   if (ntot > 0) {
     en = nlist[list_at_g(loff, nact, t, slot(0))];
-    if constexpr (WHOLE) Rn = rec[cand_node(en)];
-    else pn = fpos[cand_node(en)];
+    if constexpr (TEST_FIRST) {
+      pn = fpos[cand_node(en)];
+      okn = is_pair(en, pn);
+      if (okn) Tn = tail_of(cand_node(en));
+      if (ntot > 1) {
+        en2 = nlist[list_at_g(loff, nact, t, slot(1))];
+        pn2 = fpos[cand_node(en2)];
+      }
+      if (ntot > 2) en3 = nlist[list_at_g(loff, nact, t, slot(2))];
+    } else if constexpr (WHOLE) {
+      Rn = rec[cand_node(en)];
+    } else {
+      pn = fpos[cand_node(en)];
+    }
   }
-  if (ntot > 1) en2 = nlist[list_at_g(loff, nact, t, slot(1))];
+  if (!TEST_FIRST && ntot > 1) en2 = nlist[list_at_g(loff, nact, t, slot(1))];
   for (int kk = 0; kk < ntot; kk++) {
     const uint32_t e = en;
     const int j = cand_node(e);
-    const FRec Rw = Rn;
-    const double4 pj = WHOLE ? make_double4(Rw.x, Rw.y, Rw.z, Rw.hinv) : pn;
-    if (kk + 1 < ntot) {
-      en = en2;
-      if constexpr (WHOLE) Rn = rec[cand_node(en)];
-      else pn = fpos[cand_node(en)];
-      if (kk + 2 < ntot) en2 = nlist[list_at_g(loff, nact, t, slot(kk + 2))];
+    double4 pj;
+    FTail Tw;
+    if constexpr (TEST_FIRST) {
+      pj = pn;
+      const bool okw = okn;
+      Tw = Tn;
+      if (kk + 1 < ntot) {
+        en = en2;
+        pn = pn2;
+        okn = is_pair(en, pn);
+        if (okn) Tn = tail_of(cand_node(en));
+        if (kk + 2 < ntot) {
+          en2 = en3;
+          pn2 = fpos[cand_node(en2)];
+          if (kk + 3 < ntot) en3 = nlist[list_at_g(loff, nact, t, slot(kk + 3))];
+        }
+      }
+      if (!okw) continue;
+    } else if constexpr (WHOLE) {
+      const FRec Rw = Rn;
+      pj = make_double4(Rw.x, Rw.y, Rw.z, Rw.hinv);
+      Tw = *reinterpret_cast<const FTail *>(&Rw.vx);
+      if (kk + 1 < ntot) {
+        en = en2;
+        Rn = rec[cand_node(en)];
+        if (kk + 2 < ntot) en2 = nlist[list_at_g(loff, nact, t, slot(kk + 2))];
+      }
+    } else {
+      pj = pn;
+      if (kk + 1 < ntot) {
+        en = en2;
+        pn = fpos[cand_node(en)];
+        if (kk + 2 < ntot) en2 = nlist[list_at_g(loff, nact, t, slot(kk + 2))];
+      }
     }
     double dx, dy, dz;
     sep_of(dc, cand_code(e), xi, yi, zi, pj.x, pj.y, pj.z, dx, dy, dz);
@@ -272,7 +330,7 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *_
     // a density list also holds particles that are not force neighbours
     if (!(q2i < R2 || q2j < R2)) continue;
     npair++;
-    const FRec &Rj = WHOLE ? Rw : rec[j];
+    const FTail &Rj = WHOLE ? Tw : *reinterpret_cast<const FTail *>(&rec[j].vx);
     if (ind) {
       // force.F90:1346-1352: every neighbour of an active particle is
       // flagged to be woken; the limiter needs the largest neighbour bin

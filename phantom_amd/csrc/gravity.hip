@@ -77,7 +77,7 @@ __global__ void k_init_ghmax(int64_t nslots, const phsph_kdnode *__restrict__ nd
 __global__ void k_hmax_replay(int nleaves, const int32_t *__restrict__ leaves, const int32_t *__restrict__ leaf_flag,
                               const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi,
                               const phsph_kdnode *__restrict__ nd, const int32_t *__restrict__ p_nits,
-                              const double *__restrict__ htraj, const double *__restrict__ hconv,
+                              const double *__restrict__ htraj, size_t hstride, const double *__restrict__ hconv,
                               unsigned long long *__restrict__ ghmax, int32_t *__restrict__ flags) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= nleaves) return;
@@ -85,7 +85,7 @@ __global__ void k_hmax_replay(int nleaves, const int32_t *__restrict__ leaves, c
   if (leaf_flag[c] <= 0) return;
   double hset = 0.;
   const int set = leaf_relist_hmax(nd_lo[c], nd_hi[c], __longlong_as_double((long long)ghmax[c]), p_nits, htraj,
-                                   hconv, &hset);
+                                   hstride, hconv, &hset);
   if (set < 0) {
     atomicOr(flags, GFL_TRAJ);
     return;
@@ -851,7 +851,7 @@ int grav_hmax_replay(phsph_ctx *c, int icall) {
   int32_t zero = 0;
   PH_HIPCHECK(c, dev_set(c, c->counters.p + 4, &zero, sizeof(int32_t)));
   hipLaunchKernelGGL(k_hmax_replay, dim3(nblocks(c->nleaves, 128)), dim3(128), 0, s, c->nleaves, c->leaves.p,
-                     c->leaf_flag.p, c->nd_lo.p, c->nd_hi.p, c->nd.p, c->p_nits.p, c->htraj.p, c->hconv.p,
+                     c->leaf_flag.p, c->nd_lo.p, c->nd_hi.p, c->nd.p, c->p_nits.p, c->htraj.p, (size_t)c->htraj_stride, c->hconv.p,
                      c->ghmax.p, c->counters.p + 4);
   PH_HIPCHECK(c, hipGetLastError());
   int32_t fl = 0;

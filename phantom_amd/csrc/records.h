@@ -14,6 +14,16 @@ struct __align__(16) FRec {
   double cs, h;
 };
 static_assert(sizeof(FRec) == 128, "FRec is one 128-byte line");
+// the record after its 32-byte position part (what a pair reads once the
+// neighbour test has passed)
+struct __align__(16) FTail {
+  double vx, vy, vz, u;
+  double rho, rho1, pr, pro2;
+  double vwave;
+  float gradh, alpha;
+  double cs, h;
+};
+static_assert(sizeof(FTail) == 96 && offsetof(FRec, vx) == 32, "FTail is the last 96 bytes of FRec");
 // mixed precision (hydro): the j quantities of the dissipation terms in
 // FP32 and P/rho^2 as an unevaluated FP32 pair; the FP64 position, 1/h,
 // velocity and u of j come from 32-byte position and velocity records

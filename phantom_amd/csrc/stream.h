@@ -122,14 +122,15 @@ __device__ __forceinline__ void sep_of(const DevConst &dc, int code, double xi, 
 // The hmax a leaf ends densityiterate with where the reference lists its
 // neighbours again (compute_hmax, dens.F90:1275-1289, then set_hmaxcell):
 // the cell's iterations are replayed from each particle's h after every
-// Newton update (htraj, HTRAJ per particle) and at convergence (hconv); the
+// Newton update (htraj: iteration k of particle p at (k-1)*hstride + p, up
+// to HTRAJ iterations) and at convergence (hconv); the
 // cell's hmax after iteration k is 1.01 max h, and a relist happens when it
 // exceeds the previous hmax while the cell is not yet converged.  Returns 1
 // with *hset the hmax of the last relist, 0 when the leaf is never relisted,
 // -1 when a cell needed more than HTRAJ + 1 iterations.
 __device__ __forceinline__ int leaf_relist_hmax(int lo, int hi, double hcell, const int32_t *__restrict__ p_nits,
-                                                const double *__restrict__ htraj, const double *__restrict__ hconv,
-                                                double *hset) {
+                                                const double *__restrict__ htraj, size_t hstride,
+                                                const double *__restrict__ hconv, double *hset) {
   int nits = 0;
   for (int p = lo; p <= hi; p++) nits = max(nits, p_nits[p]);
   if (nits <= 1) return 0;  // converged at the first iteration: no relist
@@ -140,7 +141,7 @@ __device__ __forceinline__ int leaf_relist_hmax(int lo, int hi, double hcell, co
     for (int p = lo; p <= hi; p++) {
       const int np_ = p_nits[p];
       if (np_ < 1) continue;
-      const double v = (k < np_) ? htraj[(size_t)p * HTRAJ + k - 1] : hconv[p];
+      const double v = (k < np_) ? htraj[(size_t)(k - 1) * hstride + p] : hconv[p];
       hmx = fmax(hmx, v);
     }
     const double hm = 1.01 * hmx;

@@ -240,6 +240,8 @@ __device__ __forceinline__ int pick_axis(const double *box) {
 #define BN_U (BN_CH / BN_BS)
 #define BN_SEQ 1024     // positions per LDS stage of the sequential sums
 #define BN_SEQ_BS 512   // wavefront 0 sums, the others stage
+#define BN_RED 24       // doubles per chunk in ch_red: sums/size/quads, then
+#define BN_BOX 12       // from BN_BOX the two children's bounding boxes
 
 // exclusive scan of per-thread counts over the block (block total in *total)
 template <int BS>
@@ -376,7 +378,7 @@ __global__ void __launch_bounds__(BN_BS) k_bn_sums(DevConst dc, const BigNode *b
   const int kind[7] = {0, 0, 0, 0, 0, 0, 1};
   block_reduce<BN_BS, 7>(v, kind, sh);
   if (threadIdx.x == 0)
-    for (int k = 0; k < 7; k++) ch_red[(size_t)c * 12 + k] = v[k];
+    for (int k = 0; k < 7; k++) ch_red[(size_t)c * BN_RED + k] = v[k];
 }
 
 // centre of mass from the parallel sums, and the bound on its distance from
@@ -389,7 +391,7 @@ __global__ void k_bn_centre(DevConst dc, BigNode *bn, int maxch, const int32_t *
   if (nchunk == 0) return;
   double s[3] = {0., 0., 0.}, a[3] = {0., 0., 0.}, hm = 0.;
   for (int k = lane; k < nchunk; k += 64) {
-    const double *r = ch_red + (size_t)(chunk0 + k) * 12;
+    const double *r = ch_red + (size_t)(chunk0 + k) * BN_RED;
     s[0] += r[0]; s[1] += r[1]; s[2] += r[2];
     a[0] += r[3]; a[1] += r[4]; a[2] += r[5];
     hm = fmax(hm, r[6]);
@@ -537,34 +539,48 @@ __global__ void __launch_bounds__(BN_BS) k_bn_size(DevConst dc, const BigNode *b
                                                    int maxch, const int32_t *counters, const double *tx,
                                                    const double *ty, const double *tz, double *ch_red,
                                                    int32_t *ch_i, int quads) {
-  __shared__ double sh[BN_BS / 64 * 8];
+  __shared__ double sh[BN_BS / 64 * 20];
   const int c = blockIdx.x;
   int f, p0, p1;
   if (!bn_chunk(bn, ch_node, counters, maxch, c, f, p0, p1)) return;
   const double c0 = bn[f].com[0], c1 = bn[f].com[1], c2 = bn[f].com[2], pivot = bn[f].pivot;
   const double *tax = axis_arr(bn[f].iaxis, tx, ty, tz);
   const double pm = dc.pmass;
-  double v[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  // [8..13] bounding box of the particles on the left of the pivot (the
+  // left child after a split at the pivot), [14..19] of those on the right
+  double v[20] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  for (int k = 0; k < 3; k++) {
+    v[8 + k] = PH_HUGE64; v[11 + k] = -PH_HUGE64;
+    v[14 + k] = PH_HUGE64; v[17 + k] = -PH_HUGE64;
+  }
 #pragma unroll
   for (int u = 0; u < BN_U; u++) {
     const int p = p0 + u * BN_BS + threadIdx.x;
     if (p <= p1) {
-      v[0] = fmax(v[0], r2_exact(xsub(tx[p], c0), xsub(ty[p], c1), xsub(tz[p], c2)));
-      v[1] += (tax[p] <= pivot) ? 1. : 0.;
+      const double r[3] = {tx[p], ty[p], tz[p]};
+      v[0] = fmax(v[0], r2_exact(xsub(r[0], c0), xsub(r[1], c1), xsub(r[2], c2)));
+      const bool left = tax[p] <= pivot;
+      v[1] += left ? 1. : 0.;
+      const int o = left ? 8 : 14;
+      for (int k = 0; k < 3; k++) {
+        v[o + k] = fmin(v[o + k], r[k]);
+        v[o + 3 + k] = fmax(v[o + 3 + k], r[k]);
+      }
       if (quads) {
-        const double dx = tx[p] - c0, dy = ty[p] - c1, dz = tz[p] - c2;
+        const double dx = r[0] - c0, dy = r[1] - c1, dz = r[2] - c2;
         v[2] += pm * (dx * dx); v[3] += pm * (dx * dy); v[4] += pm * (dx * dz);
         v[5] += pm * (dy * dy); v[6] += pm * (dy * dz); v[7] += pm * (dz * dz);
       }
     }
   }
-  const int kind[8] = {1, 0, 0, 0, 0, 0, 0, 0};
-  block_reduce<BN_BS, 8>(v, kind, sh);
+  const int kind[20] = {1, 0, 0, 0, 0, 0, 0, 0, 2, 2, 2, 1, 1, 1, 2, 2, 2, 1, 1, 1};
+  block_reduce<BN_BS, 20>(v, kind, sh);
   if (threadIdx.x == 0) {
-    ch_red[(size_t)c * 12 + 0] = v[0];
+    ch_red[(size_t)c * BN_RED + 0] = v[0];
     ch_i[(size_t)c * 4 + 1] = (int)v[1];
     if (quads)
-      for (int k = 0; k < 6; k++) ch_red[(size_t)c * 12 + 1 + k] = v[2 + k];
+      for (int k = 0; k < 6; k++) ch_red[(size_t)c * BN_RED + 1 + k] = v[2 + k];
+    for (int k = 0; k < 12; k++) ch_red[(size_t)c * BN_RED + BN_BOX + k] = v[8 + k];
   }
 }
 
@@ -581,7 +597,7 @@ __global__ void k_bn_node(BigNode *bn, int maxch, int32_t *counters, const doubl
   double r2 = 0.;
   int cl = 0;
   for (int k = lane; k < nchunk; k += 64) {
-    r2 = fmax(r2, ch_red[(size_t)(chunk0 + k) * 12 + 0]);
+    r2 = fmax(r2, ch_red[(size_t)(chunk0 + k) * BN_RED + 0]);
     cl += ch_i[(size_t)(chunk0 + k) * 4 + 1];
   }
   r2 = wave_max(r2);
@@ -589,7 +605,7 @@ __global__ void k_bn_node(BigNode *bn, int maxch, int32_t *counters, const doubl
   if (quads) {  // second moments: chunk shares in chunk order per lane, then across lanes
     double q[6] = {0., 0., 0., 0., 0., 0.};
     for (int k = lane; k < nchunk; k += 64)
-      for (int j = 0; j < 6; j++) q[j] += ch_red[(size_t)(chunk0 + k) * 12 + 1 + j];
+      for (int j = 0; j < 6; j++) q[j] += ch_red[(size_t)(chunk0 + k) * BN_RED + 1 + j];
     for (int j = 0; j < 6; j++) q[j] = wave_sum(q[j]);
     if (lane == 0)
       for (int j = 0; j < 6; j++) quads[6 * (size_t)b->nnode + j] = q[j];
@@ -767,7 +783,8 @@ __global__ void __launch_bounds__(BN_BS) k_bn_box(const BigNode *bn, const int32
   const int c = blockIdx.x;
   int f, p0, p1;
   if (!bn_chunk(bn, ch_node, counters, maxch, c, f, p0, p1)) return;
-  if (bn[f].nl == 0) return;
+  // a node split at the pivot has its children's boxes from k_bn_size
+  if (bn[f].nl == 0 || bn[f].part) return;
   const int hA = bn[f].lo + bn[f].nl - 1;
   double v[12];
   for (int k = 0; k < 3; k++) {
@@ -789,7 +806,7 @@ __global__ void __launch_bounds__(BN_BS) k_bn_box(const BigNode *bn, const int32
   const int kind[12] = {2, 2, 2, 1, 1, 1, 2, 2, 2, 1, 1, 1};
   block_reduce<BN_BS, 12>(v, kind, sh);
   if (threadIdx.x == 0)
-    for (int k = 0; k < 12; k++) ch_red[(size_t)c * 12 + k] = v[k];
+    for (int k = 0; k < 12; k++) ch_red[(size_t)c * BN_RED + BN_BOX + k] = v[k];
 }
 
 // children: ranges, boxes, next frontier
@@ -806,7 +823,7 @@ __global__ void k_bn_finish(const BigNode *bn, int maxch, int32_t *counters, con
     v[6 + k] = PH_HUGE64; v[9 + k] = -PH_HUGE64;
   }
   for (int j = lane; j < nchunk; j += 64) {
-    const double *r = ch_red + (size_t)(chunk0 + j) * 12;
+    const double *r = ch_red + (size_t)(chunk0 + j) * BN_RED + BN_BOX;
     for (int k = 0; k < 3; k++) {
       v[k] = fmin(v[k], r[k]); v[3 + k] = fmax(v[3 + k], r[3 + k]);
       v[6 + k] = fmin(v[6 + k], r[6 + k]); v[9 + k] = fmax(v[9 + k], r[9 + k]);
@@ -1182,7 +1199,7 @@ int tree_build(phsph_ctx *c, double *d_xyzh, int ld, int np, const int8_t *d_iph
   PH_HIPCHECK(c, c->ch_node.ensure(maxch));
   PH_HIPCHECK(c, c->ch_i.ensure((size_t)maxch * 4));
   PH_HIPCHECK(c, c->ch_off.ensure((size_t)maxch * 2));
-  PH_HIPCHECK(c, c->ch_red.ensure((size_t)maxch * 12));
+  PH_HIPCHECK(c, c->ch_red.ensure((size_t)maxch * BN_RED));
   // under self-gravity the large nodes' second moments are formed here
   double *quads = nullptr;
   if (c->p.gravity) {
