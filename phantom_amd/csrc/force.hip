@@ -544,7 +544,7 @@ k_force(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *_
 // FP64 position and velocity records and the 48-byte FRec32 instead of the
 // 128-byte record.
 template <int K, bool THERMAL>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(THERMAL ? 3 : 4)))
 k_force_mixed(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *__restrict__ skip,
               const uint32_t *__restrict__ nlist, const int64_t *loff, const int32_t *__restrict__ t_nl,
               const int32_t *__restrict__ t_nlj, int capb, const int32_t *__restrict__ iparts, const FRec *__restrict__ rec,
@@ -586,23 +586,74 @@ k_force_mixed(DevConst dc, const int32_t *__restrict__ act, int nact, const int3
   const int nl = live ? t_nl[t] : 0;
   const int ntot = nl + ((live && t_nlj) ? t_nlj[t] : 0);
   int npair = 0;  // interacting pairs (entries that pass the neighbour test)
-  // entry kk+1 and its position record are read while entry kk is summed
+  // Cubic kernel (TEST_FIRST): entry kk+1's neighbour test is made from its
+  // position record while entry kk is summed, and its FP32 record and
+  // velocity record are read then only when it is a force neighbour; entry
+  // kk+2's position record and list entry kk+3 are read at the same time.
+  // Quintic kernel: entry kk+1's position record ahead, the rest after the
+  // test (its pairs carry more arithmetic, and the second test costs more
+  // than the reads it saves).
+  constexpr bool TEST_FIRST = K == PHSPH_KERNEL_CUBIC;
   auto slot = [&](int kk) { return kk < nl ? kk : capb - 1 - (kk - nl); };
-  uint32_t en = 0u, en2 = 0u;
-  double4 pn = make_double4(0., 0., 0., 0.);
+  auto is_pair = [&](uint32_t e, const double4 &pj) {
+    double sx, sy, sz;
+    sep_of(dc, cand_code(e), xi, yi, zi, pj.x, pj.y, pj.z, sx, sy, sz);
+    const double s2 = r2_exact(sx, sy, sz);
+    return xmul(s2, hi21) < R2 || xmul(s2, xmul(pj.w, pj.w)) < R2;
+  };
+  uint32_t en = 0u, en2 = 0u, en3 = 0u;
+  double4 pn = make_double4(0., 0., 0., 0.), pn2 = pn, vn = pn;
+  FRec32 An;
+  bool okn = false;
   if (ntot > 0) {
     en = nlist[list_at_g(loff, nact, t, slot(0))];
     pn = fpos[cand_node(en)];
+    if constexpr (TEST_FIRST) {
+      okn = is_pair(en, pn);
+      if (okn) {
+        An = rec32[cand_node(en)];
+        vn = fvel[cand_node(en)];
+      }
+      if (ntot > 1) {
+        en2 = nlist[list_at_g(loff, nact, t, slot(1))];
+        pn2 = fpos[cand_node(en2)];
+      }
+      if (ntot > 2) en3 = nlist[list_at_g(loff, nact, t, slot(2))];
+    } else if (ntot > 1) {
+      en2 = nlist[list_at_g(loff, nact, t, slot(1))];
+    }
   }
-  if (ntot > 1) en2 = nlist[list_at_g(loff, nact, t, slot(1))];
   for (int kk = 0; kk < ntot; kk++) {
     const uint32_t e = en;
     const int j = cand_node(e);
     const double4 pj = pn;
-    if (kk + 1 < ntot) {
-      en = en2;
-      pn = fpos[cand_node(en)];
-      if (kk + 2 < ntot) en2 = nlist[list_at_g(loff, nact, t, slot(kk + 2))];
+    FRec32 A;
+    double4 vj;
+    if constexpr (TEST_FIRST) {
+      const bool okw = okn;
+      A = An;
+      vj = vn;
+      if (kk + 1 < ntot) {
+        en = en2;
+        pn = pn2;
+        okn = is_pair(en, pn);
+        if (okn) {
+          An = rec32[cand_node(en)];
+          vn = fvel[cand_node(en)];
+        }
+        if (kk + 2 < ntot) {
+          en2 = en3;
+          pn2 = fpos[cand_node(en2)];
+          if (kk + 3 < ntot) en3 = nlist[list_at_g(loff, nact, t, slot(kk + 3))];
+        }
+      }
+      if (!okw) continue;
+    } else {
+      if (kk + 1 < ntot) {
+        en = en2;
+        pn = fpos[cand_node(en)];
+        if (kk + 2 < ntot) en2 = nlist[list_at_g(loff, nact, t, slot(kk + 2))];
+      }
     }
     double dx, dy, dz;
     sep_of(dc, cand_code(e), xi, yi, zi, pj.x, pj.y, pj.z, dx, dy, dz);
@@ -627,7 +678,7 @@ k_force_mixed(DevConst dc, const int32_t *__restrict__ act, int nact, const int3
     }
     const double rij = rij2 * rij1;
     const double grkerni = (q2i < R2) ? kern_g<K, double>(q2i, rij * hi1) * hfacgrkern : 0.;
-    const FRec32 A = rec32[j];
+    if constexpr (!TEST_FIRST) A = rec32[j];
     double grkernj = 0.;
     bool usej = THERMAL;
     if (q2j < R2) {
@@ -635,7 +686,7 @@ k_force_mixed(DevConst dc, const int32_t *__restrict__ act, int nact, const int3
       usej = true;
     }
     const double runix = dx * rij1, runiy = dy * rij1, runiz = dz * rij1;
-    const double4 vj = fvel[j];
+    if constexpr (!TEST_FIRST) vj = fvel[j];
     const double dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
     const double projv = dvx * runix + dvy * runiy + dvz * runiz;
     // signal speeds and artificial viscosity (force.F90:1416-1427, 1581-1596)

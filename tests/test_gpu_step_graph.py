@@ -107,6 +107,35 @@ def test_block_substeps_bit_exact(name, gpu):
         c.close()
 
 
+def test_block_substeps_full_size_mhd_dust(gpu):
+    """The 128^3 MHD + one-fluid dust box (2,097,152 particles) on four
+    populated bins (nbinmax 6): eight substeps, active fractions from 1/64
+    to about 1/8, graph replay against the host-driven loop."""
+    setup = setups.with_levels(setups.mhd_dust_box(nx=128, jitter=0.05), setups.LEVELS_4, nbinmax=6, istepfrac=0)
+    s0 = _ind_state(setup)
+    _level_bins(s0, setup, 6)
+    se, sg = s0.copy(device=True), s0.copy(device=True)
+    del s0
+    p = setup["params"]
+    ce, cg = sph.Context(p), sph.Context(p)
+    te = tle = tg = tlg = 0.0
+    fractions = []
+    for k in range(8):
+        te, tle, oe = integrator.evolve_ind(ce, se, 1, time=te, tlast=tle)
+        tg, tlg, _, og = integrator.evolve_graph(cg, sg, 1, time=tg, tlast=tlg, graphs=True)
+        label = f"substep {k + 1} (active {oe[0]['nactive']})"
+        print(f"  {label}", flush=True)
+        assert all(og[0][key] == oe[0][key] for key in KEYS), (label, og[0], oe[0])
+        assert (tg, tlg) == (te, tle), label
+        assert (sg.nbinmax, sg.istepfrac, sg.ibinnow) == (se.nbinmax, se.istepfrac, se.ibinnow), label
+        _arrays_equal(se, sg, label)
+        fractions.append(oe[0]["nactive"] / setup["xyzh"].shape[0])
+    # substep 1: bin 6 only (1/64); substep 8: bin 3 (7/64) and what remains in bins 3..6
+    assert min(fractions) <= 1 / 64 + 1e-3 and max(fractions) >= 7 / 64
+    ce.close()
+    cg.close()
+
+
 def test_global_steps_bit_exact(gpu):
     setup = setups.shock_tube(ny=8)
     p = setup["params"]
