@@ -513,11 +513,17 @@ int phsph_step_graph(phsph_ctx *ctx, const phsph_step_state *state,
  * the next large call); phsph_get_tuning returns the current one, whether
  * it was tuned (tuned = 1) and, if `ms`/`cfg` are given (room for 16), the
  * time of each configuration tried (cfg: cl, lw, r2, q per configuration)
- * and their count.
+ * and their count.  `rfac` receives the first search radius the next
+ * densityiterate uses, in units of radkern*h at entry (1.03, 1.05 or 1.1):
+ * it is adapted after every call, one level narrower while no particle
+ * outgrows its list, one level wider (and not narrower again) when more
+ * than 1/200 of them do.  Neighbour and iteration counts do not depend on
+ * it; the sums agree to round-off (the candidate leaves come in another
+ * order).
  */
 int phsph_set_tuning(phsph_ctx *ctx, int32_t cl, int32_t lw, int32_t r2, int32_t q);
 int phsph_get_tuning(phsph_ctx *ctx, int32_t *cl, int32_t *lw, int32_t *r2, int32_t *q,
-                     int32_t *tuned, float *ms, int32_t *cfg, int32_t *ntried);
+                     int32_t *tuned, float *ms, int32_t *cfg, int32_t *ntried, double *rfac);
 
 /* get_neighbour_stats (dens.F90:1137-1155) of the last densityiterate. */
 int phsph_get_neighbour_stats(phsph_ctx *ctx, double *trialmean,

@@ -366,8 +366,9 @@ int phsph_set_tuning(phsph_ctx *c, int32_t cl, int32_t lw, int32_t r2, int32_t q
 }
 
 int phsph_get_tuning(phsph_ctx *c, int32_t *cl, int32_t *lw, int32_t *r2, int32_t *q, int32_t *tuned, float *ms,
-                     int32_t *cfg, int32_t *ntried) {
+                     int32_t *cfg, int32_t *ntried, double *rfac) {
   if (!c) return PHSPH_ERR_INPUT;
+  if (rfac) *rfac = RFAC_LEVELS[c->rfac_level];
   if (cl) *cl = c->tune.cl;
   if (lw) *lw = c->tune.lw;
   if (r2) *r2 = c->tune.r2;

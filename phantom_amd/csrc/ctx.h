@@ -86,6 +86,7 @@ struct ListTune {
   int cl = 16, lw = 4, r2 = 0, q = 0;
 };
 enum { TUNE_FIXED = 0, TUNE_PENDING = 1, TUNE_DONE = 2 };
+constexpr double RFAC_LEVELS[3] = {1.03, 1.05, 1.1};
 constexpr int TUNE_MIN_ACTIVE = 32768;  // smallest density call the autotuner times
 constexpr int TUNE_MAX_CONFIGS = 16;
 
@@ -144,6 +145,10 @@ struct phsph_ctx {
   bool dlist_reuse = false;
   int dlist_nact = 0, dlist_cap = 0;
   int dlist_nretry = 0;             // particles of the first attempt run again
+  // first search radius of densityiterate (units of radkern*h at entry):
+  // index into RFAC_LEVELS, adapted after every call; it never goes below
+  // rfac_floor, which rises above a level that had to be widened
+  int rfac_level = 2, rfac_floor = 0;
   DBuf<int32_t> dretry0;            // their flags, by tree position
   DBuf<uint32_t> nlist_r;           // lists of retries (density) and of those particles (force)
   DBuf<double> nr2, nr2_r;          // squared separations of the own entries of the density lists

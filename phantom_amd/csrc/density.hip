@@ -1315,9 +1315,14 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   const bool dbg = getenv("PHSPH_DEBUG") != nullptr;
   // Particles whose h outgrows their neighbour list, or whose list does not
   // fit, are handed back and run again (alone) with a wider search or a
-  // longer list.
-  const double rfac0 = 1.1;  // first search radius in units of radkern*h at entry
+  // longer list.  The first search radius (units of radkern*h at entry) is
+  // adapted from call to call (end of this function): the narrowest level
+  // whose lists rarely need a wider search.  Neighbour sets and iteration
+  // counts do not depend on it; the sums agree to round-off (the group walk
+  // lists the candidate leaves in an order that depends on the radius).
+  const double rfac0 = RFAC_LEVELS[c->rfac_level];
   double rfac = rfac0;
+  bool rewalk0 = false;
   // Lists the force pass can use as its neighbour lists (no self-gravity):
   // they also cover every j whose radius reaches i, with a margin for the
   // converged h (within tolh of the h of the last summation, which is at
@@ -1413,6 +1418,7 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
     if (attempt == 0 && tuned_now && (fl & DFL_LIST)) c->tune_mode = TUNE_PENDING;
     if (fl & DFL_NEGH) PH_RET(c, PHSPH_ERR_INPUT, "densityiterate: setting negative h from hrho");
     if (attempt == 0) {
+      rewalk0 = (fl & DFL_REWALK) != 0;
       // the lists of the first attempt, and which particles they do not
       // cover (those run again)
       c->dlist_nact = na;
@@ -1446,6 +1452,21 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
     PH_HIPCHECK(c, hipMemcpyAsync(&bad, c->counters.p + 2, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     PH_HIPCHECK(c, hipStreamSynchronize(s));
     if (bad) c->dlist_reuse = false;
+  }
+  // the next call's first search radius: one level wider (and never again
+  // this narrow) when more than 1/200 of the particles outgrew their lists
+  // or the lists could not serve the force pass, one level narrower when
+  // none outgrew them
+  if (nact > 0) {
+    const bool outgrown = rewalk0 && c->dlist_nretry * 200 > nact;
+    if (outgrown || (reusable && !c->dlist_reuse)) {
+      if (c->rfac_level < 2) {
+        c->rfac_floor = std::max(c->rfac_floor, c->rfac_level + 1);
+        c->rfac_level++;
+      }
+    } else if (!rewalk0 && c->rfac_level > c->rfac_floor) {
+      c->rfac_level--;
+    }
   }
   while (dlist > c->caps.dlist && !list_fits(c, (size_t)list_cap(dlist) * nact * 3)) dlist /= 2;
   if (dlist > c->caps.dlist) c->caps.dlist = dlist;

@@ -87,7 +87,7 @@ def load_library(path=LIB_PATH):
     L.phsph_get_quads.argtypes = [vp, vp, i64]
     L.phsph_get_tree_grav.argtypes = [vp, vp, i64]
     L.phsph_set_tuning.argtypes = [vp, i32, i32, i32, i32]
-    L.phsph_get_tuning.argtypes = [vp, P(i32), P(i32), P(i32), P(i32), P(i32), vp, vp, P(i32)]
+    L.phsph_get_tuning.argtypes = [vp, P(i32), P(i32), P(i32), P(i32), P(i32), vp, vp, P(i32), P(dbl)]
     L.phsph_set_active_particles.argtypes = [vp, i32, vp, i32, vp, vp, i32, i32, P(i32), P(i32), P(i32)]
     L.phsph_init_step.argtypes = [vp, i32, dbl, vp, i32, vp]
     L.phsph_step_predictor.argtypes = [vp, i32, dbl, dbl, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp,
@@ -214,14 +214,16 @@ class Context:
         """Current configuration, whether it was tuned, and the time (ms) of
         each configuration the autotuner tried."""
         cl, lw, r2, q, tuned, nt = (C.c_int32() for _ in range(6))
+        rfac = C.c_double()
         ms = np.zeros(16, np.float32)
         cfg = np.zeros(64, np.int32)
         self._check(self.lib.phsph_get_tuning(self._h, C.byref(cl), C.byref(lw), C.byref(r2), C.byref(q),
                                               C.byref(tuned), ms.ctypes.data_as(C.c_void_p),
-                                              cfg.ctypes.data_as(C.c_void_p), C.byref(nt)))
+                                              cfg.ctypes.data_as(C.c_void_p), C.byref(nt), C.byref(rfac)))
         tried = [dict(cl=int(cfg[4 * k]), lw=int(cfg[4 * k + 1]), r2=int(cfg[4 * k + 2]), q=int(cfg[4 * k + 3]),
                       ms=float(ms[k])) for k in range(nt.value)]
-        return dict(cl=cl.value, lw=lw.value, r2=r2.value, q=q.value, tuned=bool(tuned.value), tried=tried)
+        return dict(cl=cl.value, lw=lw.value, r2=r2.value, q=q.value, tuned=bool(tuned.value), tried=tried,
+                    search_radius=rfac.value)
 
     def get_tree_grav(self):
         """node(1:ncells) in the GRAVITY layout of type(kdnode) (with

@@ -89,3 +89,36 @@ def test_set_tuning_rejects_unknown(gpu):
     ctx.set_tuning(0)  # tune again on the next large call
     assert not ctx.get_tuning()["tuned"]
     ctx.close()
+
+
+def test_search_radius_adapts_results_unchanged(gpu):
+    """The first search radius of the density pass narrows from call to call
+    while no particle outgrows its list.  Neighbour counts and iteration
+    counts stay those of the first call exactly; h, gradh, the velocity
+    gradients and the forces agree to round-off (the group walk lists the
+    candidate leaves in an order that depends on the radius, so the sums add
+    the same terms in another order)."""
+    s = setups.uniform_box(nx=48, isothermal=False)
+    p = s["params"]
+    ctx = sph.Context(p)
+    outs, radii = [], []
+    for k in range(4):
+        radii.append(ctx.get_tuning()["search_radius"])
+        xyzh = s["xyzh"].copy()
+        alphaind = s["alphaind"].copy()
+        ctx.maketree(xyzh)
+        d = ctx.densityiterate(1, xyzh, s["vxyzu"], alphaind=alphaind)
+        nneigh, nits = ctx.get_particle_stats(xyzh.shape[0])
+        c = ctx.cons2prim(xyzh, s["vxyzu"], d["dvdx"], alphaind)
+        f = ctx.force(1, xyzh, s["vxyzu"], d["gradh"], alphaind, c["eos_vars"])
+        outs.append(dict(h=xyzh[:, 3].copy(), gradh=d["gradh"].copy(), divv=d["divcurlv"].copy(),
+                         dvdx=d["dvdx"].copy(), nneigh=nneigh, nits=nits, f=f["fxyzu"].copy()))
+    ctx.close()
+    print(f"\n  search radii {radii}", flush=True)
+    assert radii[0] == 1.1 and min(radii) < 1.1
+    for o in outs[1:]:
+        for k in ("nneigh", "nits"):
+            np.testing.assert_array_equal(o[k], outs[0][k], err_msg=k)
+        assert np.max(np.abs(o["h"] - outs[0]["h"]) / outs[0]["h"]) <= 1e-14
+        for k in ("gradh", "divv", "dvdx", "f"):
+            assert parity.rel_rms(np.asarray(o[k], np.float64), np.asarray(outs[0][k], np.float64)) <= 1e-7, k
