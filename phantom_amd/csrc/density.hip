@@ -1345,7 +1345,8 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   // 12 bytes per entry: the entry and its squared separation
   while (c->caps.dlist > 16 && !list_fits(c, (size_t)list_cap(c->caps.dlist) * nact * 3)) c->caps.dlist /= 2;
   int dlist = c->caps.dlist;
-  bool tuned_now = false;
+  const int dlist0 = dlist;
+  bool tuned_now = false, retune = false;
   for (int attempt = 0; attempt < 12; attempt++) {
     const int na = c->nact;  // active slots, padding included
     if (na == 0) break;
@@ -1414,8 +1415,8 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
     if (fl & DFL_NOCONV) PH_RET(c, PHSPH_ERR_NOCONVERGE, "densityiterate: could not converge in density");
     // a tuning attempt whose lists overflowed (the first call on a context
     // grows the list capacity) times unrepresentative work: tune again on
-    // the next large call
-    if (attempt == 0 && tuned_now && (fl & DFL_LIST)) c->tune_mode = TUNE_PENDING;
+    // the next large call if that call's capacity is larger
+    if (attempt == 0 && tuned_now && (fl & DFL_LIST)) retune = true;
     if (fl & DFL_NEGH) PH_RET(c, PHSPH_ERR_INPUT, "densityiterate: setting negative h from hrho");
     if (attempt == 0) {
       rewalk0 = (fl & DFL_REWALK) != 0;
@@ -1470,6 +1471,9 @@ int dens_run(phsph_ctx *c, int icall, const DensIOArgs &a) {
   }
   while (dlist > c->caps.dlist && !list_fits(c, (size_t)list_cap(dlist) * nact * 3)) dlist /= 2;
   if (dlist > c->caps.dlist) c->caps.dlist = dlist;
+  // where the list budget holds the capacity, the overflow recurs on every
+  // call and the tuned configuration stands
+  if (retune && c->caps.dlist > dlist0) c->tune_mode = TUNE_PENDING;
   if (grav) {
     rc = grav_hmax_replay(c, icall);
     if (rc) return rc;

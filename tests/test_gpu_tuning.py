@@ -122,3 +122,34 @@ def test_search_radius_adapts_results_unchanged(gpu):
         assert np.max(np.abs(o["h"] - outs[0]["h"]) / outs[0]["h"]) <= 1e-14
         for k in ("gradh", "divv", "dvdx", "f"):
             assert parity.rel_rms(np.asarray(o[k], np.float64), np.asarray(outs[0][k], np.float64)) <= 1e-7, k
+
+
+def test_tuned_once_when_budget_holds_capacity(gpu, monkeypatch):
+    """A list budget that holds the capacity below the longest trial list:
+    the first attempt's lists overflow on every call, so a tuning run over
+    them cannot be bettered by a later call; it is kept, not repeated on
+    every call, and the outputs are those of an unconstrained context."""
+    s = setups.uniform_box(nx=64, isothermal=False)
+    n = s["xyzh"].shape[0]
+    # 16-entry lists (3 words per entry) fit, 32-entry lists do not
+    monkeypatch.setenv("PHSPH_CAPS", f"dlist=16,budget_kib={16 * n * 3 * 4 // 1024}")
+    p = s["params"]
+    ctx = sph.Context(p)
+    tried = []
+    for _ in range(3):
+        xyzh = s["xyzh"].copy()
+        alphaind = s["alphaind"].copy()
+        ctx.maketree(xyzh)
+        d = ctx.densityiterate(1, xyzh, s["vxyzu"], alphaind=alphaind)
+        t = ctx.get_tuning()
+        assert t["tuned"]
+        tried.append([(u["cl"], u["lw"], u["r2"], u["q"], u["ms"]) for u in t["tried"]])
+    ctx.close()
+    assert tried[0] == tried[1] == tried[2]
+    monkeypatch.delenv("PHSPH_CAPS")
+    ref, _ = _chain(s, calls=1)
+    # the overflowing particles are walked again into longer lists (and the
+    # search radius may have narrowed since the first call): h agrees to
+    # round-off, the sums to their summation order
+    assert np.max(np.abs(xyzh[:, 3] - ref["h"]) / ref["h"]) <= 1e-14
+    assert parity.rel_rms(np.asarray(d["gradh"], np.float64), np.asarray(ref["gradh"], np.float64)) <= 1e-7
