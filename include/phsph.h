@@ -605,9 +605,10 @@ int phsph_step_vjp(phsph_ctx *ctx, int32_t np, double dt, const double *xyzh, in
 /*
  * Precision (PHSPH_PREC_*) the pair sums of the last phsph_densityiterate
  * and phsph_force actually ran in; -1 before the first call.  With
- * params.precision = PHSPH_PREC_MIXED the hydro sums run in mixed
- * precision; the density sums with MHD, and the force pair terms with MHD,
- * dust or self-gravity, have FP64 kernels only and report PHSPH_PREC_FP64.
+ * params.precision = PHSPH_PREC_MIXED the hydro and self-gravity sums run
+ * in mixed precision (the multipole expansions stay FP64); the density sums
+ * with MHD, and the force pair terms with MHD or dust, have FP64 kernels
+ * only and report PHSPH_PREC_FP64.
  * No reference counterpart (the reference is FP64 throughout).
  */
 int phsph_get_precision(phsph_ctx *ctx, int32_t *dens_prec, int32_t *force_prec);

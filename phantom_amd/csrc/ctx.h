@@ -403,10 +403,11 @@ int step_corrector_noalloc_run(phsph_ctx *c, int np, double timei, double dtsph,
                                double ufloor, double *errmax, double *v2sum, int32_t *npc, int32_t *nwake,
                                int32_t *nvfloor);
 struct FRec;
+struct FRec32;
 int grav_tree(phsph_ctx *c);
 int grav_hmax_replay(phsph_ctx *c, int icall);
-int grav_force(phsph_ctx *c, int icall, const ForceIOArgs &a, const FRec *rec, const float *gsoft, double *dtc_out,
-               double *dtf_out);
+int grav_force(phsph_ctx *c, int icall, const ForceIOArgs &a, const FRec *rec, const FRec32 *rec32,
+               const float *gsoft, double *dtc_out, double *dtf_out);
 int set_active_run(phsph_ctx *c, int np, const double *xyzh, int ldx, int8_t *iphase, const int8_t *ibin,
                    int nbinmax, int istepfrac, int32_t *nactive, int32_t *nalive);
 // vjp.hip

@@ -1012,9 +1012,9 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
   FRecD *recD = dust ? (FRecD *)c->frecD.p : nullptr;
   if (a.ibin) PH_HIPCHECK(c, c->tbin_old.ensure(n));
   if (c->p.gravity) PH_HIPCHECK(c, c->gsoft.ensure(n));
-  // mixed precision covers the hydro pair terms; MHD, dust and self-gravity
-  // keep their FP64 pair kernels
-  const bool mixed = c->p.precision == PHSPH_PREC_MIXED && !mhd && !dust && !c->p.gravity;
+  // mixed precision covers the hydro and self-gravity pair terms; MHD and
+  // dust keep their FP64 pair kernels
+  const bool mixed = c->p.precision == PHSPH_PREC_MIXED && !mhd && !dust;
   c->force_prec_used = mixed ? PHSPH_PREC_MIXED : PHSPH_PREC_FP64;
   PH_HIPCHECK(c, c->fpos.ensure(n));
   if (mixed) {
@@ -1035,7 +1035,7 @@ int force_run(phsph_ctx *c, int icall, const ForceIOArgs &a, double *dtc_out, do
                      c->tmpi.p);
   int rc = compact_active(c, c->tmpi.p, n);
   if (rc) return rc;
-  if (c->p.gravity) return grav_force(c, icall, a, rec, c->gsoft.p, dtc_out, dtf_out);
+  if (c->p.gravity) return grav_force(c, icall, a, rec, rec32, c->gsoft.p, dtc_out, dtf_out);
   const int nact_real = c->nact;
   // the lists of the last densityiterate, if they still describe these particles
   bool reuse = c->dlist_reuse && c->nact == c->nact_dens;

@@ -585,11 +585,27 @@ struct GravArgs {
   int32_t *nmax;           // longest neighbour list met (after an overflow)
 };
 
-template <int K, bool THERMAL>
+// 1/sqrt(r2) for the mixed-precision form: an FP32 reciprocal square root
+// refined by one FP64 Newton step (relative error ~1e-14), or the FP64
+// routine where r2 is outside the normal FP32 range
+__device__ __forceinline__ double rsqrt_mixed(double r2) {
+  const float r2f = (float)r2;
+  if (!(r2f >= 1.17549435e-38f && r2f <= 3.4e38f)) return rsqrt(r2);
+  const double y = (double)rsqrtf(r2f);
+  return y * (1.5 - 0.5 * r2 * y * y);
+}
+
+// MIXED: the mixed-precision form.  Positions, the neighbour tests, the
+// multipole expansions and every sum stay FP64; 1/r comes from
+// rsqrt_mixed, and a neighbour's quantities other than position, velocity
+// and u from its FP32 record (rec32; P/rho^2 as an FP32 pair of FP64
+// accuracy) with position and velocity from the packed FP64 arrays
+template <int K, bool THERMAL, bool MIXED>
 __global__ void __launch_bounds__(128)
 k_force_grav(DevConst dc, const int32_t *__restrict__ act, int nact, const int32_t *__restrict__ pleaf,
              const int32_t *__restrict__ leaves, const int32_t *__restrict__ nd_lo, const int32_t *__restrict__ nd_hi,
              const phsph_kdnode *__restrict__ nd, const int32_t *__restrict__ iparts, const FRec *__restrict__ rec,
+             const double4 *__restrict__ fpos, const double4 *__restrict__ fvel, const FRec32 *__restrict__ rec32,
              GravArgs G, ForceIOArgs out, const int8_t *__restrict__ tbin_old, int icall,
              unsigned long long *__restrict__ dtred, int32_t *__restrict__ flags) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -664,7 +680,7 @@ k_force_grav(DevConst dc, const int32_t *__restrict__ act, int nact, const int32
           // trial neighbour outside both kernels: 1/r^2 (force.F90:1991-2053).
           // 1/sqrt(r^2) as one reciprocal square root (within 2 ulp of the
           // reference's sqrt and division); the neighbour tests above use r^2
-          const double rij1 = rsqrt(rij2);
+          const double rij1 = MIXED ? rsqrt_mixed(rij2) : rsqrt(rij2);
           const double fgravj = rij1 * rij1 * rij1 * pm;
           fx = fx - dx * fgravj;
           fy = fy - dy * fgravj;
@@ -689,11 +705,14 @@ k_force_grav(DevConst dc, const int32_t *__restrict__ act, int nact, const int32
       const uint32_t ek = G.nbuf[list_at_g(G.noff, nact, t, kk)];
       const int j = cand_node(ek), code = cand_code(ek);
       const FRec &Rj = rec[j];
+      double4 pj;
+      if constexpr (MIXED) pj = fpos[j];
+      else pj = make_double4(Rj.x, Rj.y, Rj.z, Rj.hinv);
       double dx, dy, dz;
-      sep_of(dc, code, xi, yi, zi, Rj.x, Rj.y, Rj.z, dx, dy, dz);
+      sep_of(dc, code, xi, yi, zi, pj.x, pj.y, pj.z, dx, dy, dz);
       const double rij2 = r2_exact(dx, dy, dz);
       const double q2i = xmul(rij2, hi21);
-      const double hj1 = Rj.hinv;
+      const double hj1 = pj.w;
       const double hj21 = xmul(hj1, hj1);
       const double q2j = xmul(rij2, hj21);
       nl++;
@@ -704,7 +723,7 @@ k_force_grav(DevConst dc, const int32_t *__restrict__ act, int nact, const int32
       }
       double rij1, qi;
       if (rij2 > PH_TINY64) {
-        rij1 = 1. / sqrt(rij2);
+        rij1 = MIXED ? rsqrt_mixed(rij2) : 1. / sqrt(rij2);
         qi = (rij2 * rij1) * hi1;
       } else {
         rij1 = 0.;
@@ -722,11 +741,30 @@ k_force_grav(DevConst dc, const int32_t *__restrict__ act, int nact, const int32
         phii = -rij1;
         fgravi = rij1 * rij1;
       }
+      // what the pair terms read of j
+      double vxj, vyj, vzj, uj, gradhj, rho1j_, pro2j_, vwavej_, alphaj;
+      if constexpr (MIXED) {
+        const double4 vj = fvel[j];
+        const FRec32 A = rec32[j];
+        vxj = vj.x; vyj = vj.y; vzj = vj.z; uj = vj.w;
+        gradhj = (double)A.gradh;
+        rho1j_ = (double)A.rho1;
+        pro2j_ = (double)A.pro2 + (double)A.pro2lo;
+        vwavej_ = (double)A.vwave;
+        alphaj = (double)A.alpha;
+      } else {
+        vxj = Rj.vx; vyj = Rj.vy; vzj = Rj.vz; uj = Rj.u;
+        gradhj = (double)Rj.gradh;
+        rho1j_ = Rj.rho1;
+        pro2j_ = Rj.pro2;
+        vwavej_ = Rj.vwave;
+        alphaj = (double)Rj.alpha;
+      }
       double grkernj = 0., fgravj;
       bool usej = false;
       if (q2j < R2) {
         const double qj = (rij2 * rij1) * hj1;
-        grkernj = kern_g<K, double>(q2j, qj) * hj21 * hj21 * dc.cnormk * (double)Rj.gradh;
+        grkernj = kern_g<K, double>(q2j, qj) * hj21 * hj21 * dc.cnormk * gradhj;
         double phij, fmj;
         kern_soft<K>(q2j, qj, phij, fmj);
         fmj = fmj * hj21;
@@ -737,20 +775,20 @@ k_force_grav(DevConst dc, const int32_t *__restrict__ act, int nact, const int32
       }
       const double fgrav = 0.5 * (pm * fgravi + pm * fgravj);
       const double runix = dx * rij1, runiy = dy * rij1, runiz = dz * rij1;
-      const double dvx = vxi - Rj.vx, dvy = vyi - Rj.vy, dvz = vzi - Rj.vz;
+      const double dvx = vxi - vxj, dvy = vyi - vyj, dvz = vzi - vzj;
       const double projv = dvx * runix + dvy * runiy + dvz * runiz;
-      const double denij = THERMAL ? (eni - Rj.u) : 0.;
+      const double denij = THERMAL ? (eni - uj) : 0.;
       const double vsigi = fmax(vwavei - dc.beta * projv, 0.);
       const double vsigavi = fmax(alphai * vwavei - dc.beta * projv, 0.);
       vsigmax = fmax(vsigmax, vsigi);
       double rho1j = 0., pro2j = 0., vwavej = 0., vsigavj = 0., autermj = 0.;
       if (usej) {
-        rho1j = Rj.rho1;
-        pro2j = Rj.pro2;
-        vwavej = Rj.vwave;
+        rho1j = rho1j_;
+        pro2j = pro2j_;
+        vwavej = vwavej_;
         autermj = 0.5 * pm * rho1j * dc.alphau;
         vsigmax = fmax(vsigmax, fmax(vwavej - dc.beta * projv, 0.));
-        vsigavj = fmax((double)Rj.alpha * vwavej - dc.beta * projv, 0.);
+        vsigavj = fmax(alphaj * vwavej - dc.beta * projv, 0.);
       }
       double qrho2i = 0., qrho2j = 0.;
       if (projv < 0.) {
@@ -861,8 +899,8 @@ int grav_hmax_replay(phsph_ctx *c, int icall) {
   return PHSPH_OK;
 }
 
-int grav_force(phsph_ctx *c, int icall, const ForceIOArgs &a, const FRec *rec, const float *gsoft,
-               double *dtc_out, double *dtf_out) {
+int grav_force(phsph_ctx *c, int icall, const ForceIOArgs &a, const FRec *rec, const FRec32 *rec32,
+               const float *gsoft, double *dtc_out, double *dtf_out) {
   hipStream_t s = c->stream;
   const int nl = c->nleaves, nact = c->nact;
   PH_HIPCHECK(c, c->fnode.ensure((size_t)(nl + 1) * LENFGRAV));
@@ -936,15 +974,18 @@ int grav_force(phsph_ctx *c, int icall, const ForceIOArgs &a, const FRec *rec, c
       LaunchTimer lt(c, "force");
       const dim3 gr(nblocks(nact, 128)), b(128);
       const bool th = c->p.maxvxyzu >= 4;
-#define GL(KK, T)                                                                                               \
-  hipLaunchKernelGGL((k_force_grav<KK, T>), gr, b, 0, s, c->dc, c->act.p, nact, c->pleaf.p, c->leaves.p,        \
-                     c->nd_lo.p, c->nd_hi.p, c->nd.p, c->iparts.p, rec, G, a, c->tbin_old.p, icall, c->dtred.p,  \
-                     c->counters.p)
-      if (c->p.kernel == PHSPH_KERNEL_CUBIC) {
-        if (th) GL(PHSPH_KERNEL_CUBIC, true); else GL(PHSPH_KERNEL_CUBIC, false);
+#define GL(KK, T, M)                                                                                            \
+  hipLaunchKernelGGL((k_force_grav<KK, T, M>), gr, b, 0, s, c->dc, c->act.p, nact, c->pleaf.p, c->leaves.p,     \
+                     c->nd_lo.p, c->nd_hi.p, c->nd.p, c->iparts.p, rec, c->fpos.p, c->fvel.p, rec32, G, a,      \
+                     c->tbin_old.p, icall, c->dtred.p, c->counters.p)
+#define GLK(KK, M) \
+  if (th) GL(KK, true, M); else GL(KK, false, M);
+      if (rec32) {
+        if (c->p.kernel == PHSPH_KERNEL_CUBIC) { GLK(PHSPH_KERNEL_CUBIC, true) } else { GLK(PHSPH_KERNEL_QUINTIC, true) }
       } else {
-        if (th) GL(PHSPH_KERNEL_QUINTIC, true); else GL(PHSPH_KERNEL_QUINTIC, false);
+        if (c->p.kernel == PHSPH_KERNEL_CUBIC) { GLK(PHSPH_KERNEL_CUBIC, false) } else { GLK(PHSPH_KERNEL_QUINTIC, false) }
       }
+#undef GLK
 #undef GL
     }
     PH_HIPCHECK(c, hipGetLastError());

@@ -219,10 +219,12 @@ def _mixed_errors(name, g, o):
     return e_rho, e_f
 
 
-@pytest.mark.parametrize("name", ["shock_tube_1M", "hydro_box_8M"])
+@pytest.mark.parametrize("name", ["shock_tube_1M", "hydro_box_8M", "C4_polytrope_1M_active_2^-3"])
 def test_mixed_full_size(name, gpu):
     if name == "shock_tube_1M":
         mk = lambda: setups.shock_tube()  # noqa: E731
+    elif name.startswith("C4"):
+        mk = _c4(64)  # self-gravity: FP64 multipoles, mixed pair sums
     else:
         mk = lambda: setups.uniform_box(nx=256, ny=256, nz=128, box=(-0.5, 0.5, -0.5, 0.5, -0.25, 0.25),  # noqa
                                         isothermal=False, seed=8)

@@ -7,9 +7,11 @@ must be bit-exact.  h and rho must agree with the FP64 oracle to 1e-6
 the transverse force of a shock tube, is measured against the force, not
 against its own noise), du/dt to 1e-6 of its rms, and the float32 velocity
 gradients (div v, dv/dx) to 1e-5 of the rms of |dv/dx|.  The error of each
-quantity is printed.  MHD, dust and self-gravity keep
-their FP64 pair kernels, so with mixed precision requested they must agree
-with the oracle to the FP64 tolerance."""
+quantity is printed.  Self-gravity has a mixed form (FP64 multipoles,
+far-field 1/r^2 sums with a refined FP32 reciprocal square root, FP32
+neighbour records); MHD and dust keep their FP64 pair kernels, so with
+mixed precision requested they must agree with the oracle to the FP64
+tolerance."""
 import numpy as np
 import pytest
 
@@ -32,6 +34,10 @@ def _cases():
         "box24_iso_cubic": lambda: setups.uniform_box(nx=24, seed=11),
         "random_8k": lambda: setups.random_box(n=8000, seed=3),
         "shock_tube_bins": lambda: setups.with_bins(setups.shock_tube(ny=8), nbinmax=3, istepfrac=2),
+        # self-gravity, tree opening angle 0.5
+        "polytrope_4k_cubic": lambda: setups.polytrope(n=4000, theta=0.5),
+        "polytrope_3k_quintic_bins": lambda: setups.with_bins(
+            setups.polytrope(n=3000, theta=0.5, kernel=abi.KERNEL_QUINTIC), nbinmax=3, istepfrac=2),
     }
 
 
@@ -41,6 +47,8 @@ def pair(request, gpu):
     print(f"\n[case {request.param}] n={setup['xyzh'].shape[0]} mixed", flush=True)
     g = parity.run_gpu(setup)
     o = parity.run_oracle(setup)
+    # every case here has a mixed form of both passes
+    assert g["ctx"].get_precision() == (abi.PREC_MIXED, abi.PREC_MIXED)
     return request.param, setup, g, o
 
 
