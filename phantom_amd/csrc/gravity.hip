@@ -293,7 +293,9 @@ k_gw_level(DevConst dc, int nent, GwLevel in, GwNext out, const int32_t *__restr
            int32_t *__restrict__ flags) {
   __shared__ uint32_t stk[GSTACK];
   __shared__ unsigned long long omask[GW_MASKS];  // opened candidates of the first pass, 64 per word
-  const int e = blockIdx.x;
+  // consecutive entries (nodes near each other, sharing candidate sources)
+  // on one XCD and its L2
+  const int e = xcd_block(blockIdx.x, gridDim.x);
   if (e >= nent) return;
   const int lane = threadIdx.x;
   const int A = in.node[e];
@@ -542,7 +544,7 @@ static int grav_walk(phsph_ctx *c, const int64_t *poff) {
     GwLevel in{c->gw_node[cur].p, c->gw_coff[cur].p, c->gw_ccnt[cur].p, c->gw_H[cur].p, c->gw_cand[cur].p};
     GwNext out{c->gw_node[nxt].p, c->gw_coff[nxt].p, c->gw_ccnt[nxt].p, c->gw_H[nxt].p, c->gw_cand[nxt].p,
                (int64_t)c->gw_cand[nxt].cap, c->gw_used.p, c->counters.p + 1};
-    hipLaunchKernelGGL(k_gw_level, dim3(nent), dim3(64), 0, s, c->dc, nent, in, out, c->leaf_flag.p, act, c->nd.p,
+    hipLaunchKernelGGL(k_gw_level, dim3(xcd_grid(nent)), dim3(64), 0, s, c->dc, nent, in, out, c->leaf_flag.p, act, c->nd.p,
                        c->ghmax.p, c->quads.p, c->nd_lo.p, c->pleaf.p, c->fnode.p, c->p2p.p, c->np2p.p, poff,
                        c->caps.p2p, c->counters.p);
     PH_HIPCHECK(c, hipGetLastError());
@@ -608,7 +610,9 @@ k_force_grav(DevConst dc, const int32_t *__restrict__ act, int nact, const int32
              const double4 *__restrict__ fpos, const double4 *__restrict__ fvel, const FRec32 *__restrict__ rec32,
              GravArgs G, ForceIOArgs out, const int8_t *__restrict__ tbin_old, int icall,
              unsigned long long *__restrict__ dtred, int32_t *__restrict__ flags) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  // consecutive blocks (neighbouring leaves, overlapping near-field lists)
+  // on one XCD and its L2
+  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   double dtc_o = PH_HUGE64, dtf_o = PH_HUGE64;
   int nl = 0, bmax = -1;
   if (t < nact) {
@@ -972,7 +976,7 @@ int grav_force(phsph_ctx *c, int icall, const ForceIOArgs &a, const FRec *rec, c
                c->gnear.p, list_cap(capn), noff, c->gnear_n.p, c->counters.p + 5};
     if (nact > 0) {
       LaunchTimer lt(c, "force");
-      const dim3 gr(nblocks(nact, 128)), b(128);
+      const dim3 gr(xcd_grid(nblocks(nact, 128))), b(128);
       const bool th = c->p.maxvxyzu >= 4;
 #define GL(KK, T, M)                                                                                            \
   hipLaunchKernelGGL((k_force_grav<KK, T, M>), gr, b, 0, s, c->dc, c->act.p, nact, c->pleaf.p, c->leaves.p,     \
