@@ -307,10 +307,15 @@ k_gw_level(DevConst dc, int nent, GwLevel in, GwNext out, const int32_t *__restr
   if (lfA == 0) {
     // internal destination: expansions of the well-separated sources, and
     // the opened sources as the children's candidates
+    // The opened sources are staged in LDS (the leaf stack, which internal
+    // nodes do not use) in the order the list is written, and copied out
+    // once the list has its place; a list longer than the stack is formed
+    // again in a second pass over the candidates
     int ntot = 0;
     for (int b = 0; b < nc; b += 64) {
       const int k = b + lane;
       int npush = 0;
+      uint32_t e0 = 0u, e1 = 0u;
       if (k < nc) {
         const int src = (int)in.cand[c0 + k];
         bool opened = (src == A);
@@ -325,19 +330,41 @@ k_gw_level(DevConst dc, int nent, GwLevel in, GwNext out, const int32_t *__restr
             opened = true;
           }
         }
-        if (opened)
-          npush = (leaf_flag[src] != 0) ? 1 : ((nd[src].leftchild != 0 ? 1 : 0) + (nd[src].rightchild != 0 ? 1 : 0));
+        if (opened) {
+          if (leaf_flag[src] != 0) {
+            e0 = (uint32_t)src;
+            npush = 1;
+          } else {
+            const int il = nd[src].leftchild, ir = nd[src].rightchild;
+            if (il != 0) { e0 = (uint32_t)il; npush = 1; }
+            if (ir != 0) {
+              if (npush == 0) e0 = (uint32_t)ir;
+              else e1 = (uint32_t)ir;
+              npush++;
+            }
+          }
+        }
       }
       const unsigned long long m = __ballot(npush > 0);
       if (lane == 0 && (b >> 6) < GW_MASKS) omask[b >> 6] = m;
-      ntot += wave_sum_i(npush);
+      int incl = npush;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+      }
+      const int at = ntot + incl - npush;
+      if (npush > 0 && at < GSTACK) stk[at] = e0;
+      if (npush > 1 && at + 1 < GSTACK) stk[at + 1] = e1;
+      ntot += __shfl(incl, 63, 64);
     }
     wave_lds_sync();
     unsigned long long base = 0ull;
     if (lane == 0) base = atomicAdd(out.cand_used, (unsigned long long)ntot);
     base = __shfl(base, 0, 64);
     const bool fits = (int64_t)(base + (unsigned long long)ntot) <= out.cand_cap;
-    if (fits) {
+    if (fits && ntot <= GSTACK) {
+      for (int k = lane; k < ntot; k += 64) out.cand[(size_t)base + k] = stk[k];
+    } else if (fits) {
       int w = 0;
       for (int b = 0; b < nc; b += 64) {
         const int k = b + lane;

@@ -30,6 +30,11 @@ def _moving(s, seed=7, sigma=0.1):
 def _cases():
     return {
         "polytrope_p0_open": lambda: setups.polytrope(n=1500, pressureless=True, theta=0.0),
+        # open tree at 20,000 particles: nothing is well separated, so a node
+        # passes twice as many candidates to its children as its level has
+        # nodes, more than the 2048 the walk stages in LDS on its deeper
+        # levels (those lists are formed in a second pass over the candidates)
+        "polytrope_p0_open_20k": lambda: setups.polytrope(n=20000, pressureless=True, theta=0.0),
         "polytrope_p0_theta05": lambda: setups.polytrope(n=4000, pressureless=True, theta=0.5),
         "polytrope_adiabatic_quintic": lambda: _moving(
             setups.polytrope(n=3000, theta=0.5, kernel=abi.KERNEL_QUINTIC)),
