@@ -16,7 +16,9 @@ import numpy as np
 from phantom_amd import abi
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "liboracle.so")
+# PHSPH_ORACLE_LIB names another build of the same source (the sanitized one
+# of tests/test_oracle_sanitized.py)
+_LIB = os.environ.get("PHSPH_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 
