@@ -694,11 +694,13 @@ k_force_grav(DevConst dc, const int32_t *__restrict__ act, int nact, const int32
         nlo = nd_lo[cand_node(en1)];
         nhi = nd_hi[cand_node(en1)];
       }
+      // positions from the packed array (four to a cache line), not from
+      // the 128-byte records
       double4 pn = make_double4(0., 0., 0., 0.);
-      if (j0 <= j1) pn = *reinterpret_cast<const double4 *>(rec + j0);
+      if (j0 <= j1) pn = fpos[j0];
       for (int j = j0; j <= j1; j++) {
         const double4 pj = pn;
-        if (j + 1 <= j1) pn = *reinterpret_cast<const double4 *>(rec + j + 1);
+        if (j + 1 <= j1) pn = fpos[j + 1];
         if (j == p) continue;
         double dx, dy, dz;
         sep_of(dc, code, xi, yi, zi, pj.x, pj.y, pj.z, dx, dy, dz);
